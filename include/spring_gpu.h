@@ -1,0 +1,127 @@
+/*
+ * spring_gpu.h -- C ABI of libspring_gpu.so, the MI355X (gfx950) scattering-rate path.
+ *
+ * This library replaces the hot routines of SPRING's kappa.x / LineWidth.x:
+ *   ScatterProbAllq1 + ScatterMatEl + ScatterProbability   src/Kappa/Src/ThirdOrder_m.f90:41-166, 278-439
+ *   CalculateLinewidth (Gaussian delta)                     src/Kappa/Src/ThirdOrder_m.f90:442-543
+ *                                                            src/LineWidth/Src/Linewidth_m.f90:234-323
+ *   ScatterProbq1q2_per + ScatterMatEl4 + ScatterProbability4  src/Kappa/Src/FourthOrder_m.f90:42-473
+ *   TetrahedronIntegration4th (processes, Gaussian form)   src/Kappa/Src/Tetrahedron4th_sep.f90:37-425
+ *   kappa_noIter / kappa_noIter_symm                        src/Kappa/Src/ThermalConductivity.f90:36-293
+ *   the q0 loop that drives them                            src/Kappa/Src/main.f90:552-715
+ *
+ * Conventions.
+ *   - Arrays are passed as the reference's column-major Fortran arrays, byte for byte
+ *     (no host transposes).  complex(dp) arrays are interleaved (re, im) doubles.
+ *   - Every index value stored in an array (basis atom, grid point) is 0-based; the
+ *     Fortran interface module (fortran/spring_gpu_m.f90) subtracts 1 where the
+ *     reference stores 1-based values.
+ *   - Frequencies are angular, rad/ps; sigma is given in the same unit.
+ *   - Every entry point returns 0 on success or a negative SPR_E_* code; the message is
+ *     in spr_last_error() (thread-local).  Calls are synchronous: outputs are on the host
+ *     when they return.  One context is used by one host thread at a time.
+ *   - One GPU: the context binds to `device` (HIP ordinal) at creation.
+ */
+#ifndef SPRING_GPU_H
+#define SPRING_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPR_OK 0
+#define SPR_E_ARG (-1)       /* bad argument / shape */
+#define SPR_E_HIP (-2)       /* HIP runtime error */
+#define SPR_E_OOM (-3)       /* device allocation failed */
+#define SPR_E_STATE (-4)     /* invariant violation (e.g. no FC4 loaded) */
+#define SPR_E_UNSUPPORTED (-5)
+
+/* precision of the FT + eigenvector contraction (occupations, delta arguments and the
+ * final reductions stay FP64 in every mode) */
+#define SPR_PREC_FP64 0
+#define SPR_PREC_FP32 1      /* FP32 phase sums + contraction, compensated FP32 delta sum */
+#define SPR_PREC_BF16 2      /* BF16 per-tile-scaled MFMA contraction, FP32 accumulate */
+
+/* crystal: the `cell` fields the path uses (src/Kappa/Src/unit_mod.f90:9-38) */
+typedef struct {
+    int nb;                     /* basis atoms; Ndof = 3 nb */
+    const double *mass;         /* (nb) amu */
+    double vol;                 /* unit-cell volume, A^3 */
+} spr_crystal;
+
+/* anharmonic IFC, FC3type / FC4type (src/Kappa/Src/FC3_mod.f90:30-99, FC4_mod.f90:30-97) */
+typedef struct {
+    int order;                  /* 3 or 4 */
+    int natm;                   /* leading neighbour dimension (Natm max) */
+    const int *atm_num;         /* atmNum(nb) */
+    const int *atm_indx;        /* atmIndx(4, natm, nb): cell (3 ints) + basis atom (0-based) */
+    const double *fc;           /* fC(3,3,3,natm,natm,nb) or fC(3,3,3,3,natm,natm,natm,nb), eV/A^n */
+} spr_ifc;
+
+/* Gamma-centred mesh and irreducible wedge (q_tetrahedron, TetrahedronQ_m.f90:35-331) */
+typedef struct {
+    int mesh[3];
+    int nq;                     /* mesh(1)*mesh(2)*mesh(3) */
+    const int *q_int;           /* q_pnt_int(3, nq), folded addresses; grid index order
+                                   i = sum_k mod(q_k, N_k) prod_{j<k} N_j */
+    int nq0;
+    const int *irr_grid;        /* (nq0) grid index of each irreducible q0 */
+    const int *pos;             /* pos(nq): irreducible position of every grid point */
+    const int *multiplicity;    /* multiplicity(nq0) */
+    int nsym;                   /* point operations used by kappa symm (incl. time reversal) */
+    const double *rot;          /* Rxyz(3,3,nsym) Cartesian rotations; may be NULL if nsym==0 */
+} spr_mesh;
+
+/* harmonic phonons on the full mesh (Phon, phonon_mod.f90:33-46) */
+typedef struct {
+    int nq, ndof;
+    const double *evec;         /* Evec(ndof, ndof, nq) complex(dp) */
+    const double *omega;        /* omega(ndof, nq), rad/ps */
+    const double *inv_omega;    /* OnebyOmega(ndof, nq) */
+    const double *nbe;          /* nBE(ndof, nq) */
+    const double *grp_vel;      /* grp_vel(3, ndof, nq), m/s (for kappa) */
+} spr_phonons;
+
+typedef struct spr_ctx spr_ctx;
+
+/* Upload FC3 (and FC4 if not NULL), mesh tables and phonons once; replaces the data set-up
+ * of main.f90:192-196, 279-300 and set_Phon_mp (phonon_mod.f90:156-315). */
+int spr_ctx_create(const spr_crystal *cr, const spr_ifc *fc3, const spr_ifc *fc4,
+                   const spr_mesh *mesh, const spr_phonons *ph, int device, spr_ctx **out);
+int spr_ctx_destroy(spr_ctx *ctx);
+const char *spr_last_error(void);
+
+/* Parity path of ScatterProbAllq1 (ThirdOrder_m.f90:41-166): W(s2,s1,s0,q1) =
+ * Sctrq1(Ndof,Ndof,Ndof,Nq) for irreducible q0 `i0`, every q1 (q2 = -(q0+q1) mod G),
+ * written to W_out (Ndof^3 * Nq doubles). FP64. */
+int spr_v3_w(spr_ctx *ctx, int i0, double *W_out);
+
+/* Gaussian-delta three-phonon linewidth of irreducible q0 `i0` (ScatterProbAllq1 fused with
+ * CalculateLinewidth, ThirdOrder_m.f90:442-543): lw(Ndof), rad/ps. */
+int spr_lw3(spr_ctx *ctx, int i0, double sigma, int prec, double *lw);
+
+/* All irreducible q0 in one persistent launch (the q0 loop of main.f90:552-715 restricted to
+ * three-phonon scattering): LW_Allq0(Ndof, nq0). */
+int spr_lw3_all(spr_ctx *ctx, double sigma, int prec, double *lw_all);
+
+/* Four-phonon Gaussian linewidth sum of irreducible q0 `i0` over all (q1,q2), the processes
+ * and weights of TetrahedronIntegration4th (Tetrahedron4th_sep.f90:242-343) with a Gaussian
+ * delta; main.f90:660 adds 0.5*lw to the linewidth. */
+int spr_lw4(spr_ctx *ctx, int i0, double sigma, int prec, double *lw);
+int spr_lw4_all(spr_ctx *ctx, double sigma, int prec, double *lw_all);
+
+/* RTA thermal conductivity (kappa_noIter, ThermalConductivity.f90:36-164, symm = 0; or
+ * kappa_noIter_symm + GrVelKronProdSymm, :167-293 and VelocityKronProd_symm.f90:34-93,
+ * symm = 1) from LW_Allq0(Ndof, nq0), FP64: kappa(9) W/mK, mode_k(9, Ndof) optional. */
+int spr_kappa_rta(spr_ctx *ctx, const double *lw_all, double T, int symm,
+                  double *kappa, double *mode_k);
+
+/* Timing of the last launch of the dominant kernel (HIP events on the context's stream):
+ * number of launches and summed milliseconds since the last reset. */
+int spr_kernel_time(spr_ctx *ctx, int *launches, double *ms);
+int spr_kernel_time_reset(spr_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,185 @@
+"""ctypes front end of the FP64 CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+CPU-baseline leg, never by the product package.  Every function here hands numpy arrays
+in the reference's byte layout to the C restatement and returns numpy results.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ct.CDLL(path)
+    return _LIB
+
+
+def _p(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data_as(ct.c_void_p)
+
+
+class Inputs:
+    """Arrays of one problem, converted once."""
+
+    def __init__(self, model, qm, ph, which=3):
+        cr = model.crystal
+        ifc = model.fc3 if which == 3 else model.fc4
+        self.nb = cr.nb
+        self.nd = cr.ndof
+        self.natm = ifc.natm_max
+        self.atm_num, self.p_atm_num = _p(ifc.atm_num, np.int32)
+        self.atm_indx, self.p_atm_indx = _p(ifc.atm_indx, np.int32)
+        self.fc, self.p_fc = _p(ifc.fc, np.float64)
+        self.latvec, self.p_latvec = _p(cr.latvec, np.float64)
+        self.osm, self.p_osm = _p(cr.one_by_sqrt_mass, np.float64)
+        self.nq = qm.nq
+        self.mesh, self.p_mesh = _p(qm.mesh, np.int32)
+        self.q_int, self.p_q_int = _p(qm.q_int, np.int32)
+        self.q_cart, self.p_q_cart = _p(qm.q_cart, np.float64)
+        self.evec, self.p_evec = _p(ph.evec, np.complex128)
+        self.omega, self.p_omega = _p(ph.omega, np.float64)
+        self.inv_omega, self.p_inv_omega = _p(ph.inv_omega, np.float64)
+        self.nbe, self.p_nbe = _p(ph.nbe, np.float64)
+
+
+def scatter_mat_el(inp: Inputs, q1c, q2c, chk, e0, e1, e2, literal=True):
+    nd = inp.nd
+    out = np.zeros((nd, nd, nd), np.complex128)
+    _, pq1 = _p(q1c, np.float64)
+    _, pq2 = _p(q2c, np.float64)
+    c, pc = _p(chk, np.int32)
+    a0, p0 = _p(e0, np.complex128)
+    a1, p1 = _p(e1, np.complex128)
+    a2, p2 = _p(e2, np.complex128)
+    fn = lib().orc_scatter_mat_el if literal else lib().orc_v3_factorised
+    q1a = np.ascontiguousarray(q1c, np.float64)
+    q2a = np.ascontiguousarray(q2c, np.float64)
+    fn(ct.c_int(inp.nb), ct.c_int(inp.natm), inp.p_atm_num, inp.p_atm_indx, inp.p_fc,
+       inp.p_latvec, inp.p_osm, q1a.ctypes.data_as(ct.c_void_p), q2a.ctypes.data_as(ct.c_void_p),
+       pc, p0, p1, p2, out.ctypes.data_as(ct.c_void_p))
+    return out
+
+
+def scatter_mat_el4(inp: Inputs, q1c, q2c, q3c, chk, e0, e1, e2, e3, literal=True):
+    nd = inp.nd
+    out = np.zeros((nd, nd, nd, nd), np.complex128)
+    qa = [np.ascontiguousarray(x, np.float64) for x in (q1c, q2c, q3c)]
+    c, pc = _p(chk, np.int32)
+    es = [np.ascontiguousarray(x, np.complex128) for x in (e0, e1, e2, e3)]
+    fn = lib().orc_scatter_mat_el4 if literal else lib().orc_v4_factorised
+    fn(ct.c_int(inp.nb), ct.c_int(inp.natm), inp.p_atm_num, inp.p_atm_indx, inp.p_fc,
+       inp.p_latvec, inp.p_osm, *[x.ctypes.data_as(ct.c_void_p) for x in qa], pc,
+       *[x.ctypes.data_as(ct.c_void_p) for x in es], out.ctypes.data_as(ct.c_void_p))
+    return out
+
+
+def scatter_prob(nd, chk, iw0, iw1, iw2, phi, factor):
+    W = np.zeros((nd, nd, nd))
+    Wp = np.zeros((nd, nd, nd))
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (iw0, iw1, iw2)]
+    c, pc = _p(chk, np.int32)
+    ph = np.ascontiguousarray(phi, np.complex128)
+    lib().orc_scatter_prob(ct.c_int(nd), pc, *[x.ctypes.data_as(ct.c_void_p) for x in arrs],
+                           ph.ctypes.data_as(ct.c_void_p), ct.c_double(factor),
+                           W.ctypes.data_as(ct.c_void_p), Wp.ctypes.data_as(ct.c_void_p))
+    return W, Wp
+
+
+def lw3_gauss(inp: Inputs, i0_grid: int, sigma: float, q1_lo=0, q1_hi=None, want_w=False):
+    q1_hi = inp.nq if q1_hi is None else q1_hi
+    nd = inp.nd
+    lw = np.zeros(nd)
+    W = np.zeros((q1_hi - q1_lo, nd, nd, nd)) if want_w else None
+    lib().orc_lw3_gauss(ct.c_int(inp.nb), ct.c_int(inp.natm), inp.p_atm_num, inp.p_atm_indx,
+                        inp.p_fc, inp.p_latvec, inp.p_osm, ct.c_int(inp.nq), inp.p_mesh,
+                        inp.p_q_int, inp.p_q_cart, inp.p_evec, inp.p_omega, inp.p_inv_omega,
+                        inp.p_nbe, ct.c_int(i0_grid), ct.c_double(sigma), ct.c_int(q1_lo),
+                        ct.c_int(q1_hi), lw.ctypes.data_as(ct.c_void_p),
+                        W.ctypes.data_as(ct.c_void_p) if want_w else None)
+    return (lw, W) if want_w else lw
+
+
+def lw4_gauss(inp: Inputs, i0_grid: int, sigma: float, q1_lo=0, q1_hi=None):
+    q1_hi = inp.nq if q1_hi is None else q1_hi
+    lw = np.zeros(inp.nd)
+    lib().orc_lw4_gauss(ct.c_int(inp.nb), ct.c_int(inp.natm), inp.p_atm_num, inp.p_atm_indx,
+                        inp.p_fc, inp.p_latvec, inp.p_osm, ct.c_int(inp.nq), inp.p_mesh,
+                        inp.p_q_int, inp.p_q_cart, inp.p_evec, inp.p_omega, inp.p_inv_omega,
+                        inp.p_nbe, ct.c_int(i0_grid), ct.c_double(sigma), ct.c_int(q1_lo),
+                        ct.c_int(q1_hi), lw.ctypes.data_as(ct.c_void_p))
+    return lw
+
+
+def lw3_gauss_all(inp: Inputs, irr_idx, sigma: float, threads: int = 1):
+    """Linewidth of every irreducible q0 (rows), threaded over q0 by the caller."""
+    irr_idx = list(int(i) for i in irr_idx)
+    if threads <= 1:
+        return np.array([lw3_gauss(inp, i, sigma) for i in irr_idx])
+    with ThreadPoolExecutor(threads) as ex:
+        return np.array(list(ex.map(lambda i: lw3_gauss(inp, i, sigma), irr_idx)))
+
+
+def lw4_gauss_all(inp: Inputs, irr_idx, sigma: float, threads: int = 1):
+    irr_idx = list(int(i) for i in irr_idx)
+    if threads <= 1:
+        return np.array([lw4_gauss(inp, i, sigma) for i in irr_idx])
+    with ThreadPoolExecutor(threads) as ex:
+        return np.array(list(ex.map(lambda i: lw4_gauss(inp, i, sigma), irr_idx)))
+
+
+def kappa_rta(qm, ph, lw_irr, T, vol):
+    nd = ph.omega.shape[1]
+    k = np.zeros(9)
+    arrs = [np.ascontiguousarray(qm.q_int, np.int32), np.ascontiguousarray(qm.pos, np.int32),
+            np.ascontiguousarray(ph.omega), np.ascontiguousarray(ph.nbe),
+            np.ascontiguousarray(ph.grp_vel), np.ascontiguousarray(lw_irr, np.float64)]
+    lib().orc_kappa_rta(ct.c_int(nd), ct.c_int(qm.nq), *[a.ctypes.data_as(ct.c_void_p) for a in arrs],
+                        ct.c_double(T), ct.c_double(vol), k.ctypes.data_as(ct.c_void_p))
+    return k.reshape(3, 3)
+
+
+def kappa_rta_symm(qm, ph, lw_irr, T, vol):
+    nd = ph.omega.shape[1]
+    k = np.zeros(9)
+    irr = qm.irr_idx
+    arrs = [np.ascontiguousarray(qm.irr_q_int, np.int32), np.ascontiguousarray(qm.multiplicity, np.int32)]
+    rot = np.ascontiguousarray(qm.rot_cart, np.float64)
+    arrs2 = [np.ascontiguousarray(ph.omega[irr]), np.ascontiguousarray(ph.nbe[irr]),
+             np.ascontiguousarray(ph.grp_vel[irr]), np.ascontiguousarray(lw_irr, np.float64)]
+    nsym = rot.shape[0] * (2 if qm.time_rev else 1)
+    if qm.time_rev:
+        rot = np.ascontiguousarray(np.concatenate([rot, -rot]))
+    lib().orc_kappa_rta_symm(ct.c_int(nd), ct.c_int(qm.nq), ct.c_int(qm.nq0),
+                             *[a.ctypes.data_as(ct.c_void_p) for a in arrs], ct.c_int(nsym),
+                             rot.ctypes.data_as(ct.c_void_p),
+                             *[a.ctypes.data_as(ct.c_void_p) for a in arrs2], ct.c_double(T),
+                             ct.c_double(vol), k.ctypes.data_as(ct.c_void_p))
+    return k.reshape(3, 3)
+
+
+def lw3_from_w(W, w0, w1, w2, n1, n2, sigma):
+    """Gaussian-delta stage alone: W[s0][s1][s2] of one pair -> lw(s0)."""
+    nd = W.shape[0]
+    lw = np.zeros(nd)
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (W, w0, w1, w2, n1, n2)]
+    lib().orc_lw3_from_w(ct.c_int(nd), *[a.ctypes.data_as(ct.c_void_p) for a in arrs],
+                         ct.c_double(sigma), lw.ctypes.data_as(ct.c_void_p))
+    return lw

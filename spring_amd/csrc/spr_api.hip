@@ -1,0 +1,406 @@
+// C ABI of libspring_gpu.so: context set-up (uploads), entry points, error reporting.
+// Declarations and reference citations: include/spring_gpu.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "spr_internal.h"
+
+namespace spr {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int ensure(void **p, size_t *cap, size_t bytes) {
+    if (*cap >= bytes && *p) return SPR_OK;
+    if (*p) {
+        SPR_HIP(hipFree(*p));
+        *p = nullptr;
+        *cap = 0;
+    }
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        return SPR_E_OOM;
+    }
+    *cap = bytes;
+    return SPR_OK;
+}
+
+template <typename T>
+static int upload(T **dst, const T *src, size_t n) {
+    if (n == 0) {
+        *dst = nullptr;
+        return SPR_OK;
+    }
+    if (hipMalloc((void **)dst, n * sizeof(T)) != hipSuccess) {
+        set_error("hipMalloc failed in upload");
+        return SPR_E_OOM;
+    }
+    SPR_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return SPR_OK;
+}
+
+static long lcm(long a, long b) { return a / std::gcd(a, b) * b; }
+
+// Build the block-grouped tuple list of an IFC (see DevIFC).
+static int build_ifc(Ctx &c, const spr_ifc *f, const double *osm, DevIFC &d) {
+    const int nb = c.nb, order = f->order, natm = f->natm;
+    if ((order != 3 && order != 4) || natm <= 0) {
+        set_error("spr_ifc: order must be 3 or 4 and natm > 0");
+        return SPR_E_ARG;
+    }
+    const int ncomp = order == 3 ? 27 : 81;
+    const int nsub = order - 1;
+    std::vector<int> site_off(nb + 1, 0);
+    for (int mu = 0; mu < nb; ++mu) {
+        if (f->atm_num[mu] < 1 || f->atm_num[mu] > natm) {
+            set_error("spr_ifc: atm_num out of range");
+            return SPR_E_ARG;
+        }
+        site_off[mu + 1] = site_off[mu] + f->atm_num[mu];
+    }
+    const int nsite = site_off[nb];
+    std::vector<int4> site(nsite);
+    for (int mu = 0; mu < nb; ++mu)
+        for (int n = 0; n < f->atm_num[mu]; ++n) {
+            const int *ix = f->atm_indx + ((size_t)mu * natm + n) * 4;
+            if (ix[3] < 0 || ix[3] >= nb) {
+                set_error("spr_ifc: basis index in atm_indx out of range (expects 0-based)");
+                return SPR_E_ARG;
+            }
+            int cc[3];
+            for (int k = 0; k < 3; ++k) {
+                long m = ((long)ix[k] % c.mesh[k] + c.mesh[k]) % c.mesh[k];
+                cc[k] = (int)((m * (c.L / c.mesh[k])) % c.L);
+            }
+            site[site_off[mu] + n] = make_int4(cc[0], cc[1], cc[2], ix[3]);
+        }
+    const int nblk = order == 3 ? nb * nb * nb : nb * nb * nb * nb;
+    std::vector<std::vector<int>> tup(nblk);
+    std::vector<std::vector<double>> val(nblk);
+    auto basis = [&](int mu, int n) { return f->atm_indx[((size_t)mu * natm + n) * 4 + 3]; };
+    for (int mu = 0; mu < nb; ++mu) {
+        const int na = f->atm_num[mu];
+        const int n3max = order == 4 ? na : 1;
+        for (int n1 = 0; n1 < na; ++n1)
+            for (int n2 = 0; n2 < na; ++n2)
+                for (int n3 = 0; n3 < n3max; ++n3) {
+                    const double *src = order == 3
+                        ? f->fc + (((size_t)mu * natm + n1) * natm + n2) * 27
+                        : f->fc + ((((size_t)mu * natm + n1) * natm + n2) * natm + n3) * 81;
+                    bool nz = false;
+                    for (int i = 0; i < ncomp; ++i) nz |= src[i] != 0.0;
+                    if (!nz) continue;
+                    const int nu = basis(mu, n1), eta = basis(mu, n2);
+                    double m = osm[mu] * osm[nu] * osm[eta];
+                    int b;
+                    if (order == 3) {
+                        b = (mu * nb + nu) * nb + eta;
+                    } else {
+                        const int rho = basis(mu, n3);
+                        m *= osm[rho];
+                        b = ((mu * nb + nu) * nb + eta) * nb + rho;
+                    }
+                    tup[b].push_back(site_off[mu] + n1);
+                    tup[b].push_back(site_off[mu] + n2);
+                    if (order == 4) tup[b].push_back(site_off[mu] + n3);
+                    for (int i = 0; i < ncomp; ++i) val[b].push_back(src[i] * m);
+                }
+    }
+    std::vector<int> start(nblk + 1, 0), all_tup;
+    std::vector<double> all_val;
+    for (int b = 0; b < nblk; ++b) {
+        start[b + 1] = start[b] + (int)(tup[b].size() / nsub);
+        all_tup.insert(all_tup.end(), tup[b].begin(), tup[b].end());
+        all_val.insert(all_val.end(), val[b].begin(), val[b].end());
+    }
+    d.order = order;
+    d.nblk = nblk;
+    d.nterm = start[nblk];
+    d.nsite = nsite;
+    d.h_blk_start = start;
+    std::vector<float> v32(all_val.begin(), all_val.end());
+    int rc;
+    if ((rc = upload(&d.blk_start, start.data(), start.size()))) return rc;
+    if ((rc = upload(&d.tup_site, all_tup.data(), all_tup.size()))) return rc;
+    if ((rc = upload(&d.fc64, all_val.data(), all_val.size()))) return rc;
+    if ((rc = upload(&d.fc32, v32.data(), v32.size()))) return rc;
+    if ((rc = upload(&d.site, site.data(), site.size()))) return rc;
+    return SPR_OK;
+}
+
+static void free_ifc(DevIFC &d) {
+    (void)hipFree(d.blk_start);
+    (void)hipFree(d.tup_site);
+    (void)hipFree(d.fc64);
+    (void)hipFree(d.fc32);
+    (void)hipFree(d.site);
+    d = DevIFC();
+}
+
+}  // namespace spr
+
+using namespace spr;
+
+struct spr_ctx {
+    Ctx c;
+};
+
+extern "C" const char *spr_last_error(void) { return spr::g_err.c_str(); }
+
+extern "C" int spr_ctx_create(const spr_crystal *cr, const spr_ifc *fc3, const spr_ifc *fc4,
+                              const spr_mesh *mesh, const spr_phonons *ph, int device,
+                              spr_ctx **out) {
+    if (!out || !cr || !mesh || !ph || !fc3) {
+        set_error("spr_ctx_create: NULL argument");
+        return SPR_E_ARG;
+    }
+    *out = nullptr;
+    if (cr->nb < 1 || ph->ndof != 3 * cr->nb || ph->nq != mesh->nq ||
+        mesh->nq != mesh->mesh[0] * mesh->mesh[1] * mesh->mesh[2] || mesh->nq0 < 1) {
+        set_error("spr_ctx_create: inconsistent sizes (ndof != 3 nb, or nq != prod(mesh))");
+        return SPR_E_ARG;
+    }
+    int ndev = 0;
+    SPR_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) {
+        set_error("spr_ctx_create: no HIP device " + std::to_string(device));
+        return SPR_E_ARG;
+    }
+    SPR_HIP(hipSetDevice(device));
+    auto *h = new spr_ctx();
+    Ctx &c = h->c;
+    c.device = device;
+    c.nb = cr->nb;
+    c.nd = 3 * cr->nb;
+    c.nq = mesh->nq;
+    c.nq0 = mesh->nq0;
+    c.vol = cr->vol;
+    for (int k = 0; k < 3; ++k) c.mesh[k] = mesh->mesh[k];
+    c.L = (int)lcm(lcm(c.mesh[0], c.mesh[1]), c.mesh[2]);
+    int rc = SPR_OK;
+    auto fail = [&](int r) {
+        spr_ctx_destroy(h);
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c.ev0) != hipSuccess || hipEventCreate(&c.ev1) != hipSuccess) {
+        set_error("spr_ctx_create: stream/event creation failed");
+        return fail(SPR_E_HIP);
+    }
+    std::vector<double> osm(c.nb);
+    for (int i = 0; i < c.nb; ++i) {
+        if (!(cr->mass[i] > 0.0)) {
+            set_error("spr_ctx_create: masses must be positive");
+            return fail(SPR_E_ARG);
+        }
+        osm[i] = 1.0 / std::sqrt(cr->mass[i]);
+    }
+    if ((rc = build_ifc(c, fc3, osm.data(), c.fc3))) return fail(rc);
+    if (fc4 && (rc = build_ifc(c, fc4, osm.data(), c.fc4))) return fail(rc);
+    // phase table exp(2 pi i m / L)
+    std::vector<double2> tab(c.L);
+    std::vector<float2> tab32(c.L);
+    for (int m = 0; m < c.L; ++m) {
+        const double x = 2.0 * M_PI * (double)m / (double)c.L;
+        tab[m] = make_double2(std::cos(x), std::sin(x));
+        tab32[m] = make_float2((float)tab[m].x, (float)tab[m].y);
+    }
+    if ((rc = upload(&c.phase64, tab.data(), tab.size()))) return fail(rc);
+    if ((rc = upload(&c.phase32, tab32.data(), tab32.size()))) return fail(rc);
+    const size_t ne = (size_t)c.nq * c.nd * c.nd;
+    if ((rc = upload(&c.evec, reinterpret_cast<const double2 *>(ph->evec), ne))) return fail(rc);
+    {
+        std::vector<float2> e32(ne);
+        for (size_t i = 0; i < ne; ++i) e32[i] = make_float2((float)ph->evec[2 * i], (float)ph->evec[2 * i + 1]);
+        if ((rc = upload(&c.evec32, e32.data(), ne))) return fail(rc);
+    }
+    const size_t nm = (size_t)c.nq * c.nd;
+    if ((rc = upload(&c.omega, ph->omega, nm))) return fail(rc);
+    if ((rc = upload(&c.inv_omega, ph->inv_omega, nm))) return fail(rc);
+    if ((rc = upload(&c.nbe, ph->nbe, nm))) return fail(rc);
+    c.h_omega.assign(ph->omega, ph->omega + nm);
+    c.h_nbe.assign(ph->nbe, ph->nbe + nm);
+    if (ph->grp_vel) {
+        if ((rc = upload(&c.vel, ph->grp_vel, nm * 3))) return fail(rc);
+        c.h_vel.assign(ph->grp_vel, ph->grp_vel + nm * 3);
+    }
+    for (int i = 0; i < c.nq0; ++i)
+        if (mesh->irr_grid[i] < 0 || mesh->irr_grid[i] >= c.nq) {
+            set_error("spr_ctx_create: irr_grid out of range");
+            return fail(SPR_E_ARG);
+        }
+    c.h_irr_grid.assign(mesh->irr_grid, mesh->irr_grid + c.nq0);
+    if (mesh->pos) c.h_pos.assign(mesh->pos, mesh->pos + c.nq);
+    if (mesh->multiplicity) c.h_mult.assign(mesh->multiplicity, mesh->multiplicity + c.nq0);
+    c.nsym = mesh->nsym;
+    if (mesh->nsym > 0 && mesh->rot) c.h_rot.assign(mesh->rot, mesh->rot + 9 * mesh->nsym);
+    if ((rc = upload(&c.irr_grid, mesh->irr_grid, c.nq0))) return fail(rc);
+    if (hipMalloc((void **)&c.counter, 64 * sizeof(int)) != hipSuccess) {
+        set_error("hipMalloc counter failed");
+        return fail(SPR_E_OOM);
+    }
+    *out = h;
+    return SPR_OK;
+}
+
+extern "C" int spr_ctx_destroy(spr_ctx *h) {
+    if (!h) return SPR_OK;
+    Ctx &c = h->c;
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    free_ifc(c.fc3);
+    free_ifc(c.fc4);
+    for (void *p : {(void *)c.phase64, (void *)c.phase32, (void *)c.evec, (void *)c.evec32,
+                    (void *)c.omega, (void *)c.inv_omega, (void *)c.nbe, (void *)c.vel,
+                    (void *)c.irr_grid, (void *)c.partial, (void *)c.out, (void *)c.counter})
+        if (p) (void)hipFree(p);
+    if (c.ev0) (void)hipEventDestroy(c.ev0);
+    if (c.ev1) (void)hipEventDestroy(c.ev1);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    delete h;
+    return SPR_OK;
+}
+
+static int check_i0(Ctx &c, int i0) {
+    if (i0 < 0 || i0 >= c.nq0) {
+        set_error("irreducible index i0 out of range");
+        return SPR_E_ARG;
+    }
+    return SPR_OK;
+}
+
+static int check_prec(int prec) {
+    if (prec != SPR_PREC_FP64 && prec != SPR_PREC_FP32 && prec != SPR_PREC_BF16) {
+        set_error("unknown precision mode");
+        return SPR_E_ARG;
+    }
+    return SPR_OK;
+}
+
+extern "C" int spr_v3_w(spr_ctx *h, int i0, double *W_out) {
+    if (!h || !W_out) {
+        set_error("spr_v3_w: NULL argument");
+        return SPR_E_ARG;
+    }
+    Ctx &c = h->c;
+    int rc;
+    if ((rc = check_i0(c, i0))) return rc;
+    SPR_HIP(hipSetDevice(c.device));
+    const size_t n = (size_t)c.nd * c.nd * c.nd * c.nq;
+    if ((rc = ensure((void **)&c.out, &c.out_cap, n * sizeof(double) + 64 * sizeof(double)))) return rc;
+    if ((rc = launch_lw3(c, c.irr_grid + i0, 1, 1.0, SPR_PREC_FP64, nullptr, c.out))) return rc;
+    SPR_HIP(hipMemcpyAsync(W_out, c.out, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SPR_HIP(hipStreamSynchronize(c.stream));
+    return SPR_OK;
+}
+
+static int lw_common(spr_ctx *h, int first, int count, double sigma, int prec, double *lw, int order) {
+    if (!h || !lw) {
+        set_error("linewidth: NULL argument");
+        return SPR_E_ARG;
+    }
+    Ctx &c = h->c;
+    int rc;
+    if ((rc = check_prec(prec))) return rc;
+    if (!(sigma > 0.0)) {
+        set_error("sigma must be positive");
+        return SPR_E_ARG;
+    }
+    if (order == 4 && c.fc4.nterm == 0 && c.fc4.order == 0) {
+        set_error("no FC4 was given to spr_ctx_create");
+        return SPR_E_STATE;
+    }
+    SPR_HIP(hipSetDevice(c.device));
+    const size_t n = (size_t)count * c.nd;
+    if ((rc = ensure((void **)&c.out, &c.out_cap, n * sizeof(double)))) return rc;
+    rc = order == 3 ? launch_lw3(c, c.irr_grid + first, count, sigma, prec, c.out, nullptr)
+                    : launch_lw4(c, c.irr_grid + first, count, sigma, prec, c.out);
+    if (rc) return rc;
+    SPR_HIP(hipMemcpyAsync(lw, c.out, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SPR_HIP(hipStreamSynchronize(c.stream));
+    return SPR_OK;
+}
+
+extern "C" int spr_lw3(spr_ctx *h, int i0, double sigma, int prec, double *lw) {
+    if (!h) return SPR_E_ARG;
+    int rc;
+    if ((rc = check_i0(h->c, i0))) return rc;
+    return lw_common(h, i0, 1, sigma, prec, lw, 3);
+}
+
+extern "C" int spr_lw3_all(spr_ctx *h, double sigma, int prec, double *lw_all) {
+    if (!h) return SPR_E_ARG;
+    return lw_common(h, 0, h->c.nq0, sigma, prec, lw_all, 3);
+}
+
+extern "C" int spr_lw4(spr_ctx *h, int i0, double sigma, int prec, double *lw) {
+    if (!h) return SPR_E_ARG;
+    int rc;
+    if ((rc = check_i0(h->c, i0))) return rc;
+    return lw_common(h, i0, 1, sigma, prec, lw, 4);
+}
+
+extern "C" int spr_lw4_all(spr_ctx *h, double sigma, int prec, double *lw_all) {
+    if (!h) return SPR_E_ARG;
+    return lw_common(h, 0, h->c.nq0, sigma, prec, lw_all, 4);
+}
+
+extern "C" int spr_kernel_time(spr_ctx *h, int *launches, double *ms) {
+    if (!h) return SPR_E_ARG;
+    if (launches) *launches = h->c.launches;
+    if (ms) *ms = h->c.kernel_ms;
+    return SPR_OK;
+}
+
+extern "C" int spr_kernel_time_reset(spr_ctx *h) {
+    if (!h) return SPR_E_ARG;
+    h->c.launches = 0;
+    h->c.kernel_ms = 0.0;
+    return SPR_OK;
+}
+
+extern "C" int spr_kappa_rta(spr_ctx *h, const double *lw_all, double T, int symm, double *kappa,
+                             double *mode_k) {
+    if (!h || !lw_all || !kappa) {
+        set_error("spr_kappa_rta: NULL argument");
+        return SPR_E_ARG;
+    }
+    if (!(T > 0.0)) {
+        set_error("spr_kappa_rta: T must be positive");
+        return SPR_E_ARG;
+    }
+    Ctx &c = h->c;
+    SPR_HIP(hipSetDevice(c.device));
+    const size_t nl = (size_t)c.nq0 * c.nd, nm = (size_t)c.nd * 9;
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, (nl + nm) * sizeof(double)) != hipSuccess) {
+        set_error("spr_kappa_rta: hipMalloc failed");
+        return SPR_E_OOM;
+    }
+    int rc = SPR_OK;
+    std::vector<double> modes(nm);
+    if (hipMemcpy(d, lw_all, nl * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("spr_kappa_rta: upload failed");
+        rc = SPR_E_HIP;
+    }
+    if (!rc) rc = launch_kappa(c, d, T, symm, d + nl);
+    if (!rc && hipMemcpy(modes.data(), d + nl, nm * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("spr_kappa_rta: download failed");
+        rc = SPR_E_HIP;
+    }
+    (void)hipFree(d);
+    if (rc) return rc;
+    for (int k = 0; k < 9; ++k) {
+        double s = 0.0;
+        for (int m = 0; m < c.nd; ++m) s += modes[(size_t)m * 9 + k];
+        kappa[k] = s;
+    }
+    if (mode_k) std::memcpy(mode_k, modes.data(), nm * sizeof(double));
+    return SPR_OK;
+}

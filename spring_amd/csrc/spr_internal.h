@@ -1,0 +1,79 @@
+// Internal declarations of libspring_gpu.so (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/spring_gpu.h"
+
+namespace spr {
+
+void set_error(const std::string &msg);
+
+#define SPR_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            ::spr::set_error(std::string(#call) + ": " + hipGetErrorString(e_));        \
+            return SPR_E_HIP;                                                             \
+        }                                                                                 \
+    } while (0)
+
+// Anharmonic IFC re-laid out for the phase sum.  The sum over (N1, N2[, N3]) is grouped by
+// the basis atoms of the sites: for every (mu, nu, eta[, rho]) "block" the list of
+// contributing site tuples, each with its 27 (81) FC values premultiplied by
+// 1/sqrt(m_mu m_nu m_eta [m_rho]).  Only tuples with a nonzero FC block are kept.
+struct DevIFC {
+    int order = 0;
+    int nblk = 0;            // Nb^(order)
+    int nterm = 0;           // number of (mu, N1, N2[, N3]) tuples kept
+    int nsite = 0;           // sum_mu atm_num[mu]
+    int *blk_start = nullptr;  // (nblk + 1) offsets into the tuple list
+    int *tup_site = nullptr;   // (nterm, order-1) global site index of N1, N2[, N3]
+    double *fc64 = nullptr;    // (nterm, 3^order) mass-scaled FC values, (a,b,c[,d]) order
+    float *fc32 = nullptr;
+    int4 *site = nullptr;      // (nsite): phase coefficients (c1,c2,c3) and basis atom
+    std::vector<int> h_blk_start;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int nb = 0, nd = 0, nq = 0, nq0 = 0;
+    int mesh[3] = {0, 0, 0};
+    int L = 0;                     // phase table length: lcm of the mesh dimensions
+    double vol = 0.0;
+    DevIFC fc3, fc4;
+    double2 *phase64 = nullptr;    // (L) exp(2 pi i m / L)
+    float2 *phase32 = nullptr;
+    double2 *evec = nullptr;       // (nq, nd, nd): evec[q][s][row] = Evec(row, s, q)
+    float2 *evec32 = nullptr;
+    double *omega = nullptr, *inv_omega = nullptr, *nbe = nullptr, *vel = nullptr;
+    int *irr_grid = nullptr;       // (nq0)
+    std::vector<int> h_irr_grid, h_pos, h_mult;
+    std::vector<double> h_rot;
+    int nsym = 0;
+    std::vector<double> h_omega, h_nbe, h_vel;
+    // scratch
+    double *partial = nullptr;
+    size_t partial_cap = 0;
+    double *out = nullptr;
+    size_t out_cap = 0;
+    int *counter = nullptr;
+    // timing of the dominant kernel
+    int launches = 0;
+    double kernel_ms = 0.0;
+};
+
+int ensure(void **p, size_t *cap, size_t bytes);
+
+// kernels launched from the API layer
+int launch_lw3(Ctx &c, const int *d_q0_grid, int nq0, double sigma, int prec, double *d_lw,
+               double *d_w_out);
+int launch_lw4(Ctx &c, const int *d_q0_grid, int nq0, double sigma, int prec, double *d_lw);
+int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_mode_k);
+
+}  // namespace spr

@@ -1,0 +1,182 @@
+"""ctypes binding of libspring_gpu.so (include/spring_gpu.h).
+
+The product path has no CPU fallback: if the library is missing or no HIP device is
+present, constructing a context raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspring_gpu.so")
+
+PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
+
+SYMBOLS = [
+    "spr_ctx_create", "spr_ctx_destroy", "spr_last_error", "spr_v3_w", "spr_lw3",
+    "spr_lw3_all", "spr_lw4", "spr_lw4_all", "spr_kappa_rta", "spr_kernel_time",
+    "spr_kernel_time_reset",
+]
+
+
+class SprError(RuntimeError):
+    pass
+
+
+class _Crystal(ct.Structure):
+    _fields_ = [("nb", ct.c_int), ("mass", ct.c_void_p), ("vol", ct.c_double)]
+
+
+class _IFC(ct.Structure):
+    _fields_ = [("order", ct.c_int), ("natm", ct.c_int), ("atm_num", ct.c_void_p),
+                ("atm_indx", ct.c_void_p), ("fc", ct.c_void_p)]
+
+
+class _Mesh(ct.Structure):
+    _fields_ = [("mesh", ct.c_int * 3), ("nq", ct.c_int), ("q_int", ct.c_void_p),
+                ("nq0", ct.c_int), ("irr_grid", ct.c_void_p), ("pos", ct.c_void_p),
+                ("multiplicity", ct.c_void_p), ("nsym", ct.c_int), ("rot", ct.c_void_p)]
+
+
+class _Phonons(ct.Structure):
+    _fields_ = [("nq", ct.c_int), ("ndof", ct.c_int), ("evec", ct.c_void_p),
+                ("omega", ct.c_void_p), ("inv_omega", ct.c_void_p), ("nbe", ct.c_void_p),
+                ("grp_vel", ct.c_void_p)]
+
+
+_lib = None
+
+
+def load():
+    """Load libspring_gpu.so; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SprError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                           "(or python -m spring_amd.build)")
+        lib = ct.CDLL(LIB_PATH)
+        lib.spr_last_error.restype = ct.c_char_p
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+class GpuContext:
+    """One spr_ctx: model + mesh + phonons resident on the GPU."""
+
+    def __init__(self, model, qm, ph, device: int = 0, with_fc4: bool | None = None):
+        self.lib = load()
+        cr = model.crystal
+        self.nd = cr.ndof
+        self.nq0 = qm.nq0
+        self.nq = qm.nq
+        keep = []
+
+        def arr(x, dt):
+            a = np.ascontiguousarray(x, dtype=dt)
+            keep.append(a)
+            return a
+
+        mass = arr(cr.mass, np.float64)
+        self._cr = _Crystal(cr.nb, _ptr(mass), cr.vol)
+
+        def ifc(f):
+            if f is None:
+                return None
+            s = _IFC(f.order, f.natm_max, _ptr(arr(f.atm_num, np.int32)),
+                     _ptr(arr(f.atm_indx, np.int32)), _ptr(arr(f.fc, np.float64)))
+            return s
+
+        self._fc3 = ifc(model.fc3)
+        use4 = model.fc4 is not None if with_fc4 is None else with_fc4
+        self._fc4 = ifc(model.fc4) if use4 else None
+        rot = qm.rot_cart
+        if qm.time_rev:
+            rot = np.concatenate([rot, -rot])
+        # Fortran Rxyz(3,3,nsym): R(a,b) at a + 3 b  -> C array [r][b][a] = R^T rows
+        rotf = arr(np.transpose(rot, (0, 2, 1)), np.float64)
+        self._mesh = _Mesh((ct.c_int * 3)(*[int(x) for x in qm.mesh]), qm.nq,
+                           _ptr(arr(qm.q_int, np.int32)), qm.nq0, _ptr(arr(qm.irr_idx, np.int32)),
+                           _ptr(arr(qm.pos, np.int32)), _ptr(arr(qm.multiplicity, np.int32)),
+                           rot.shape[0], _ptr(rotf))
+        self._ph = _Phonons(ph.nq, self.nd, _ptr(arr(ph.evec, np.complex128)),
+                            _ptr(arr(ph.omega, np.float64)), _ptr(arr(ph.inv_omega, np.float64)),
+                            _ptr(arr(ph.nbe, np.float64)), _ptr(arr(ph.grp_vel, np.float64)))
+        self._keep = keep
+        h = ct.c_void_p()
+        rc = self.lib.spr_ctx_create(ct.byref(self._cr), ct.byref(self._fc3),
+                                     ct.byref(self._fc4) if self._fc4 is not None else None,
+                                     ct.byref(self._mesh), ct.byref(self._ph), ct.c_int(device),
+                                     ct.byref(h))
+        self._check(rc, "spr_ctx_create")
+        self.h = h
+        self._keep = None   # the library holds device copies now
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise SprError(f"{what} failed ({rc}): {self.lib.spr_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.spr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- entry points ------------------------------------------------------------------
+    def v3_w(self, i0: int) -> np.ndarray:
+        """Sctrq1(s2,s1,s0,q1) as numpy [q1][s0][s1][s2]."""
+        W = np.zeros((self.nq, self.nd, self.nd, self.nd))
+        self._check(self.lib.spr_v3_w(self.h, ct.c_int(i0), _ptr(W)), "spr_v3_w")
+        return W
+
+    def lw3(self, i0: int, sigma: float, prec: str = "fp64") -> np.ndarray:
+        lw = np.zeros(self.nd)
+        self._check(self.lib.spr_lw3(self.h, ct.c_int(i0), ct.c_double(sigma),
+                                     ct.c_int(PREC[prec]), _ptr(lw)), "spr_lw3")
+        return lw
+
+    def lw3_all(self, sigma: float, prec: str = "fp64") -> np.ndarray:
+        lw = np.zeros((self.nq0, self.nd))
+        self._check(self.lib.spr_lw3_all(self.h, ct.c_double(sigma), ct.c_int(PREC[prec]),
+                                         _ptr(lw)), "spr_lw3_all")
+        return lw
+
+    def lw4(self, i0: int, sigma: float, prec: str = "fp64") -> np.ndarray:
+        lw = np.zeros(self.nd)
+        self._check(self.lib.spr_lw4(self.h, ct.c_int(i0), ct.c_double(sigma),
+                                     ct.c_int(PREC[prec]), _ptr(lw)), "spr_lw4")
+        return lw
+
+    def lw4_all(self, sigma: float, prec: str = "fp64") -> np.ndarray:
+        lw = np.zeros((self.nq0, self.nd))
+        self._check(self.lib.spr_lw4_all(self.h, ct.c_double(sigma), ct.c_int(PREC[prec]),
+                                         _ptr(lw)), "spr_lw4_all")
+        return lw
+
+    def kappa(self, lw_all: np.ndarray, T: float, symm: bool = True):
+        k = np.zeros(9)
+        modes = np.zeros((self.nd, 9))
+        lw = np.ascontiguousarray(lw_all, np.float64)
+        self._check(self.lib.spr_kappa_rta(self.h, _ptr(lw), ct.c_double(T), ct.c_int(int(symm)),
+                                           _ptr(k), _ptr(modes)), "spr_kappa_rta")
+        return k.reshape(3, 3), modes
+
+    def kernel_time(self):
+        n = ct.c_int()
+        ms = ct.c_double()
+        self._check(self.lib.spr_kernel_time(self.h, ct.byref(n), ct.byref(ms)), "spr_kernel_time")
+        return n.value, ms.value
+
+    def kernel_time_reset(self):
+        self._check(self.lib.spr_kernel_time_reset(self.h), "spr_kernel_time_reset")

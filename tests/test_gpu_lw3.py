@@ -1,0 +1,98 @@
+"""GPU parity of the three-phonon path against the FP64 CPU oracle (same inputs)."""
+import numpy as np
+import pytest
+
+from tests.conftest import problem, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small(rocksalt):
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    qm, ph = problem(rocksalt, (4, 4, 4))
+    ctx = GpuContext(rocksalt, qm, ph)
+    return rocksalt, qm, ph, ctx, O.Inputs(rocksalt, qm, ph)
+
+
+def test_w_parity_fp64(small):
+    from oracle import oracle as O
+    m, qm, ph, ctx, inp = small
+    for i0 in range(qm.nq0):
+        print(f"W parity i0={i0}", flush=True)
+        W = ctx.v3_w(i0)
+        _, Wo = O.lw3_gauss(inp, int(qm.irr_idx[i0]), 0.1, want_w=True)
+        assert rel_err(W, Wo, 1e-12) < 1e-10
+
+
+def test_lw3_parity_fp64_4(small):
+    from oracle import oracle as O
+    m, qm, ph, ctx, inp = small
+    lw = ctx.lw3_all(0.1, "fp64")
+    ref = O.lw3_gauss_all(inp, qm.irr_idx, 0.1)
+    assert rel_err(lw, ref, 1e-9) < 1e-10
+
+
+def test_lw3_single_matches_all(small):
+    m, qm, ph, ctx, inp = small
+    lw_all = ctx.lw3_all(0.1, "fp64")
+    for i0 in (0, qm.nq0 - 1):
+        assert np.allclose(ctx.lw3(i0, 0.1, "fp64"), lw_all[i0], rtol=1e-12, atol=0)
+
+
+def test_lw3_deterministic(small):
+    m, qm, ph, ctx, inp = small
+    a = ctx.lw3_all(0.1, "fp64")
+    b = ctx.lw3_all(0.1, "fp64")
+    assert np.array_equal(a, b)
+
+
+def test_lw3_fp32_close(small):
+    m, qm, ph, ctx, inp = small
+    ref = ctx.lw3_all(0.1, "fp64")
+    lw = ctx.lw3_all(0.1, "fp32")
+    assert rel_err(lw, ref, 1e-6) < 1e-4
+
+
+def test_lw3_parity_cfg1_odd_and_even(rocksalt):
+    """8^3 (the reference's own CPU-runnable configuration) and an odd 5^3 mesh."""
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    for dims in ((5, 5, 5), (8, 8, 8)):
+        qm, ph = problem(rocksalt, dims)
+        ctx = GpuContext(rocksalt, qm, ph)
+        inp = O.Inputs(rocksalt, qm, ph)
+        lw = ctx.lw3_all(0.1, "fp64")
+        ref = O.lw3_gauss_all(inp, qm.irr_idx, 0.1, threads=8)
+        print(dims, "max rel", rel_err(lw, ref, 1e-9), flush=True)
+        assert rel_err(lw, ref, 1e-9) < 1e-10
+        ctx.close()
+
+
+def test_kappa_parity(rocksalt):
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    qm, ph = problem(rocksalt, (5, 5, 5))
+    ctx = GpuContext(rocksalt, qm, ph)
+    lw = ctx.lw3_all(0.1, "fp64")
+    k_full, _ = ctx.kappa(lw, 300.0, symm=False)
+    k_sym, _ = ctx.kappa(lw, 300.0, symm=True)
+    ref_full = O.kappa_rta(qm, ph, lw, 300.0, rocksalt.crystal.vol)
+    ref_sym = O.kappa_rta_symm(qm, ph, lw, 300.0, rocksalt.crystal.vol)
+    assert np.allclose(k_full, ref_full, rtol=1e-12, atol=1e-14 * np.abs(ref_full).max())
+    assert np.allclose(k_sym, ref_sym, rtol=1e-12, atol=1e-14 * np.abs(ref_sym).max())
+
+
+def test_lw3_parity_10atom(rocksalt10):
+    """Ndof = 30 on a 2x2x2 mesh, P1 crystal (all 8 q0)."""
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    qm, ph = problem(rocksalt10, (2, 2, 2))
+    ctx = GpuContext(rocksalt10, qm, ph)
+    inp = O.Inputs(rocksalt10, qm, ph)
+    lw = ctx.lw3_all(0.1, "fp64")
+    ref = O.lw3_gauss_all(inp, qm.irr_idx, 0.1, threads=8)
+    assert rel_err(lw, ref, 1e-9) < 1e-10
+    lw32 = ctx.lw3_all(0.1, "fp32")
+    assert rel_err(lw32, ref, 1e-6) < 1e-3

@@ -1,0 +1,151 @@
+"""Benchmark: three-phonon scattering rates on one MI355X.
+
+Workload (default): 2-atom rocksalt model, 16x16x16 Gamma-centred mesh (145 irreducible
+q0 under Oh, 4096 q1 each, 6^3 branch triplets per pair), Gaussian sigma = 0.1 rad/ps,
+T = 300 K.  One step = the linewidths of every irreducible q0 (all q1, pair-permutation
+reduced) plus the RTA kappa tensor, from device-resident inputs.
+
+Prints one JSON line:
+  metric/value   |V3|^2 triplets per second (sum over irreducible q0 of Nq * Ndof^3 per
+                 step, divided by the step time), FP64 path
+  roofline       the dominant kernel's algorithmic FLOP rate (HIP events on the library's
+                 stream) against MI355X's FP64 peak
+  cpu_baseline   the FP64 CPU oracle timed on a bounded sample of q0 on the host cores
+  fp32           the FP32 path (compensated FP32 delta sum): rate, speedup, error vs FP64
+  kappa_tts_s    wall time from FC + structure in memory to kappa (phonons included)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and matrix are the same rate)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--mesh", type=int, default=16)
+    p.add_argument("--sigma", type=float, default=0.1)
+    p.add_argument("--T", type=float, default=300.0)
+    p.add_argument("--cpu-q0", type=int, default=16, help="irreducible q0 in the CPU sample")
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    from spring_amd import model, mesh, phonons, lib
+
+    t_in0 = time.perf_counter()
+    m = model.rocksalt2()
+    t_model = time.perf_counter() - t_in0
+    n = args.mesh
+    t0 = time.perf_counter()
+    qm = mesh.make_mesh(m.crystal, (n, n, n))
+    ph = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
+    ctx = lib.GpuContext(m, qm, ph)
+    lw = ctx.lw3_all(args.sigma, "fp64")          # also builds the pair lists
+    kap, _ = ctx.kappa(lw, args.T, symm=True)
+    kappa_tts = time.perf_counter() - t0
+
+    nd = m.crystal.ndof
+    triplets = qm.nq0 * qm.nq * nd ** 3
+
+    def run(prec, steps, warmup):
+        for _ in range(warmup):
+            l = ctx.lw3_all(args.sigma, prec)
+            ctx.kappa(l, args.T, symm=True)
+        ctx.kernel_time_reset()
+        t = time.perf_counter()
+        for _ in range(steps):
+            l = ctx.lw3_all(args.sigma, prec)
+            k, _ = ctx.kappa(l, args.T, symm=True)
+        dt = time.perf_counter() - t
+        nl, kms = ctx.kernel_time()
+        return dt / steps, kms / max(nl, 1), l, k
+
+    step64, kern64, lw64, k64 = run("fp64", args.steps, args.warmup)
+    step32, kern32, lw32, k32 = run("fp32", args.steps, args.warmup)
+
+    # algorithmic FLOP per executed pair: phase sum over the nonzero FC3 blocks (4 flop per
+    # real x complex MAC, 27 per block) + three complex mode products (24 Ndof^4) + ~30 per
+    # branch triplet for |V|^2, 1/(w w w) and the three Gaussians
+    nblocks = int(np.count_nonzero(np.abs(m.fc3.fc).sum(axis=(3, 4, 5))))
+    flop_pair = 4 * 27 * nblocks + 24 * nd ** 4 + 30 * nd ** 3
+    npairs = sum(len(mesh.pair_list(qm, int(i))[0]) for i in qm.irr_idx)
+    achieved = flop_pair * npairs / (kern64 * 1e-3) / 1e12
+
+    err_k = float(np.max(np.abs(k32 - k64)) / np.max(np.abs(k64)))
+    sig = np.abs(lw64) > 1e-6 * np.abs(lw64).max()
+    err_lw = float(np.max(np.abs(lw32 - lw64)[sig] / np.abs(lw64)[sig]))
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        inp = O.Inputs(m, qm, ph)
+        threads = max(1, min(16, os.cpu_count() or 1))
+        rng = np.random.default_rng(0)
+        sample = rng.choice(qm.irr_idx, size=min(args.cpu_q0, qm.nq0), replace=False)
+        t = time.perf_counter()
+        O.lw3_gauss_all(inp, sample, args.sigma, threads=threads)
+        dt = time.perf_counter() - t
+        cpu = {"value": len(sample) * qm.nq * nd ** 3 / dt, "unit": "triplets/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{len(sample)} seeded irreducible q0 of the same workload, all {qm.nq} q1 "
+                         f"each (no pair-permutation saving), FP64 C restatement, {threads} threads, "
+                         f"{dt:.1f} s wall"}
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "lw3_small_fp64_rocksalt16_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+
+    out = {
+        "metric": "|V3|^2 triplets/s and kappa time-to-solution at <=1e-3 rel err vs FP64 full sum",
+        "value": triplets / step64,
+        "unit": "triplets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step64 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded rocksalt spring model + random permutation-symmetric ASR FC3",
+        "config": {"workload": f"3-phonon Gaussian linewidths + RTA kappa, 2-atom rocksalt, {n}^3 mesh, "
+                               f"{qm.nq0} irreducible q0 x {qm.nq} q1 x {nd}^3 branches",
+                   "mesh": n, "nq0": qm.nq0, "ndof": nd, "sigma_rad_per_ps": args.sigma, "T_K": args.T,
+                   "pairs_executed_per_step": npairs, "triplets_per_step": triplets},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "k_lw3_small<double,2>", "kernel_ms": kern64,
+                     "flop_per_pair": flop_pair, "pairs_per_launch": npairs},
+        "cpu_baseline": cpu,
+        "fp32": {"value": triplets / step32, "ms_per_step": step32 * 1e3, "kernel_ms": kern32,
+                 "speedup_vs_fp64": step64 / step32, "kappa_rel_err_vs_fp64": err_k,
+                 "linewidth_max_rel_err_vs_fp64": err_lw},
+        "kappa_tts_s": kappa_tts,
+        "kappa_W_mK": [float(k64[0, 0]), float(k64[1, 1]), float(k64[2, 2])],
+        "host_model_s": t_model,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,193 @@
+! kappa_gpu -- Fortran host driver of the GPU scattering-rate path.
+!
+! Follows the three(+four)-phonon part of src/Kappa/Src/main.f90: the irreducible-q0
+! loop (:552-715) producing LW_Allq0 and the per-q0 lines of linewidth_IBZ_<prefix><T>K.txt
+! (WriteAsciiOutFile, Helper_m.f90:597-667, same header and G20.12 columns), then the RTA
+! kappa (kappa_noIter_symm by default, -Symm of parse_cmd_line.f90) printed as in
+! main.f90:799-819.  Inputs come from a directory written by spring_amd.bundle.write_bundle
+! (little-endian raw arrays in the reference's column-major layout plus a namelist
+! `bundle.nml`), because the HDF5 Fortran module is not available to this compiler.
+!
+! usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all]
+!   loop: one spr_lw3 call per q0 (the reference's q0 loop); all: one spr_lw3_all launch
+program kappa_gpu
+
+    use spring_gpu_m
+    implicit none
+
+    character(len=512)      :: dir, arg
+    character(len=64)       :: prefix
+    integer                 :: Nbasis, Natm3, Natm4, mesh(3), Nq, Nq0, Nsym, FourPhonon, prec, i0, u
+    real(dp)                :: T, Sigma, vol
+    integer, allocatable    :: atmNum3(:), atmIndx3(:,:,:), atmNum4(:), atmIndx4(:,:,:)
+    integer, allocatable    :: q_int(:,:), irr_grid(:), pos(:), mult(:)
+    real(dp), allocatable   :: mass(:), fC3(:,:,:,:,:,:), fC4(:,:,:,:,:,:,:,:), Rxyz(:,:,:)
+    complex(dp), allocatable :: Evec(:,:,:)
+    real(dp), allocatable   :: omega(:,:), OnebyOmega(:,:), nBE(:,:), grp_vel(:,:,:)
+    real(dp), allocatable   :: LW_Allq0(:,:), lw4(:,:), Mode_k(:,:), lw(:)
+    real(dp)                :: kappa(9)
+    type(gpu_ctx)           :: ctx
+    logical                 :: HeaderWrite, loop_mode
+    character(len=32)       :: Temp_char, frmt, Ndof_char
+    character(len=600)      :: outfile_ascii, outfile_frmt
+    integer                 :: Ndof
+
+    namelist /bundle/ prefix, Nbasis, Natm3, Natm4, mesh, Nq, Nq0, Nsym, T, Sigma, vol, FourPhonon
+
+    call get_command_argument(1, dir)
+    if ( len_trim(dir) == 0 ) then
+        write(*, *) 'usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all]'
+        ERROR STOP
+    end if
+    prec = SPR_PREC_FP64
+    call get_command_argument(2, arg)
+    if ( trim(arg) == 'fp32' ) prec = SPR_PREC_FP32
+    if ( trim(arg) == 'bf16' ) prec = SPR_PREC_BF16
+    call get_command_argument(3, arg)
+    loop_mode = trim(arg) == 'loop'
+
+    FourPhonon = 0
+    Natm4 = 1
+    open(newunit=u, file=trim(dir)//'/bundle.nml', status='old', action='read')
+    read(u, nml=bundle)
+    close(u)
+    Ndof = 3 * Nbasis
+
+    allocate( mass(Nbasis), atmNum3(Nbasis), atmIndx3(4, Natm3, Nbasis), fC3(3,3,3,Natm3,Natm3,Nbasis) )
+    allocate( q_int(3, Nq), irr_grid(Nq0), pos(Nq), mult(Nq0), Rxyz(3, 3, Nsym) )
+    allocate( Evec(Ndof, Ndof, Nq), omega(Ndof, Nq), OnebyOmega(Ndof, Nq), nBE(Ndof, Nq), grp_vel(3, Ndof, Nq) )
+    call rd_r(trim(dir)//'/mass.bin', mass)
+    call rd_i(trim(dir)//'/atm_num3.bin', atmNum3)
+    call rd_i3(trim(dir)//'/atm_indx3.bin', atmIndx3)
+    call rd_r6(trim(dir)//'/fc3.bin', fC3)
+    call rd_i2(trim(dir)//'/q_int.bin', q_int)
+    call rd_i(trim(dir)//'/irr_grid.bin', irr_grid)
+    call rd_i(trim(dir)//'/pos.bin', pos)
+    call rd_i(trim(dir)//'/multiplicity.bin', mult)
+    call rd_r3(trim(dir)//'/rot.bin', Rxyz)
+    call rd_c3(trim(dir)//'/evec.bin', Evec)
+    call rd_r2(trim(dir)//'/omega.bin', omega)
+    call rd_r2(trim(dir)//'/inv_omega.bin', OnebyOmega)
+    call rd_r2(trim(dir)//'/nbe.bin', nBE)
+    call rd_r3(trim(dir)//'/grp_vel.bin', grp_vel)
+
+    if ( FourPhonon /= 0 ) then
+        allocate( atmNum4(Nbasis), atmIndx4(4, Natm4, Nbasis), fC4(3,3,3,3,Natm4,Natm4,Natm4,Nbasis) )
+        call rd_i(trim(dir)//'/atm_num4.bin', atmNum4)
+        call rd_i3(trim(dir)//'/atm_indx4.bin', atmIndx4)
+        open(newunit=u, file=trim(dir)//'/fc4.bin', access='stream', form='unformatted', status='old')
+        read(u) fC4
+        close(u)
+        call gpu_ctx_create(ctx, Nbasis, mass, vol, atmNum3, atmIndx3, fC3, mesh, q_int, irr_grid, pos, &
+                          & mult, Rxyz, Evec, omega, OnebyOmega, nBE, grp_vel, 0, atmNum4, atmIndx4, fC4)
+    else
+        call gpu_ctx_create(ctx, Nbasis, mass, vol, atmNum3, atmIndx3, fC3, mesh, q_int, irr_grid, pos, &
+                          & mult, Rxyz, Evec, omega, OnebyOmega, nBE, grp_vel, 0)
+    end if
+
+    ! ---------------------------- three-phonon linewidths (main.f90:552-715) ----------------------------
+    allocate( LW_Allq0(Ndof, Nq0), lw(Ndof) )
+    if ( loop_mode ) then
+        do i0 = 1, Nq0
+            call CalculateLinewidth_gpu(ctx, i0, Sigma, prec, lw)
+            LW_Allq0(:, i0) = lw    !** third-order linewidth, no 0.5 factor (main.f90:585) **!
+        end do
+    else
+        call LinewidthAllq0_gpu(ctx, Sigma, prec, LW_Allq0)
+    end if
+    if ( FourPhonon /= 0 ) then
+        allocate( lw4(Ndof, Nq0) )
+        call FourPhononLw_gpu(ctx, Sigma, prec, lw4)
+        LW_Allq0 = LW_Allq0 + 0.5_dp * lw4      ! main.f90:660
+    end if
+
+    write(Temp_char, '(F6.1)') T
+    outfile_ascii = 'linewidth_IBZ_'//trim(prefix)//trim(adjustl(Temp_char))//'K.txt'
+    write(Ndof_char, '(I4)') (2 * Ndof - 1)
+    outfile_frmt = "("//trim(adjustl(Ndof_char))//"(G20.12, '  '), G20.12)"
+    HeaderWrite = .true.
+    do i0 = 1, Nq0
+        call WriteAsciiOutFile(trim(outfile_ascii), trim(outfile_frmt), omega(:, irr_grid(i0)), &
+                             & LW_Allq0(:, i0), Ndof, HeaderWrite)
+    end do
+
+    ! ---------------------------------- kappa (kappa_noIter_symm) ---------------------------------------
+    allocate( Mode_k(9, Ndof) )
+    call kappa_noIter_gpu(ctx, LW_Allq0, T, .true., kappa, Mode_k)
+    write(*, '(A)') 'Thermal conductivity tensor (W/mK):'
+    write(*, '(3(ES22.14, 2X))') kappa(1:3)
+    write(*, '(3(ES22.14, 2X))') kappa(4:6)
+    write(*, '(3(ES22.14, 2X))') kappa(7:9)
+    call gpu_ctx_destroy(ctx)
+
+contains
+
+    subroutine WriteAsciiOutFile(filename, write_frmt, freq, LW_q0, Ndof, HeaderWrite)
+        character(len=*), intent(in)    :: filename, write_frmt
+        real(dp), intent(in)            :: freq(:), LW_q0(:)
+        integer, intent(in)             :: Ndof
+        logical, intent(inout)          :: HeaderWrite
+        integer                         :: ui
+        if ( HeaderWrite ) then
+            open(newunit=ui, file=filename, status='REPLACE', action='WRITE')
+            write(ui, 20) Ndof, Ndof
+            HeaderWrite = .false.
+        else
+            open(newunit=ui, file=filename, status='OLD', action='WRITE', position='APPEND')
+        end if
+        write(ui, write_frmt) freq, LW_q0
+        close(ui)
+        20 FORMAT( '#--------- phonon_frequency(rad.THz) ( ', I4, ' columns) ---------      ', &
+                 & '---------- linewidth(rad.THz) ( ', I4, ' columns) ----------')
+    end subroutine WriteAsciiOutFile
+
+    subroutine rd_r(f, a)
+        character(len=*), intent(in) :: f
+        real(dp), intent(out) :: a(:)
+        integer :: v
+        open(newunit=v, file=f, access='stream', form='unformatted', status='old'); read(v) a; close(v)
+    end subroutine rd_r
+    subroutine rd_r2(f, a)
+        character(len=*), intent(in) :: f
+        real(dp), intent(out) :: a(:,:)
+        integer :: v
+        open(newunit=v, file=f, access='stream', form='unformatted', status='old'); read(v) a; close(v)
+    end subroutine rd_r2
+    subroutine rd_r3(f, a)
+        character(len=*), intent(in) :: f
+        real(dp), intent(out) :: a(:,:,:)
+        integer :: v
+        open(newunit=v, file=f, access='stream', form='unformatted', status='old'); read(v) a; close(v)
+    end subroutine rd_r3
+    subroutine rd_r6(f, a)
+        character(len=*), intent(in) :: f
+        real(dp), intent(out) :: a(:,:,:,:,:,:)
+        integer :: v
+        open(newunit=v, file=f, access='stream', form='unformatted', status='old'); read(v) a; close(v)
+    end subroutine rd_r6
+    subroutine rd_c3(f, a)
+        character(len=*), intent(in) :: f
+        complex(dp), intent(out) :: a(:,:,:)
+        integer :: v
+        open(newunit=v, file=f, access='stream', form='unformatted', status='old'); read(v) a; close(v)
+    end subroutine rd_c3
+    subroutine rd_i(f, a)
+        character(len=*), intent(in) :: f
+        integer, intent(out) :: a(:)
+        integer :: v
+        open(newunit=v, file=f, access='stream', form='unformatted', status='old'); read(v) a; close(v)
+    end subroutine rd_i
+    subroutine rd_i2(f, a)
+        character(len=*), intent(in) :: f
+        integer, intent(out) :: a(:,:)
+        integer :: v
+        open(newunit=v, file=f, access='stream', form='unformatted', status='old'); read(v) a; close(v)
+    end subroutine rd_i2
+    subroutine rd_i3(f, a)
+        character(len=*), intent(in) :: f
+        integer, intent(out) :: a(:,:,:)
+        integer :: v
+        open(newunit=v, file=f, access='stream', form='unformatted', status='old'); read(v) a; close(v)
+    end subroutine rd_i3
+
+end program kappa_gpu

@@ -1,0 +1,266 @@
+! spring_gpu_m -- Fortran side of the libspring_gpu.so boundary (iso_c_binding).
+!
+! The derived types mirror the C structs of include/spring_gpu.h byte for byte; the
+! wrapper routines keep the reference's routine names and argument meaning for the hot
+! path and its failure behaviour (print the message, ERROR STOP):
+!   ScatterProbAllq1_gpu   <- ThirdOrder::ScatterProbAllq1 (src/Kappa/Src/ThirdOrder_m.f90:41-166),
+!                             all q1 of one q0; fills Sctrq1(Ndof,Ndof,Ndof,Nq)
+!   CalculateLinewidth_gpu <- ThirdOrder::CalculateLinewidth (ThirdOrder_m.f90:442-543) with the
+!                             scattering matrix computed on the fly
+!   LinewidthAllq0_gpu     <- the q0 loop of src/Kappa/Src/main.f90:552-715 (three-phonon part)
+!   FourPhononLw_gpu       <- ScatterProbq1q2_per + TetrahedronIntegration4th (Gaussian form),
+!                             FourthOrder_m.f90:42-283, Tetrahedron4th_sep.f90:37-171
+!   kappa_noIter_gpu       <- kappa_noIter / kappa_noIter_symm (ThermalConductivity.f90:36-293)
+! Index arrays arrive 1-based as in the reference (atmIndx(4,:,:), pos, the irreducible
+! grid index) and are shifted to the C ABI's 0-based convention here.
+module spring_gpu_m
+
+    use iso_c_binding
+    implicit none
+    private
+
+    integer, parameter, public          :: dp = c_double
+    integer(c_int), parameter, public   :: SPR_PREC_FP64 = 0, SPR_PREC_FP32 = 1, SPR_PREC_BF16 = 2
+
+    type, bind(C) :: spr_crystal
+        integer(c_int)  :: nb
+        type(c_ptr)     :: mass
+        real(c_double)  :: vol
+    end type spr_crystal
+
+    type, bind(C) :: spr_ifc
+        integer(c_int)  :: order, natm
+        type(c_ptr)     :: atm_num, atm_indx, fc
+    end type spr_ifc
+
+    type, bind(C) :: spr_mesh
+        integer(c_int)  :: mesh(3), nq
+        type(c_ptr)     :: q_int
+        integer(c_int)  :: nq0
+        type(c_ptr)     :: irr_grid, pos, multiplicity
+        integer(c_int)  :: nsym
+        type(c_ptr)     :: rot
+    end type spr_mesh
+
+    type, bind(C) :: spr_phonons
+        integer(c_int)  :: nq, ndof
+        type(c_ptr)     :: evec, omega, inv_omega, nbe, grp_vel
+    end type spr_phonons
+
+    type, public :: gpu_ctx
+        type(c_ptr)     :: h = c_null_ptr
+        integer         :: Ndof = 0, Nq = 0, Nq0 = 0
+    end type gpu_ctx
+
+    interface
+        integer(c_int) function spr_ctx_create(cr, fc3, fc4, mesh, ph, device, out) &
+                & bind(C, name="spr_ctx_create")
+            import :: c_int, c_ptr, spr_crystal, spr_ifc, spr_mesh, spr_phonons
+            type(spr_crystal), intent(in)   :: cr
+            type(spr_ifc), intent(in)       :: fc3
+            type(c_ptr), value              :: fc4
+            type(spr_mesh), intent(in)      :: mesh
+            type(spr_phonons), intent(in)   :: ph
+            integer(c_int), value           :: device
+            type(c_ptr), intent(out)        :: out
+        end function spr_ctx_create
+
+        integer(c_int) function spr_ctx_destroy(h) bind(C, name="spr_ctx_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+        end function spr_ctx_destroy
+
+        type(c_ptr) function spr_last_error() bind(C, name="spr_last_error")
+            import :: c_ptr
+        end function spr_last_error
+
+        integer(c_int) function spr_v3_w(h, i0, W) bind(C, name="spr_v3_w")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            integer(c_int), value   :: i0
+            real(c_double)          :: W(*)
+        end function spr_v3_w
+
+        integer(c_int) function spr_lw3(h, i0, sigma, prec, lw) bind(C, name="spr_lw3")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            integer(c_int), value   :: i0, prec
+            real(c_double), value   :: sigma
+            real(c_double)          :: lw(*)
+        end function spr_lw3
+
+        integer(c_int) function spr_lw3_all(h, sigma, prec, lw) bind(C, name="spr_lw3_all")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            integer(c_int), value   :: prec
+            real(c_double), value   :: sigma
+            real(c_double)          :: lw(*)
+        end function spr_lw3_all
+
+        integer(c_int) function spr_lw4_all(h, sigma, prec, lw) bind(C, name="spr_lw4_all")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            integer(c_int), value   :: prec
+            real(c_double), value   :: sigma
+            real(c_double)          :: lw(*)
+        end function spr_lw4_all
+
+        integer(c_int) function spr_kappa_rta(h, lw, T, symm, kappa, mode_k) bind(C, name="spr_kappa_rta")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            real(c_double)          :: lw(*)
+            real(c_double), value   :: T
+            integer(c_int), value   :: symm
+            real(c_double)          :: kappa(9), mode_k(*)
+        end function spr_kappa_rta
+
+        integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+            import :: c_size_t, c_ptr
+            type(c_ptr), value :: s
+        end function c_strlen
+    end interface
+
+    public :: gpu_ctx_create, gpu_ctx_destroy, ScatterProbAllq1_gpu, CalculateLinewidth_gpu, &
+            & LinewidthAllq0_gpu, FourPhononLw_gpu, kappa_noIter_gpu, spr_error_text
+
+contains
+
+    function spr_error_text() result(msg)
+        character(len=:), allocatable   :: msg
+        type(c_ptr)                     :: p
+        character(kind=c_char), pointer :: chars(:)
+        integer                         :: n, i
+        p = spr_last_error()
+        n = int(c_strlen(p))
+        call c_f_pointer(p, chars, [n])
+        allocate(character(len=n) :: msg)
+        do i = 1, n
+            msg(i:i) = chars(i)
+        end do
+    end function spr_error_text
+
+    subroutine check(rc, what)
+        integer(c_int), intent(in)      :: rc
+        character(len=*), intent(in)    :: what
+        if ( rc /= 0 ) then
+            write(*, '(A, A, I0, A, A)') what, ' failed (', rc, '): ', spr_error_text()
+            ERROR STOP
+        end if
+    end subroutine check
+
+    ! Upload everything once.  atmIndx(4,:,:) and pos/irr_grid are 1-based (reference layout).
+    subroutine gpu_ctx_create(ctx, Nbasis, mass, vol, atmNum3, atmIndx3, fC3, &
+                            & mesh, q_pnt_int, irr_grid, pos, multiplicity, Rxyz, &
+                            & Evec, omega, OnebyOmega, nBE, grp_vel, device, &
+                            & atmNum4, atmIndx4, fC4)
+        type(gpu_ctx), intent(out)                          :: ctx
+        integer, intent(in)                                 :: Nbasis, device
+        real(dp), intent(in), target                        :: mass(:), vol
+        integer, intent(in)                                 :: atmNum3(:), atmIndx3(:,:,:)
+        real(dp), intent(in), target                        :: fC3(:,:,:,:,:,:)
+        integer, intent(in)                                 :: mesh(3), q_pnt_int(:,:), irr_grid(:), &
+                                                             & pos(:), multiplicity(:)
+        real(dp), intent(in), target                        :: Rxyz(:,:,:)
+        complex(dp), intent(in), target                     :: Evec(:,:,:)
+        real(dp), intent(in), target                        :: omega(:,:), OnebyOmega(:,:), nBE(:,:), &
+                                                             & grp_vel(:,:,:)
+        integer, intent(in), optional                       :: atmNum4(:), atmIndx4(:,:,:)
+        real(dp), intent(in), target, optional              :: fC4(:,:,:,:,:,:,:,:)
+
+        integer(c_int), allocatable, target, save   :: num3(:), indx3(:,:,:), num4(:), indx4(:,:,:)
+        integer(c_int), allocatable, target, save   :: qint(:,:), irr0(:), pos0(:), mult(:)
+        real(dp), allocatable, target, save         :: mass_c(:)
+        type(spr_crystal)                           :: cr
+        type(spr_ifc), target                       :: f3, f4
+        type(spr_mesh)                              :: ms
+        type(spr_phonons)                           :: ph
+        type(c_ptr)                                 :: p4
+        integer(c_int)                              :: rc
+
+        num3 = atmNum3
+        indx3 = atmIndx3
+        indx3(4, :, :) = indx3(4, :, :) - 1
+        mass_c = mass
+        cr = spr_crystal(int(Nbasis, c_int), c_loc(mass_c), vol)
+        f3 = spr_ifc(3, int(size(atmIndx3, 2), c_int), c_loc(num3), c_loc(indx3), c_loc(fC3))
+        p4 = c_null_ptr
+        if ( present(fC4) ) then
+            num4 = atmNum4
+            indx4 = atmIndx4
+            indx4(4, :, :) = indx4(4, :, :) - 1
+            f4 = spr_ifc(4, int(size(atmIndx4, 2), c_int), c_loc(num4), c_loc(indx4), c_loc(fC4))
+            p4 = c_loc(f4)
+        end if
+        qint = q_pnt_int
+        irr0 = irr_grid - 1
+        pos0 = pos - 1
+        mult = multiplicity
+        ms = spr_mesh(int(mesh, c_int), int(size(q_pnt_int, 2), c_int), c_loc(qint), &
+                    & int(size(irr_grid), c_int), c_loc(irr0), c_loc(pos0), c_loc(mult), &
+                    & int(size(Rxyz, 3), c_int), c_loc(Rxyz))
+        ph = spr_phonons(int(size(omega, 2), c_int), int(size(omega, 1), c_int), c_loc(Evec), &
+                       & c_loc(omega), c_loc(OnebyOmega), c_loc(nBE), c_loc(grp_vel))
+        rc = spr_ctx_create(cr, f3, p4, ms, ph, int(device, c_int), ctx%h)
+        call check(rc, 'spr_ctx_create')
+        ctx%Ndof = size(omega, 1)
+        ctx%Nq = size(omega, 2)
+        ctx%Nq0 = size(irr_grid)
+    end subroutine gpu_ctx_create
+
+    subroutine gpu_ctx_destroy(ctx)
+        type(gpu_ctx), intent(inout) :: ctx
+        integer(c_int) :: rc
+        rc = spr_ctx_destroy(ctx%h)
+        ctx%h = c_null_ptr
+    end subroutine gpu_ctx_destroy
+
+    ! Sctrq1(s2,s1,s0,q1) for irreducible q0 number i0 (1-based), every q1.
+    subroutine ScatterProbAllq1_gpu(ctx, i0, Sctrq1)
+        type(gpu_ctx), intent(in)   :: ctx
+        integer, intent(in)         :: i0
+        real(dp), intent(out)       :: Sctrq1(:,:,:,:)
+        call check(spr_v3_w(ctx%h, int(i0 - 1, c_int), Sctrq1), 'ScatterProbAllq1_gpu')
+    end subroutine ScatterProbAllq1_gpu
+
+    ! Gaussian-delta three-phonon linewidth lw(Ndof) of irreducible q0 number i0 (1-based).
+    subroutine CalculateLinewidth_gpu(ctx, i0, Sigma, prec, lw)
+        type(gpu_ctx), intent(in)   :: ctx
+        integer, intent(in)         :: i0, prec
+        real(dp), intent(in)        :: Sigma
+        real(dp), intent(out)       :: lw(:)
+        call check(spr_lw3(ctx%h, int(i0 - 1, c_int), Sigma, int(prec, c_int), lw), 'CalculateLinewidth_gpu')
+    end subroutine CalculateLinewidth_gpu
+
+    ! LW_Allq0(Ndof, Nq0) of every irreducible q0 in one launch.
+    subroutine LinewidthAllq0_gpu(ctx, Sigma, prec, LW_Allq0)
+        type(gpu_ctx), intent(in)   :: ctx
+        integer, intent(in)         :: prec
+        real(dp), intent(in)        :: Sigma
+        real(dp), intent(out)       :: LW_Allq0(:,:)
+        call check(spr_lw3_all(ctx%h, Sigma, int(prec, c_int), LW_Allq0), 'LinewidthAllq0_gpu')
+    end subroutine LinewidthAllq0_gpu
+
+    ! Raw four-phonon sum lw4(Ndof, Nq0); the caller adds 0.5*lw4 (main.f90:660).
+    subroutine FourPhononLw_gpu(ctx, Sigma, prec, lw4)
+        type(gpu_ctx), intent(in)   :: ctx
+        integer, intent(in)         :: prec
+        real(dp), intent(in)        :: Sigma
+        real(dp), intent(out)       :: lw4(:,:)
+        call check(spr_lw4_all(ctx%h, Sigma, int(prec, c_int), lw4), 'FourPhononLw_gpu')
+    end subroutine FourPhononLw_gpu
+
+    ! kappa(9) in W/mK (column-major 3x3) and Mode_k(9, Ndof).
+    subroutine kappa_noIter_gpu(ctx, LW_Allq0, T, Symm, kappa, Mode_k)
+        type(gpu_ctx), intent(in)   :: ctx
+        real(dp), intent(in)        :: LW_Allq0(:,:), T
+        logical, intent(in)         :: Symm
+        real(dp), intent(out)       :: kappa(9), Mode_k(:,:)
+        integer(c_int)              :: s
+        real(dp), allocatable       :: lw(:,:)
+        s = 0
+        if ( Symm ) s = 1
+        lw = LW_Allq0
+        call check(spr_kappa_rta(ctx%h, lw, T, s, kappa, Mode_k), 'kappa_noIter_gpu')
+    end subroutine kappa_noIter_gpu
+
+end module spring_gpu_m

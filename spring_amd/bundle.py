@@ -1,0 +1,63 @@
+"""Input bundle for the Fortran host driver (fortran/kappa_gpu).
+
+A directory holding `bundle.nml` (sizes, T, sigma) and one raw little-endian file per
+array, in the reference's column-major layout with its 1-based index conventions
+(atmIndx(4,:,:) basis index, pos, irreducible grid index), which is what the Fortran side
+of the boundary receives in the reference program.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+
+def write_bundle(path, model, qm, ph, T: float, sigma: float, prefix: str = "synth",
+                 with_fc4: bool = False):
+    os.makedirs(path, exist_ok=True)
+    cr = model.crystal
+
+    def put(name, arr, dt):
+        np.ascontiguousarray(arr, dtype=dt).tofile(os.path.join(path, name))
+
+    f3 = model.fc3
+    idx3 = f3.atm_indx.copy()
+    idx3[..., 3] += 1
+    put("mass.bin", cr.mass, "<f8")
+    put("atm_num3.bin", f3.atm_num, "<i4")
+    put("atm_indx3.bin", idx3, "<i4")
+    put("fc3.bin", f3.fc, "<f8")
+    put("q_int.bin", qm.q_int, "<i4")
+    put("irr_grid.bin", qm.irr_idx + 1, "<i4")
+    put("pos.bin", qm.pos + 1, "<i4")
+    put("multiplicity.bin", qm.multiplicity, "<i4")
+    rot = qm.rot_cart if not qm.time_rev else np.concatenate([qm.rot_cart, -qm.rot_cart])
+    put("rot.bin", np.transpose(rot, (0, 2, 1)), "<f8")          # Rxyz(3,3,nsym)
+    put("evec.bin", ph.evec, "<c16")
+    put("omega.bin", ph.omega, "<f8")
+    put("inv_omega.bin", ph.inv_omega, "<f8")
+    put("nbe.bin", ph.nbe, "<f8")
+    put("grp_vel.bin", ph.grp_vel, "<f8")
+    natm4 = 1
+    if with_fc4:
+        f4 = model.fc4
+        idx4 = f4.atm_indx.copy()
+        idx4[..., 3] += 1
+        put("atm_num4.bin", f4.atm_num, "<i4")
+        put("atm_indx4.bin", idx4, "<i4")
+        put("fc4.bin", f4.fc, "<f8")
+        natm4 = f4.natm_max
+    with open(os.path.join(path, "bundle.nml"), "w") as fh:
+        fh.write("&bundle\n")
+        fh.write(f"  prefix = '{prefix}'\n  Nbasis = {cr.nb}\n  Natm3 = {f3.natm_max}\n")
+        fh.write(f"  Natm4 = {natm4}\n  mesh = {int(qm.mesh[0])}, {int(qm.mesh[1])}, {int(qm.mesh[2])}\n")
+        fh.write(f"  Nq = {qm.nq}\n  Nq0 = {qm.nq0}\n  Nsym = {rot.shape[0]}\n")
+        fh.write(f"  T = {T!r}\n  Sigma = {sigma!r}\n  vol = {cr.vol!r}\n  FourPhonon = {int(with_fc4)}\n/\n")
+    return path
+
+
+def read_linewidth_file(fname, ndof):
+    """Rows of linewidth_IBZ_*.txt: (omega(ndof), lw(ndof)) per irreducible q0."""
+    data = np.loadtxt(fname, comments="#")
+    data = np.atleast_2d(data)
+    return data[:, :ndof], data[:, ndof:2 * ndof]

@@ -1,0 +1,336 @@
+// Three-phonon scattering for small cells (Nb <= 2, Ndof <= 6): the benchmark kernel.
+//
+// Work unit: a batch of up to 64 (q1, q2) pairs of one irreducible q0, from the
+// pair-permutation reduced list (q1 <= q2, weight 2 if q1 < q2; ScatterProbAllq1's
+// q1q2_permuted, ThirdOrder_m.f90:98-164).  Workgroup = Ndof waves; wave `a` owns row
+// a = (mu, alpha) of the phase-summed tensor, lane `l` owns pair l of the batch.
+//
+//   1. phase sum (ScatterMatEl, ThirdOrder_m.f90:316-384, grouped by basis atoms):
+//        T_a(b, c) = sum_{(N1,N2) in block(mu, nu, eta)} Phi(alpha, beta, gamma) e^{2 pi i m / L}
+//      the FC values are wave-uniform (scalar loads), the phase index m = m1[N1] + m2[N2]
+//      is per lane (integer addresses, table lookup in LDS), T_a lives in registers.
+//   2. T_a <- T_a e2 then e1^T T_a (per lane, eigenvectors staged in LDS lane-major).
+//   3. the e0 contraction crosses waves: per s1, every wave writes U(a, s1, :) to LDS and
+//      wave s0 forms V(s0, s1, s2) = sum_a e0(a, s0) U(a, s1, s2).
+//   4. W = |V|^2 /(w0 w1 w2) Sctr3THz2/Nq and the Gaussian-delta occupation weights
+//      (CalculateLinewidth, ThirdOrder_m.f90:514-537) are accumulated per lane, then
+//      reduced over the wave: partial[item][s0].  A second kernel sums the items of each
+//      q0 in a fixed order (bitwise reproducible).
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+#include "cplx.h"
+#include "spr_internal.h"
+
+namespace spr {
+
+struct SmallArgs {
+    const int *item_q0;      // (nitems) irreducible index of each batch
+    const int *item_off;     // (nitems) offset into the pair list
+    const int *item_cnt;     // (nitems) pairs in the batch (<= 64)
+    const int2 *pairs;       // (npairs) (q1, q2)
+    const int *q0_grid;      // (nq0) grid index of each irreducible q0
+    int L, m0, m1;
+    const int *blk_start;
+    const int *tup_site;
+    const void *fc;
+    const int4 *site;
+    int nsite;
+    const void *phase;
+    const void *evec;
+    const double *omega, *inv_omega, *nbe;
+    double sigma, factor;
+    double *partial;         // (nitems, ndof)
+};
+
+template <typename R, int NB>
+__global__ __launch_bounds__(NB * 3 * 64) void k_lw3_small(SmallArgs A) {
+    using Z = cplx_t<R>;
+    constexpr int ND = 3 * NB;
+    constexpr int ND2 = ND * ND;
+    extern __shared__ __align__(16) unsigned char smem[];
+    // layout: e1[ND2][64], e2[ND2][64] (aliased by the exchange buffer x[ND][ND][64]),
+    //         m1[nsite][64], m2[nsite][64] (int), phase[L]
+    Z *s_e1 = reinterpret_cast<Z *>(smem);
+    Z *s_e2 = s_e1 + ND2 * 64;
+    Z *s_x = s_e1;
+    int *s_m1 = reinterpret_cast<int *>(s_e2 + ND2 * 64);
+    int *s_m2 = s_m1 + A.nsite * 64;
+    Z *s_ph = reinterpret_cast<Z *>(s_m2 + A.nsite * 64);
+
+    const int lane = threadIdx.x & 63;
+    const int a = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave index
+    const int mu = a / 3, al = a % 3;
+    const int item = blockIdx.x;
+    const int qi = A.item_q0[item];
+    const int i0 = A.q0_grid[qi];
+    const int cnt = A.item_cnt[item];
+    const int2 pr = A.pairs[A.item_off[item] + (lane < cnt ? lane : 0)];
+    const int q1 = pr.x, q2 = pr.y;
+    const double wpair = lane < cnt ? (q1 == q2 ? 1.0 : 2.0) : 0.0;
+    const Z *evec = reinterpret_cast<const Z *>(A.evec);
+    const Z *phase = reinterpret_cast<const Z *>(A.phase);
+    const R *fc = reinterpret_cast<const R *>(A.fc);
+
+    // ---- stage per-lane data: eigenvectors (lane-major), phase indices, table ----
+    for (int i = a; i < ND2; i += ND) {
+        s_e1[i * 64 + lane] = evec[(size_t)q1 * ND2 + i];
+        s_e2[i * 64 + lane] = evec[(size_t)q2 * ND2 + i];
+    }
+    {
+        const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
+        const int k20 = q2 % A.m0, k21 = (q2 / A.m0) % A.m1, k22 = q2 / (A.m0 * A.m1);
+        for (int s = a; s < A.nsite; s += ND) {
+            const int4 c = A.site[s];
+            s_m1[s * 64 + lane] = (k10 * c.x + k11 * c.y + k12 * c.z) % A.L;
+            s_m2[s * 64 + lane] = (k20 * c.x + k21 * c.y + k22 * c.z) % A.L;
+        }
+    }
+    for (int i = threadIdx.x; i < A.L; i += ND * 64) s_ph[i] = phase[i];
+    __syncthreads();
+
+    // ---- 1. phase sum of row a ----
+    Z T[ND][ND];
+#pragma unroll
+    for (int b = 0; b < ND; ++b)
+#pragma unroll
+        for (int c = 0; c < ND; ++c) T[b][c] = cmk<R>(0, 0);
+#pragma unroll
+    for (int nu = 0; nu < NB; ++nu)
+#pragma unroll
+        for (int eta = 0; eta < NB; ++eta) {
+            const int blk = (mu * NB + nu) * NB + eta;
+            const int u0 = A.blk_start[blk], u1 = A.blk_start[blk + 1];
+            for (int u = u0; u < u1; ++u) {
+                const int sa = A.tup_site[2 * u], sb = A.tup_site[2 * u + 1];
+                int m = s_m1[sa * 64 + lane] + s_m2[sb * 64 + lane];
+                m -= m >= A.L ? A.L : 0;
+                const Z ph = s_ph[m];
+                const R *f = fc + (size_t)u * 27 + al * 9;
+#pragma unroll
+                for (int be = 0; be < 3; ++be)
+#pragma unroll
+                    for (int ga = 0; ga < 3; ++ga)
+                        T[3 * nu + be][3 * eta + ga] = crfma(f[be * 3 + ga], ph, T[3 * nu + be][3 * eta + ga]);
+            }
+        }
+
+    // ---- 2. T <- T e2 (c -> s2), then e1^T T (b -> s1) ----
+#pragma unroll
+    for (int b = 0; b < ND; ++b) {
+        Z t[ND];
+#pragma unroll
+        for (int s2 = 0; s2 < ND; ++s2) {
+            Z z = cmk<R>(0, 0);
+#pragma unroll
+            for (int c = 0; c < ND; ++c) z = cfma(T[b][c], s_e2[(s2 * ND + c) * 64 + lane], z);
+            t[s2] = z;
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < ND; ++s2) T[b][s2] = t[s2];
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < ND; ++s2) {
+        Z t[ND];
+#pragma unroll
+        for (int s1 = 0; s1 < ND; ++s1) {
+            Z z = cmk<R>(0, 0);
+#pragma unroll
+            for (int b = 0; b < ND; ++b) z = cfma(s_e1[(s1 * ND + b) * 64 + lane], T[b][s2], z);
+            t[s1] = z;
+        }
+#pragma unroll
+        for (int s1 = 0; s1 < ND; ++s1) T[s1][s2] = t[s1];
+    }
+
+    // ---- 3 + 4. e0 contraction across waves, W and the delta sum (wave = s0) ----
+    const int s0 = a;
+    Z e0c[ND];
+#pragma unroll
+    for (int r = 0; r < ND; ++r) e0c[r] = evec[(size_t)i0 * ND2 + s0 * ND + r];   // e0(r, s0)
+    const double w0 = A.omega[(size_t)i0 * ND + s0];
+    const double iw0f = A.inv_omega[(size_t)i0 * ND + s0] * A.factor;
+    double w2[ND], n2[ND], iw2[ND];
+#pragma unroll
+    for (int s2 = 0; s2 < ND; ++s2) {
+        w2[s2] = A.omega[(size_t)q2 * ND + s2];
+        n2[s2] = A.nbe[(size_t)q2 * ND + s2];
+        iw2[s2] = A.inv_omega[(size_t)q2 * ND + s2];
+    }
+    const double c0 = 1.0 / (A.sigma * 2.5066282746310002);
+    const double hinv = 0.5 / (A.sigma * A.sigma);
+    KSum<R> kacc;
+    double dacc = 0.0;
+    __syncthreads();   // every wave is done reading e1/e2 before the exchange overwrites them
+#pragma unroll
+    for (int s1 = 0; s1 < ND; ++s1) {
+#pragma unroll
+        for (int s2 = 0; s2 < ND; ++s2) s_x[(a * ND + s2) * 64 + lane] = T[s1][s2];
+        __syncthreads();
+        const double w1 = A.omega[(size_t)q1 * ND + s1];
+        const double n1 = A.nbe[(size_t)q1 * ND + s1];
+        const double iw01 = iw0f * A.inv_omega[(size_t)q1 * ND + s1];
+#pragma unroll
+        for (int s2 = 0; s2 < ND; ++s2) {
+            Z v = cmk<R>(0, 0);
+#pragma unroll
+            for (int r = 0; r < ND; ++r) v = cfma(e0c[r], s_x[(r * ND + s2) * 64 + lane], v);
+            const double W = (double)cabs2(v) * iw01 * iw2[s2];
+            const double d1 = w0 - w1 - w2[s2], d2 = w0 + w1 - w2[s2], d3 = w0 - w1 + w2[s2];
+            const double g = (n1 + n2[s2] + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
+                             (n1 - n2[s2]) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
+            if (sizeof(R) == 8)
+                dacc += wpair * W * g;
+            else
+                kacc.add((R)(wpair * W * g));
+        }
+        __syncthreads();
+    }
+    double tot = sizeof(R) == 8 ? dacc : kacc.value();
+    // wave reduction (fixed order)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_down(tot, off, 64);
+    if (lane == 0) A.partial[(size_t)item * ND + s0] = tot;
+}
+
+// lw[q][s] = sum of partial[item][s] over the items [first[q], first[q+1]) in order
+__global__ void k_reduce_items(const double *partial, const int *first, int base, int nq0, int nd,
+                               double *lw) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq0 * nd) return;
+    const int q = i / nd, s = i % nd;
+    double acc = 0.0;
+    for (int it = first[q]; it < first[q + 1]; ++it) acc += partial[(size_t)(it - base) * nd + s];
+    lw[i] = acc;
+}
+
+template <typename R, int NB>
+static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems) {
+    constexpr int ND = 3 * NB;
+    SmallArgs A = A0;
+    A.fc = sizeof(R) == 8 ? (const void *)c.fc3.fc64 : (const void *)c.fc3.fc32;
+    A.phase = sizeof(R) == 8 ? (const void *)c.phase64 : (const void *)c.phase32;
+    A.evec = sizeof(R) == 8 ? (const void *)c.evec : (const void *)c.evec32;
+    const size_t dyn = (size_t)2 * ND * ND * 64 * sizeof(cplx_t<R>) + (size_t)2 * A.nsite * 64 * sizeof(int) +
+                       (size_t)c.L * sizeof(cplx_t<R>);
+    if (dyn > 160 * 1024) {
+        set_error("small-cell kernel: LDS need exceeds 160 KB (too many neighbour sites)");
+        return SPR_E_UNSUPPORTED;
+    }
+    SPR_HIP(hipFuncSetAttribute((const void *)k_lw3_small<R, NB>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    hipLaunchKernelGGL((k_lw3_small<R, NB>), dim3(nitems), dim3(ND * 64), dyn, c.stream, A);
+    SPR_HIP(hipGetLastError());
+    return SPR_OK;
+}
+
+// Builds (once per context) the pair lists of every irreducible q0 and the 64-pair batches.
+int ensure_pairs(Ctx &c) {
+    if (c.pairs_built) return SPR_OK;
+    const int nq = c.nq, m0 = c.mesh[0], m1 = c.mesh[1], m2 = c.mesh[2];
+    std::vector<int2> pairs;
+    std::vector<int> iq0, ioff, icnt, first(c.nq0 + 1, 0);
+    pairs.reserve((size_t)c.nq0 * (nq / 2 + 8));
+    for (int qi = 0; qi < c.nq0; ++qi) {
+        const int i0 = c.h_irr_grid[qi];
+        const int k0 = i0 % m0, k1 = (i0 / m0) % m1, k2 = i0 / (m0 * m1);
+        const size_t start = pairs.size();
+        for (int q1 = 0; q1 < nq; ++q1) {
+            const int a0 = ((-(k0 + q1 % m0)) % m0 + m0) % m0;
+            const int a1 = ((-(k1 + (q1 / m0) % m1)) % m1 + m1) % m1;
+            const int a2 = ((-(k2 + q1 / (m0 * m1))) % m2 + m2) % m2;
+            const int q2 = a0 + m0 * (a1 + m1 * a2);
+            if (q1 <= q2) pairs.push_back(make_int2(q1, q2));
+        }
+        first[qi] = (int)iq0.size();
+        for (size_t o = start; o < pairs.size(); o += 64) {
+            iq0.push_back(qi);
+            ioff.push_back((int)o);
+            icnt.push_back((int)std::min<size_t>(64, pairs.size() - o));
+        }
+    }
+    first[c.nq0] = (int)iq0.size();
+    c.n_pairs = (long)pairs.size();
+    c.n_items = (int)iq0.size();
+    auto up = [&](auto **dst, const auto &v) -> int {
+        using T = typename std::remove_reference<decltype(v)>::type::value_type;
+        if (hipMalloc((void **)dst, v.size() * sizeof(T)) != hipSuccess) {
+            set_error("pair list: hipMalloc failed");
+            return SPR_E_OOM;
+        }
+        if (hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("pair list: upload failed");
+            return SPR_E_HIP;
+        }
+        return SPR_OK;
+    };
+    int rc;
+    if ((rc = up(&c.d_pairs, pairs))) return rc;
+    if ((rc = up(&c.d_item_q0, iq0))) return rc;
+    if ((rc = up(&c.d_item_off, ioff))) return rc;
+    if ((rc = up(&c.d_item_cnt, icnt))) return rc;
+    if ((rc = up(&c.d_item_first, first))) return rc;
+    c.h_item_first = first;
+    c.pairs_built = true;
+    return SPR_OK;
+}
+
+int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, double *d_lw) {
+    hipLaunchKernelGGL(k_reduce_items, dim3((q0_count * c.nd + 255) / 256), dim3(256), 0, c.stream,
+                       c.partial, c.d_item_first + q0_first, it0, q0_count, c.nd, d_lw);
+    SPR_HIP(hipGetLastError());
+    return SPR_OK;
+}
+
+// Linewidths of irreducible q0 [first, first + count) with the small-cell kernel.
+int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw) {
+    int rc;
+    if ((rc = ensure_pairs(c))) return rc;
+    const int it0 = c.h_item_first[q0_first], it1 = c.h_item_first[q0_first + q0_count];
+    const int nitems = it1 - it0;
+    if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)c.n_items * c.nd * sizeof(double))))
+        return rc;
+    SmallArgs A;
+    A.item_q0 = c.d_item_q0 + it0;
+    A.item_off = c.d_item_off + it0;
+    A.item_cnt = c.d_item_cnt + it0;
+    A.pairs = c.d_pairs;
+    A.q0_grid = c.irr_grid;
+    A.L = c.L;
+    A.m0 = c.mesh[0];
+    A.m1 = c.mesh[1];
+    A.blk_start = c.fc3.blk_start;
+    A.tup_site = c.fc3.tup_site;
+    A.site = c.fc3.site;
+    A.nsite = c.fc3.nsite;
+    A.omega = c.omega;
+    A.inv_omega = c.inv_omega;
+    A.nbe = c.nbe;
+    A.sigma = sigma;
+    const double pi = 4.0 * std::atan(1.0);
+    A.factor = 1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) *
+               1000000.0 * pi * 0.0625 / (double)c.nq;   // Sctr3THz2 / Nq, constants.f90:59-60
+    A.partial = c.partial;
+    SPR_HIP(hipEventRecord(c.ev0, c.stream));
+    if (c.nb == 2)
+        rc = prec == SPR_PREC_FP64 ? launch_small_t<double, 2>(c, A, nitems) : launch_small_t<float, 2>(c, A, nitems);
+    else if (c.nb == 1)
+        rc = prec == SPR_PREC_FP64 ? launch_small_t<double, 1>(c, A, nitems) : launch_small_t<float, 1>(c, A, nitems);
+    else {
+        set_error("small-cell kernel supports Nb <= 2");
+        return SPR_E_UNSUPPORTED;
+    }
+    if (rc) return rc;
+    SPR_HIP(hipEventRecord(c.ev1, c.stream));
+    if ((rc = launch_reduce_items(c, q0_first, q0_count, it0, d_lw))) return rc;
+    SPR_HIP(hipEventSynchronize(c.ev1));
+    float ms = 0.f;
+    SPR_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    c.launches += 1;
+    c.kernel_ms += ms;
+    return SPR_OK;
+}
+
+}  // namespace spr

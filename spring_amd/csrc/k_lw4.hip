@@ -1,0 +1,250 @@
+// Four-phonon scattering (Ndof <= 6), FP64 or FP32, Gaussian delta.
+//
+// One workgroup takes one irreducible q0 and a chunk of q1 grid points; for every q1 it
+// runs over all q2 of the mesh with q3 = -(q0+q1+q2) mod G (MomentumConsrv4th,
+// QuasiMomentumConsrv4th_m.f90:32-206, without the permutation dedup, which the reference
+// needs only to store W4) and
+//   1. phase-sums FC4 into T(a,b,c,d) over the nonzero (N1,N2,N3) blocks
+//      (ScatterMatEl4, FourthOrder_m.f90:286-422), integer-address phases from a table;
+//   2. contracts d, c, b, a with e3, e2, e1, e0 (no conjugation);
+//   3. W4 = |V4|^2 prod(1/w) Sctr4THz2/Nq^2 (ScatterProbability4 :425-473) and adds
+//        W4 [ 1/6 n1 n2 n3/n0 d(w0-w1-w2-w3) + 1/2 (1+n1) n2 n3/n0 d(w0+w1-w2-w3)
+//           + 1/2 (1+n1)(1+n2) n3/n0 d(w0+w1+w2-w3) ]
+//      the processes and weights of SumOverPolarization / FindFandOmegaVer_4th
+//      (Tetrahedron4th_sep.f90:174-425) with a Gaussian in place of the tetrahedron
+//      integral; NaN terms are zeroed as there (:238-263); s0 < 3 skipped at q0 = Gamma.
+// The returned sum is the raw lw of TetrahedronIntegration4th; main.f90:660 adds 0.5 lw.
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "cplx.h"
+#include "spr_internal.h"
+
+namespace spr {
+
+struct Lw4Args {
+    const int *q0_grid;
+    int nq, L, m0, m1, m2;
+    int chunk, nchunks;
+    const int *blk_start, *tup_site;
+    const void *fc;
+    const int4 *site;
+    int nsite;
+    const void *phase, *evec;
+    const double *omega, *inv_omega, *nbe;
+    double sigma, factor;
+    double *partial;   // [q0][chunk][ND]
+};
+
+template <typename R, int ND, int BLK>
+__global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
+    using Z = cplx_t<R>;
+    constexpr int NB = ND / 3;
+    constexpr int ND2 = ND * ND, ND3 = ND2 * ND, ND4 = ND3 * ND;
+    constexpr int PER = (ND4 + BLK - 1) / BLK;
+    __shared__ Z s_a[ND4], s_b[ND4];
+    __shared__ Z s_e[4][ND2];
+    __shared__ double s_md[4][3][ND];   // omega, nbe, 1/omega of q0..q3
+    extern __shared__ __align__(16) int s_dyn[];
+    int *s_m = s_dyn;                       // [3][nsite]
+    Z *s_ph = reinterpret_cast<Z *>(s_dyn + ((3 * A.nsite + 3) & ~3));
+
+    const int tid = threadIdx.x;
+    const int ch = blockIdx.x, qb = blockIdx.y;
+    const int i0 = A.q0_grid[qb];
+    const int k00 = i0 % A.m0, k01 = (i0 / A.m0) % A.m1, k02 = i0 / (A.m0 * A.m1);
+    const bool g0 = i0 == 0;
+    const Z *evec = reinterpret_cast<const Z *>(A.evec);
+    const Z *phase = reinterpret_cast<const Z *>(A.phase);
+    const R *fc = reinterpret_cast<const R *>(A.fc);
+    for (int i = tid; i < A.L; i += BLK) s_ph[i] = phase[i];
+    for (int i = tid; i < ND2; i += BLK) s_e[0][i] = evec[(size_t)i0 * ND2 + i];
+    if (tid < ND) {
+        s_md[0][0][tid] = A.omega[(size_t)i0 * ND + tid];
+        s_md[0][1][tid] = A.nbe[(size_t)i0 * ND + tid];
+        s_md[0][2][tid] = A.inv_omega[(size_t)i0 * ND + tid];
+    }
+    const double c0 = 1.0 / (A.sigma * 2.5066282746310002);
+    const double hinv = 0.5 / (A.sigma * A.sigma);
+    double acc[ND];
+#pragma unroll
+    for (int s = 0; s < ND; ++s) acc[s] = 0.0;
+
+    const int q1_lo = ch * A.chunk, q1_hi = min(A.nq, q1_lo + A.chunk);
+    for (int q1 = q1_lo; q1 < q1_hi; ++q1) {
+        const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
+        for (int q2 = 0; q2 < A.nq; ++q2) {
+            const int k20 = q2 % A.m0, k21 = (q2 / A.m0) % A.m1, k22 = q2 / (A.m0 * A.m1);
+            int a0 = -(k00 + k10 + k20) % A.m0, a1 = -(k01 + k11 + k21) % A.m1, a2 = -(k02 + k12 + k22) % A.m2;
+            a0 += a0 < 0 ? A.m0 : 0;
+            a1 += a1 < 0 ? A.m1 : 0;
+            a2 += a2 < 0 ? A.m2 : 0;
+            const int q3 = a0 + A.m0 * (a1 + A.m1 * a2);
+            const int qs[3] = {q1, q2, q3};
+            __syncthreads();
+            for (int i = tid; i < 3 * ND2; i += BLK) {
+                const int k = i / ND2, e = i % ND2;
+                s_e[k + 1][e] = evec[(size_t)qs[k] * ND2 + e];
+            }
+            if (tid < 3 * ND) {
+                const int k = tid / ND, s = tid % ND;
+                s_md[k + 1][0][s] = A.omega[(size_t)qs[k] * ND + s];
+                s_md[k + 1][1][s] = A.nbe[(size_t)qs[k] * ND + s];
+                s_md[k + 1][2][s] = A.inv_omega[(size_t)qs[k] * ND + s];
+            }
+            for (int s = tid; s < A.nsite; s += BLK) {
+                const int4 c = A.site[s];
+                s_m[s] = (k10 * c.x + k11 * c.y + k12 * c.z) % A.L;
+                s_m[A.nsite + s] = (k20 * c.x + k21 * c.y + k22 * c.z) % A.L;
+                s_m[2 * A.nsite + s] = (a0 * c.x + a1 * c.y + a2 * c.z) % A.L;
+            }
+            __syncthreads();
+            // 1. phase sum into s_a[a][b][c][d]
+            for (int e = tid; e < ND4; e += BLK) {
+                const int a = e / ND3, b = (e / ND2) % ND, c = (e / ND) % ND, d = e % ND;
+                const int blk = ((a / 3 * NB + b / 3) * NB + c / 3) * NB + d / 3;
+                const int comp = (a % 3) * 27 + (b % 3) * 9 + (c % 3) * 3 + (d % 3);
+                Z t = cmk<R>(0, 0);
+                const int u1 = A.blk_start[blk + 1];
+                for (int u = A.blk_start[blk]; u < u1; ++u) {
+                    int m = s_m[A.tup_site[3 * u]] + s_m[A.nsite + A.tup_site[3 * u + 1]] +
+                            s_m[2 * A.nsite + A.tup_site[3 * u + 2]];
+                    m %= A.L;
+                    t = crfma(fc[(size_t)u * 81 + comp], s_ph[m], t);
+                }
+                s_a[e] = t;
+            }
+            // 2. four mode products; each pass contracts the last index and moves the new
+            //    mode index to the front: (x,y,z,c) -> (s,x,y,z)
+            Z *src = s_a, *dst = s_b;
+#pragma unroll
+            for (int pass = 0; pass < 4; ++pass) {
+                __syncthreads();
+                const Z *E = s_e[3 - pass];   // e3, e2, e1, e0
+                for (int e = tid; e < ND4; e += BLK) {
+                    const int s = e / ND3, xyz = e % ND3;
+                    Z z = cmk<R>(0, 0);
+#pragma unroll
+                    for (int c = 0; c < ND; ++c) z = cfma(src[xyz * ND + c], E[s * ND + c], z);
+                    dst[e] = z;
+                }
+                Z *t = src;
+                src = dst;
+                dst = t;
+            }
+            __syncthreads();
+            // 3. W4 and the three processes; src holds V[s0][s1][s2][s3]
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int e = tid + j * BLK;
+                if (e >= ND4) break;
+                const int s0 = e / ND3, s1 = (e / ND2) % ND, s2 = (e / ND) % ND, s3 = e % ND;
+                if (g0 && s0 < 3) continue;
+                const double w0 = s_md[0][0][s0], n0 = s_md[0][1][s0];
+                const double w1 = s_md[1][0][s1], n1 = s_md[1][1][s1];
+                const double w2 = s_md[2][0][s2], n2 = s_md[2][1][s2];
+                const double w3 = s_md[3][0][s3], n3 = s_md[3][1][s3];
+                const double W = (double)cabs2(src[e]) * (s_md[0][2][s0] * s_md[1][2][s1] * s_md[2][2][s2] *
+                                                          s_md[3][2][s3]) * A.factor;
+                const double x1 = w0 - (w1 + w2 + w3), x2 = w0 - (w2 + w3 - w1), x3 = w0 - (w3 - w1 - w2);
+                const double F1 = n1 * n2 * n3 / n0, F2 = (1.0 + n1) * n2 * n3 / n0,
+                             F3 = (1.0 + n1) * (1.0 + n2) * n3 / n0;
+                double t = W * (F1 * (c0 * exp(-hinv * x1 * x1)) / 6.0 + 0.5 * F2 * (c0 * exp(-hinv * x2 * x2)) +
+                                0.5 * F3 * (c0 * exp(-hinv * x3 * x3)));
+                if (t != t) t = 0.0;
+#pragma unroll
+                for (int s = 0; s < ND; ++s)
+                    if (s == s0) acc[s] += t;
+            }
+        }
+    }
+    __shared__ double s_red[BLK];
+    for (int s = 0; s < ND; ++s) {
+        __syncthreads();
+        s_red[tid] = acc[s];
+        __syncthreads();
+        for (int w = BLK / 2; w > 0; w >>= 1) {
+            if (tid < w) s_red[tid] += s_red[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) A.partial[((size_t)qb * A.nchunks + ch) * ND + s] = s_red[0];
+    }
+}
+
+__global__ void k_reduce_chunks(const double *partial, int nq0, int nchunks, int nd, double *lw) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq0 * nd) return;
+    const int q = i / nd, s = i % nd;
+    double acc = 0.0;
+    for (int c = 0; c < nchunks; ++c) acc += partial[((size_t)q * nchunks + c) * nd + s];
+    lw[i] = acc;
+}
+
+template <typename R, int ND>
+static int launch_lw4_t(Ctx &c, int q0_first, int q0_count, double sigma, double *d_lw) {
+    constexpr int BLK = 256;
+    Lw4Args A;
+    A.q0_grid = c.irr_grid + q0_first;
+    A.nq = c.nq;
+    A.L = c.L;
+    A.m0 = c.mesh[0];
+    A.m1 = c.mesh[1];
+    A.m2 = c.mesh[2];
+    const long target = 2048;
+    int nch = (int)std::min<long>(c.nq, std::max<long>(1, (target + q0_count - 1) / q0_count));
+    A.chunk = (c.nq + nch - 1) / nch;
+    A.nchunks = (c.nq + A.chunk - 1) / A.chunk;
+    A.blk_start = c.fc4.blk_start;
+    A.tup_site = c.fc4.tup_site;
+    A.fc = sizeof(R) == 8 ? (const void *)c.fc4.fc64 : (const void *)c.fc4.fc32;
+    A.site = c.fc4.site;
+    A.nsite = c.fc4.nsite;
+    A.phase = sizeof(R) == 8 ? (const void *)c.phase64 : (const void *)c.phase32;
+    A.evec = sizeof(R) == 8 ? (const void *)c.evec : (const void *)c.evec32;
+    A.omega = c.omega;
+    A.inv_omega = c.inv_omega;
+    A.nbe = c.nbe;
+    A.sigma = sigma;
+    const double pi = 4.0 * std::atan(1.0);
+    // Sctr4THz2 (constants.f90:62-63) / Nq^2
+    A.factor = 0.125 * pi * std::pow(1.054571817 * 1.602176565, 2) * std::pow(6.022140857, 4) * 1000000.0 /
+               ((double)c.nq * (double)c.nq);
+    int rc;
+    if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)q0_count * A.nchunks * ND * sizeof(double))))
+        return rc;
+    A.partial = c.partial;
+    const size_t dyn = (size_t)((3 * A.nsite + 3) & ~3) * sizeof(int) + (size_t)c.L * sizeof(cplx_t<R>);
+    SPR_HIP(hipEventRecord(c.ev0, c.stream));
+    hipLaunchKernelGGL((k_lw4<R, ND, BLK>), dim3(A.nchunks, q0_count), dim3(BLK), dyn, c.stream, A);
+    SPR_HIP(hipGetLastError());
+    SPR_HIP(hipEventRecord(c.ev1, c.stream));
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((q0_count * ND + 255) / 256), dim3(256), 0, c.stream, c.partial,
+                       q0_count, A.nchunks, ND, d_lw);
+    SPR_HIP(hipGetLastError());
+    SPR_HIP(hipEventSynchronize(c.ev1));
+    float ms = 0.f;
+    SPR_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    c.launches += 1;
+    c.kernel_ms += ms;
+    return SPR_OK;
+}
+
+int launch_lw4(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw) {
+    if (prec == SPR_PREC_BF16) {
+        set_error("four-phonon path: BF16 mode is not built (FP64 and FP32 are)");
+        return SPR_E_UNSUPPORTED;
+    }
+    const bool f64 = prec == SPR_PREC_FP64;
+    switch (c.nd) {
+        case 3: return f64 ? launch_lw4_t<double, 3>(c, q0_first, q0_count, sigma, d_lw)
+                           : launch_lw4_t<float, 3>(c, q0_first, q0_count, sigma, d_lw);
+        case 6: return f64 ? launch_lw4_t<double, 6>(c, q0_first, q0_count, sigma, d_lw)
+                           : launch_lw4_t<float, 6>(c, q0_first, q0_count, sigma, d_lw);
+        default:
+            set_error("four-phonon path: Ndof " + std::to_string(c.nd) + " not built (3, 6)");
+            return SPR_E_UNSUPPORTED;
+    }
+}
+
+}  // namespace spr

@@ -258,7 +258,9 @@ extern "C" int spr_ctx_destroy(spr_ctx *h) {
     free_ifc(c.fc4);
     for (void *p : {(void *)c.phase64, (void *)c.phase32, (void *)c.evec, (void *)c.evec32,
                     (void *)c.omega, (void *)c.inv_omega, (void *)c.nbe, (void *)c.vel,
-                    (void *)c.irr_grid, (void *)c.partial, (void *)c.out, (void *)c.counter})
+                    (void *)c.irr_grid, (void *)c.partial, (void *)c.out, (void *)c.counter,
+                    (void *)c.d_pairs, (void *)c.d_item_q0, (void *)c.d_item_off, (void *)c.d_item_cnt,
+                    (void *)c.d_item_first, (void *)c.omega32})
         if (p) (void)hipFree(p);
     if (c.ev0) (void)hipEventDestroy(c.ev0);
     if (c.ev1) (void)hipEventDestroy(c.ev1);
@@ -294,7 +296,7 @@ extern "C" int spr_v3_w(spr_ctx *h, int i0, double *W_out) {
     SPR_HIP(hipSetDevice(c.device));
     const size_t n = (size_t)c.nd * c.nd * c.nd * c.nq;
     if ((rc = ensure((void **)&c.out, &c.out_cap, n * sizeof(double) + 64 * sizeof(double)))) return rc;
-    if ((rc = launch_lw3(c, c.irr_grid + i0, 1, 1.0, SPR_PREC_FP64, nullptr, c.out))) return rc;
+    if ((rc = launch_lw3(c, i0, 1, 1.0, SPR_PREC_FP64, nullptr, c.out))) return rc;
     SPR_HIP(hipMemcpyAsync(W_out, c.out, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SPR_HIP(hipStreamSynchronize(c.stream));
     return SPR_OK;
@@ -319,8 +321,8 @@ static int lw_common(spr_ctx *h, int first, int count, double sigma, int prec, d
     SPR_HIP(hipSetDevice(c.device));
     const size_t n = (size_t)count * c.nd;
     if ((rc = ensure((void **)&c.out, &c.out_cap, n * sizeof(double)))) return rc;
-    rc = order == 3 ? launch_lw3(c, c.irr_grid + first, count, sigma, prec, c.out, nullptr)
-                    : launch_lw4(c, c.irr_grid + first, count, sigma, prec, c.out);
+    rc = order == 3 ? launch_lw3(c, first, count, sigma, prec, c.out, nullptr)
+                    : launch_lw4(c, first, count, sigma, prec, c.out);
     if (rc) return rc;
     SPR_HIP(hipMemcpyAsync(lw, c.out, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SPR_HIP(hipStreamSynchronize(c.stream));

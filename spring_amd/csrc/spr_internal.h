@@ -63,6 +63,14 @@ struct Ctx {
     double *out = nullptr;
     size_t out_cap = 0;
     int *counter = nullptr;
+    // pair-permutation reduced (q1, q2) lists of every irreducible q0, in 64-pair batches
+    bool pairs_built = false;
+    long n_pairs = 0;
+    int n_items = 0;
+    int2 *d_pairs = nullptr;
+    int *d_item_q0 = nullptr, *d_item_off = nullptr, *d_item_cnt = nullptr, *d_item_first = nullptr;
+    std::vector<int> h_item_first;
+    float *omega32 = nullptr;      // (3, nq, nd): omega, 1/omega, nBE in FP32 for the MFMA kernel
     // timing of the dominant kernel
     int launches = 0;
     double kernel_ms = 0.0;
@@ -71,9 +79,12 @@ struct Ctx {
 int ensure(void **p, size_t *cap, size_t bytes);
 
 // kernels launched from the API layer
-int launch_lw3(Ctx &c, const int *d_q0_grid, int nq0, double sigma, int prec, double *d_lw,
+// irreducible q0 [q0_first, q0_first + q0_count); d_w_out (FP64 W of every q1) needs count 1
+int launch_lw3(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw,
                double *d_w_out);
-int launch_lw4(Ctx &c, const int *d_q0_grid, int nq0, double sigma, int prec, double *d_lw);
+int launch_lw4(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw);
+int ensure_pairs(Ctx &c);
+int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, double *d_lw);
 int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_mode_k);
 
 }  // namespace spr

@@ -96,3 +96,13 @@ def test_lw3_parity_10atom(rocksalt10):
     assert rel_err(lw, ref, 1e-9) < 1e-10
     lw32 = ctx.lw3_all(0.1, "fp32")
     assert rel_err(lw32, ref, 1e-6) < 1e-3
+
+
+def test_small_kernel_matches_generic(small, monkeypatch):
+    """The 2-atom batch kernel and the generic kernel agree (same FP64 sums, other order)."""
+    m, qm, ph, ctx, inp = small
+    a = ctx.lw3_all(0.1, "fp64")
+    monkeypatch.setenv("SPR_KERNEL", "generic")
+    b = ctx.lw3_all(0.1, "fp64")
+    monkeypatch.delenv("SPR_KERNEL")
+    assert rel_err(a, b, 1e-9) < 1e-12

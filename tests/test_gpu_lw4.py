@@ -1,0 +1,35 @@
+"""GPU parity of the four-phonon Gaussian path against the FP64 CPU oracle."""
+import numpy as np
+import pytest
+
+from tests.conftest import problem, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def test_lw4_parity_fp64_and_fp32(rocksalt):
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    qm, ph = problem(rocksalt, (3, 3, 3))
+    ctx = GpuContext(rocksalt, qm, ph)
+    inp = O.Inputs(rocksalt, qm, ph, which=4)
+    lw = ctx.lw4_all(0.1, "fp64")
+    ref = O.lw4_gauss_all(inp, qm.irr_idx, 0.1, threads=8)
+    print("lw4 max rel", rel_err(lw, ref, 1e-9), flush=True)
+    assert rel_err(lw, ref, 1e-9) < 1e-10
+    lw32 = ctx.lw4_all(0.1, "fp32")
+    assert rel_err(lw32, ref, 1e-6) < 1e-4
+    # single-q0 entry point agrees with the batched one
+    assert np.allclose(ctx.lw4(1, 0.1, "fp64"), lw[1], rtol=1e-12, atol=0)
+
+
+def test_lw4_even_mesh(rocksalt):
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    qm, ph = problem(rocksalt, (4, 4, 4))
+    ctx = GpuContext(rocksalt, qm, ph)
+    inp = O.Inputs(rocksalt, qm, ph, which=4)
+    sel = [0, 3, qm.nq0 - 1]
+    lw = ctx.lw4_all(0.1, "fp64")[sel]
+    ref = np.array([O.lw4_gauss(inp, int(qm.irr_idx[i]), 0.1) for i in sel])
+    assert rel_err(lw, ref, 1e-9) < 1e-10

@@ -57,7 +57,16 @@ def write_bundle(path, model, qm, ph, T: float, sigma: float, prefix: str = "syn
 
 
 def read_linewidth_file(fname, ndof):
-    """Rows of linewidth_IBZ_*.txt: (omega(ndof), lw(ndof)) per irreducible q0."""
-    data = np.loadtxt(fname, comments="#")
-    data = np.atleast_2d(data)
+    """Rows of linewidth_IBZ_*.txt: (omega(ndof), lw(ndof)) per irreducible q0.
+
+    Fortran's G20.12 drops the exponent letter when the exponent has three digits
+    ('0.4920108584-171'), so such fields are repaired before parsing."""
+    import re
+    rows = []
+    for line in open(fname):
+        if line.lstrip().startswith("#") or not line.strip():
+            continue
+        fields = [re.sub(r"(\d)([-+]\d{3})$", r"\1E\2", f) for f in line.split()]
+        rows.append([float(f) for f in fields])
+    data = np.atleast_2d(np.array(rows))
     return data[:, :ndof], data[:, ndof:2 * ndof]

@@ -30,6 +30,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 struct MfmaArgs {
+    ItemCommon ic;
     const int *item_q0, *item_off, *item_cnt;
     const int2 *pairs;
     const int *q0_grid;
@@ -42,7 +43,6 @@ struct MfmaArgs {
     const float2 *evec;
     const float *omega, *inv_omega, *nbe;
     float sigma, factor;
-    double *partial;
 };
 
 template <bool BF> struct Store;
@@ -87,7 +87,7 @@ __device__ __forceinline__ v4f mma_tile(FA A, FB B, int m_base, int n_base, v4f 
 }
 
 template <bool BF, int NB>
-__global__ __launch_bounds__(512) void k_lw3_mfma(MfmaArgs A) {
+__device__ __forceinline__ void mfma_item(const MfmaArgs &A, const int item) {
     using S = typename Store<BF>::S;
     constexpr int ND = 3 * NB;
     constexpr int ND2 = ND * ND;
@@ -118,11 +118,11 @@ __global__ __launch_bounds__(512) void k_lw3_mfma(MfmaArgs A) {
 
     const int tid = threadIdx.x, l = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int item = blockIdx.x;
     const int qi = A.item_q0[item];
     const int i0 = A.q0_grid[qi];
     const int cnt = A.item_cnt[item];
     const int poff = A.item_off[item];
+    __syncthreads();   // the previous item of a persistent workgroup is done with LDS
 
     for (int i = tid; i < A.L; i += 512) s_ph[i] = A.phase[i];
     for (int i = tid; i < ND2; i += 512) {
@@ -290,7 +290,24 @@ __global__ __launch_bounds__(512) void k_lw3_mfma(MfmaArgs A) {
         for (int ww = nt; ww < NWAVE; ww += 4)
             for (int g = 0; g < 4; ++g)
                 for (int q = 0; q < 2; ++q) tot += s_red[ww * 64 + g * 16 + sj + q];
-        A.partial[(size_t)item * ND + tid] = tot;
+        A.ic.partial[(size_t)item * ND + tid] = tot;
+    }
+}
+
+template <bool BF, int NB>
+__global__ __launch_bounds__(512) void k_lw3_mfma(MfmaArgs A) {
+    if (!A.ic.counter) {
+        mfma_item<BF, NB>(A, blockIdx.x);
+        return;
+    }
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(A.ic.counter, 1);
+        __syncthreads();
+        const int item = s_item;
+        if (item >= A.ic.nitems) break;
+        mfma_item<BF, NB>(A, item);
     }
 }
 
@@ -307,19 +324,19 @@ static size_t mfma_lds(int nsite, int L) {
 }
 
 template <bool BF, int NB>
-static int launch_mfma_t(Ctx &c, MfmaArgs A, int nitems) {
+static int launch_mfma_t(Ctx &c, MfmaArgs A, int nitems, int q0_first, int q0_count, int it0, double *d_lw) {
     const size_t dyn = mfma_lds<BF, NB>(A.nsite, A.L);
     if (dyn > 160 * 1024) {
         set_error("MFMA kernel: LDS need " + std::to_string(dyn) + " B exceeds 160 KB");
         return SPR_E_UNSUPPORTED;
     }
-    SPR_HIP(hipFuncSetAttribute((const void *)k_lw3_mfma<BF, NB>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
-    hipLaunchKernelGGL((k_lw3_mfma<BF, NB>), dim3(nitems), dim3(512), dyn, c.stream, A);
-    SPR_HIP(hipGetLastError());
-    return SPR_OK;
+    const void *fn = (const void *)k_lw3_mfma<BF, NB>;
+    SPR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 512, dyn) != hipSuccess || per_cu < 1) per_cu = 1;
+    ItemLaunch L{fn, &A, sizeof(A), 512, dyn, nitems, per_cu, std::string(BF ? "mfma_bf16" : "mfma_f32")};
+    return run_items(c, L, q0_first, q0_count, it0, d_lw);
 }
-
 
 int launch_lw3_mfma(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw) {
     int rc;
@@ -341,8 +358,6 @@ int launch_lw3_mfma(Ctx &c, int q0_first, int q0_count, double sigma, int prec, 
     }
     const int it0 = c.h_item_first[q0_first], it1 = c.h_item_first[q0_first + q0_count];
     const int nitems = it1 - it0;
-    if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)c.n_items * c.nd * sizeof(double))))
-        return rc;
     const size_t nm = (size_t)c.nq * c.nd;
     MfmaArgs A;
     A.item_q0 = c.d_item_q0 + it0;
@@ -367,29 +382,19 @@ int launch_lw3_mfma(Ctx &c, int q0_first, int q0_count, double sigma, int prec, 
     const double pi = 4.0 * std::atan(1.0);
     A.factor = (float)(1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) *
                        1000000.0 * pi * 0.0625 / (double)c.nq);
-    A.partial = c.partial;
+    A.ic.partial = c.ipartial;
     const bool bf = prec == SPR_PREC_BF16;
-    SPR_HIP(hipEventRecord(c.ev0, c.stream));
     switch (c.nb) {
-#define SPR_NB(n)                                                                      \
-    case n:                                                                            \
-        rc = bf ? launch_mfma_t<true, n>(c, A, nitems) : launch_mfma_t<false, n>(c, A, nitems); \
-        break;
+#define SPR_NB(n)                                                                                    \
+    case n:                                                                                          \
+        return bf ? launch_mfma_t<true, n>(c, A, nitems, q0_first, q0_count, it0, d_lw)               \
+                  : launch_mfma_t<false, n>(c, A, nitems, q0_first, q0_count, it0, d_lw);
         SPR_NB(2) SPR_NB(3) SPR_NB(4) SPR_NB(5) SPR_NB(6) SPR_NB(8) SPR_NB(10)
 #undef SPR_NB
         default:
             set_error("MFMA kernel: Nb " + std::to_string(c.nb) + " not built (2-6, 8, 10)");
             return SPR_E_UNSUPPORTED;
     }
-    if (rc) return rc;
-    SPR_HIP(hipEventRecord(c.ev1, c.stream));
-    if ((rc = launch_reduce_items(c, q0_first, q0_count, it0, d_lw))) return rc;
-    SPR_HIP(hipEventSynchronize(c.ev1));
-    float ms = 0.f;
-    SPR_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
-    c.launches += 1;
-    c.kernel_ms += ms;
-    return SPR_OK;
 }
 
 }  // namespace spr

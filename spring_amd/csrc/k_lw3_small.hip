@@ -27,6 +27,7 @@
 namespace spr {
 
 struct SmallArgs {
+    ItemCommon ic;
     const int *item_q0;      // (nitems) irreducible index of each batch
     const int *item_off;     // (nitems) offset into the pair list
     const int *item_cnt;     // (nitems) pairs in the batch (<= 64)
@@ -42,11 +43,10 @@ struct SmallArgs {
     const void *evec;
     const double *omega, *inv_omega, *nbe;
     double sigma, factor;
-    double *partial;         // (nitems, ndof)
 };
 
 template <typename R, int NB>
-__global__ __launch_bounds__(NB * 3 * 64) void k_lw3_small(SmallArgs A) {
+__device__ __forceinline__ void small_item(const SmallArgs &A, const int item) {
     using Z = cplx_t<R>;
     constexpr int ND = 3 * NB;
     constexpr int ND2 = ND * ND;
@@ -63,7 +63,6 @@ __global__ __launch_bounds__(NB * 3 * 64) void k_lw3_small(SmallArgs A) {
     const int lane = threadIdx.x & 63;
     const int a = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave index
     const int mu = a / 3, al = a % 3;
-    const int item = blockIdx.x;
     const int qi = A.item_q0[item];
     const int i0 = A.q0_grid[qi];
     const int cnt = A.item_cnt[item];
@@ -192,7 +191,25 @@ __global__ __launch_bounds__(NB * 3 * 64) void k_lw3_small(SmallArgs A) {
     // wave reduction (fixed order)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_down(tot, off, 64);
-    if (lane == 0) A.partial[(size_t)item * ND + s0] = tot;
+    if (lane == 0) A.ic.partial[(size_t)item * ND + s0] = tot;
+}
+
+// One workgroup per item, or (counter set) a persistent grid pulling items until none are left.
+template <typename R, int NB>
+__global__ __launch_bounds__(NB * 3 * 64) void k_lw3_small(SmallArgs A) {
+    if (!A.ic.counter) {
+        small_item<R, NB>(A, blockIdx.x);
+        return;
+    }
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(A.ic.counter, 1);
+        __syncthreads();
+        const int item = s_item;
+        if (item >= A.ic.nitems) break;
+        small_item<R, NB>(A, item);
+    }
 }
 
 // lw[q][s] = sum of partial[item][s] over the items [first[q], first[q+1]) in order
@@ -207,7 +224,8 @@ __global__ void k_reduce_items(const double *partial, const int *first, int base
 }
 
 template <typename R, int NB>
-static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems) {
+static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first, int q0_count, int it0,
+                          double *d_lw) {
     constexpr int ND = 3 * NB;
     SmallArgs A = A0;
     A.fc = sizeof(R) == 8 ? (const void *)c.fc3.fc64 : (const void *)c.fc3.fc32;
@@ -219,11 +237,14 @@ static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems) {
         set_error("small-cell kernel: LDS need exceeds 160 KB (too many neighbour sites)");
         return SPR_E_UNSUPPORTED;
     }
-    SPR_HIP(hipFuncSetAttribute((const void *)k_lw3_small<R, NB>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
-    hipLaunchKernelGGL((k_lw3_small<R, NB>), dim3(nitems), dim3(ND * 64), dyn, c.stream, A);
-    SPR_HIP(hipGetLastError());
-    return SPR_OK;
+    const void *fn = (const void *)k_lw3_small<R, NB>;
+    if (dyn > 64 * 1024) SPR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, ND * 64, dyn) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    ItemLaunch L{fn, &A, sizeof(A), ND * 64, dyn, nitems, per_cu,
+                 std::string("small") + (sizeof(R) == 8 ? "64" : "32")};
+    return run_items(c, L, q0_first, q0_count, it0, d_lw);
 }
 
 // Builds (once per context) the pair lists of every irreducible q0 and the 64-pair batches.
@@ -272,6 +293,10 @@ int ensure_pairs(Ctx &c) {
     if ((rc = up(&c.d_item_off, ioff))) return rc;
     if ((rc = up(&c.d_item_cnt, icnt))) return rc;
     if ((rc = up(&c.d_item_first, first))) return rc;
+    if (hipMalloc((void **)&c.ipartial, (size_t)c.n_items * c.nd * sizeof(double)) != hipSuccess) {
+        set_error("item partials: hipMalloc failed");
+        return SPR_E_OOM;
+    }
     c.h_item_first = first;
     c.pairs_built = true;
     return SPR_OK;
@@ -279,7 +304,7 @@ int ensure_pairs(Ctx &c) {
 
 int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, double *d_lw) {
     hipLaunchKernelGGL(k_reduce_items, dim3((q0_count * c.nd + 255) / 256), dim3(256), 0, c.stream,
-                       c.partial, c.d_item_first + q0_first, it0, q0_count, c.nd, d_lw);
+                       c.ipartial, c.d_item_first + q0_first, it0, q0_count, c.nd, d_lw);
     SPR_HIP(hipGetLastError());
     return SPR_OK;
 }
@@ -290,8 +315,6 @@ int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec,
     if ((rc = ensure_pairs(c))) return rc;
     const int it0 = c.h_item_first[q0_first], it1 = c.h_item_first[q0_first + q0_count];
     const int nitems = it1 - it0;
-    if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)c.n_items * c.nd * sizeof(double))))
-        return rc;
     SmallArgs A;
     A.item_q0 = c.d_item_q0 + it0;
     A.item_off = c.d_item_off + it0;
@@ -312,25 +335,16 @@ int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec,
     const double pi = 4.0 * std::atan(1.0);
     A.factor = 1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) *
                1000000.0 * pi * 0.0625 / (double)c.nq;   // Sctr3THz2 / Nq, constants.f90:59-60
-    A.partial = c.partial;
-    SPR_HIP(hipEventRecord(c.ev0, c.stream));
+    A.ic.partial = c.ipartial;
+    const bool f64 = prec == SPR_PREC_FP64;
     if (c.nb == 2)
-        rc = prec == SPR_PREC_FP64 ? launch_small_t<double, 2>(c, A, nitems) : launch_small_t<float, 2>(c, A, nitems);
-    else if (c.nb == 1)
-        rc = prec == SPR_PREC_FP64 ? launch_small_t<double, 1>(c, A, nitems) : launch_small_t<float, 1>(c, A, nitems);
-    else {
-        set_error("small-cell kernel supports Nb <= 2");
-        return SPR_E_UNSUPPORTED;
-    }
-    if (rc) return rc;
-    SPR_HIP(hipEventRecord(c.ev1, c.stream));
-    if ((rc = launch_reduce_items(c, q0_first, q0_count, it0, d_lw))) return rc;
-    SPR_HIP(hipEventSynchronize(c.ev1));
-    float ms = 0.f;
-    SPR_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
-    c.launches += 1;
-    c.kernel_ms += ms;
-    return SPR_OK;
+        return f64 ? launch_small_t<double, 2>(c, A, nitems, q0_first, q0_count, it0, d_lw)
+                   : launch_small_t<float, 2>(c, A, nitems, q0_first, q0_count, it0, d_lw);
+    if (c.nb == 1)
+        return f64 ? launch_small_t<double, 1>(c, A, nitems, q0_first, q0_count, it0, d_lw)
+                   : launch_small_t<float, 1>(c, A, nitems, q0_first, q0_count, it0, d_lw);
+    set_error("small-cell kernel supports Nb <= 2");
+    return SPR_E_UNSUPPORTED;
 }
 
 }  // namespace spr

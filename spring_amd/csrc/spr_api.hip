@@ -241,6 +241,16 @@ extern "C" int spr_ctx_create(const spr_crystal *cr, const spr_ifc *fc3, const s
     c.nsym = mesh->nsym;
     if (mesh->nsym > 0 && mesh->rot) c.h_rot.assign(mesh->rot, mesh->rot + 9 * mesh->nsym);
     if ((rc = upload(&c.irr_grid, mesh->irr_grid, c.nq0))) return fail(rc);
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            c.ncu = prop.multiProcessorCount;
+    }
+    if (hipMalloc((void **)&c.out, (size_t)c.nq0 * c.nd * sizeof(double)) != hipSuccess) {
+        set_error("hipMalloc output failed");
+        return fail(SPR_E_OOM);
+    }
+    c.out_cap = (size_t)c.nq0 * c.nd * sizeof(double);
     if (hipMalloc((void **)&c.counter, 64 * sizeof(int)) != hipSuccess) {
         set_error("hipMalloc counter failed");
         return fail(SPR_E_OOM);
@@ -254,13 +264,15 @@ extern "C" int spr_ctx_destroy(spr_ctx *h) {
     Ctx &c = h->c;
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
+    drop_graphs(c);
     free_ifc(c.fc3);
     free_ifc(c.fc4);
     for (void *p : {(void *)c.phase64, (void *)c.phase32, (void *)c.evec, (void *)c.evec32,
                     (void *)c.omega, (void *)c.inv_omega, (void *)c.nbe, (void *)c.vel,
                     (void *)c.irr_grid, (void *)c.partial, (void *)c.out, (void *)c.counter,
                     (void *)c.d_pairs, (void *)c.d_item_q0, (void *)c.d_item_off, (void *)c.d_item_cnt,
-                    (void *)c.d_item_first, (void *)c.omega32})
+                    (void *)c.d_item_first, (void *)c.omega32,
+                    (void *)c.ipartial, (void *)c.wout})
         if (p) (void)hipFree(p);
     if (c.ev0) (void)hipEventDestroy(c.ev0);
     if (c.ev1) (void)hipEventDestroy(c.ev1);
@@ -295,9 +307,9 @@ extern "C" int spr_v3_w(spr_ctx *h, int i0, double *W_out) {
     if ((rc = check_i0(c, i0))) return rc;
     SPR_HIP(hipSetDevice(c.device));
     const size_t n = (size_t)c.nd * c.nd * c.nd * c.nq;
-    if ((rc = ensure((void **)&c.out, &c.out_cap, n * sizeof(double) + 64 * sizeof(double)))) return rc;
-    if ((rc = launch_lw3(c, i0, 1, 1.0, SPR_PREC_FP64, nullptr, c.out))) return rc;
-    SPR_HIP(hipMemcpyAsync(W_out, c.out, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if ((rc = ensure((void **)&c.wout, &c.wout_cap, n * sizeof(double)))) return rc;
+    if ((rc = launch_lw3(c, i0, 1, 1.0, SPR_PREC_FP64, nullptr, c.wout))) return rc;
+    SPR_HIP(hipMemcpyAsync(W_out, c.wout, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SPR_HIP(hipStreamSynchronize(c.stream));
     return SPR_OK;
 }
@@ -319,12 +331,13 @@ static int lw_common(spr_ctx *h, int first, int count, double sigma, int prec, d
         return SPR_E_STATE;
     }
     SPR_HIP(hipSetDevice(c.device));
+    // c.out (nq0 x Ndof) is allocated once; captured graphs keep pointers into it
     const size_t n = (size_t)count * c.nd;
-    if ((rc = ensure((void **)&c.out, &c.out_cap, n * sizeof(double)))) return rc;
-    rc = order == 3 ? launch_lw3(c, first, count, sigma, prec, c.out, nullptr)
-                    : launch_lw4(c, first, count, sigma, prec, c.out);
+    double *d_lw = c.out + (size_t)first * c.nd;
+    rc = order == 3 ? launch_lw3(c, first, count, sigma, prec, d_lw, nullptr)
+                    : launch_lw4(c, first, count, sigma, prec, d_lw);
     if (rc) return rc;
-    SPR_HIP(hipMemcpyAsync(lw, c.out, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SPR_HIP(hipMemcpyAsync(lw, d_lw, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SPR_HIP(hipStreamSynchronize(c.stream));
     return SPR_OK;
 }

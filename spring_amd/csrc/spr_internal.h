@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/spring_gpu.h"
@@ -20,6 +21,25 @@ void set_error(const std::string &msg);
             return SPR_E_HIP;                                                             \
         }                                                                                 \
     } while (0)
+
+// First member of every item-kernel argument struct: the launcher sets these.
+struct ItemCommon {
+    int *counter;       // persistent mode: atomic work counter (nullptr: one workgroup per item)
+    int nitems;
+    double *partial;    // (nitems, ndof) per-item partial linewidths
+};
+
+// One item-kernel launch as the launcher (k_launch.hip) sees it.
+struct ItemLaunch {
+    const void *func;
+    void *args;             // points to the kernel's argument struct (ItemCommon first)
+    size_t args_size;
+    int block;
+    size_t dyn;
+    int nitems;
+    int per_cu;             // resident workgroups per compute unit
+    std::string name;
+};
 
 // Anharmonic IFC re-laid out for the phase sum.  The sum over (N1, N2[, N3]) is grouped by
 // the basis atoms of the sites: for every (mu, nu, eta[, rho]) "block" the list of
@@ -71,6 +91,12 @@ struct Ctx {
     int *d_item_q0 = nullptr, *d_item_off = nullptr, *d_item_cnt = nullptr, *d_item_first = nullptr;
     std::vector<int> h_item_first;
     float *omega32 = nullptr;      // (3, nq, nd): omega, 1/omega, nBE in FP32 for the MFMA kernel
+    int ncu = 256;                 // compute units of the device
+    int last_mode = -1;            // launch mode of the last item launch (0: one WG per item)
+    std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+    double *ipartial = nullptr;    // (n_items, nd) item partials (stable: graphs keep the pointer)
+    double *wout = nullptr;        // W parity output
+    size_t wout_cap = 0;
     // timing of the dominant kernel
     int launches = 0;
     double kernel_ms = 0.0;
@@ -84,6 +110,8 @@ int launch_lw3(Ctx &c, int q0_first, int q0_count, double sigma, int prec, doubl
                double *d_w_out);
 int launch_lw4(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw);
 int ensure_pairs(Ctx &c);
+int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double *d_lw);
+void drop_graphs(Ctx &c);
 int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, double *d_lw);
 int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_mode_k);
 

@@ -75,7 +75,8 @@ def test_kappa_parity(rocksalt):
     from oracle import oracle as O
     qm, ph = problem(rocksalt, (5, 5, 5))
     ctx = GpuContext(rocksalt, qm, ph)
-    lw = ctx.lw3_all(0.1, "fp64")
+    lw = ctx.lw3_all(1.0, "fp64")      # broad sigma: finite linewidth for every mode
+    assert np.all(lw[0, 3:] > 0) and np.all(lw[1:] > 0)
     k_full, _ = ctx.kappa(lw, 300.0, symm=False)
     k_sym, _ = ctx.kappa(lw, 300.0, symm=True)
     ref_full = O.kappa_rta(qm, ph, lw, 300.0, rocksalt.crystal.vol)

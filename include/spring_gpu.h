@@ -40,7 +40,7 @@ extern "C" {
  * final reductions stay FP64 in every mode) */
 #define SPR_PREC_FP64 0
 #define SPR_PREC_FP32 1      /* FP32 phase sums + contraction, compensated FP32 delta sum */
-#define SPR_PREC_BF16 2      /* BF16 per-tile-scaled MFMA contraction, FP32 accumulate */
+#define SPR_PREC_BF16 2      /* BF16-operand MFMA contraction, FP32 accumulate */
 
 /* crystal: the `cell` fields the path uses (src/Kappa/Src/unit_mod.f90:9-38) */
 typedef struct {
@@ -96,6 +96,11 @@ const char *spr_last_error(void);
  * written to W_out (Ndof^3 * Nq doubles). FP64. */
 int spr_v3_w(spr_ctx *ctx, int i0, double *W_out);
 
+/* The same W for a list of (irreducible q0 index, q1 grid index) pairs: W_out(Ndof^3, n),
+ * W_out(s2,s1,s0,k) for pair k (q2 = -(q0+q1) mod G).  FP64.  Source of the training samples
+ * of the learned proposal (spr_mlp_*). */
+int spr_v3_w_pairs(spr_ctx *ctx, int n, const int *q0_index, const int *q1_grid, double *W_out);
+
 /* Gaussian-delta three-phonon linewidth of irreducible q0 `i0` (ScatterProbAllq1 fused with
  * CalculateLinewidth, ThirdOrder_m.f90:442-543): lw(Ndof), rad/ps. */
 int spr_lw3(spr_ctx *ctx, int i0, double sigma, int prec, double *lw);
@@ -115,6 +120,32 @@ int spr_lw4_all(spr_ctx *ctx, double sigma, int prec, double *lw_all);
  * symm = 1) from LW_Allq0(Ndof, nq0), FP64: kappa(9) W/mK, mode_k(9, Ndof) optional. */
 int spr_kappa_rta(spr_ctx *ctx, const double *lw_all, double T, int symm,
                   double *kappa, double *mode_k);
+
+/* Per-pair terms of the three-phonon Gaussian sum for caller-chosen (q0, q1) pairs:
+ * out(Ndof, n): out(s0, k) = sum_{s1,s2} W(s2,s1,s0; q0_index(k), q1_grid(k)) [delta terms], the q1
+ * term of CalculateLinewidth's sum (ThirdOrder_m.f90:490-541).  Used by the sampled (importance)
+ * estimate of the linewidth.  FP64 needs Nb <= 2; FP32/BF16 any supported Nb. */
+int spr_lw3_pairs(spr_ctx *ctx, int n, const int *q0_index, const int *q1_grid, double sigma,
+                  int prec, double *out);
+
+/* Learned proposal for importance sampling of the linewidth sum: a 3-layer MLP
+ * (n_features -> 128 -> 128 -> 1, tanh) predicting log W of a branch triplet from
+ * (q0, q1, omegas, branch ids).  No counterpart in the reference.  FP32 training with Adam
+ * on the GPU; inference in FP32 or BF16 (MFMA).  Parameters: W1[F][128] b1 W2[128][128] b2
+ * W3[128] b3, FP32. */
+typedef struct spr_mlp spr_mlp;
+int spr_mlp_create(int n_features, unsigned long long seed, int device, spr_mlp **out);
+int spr_mlp_destroy(spr_mlp *m);
+int spr_mlp_num_params(spr_mlp *m, long *n_params);
+int spr_mlp_get_params(spr_mlp *m, float *params);
+int spr_mlp_set_params(spr_mlp *m, const float *params);
+/* gradient of mean((f(X) - y)^2) w.r.t. the parameters, and the loss (no update) */
+int spr_mlp_grad(spr_mlp *m, const float *X, const float *y, long n, float *grad, double *loss);
+/* Adam, mini-batches in the given order; loss_per_epoch[epochs] */
+int spr_mlp_train(spr_mlp *m, const float *X, const float *y, long n, int epochs, int batch,
+                  float lr, double *loss_per_epoch);
+/* prec: SPR_PREC_FP32 or SPR_PREC_BF16 */
+int spr_mlp_predict(spr_mlp *m, const float *X, long n, int prec, float *out);
 
 /* Timing of the last launch of the dominant kernel (HIP events on the context's stream):
  * number of launches and summed milliseconds since the last reset. */

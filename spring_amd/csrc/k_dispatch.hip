@@ -1,5 +1,6 @@
 // Chooses the kernel variant for each entry point.
 #include <cstdlib>
+#include <vector>
 #include <string>
 
 #include "spr_internal.h"
@@ -34,5 +35,78 @@ int launch_lw3(Ctx &c, int q0_first, int q0_count, double sigma, int prec, doubl
     return launch_lw3_generic(c, c.irr_grid + q0_first, q0_count, sigma, prec, d_lw, nullptr);
 }
 
+
+}  // namespace spr
+
+namespace spr {
+
+// Per-pair linewidth contributions for caller-chosen (q0, q1) pairs: out[k][s0] =
+// sum_{s1,s2} W [delta terms] of pair k alone (the q1 term of CalculateLinewidth's sum).
+int launch_lw3_pairs(Ctx &c, int n, const int *q0_index, const int *q1_grid, double sigma, int prec,
+                     double *h_out) {
+    if (n < 1) return SPR_OK;
+    if (prec == SPR_PREC_FP64 && c.nb > 2) {
+        set_error("spr_lw3_pairs: FP64 needs Nb <= 2 (FP32/BF16 use the MFMA kernel)");
+        return SPR_E_UNSUPPORTED;
+    }
+    int rc;
+    if ((rc = ensure_pairs(c))) return rc;
+    std::vector<int> iq0(n), ioff(n), icnt(n, 1);
+    std::vector<int2> pr(n);
+    const int m0 = c.mesh[0], m1 = c.mesh[1], m2 = c.mesh[2];
+    for (int k = 0; k < n; ++k) {
+        if (q0_index[k] < 0 || q0_index[k] >= c.nq0 || q1_grid[k] < 0 || q1_grid[k] >= c.nq) {
+            set_error("spr_lw3_pairs: index out of range");
+            return SPR_E_ARG;
+        }
+        const int i0 = c.h_irr_grid[q0_index[k]], q1 = q1_grid[k];
+        const int a0 = ((-(i0 % m0 + q1 % m0)) % m0 + m0) % m0;
+        const int a1 = ((-((i0 / m0) % m1 + (q1 / m0) % m1)) % m1 + m1) % m1;
+        const int a2 = ((-(i0 / (m0 * m1) + q1 / (m0 * m1))) % m2 + m2) % m2;
+        iq0[k] = q0_index[k];
+        ioff[k] = k;
+        pr[k] = make_int2(q1, a0 + m0 * (a1 + m1 * a2));
+    }
+    int *d_i = nullptr;
+    int2 *d_p = nullptr;
+    double *d_out = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(d_i);
+        (void)hipFree(d_p);
+        (void)hipFree(d_out);
+        c.override_items = nullptr;
+    };
+    if (hipMalloc((void **)&d_i, 3 * (size_t)n * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&d_p, (size_t)n * sizeof(int2)) != hipSuccess ||
+        hipMalloc((void **)&d_out, (size_t)n * c.nd * sizeof(double)) != hipSuccess) {
+        cleanup();
+        set_error("spr_lw3_pairs: hipMalloc failed");
+        return SPR_E_OOM;
+    }
+    if (hipMemcpy(d_i, iq0.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_i + n, ioff.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_i + 2 * n, icnt.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_p, pr.data(), n * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+        cleanup();
+        set_error("spr_lw3_pairs: upload failed");
+        return SPR_E_HIP;
+    }
+    ItemSet set{d_i, d_i + n, d_i + 2 * n, d_p, n, d_out};
+    c.override_items = &set;
+    rc = (prec == SPR_PREC_BF16 || (prec == SPR_PREC_FP32 && c.nb > 2))
+             ? launch_lw3_mfma(c, 0, 0, sigma, prec, nullptr)
+             : launch_lw3_small(c, 0, 0, sigma, prec, nullptr);
+    c.override_items = nullptr;
+    if (!rc && hipMemcpy(h_out, d_out, (size_t)n * c.nd * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("spr_lw3_pairs: download failed");
+        rc = SPR_E_HIP;
+    }
+    cleanup();
+    if (rc) return rc;
+    for (int k = 0; k < n; ++k)
+        if (pr[k].x != pr[k].y)
+            for (int s = 0; s < c.nd; ++s) h_out[(size_t)k * c.nd + s] *= 0.5;   // kernels weight q1 != q2 by 2
+    return SPR_OK;
+}
 
 }  // namespace spr

@@ -100,6 +100,11 @@ int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double
     if (L.dyn > 64 * 1024)
         SPR_HIP(hipFuncSetAttribute(L.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.dyn));
     // ---- launch mode: env override, tuned value, or default (persistent 2 x CUs) ----
+    if (L.direct) {
+        if ((rc = enqueue(c, L, 0, 0, 0, 0, nullptr, true))) return rc;
+        SPR_HIP(hipEventSynchronize(c.ev1));
+        return SPR_OK;
+    }
     int mode = 2;
     const std::string key = L.name + "|nd" + std::to_string(c.nd) + "|nq" + std::to_string(c.nq) + "|items" +
                             std::to_string(L.nitems);

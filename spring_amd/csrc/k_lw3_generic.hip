@@ -35,7 +35,9 @@ struct Lw3Args {
     const double *omega, *inv_omega, *nbe;
     double sigma, factor;
     double *partial;   // [q0][s0b][chunk][S0B]
-    double *w_out;     // [q1][s0][s1][s2]
+    double *w_out;     // [q1][s0][s1][s2]  (pair mode: [pair][s0][s1][s2])
+    const int *pair_q0;   // pair mode (W_OUT only): block x = pair, q0 index and q1 per pair
+    const int *pair_q1;
 };
 
 __device__ __forceinline__ void grid_coords(int i, int m0, int m1, int &k0, int &k1, int &k2) {
@@ -67,7 +69,8 @@ __global__ __launch_bounds__(BLK) void k_lw3_generic(Lw3Args A) {
 
     const int tid = threadIdx.x;
     const int ch = blockIdx.x, s0b = blockIdx.y, qb = blockIdx.z;
-    const int i0 = A.q0_grid[qb];
+    const bool pair_mode = W_OUT && A.pair_q0 != nullptr;
+    const int i0 = A.q0_grid[pair_mode ? A.pair_q0[ch] : qb];
     int k00, k01, k02;
     grid_coords(i0, A.m0, A.m1, k00, k01, k02);
     const Z *evec = reinterpret_cast<const Z *>(A.evec);
@@ -87,8 +90,8 @@ __global__ __launch_bounds__(BLK) void k_lw3_generic(Lw3Args A) {
 #pragma unroll
     for (int l = 0; l < S0B; ++l) accd[l] = 0.0;
 
-    const int q1_lo = ch * A.chunk;
-    const int q1_hi = min(A.nq, q1_lo + A.chunk);
+    const int q1_lo = pair_mode ? A.pair_q1[ch] : ch * A.chunk;
+    const int q1_hi = pair_mode ? q1_lo + 1 : min(A.nq, q1_lo + A.chunk);
     for (int q1 = q1_lo; q1 < q1_hi; ++q1) {
         int k10, k11, k12;
         grid_coords(q1, A.m0, A.m1, k10, k11, k12);
@@ -161,7 +164,7 @@ __global__ __launch_bounds__(BLK) void k_lw3_generic(Lw3Args A) {
                 const double iw = A.inv_omega[(size_t)i0 * ND + s0] * iw12;
                 const double W = (double)cabs2(V[l]) * iw * A.factor;
                 if (W_OUT) {
-                    A.w_out[(((size_t)q1 * ND + s0) * ND + js1) * ND + js2] = W;
+                    A.w_out[(((size_t)(pair_mode ? ch : q1) * ND + s0) * ND + js1) * ND + js2] = W;
                 } else {
                     const double g = gauss_g(A.omega[(size_t)i0 * ND + s0], w1, w2, n1, n2, c0, hinv);
                     if (sizeof(R) == 8)
@@ -240,7 +243,9 @@ static int launch_nd(Ctx &c, const int *d_q0, int nq0, double sigma, double *d_l
     }
     A.w_out = d_w;
     const size_t dyn = (size_t)2 * ((A.nsite + 1) & ~1) * sizeof(int) + (size_t)c.L * sizeof(cplx_t<R>);
-    dim3 grid(A.nchunks, NS0B, nq0);
+    A.pair_q0 = c.w_pair_q0;
+    A.pair_q1 = c.w_pair_q1;
+    dim3 grid(A.pair_q0 ? c.w_npairs : A.nchunks, NS0B, A.pair_q0 ? 1 : nq0);
     if (d_w) {
         A.partial = nullptr;
         hipLaunchKernelGGL((k_lw3_generic<R, ND, S0B, BLK, true>), grid, dim3(BLK), dyn, c.stream, A);

@@ -100,7 +100,7 @@ __device__ __forceinline__ void mfma_item(const MfmaArgs &A, const int item) {
     constexpr int STASH_ROWS = 3 * STASH_ATOMS;
     constexpr int KP_D = BF ? ((2 * STASH_ROWS + 31) / 32) * 32 : ((2 * STASH_ROWS + 3) / 4) * 4;
     constexpr int NWAVE = 8;
-    constexpr int DT_PER_WAVE = (MT_D * NT + NWAVE - 1) / NWAVE;
+    constexpr int DT_PER_WAVE = (MT_D + 1) / 2;   // wave w: n tile w % 4, m tiles w / 4 + 2 t
     static_assert(NT <= 4, "Ndof must be <= 32");
 
     extern __shared__ __align__(16) unsigned char smem[];
@@ -335,6 +335,7 @@ static int launch_mfma_t(Ctx &c, MfmaArgs A, int nitems, int q0_first, int q0_co
     int per_cu = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 512, dyn) != hipSuccess || per_cu < 1) per_cu = 1;
     ItemLaunch L{fn, &A, sizeof(A), 512, dyn, nitems, per_cu, std::string(BF ? "mfma_bf16" : "mfma_f32")};
+    L.direct = c.override_items != nullptr;
     return run_items(c, L, q0_first, q0_count, it0, d_lw);
 }
 
@@ -356,14 +357,15 @@ int launch_lw3_mfma(Ctx &c, int q0_first, int q0_count, double sigma, int prec, 
         SPR_HIP(hipMemcpy(c.omega32 + nm, iw.data(), nm * sizeof(float), hipMemcpyHostToDevice));
         SPR_HIP(hipMemcpy(c.omega32 + 2 * nm, nb.data(), nm * sizeof(float), hipMemcpyHostToDevice));
     }
-    const int it0 = c.h_item_first[q0_first], it1 = c.h_item_first[q0_first + q0_count];
-    const int nitems = it1 - it0;
+    const ItemSet *ov = c.override_items;
+    const int it0 = ov ? 0 : c.h_item_first[q0_first];
+    const int nitems = ov ? ov->nitems : c.h_item_first[q0_first + q0_count] - it0;
     const size_t nm = (size_t)c.nq * c.nd;
     MfmaArgs A;
-    A.item_q0 = c.d_item_q0 + it0;
-    A.item_off = c.d_item_off + it0;
-    A.item_cnt = c.d_item_cnt + it0;
-    A.pairs = c.d_pairs;
+    A.item_q0 = ov ? ov->item_q0 : c.d_item_q0 + it0;
+    A.item_off = ov ? ov->item_off : c.d_item_off + it0;
+    A.item_cnt = ov ? ov->item_cnt : c.d_item_cnt + it0;
+    A.pairs = ov ? ov->pairs : c.d_pairs;
     A.q0_grid = c.irr_grid;
     A.L = c.L;
     A.m0 = c.mesh[0];
@@ -382,7 +384,7 @@ int launch_lw3_mfma(Ctx &c, int q0_first, int q0_count, double sigma, int prec, 
     const double pi = 4.0 * std::atan(1.0);
     A.factor = (float)(1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) *
                        1000000.0 * pi * 0.0625 / (double)c.nq);
-    A.ic.partial = c.ipartial;
+    A.ic.partial = ov ? ov->partial : c.ipartial;
     const bool bf = prec == SPR_PREC_BF16;
     switch (c.nb) {
 #define SPR_NB(n)                                                                                    \

@@ -244,6 +244,7 @@ static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first,
         per_cu = 1;
     ItemLaunch L{fn, &A, sizeof(A), ND * 64, dyn, nitems, per_cu,
                  std::string("small") + (sizeof(R) == 8 ? "64" : "32")};
+    L.direct = c.override_items != nullptr;
     return run_items(c, L, q0_first, q0_count, it0, d_lw);
 }
 
@@ -313,13 +314,14 @@ int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, double *d_l
 int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw) {
     int rc;
     if ((rc = ensure_pairs(c))) return rc;
-    const int it0 = c.h_item_first[q0_first], it1 = c.h_item_first[q0_first + q0_count];
-    const int nitems = it1 - it0;
+    const ItemSet *ov = c.override_items;
+    const int it0 = ov ? 0 : c.h_item_first[q0_first];
+    const int nitems = ov ? ov->nitems : c.h_item_first[q0_first + q0_count] - it0;
     SmallArgs A;
-    A.item_q0 = c.d_item_q0 + it0;
-    A.item_off = c.d_item_off + it0;
-    A.item_cnt = c.d_item_cnt + it0;
-    A.pairs = c.d_pairs;
+    A.item_q0 = ov ? ov->item_q0 : c.d_item_q0 + it0;
+    A.item_off = ov ? ov->item_off : c.d_item_off + it0;
+    A.item_cnt = ov ? ov->item_cnt : c.d_item_cnt + it0;
+    A.pairs = ov ? ov->pairs : c.d_pairs;
     A.q0_grid = c.irr_grid;
     A.L = c.L;
     A.m0 = c.mesh[0];
@@ -335,7 +337,7 @@ int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec,
     const double pi = 4.0 * std::atan(1.0);
     A.factor = 1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) *
                1000000.0 * pi * 0.0625 / (double)c.nq;   // Sctr3THz2 / Nq, constants.f90:59-60
-    A.ic.partial = c.ipartial;
+    A.ic.partial = ov ? ov->partial : c.ipartial;
     const bool f64 = prec == SPR_PREC_FP64;
     if (c.nb == 2)
         return f64 ? launch_small_t<double, 2>(c, A, nitems, q0_first, q0_count, it0, d_lw)

@@ -1,0 +1,439 @@
+// Learned importance-sampling proposal: a 3-layer MLP (F -> 128 -> 128 -> 1, tanh) that
+// maps (q0, q1, w0, w1, w2, branch ids) to log W, with hand-written training (FP32, Adam)
+// and inference (FP32 VALU, or BF16 operands on v_mfma_f32_16x16x32_bf16 for the 128x128
+// layer with FP32 accumulation).  No counterpart in the reference; it serves the sampled
+// estimate of the linewidth sum whose deterministic form is CalculateLinewidth
+// (src/Kappa/Src/ThirdOrder_m.f90:442-543).
+//
+// Parameter vector (FP32): W1[F][H] b1[H] W2[H][H] b2[H] W3[H] b3.
+// Training step: forward + backward per 32-row tile (per-tile gradient partials), then a
+// fixed-order reduction over tiles fused with the Adam update, so training is deterministic.
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "spr_internal.h"
+
+namespace spr {
+
+constexpr int MH = 128;      // hidden width
+constexpr int MROWS = 32;    // rows per training tile
+constexpr int MFMAX = 32;    // max input features
+
+typedef float v4f_m __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_m __attribute__((ext_vector_type(8)));
+
+struct MlpParamsView {
+    const float *W1, *b1, *W2, *b2, *W3, *b3;
+};
+
+__host__ __device__ inline size_t mlp_nparam(int F) { return (size_t)F * MH + MH + MH * MH + MH + MH + 1; }
+
+__device__ inline MlpParamsView mlp_view(const float *p, int F) {
+    MlpParamsView v;
+    v.W1 = p;
+    v.b1 = v.W1 + F * MH;
+    v.W2 = v.b1 + MH;
+    v.b2 = v.W2 + MH * MH;
+    v.W3 = v.b2 + MH;
+    v.b3 = v.W3 + MH;
+    return v;
+}
+
+// Forward (and, with TRAIN, backward) over one tile of MROWS rows.  256 threads: row
+// r = tid / 8, hidden slice [16 (tid % 8), +16).
+template <bool TRAIN>
+__global__ __launch_bounds__(256) void k_mlp_tile(const float *param, int F, const float *X, const float *y,
+                                                  int n, float *pred, float *grad_part, double *loss_part) {
+    extern __shared__ __align__(16) float sm[];
+    float *s_W2 = sm;                        // [MH][MH]
+    float *s_h1 = s_W2 + MH * MH;            // [MROWS][MH]
+    float *s_h2 = s_h1 + MROWS * MH;         // [MROWS][MH]
+    float *s_dz2 = s_h2 + MROWS * MH;        // [MROWS][MH]
+    float *s_x = s_dz2 + MROWS * MH;         // [MROWS][MFMAX]
+    float *s_dy = s_x + MROWS * MFMAX;       // [MROWS]
+    const int tid = threadIdx.x;
+    const int r = tid >> 3, js = (tid & 7) * 16;
+    const int row = blockIdx.x * MROWS + r;
+    const bool valid = row < n;
+    const MlpParamsView P = mlp_view(param, F);
+    for (int i = tid; i < MH * MH; i += 256) s_W2[i] = P.W2[i];
+    for (int i = tid; i < MROWS * F; i += 256) {
+        const int rr = i / F, ff = i % F;
+        const int gr = blockIdx.x * MROWS + rr;
+        s_x[rr * MFMAX + ff] = gr < n ? X[(size_t)gr * F + ff] : 0.f;
+    }
+    __syncthreads();
+    // layer 1
+    for (int jj = 0; jj < 16; ++jj) {
+        const int j = js + jj;
+        float z = P.b1[j];
+        for (int f = 0; f < F; ++f) z = fmaf(s_x[r * MFMAX + f], P.W1[f * MH + j], z);
+        s_h1[r * MH + j] = tanhf(z);
+    }
+    __syncthreads();
+    // layer 2
+    float h2[16];
+    float ypart = 0.f;
+    for (int jj = 0; jj < 16; ++jj) {
+        const int j = js + jj;
+        float z = P.b2[j];
+        for (int k = 0; k < MH; ++k) z = fmaf(s_h1[r * MH + k], s_W2[k * MH + j], z);
+        h2[jj] = tanhf(z);
+        s_h2[r * MH + j] = h2[jj];
+        ypart = fmaf(h2[jj], P.W3[j], ypart);
+    }
+    // sum over the 8 threads of the row (consecutive lanes of one wave)
+    ypart += __shfl_xor(ypart, 1, 64);
+    ypart += __shfl_xor(ypart, 2, 64);
+    ypart += __shfl_xor(ypart, 4, 64);
+    const float yp = ypart + P.b3[0];
+    if (!TRAIN) {
+        if (valid && (tid & 7) == 0) pred[row] = yp;
+        return;
+    }
+    const float err = valid ? yp - y[row] : 0.f;
+    const float dy = 2.f * err / (float)n;
+    if ((tid & 7) == 0) s_dy[r] = dy;
+    for (int jj = 0; jj < 16; ++jj) {
+        const int j = js + jj;
+        s_dz2[r * MH + j] = dy * P.W3[j] * (1.f - h2[jj] * h2[jj]);
+    }
+    __syncthreads();
+    float *g = grad_part + (size_t)blockIdx.x * mlp_nparam(F);
+    float *gW1 = g, *gb1 = gW1 + F * MH, *gW2 = gb1 + MH, *gb2 = gW2 + MH * MH, *gW3 = gb2 + MH, *gb3 = gW3 + MH;
+    // dW2[k][j] = sum_r h1[r][k] dz2[r][j]: thread owns an 8 x 8 block
+    {
+        const int k0 = (tid >> 4) * 8, j0 = (tid & 15) * 8;
+        float acc[8][8];
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b) acc[a][b] = 0.f;
+        for (int rr = 0; rr < MROWS; ++rr)
+            for (int a = 0; a < 8; ++a) {
+                const float h = s_h1[rr * MH + k0 + a];
+                for (int b = 0; b < 8; ++b) acc[a][b] = fmaf(h, s_dz2[rr * MH + j0 + b], acc[a][b]);
+            }
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b) gW2[(k0 + a) * MH + j0 + b] = acc[a][b];
+    }
+    // db2, dW3, db3
+    if (tid < MH) {
+        float s2 = 0.f, s3 = 0.f;
+        for (int rr = 0; rr < MROWS; ++rr) {
+            s2 += s_dz2[rr * MH + tid];
+            s3 = fmaf(s_h2[rr * MH + tid], s_dy[rr], s3);
+        }
+        gb2[tid] = s2;
+        gW3[tid] = s3;
+    }
+    if (tid == 0) {
+        float s = 0.f;
+        for (int rr = 0; rr < MROWS; ++rr) s += s_dy[rr];
+        gb3[0] = s;
+    }
+    // dz1[r][k] = (sum_j dz2[r][j] W2[k][j]) (1 - h1^2); reuse s_h2 for dz1
+    __syncthreads();
+    for (int kk = 0; kk < 16; ++kk) {
+        const int k = js + kk;
+        float d = 0.f;
+        for (int j = 0; j < MH; ++j) d = fmaf(s_dz2[r * MH + j], s_W2[k * MH + j], d);
+        const float h = s_h1[r * MH + k];
+        s_h2[r * MH + k] = d * (1.f - h * h);
+    }
+    __syncthreads();
+    for (int e = tid; e < F * MH; e += 256) {
+        const int f = e / MH, k = e % MH;
+        float s = 0.f;
+        for (int rr = 0; rr < MROWS; ++rr) s = fmaf(s_x[rr * MFMAX + f], s_h2[rr * MH + k], s);
+        gW1[e] = s;
+    }
+    if (tid < MH) {
+        float s = 0.f;
+        for (int rr = 0; rr < MROWS; ++rr) s += s_h2[rr * MH + tid];
+        gb1[tid] = s;
+    }
+    if (tid == 0) {
+        double l = 0.0;
+        for (int rr = 0; rr < MROWS; ++rr) {
+            const int gr = blockIdx.x * MROWS + rr;
+            if (gr < n) {
+                const double e = (double)s_dy[rr] * (double)n * 0.5;
+                l += e * e;
+            }
+        }
+        loss_part[blockIdx.x] = l;
+    }
+}
+
+// g = sum over tiles (fixed order); optional Adam step on the parameters.
+__global__ void k_mlp_reduce_adam(const float *grad_part, int ntiles, size_t np, float *grad, float *param,
+                                  float *m, float *v, int step, float lr, int do_step) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float g = 0.f;
+    for (int t = 0; t < ntiles; ++t) g += grad_part[(size_t)t * np + i];
+    grad[i] = g;
+    if (!do_step) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float mi = b1 * m[i] + (1.f - b1) * g;
+    const float vi = b2 * v[i] + (1.f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / (1.f - powf(b1, (float)step));
+    const float vh = vi / (1.f - powf(b2, (float)step));
+    param[i] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// BF16 inference: 64 rows per workgroup (4 waves).  Layer 1 and the output layer on VALU
+// (FP32, inputs rounded to BF16), layer 2 as a 64 x 128 x 128 GEMM on BF16 MFMA.
+__global__ __launch_bounds__(256) void k_mlp_infer_bf16(const float *param, int F, const float *X, int n,
+                                                        float *pred) {
+    constexpr int R = 64;
+    extern __shared__ __align__(16) unsigned char smb[];
+    __bf16 *s_W2 = reinterpret_cast<__bf16 *>(smb);          // [MH][MH] (k, j)
+    __bf16 *s_h1 = s_W2 + MH * MH;                            // [R][MH]
+    float *s_y = reinterpret_cast<float *>(s_h1 + R * MH);    // [R][8] partials
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+    const MlpParamsView P = mlp_view(param, F);
+    for (int i = tid; i < MH * MH; i += 256) s_W2[i] = (__bf16)P.W2[i];
+    // layer 1: thread -> (row tid/4, 32 hidden units)
+    {
+        const int r = tid >> 2, j0 = (tid & 3) * 32;
+        const int row = blockIdx.x * R + r;
+        float x[MFMAX];
+        for (int f = 0; f < F; ++f) x[f] = row < n ? (float)(__bf16)X[(size_t)row * F + f] : 0.f;
+        for (int jj = 0; jj < 32; ++jj) {
+            const int j = j0 + jj;
+            float z = (float)(__bf16)P.b1[j];
+            for (int f = 0; f < F; ++f) z = fmaf(x[f], (float)(__bf16)P.W1[f * MH + j], z);
+            s_h1[r * MH + j] = (__bf16)tanhf(z);
+        }
+    }
+    __syncthreads();
+    // layer 2 on MFMA: wave w takes row tile w (16 rows) x all 8 column tiles
+    float yacc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int nt = 0; nt < MH / 16; ++nt) {
+        v4f_m acc = {0.f, 0.f, 0.f, 0.f};
+        for (int kk = 0; kk < MH; kk += 32) {
+            bf16x8_m a, b;
+            for (int t = 0; t < 8; ++t) {
+                a[t] = s_h1[(w * 16 + (l & 15)) * MH + kk + 8 * (l >> 4) + t];
+                b[t] = s_W2[(kk + 8 * (l >> 4) + t) * MH + nt * 16 + (l & 15)];
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        }
+        const int j = nt * 16 + (l & 15);
+        const float b2 = (float)(__bf16)P.b2[j], w3 = (float)(__bf16)P.W3[j];
+        for (int q = 0; q < 4; ++q) yacc[q] = fmaf((float)(__bf16)tanhf(acc[q] + b2), w3, yacc[q]);
+    }
+    // reduce over the 16 columns held by lanes with the same row group
+    for (int q = 0; q < 4; ++q) {
+        float v = yacc[q];
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 8, 64);
+        yacc[q] = v;
+    }
+    if ((l & 15) == 0)
+        for (int q = 0; q < 4; ++q) {
+            const int r = w * 16 + 4 * (l >> 4) + q;
+            const int row = blockIdx.x * R + r;
+            if (row < n) pred[row] = yacc[q] + P.b3[0];
+        }
+}
+
+}  // namespace spr
+
+using namespace spr;
+
+struct spr_mlp {
+    int F = 0;
+    size_t np = 0;
+    float *param = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *gpart = nullptr;
+    float *X = nullptr, *y = nullptr, *pred = nullptr;
+    double *lpart = nullptr;
+    size_t cap_rows = 0, cap_tiles = 0;
+    int step = 0;
+    hipStream_t stream = nullptr;
+};
+
+static int mlp_reserve(spr_mlp *h, size_t rows) {
+    if (rows <= h->cap_rows) return SPR_OK;
+    (void)hipFree(h->X);
+    (void)hipFree(h->y);
+    (void)hipFree(h->pred);
+    (void)hipFree(h->gpart);
+    (void)hipFree(h->lpart);
+    const size_t tiles = (rows + MROWS - 1) / MROWS;
+    if (hipMalloc((void **)&h->X, rows * h->F * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&h->y, rows * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&h->pred, rows * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&h->gpart, tiles * h->np * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&h->lpart, tiles * sizeof(double)) != hipSuccess) {
+        set_error("spr_mlp: hipMalloc failed");
+        return SPR_E_OOM;
+    }
+    h->cap_rows = rows;
+    h->cap_tiles = tiles;
+    return SPR_OK;
+}
+
+extern "C" int spr_mlp_create(int n_features, unsigned long long seed, int device, spr_mlp **out) {
+    if (!out || n_features < 1 || n_features > MFMAX) {
+        set_error("spr_mlp_create: n_features must be 1..32");
+        return SPR_E_ARG;
+    }
+    SPR_HIP(hipSetDevice(device));
+    auto *h = new spr_mlp();
+    h->F = n_features;
+    h->np = mlp_nparam(n_features);
+    std::vector<float> p(h->np, 0.f);
+    std::mt19937_64 g(seed);
+    auto init = [&](size_t off, int fan_in, size_t count) {
+        std::normal_distribution<float> d(0.f, 1.f / std::sqrt((float)fan_in));
+        for (size_t i = 0; i < count; ++i) p[off + i] = d(g);
+    };
+    const int F = n_features;
+    init(0, F, (size_t)F * MH);                                   // W1
+    init((size_t)F * MH + MH, MH, (size_t)MH * MH);               // W2
+    init((size_t)F * MH + MH + MH * MH + MH, MH, MH);             // W3
+    if (hipMalloc((void **)&h->param, h->np * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&h->m, h->np * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&h->v, h->np * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&h->grad, h->np * sizeof(float)) != hipSuccess) {
+        delete h;
+        set_error("spr_mlp_create: hipMalloc failed");
+        return SPR_E_OOM;
+    }
+    SPR_HIP(hipMemcpy(h->param, p.data(), h->np * sizeof(float), hipMemcpyHostToDevice));
+    SPR_HIP(hipMemset(h->m, 0, h->np * sizeof(float)));
+    SPR_HIP(hipMemset(h->v, 0, h->np * sizeof(float)));
+    SPR_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    *out = h;
+    return SPR_OK;
+}
+
+extern "C" int spr_mlp_destroy(spr_mlp *h) {
+    if (!h) return SPR_OK;
+    for (void *p : {(void *)h->param, (void *)h->m, (void *)h->v, (void *)h->grad, (void *)h->gpart,
+                    (void *)h->X, (void *)h->y, (void *)h->pred, (void *)h->lpart})
+        if (p) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SPR_OK;
+}
+
+extern "C" int spr_mlp_num_params(spr_mlp *h, long *np) {
+    if (!h || !np) return SPR_E_ARG;
+    *np = (long)h->np;
+    return SPR_OK;
+}
+
+extern "C" int spr_mlp_get_params(spr_mlp *h, float *out) {
+    if (!h || !out) return SPR_E_ARG;
+    SPR_HIP(hipMemcpy(out, h->param, h->np * sizeof(float), hipMemcpyDeviceToHost));
+    return SPR_OK;
+}
+
+extern "C" int spr_mlp_set_params(spr_mlp *h, const float *in) {
+    if (!h || !in) return SPR_E_ARG;
+    SPR_HIP(hipMemcpy(h->param, in, h->np * sizeof(float), hipMemcpyHostToDevice));
+    return SPR_OK;
+}
+
+static const size_t kTileLds = (size_t)(MH * MH + 3 * MROWS * MH + MROWS * MFMAX + MROWS) * sizeof(float);
+
+// Mean-squared-error gradient of the loss over n rows (no update): grad(np), loss.
+extern "C" int spr_mlp_grad(spr_mlp *h, const float *X, const float *y, long n, float *grad, double *loss) {
+    if (!h || !X || !y || n < 1 || !grad) {
+        set_error("spr_mlp_grad: bad argument");
+        return SPR_E_ARG;
+    }
+    int rc;
+    if ((rc = mlp_reserve(h, n))) return rc;
+    SPR_HIP(hipMemcpy(h->X, X, n * h->F * sizeof(float), hipMemcpyHostToDevice));
+    SPR_HIP(hipMemcpy(h->y, y, n * sizeof(float), hipMemcpyHostToDevice));
+    const int tiles = (int)((n + MROWS - 1) / MROWS);
+    SPR_HIP(hipFuncSetAttribute((const void *)k_mlp_tile<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTileLds));
+    hipLaunchKernelGGL(k_mlp_tile<true>, dim3(tiles), dim3(256), kTileLds, h->stream, h->param, h->F, h->X, h->y,
+                       (int)n, h->pred, h->gpart, h->lpart);
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((h->np + 255) / 256), dim3(256), 0, h->stream, h->gpart, tiles,
+                       h->np, h->grad, h->param, h->m, h->v, 1, 0.f, 0);
+    SPR_HIP(hipGetLastError());
+    SPR_HIP(hipMemcpyAsync(grad, h->grad, h->np * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    std::vector<double> lp(tiles);
+    SPR_HIP(hipMemcpyAsync(lp.data(), h->lpart, tiles * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SPR_HIP(hipStreamSynchronize(h->stream));
+    if (loss) {
+        double s = 0.0;
+        for (double v : lp) s += v;
+        *loss = s / (double)n;
+    }
+    return SPR_OK;
+}
+
+// Mini-batch Adam over the data set for `epochs` passes (batches in order, data shuffled
+// by the caller); loss_per_epoch[epochs] gets the mean training loss of each epoch.
+extern "C" int spr_mlp_train(spr_mlp *h, const float *X, const float *y, long n, int epochs, int batch, float lr,
+                             double *loss_per_epoch) {
+    if (!h || !X || !y || n < 1 || epochs < 1 || batch < MROWS) {
+        set_error("spr_mlp_train: bad argument (batch must be >= 32)");
+        return SPR_E_ARG;
+    }
+    int rc;
+    if ((rc = mlp_reserve(h, n))) return rc;
+    SPR_HIP(hipMemcpy(h->X, X, n * h->F * sizeof(float), hipMemcpyHostToDevice));
+    SPR_HIP(hipMemcpy(h->y, y, n * sizeof(float), hipMemcpyHostToDevice));
+    SPR_HIP(hipFuncSetAttribute((const void *)k_mlp_tile<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTileLds));
+    std::vector<double> lp;
+    for (int ep = 0; ep < epochs; ++ep) {
+        double lsum = 0.0;
+        for (long b0 = 0; b0 < n; b0 += batch) {
+            const long nb = std::min<long>(batch, n - b0);
+            const int tiles = (int)((nb + MROWS - 1) / MROWS);
+            h->step += 1;
+            hipLaunchKernelGGL(k_mlp_tile<true>, dim3(tiles), dim3(256), kTileLds, h->stream, h->param, h->F,
+                               h->X + b0 * h->F, h->y + b0, (int)nb, h->pred, h->gpart, h->lpart);
+            hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((h->np + 255) / 256), dim3(256), 0, h->stream, h->gpart,
+                               tiles, h->np, h->grad, h->param, h->m, h->v, h->step, lr, 1);
+            lp.resize(tiles);
+            SPR_HIP(hipMemcpyAsync(lp.data(), h->lpart, tiles * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+            SPR_HIP(hipStreamSynchronize(h->stream));
+            for (double v : lp) lsum += v;
+        }
+        SPR_HIP(hipGetLastError());
+        if (loss_per_epoch) loss_per_epoch[ep] = lsum / (double)n;
+    }
+    return SPR_OK;
+}
+
+// Predictions for n rows; prec SPR_PREC_FP32 (VALU) or SPR_PREC_BF16 (MFMA layer 2).
+extern "C" int spr_mlp_predict(spr_mlp *h, const float *X, long n, int prec, float *out) {
+    if (!h || !X || n < 1 || !out || (prec != SPR_PREC_FP32 && prec != SPR_PREC_BF16)) {
+        set_error("spr_mlp_predict: bad argument (prec must be FP32 or BF16)");
+        return SPR_E_ARG;
+    }
+    int rc;
+    if ((rc = mlp_reserve(h, n))) return rc;
+    SPR_HIP(hipMemcpy(h->X, X, n * h->F * sizeof(float), hipMemcpyHostToDevice));
+    if (prec == SPR_PREC_FP32) {
+        const int tiles = (int)((n + MROWS - 1) / MROWS);
+        SPR_HIP(hipFuncSetAttribute((const void *)k_mlp_tile<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)kTileLds));
+        hipLaunchKernelGGL(k_mlp_tile<false>, dim3(tiles), dim3(256), kTileLds, h->stream, h->param, h->F, h->X,
+                           nullptr, (int)n, h->pred, nullptr, nullptr);
+    } else {
+        const size_t lds = (size_t)MH * MH * 2 + 64 * MH * 2 + 64 * 8 * sizeof(float);
+        SPR_HIP(hipFuncSetAttribute((const void *)k_mlp_infer_bf16, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+        hipLaunchKernelGGL(k_mlp_infer_bf16, dim3((n + 63) / 64), dim3(256), lds, h->stream, h->param, h->F, h->X,
+                           (int)n, h->pred);
+    }
+    SPR_HIP(hipGetLastError());
+    SPR_HIP(hipMemcpyAsync(out, h->pred, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    SPR_HIP(hipStreamSynchronize(h->stream));
+    return SPR_OK;
+}

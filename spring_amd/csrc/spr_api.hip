@@ -419,3 +419,61 @@ extern "C" int spr_kappa_rta(spr_ctx *h, const double *lw_all, double T, int sym
     if (mode_k) std::memcpy(mode_k, modes.data(), nm * sizeof(double));
     return SPR_OK;
 }
+
+namespace spr {
+int launch_lw3_pairs(Ctx &c, int n, const int *q0_index, const int *q1_grid, double sigma, int prec, double *h_out);
+}
+
+extern "C" int spr_lw3_pairs(spr_ctx *h, int n, const int *q0_index, const int *q1_grid, double sigma, int prec,
+                             double *out) {
+    if (!h || n < 0 || (n > 0 && (!q0_index || !q1_grid || !out))) {
+        set_error("spr_lw3_pairs: bad argument");
+        return SPR_E_ARG;
+    }
+    int rc;
+    if ((rc = check_prec(prec))) return rc;
+    SPR_HIP(hipSetDevice(h->c.device));
+    return launch_lw3_pairs(h->c, n, q0_index, q1_grid, sigma, prec, out);
+}
+
+extern "C" int spr_v3_w_pairs(spr_ctx *h, int n, const int *q0_index, const int *q1_grid, double *W_out) {
+    if (!h || n < 1 || !q0_index || !q1_grid || !W_out) {
+        set_error("spr_v3_w_pairs: bad argument");
+        return SPR_E_ARG;
+    }
+    Ctx &c = h->c;
+    for (int k = 0; k < n; ++k)
+        if (q0_index[k] < 0 || q0_index[k] >= c.nq0 || q1_grid[k] < 0 || q1_grid[k] >= c.nq) {
+            set_error("spr_v3_w_pairs: index out of range");
+            return SPR_E_ARG;
+        }
+    SPR_HIP(hipSetDevice(c.device));
+    const size_t nw = (size_t)c.nd * c.nd * c.nd * n;
+    int rc;
+    if ((rc = ensure((void **)&c.wout, &c.wout_cap, nw * sizeof(double)))) return rc;
+    int *d_idx = nullptr;
+    if (hipMalloc((void **)&d_idx, 2 * (size_t)n * sizeof(int)) != hipSuccess) {
+        set_error("spr_v3_w_pairs: hipMalloc failed");
+        return SPR_E_OOM;
+    }
+    rc = SPR_OK;
+    if (hipMemcpy(d_idx, q0_index, n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_idx + n, q1_grid, n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("spr_v3_w_pairs: upload failed");
+        rc = SPR_E_HIP;
+    }
+    if (!rc) {
+        c.w_pair_q0 = d_idx;
+        c.w_pair_q1 = d_idx + n;
+        c.w_npairs = n;
+        rc = launch_lw3(c, 0, 1, 1.0, SPR_PREC_FP64, nullptr, c.wout);
+        c.w_pair_q0 = c.w_pair_q1 = nullptr;
+        c.w_npairs = 0;
+    }
+    if (!rc && hipMemcpy(W_out, c.wout, nw * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("spr_v3_w_pairs: download failed");
+        rc = SPR_E_HIP;
+    }
+    (void)hipFree(d_idx);
+    return rc;
+}

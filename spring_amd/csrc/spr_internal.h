@@ -39,6 +39,15 @@ struct ItemLaunch {
     int nitems;
     int per_cu;             // resident workgroups per compute unit
     std::string name;
+    bool direct = false;    // one workgroup per item, no graph, no tuning, no reduction
+};
+
+// Caller-supplied work items (spr_lw3_pairs): partial[item] receives the item's sum.
+struct ItemSet {
+    const int *item_q0, *item_off, *item_cnt;
+    const int2 *pairs;
+    int nitems;
+    double *partial;
 };
 
 // Anharmonic IFC re-laid out for the phase sum.  The sum over (N1, N2[, N3]) is grouped by
@@ -96,6 +105,9 @@ struct Ctx {
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
     double *ipartial = nullptr;    // (n_items, nd) item partials (stable: graphs keep the pointer)
     double *wout = nullptr;        // W parity output
+    const ItemSet *override_items = nullptr;
+    const int *w_pair_q0 = nullptr, *w_pair_q1 = nullptr;   // W parity path over a pair list
+    int w_npairs = 0;
     size_t wout_cap = 0;
     // timing of the dominant kernel
     int launches = 0;

@@ -18,7 +18,9 @@ PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
 SYMBOLS = [
     "spr_ctx_create", "spr_ctx_destroy", "spr_last_error", "spr_v3_w", "spr_lw3",
     "spr_lw3_all", "spr_lw4", "spr_lw4_all", "spr_kappa_rta", "spr_kernel_time",
-    "spr_kernel_time_reset",
+    "spr_kernel_time_reset", "spr_lw3_pairs", "spr_v3_w_pairs", "spr_mlp_create", "spr_mlp_destroy",
+    "spr_mlp_num_params", "spr_mlp_get_params", "spr_mlp_set_params", "spr_mlp_grad",
+    "spr_mlp_train", "spr_mlp_predict",
 ]
 
 
@@ -140,6 +142,15 @@ class GpuContext:
         self._check(self.lib.spr_v3_w(self.h, ct.c_int(i0), _ptr(W)), "spr_v3_w")
         return W
 
+    def v3_w_pairs(self, q0_index, q1_grid) -> np.ndarray:
+        """W for (irreducible q0, q1) pairs as numpy [pair][s0][s1][s2]."""
+        q0 = np.ascontiguousarray(q0_index, np.int32)
+        q1 = np.ascontiguousarray(q1_grid, np.int32)
+        W = np.zeros((q0.size, self.nd, self.nd, self.nd))
+        self._check(self.lib.spr_v3_w_pairs(self.h, ct.c_int(q0.size), _ptr(q0), _ptr(q1), _ptr(W)),
+                    "spr_v3_w_pairs")
+        return W
+
     def lw3(self, i0: int, sigma: float, prec: str = "fp64") -> np.ndarray:
         lw = np.zeros(self.nd)
         self._check(self.lib.spr_lw3(self.h, ct.c_int(i0), ct.c_double(sigma),
@@ -172,6 +183,15 @@ class GpuContext:
                                            _ptr(k), _ptr(modes)), "spr_kappa_rta")
         return k.reshape(3, 3), modes
 
+    def lw3_pairs(self, q0_index, q1_grid, sigma: float, prec: str = "fp32") -> np.ndarray:
+        """Per-pair linewidth terms (n, Ndof) for the given (irreducible q0, q1) pairs."""
+        q0 = np.ascontiguousarray(q0_index, np.int32)
+        q1 = np.ascontiguousarray(q1_grid, np.int32)
+        out = np.zeros((q0.size, self.nd))
+        self._check(self.lib.spr_lw3_pairs(self.h, ct.c_int(q0.size), _ptr(q0), _ptr(q1), ct.c_double(sigma),
+                                           ct.c_int(PREC[prec]), _ptr(out)), "spr_lw3_pairs")
+        return out
+
     def kernel_time(self):
         n = ct.c_int()
         ms = ct.c_double()
@@ -180,3 +200,67 @@ class GpuContext:
 
     def kernel_time_reset(self):
         self._check(self.lib.spr_kernel_time_reset(self.h), "spr_kernel_time_reset")
+
+
+class Mlp:
+    """Device MLP (spr_mlp_*): F -> 128 -> 128 -> 1, tanh."""
+
+    def __init__(self, n_features: int, seed: int = 0, device: int = 0):
+        self.lib = load()
+        self.F = n_features
+        h = ct.c_void_p()
+        self._check(self.lib.spr_mlp_create(ct.c_int(n_features), ct.c_ulonglong(seed), ct.c_int(device),
+                                            ct.byref(h)), "spr_mlp_create")
+        self.h = h
+        n = ct.c_long()
+        self._check(self.lib.spr_mlp_num_params(self.h, ct.byref(n)), "spr_mlp_num_params")
+        self.n_params = n.value
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise SprError(f"{what} failed ({rc}): {self.lib.spr_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.spr_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        self._check(self.lib.spr_mlp_get_params(self.h, _ptr(p)), "spr_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        self._check(self.lib.spr_mlp_set_params(self.h, _ptr(p)), "spr_mlp_set_params")
+
+    def grad(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = ct.c_double()
+        self._check(self.lib.spr_mlp_grad(self.h, _ptr(X), _ptr(y), ct.c_long(y.size), _ptr(g), ct.byref(loss)),
+                    "spr_mlp_grad")
+        return g, loss.value
+
+    def train(self, X, y, epochs: int, batch: int = 4096, lr: float = 1e-3):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        losses = np.zeros(epochs)
+        self._check(self.lib.spr_mlp_train(self.h, _ptr(X), _ptr(y), ct.c_long(y.size), ct.c_int(epochs),
+                                           ct.c_int(batch), ct.c_float(lr), _ptr(losses)), "spr_mlp_train")
+        return losses
+
+    def predict(self, X, prec: str = "fp32") -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(X.shape[0], np.float32)
+        self._check(self.lib.spr_mlp_predict(self.h, _ptr(X), ct.c_long(X.shape[0]), ct.c_int(PREC[prec]),
+                                             _ptr(out)), "spr_mlp_predict")
+        return out

@@ -516,3 +516,98 @@ void orc_kappa_rta_symm(int nd, int nq, int nq0, const int *irr_q_int, const int
     }
     for (int i = 0; i < 9; ++i) kappa[i] = k[i] * f2;
 }
+
+/* --------------------------------------------------------------------------------------
+ * Linear tetrahedron method for one tetrahedron (FindgAndIver and FindIverxFver,
+ * TetrahedronIntegration_m.f90:93-295): vertex values M(4) are sorted ascending (stable for
+ * ties, the order my_sort's first-uncovered search produces), then the three cases give
+ * g (density at Mqs) and the vertex weights I(4); returns g * sum(I F) / 6 in *gIF and g.
+ * ------------------------------------------------------------------------------------*/
+int orc_tetra_weights(double Mqs, const double *M_in, const double *F_in, double *g_out, double *I_out,
+                      double *gIF_out)
+{
+    double M[4], F[4];
+    int idx[4] = {0, 1, 2, 3};
+    for (int a = 1; a < 4; ++a)   /* stable insertion sort of the indices by value */
+        for (int b = a; b > 0 && M_in[idx[b - 1]] > M_in[idx[b]]; --b) {
+            int t = idx[b]; idx[b] = idx[b - 1]; idx[b - 1] = t;
+        }
+    for (int v = 0; v < 4; ++v) { M[v] = M_in[idx[v]]; F[v] = F_in[idx[v]]; }
+    double g = 0.0, Iv[4] = {0, 0, 0, 0};
+    int enter = 0;
+    if (Mqs >= M[0] && Mqs < M[1]) {
+        g = 3.0 * (Mqs - M[0]) * (Mqs - M[0]) / ((M[1] - M[0]) * (M[2] - M[0]) * (M[3] - M[0]));
+        Iv[0] = ((Mqs - M[1]) / (M[0] - M[1]) + (Mqs - M[2]) / (M[0] - M[2]) + (Mqs - M[3]) / (M[0] - M[3])) / 3.0;
+        for (int v = 1; v < 4; ++v) Iv[v] = ((Mqs - M[0]) / (M[v] - M[0])) / 3.0;
+        enter = 1;
+    } else if (Mqs >= M[1] && Mqs < M[2]) {
+        g = (3.0 / ((M[1] - M[2]) * (M[3] - M[0]))) *
+            (((Mqs - M[0]) * (Mqs - M[2])) / (M[2] - M[0]) - ((Mqs - M[1]) * (Mqs - M[3])) / (M[1] - M[3]));
+        Iv[0] = ((Mqs - M[3]) / (M[0] - M[3])) / 3.0 +
+               ((Mqs - M[2]) / (M[0] - M[2])) * ((Mqs - M[0]) / (M[2] - M[0])) * ((Mqs - M[2]) / (M[1] - M[2])) /
+                   (g * (M[3] - M[0]));
+        Iv[1] = ((Mqs - M[2]) / (M[1] - M[2])) / 3.0 +
+               ((Mqs - M[3]) / (M[1] - M[3])) * ((Mqs - M[3]) / (M[1] - M[3])) * ((Mqs - M[1]) / (M[2] - M[1])) /
+                   (g * (M[3] - M[0]));
+        Iv[2] = ((Mqs - M[1]) / (M[2] - M[1])) / 3.0 +
+               ((Mqs - M[0]) / (M[2] - M[0])) * ((Mqs - M[0]) / (M[2] - M[0])) * ((Mqs - M[2]) / (M[1] - M[2])) /
+                   (g * (M[3] - M[0]));
+        Iv[3] = ((Mqs - M[0]) / (M[3] - M[0])) / 3.0 +
+               ((Mqs - M[1]) / (M[3] - M[1])) * ((Mqs - M[3]) / (M[1] - M[3])) * ((Mqs - M[1]) / (M[2] - M[1])) /
+                   (g * (M[3] - M[0]));
+        enter = 1;
+    } else if (Mqs >= M[2] && Mqs <= M[3]) {
+        g = 3.0 * (Mqs - M[3]) * (Mqs - M[3]) / ((M[3] - M[0]) * (M[3] - M[1]) * (M[3] - M[2]));
+        for (int v = 0; v < 3; ++v) Iv[v] = ((Mqs - M[3]) / (M[v] - M[3])) / 3.0;
+        Iv[3] = ((Mqs - M[0]) / (M[3] - M[0]) + (Mqs - M[1]) / (M[3] - M[1]) + (Mqs - M[2]) / (M[3] - M[2])) / 3.0;
+        enter = 1;
+    }
+    double ixf = 0.0;
+    if (enter)
+        for (int v = 0; v < 4; ++v) ixf += Iv[v] * F[v];
+    ixf /= 6.0;
+    if (g_out) *g_out = g;
+    if (I_out) for (int v = 0; v < 4; ++v) I_out[v] = Iv[v];
+    if (gIF_out) *gIF_out = g * ixf;
+    return enter;
+}
+
+/* CalculateLwTetrahedron (ThirdOrder_m.f90:169-275) for one q0 given W(s2,s1,s0,q1) of every q1
+ * (W[q1][s0][s1][s2]), with q2 = partner(q1) (Allq1q2(4,:)): processes M1 = w1+w2 with
+ * F1 = W(n1+n2+1), M2 = w2-w1 with F2 = W(n1-n2), M3 = w1-w2 with F3 = F2 entering with a minus
+ * sign; P3_q0 accumulates g of all three.  tetra[6 nq][4] grid indices. */
+void orc_lw3_tetra(int nd, int nq, const int *tetra, const int *partner, const double *W,
+                   const double *omega, const double *nbe, const double *omega0,
+                   double *lw, double *p3q0)
+{
+    for (int s = 0; s < nd; ++s) { lw[s] = 0.0; p3q0[s] = 0.0; }
+    for (int s0 = 0; s0 < nd; ++s0) {
+        const double Mq0 = omega0[s0];
+        double acc = 0.0, pacc = 0.0;
+        for (int t = 0; t < 6 * nq; ++t) {
+            int v1[4], v2[4];
+            for (int v = 0; v < 4; ++v) { v1[v] = tetra[4 * t + v]; v2[v] = partner[v1[v]]; }
+            for (int s1 = 0; s1 < nd; ++s1)
+                for (int s2 = 0; s2 < nd; ++s2) {
+                    double F1[4], F2[4], F3[4], M1[4], M2[4], M3[4];
+                    for (int v = 0; v < 4; ++v) {
+                        const double w = W[(((size_t)v1[v] * nd + s0) * nd + s1) * nd + s2];
+                        const double n1 = nbe[(size_t)v1[v] * nd + s1], n2 = nbe[(size_t)v2[v] * nd + s2];
+                        const double w1 = omega[(size_t)v1[v] * nd + s1], w2 = omega[(size_t)v2[v] * nd + s2];
+                        F1[v] = w * (n1 + n2 + 1.0);
+                        F2[v] = w * (n1 - n2);
+                        F3[v] = F2[v];
+                        M1[v] = w1 + w2;
+                        M2[v] = w2 - w1;
+                        M3[v] = w1 - w2;
+                    }
+                    double g, gif;
+                    orc_tetra_weights(Mq0, M1, F1, &g, 0, &gif); acc += gif; pacc += g;
+                    orc_tetra_weights(Mq0, M2, F2, &g, 0, &gif); acc += gif; pacc += g;
+                    orc_tetra_weights(Mq0, M3, F3, &g, 0, &gif); acc -= gif; pacc += g;
+                }
+        }
+        lw[s0] = acc;
+        p3q0[s0] = pacc;
+    }
+}

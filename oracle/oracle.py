@@ -183,3 +183,30 @@ def lw3_from_w(W, w0, w1, w2, n1, n2, sigma):
     lib().orc_lw3_from_w(ct.c_int(nd), *[a.ctypes.data_as(ct.c_void_p) for a in arrs],
                          ct.c_double(sigma), lw.ctypes.data_as(ct.c_void_p))
     return lw
+
+
+def tetra_weights(Mqs, M, F):
+    """(entered, g, I(4) in sorted-vertex order, g * sum(I F) / 6) of one tetrahedron."""
+    g = ct.c_double()
+    I = np.zeros(4)
+    gif = ct.c_double()
+    Ma = np.ascontiguousarray(M, np.float64)
+    Fa = np.ascontiguousarray(F, np.float64)
+    enter = lib().orc_tetra_weights(ct.c_double(Mqs), Ma.ctypes.data_as(ct.c_void_p), Fa.ctypes.data_as(ct.c_void_p),
+                                    ct.byref(g), I.ctypes.data_as(ct.c_void_p), ct.byref(gif))
+    return enter, g.value, I, gif.value
+
+
+def lw3_tetra(qm, ph, i0_grid, W):
+    """CalculateLwTetrahedron for one q0 from W[q1][s0][s1][s2]: (lw, P3_q0)."""
+    nd = ph.omega.shape[1]
+    partner = np.ascontiguousarray(qm.index(-qm.q_int[i0_grid][None, :].astype(np.int64)
+                                            - qm.q_int.astype(np.int64)), np.int32)
+    lw = np.zeros(nd)
+    p3 = np.zeros(nd)
+    arrs = [np.ascontiguousarray(qm.tetra, np.int32), partner, np.ascontiguousarray(W, np.float64),
+            np.ascontiguousarray(ph.omega), np.ascontiguousarray(ph.nbe),
+            np.ascontiguousarray(ph.omega[i0_grid])]
+    lib().orc_lw3_tetra(ct.c_int(nd), ct.c_int(qm.nq), *[a.ctypes.data_as(ct.c_void_p) for a in arrs],
+                        lw.ctypes.data_as(ct.c_void_p), p3.ctypes.data_as(ct.c_void_p))
+    return lw, p3

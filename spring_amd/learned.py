@@ -1,0 +1,151 @@
+"""Learned importance-sampling proposal for the three-phonon linewidth sum.
+
+The deterministic linewidth of mode (q0, s0) is a sum over q1 of per-pair terms
+t(q1, s0) = sum_{s1,s2} W(s0,s1,s2; q0,q1) G(s0,s1,s2) (CalculateLinewidth,
+src/Kappa/Src/ThirdOrder_m.f90:490-541, with G the occupation-weighted Gaussians).  A sampled
+estimate evaluates K of the Nq pair terms, drawn with probability p(q1), and returns
+(1/K) sum_k t(q1_k)/p(q1_k) (unbiased for any p > 0).  The proposal is
+
+    p(q1) = (1 - lam) S(q1) / sum S + lam / Nq,   S(q1) = sum_{(s0,s1,s2) resonant} exp(f(x)) G
+
+where f is an MLP (spring_amd.lib.Mlp, trained on the GPU) that predicts log W from
+x = (q0, q1, w0, w1, w2, s0, s1, s2), trained only on W values the method itself produced
+(spr_v3_w_pairs), and G is known exactly.  The uniform floor lam keeps the weights bounded.
+`physics_only=True` replaces exp(f) by 1 (the proposal without the network), so the
+network's own gain can be measured against it and against uniform sampling.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import constants as C
+from .lib import Mlp
+
+NF = 12
+
+
+def features(qm, ph, i0g, q1, s0, s1, s2, wmax):
+    """Feature rows (float32) for triplets; arrays broadcast against each other."""
+    q1 = np.asarray(q1)
+    q2 = qm.index(-qm.q_int[i0g][None, :].astype(np.int64) - qm.q_int[q1].astype(np.int64))
+    nd = ph.omega.shape[1]
+    k0 = np.broadcast_to(qm.q_int[i0g] / qm.mesh, q1.shape + (3,))
+    k1 = qm.q_int[q1] / qm.mesh
+    cols = [k0[..., 0], k0[..., 1], k0[..., 2], k1[..., 0], k1[..., 1], k1[..., 2],
+            ph.omega[i0g][s0] / wmax, ph.omega[q1, s1] / wmax, ph.omega[q2, s2] / wmax,
+            s0 / (nd - 1.0), s1 / (nd - 1.0), s2 / (nd - 1.0)]
+    return np.stack([np.broadcast_to(c, np.broadcast(*cols).shape) for c in cols], axis=-1).astype(np.float32)
+
+
+def training_set(ctx, qm, ph, n_pairs: int, per_pair: int, seed: int = 0, chunk: int = 128):
+    """(X, y) with y = log W of random branch triplets of random (q0, q1) pairs."""
+    rng = np.random.default_rng(seed)
+    nd = ph.omega.shape[1]
+    wmax = float(np.abs(ph.omega).max())
+    q0i = rng.integers(0, qm.nq0, n_pairs)
+    q1 = rng.integers(0, qm.nq, n_pairs)
+    Xs, ys = [], []
+    for lo in range(0, n_pairs, chunk):
+        W = ctx.v3_w_pairs(q0i[lo:lo + chunk], q1[lo:lo + chunk])
+        for k in range(W.shape[0]):
+            flat = W[k].ravel()
+            nz = np.nonzero(flat > 0)[0]
+            if nz.size == 0:
+                continue
+            pick = rng.choice(nz, size=min(per_pair, nz.size), replace=False)
+            s0, s1, s2 = np.unravel_index(pick, (nd, nd, nd))
+            ig = int(qm.irr_idx[q0i[lo + k]])
+            Xs.append(features(qm, ph, ig, np.full(pick.size, q1[lo + k]), s0, s1, s2, wmax))
+            ys.append(np.log(flat[pick]))
+    X = np.concatenate(Xs)
+    y = np.concatenate(ys).astype(np.float32)
+    perm = rng.permutation(y.size)
+    return X[perm], y[perm]
+
+
+class Proposal:
+    """Trained network + normalisation."""
+
+    def __init__(self, mlp: Mlp, y_mean: float, y_std: float, wmax: float):
+        self.mlp, self.mu, self.sd, self.wmax = mlp, y_mean, y_std, wmax
+
+    @classmethod
+    def train(cls, X, y, wmax, epochs=8, batch=4096, lr=2e-3, seed=0):
+        mu, sd = float(y.mean()), float(y.std() + 1e-12)
+        mlp = Mlp(NF, seed=seed)
+        losses = mlp.train(X, (y - mu) / sd, epochs=epochs, batch=batch, lr=lr)
+        p = cls(mlp, mu, sd, wmax)
+        p.losses = losses
+        return p
+
+    def log_w(self, X, prec="bf16"):
+        return self.mlp.predict(X, prec) * self.sd + self.mu
+
+
+def gauss_weights(ph, i0g, q1, q2, sigma):
+    """G(s0,s1,s2) for pairs (q1, q2): array (n, nd, nd, nd)."""
+    w0 = ph.omega[i0g][None, :, None, None]
+    w1 = ph.omega[q1][:, None, :, None]
+    w2 = ph.omega[q2][:, None, None, :]
+    n1 = ph.nbe[q1][:, None, :, None]
+    n2 = ph.nbe[q2][:, None, None, :]
+    c0 = C.ONE_BY_SQRT_2PI / sigma
+    h = 0.5 / sigma ** 2
+    d = lambda x: c0 * np.exp(-h * x * x)
+    return (n1 + n2 + 1.0) * d(w0 - w1 - w2) + (n1 - n2) * (d(w0 + w1 - w2) - d(w0 - w1 + w2))
+
+
+def proposal_scores(prop: Proposal | None, qm, ph, i0g, sigma, resonant=1e-3, prec="bf16", chunk=256):
+    """S(q1) over the whole mesh for q0 (grid index i0g); prop=None gives the G-only proposal.
+
+    Returns (S, n_evals): n_evals = number of network evaluations used."""
+    nd = ph.omega.shape[1]
+    q1_all = np.arange(qm.nq)
+    q2_all = qm.index(-qm.q_int[i0g][None, :].astype(np.int64) - qm.q_int.astype(np.int64))
+    S = np.zeros(qm.nq)
+    n_eval = 0
+    iw0 = ph.inv_omega[i0g]
+    for lo in range(0, qm.nq, chunk):
+        q1 = q1_all[lo:lo + chunk]
+        q2 = q2_all[lo:lo + chunk]
+        Graw = np.abs(gauss_weights(ph, i0g, q1, q2, sigma))
+        # W = |V|^2 (1/w0)(1/w1)(1/w2) f: select triplets by G times the 1/omega factors
+        scale = np.abs(iw0[None, :, None, None] * ph.inv_omega[q1][:, None, :, None] *
+                       ph.inv_omega[q2][:, None, None, :])
+        Gs = Graw * scale
+        thr = resonant * Gs.max() if Gs.size else 0.0
+        sel = np.nonzero(Gs > thr)
+        if sel[0].size == 0:
+            continue
+        if prop is None:
+            np.add.at(S, lo + sel[0], Gs[sel])
+            continue
+        X = features(qm, ph, i0g, q1[sel[0]], sel[1], sel[2], sel[3], prop.wmax)
+        lw = prop.log_w(X, prec)
+        n_eval += X.shape[0]
+        np.add.at(S, lo + sel[0], np.exp(np.minimum(lw - prop.mu, 80.0)) * Graw[sel])
+    return S, n_eval
+
+
+def mixture(S, lam=0.1):
+    nq = S.size
+    tot = S.sum()
+    base = S / tot if tot > 0 else np.full(nq, 1.0 / nq)
+    return (1.0 - lam) * base + lam / nq
+
+
+def is_rms_error(terms, p, K):
+    """Relative RMS error of the K-sample importance estimate of sum(terms) (per s0),
+    exact expectation: Var = (sum t^2/p - (sum t)^2) / K."""
+    tot = terms.sum(axis=0)
+    var = ((terms ** 2) / p[:, None]).sum(axis=0) - tot ** 2
+    var = np.maximum(var, 0.0) / K
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.sqrt(var) / np.abs(tot)
+
+
+def sample_estimate(ctx, qi, p, K, sigma, prec, rng):
+    """Draw K q1 from p, evaluate their pair terms on the GPU, return the estimate (Ndof)."""
+    q1 = rng.choice(p.size, size=K, replace=True, p=p)
+    t = ctx.lw3_pairs(np.full(K, qi), q1, sigma, prec)
+    return (t / p[q1][:, None]).mean(axis=0)

@@ -259,3 +259,26 @@ def test_kappa_single_mode_closed_form():
     v = PH.grp_vel[0, 5]
     expect = C.KW_M_K / (300.0 ** 2 * 40.0) * 400.0 * 0.7 * 1.7 * 2.0 * np.outer(v, v)
     assert np.allclose(k, expect, rtol=1e-13, atol=1e-300)
+
+
+# ---------------------------------------------------------------- tetrahedron method
+def test_tetra_weights_closed_forms():
+    """Sum of vertex weights is 1; with F = M (linear), sum I F = Mqs (the iso-surface value);
+    g integrates to 1 over [M_min, M_max]; a constant F passes through unchanged."""
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        M = rng.uniform(-1, 3, 4)
+        lo, hi = M.min(), M.max()
+        x = rng.uniform(lo, hi)
+        enter, g, I, gif = O.tetra_weights(x, M, np.full(4, 2.5))
+        assert enter == 1
+        assert abs(I.sum() - 1.0) < 1e-10
+        assert abs(gif - g * 2.5 / 6.0) < 1e-10 * max(1.0, abs(g))
+        order = np.argsort(M, kind="stable")
+        _, _, I2, _ = O.tetra_weights(x, M, M)
+        assert abs(np.dot(I2, np.sort(M)) - x) < 1e-9 * max(1.0, abs(x))
+    M = np.array([0.1, 0.7, 1.3, 2.0])
+    xs = np.linspace(0.1, 2.0, 20001)
+    gs = np.array([O.tetra_weights(x, M, np.ones(4))[1] for x in xs])
+    assert abs(np.trapezoid(gs, xs) - 1.0) < 1e-6
+    assert O.tetra_weights(2.5, M, np.ones(4))[0] == 0

@@ -74,7 +74,7 @@ static int enqueue(Ctx &c, ItemLaunch &L, int mode, int q0_first, int q0_count, 
     ic->counter = mode == 0 ? nullptr : c.counter;
     if (mode) hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.counter, 1, 0);
     if (timed) SPR_HIP(hipEventRecord(c.ev0, c.stream));
-    void *args[] = {L.args};
+    void *args[5] = {L.args, L.extra[0], L.extra[1], L.extra[2], L.extra[3]};
     SPR_HIP(hipLaunchKernel(L.func, dim3(grid_for(c, L, mode)), dim3(L.block), args, L.dyn, c.stream));
     if (timed) SPR_HIP(hipEventRecord(c.ev1, c.stream));
     if (d_lw) return launch_reduce_items(c, q0_first, q0_count, it0, d_lw);
@@ -143,6 +143,7 @@ int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double
         os << key << "|m" << mode << "|q" << q0_first << ":" << q0_count << "|out" << (const void *)d_lw << "|";
         const unsigned char *b = reinterpret_cast<const unsigned char *>(L.args);
         for (size_t i = 0; i < L.args_size; ++i) os << std::hex << (int)b[i] << ".";
+        for (int e = 0; e < L.nextra; ++e) os << "|" << *reinterpret_cast<void **>(L.extra[e]);
         gkey = os.str();
     }
     if (use_graph) {

@@ -46,22 +46,27 @@ struct SmallArgs {
 };
 
 template <typename R, int NB>
-__device__ __forceinline__ void small_item(const SmallArgs &A, const int item) {
+__device__ __forceinline__ void small_item(const SmallArgs &A, const int item, const R *__restrict__ fc,
+                                           const int *__restrict__ blk_start, const int *__restrict__ tup_site) {
     using Z = cplx_t<R>;
     constexpr int ND = 3 * NB;
     constexpr int ND2 = ND * ND;
+    constexpr int NW = ND * NB;            // waves: row a x half nu
     extern __shared__ __align__(16) unsigned char smem[];
-    // layout: e1[ND2][64], e2[ND2][64] (aliased by the exchange buffer x[ND][ND][64]),
-    //         m1[nsite][64], m2[nsite][64] (int), phase[L]
+    // layout: e1[ND2][64] | x[NW][ND][64] (exchange; the e2[ND2][64] staging aliases its start)
+    //         | m1[nsite][64], m2[nsite][64] (int) | md[6 ND][64] (double) | phase[L] | e0[ND2]
     Z *s_e1 = reinterpret_cast<Z *>(smem);
-    Z *s_e2 = s_e1 + ND2 * 64;
-    Z *s_x = s_e1;
-    int *s_m1 = reinterpret_cast<int *>(s_e2 + ND2 * 64);
+    Z *s_x = s_e1 + ND2 * 64;
+    Z *s_e2 = s_x;
+    int *s_m1 = reinterpret_cast<int *>(s_x + NW * ND * 64);
     int *s_m2 = s_m1 + A.nsite * 64;
-    Z *s_ph = reinterpret_cast<Z *>(s_m2 + A.nsite * 64);
+    double *s_md = reinterpret_cast<double *>(s_m2 + A.nsite * 64);
+    Z *s_ph = reinterpret_cast<Z *>(s_md + 6 * ND * 64);
+    Z *s_e0 = s_ph + A.L;
 
     const int lane = threadIdx.x & 63;
-    const int a = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave index
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int a = w / NB, nu = w % NB;             // phase-sum row a, rows b = 3 nu + beta
     const int mu = a / 3, al = a % 3;
     const int qi = A.item_q0[item];
     const int i0 = A.q0_grid[qi];
@@ -71,115 +76,103 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item) {
     const double wpair = lane < cnt ? (q1 == q2 ? 1.0 : 2.0) : 0.0;
     const Z *evec = reinterpret_cast<const Z *>(A.evec);
     const Z *phase = reinterpret_cast<const Z *>(A.phase);
-    const R *fc = reinterpret_cast<const R *>(A.fc);
 
-    // ---- stage per-lane data: eigenvectors (lane-major), phase indices, table ----
-    for (int i = a; i < ND2; i += ND) {
+    // ---- stage per-lane data: eigenvectors (lane-major), phase indices, mode data ----
+    for (int i = w; i < ND2; i += NW) {
         s_e1[i * 64 + lane] = evec[(size_t)q1 * ND2 + i];
         s_e2[i * 64 + lane] = evec[(size_t)q2 * ND2 + i];
     }
     {
         const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
         const int k20 = q2 % A.m0, k21 = (q2 / A.m0) % A.m1, k22 = q2 / (A.m0 * A.m1);
-        for (int s = a; s < A.nsite; s += ND) {
+        for (int s = w; s < A.nsite; s += NW) {
             const int4 c = A.site[s];
             s_m1[s * 64 + lane] = (k10 * c.x + k11 * c.y + k12 * c.z) % A.L;
             s_m2[s * 64 + lane] = (k20 * c.x + k21 * c.y + k22 * c.z) % A.L;
         }
     }
-    for (int i = threadIdx.x; i < A.L; i += ND * 64) s_ph[i] = phase[i];
+    for (int k = w; k < 6 * ND; k += NW) {   // omega, nBE, 1/omega of q1 (k < 3 ND), then of q2
+        const int qq = k < 3 * ND ? q1 : q2, kk = k % (3 * ND), what = kk / ND, s = kk % ND;
+        const double *src = what == 0 ? A.omega : what == 1 ? A.nbe : A.inv_omega;
+        s_md[k * 64 + lane] = src[(size_t)qq * ND + s];
+    }
+    for (int i = threadIdx.x; i < A.L; i += NW * 64) s_ph[i] = phase[i];
+    for (int i = threadIdx.x; i < ND2; i += NW * 64) s_e0[i] = evec[(size_t)i0 * ND2 + i];
     __syncthreads();
 
-    // ---- 1. phase sum of row a ----
-    Z T[ND][ND];
+    // ---- 1. phase sum: T(beta, c) = T_a(3 nu + beta, c), FC values wave-uniform ----
+    Z T[3][ND];
 #pragma unroll
-    for (int b = 0; b < ND; ++b)
+    for (int be = 0; be < 3; ++be)
 #pragma unroll
-        for (int c = 0; c < ND; ++c) T[b][c] = cmk<R>(0, 0);
+        for (int c = 0; c < ND; ++c) T[be][c] = cmk<R>(0, 0);
 #pragma unroll
-    for (int nu = 0; nu < NB; ++nu)
+    for (int eta = 0; eta < NB; ++eta) {
+        const int blk = (mu * NB + nu) * NB + eta;
+        const int u0 = blk_start[blk], u1 = blk_start[blk + 1];
+        for (int u = u0; u < u1; ++u) {
+            const int sa = tup_site[2 * u], sb = tup_site[2 * u + 1];
+            int m = s_m1[sa * 64 + lane] + s_m2[sb * 64 + lane];
+            m -= m >= A.L ? A.L : 0;
+            const Z ph = s_ph[m];
+            const R *f = fc + (size_t)u * 27 + al * 9;
 #pragma unroll
-        for (int eta = 0; eta < NB; ++eta) {
-            const int blk = (mu * NB + nu) * NB + eta;
-            const int u0 = A.blk_start[blk], u1 = A.blk_start[blk + 1];
-            for (int u = u0; u < u1; ++u) {
-                const int sa = A.tup_site[2 * u], sb = A.tup_site[2 * u + 1];
-                int m = s_m1[sa * 64 + lane] + s_m2[sb * 64 + lane];
-                m -= m >= A.L ? A.L : 0;
-                const Z ph = s_ph[m];
-                const R *f = fc + (size_t)u * 27 + al * 9;
+            for (int be = 0; be < 3; ++be)
 #pragma unroll
-                for (int be = 0; be < 3; ++be)
-#pragma unroll
-                    for (int ga = 0; ga < 3; ++ga)
-                        T[3 * nu + be][3 * eta + ga] = crfma(f[be * 3 + ga], ph, T[3 * nu + be][3 * eta + ga]);
-            }
+                for (int ga = 0; ga < 3; ++ga)
+                    T[be][3 * eta + ga] = crfma(f[be * 3 + ga], ph, T[be][3 * eta + ga]);
         }
-
-    // ---- 2. T <- T e2 (c -> s2), then e1^T T (b -> s1) ----
+    }
+    // ---- 2. U1(beta, s2) = sum_c T(beta, c) e2(c, s2), in place per row ----
 #pragma unroll
-    for (int b = 0; b < ND; ++b) {
+    for (int be = 0; be < 3; ++be) {
         Z t[ND];
 #pragma unroll
         for (int s2 = 0; s2 < ND; ++s2) {
             Z z = cmk<R>(0, 0);
 #pragma unroll
-            for (int c = 0; c < ND; ++c) z = cfma(T[b][c], s_e2[(s2 * ND + c) * 64 + lane], z);
+            for (int c = 0; c < ND; ++c) z = cfma(T[be][c], s_e2[(s2 * ND + c) * 64 + lane], z);
             t[s2] = z;
         }
 #pragma unroll
-        for (int s2 = 0; s2 < ND; ++s2) T[b][s2] = t[s2];
-    }
-#pragma unroll
-    for (int s2 = 0; s2 < ND; ++s2) {
-        Z t[ND];
-#pragma unroll
-        for (int s1 = 0; s1 < ND; ++s1) {
-            Z z = cmk<R>(0, 0);
-#pragma unroll
-            for (int b = 0; b < ND; ++b) z = cfma(s_e1[(s1 * ND + b) * 64 + lane], T[b][s2], z);
-            t[s1] = z;
-        }
-#pragma unroll
-        for (int s1 = 0; s1 < ND; ++s1) T[s1][s2] = t[s1];
+        for (int s2 = 0; s2 < ND; ++s2) T[be][s2] = t[s2];
     }
 
-    // ---- 3 + 4. e0 contraction across waves, W and the delta sum (wave = s0) ----
-    const int s0 = a;
-    Z e0c[ND];
-#pragma unroll
-    for (int r = 0; r < ND; ++r) e0c[r] = evec[(size_t)i0 * ND2 + s0 * ND + r];   // e0(r, s0)
+    // ---- 3 + 4. per s1: partial U2 over this wave's rows b -> LDS; wave (s0, s2 half)
+    //      forms V(s0, s1, s2) = sum_{a, nu} e0(a, s0) U2, then W and the delta sum ----
+    const int s0 = w / NB, s2_0 = 3 * (w % NB);
     const double w0 = A.omega[(size_t)i0 * ND + s0];
     const double iw0f = A.inv_omega[(size_t)i0 * ND + s0] * A.factor;
-    double w2[ND], n2[ND], iw2[ND];
-#pragma unroll
-    for (int s2 = 0; s2 < ND; ++s2) {
-        w2[s2] = A.omega[(size_t)q2 * ND + s2];
-        n2[s2] = A.nbe[(size_t)q2 * ND + s2];
-        iw2[s2] = A.inv_omega[(size_t)q2 * ND + s2];
-    }
     const double c0 = 1.0 / (A.sigma * 2.5066282746310002);
     const double hinv = 0.5 / (A.sigma * A.sigma);
     KSum<R> kacc;
     double dacc = 0.0;
-    __syncthreads();   // every wave is done reading e1/e2 before the exchange overwrites them
-#pragma unroll
+    __syncthreads();   // every wave is done with e2 before the exchange buffer overwrites it
+#pragma unroll 1
     for (int s1 = 0; s1 < ND; ++s1) {
 #pragma unroll
-        for (int s2 = 0; s2 < ND; ++s2) s_x[(a * ND + s2) * 64 + lane] = T[s1][s2];
-        __syncthreads();
-        const double w1 = A.omega[(size_t)q1 * ND + s1];
-        const double n1 = A.nbe[(size_t)q1 * ND + s1];
-        const double iw01 = iw0f * A.inv_omega[(size_t)q1 * ND + s1];
-#pragma unroll
         for (int s2 = 0; s2 < ND; ++s2) {
+            Z z = cmk<R>(0, 0);
+#pragma unroll
+            for (int be = 0; be < 3; ++be) z = cfma(s_e1[(s1 * ND + 3 * nu + be) * 64 + lane], T[be][s2], z);
+            s_x[(w * ND + s2) * 64 + lane] = z;
+        }
+        __syncthreads();
+        const double w1 = s_md[(0 * ND + s1) * 64 + lane];
+        const double n1 = s_md[(1 * ND + s1) * 64 + lane];
+        const double iw01 = iw0f * s_md[(2 * ND + s1) * 64 + lane];
+#pragma unroll 1
+        for (int j = 0; j < 3; ++j) {
+            const int s2 = s2_0 + j;
             Z v = cmk<R>(0, 0);
 #pragma unroll
-            for (int r = 0; r < ND; ++r) v = cfma(e0c[r], s_x[(r * ND + s2) * 64 + lane], v);
-            const double W = (double)cabs2(v) * iw01 * iw2[s2];
-            const double d1 = w0 - w1 - w2[s2], d2 = w0 + w1 - w2[s2], d3 = w0 - w1 + w2[s2];
-            const double g = (n1 + n2[s2] + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
-                             (n1 - n2[s2]) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
+            for (int ww = 0; ww < NW; ++ww) v = cfma(s_e0[s0 * ND + ww / NB], s_x[(ww * ND + s2) * 64 + lane], v);
+            const double w2 = s_md[(3 * ND + s2) * 64 + lane];
+            const double n2 = s_md[(4 * ND + s2) * 64 + lane];
+            const double W = (double)cabs2(v) * iw01 * s_md[(5 * ND + s2) * 64 + lane];
+            const double d1 = w0 - w1 - w2, d2 = w0 + w1 - w2, d3 = w0 - w1 + w2;
+            const double g = (n1 + n2 + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
+                             (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
             if (sizeof(R) == 8)
                 dacc += wpair * W * g;
             else
@@ -188,17 +181,26 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item) {
         __syncthreads();
     }
     double tot = sizeof(R) == 8 ? dacc : kacc.value();
-    // wave reduction (fixed order)
+    // wave reduction, then the NB waves sharing s0 in a fixed order
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_down(tot, off, 64);
-    if (lane == 0) A.ic.partial[(size_t)item * ND + s0] = tot;
+    double *s_red = reinterpret_cast<double *>(s_x);
+    if (lane == 0) s_red[w] = tot;
+    __syncthreads();
+    if (threadIdx.x < ND) {
+        double sum = 0.0;
+        for (int h = 0; h < NB; ++h) sum += s_red[threadIdx.x * NB + h];
+        A.ic.partial[(size_t)item * ND + threadIdx.x] = sum;
+    }
 }
 
 // One workgroup per item, or (counter set) a persistent grid pulling items until none are left.
 template <typename R, int NB>
-__global__ __launch_bounds__(NB * 3 * 64) void k_lw3_small(SmallArgs A) {
+__global__ __launch_bounds__(NB * NB * 3 * 64) void k_lw3_small(SmallArgs A, const R *__restrict__ fc,
+                                                               const int *__restrict__ blk_start,
+                                                               const int *__restrict__ tup_site) {
     if (!A.ic.counter) {
-        small_item<R, NB>(A, blockIdx.x);
+        small_item<R, NB>(A, blockIdx.x, fc, blk_start, tup_site);
         return;
     }
     __shared__ int s_item;
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(NB * 3 * 64) void k_lw3_small(SmallArgs A) {
         __syncthreads();
         const int item = s_item;
         if (item >= A.ic.nitems) break;
-        small_item<R, NB>(A, item);
+        small_item<R, NB>(A, item, fc, blk_start, tup_site);
     }
 }
 
@@ -231,8 +233,9 @@ static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first,
     A.fc = sizeof(R) == 8 ? (const void *)c.fc3.fc64 : (const void *)c.fc3.fc32;
     A.phase = sizeof(R) == 8 ? (const void *)c.phase64 : (const void *)c.phase32;
     A.evec = sizeof(R) == 8 ? (const void *)c.evec : (const void *)c.evec32;
-    const size_t dyn = (size_t)2 * ND * ND * 64 * sizeof(cplx_t<R>) + (size_t)2 * A.nsite * 64 * sizeof(int) +
-                       (size_t)c.L * sizeof(cplx_t<R>);
+    constexpr int NW = ND * NB;
+    const size_t dyn = (size_t)(ND * ND + NW * ND) * 64 * sizeof(cplx_t<R>) + (size_t)2 * A.nsite * 64 * sizeof(int) +
+                       (size_t)6 * ND * 64 * sizeof(double) + (size_t)(c.L + ND * ND) * sizeof(cplx_t<R>);
     if (dyn > 160 * 1024) {
         set_error("small-cell kernel: LDS need exceeds 160 KB (too many neighbour sites)");
         return SPR_E_UNSUPPORTED;
@@ -240,10 +243,16 @@ static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first,
     const void *fn = (const void *)k_lw3_small<R, NB>;
     if (dyn > 64 * 1024) SPR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
     int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, ND * 64, dyn) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NW * 64, dyn) != hipSuccess || per_cu < 1)
         per_cu = 1;
-    ItemLaunch L{fn, &A, sizeof(A), ND * 64, dyn, nitems, per_cu,
+    const R *fcp = reinterpret_cast<const R *>(A.fc);
+    const int *bsp = A.blk_start, *tsp = A.tup_site;
+    ItemLaunch L{fn, &A, sizeof(A), NW * 64, dyn, nitems, per_cu,
                  std::string("small") + (sizeof(R) == 8 ? "64" : "32")};
+    L.extra[0] = (void *)&fcp;
+    L.extra[1] = (void *)&bsp;
+    L.extra[2] = (void *)&tsp;
+    L.nextra = 3;
     L.direct = c.override_items != nullptr;
     return run_items(c, L, q0_first, q0_count, it0, d_lw);
 }

@@ -40,6 +40,8 @@ struct ItemLaunch {
     int per_cu;             // resident workgroups per compute unit
     std::string name;
     bool direct = false;    // one workgroup per item, no graph, no tuning, no reduction
+    void *extra[4] = {nullptr, nullptr, nullptr, nullptr};   // further kernel arguments
+    int nextra = 0;
 };
 
 // Caller-supplied work items (spr_lw3_pairs): partial[item] receives the item's sum.

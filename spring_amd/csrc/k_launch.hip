@@ -101,7 +101,7 @@ int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double
         SPR_HIP(hipFuncSetAttribute(L.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.dyn));
     // ---- launch mode: env override, tuned value, or default (persistent 2 x CUs) ----
     if (L.direct) {
-        if ((rc = enqueue(c, L, 0, 0, 0, 0, nullptr, true))) return rc;
+        if ((rc = enqueue(c, L, L.fixed_mode >= 0 ? L.fixed_mode : 0, 0, 0, 0, nullptr, true))) return rc;
         SPR_HIP(hipEventSynchronize(c.ev1));
         return SPR_OK;
     }
@@ -110,7 +110,9 @@ int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double
                             std::to_string(L.nitems);
     const char *force = getenv("SPR_LAUNCH_MODE");
     const char *at = getenv("SPR_AUTOTUNE");
-    if (force && *force) {
+    if (L.fixed_mode >= 0) {
+        mode = L.fixed_mode;
+    } else if (force && *force) {
         mode = atoi(force);
     } else if (tune_table().count(key)) {
         mode = tune_table()[key];

@@ -43,11 +43,14 @@ struct MfmaArgs {
     const float2 *evec;
     const float *omega, *inv_omega, *nbe;
     float sigma, factor;
+    float *scratch;          // FP32 mode: per-workgroup U2 stash slots (persistent grid only)
+    int nslots;
 };
 
 template <bool BF> struct Store;
 template <> struct Store<false> { using S = float; };
 template <> struct Store<true> { using S = __bf16; };
+template <bool BF> using S_of = typename Store<BF>::S;
 
 // B' element of the real-expanded complex eigenvector matrix E(k, n) = evec row k, mode n,
 // stored as E[n * ND + k] (interleaved re/im in S): rows k' = 2k + p, columns n' = 2n + q.
@@ -87,7 +90,7 @@ __device__ __forceinline__ v4f mma_tile(FA A, FB B, int m_base, int n_base, v4f 
 }
 
 template <bool BF, int NB>
-__device__ __forceinline__ void mfma_item(const MfmaArgs &A, const int item) {
+__device__ __forceinline__ void mfma_item(const MfmaArgs &A, const int item, S_of<BF> *st_global) {
     using S = typename Store<BF>::S;
     constexpr int ND = 3 * NB;
     constexpr int ND2 = ND * ND;
@@ -96,18 +99,19 @@ __device__ __forceinline__ void mfma_item(const MfmaArgs &A, const int item) {
     constexpr int MT_BC = (3 * ND + 15) / 16;          // m tiles of GEMMs B and C
     constexpr int NT = (W2 + 15) / 16;                 // n tiles (<= 4)
     constexpr int MT_D = (ND2 + 15) / 16;              // m tiles of GEMM D
-    constexpr int STASH_ATOMS = BF ? NB : (NB < 3 ? NB : 3);
-    constexpr int STASH_ROWS = 3 * STASH_ATOMS;
-    constexpr int KP_D = BF ? ((2 * STASH_ROWS + 31) / 32) * 32 : ((2 * STASH_ROWS + 3) / 4) * 4;
+    constexpr int KP_D = BF ? ((2 * ND + 31) / 32) * 32 : ((2 * ND + 3) / 4) * 4;
     constexpr int NWAVE = 8;
     constexpr int DT_PER_WAVE = (MT_D + 1) / 2;   // wave w: n tile w % 4, m tiles w / 4 + 2 t
     static_assert(NT <= 4, "Ndof must be <= 32");
 
+    // The stash U2[a][s1 s2] (all ND rows) lives in LDS for BF16 and in a per-workgroup
+    // global slot for FP32 (FP32 U2 of Ndof = 30 needs 216 KB).
     extern __shared__ __align__(16) unsigned char smem[];
     S *s_T = reinterpret_cast<S *>(smem);                       // [3 ND][W2]
     S *s_U1 = s_T + 3 * ND * W2;                                // [3 ND][W2]
-    S *s_st = s_U1 + 3 * ND * W2;                               // [STASH_ROWS][ND2][2]
-    S *s_e0 = s_st + STASH_ROWS * ND2 * 2;                      // [ND][ND][2]  (mode, row)
+    S *s_stl = s_U1 + 3 * ND * W2;                              // [ND][ND2][2] (BF16 only)
+    S *s_st = BF ? s_stl : st_global;
+    S *s_e0 = s_stl + (BF ? ND * ND2 * 2 : 0);                  // [ND][ND][2]  (mode, row)
     S *s_e1 = s_e0 + ND2 * 2;
     S *s_e2 = s_e1 + ND2 * 2;
     float *s_mode = reinterpret_cast<float *>(s_e2 + ND2 * 2 + ((ND2 * 2 * 3) & 1));  // 8 x ND
@@ -171,13 +175,10 @@ __device__ __forceinline__ void mfma_item(const MfmaArgs &A, const int item) {
                 s_m2[s] = (k20 * c.x + k21 * c.y + k22 * c.z) % A.L;
             }
         }
-        v4f vacc[DT_PER_WAVE];
-#pragma unroll
-        for (int t = 0; t < DT_PER_WAVE; ++t) vacc[t] = v4f{0.f, 0.f, 0.f, 0.f};
         __syncthreads();
 
         for (int mu = 0; mu < NB; ++mu) {
-            const int srow0 = (mu % STASH_ATOMS) * 3;   // stash rows of this atom
+            const int srow0 = 3 * mu;   // stash rows of this atom
             // ---- FT: T(alpha, b, c) for the three rows of atom mu ----
             for (int o = tid; o < 3 * ND2; o += 512) {
                 const int al = o / ND2, b = (o / ND) % ND, c = o % ND;
@@ -229,54 +230,44 @@ __device__ __forceinline__ void mfma_item(const MfmaArgs &A, const int item) {
                     }
                 }
             }
-            // ---- GEMM D when the stash is full (or at the last atom) ----
-            const bool flush = (mu % STASH_ATOMS == STASH_ATOMS - 1) || (mu == NB - 1);
-            if (flush) {
-                __syncthreads();
-                const int rows = srow0 + 3;            // valid stash rows
-                const int arow0 = (mu - mu % STASH_ATOMS) * 3;
-                auto fa = [&](int m, int kp) -> S {
-                    const int a = kp >> 1;
-                    return (m < ND2 && a < rows) ? s_st[(a * ND2 + m) * 2 + (kp & 1)] : (S)0.0f;
-                };
-                auto fb = [&](int kp, int np) -> S {
-                    const int a = kp >> 1;
-                    if (a >= rows) return (S)0.0f;
-                    return bprime<S, ND>(s_e0, 2 * (arow0 + a) + (kp & 1), np);
-                };
-#pragma unroll
-                for (int t = 0; t < DT_PER_WAVE; ++t) {
-                    const int mt = mt_d0 + 2 * t;
-                    if (mt < MT_D) vacc[t] = mma_tile<BF, KP_D>(fa, fb, mt * 16, nt_d * 16, vacc[t]);
-                }
-            }
         }
-        // ---- step E: W and the Gaussian-delta weights (each lane: one s0, rows r) ----
-        const bool s0ok = s0 < ND;
-        const float w0 = s0ok ? s_mode[s0] : 0.f;
-        const float iw0 = s0ok ? s_mode[ND + s0] : 0.f;
+        // ---- GEMM D fused with step E, one 16x16 tile at a time:
+        //      V[(s1 s2)][(s0,q)] = U2^T[(s1 s2)][(a,p)] x E0'[(a,p)][(s0,q)], then W and the
+        //      Gaussian-delta weights (each lane: one s0 and rows r) ----
+        __syncthreads();
+        {
+            auto fa = [&](int m, int kp) -> S {
+                const int a = kp >> 1;
+                return (m < ND2 && a < ND) ? s_st[(a * ND2 + m) * 2 + (kp & 1)] : (S)0.0f;
+            };
+            auto fb = [&](int kp, int np) -> S { return bprime<S, ND>(s_e0, kp, np); };
+            const bool s0ok = s0 < ND;
+            const float w0 = s0ok ? s_mode[s0] : 0.f;
+            const float iw0 = s0ok ? s_mode[ND + s0] : 0.f;
+#pragma unroll 1
+            for (int t = 0; t < DT_PER_WAVE; ++t) {
+                const int mt = mt_d0 + 2 * t;
+                if (mt >= MT_D || nt_d >= NT) continue;
+                const v4f vt = mma_tile<BF, KP_D>(fa, fb, mt * 16, nt_d * 16, v4f{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
-        for (int t = 0; t < DT_PER_WAVE; ++t) {
-            const int mt = mt_d0 + 2 * t;
-            if (mt >= MT_D) continue;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float x = vacc[t][r];
-                const float y = __shfl_xor(x, 1, 64);
-                const float v2 = x * x + y * y;
-                // lanes with q = 0 take rows 0,1, lanes with q = 1 rows 2,3
-                if (((r >> 1) != (j & 1)) || !s0ok) continue;
-                const int m = mt * 16 + 4 * (l >> 4) + r;
-                if (m >= ND2) continue;
-                const int s1 = m / ND, s2 = m % ND;
-                const float w1 = s_mode[2 * ND + s1], n1 = s_mode[3 * ND + s1], iw1 = s_mode[4 * ND + s1];
-                const float w2 = s_mode[5 * ND + s2], n2 = s_mode[6 * ND + s2], iw2 = s_mode[7 * ND + s2];
-                const float W = v2 * iw0 * iw1 * iw2;
-                const float d1 = w0 - w1 - w2, d2 = w0 + w1 - w2, d3 = w0 - w1 + w2;
-                const float g1 = c0 * __expf(-hinv * d1 * d1);
-                const float g2 = c0 * __expf(-hinv * d2 * d2);
-                const float g3 = c0 * __expf(-hinv * d3 * d3);
-                acc.add(wpair * W * ((n1 + n2 + 1.0f) * g1 + (n1 - n2) * (g2 - g3)));
+                for (int r = 0; r < 4; ++r) {
+                    const float x = vt[r];
+                    const float y = __shfl_xor(x, 1, 64);
+                    const float v2 = x * x + y * y;
+                    // lanes with q = 0 take rows 0,1, lanes with q = 1 rows 2,3
+                    if (((r >> 1) != (j & 1)) || !s0ok) continue;
+                    const int m = mt * 16 + 4 * (l >> 4) + r;
+                    if (m >= ND2) continue;
+                    const int s1 = m / ND, s2 = m % ND;
+                    const float w1 = s_mode[2 * ND + s1], n1 = s_mode[3 * ND + s1], iw1 = s_mode[4 * ND + s1];
+                    const float w2 = s_mode[5 * ND + s2], n2 = s_mode[6 * ND + s2], iw2 = s_mode[7 * ND + s2];
+                    const float W = v2 * iw0 * iw1 * iw2;
+                    const float d1 = w0 - w1 - w2, d2 = w0 + w1 - w2, d3 = w0 - w1 + w2;
+                    const float g1 = c0 * __expf(-hinv * d1 * d1);
+                    const float g2 = c0 * __expf(-hinv * d2 * d2);
+                    const float g3 = c0 * __expf(-hinv * d3 * d3);
+                    acc.add(wpair * W * ((n1 + n2 + 1.0f) * g1 + (n1 - n2) * (g2 - g3)));
+                }
             }
         }
     }
@@ -296,8 +287,11 @@ __device__ __forceinline__ void mfma_item(const MfmaArgs &A, const int item) {
 
 template <bool BF, int NB>
 __global__ __launch_bounds__(512) void k_lw3_mfma(MfmaArgs A) {
+    constexpr int ND = 3 * NB;
+    if (!BF && (int)blockIdx.x >= A.nslots) return;   // persistent grid: the others take the items
+    S_of<BF> *slot = BF ? nullptr : reinterpret_cast<S_of<BF> *>(A.scratch) + (size_t)blockIdx.x * ND * ND * ND * 2;
     if (!A.ic.counter) {
-        mfma_item<BF, NB>(A, blockIdx.x);
+        mfma_item<BF, NB>(A, blockIdx.x, slot);
         return;
     }
     __shared__ int s_item;
@@ -307,7 +301,7 @@ __global__ __launch_bounds__(512) void k_lw3_mfma(MfmaArgs A) {
         __syncthreads();
         const int item = s_item;
         if (item >= A.ic.nitems) break;
-        mfma_item<BF, NB>(A, item);
+        mfma_item<BF, NB>(A, item, slot);
     }
 }
 
@@ -315,8 +309,7 @@ template <bool BF, int NB>
 static size_t mfma_lds(int nsite, int L) {
     using S = typename Store<BF>::S;
     constexpr int ND = 3 * NB, ND2 = ND * ND, W2 = 2 * ND;
-    constexpr int STASH_ATOMS = BF ? NB : (NB < 3 ? NB : 3);
-    size_t nS = (size_t)2 * 3 * ND * W2 + (size_t)3 * STASH_ATOMS * ND2 * 2 + 3 * ND2 * 2;
+    size_t nS = (size_t)2 * 3 * ND * W2 + (BF ? (size_t)ND * ND2 * 2 : 0) + 3 * ND2 * 2;
     size_t b = nS * sizeof(S) + ((ND2 * 2 * 3) & 1) * sizeof(S) + 8 * ND * sizeof(float) +
                (size_t)L * sizeof(float2) + 2 * (size_t)nsite * sizeof(int);
     b = (b + 15) & ~(size_t)15;
@@ -336,6 +329,27 @@ static int launch_mfma_t(Ctx &c, MfmaArgs A, int nitems, int q0_first, int q0_co
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 512, dyn) != hipSuccess || per_cu < 1) per_cu = 1;
     ItemLaunch L{fn, &A, sizeof(A), 512, dyn, nitems, per_cu, std::string(BF ? "mfma_bf16" : "mfma_f32")};
     L.direct = c.override_items != nullptr;
+    if (!BF) {
+        // FP32: the U2 stash of each resident workgroup lives in its own global slot, so the
+        // grid is persistent with exactly one slot per workgroup (mode 1: ncu x per_cu)
+        constexpr int ND = 3 * NB;
+        const int slots = c.ncu * per_cu;
+        const size_t bytes = (size_t)slots * ND * ND * ND * 2 * sizeof(float);
+        if (bytes > c.mfma_scratch_bytes) {
+            if (c.mfma_scratch) SPR_HIP(hipFree(c.mfma_scratch));
+            c.mfma_scratch = nullptr;
+            c.mfma_scratch_bytes = 0;
+            drop_graphs(c);   // captured graphs hold the old slot address
+            if (hipMalloc((void **)&c.mfma_scratch, bytes) != hipSuccess) {
+                set_error("MFMA kernel: cannot allocate " + std::to_string(bytes) + " B of stash slots");
+                return SPR_E_OOM;
+            }
+            c.mfma_scratch_bytes = bytes;
+        }
+        A.scratch = c.mfma_scratch;
+        A.nslots = slots;
+        L.fixed_mode = 1;
+    }
     return run_items(c, L, q0_first, q0_count, it0, d_lw);
 }
 
@@ -361,7 +375,7 @@ int launch_lw3_mfma(Ctx &c, int q0_first, int q0_count, double sigma, int prec, 
     const int it0 = ov ? 0 : c.h_item_first[q0_first];
     const int nitems = ov ? ov->nitems : c.h_item_first[q0_first + q0_count] - it0;
     const size_t nm = (size_t)c.nq * c.nd;
-    MfmaArgs A;
+    MfmaArgs A{};
     A.item_q0 = ov ? ov->item_q0 : c.d_item_q0 + it0;
     A.item_off = ov ? ov->item_off : c.d_item_off + it0;
     A.item_cnt = ov ? ov->item_cnt : c.d_item_cnt + it0;

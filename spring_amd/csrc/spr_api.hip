@@ -271,7 +271,7 @@ extern "C" int spr_ctx_destroy(spr_ctx *h) {
                     (void *)c.omega, (void *)c.inv_omega, (void *)c.nbe, (void *)c.vel,
                     (void *)c.irr_grid, (void *)c.partial, (void *)c.out, (void *)c.counter,
                     (void *)c.d_pairs, (void *)c.d_item_q0, (void *)c.d_item_off, (void *)c.d_item_cnt,
-                    (void *)c.d_item_first, (void *)c.omega32,
+                    (void *)c.d_item_first, (void *)c.omega32, (void *)c.mfma_scratch,
                     (void *)c.ipartial, (void *)c.wout})
         if (p) (void)hipFree(p);
     if (c.ev0) (void)hipEventDestroy(c.ev0);

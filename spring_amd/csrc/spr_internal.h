@@ -42,6 +42,7 @@ struct ItemLaunch {
     bool direct = false;    // one workgroup per item, no graph, no tuning, no reduction
     void *extra[4] = {nullptr, nullptr, nullptr, nullptr};   // further kernel arguments
     int nextra = 0;
+    int fixed_mode = -1;    // >= 0: always this launch mode (kernels with per-workgroup slots)
 };
 
 // Caller-supplied work items (spr_lw3_pairs): partial[item] receives the item's sum.
@@ -102,6 +103,8 @@ struct Ctx {
     int *d_item_q0 = nullptr, *d_item_off = nullptr, *d_item_cnt = nullptr, *d_item_first = nullptr;
     std::vector<int> h_item_first;
     float *omega32 = nullptr;      // (3, nq, nd): omega, 1/omega, nBE in FP32 for the MFMA kernel
+    float *mfma_scratch = nullptr; // FP32 MFMA kernel: one U2 stash slot per resident workgroup
+    size_t mfma_scratch_bytes = 0;
     int ncu = 256;                 // compute units of the device
     int last_mode = -1;            // launch mode of the last item launch (0: one WG per item)
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;

@@ -38,30 +38,59 @@ class Phonons:
 
 
 def dynamical_matrix(cr: Crystal, fc2: IFC, q_cart: np.ndarray, with_derivative=True):
-    """D(q) and dD/dq for q (nq, 3) in 1/Angstrom; rows (mu,a), columns (nu,b)."""
+    """D(q) and dD/dq for q (nq, 3) in 1/Angstrom; rows (mu,a), columns (nu,b).
+
+    Every neighbour term is scattered into a dense (terms, ndof^2) table once, so D is one
+    complex matrix product phases(q, terms) @ table (and dD/dq_k the same with i R_k)."""
     nb = cr.nb
     nq = q_cart.shape[0]
     ndof = 3 * nb
-    D = np.zeros((nq, ndof, ndof), complex)
-    dD = np.zeros((3, nq, ndof, ndof), complex) if with_derivative else None
     osm = cr.one_by_sqrt_mass
+    rows, Rs = [], []
     for mu in range(nb):
         for n in range(int(fc2.atm_num[mu])):
             cell = fc2.atm_indx[mu, n, :3].astype(float)
             nu = int(fc2.atm_indx[mu, n, 3])
-            R = cr.latvec @ cell
-            ph = np.exp(1j * (q_cart @ R))                      # (nq,)
-            blk = fc2.fc[mu, n] * (osm[mu] * osm[nu])           # (a, b)
-            D[:, 3 * mu:3 * mu + 3, 3 * nu:3 * nu + 3] += ph[:, None, None] * blk[None]
-            if with_derivative:
-                for k in range(3):
-                    dD[k, :, 3 * mu:3 * mu + 3, 3 * nu:3 * nu + 3] += \
-                        (1j * R[k] * ph)[:, None, None] * blk[None]
+            blk = np.zeros((ndof, ndof))
+            blk[3 * mu:3 * mu + 3, 3 * nu:3 * nu + 3] = fc2.fc[mu, n] * (osm[mu] * osm[nu])
+            rows.append(blk.ravel())
+            Rs.append(cr.latvec @ cell)
+    table = np.array(rows)                                    # (terms, ndof^2)
+    R = np.array(Rs)                                          # (terms, 3)
+    ph = np.exp(1j * (q_cart @ R.T))                          # (nq, terms)
+    D = (ph @ table).reshape(nq, ndof, ndof)
+    dD = None
+    if with_derivative:
+        dD = np.stack([((1j * ph * R[None, :, k]) @ table).reshape(nq, ndof, ndof) for k in range(3)])
     return D, dD
 
 
+def _solve_chunk(cr, fc2, q_cart, q_int, T):
+    D, dD = dynamical_matrix(cr, fc2, q_cart)
+    lam, V = np.linalg.eigh(D, UPLO='U')                        # V[q, row, s]; zheevd uplo='U'
+    neg = (np.abs(lam) > C.EPS) & (lam < 0.0)
+    w = np.sqrt(np.abs(lam))
+    w = np.where(neg, -w, w)
+    gamma = np.all(q_int == 0, axis=1)
+    # <e|dD/dq_k|e> for every mode: (q, s, k)
+    cv = np.transpose((np.conj(V)[None] * (dD @ V[None])).sum(axis=2), (1, 2, 0))
+    with np.errstate(divide='ignore', invalid='ignore'):
+        v = np.real(cv) / (2.0 * w[:, :, None]) * C.VG_MS
+        n = 1.0 / (np.exp(C.BE_CONV * np.abs(w) / T) - 1.0)
+        iw = 1.0 / (w * C.W_THZ)
+    acoustic = np.zeros_like(w, dtype=bool)
+    acoustic[gamma, :3] = True
+    v[acoustic] = 0.0
+    n[acoustic] = 0.0
+    iw[acoustic] = 0.0
+    return np.transpose(V, (0, 2, 1)), w * C.W_THZ, iw, n, v
+
+
 def solve(cr: Crystal, fc2: IFC, q_cart: np.ndarray, q_int: np.ndarray, T: float,
-          chunk: int = 4096) -> Phonons:
+          chunk: int = 512, threads: int | None = None) -> Phonons:
+    """Phonons of every q, in chunks spread over host threads (LAPACK releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    import os
     nq = q_cart.shape[0]
     ndof = cr.ndof
     evec = np.zeros((nq, ndof, ndof), complex)
@@ -69,29 +98,18 @@ def solve(cr: Crystal, fc2: IFC, q_cart: np.ndarray, q_int: np.ndarray, T: float
     inv_omega = np.zeros((nq, ndof))
     nbe = np.zeros((nq, ndof))
     vel = np.zeros((nq, ndof, 3))
-    for lo in range(0, nq, chunk):
+    los = list(range(0, nq, chunk))
+
+    def one(lo):
         hi = min(nq, lo + chunk)
-        D, dD = dynamical_matrix(cr, fc2, q_cart[lo:hi])
-        lam, V = np.linalg.eigh(D, UPLO='U')                    # V[q, row, s]; zheevd uplo='U'
-        neg = (np.abs(lam) > C.EPS) & (lam < 0.0)
-        w = np.sqrt(np.abs(lam))
-        w = np.where(neg, -w, w)
-        gamma = np.all(q_int[lo:hi] == 0, axis=1)
-        # <e|dD|e> for every mode
-        ev = np.einsum('kqrc,qcs->kqrs', dD, V)
-        cv = np.einsum('qrs,kqrs->qsk', np.conj(V), ev)          # (q, s, 3)
-        with np.errstate(divide='ignore', invalid='ignore'):
-            v = np.real(cv) / (2.0 * w[:, :, None]) * C.VG_MS
-            n = 1.0 / (np.exp(C.BE_CONV * np.abs(w) / T) - 1.0)
-            iw = 1.0 / (w * C.W_THZ)
-        acoustic = np.zeros_like(w, dtype=bool)
-        acoustic[gamma, :3] = True
-        v[acoustic] = 0.0
-        n[acoustic] = 0.0
-        iw[acoustic] = 0.0
-        evec[lo:hi] = np.transpose(V, (0, 2, 1))
-        omega[lo:hi] = w * C.W_THZ
-        inv_omega[lo:hi] = iw
-        nbe[lo:hi] = n
-        vel[lo:hi] = v
+        evec[lo:hi], omega[lo:hi], inv_omega[lo:hi], nbe[lo:hi], vel[lo:hi] = \
+            _solve_chunk(cr, fc2, q_cart[lo:hi], q_int[lo:hi], T)
+
+    nt = threads or max(1, min(16, os.cpu_count() or 1, len(los)))
+    if nt == 1:
+        for lo in los:
+            one(lo)
+    else:
+        with ThreadPoolExecutor(nt) as ex:
+            list(ex.map(one, los))
     return Phonons(evec, omega, inv_omega, nbe, vel, float(T))

@@ -40,6 +40,7 @@ module spring_gpu_m
         type(c_ptr)     :: irr_grid, pos, multiplicity
         integer(c_int)  :: nsym
         type(c_ptr)     :: rot
+        type(c_ptr)     :: tetra
     end type spr_mesh
 
     type, bind(C) :: spr_phonons
@@ -97,6 +98,20 @@ module spring_gpu_m
             real(c_double)          :: lw(*)
         end function spr_lw3_all
 
+        integer(c_int) function spr_lw3_tetra(h, i0, lw, p3) bind(C, name="spr_lw3_tetra")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            integer(c_int), value   :: i0
+            real(c_double)          :: lw(*), p3(*)
+        end function spr_lw3_tetra
+
+        integer(c_int) function spr_lw3_tetra_all(h, lw, p3) bind(C, name="spr_lw3_tetra_all")
+            import :: c_int, c_ptr
+            import :: c_double
+            type(c_ptr), value      :: h
+            real(c_double)          :: lw(*), p3(*)
+        end function spr_lw3_tetra_all
+
         integer(c_int) function spr_lw4_all(h, sigma, prec, lw) bind(C, name="spr_lw4_all")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value      :: h
@@ -121,7 +136,8 @@ module spring_gpu_m
     end interface
 
     public :: gpu_ctx_create, gpu_ctx_destroy, ScatterProbAllq1_gpu, CalculateLinewidth_gpu, &
-            & LinewidthAllq0_gpu, FourPhononLw_gpu, kappa_noIter_gpu, spr_error_text
+            & LinewidthAllq0_gpu, FourPhononLw_gpu, kappa_noIter_gpu, spr_error_text, &
+            & CalculateLwTetrahedron_gpu
 
 contains
 
@@ -152,7 +168,7 @@ contains
     subroutine gpu_ctx_create(ctx, Nbasis, mass, vol, atmNum3, atmIndx3, fC3, &
                             & mesh, q_pnt_int, irr_grid, pos, multiplicity, Rxyz, &
                             & Evec, omega, OnebyOmega, nBE, grp_vel, device, &
-                            & atmNum4, atmIndx4, fC4)
+                            & atmNum4, atmIndx4, fC4, tetrahedrons)
         type(gpu_ctx), intent(out)                          :: ctx
         integer, intent(in)                                 :: Nbasis, device
         real(dp), intent(in), target                        :: mass(:), vol
@@ -166,9 +182,10 @@ contains
                                                              & grp_vel(:,:,:)
         integer, intent(in), optional                       :: atmNum4(:), atmIndx4(:,:,:)
         real(dp), intent(in), target, optional              :: fC4(:,:,:,:,:,:,:,:)
+        integer, intent(in), optional                       :: tetrahedrons(:,:)   ! (4, 6 Nq), 1-based
 
         integer(c_int), allocatable, target, save   :: num3(:), indx3(:,:,:), num4(:), indx4(:,:,:)
-        integer(c_int), allocatable, target, save   :: qint(:,:), irr0(:), pos0(:), mult(:)
+        integer(c_int), allocatable, target, save   :: qint(:,:), irr0(:), pos0(:), mult(:), tet0(:,:)
         real(dp), allocatable, target, save         :: mass_c(:)
         type(spr_crystal)                           :: cr
         type(spr_ifc), target                       :: f3, f4
@@ -197,7 +214,11 @@ contains
         mult = multiplicity
         ms = spr_mesh(int(mesh, c_int), int(size(q_pnt_int, 2), c_int), c_loc(qint), &
                     & int(size(irr_grid), c_int), c_loc(irr0), c_loc(pos0), c_loc(mult), &
-                    & int(size(Rxyz, 3), c_int), c_loc(Rxyz))
+                    & int(size(Rxyz, 3), c_int), c_loc(Rxyz), c_null_ptr)
+        if ( present(tetrahedrons) ) then
+            tet0 = tetrahedrons - 1
+            ms%tetra = c_loc(tet0)
+        end if
         ph = spr_phonons(int(size(omega, 2), c_int), int(size(omega, 1), c_int), c_loc(Evec), &
                        & c_loc(omega), c_loc(OnebyOmega), c_loc(nBE), c_loc(grp_vel))
         rc = spr_ctx_create(cr, f3, p4, ms, ph, int(device, c_int), ctx%h)
@@ -239,6 +260,17 @@ contains
         real(dp), intent(out)       :: LW_Allq0(:,:)
         call check(spr_lw3_all(ctx%h, Sigma, int(prec, c_int), LW_Allq0), 'LinewidthAllq0_gpu')
     end subroutine LinewidthAllq0_gpu
+
+    ! Tetrahedron-method linewidth lw(Ndof) and P3_q0(Ndof) of irreducible q0 number i0 (1-based),
+    ! the outputs of CalculateLwTetrahedron (ThirdOrder_m.f90:169-188); P3 = sum(P3_q0).
+    ! Needs the tetrahedra at gpu_ctx_create.
+    subroutine CalculateLwTetrahedron_gpu(ctx, i0, lw, P3_q0, P3)
+        type(gpu_ctx), intent(in)   :: ctx
+        integer, intent(in)         :: i0
+        real(dp), intent(out)       :: lw(:), P3_q0(:), P3
+        call check(spr_lw3_tetra(ctx%h, int(i0 - 1, c_int), lw, P3_q0), 'CalculateLwTetrahedron_gpu')
+        P3 = sum(P3_q0)
+    end subroutine CalculateLwTetrahedron_gpu
 
     ! Raw four-phonon sum lw4(Ndof, Nq0); the caller adds 0.5*lw4 (main.f90:660).
     subroutine FourPhononLw_gpu(ctx, Sigma, prec, lw4)

@@ -70,6 +70,8 @@ typedef struct {
     const int *multiplicity;    /* multiplicity(nq0) */
     int nsym;                   /* point operations used by kappa symm (incl. time reversal) */
     const double *rot;          /* Rxyz(3,3,nsym) Cartesian rotations; may be NULL if nsym==0 */
+    const int *tetra;           /* tetrahedrons(4, 6 nq) grid indices (TetrahedronQ_m.f90:172-188);
+                                   may be NULL (then spr_lw3_tetra* fail) */
 } spr_mesh;
 
 /* harmonic phonons on the full mesh (Phon, phonon_mod.f90:33-46) */
@@ -108,6 +110,14 @@ int spr_lw3(spr_ctx *ctx, int i0, double sigma, int prec, double *lw);
 /* All irreducible q0 in one persistent launch (the q0 loop of main.f90:552-715 restricted to
  * three-phonon scattering): LW_Allq0(Ndof, nq0). */
 int spr_lw3_all(spr_ctx *ctx, double sigma, int prec, double *lw_all);
+
+/* Tetrahedron-method three-phonon linewidth of irreducible q0 `i0` (CalculateLwTetrahedron,
+ * ThirdOrder_m.f90:169-275, the reference's default delta): lw(Ndof) rad/ps and P3_q0(Ndof);
+ * the reference's P3 of the q0 is sum(P3_q0).  W of every q1 comes from the FP64 W path of
+ * spr_v3_w; FP64 throughout.  Needs mesh->tetra. */
+int spr_lw3_tetra(spr_ctx *ctx, int i0, double *lw, double *p3_q0);
+/* every irreducible q0: lw_all(Ndof, nq0), p3_all(Ndof, nq0) */
+int spr_lw3_tetra_all(spr_ctx *ctx, double *lw_all, double *p3_all);
 
 /* Four-phonon Gaussian linewidth sum of irreducible q0 `i0` over all (q1,q2), the processes
  * and weights of TetrahedronIntegration4th (Tetrahedron4th_sep.f90:242-343) with a Gaussian

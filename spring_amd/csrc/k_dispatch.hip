@@ -11,13 +11,13 @@ int launch_lw3_generic(Ctx &c, const int *d_q0, int nq0, double sigma, int prec,
                        double *d_w);
 
 int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw);
-int launch_lw3_mfma(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw);
+int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw);
 
 // Kernel choice.
 //   W parity path (FP64 W of every q1)          generic kernel
-//   FP64:  Nb <= 2 small-cell batch kernel, else generic kernel
+//   FP64:  Nb <= 2 small-cell batch kernel, else MFMA kernel (FP64 MFMA)
 //   FP32:  Nb <= 2 small-cell batch kernel (FP32 VALU), else MFMA kernel (FP32 MFMA)
-//   BF16:  MFMA kernel (BF16 MFMA)
+//   BF16:  MFMA kernel (split-BF16 MFMA)
 // SPR_KERNEL=generic|small|mfma overrides the choice where the variant supports the mode.
 int launch_lw3(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw,
                double *d_w_out) {
@@ -26,13 +26,11 @@ int launch_lw3(Ctx &c, int q0_first, int q0_count, double sigma, int prec, doubl
     const std::string force = f ? f : "";
     if (force == "generic" && prec != SPR_PREC_BF16)
         return launch_lw3_generic(c, c.irr_grid + q0_first, q0_count, sigma, prec, d_lw, nullptr);
-    if (force == "mfma" && prec != SPR_PREC_FP64) return launch_lw3_mfma(c, q0_first, q0_count, sigma, prec, d_lw);
+    if (force == "mfma") return launch_lw3_mm(c, q0_first, q0_count, sigma, prec, d_lw);
     if (force == "small" && c.nb <= 2 && prec != SPR_PREC_BF16)
         return launch_lw3_small(c, q0_first, q0_count, sigma, prec, d_lw);
-    if (prec == SPR_PREC_BF16) return launch_lw3_mfma(c, q0_first, q0_count, sigma, prec, d_lw);
-    if (c.nb <= 2) return launch_lw3_small(c, q0_first, q0_count, sigma, prec, d_lw);
-    if (prec == SPR_PREC_FP32) return launch_lw3_mfma(c, q0_first, q0_count, sigma, prec, d_lw);
-    return launch_lw3_generic(c, c.irr_grid + q0_first, q0_count, sigma, prec, d_lw, nullptr);
+    if (prec != SPR_PREC_BF16 && c.nb <= 2) return launch_lw3_small(c, q0_first, q0_count, sigma, prec, d_lw);
+    return launch_lw3_mm(c, q0_first, q0_count, sigma, prec, d_lw);
 }
 
 
@@ -45,10 +43,6 @@ namespace spr {
 int launch_lw3_pairs(Ctx &c, int n, const int *q0_index, const int *q1_grid, double sigma, int prec,
                      double *h_out) {
     if (n < 1) return SPR_OK;
-    if (prec == SPR_PREC_FP64 && c.nb > 2) {
-        set_error("spr_lw3_pairs: FP64 needs Nb <= 2 (FP32/BF16 use the MFMA kernel)");
-        return SPR_E_UNSUPPORTED;
-    }
     int rc;
     if ((rc = ensure_pairs(c))) return rc;
     std::vector<int> iq0(n), ioff(n), icnt(n, 1);
@@ -78,7 +72,7 @@ int launch_lw3_pairs(Ctx &c, int n, const int *q0_index, const int *q1_grid, dou
     };
     if (hipMalloc((void **)&d_i, 3 * (size_t)n * sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&d_p, (size_t)n * sizeof(int2)) != hipSuccess ||
-        hipMalloc((void **)&d_out, (size_t)n * c.nd * sizeof(double)) != hipSuccess) {
+        hipMalloc((void **)&d_out, (size_t)n * kMaxUnitsPerItem * c.nd * sizeof(double)) != hipSuccess) {
         cleanup();
         set_error("spr_lw3_pairs: hipMalloc failed");
         return SPR_E_OOM;
@@ -93,16 +87,24 @@ int launch_lw3_pairs(Ctx &c, int n, const int *q0_index, const int *q1_grid, dou
     }
     ItemSet set{d_i, d_i + n, d_i + 2 * n, d_p, n, d_out};
     c.override_items = &set;
-    rc = (prec == SPR_PREC_BF16 || (prec == SPR_PREC_FP32 && c.nb > 2))
-             ? launch_lw3_mfma(c, 0, 0, sigma, prec, nullptr)
-             : launch_lw3_small(c, 0, 0, sigma, prec, nullptr);
+    rc = (prec == SPR_PREC_BF16 || c.nb > 2) ? launch_lw3_mm(c, 0, 0, sigma, prec, nullptr)
+                                             : launch_lw3_small(c, 0, 0, sigma, prec, nullptr);
     c.override_items = nullptr;
-    if (!rc && hipMemcpy(h_out, d_out, (size_t)n * c.nd * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+    // every unit of a pair wrote its own partial row: rows k * upi + h, h < upi
+    const int upi = c.last_upi;
+    std::vector<double> rows((size_t)n * upi * c.nd);
+    if (!rc && hipMemcpy(rows.data(), d_out, rows.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
         set_error("spr_lw3_pairs: download failed");
         rc = SPR_E_HIP;
     }
     cleanup();
     if (rc) return rc;
+    for (int k = 0; k < n; ++k)
+        for (int s = 0; s < c.nd; ++s) {
+            double t = 0.0;
+            for (int h = 0; h < upi; ++h) t += rows[((size_t)k * upi + h) * c.nd + s];
+            h_out[(size_t)k * c.nd + s] = t;
+        }
     for (int k = 0; k < n; ++k)
         if (pr[k].x != pr[k].y)
             for (int s = 0; s < c.nd; ++s) h_out[(size_t)k * c.nd + s] *= 0.5;   // kernels weight q1 != q2 by 2

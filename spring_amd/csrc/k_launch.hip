@@ -77,7 +77,7 @@ static int enqueue(Ctx &c, ItemLaunch &L, int mode, int q0_first, int q0_count, 
     void *args[5] = {L.args, L.extra[0], L.extra[1], L.extra[2], L.extra[3]};
     SPR_HIP(hipLaunchKernel(L.func, dim3(grid_for(c, L, mode)), dim3(L.block), args, L.dyn, c.stream));
     if (timed) SPR_HIP(hipEventRecord(c.ev1, c.stream));
-    if (d_lw) return launch_reduce_items(c, q0_first, q0_count, it0, d_lw);
+    if (d_lw) return launch_reduce_items(c, q0_first, q0_count, it0, L.upi, d_lw);
     return SPR_OK;
 }
 
@@ -97,11 +97,12 @@ static int time_mode(Ctx &c, ItemLaunch &L, int mode, int sub, float *ms) {
 
 int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double *d_lw) {
     int rc;
+    c.last_upi = L.upi;
     if (L.dyn > 64 * 1024)
         SPR_HIP(hipFuncSetAttribute(L.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.dyn));
     // ---- launch mode: env override, tuned value, or default (persistent 2 x CUs) ----
     if (L.direct) {
-        if ((rc = enqueue(c, L, L.fixed_mode >= 0 ? L.fixed_mode : 0, 0, 0, 0, nullptr, true))) return rc;
+        if ((rc = enqueue(c, L, 0, 0, 0, 0, nullptr, true))) return rc;
         SPR_HIP(hipEventSynchronize(c.ev1));
         return SPR_OK;
     }
@@ -110,9 +111,7 @@ int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double
                             std::to_string(L.nitems);
     const char *force = getenv("SPR_LAUNCH_MODE");
     const char *at = getenv("SPR_AUTOTUNE");
-    if (L.fixed_mode >= 0) {
-        mode = L.fixed_mode;
-    } else if (force && *force) {
+    if (force && *force) {
         mode = atoi(force);
     } else if (tune_table().count(key)) {
         mode = tune_table()[key];

@@ -1,0 +1,576 @@
+// Three-phonon Gaussian linewidths for cells with Nb >= 3 (Ndof = 3 Nb up to 30) on the
+// matrix cores, in FP64, FP32 or split BF16.
+//
+// Work unit: a batch of up to 64 (q1, q2) pairs of one irreducible q0 (pair-permutation
+// reduced list, ScatterProbAllq1 ThirdOrder_m.f90:98-164) and one block of s2 branches (the
+// s2 split keeps V of a unit within 64 accumulator registers per lane: Ndof = 30 takes two
+// blocks in FP32/BF16 and four in FP64; only the FT is repeated per block).  One workgroup
+// (8 waves) runs the pairs of its unit one after another.  Per pair, per basis atom mu
+// (rows a = 3 mu + alpha):
+//   FT      T(alpha, b, c) = sum_{(N1,N2) of mu} Phi(alpha,beta,gamma) e^{2 pi i m/L}   (VALU)
+//   GEMM B  U1(alpha, b, s2) = sum_c  T(alpha, b, c) e2(c, s2)
+//   GEMM C  U2(alpha, s1, s2) = sum_b U1(alpha, b, s2) e1(b, s1)
+//   GEMM D  V(s1 s2, s0) += sum_alpha U2(alpha, s1 s2) e0(3 mu + alpha, s0)
+// with V held in the accumulator registers of the waves across mu (ScatterMatEl,
+// ThirdOrder_m.f90:278-386, factorised; no eigenvector is conjugated).  After the last mu,
+// W = |V|^2 /(w0 w1 w2) Sctr3THz2/Nq and the occupation-weighted Gaussians
+// (CalculateLinewidth, ThirdOrder_m.f90:514-537) are summed per s0.
+//
+// Complex GEMMs run as real GEMMs: with re/im interleaved along K and N,
+//   [Y_re Y_im] = [X_re X_im] x [[E_re, E_im], [-E_im, E_re]],
+// the A operand is the interleaved intermediate (rows in LDS, K contiguous) and the B operand
+// is expanded from an eigenvector matrix held as complex E[s][c] in LDS.
+// Matrix instructions (16x16 output tiles, lane l owns column l%16 and four rows):
+//   FP64   v_mfma_f64_16x16x4_f64: A lane l -> A[l%16][k], B lane l -> B[k][l%16]; lane group
+//          g = l/16 walks K in its own contiguous quarter (any consistent K order sums the
+//          same), so each lane's fragment is one contiguous LDS vector.  Output rows of lane l:
+//          4 r + l/16 (r = 0..3; measured on gfx950).
+//   FP32   v_mfma_f32_16x16x4_f32, same operand scheme; output rows 4 (l/16) + r.
+//   BF16   v_mfma_f32_16x16x32_bf16 (lane l: A[l%16][8 g + t], B[8 g + t][l%16], t < 8) on
+//          split operands x = hi + lo (hi = bf16(x), lo = bf16(x - hi)): X Y ~ Xh Yh + Xh Yl +
+//          Xl Yh, FP32 accumulation, so GEMMs B and C keep about 16 mantissa bits; GEMM D
+//          (K = 3 complex per atom) runs on the FP32 instruction.
+// Delta arguments w0 -+ w1 -+ w2 are formed in FP64 in every mode; FP32 and BF16 sum the
+// delta-weighted terms with a compensated FP32 sum per lane; the per-s0 totals are reduced in
+// FP64 in a fixed order (bitwise reproducible).
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+#include "cplx.h"
+#include "spr_internal.h"
+
+namespace spr {
+
+enum { MM_F64 = 0, MM_F32 = 1, MM_BF16 = 2 };
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct MmArgs {
+    ItemCommon ic;
+    const int *item_q0, *item_off, *item_cnt;
+    const int2 *pairs;
+    const int *q0_grid;
+    int L, m0, m1;
+    const int *blk_start, *tup_site;
+    const void *fc;              // (nterm, 27) R
+    const int4 *site;
+    int nsite;
+    const void *phase;           // (L) complex R
+    const void *evec;            // (nq, nd, nd) complex R
+    const double *omega, *inv_omega, *nbe;
+    double sigma, factor;
+    int npmax;                   // most clusters of one basis atom
+};
+
+constexpr int pow2ceil(int x) { return x <= 1 ? 1 : x <= 2 ? 2 : x <= 4 ? 4 : 8; }
+
+// V tiles per wave when each unit covers s2 blocks of 8 nts values
+constexpr int v_tiles(int nd, int nt, int nts) { return ((nd * 8 * nts + 15) / 16 * pow2ceil(nt) + 7) / 8; }
+// largest s2 block (in N tiles of 8 s2) whose V accumulators fit 64 registers per lane
+constexpr int pick_nts(int nd, int nt, int words) {
+    return v_tiles(nd, nt, nt) * words <= 64 ? nt : v_tiles(nd, nt, 2) * words <= 64 && nt >= 2 ? 2 : 1;
+}
+
+template <int MODE, int NB> struct MmShape {
+    using R = std::conditional_t<MODE == MM_F64, double, float>;
+    using Z = cplx_t<R>;
+    using ACC = std::conditional_t<MODE == MM_F64, v4d, v4f>;
+    static constexpr int ND = 3 * NB, ND2 = ND * ND, K2 = 2 * ND;
+    static constexpr int KP = MODE == MM_BF16 ? ((K2 + 31) / 32) * 32 : ((K2 + 15) / 16) * 16;
+    static constexpr int S = KP / 4;                        // K per lane group (FP64 / FP32)
+    static constexpr int PA = KP + (MODE == MM_F64 ? 2 : 4);    // A row pitch (R)
+    static constexpr int PE = KP / 2 + 2;                   // eigenvector row pitch (complex)
+    static constexpr int NT = (K2 + 15) / 16;               // N tiles of any (s, q) column set
+    static constexpr int MT_B = (3 * ND + 15) / 16;         // GEMM B: rows (alpha, b)
+    static constexpr int MB = MT_B * 16;
+    // s2 split: a unit covers s2 in [h SB, (h + 1) SB), i.e. NTS N tiles of GEMM B
+    static constexpr int NTS = pick_nts(ND, NT, MODE == MM_F64 ? 8 : 4);
+    static constexpr int SB = 8 * NTS;
+    static constexpr int NSPLIT = (NT + NTS - 1) / NTS;
+    static constexpr int NTSB = pow2ceil(NTS);              // waves per M tile, GEMM B
+    static constexpr int MT_C = (3 * SB + 15) / 16;         // GEMM C: rows (alpha, s2 local)
+    static constexpr int MC = MT_C * 16;
+    static constexpr int NTC = pow2ceil(NT);                // waves per M tile, GEMMs C and D
+    static constexpr int MT_D = (ND * SB + 15) / 16;        // GEMM D: rows (s1, s2 local)
+    static constexpr int VT = (MT_D * NTC + 7) / 8;         // V tiles per wave
+    static_assert(NT <= 4, "Ndof must be <= 32");
+    static_assert(NSPLIT <= kMaxUnitsPerItem, "too many s2 blocks");
+};
+
+// LDS layout (bytes), shared by the kernel and the host-side size check
+template <int MODE, int NB> struct MmLds {
+    using Sh = MmShape<MODE, NB>;
+    using R = typename Sh::R;
+    using Z = typename Sh::Z;
+    static constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
+    static constexpr size_t TU = cmax((size_t)Sh::MB * Sh::PA * sizeof(R), (size_t)Sh::MT_D * 16 * 3 * sizeof(Z));
+    static constexpr size_t AC = cmax((size_t)Sh::MC * Sh::PA * sizeof(R), (size_t)512 * sizeof(double));
+    static constexpr size_t E = (size_t)Sh::ND * Sh::PE * sizeof(Z);
+    __host__ __device__ static constexpr size_t tu() { return TU; }
+    __host__ __device__ static constexpr size_t ac() { return AC; }
+    __host__ __device__ static constexpr size_t e() { return E; }
+    static size_t total(int npmax, int nsite, int L) {
+        size_t b = tu() + ac() + 2 * e();
+        b += (size_t)npmax * sizeof(Z) + (size_t)L * sizeof(Z) + 8 * Sh::ND * sizeof(double);
+        b += 2 * (size_t)nsite * sizeof(int);
+        return (b + 15) & ~(size_t)15;
+    }
+};
+
+// output row (within a 16 x 16 tile) of accumulator element r in lane group g
+template <bool F64OUT>
+__device__ __forceinline__ int tile_row(int g, int r) { return F64OUT ? 4 * r + g : 4 * g + r; }
+
+// 16 x 16 output tile of A (rows in LDS, K contiguous, pitch PA) times the B' expansion of
+// the complex eigenvector matrix E (LDS, [s][c] pitch PE): acc[m][n'] += sum_k' A[m][k'] B'[k'][n'].
+// The B' fragment is built once per call site and passed in `bf`.
+template <int MODE, int NB>
+struct Frag {
+    using Sh = MmShape<MODE, NB>;
+    using R = typename Sh::R;
+    using Z = typename Sh::Z;
+    // FP64 / FP32: S values per lane; BF16: KP/32 slabs of (hi, lo) bf16x8
+    static constexpr int NSLAB = Sh::KP / 32;
+    struct BF {
+        std::conditional_t<MODE == MM_BF16, bf16x8, R> h[MODE == MM_BF16 ? NSLAB : Sh::S];
+        bf16x8 lo[MODE == MM_BF16 ? NSLAB : 1];
+    };
+
+    // B'[k'][n'] with n' = (s, q), k' = (c, p): p == q -> Re E(c,s); p=0,q=1 -> Im; p=1,q=0 -> -Im
+    __device__ static BF load_b(const Z *sE, int n_base) {
+        const int l = threadIdx.x & 63, g = l >> 4;
+        const int np = n_base + (l & 15);
+        const int s = np >> 1, q = np & 1;
+        const bool ok = np < Sh::K2;
+        BF b;
+        if constexpr (MODE != MM_BF16) {
+#pragma unroll
+            for (int i = 0; i < Sh::S / 2; ++i) {
+                const int c = g * (Sh::S / 2) + i;
+                Z e = ok ? sE[s * Sh::PE + c] : Z{0, 0};
+                b.h[2 * i] = q ? e.y : e.x;          // p = 0
+                b.h[2 * i + 1] = q ? e.x : -e.y;     // p = 1
+            }
+        } else {
+#pragma unroll
+            for (int sl = 0; sl < NSLAB; ++sl) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int c = 16 * sl + 4 * g + i;
+                    Z e = ok ? sE[s * Sh::PE + c] : Z{0, 0};
+                    const float v0 = q ? e.y : e.x, v1 = q ? e.x : -e.y;
+                    const __bf16 h0 = (__bf16)v0, h1 = (__bf16)v1;
+                    b.h[sl][2 * i] = h0;
+                    b.h[sl][2 * i + 1] = h1;
+                    b.lo[sl][2 * i] = (__bf16)(v0 - (float)h0);
+                    b.lo[sl][2 * i + 1] = (__bf16)(v1 - (float)h1);
+                }
+            }
+        }
+        return b;
+    }
+
+    __device__ static typename Sh::ACC mma(const R *sA, int m_base, const BF &b, typename Sh::ACC acc) {
+        const int l = threadIdx.x & 63, g = l >> 4;
+        const R *row = sA + (m_base + (l & 15)) * Sh::PA;
+        if constexpr (MODE == MM_F64) {
+            const double2 *p = reinterpret_cast<const double2 *>(row + g * Sh::S);
+#pragma unroll
+            for (int i = 0; i < Sh::S / 2; ++i) {
+                const double2 a = p[i];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.h[2 * i], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.h[2 * i + 1], acc, 0, 0, 0);
+            }
+        } else if constexpr (MODE == MM_F32) {
+            const float4 *p = reinterpret_cast<const float4 *>(row + g * Sh::S);
+#pragma unroll
+            for (int i = 0; i < Sh::S / 4; ++i) {
+                const float4 a = p[i];
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.h[4 * i], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.h[4 * i + 1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.h[4 * i + 2], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.h[4 * i + 3], acc, 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int sl = 0; sl < NSLAB; ++sl) {
+                const float4 *p = reinterpret_cast<const float4 *>(row + 32 * sl + 8 * g);
+                const float4 x0 = p[0], x1 = p[1];
+                const float xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+                bf16x8 ah, al;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const __bf16 h = (__bf16)xs[t];
+                    ah[t] = h;
+                    al[t] = (__bf16)(xs[t] - (float)h);
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b.h[sl], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b.lo[sl], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b.h[sl], acc, 0, 0, 0);
+            }
+        }
+        return acc;
+    }
+};
+
+template <int MODE, int NB>
+__device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
+    using Sh = MmShape<MODE, NB>;
+    using L = MmLds<MODE, NB>;
+    using R = typename Sh::R;
+    using Z = typename Sh::Z;
+    using ACC = typename Sh::ACC;
+    using F = Frag<MODE, NB>;
+    constexpr int ND = Sh::ND, K2 = Sh::K2, PA = Sh::PA, PE = Sh::PE, SB = Sh::SB;
+    // GEMM D runs on the FP64 (FP64 mode) or FP32 instruction
+    using DACC = std::conditional_t<MODE == MM_F64, v4d, v4f>;
+    using DR = std::conditional_t<MODE == MM_F64, double, float>;
+
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned char *p = smem;
+    R *sT = reinterpret_cast<R *>(p);                 // T rows (alpha, b); then U2 (aliased)
+    Z *sU2 = reinterpret_cast<Z *>(p);                //   U2[(s1, s2 local)][alpha], 3 per row
+    p += L::tu();
+    R *sAC = reinterpret_cast<R *>(p);                // U1 as rows (alpha, s2 local), K = (b, p)
+    double *sRed = reinterpret_cast<double *>(p);     //   (end of unit only)
+    p += L::ac();
+    Z *sE1 = reinterpret_cast<Z *>(p);
+    p += L::e();
+    Z *sE2 = reinterpret_cast<Z *>(p);
+    p += L::e();
+    Z *sP = reinterpret_cast<Z *>(p);
+    p += (size_t)A.npmax * sizeof(Z);
+    Z *sPh = reinterpret_cast<Z *>(p);
+    p += (size_t)A.L * sizeof(Z);
+    double *sMode = reinterpret_cast<double *>(p);    // w0 iw0f | w1 n1 iw1 | w2 n2 iw2 | -
+    p += 8 * ND * sizeof(double);
+    int *sM1 = reinterpret_cast<int *>(p);
+    int *sM2 = sM1 + A.nsite;
+
+    const int tid = threadIdx.x, l = tid & 63, g = l >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int item = unit / Sh::NSPLIT, h = unit % Sh::NSPLIT;
+    const int qi = A.item_q0[item];
+    const int i0 = A.q0_grid[qi];
+    const int cnt = A.item_cnt[item];
+    const int poff = A.item_off[item];
+    const R *fc = reinterpret_cast<const R *>(A.fc);
+    const Z *phase = reinterpret_cast<const Z *>(A.phase);
+    const Z *evec = reinterpret_cast<const Z *>(A.evec);
+    __syncthreads();   // the previous unit of a persistent workgroup is done with LDS
+
+    // ---- per unit: zero padding, phase table, q0 modes ----
+    for (int i = tid; i < Sh::MC * PA; i += 512) sAC[i] = (R)0;
+    for (int i = tid; i < ND * PE; i += 512) {
+        if (i % PE >= ND) {
+            sE1[i] = Z{0, 0};
+            sE2[i] = Z{0, 0};
+        }
+    }
+    for (int i = tid; i < A.L; i += 512) sPh[i] = phase[i];
+    if (tid < ND) {
+        sMode[tid] = A.omega[(size_t)i0 * ND + tid];
+        sMode[ND + tid] = A.inv_omega[(size_t)i0 * ND + tid] * A.factor;
+    }
+    const double c0 = 1.0 / (A.sigma * 2.5066282746310002);   // 1/(sigma sqrt(2 pi))
+    const double hinv = 0.5 / (A.sigma * A.sigma);
+
+    // GEMM B: N tiles h NTS + (w % NTSB), M tiles w / NTSB + (8 / NTSB) j
+    const int ntB = h * Sh::NTS + (w % Sh::NTSB);
+    const bool bOK = (w % Sh::NTSB) < Sh::NTS && ntB < Sh::NT;
+    const int mtB0 = w / Sh::NTSB;
+    constexpr int MSTEPB = 8 / Sh::NTSB;
+    // GEMMs C and D: N tile w % NTC, M tiles w / NTC + (8 / NTC) j
+    const int ntC = w % Sh::NTC;
+    const bool cOK = ntC < Sh::NT;
+    const int mtC0 = w / Sh::NTC;
+    constexpr int MSTEPC = 8 / Sh::NTC;
+    const int npD = ntC * 16 + (l & 15);      // this lane's column (s0, q) in GEMM D
+    const int s0 = npD >> 1, qD = npD & 1;
+    const bool s0ok = cOK && s0 < ND;
+
+    KSum<float> accf;
+    double accd = 0.0;
+
+    for (int pi = 0; pi < cnt; ++pi) {
+        const int2 pr = A.pairs[poff + pi];
+        const int q1 = pr.x, q2 = pr.y;
+        const double wpair = q1 == q2 ? 1.0 : 2.0;
+        __syncthreads();   // the previous pair is done with sE1, sE2, sMode, sM1, sM2
+        for (int i = tid; i < ND * ND; i += 512) {
+            const int s = i / ND, c = i % ND;
+            sE1[s * PE + c] = evec[(size_t)q1 * ND * ND + i];
+            sE2[s * PE + c] = evec[(size_t)q2 * ND * ND + i];
+        }
+        if (tid < ND) {
+            sMode[2 * ND + tid] = A.omega[(size_t)q1 * ND + tid];
+            sMode[3 * ND + tid] = A.nbe[(size_t)q1 * ND + tid];
+            sMode[4 * ND + tid] = A.inv_omega[(size_t)q1 * ND + tid];
+            sMode[5 * ND + tid] = A.omega[(size_t)q2 * ND + tid];
+            sMode[6 * ND + tid] = A.nbe[(size_t)q2 * ND + tid];
+            sMode[7 * ND + tid] = A.inv_omega[(size_t)q2 * ND + tid];
+        }
+        {
+            const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
+            const int k20 = q2 % A.m0, k21 = (q2 / A.m0) % A.m1, k22 = q2 / (A.m0 * A.m1);
+            for (int s = tid; s < A.nsite; s += 512) {
+                const int4 c = A.site[s];
+                sM1[s] = (k10 * c.x + k11 * c.y + k12 * c.z) % A.L;
+                sM2[s] = (k20 * c.x + k21 * c.y + k22 * c.z) % A.L;
+            }
+        }
+        DACC vacc[Sh::VT];
+#pragma unroll
+        for (int j = 0; j < Sh::VT; ++j) vacc[j] = DACC{0, 0, 0, 0};
+        __syncthreads();
+
+#pragma unroll 1
+        for (int mu = 0; mu < NB; ++mu) {
+            // ---- cluster phases of atom mu ----
+            const int u0 = A.blk_start[mu * NB * NB], u1 = A.blk_start[(mu + 1) * NB * NB];
+            for (int u = u0 + tid; u < u1; u += 512) {
+                int m = sM1[A.tup_site[2 * u]] + sM2[A.tup_site[2 * u + 1]];
+                m -= m >= A.L ? A.L : 0;
+                sP[u - u0] = sPh[m];
+            }
+            __syncthreads();
+            // ---- FT: T[(alpha, 3 nu + beta)][2 (3 eta + gamma) + p]; lanes run along the 27
+            //      components of a block, so FC loads are contiguous ----
+            for (int e = tid; e < NB * NB * 27; e += 512) {
+                const int bl = e / 27, j = e % 27;
+                const int nu = bl / NB, eta = bl % NB;
+                const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
+                const int blk = (mu * NB + nu) * NB + eta;
+                const int ua = A.blk_start[blk], ub = A.blk_start[blk + 1];
+                R re = 0, im = 0;
+                for (int u = ua; u < ub; ++u) {
+                    const R f = fc[(size_t)u * 27 + j];
+                    const Z ph = sP[u - u0];
+                    re = fma(f, ph.x, re);
+                    im = fma(f, ph.y, im);
+                }
+                R *dst = sT + (al * ND + 3 * nu + be) * PA + 2 * (3 * eta + ga);
+                dst[0] = re;
+                dst[1] = im;
+            }
+            // padding of T (the region holds U2 between the GEMMs)
+            for (int i = tid; i < Sh::MB * (PA - K2); i += 512) {
+                const int m = i / (PA - K2), k = K2 + i % (PA - K2);
+                sT[m * PA + k] = (R)0;
+            }
+            for (int i = tid; i < (Sh::MB - 3 * ND) * K2; i += 512) {
+                const int m = 3 * ND + i / K2, k = i % K2;
+                sT[m * PA + k] = (R)0;
+            }
+            __syncthreads();
+            // ---- GEMM B: U1[(alpha, b)][(s2, q)] = T x E2', s2 in this unit's block
+            //      -> sAC[(alpha, s2 - h SB)][(b, q)] ----
+            if (bOK) {
+                const typename F::BF bf = F::load_b(sE2, ntB * 16);
+                const int np = ntB * 16 + (l & 15);
+                const int s2 = np >> 1, q = np & 1;
+                for (int mt = mtB0; mt < Sh::MT_B; mt += MSTEPB) {
+                    ACC d = F::mma(sT, mt * 16, bf, ACC{0, 0, 0, 0});
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int m = mt * 16 + tile_row<MODE == MM_F64>(g, r);
+                        if (m < 3 * ND && s2 < ND) {
+                            const int al = m / ND, b = m % ND;
+                            sAC[(al * SB + s2 - h * SB) * PA + 2 * b + q] = (R)d[r];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            // ---- GEMM C: U2[(alpha, s2 local)][(s1, q)] = sAC x E1' -> sU2[(s1, s2 local)][alpha] ----
+            if (cOK) {
+                const typename F::BF bf = F::load_b(sE1, ntC * 16);
+                const int np = ntC * 16 + (l & 15);
+                const int s1 = np >> 1, q = np & 1;
+                for (int mt = mtC0; mt < Sh::MT_C; mt += MSTEPC) {
+                    ACC d = F::mma(sAC, mt * 16, bf, ACC{0, 0, 0, 0});
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int m = mt * 16 + tile_row<MODE == MM_F64>(g, r);
+                        if (m < 3 * SB && s1 < ND) {
+                            const int al = m / SB, sl = m % SB;
+                            R *z = reinterpret_cast<R *>(sU2 + (s1 * SB + sl) * 3 + al);
+                            z[q] = (R)d[r];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            // ---- GEMM D: V[(s1, s2 local)][(s0, q)] += U2[.][(alpha, p)] E0'[(alpha, p)][(s0, q)]
+            //      K' = 8: step i, lane group g -> (alpha = g, p = i); alpha = 3 is padding ----
+            if (cOK) {
+                DR b0 = 0, b1 = 0;
+                if (g < 3 && s0 < ND) {
+                    const Z e = evec[((size_t)i0 * ND + s0) * ND + 3 * mu + g];
+                    b0 = (DR)(qD ? e.y : e.x);
+                    b1 = (DR)(qD ? e.x : -e.y);
+                }
+#pragma unroll
+                for (int j = 0; j < Sh::VT; ++j) {
+                    const int mt = mtC0 + MSTEPC * j;
+                    if (mt < Sh::MT_D) {
+                        const int m = mt * 16 + (l & 15);
+                        Z a = Z{0, 0};
+                        if (g < 3) a = sU2[m * 3 + g];
+                        if constexpr (MODE == MM_F64) {
+                            vacc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b0, vacc[j], 0, 0, 0);
+                            vacc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b1, vacc[j], 0, 0, 0);
+                        } else {
+                            vacc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b0, vacc[j], 0, 0, 0);
+                            vacc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b1, vacc[j], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+            __syncthreads();   // sU2 (= sT) is rewritten by the next atom's FT
+        }
+
+        // ---- W and the Gaussian-delta weights; lanes with q = 0 take rows 0,1, q = 1 rows 2,3 ----
+        if (s0ok) {
+            const double w0 = sMode[s0], iw0 = sMode[ND + s0];
+#pragma unroll
+            for (int j = 0; j < Sh::VT; ++j) {
+                const int mt = mtC0 + MSTEPC * j;
+                if (mt >= Sh::MT_D) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const DR x = vacc[j][r];
+                    const DR y = __shfl_xor(x, 1, 64);
+                    if ((r >> 1) != qD) continue;
+                    const int m = mt * 16 + tile_row<MODE == MM_F64>(g, r);
+                    const int s1 = m / SB, s2 = h * SB + m % SB;
+                    if (s1 >= ND || s2 >= ND) continue;
+                    const double w1 = sMode[2 * ND + s1], n1 = sMode[3 * ND + s1], iw1 = sMode[4 * ND + s1];
+                    const double w2 = sMode[5 * ND + s2], n2 = sMode[6 * ND + s2], iw2 = sMode[7 * ND + s2];
+                    const double d1 = w0 - w1 - w2, d2 = w0 + w1 - w2, d3 = w0 - w1 + w2;
+                    if constexpr (MODE == MM_F64) {
+                        const double W = (x * x + y * y) * iw0 * iw1 * iw2;
+                        const double G = (n1 + n2 + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
+                                         (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
+                        accd += wpair * W * G;
+                    } else {
+                        const float W = (x * x + y * y) * (float)(iw0 * iw1 * iw2);
+                        const float g1 = (float)c0 * __expf((float)(-hinv * d1 * d1));
+                        const float g2 = (float)c0 * __expf((float)(-hinv * d2 * d2));
+                        const float g3 = (float)c0 * __expf((float)(-hinv * d3 * d3));
+                        accf.add((float)wpair * W * ((float)(n1 + n2 + 1.0) * g1 + (float)(n1 - n2) * (g2 - g3)));
+                    }
+                }
+            }
+        }
+    }
+    // ---- reduce per s0 in a fixed order: wave w holds the columns of N tile w % NTC ----
+    __syncthreads();
+    sRed[tid] = s0ok ? (MODE == MM_F64 ? accd : accf.value()) : 0.0;
+    __syncthreads();
+    if (tid < ND) {
+        const int s = tid;
+        const int nt = (2 * s) >> 4, sj = (2 * s) & 15;
+        double tot = 0.0;
+        for (int ww = nt; ww < 8; ww += Sh::NTC)
+            for (int gg = 0; gg < 4; ++gg)
+                for (int q = 0; q < 2; ++q) tot += sRed[ww * 64 + gg * 16 + sj + q];
+        A.ic.partial[(size_t)unit * ND + s] = tot;
+    }
+}
+
+template <int MODE, int NB>
+__global__ __launch_bounds__(512) void k_lw3_mm(MmArgs A) {
+    if (!A.ic.counter) {
+        mm_unit<MODE, NB>(A, blockIdx.x);
+        return;
+    }
+    __shared__ int s_unit;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_unit = atomicAdd(A.ic.counter, 1);
+        __syncthreads();
+        const int unit = s_unit;
+        if (unit >= A.ic.nitems) break;
+        mm_unit<MODE, NB>(A, unit);
+    }
+}
+
+template <int MODE, int NB>
+static int launch_mm_t(Ctx &c, MmArgs A, int nitems, int q0_first, int q0_count, int it0, double *d_lw) {
+    using Sh = MmShape<MODE, NB>;
+    const size_t dyn = MmLds<MODE, NB>::total(A.npmax, A.nsite, A.L);
+    if (dyn > 160 * 1024) {
+        set_error("three-phonon MFMA kernel: LDS need " + std::to_string(dyn) + " B exceeds 160 KB");
+        return SPR_E_UNSUPPORTED;
+    }
+    const void *fn = (const void *)k_lw3_mm<MODE, NB>;
+    SPR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 512, dyn) != hipSuccess || per_cu < 1) per_cu = 1;
+    static const char *names[3] = {"mm_f64", "mm_f32", "mm_bf16"};
+    ItemLaunch L{fn, &A, sizeof(A), 512, dyn, nitems * Sh::NSPLIT, per_cu, std::string(names[MODE])};
+    L.direct = c.override_items != nullptr;
+    L.upi = Sh::NSPLIT;
+    return run_items(c, L, q0_first, q0_count, it0, d_lw);
+}
+
+int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw) {
+    int rc;
+    if ((rc = ensure_pairs(c))) return rc;
+    const ItemSet *ov = c.override_items;
+    const int it0 = ov ? 0 : c.h_item_first[q0_first];
+    const int nitems = ov ? ov->nitems : c.h_item_first[q0_first + q0_count] - it0;
+    const bool f64 = prec == SPR_PREC_FP64;
+    MmArgs A;
+    A.item_q0 = ov ? ov->item_q0 : c.d_item_q0 + it0;
+    A.item_off = ov ? ov->item_off : c.d_item_off + it0;
+    A.item_cnt = ov ? ov->item_cnt : c.d_item_cnt + it0;
+    A.pairs = ov ? ov->pairs : c.d_pairs;
+    A.q0_grid = c.irr_grid;
+    A.L = c.L;
+    A.m0 = c.mesh[0];
+    A.m1 = c.mesh[1];
+    A.blk_start = c.fc3.blk_start;
+    A.tup_site = c.fc3.tup_site;
+    A.fc = f64 ? (const void *)c.fc3.fc64 : (const void *)c.fc3.fc32;
+    A.site = c.fc3.site;
+    A.nsite = c.fc3.nsite;
+    A.phase = f64 ? (const void *)c.phase64 : (const void *)c.phase32;
+    A.evec = f64 ? (const void *)c.evec : (const void *)c.evec32;
+    A.omega = c.omega;
+    A.inv_omega = c.inv_omega;
+    A.nbe = c.nbe;
+    A.sigma = sigma;
+    {
+        // Sctr3THz2 from its defining factors (constants.f90:59-60)
+        const double pi = 4.0 * std::atan(1.0);
+        A.factor = 1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) *
+                   1000000.0 * pi * 0.0625 / (double)c.nq;
+    }
+    int npmax = 0;
+    const int nb2 = c.nb * c.nb;
+    for (int mu = 0; mu < c.nb; ++mu)
+        npmax = std::max(npmax, c.fc3.h_blk_start[(mu + 1) * nb2] - c.fc3.h_blk_start[mu * nb2]);
+    A.npmax = std::max(npmax, 1);
+    A.ic.partial = ov ? ov->partial : c.ipartial;
+    const int mode = prec == SPR_PREC_FP64 ? MM_F64 : prec == SPR_PREC_FP32 ? MM_F32 : MM_BF16;
+    switch (c.nb) {
+#define SPR_NB(n)                                                                                    \
+    case n:                                                                                          \
+        return mode == MM_F64   ? launch_mm_t<MM_F64, n>(c, A, nitems, q0_first, q0_count, it0, d_lw) \
+               : mode == MM_F32 ? launch_mm_t<MM_F32, n>(c, A, nitems, q0_first, q0_count, it0, d_lw) \
+                                : launch_mm_t<MM_BF16, n>(c, A, nitems, q0_first, q0_count, it0, d_lw);
+        SPR_NB(2) SPR_NB(3) SPR_NB(4) SPR_NB(5) SPR_NB(6) SPR_NB(8) SPR_NB(10)
+#undef SPR_NB
+        default:
+            set_error("three-phonon MFMA kernel: Nb " + std::to_string(c.nb) + " not built (2-6, 8, 10)");
+            return SPR_E_UNSUPPORTED;
+    }
+}
+
+}  // namespace spr

@@ -214,14 +214,15 @@ __global__ __launch_bounds__(NB * NB * 3 * 64) void k_lw3_small(SmallArgs A, con
     }
 }
 
-// lw[q][s] = sum of partial[item][s] over the items [first[q], first[q+1]) in order
-__global__ void k_reduce_items(const double *partial, const int *first, int base, int nq0, int nd,
+// lw[q][s] = sum of partial[unit][s] over the units of items [first[q], first[q+1]) in order
+// (upi units per item, unit = item * upi + h)
+__global__ void k_reduce_items(const double *partial, const int *first, int base, int upi, int nq0, int nd,
                                double *lw) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq0 * nd) return;
     const int q = i / nd, s = i % nd;
     double acc = 0.0;
-    for (int it = first[q]; it < first[q + 1]; ++it) acc += partial[(size_t)(it - base) * nd + s];
+    for (int u = (first[q] - base) * upi; u < (first[q + 1] - base) * upi; ++u) acc += partial[(size_t)u * nd + s];
     lw[i] = acc;
 }
 
@@ -303,7 +304,8 @@ int ensure_pairs(Ctx &c) {
     if ((rc = up(&c.d_item_off, ioff))) return rc;
     if ((rc = up(&c.d_item_cnt, icnt))) return rc;
     if ((rc = up(&c.d_item_first, first))) return rc;
-    if (hipMalloc((void **)&c.ipartial, (size_t)c.n_items * c.nd * sizeof(double)) != hipSuccess) {
+    if (hipMalloc((void **)&c.ipartial, (size_t)c.n_items * kMaxUnitsPerItem * c.nd * sizeof(double)) !=
+        hipSuccess) {
         set_error("item partials: hipMalloc failed");
         return SPR_E_OOM;
     }
@@ -312,9 +314,9 @@ int ensure_pairs(Ctx &c) {
     return SPR_OK;
 }
 
-int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, double *d_lw) {
+int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, int upi, double *d_lw) {
     hipLaunchKernelGGL(k_reduce_items, dim3((q0_count * c.nd + 255) / 256), dim3(256), 0, c.stream,
-                       c.ipartial, c.d_item_first + q0_first, it0, q0_count, c.nd, d_lw);
+                       c.ipartial, c.d_item_first + q0_first, it0, upi, q0_count, c.nd, d_lw);
     SPR_HIP(hipGetLastError());
     return SPR_OK;
 }

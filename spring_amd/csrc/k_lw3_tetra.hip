@@ -1,0 +1,191 @@
+// Three-phonon linewidths with the linear tetrahedron method (CalculateLwTetrahedron,
+// ThirdOrder_m.f90:169-275, with FindgAndIver / FindIverxFver, TetrahedronIntegration_m.f90:
+// 96-256): for every s0, tetrahedron of the q1 mesh and branch pair (s1, s2), three processes
+//   M1 = w1 + w2, F1 = W (n1 + n2 + 1)        (+)
+//   M2 = w2 - w1, F2 = W (n1 - n2)            (+)
+//   M3 = w1 - w2, F3 = F2                     (-)
+// with q2 = -(q0 + q1) at each vertex, each contributing g sum_v I_v F_v / 6 at Mq = w0(s0);
+// P3_q0(s0) accumulates g of all three processes.  W(q1, s0, s1, s2) of the q0 comes from the
+// FP64 W path; all arithmetic is FP64.
+//
+// Mapping: one workgroup per (tetrahedron chunk, s0); a thread owns (tetrahedron, s1 s2) and
+// walks its chunk; W reads are contiguous in (s1 s2).  Per-workgroup partials are reduced in a
+// fixed order by a second kernel (bitwise reproducible).
+#include <vector>
+
+#include "spr_internal.h"
+
+namespace spr {
+
+namespace {
+
+// g and sum_v I_v F_v / 6 for one process; M and F are sorted together by M (stable: equal M
+// keep the vertex order, as the reference's sort keeps the first of equal values first)
+__device__ __forceinline__ void tetra_gif(double mq, const double *Min, const double *Fin, double &g, double &gif) {
+    int idx[4] = {0, 1, 2, 3};
+#pragma unroll
+    for (int a = 1; a < 4; ++a)
+#pragma unroll
+        for (int b = a; b > 0; --b)
+            if (Min[idx[b - 1]] > Min[idx[b]]) {
+                const int t = idx[b];
+                idx[b] = idx[b - 1];
+                idx[b - 1] = t;
+            }
+    double M[4], F[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        M[v] = Min[idx[v]];
+        F[v] = Fin[idx[v]];
+    }
+    double I0 = 0, I1 = 0, I2 = 0, I3 = 0;
+    g = 0.0;
+    gif = 0.0;
+    if (mq >= M[0] && mq < M[1]) {
+        g = 3.0 * (mq - M[0]) * (mq - M[0]) / ((M[1] - M[0]) * (M[2] - M[0]) * (M[3] - M[0]));
+        I0 = ((mq - M[1]) / (M[0] - M[1]) + (mq - M[2]) / (M[0] - M[2]) + (mq - M[3]) / (M[0] - M[3])) / 3.0;
+        I1 = ((mq - M[0]) / (M[1] - M[0])) / 3.0;
+        I2 = ((mq - M[0]) / (M[2] - M[0])) / 3.0;
+        I3 = ((mq - M[0]) / (M[3] - M[0])) / 3.0;
+    } else if (mq >= M[1] && mq < M[2]) {
+        g = (3.0 / ((M[1] - M[2]) * (M[3] - M[0]))) *
+            (((mq - M[0]) * (mq - M[2])) / (M[2] - M[0]) - ((mq - M[1]) * (mq - M[3])) / (M[1] - M[3]));
+        const double gd = g * (M[3] - M[0]);
+        I0 = ((mq - M[3]) / (M[0] - M[3])) / 3.0 +
+             ((mq - M[2]) / (M[0] - M[2])) * ((mq - M[0]) / (M[2] - M[0])) * ((mq - M[2]) / (M[1] - M[2])) / gd;
+        I1 = ((mq - M[2]) / (M[1] - M[2])) / 3.0 +
+             ((mq - M[3]) / (M[1] - M[3])) * ((mq - M[3]) / (M[1] - M[3])) * ((mq - M[1]) / (M[2] - M[1])) / gd;
+        I2 = ((mq - M[1]) / (M[2] - M[1])) / 3.0 +
+             ((mq - M[0]) / (M[2] - M[0])) * ((mq - M[0]) / (M[2] - M[0])) * ((mq - M[2]) / (M[1] - M[2])) / gd;
+        I3 = ((mq - M[0]) / (M[3] - M[0])) / 3.0 +
+             ((mq - M[1]) / (M[3] - M[1])) * ((mq - M[3]) / (M[1] - M[3])) * ((mq - M[1]) / (M[2] - M[1])) / gd;
+    } else if (mq >= M[2] && mq <= M[3]) {
+        g = 3.0 * (mq - M[3]) * (mq - M[3]) / ((M[3] - M[0]) * (M[3] - M[1]) * (M[3] - M[2]));
+        I0 = ((mq - M[3]) / (M[0] - M[3])) / 3.0;
+        I1 = ((mq - M[3]) / (M[1] - M[3])) / 3.0;
+        I2 = ((mq - M[3]) / (M[2] - M[3])) / 3.0;
+        I3 = ((mq - M[0]) / (M[3] - M[0]) + (mq - M[1]) / (M[3] - M[1]) + (mq - M[2]) / (M[3] - M[2])) / 3.0;
+    } else {
+        return;
+    }
+    gif = g * ((I0 * F[0] + I1 * F[1] + I2 * F[2] + I3 * F[3]) / 6.0);
+}
+
+constexpr int TBLK = 256;
+
+__global__ __launch_bounds__(TBLK) void k_lw3_tetra(const double *W, const int *tetra, const int *partner,
+                                                    const double *omega, const double *nbe, const double *omega0,
+                                                    int nd, int ntet, int chunk, double *partial) {
+    const int ch = blockIdx.x, s0 = blockIdx.y, nch = gridDim.x;
+    const int nd2 = nd * nd;
+    const int t_lo = ch * chunk, t_hi = min(ntet, t_lo + chunk);
+    const double mq = omega0[s0];
+    double lw = 0.0, p3 = 0.0;
+    for (int i = threadIdx.x; i < (t_hi - t_lo) * nd2; i += TBLK) {
+        const int t = t_lo + i / nd2, p = i % nd2;
+        const int s1 = p / nd, s2 = p % nd;
+        double F1[4], F2[4], M1[4], M2[4], M3[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int q1 = tetra[4 * t + v], q2 = partner[q1];
+            const double w = W[((size_t)q1 * nd + s0) * nd2 + p];
+            const double n1 = nbe[(size_t)q1 * nd + s1], n2 = nbe[(size_t)q2 * nd + s2];
+            const double w1 = omega[(size_t)q1 * nd + s1], w2 = omega[(size_t)q2 * nd + s2];
+            F1[v] = w * (n1 + n2 + 1.0);
+            F2[v] = w * (n1 - n2);
+            M1[v] = w1 + w2;
+            M2[v] = w2 - w1;
+            M3[v] = w1 - w2;
+        }
+        double g, gif;
+        tetra_gif(mq, M1, F1, g, gif);
+        lw += gif;
+        p3 += g;
+        tetra_gif(mq, M2, F2, g, gif);
+        lw += gif;
+        p3 += g;
+        tetra_gif(mq, M3, F2, g, gif);
+        lw -= gif;
+        p3 += g;
+    }
+    __shared__ double s_lw[TBLK], s_p3[TBLK];
+    s_lw[threadIdx.x] = lw;
+    s_p3[threadIdx.x] = p3;
+    __syncthreads();
+    for (int w = TBLK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s_lw[threadIdx.x] += s_lw[threadIdx.x + w];
+            s_p3[threadIdx.x] += s_p3[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[((size_t)s0 * nch + ch) * 2] = s_lw[0];
+        partial[((size_t)s0 * nch + ch) * 2 + 1] = s_p3[0];
+    }
+}
+
+// out[s0] = sum over chunks in order (lw), out[nd + s0] (P3_q0)
+__global__ void k_tetra_reduce(const double *partial, int nd, int nch, double *lw, double *p3) {
+    const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s0 >= nd) return;
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < nch; ++c) {
+        a += partial[((size_t)s0 * nch + c) * 2];
+        b += partial[((size_t)s0 * nch + c) * 2 + 1];
+    }
+    lw[s0] = a;
+    p3[s0] = b;
+}
+
+}  // namespace
+
+// Tetrahedron linewidth of irreducible q0 `qi` into d_lw[nd], d_p3[nd] (device), using c.wout
+// (filled here with W of every q1 by the FP64 W path).
+int launch_lw3_tetra(Ctx &c, int qi, double *d_lw, double *d_p3) {
+    int rc;
+    const size_t nw = (size_t)c.nq * c.nd * c.nd * c.nd;
+    if ((rc = ensure((void **)&c.wout, &c.wout_cap, nw * sizeof(double)))) return rc;
+    if ((rc = launch_lw3(c, qi, 1, 0.1, SPR_PREC_FP64, nullptr, c.wout))) return rc;
+    if (!c.tetra) {
+        if (c.h_tetra.empty()) {
+            set_error("tetrahedron linewidth: the context was created without tetrahedra");
+            return SPR_E_STATE;
+        }
+        SPR_HIP(hipMalloc((void **)&c.tetra, c.h_tetra.size() * sizeof(int)));
+        SPR_HIP(hipMemcpy(c.tetra, c.h_tetra.data(), c.h_tetra.size() * sizeof(int), hipMemcpyHostToDevice));
+        SPR_HIP(hipMalloc((void **)&c.partner, (size_t)c.nq * sizeof(int)));
+    }
+    // partner table of this q0: q2 = -(q0 + q1) on the grid
+    const int i0 = c.h_irr_grid[qi];
+    const int m0 = c.mesh[0], m1 = c.mesh[1], m2 = c.mesh[2];
+    std::vector<int> part(c.nq);
+    for (int q1 = 0; q1 < c.nq; ++q1) {
+        const int a0 = ((-(i0 % m0 + q1 % m0)) % m0 + m0) % m0;
+        const int a1 = ((-((i0 / m0) % m1 + (q1 / m0) % m1)) % m1 + m1) % m1;
+        const int a2 = ((-(i0 / (m0 * m1) + q1 / (m0 * m1))) % m2 + m2) % m2;
+        part[q1] = a0 + m0 * (a1 + m1 * a2);
+    }
+    SPR_HIP(hipMemcpyAsync(c.partner, part.data(), part.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    const int ntet = 6 * c.nq;
+    const int chunk = std::max(1, (ntet + 255) / 256);
+    const int nch = (ntet + chunk - 1) / chunk;
+    if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)nch * c.nd * 2 * sizeof(double)))) return rc;
+    SPR_HIP(hipEventRecord(c.ev0, c.stream));
+    hipLaunchKernelGGL(k_lw3_tetra, dim3(nch, c.nd), dim3(TBLK), 0, c.stream, c.wout, c.tetra, c.partner, c.omega,
+                       c.nbe, c.omega + (size_t)i0 * c.nd, c.nd, ntet, chunk, c.partial);
+    SPR_HIP(hipGetLastError());
+    SPR_HIP(hipEventRecord(c.ev1, c.stream));
+    hipLaunchKernelGGL(k_tetra_reduce, dim3(1), dim3(64), 0, c.stream, c.partial, c.nd, nch, d_lw, d_p3);
+    SPR_HIP(hipGetLastError());
+    // the host partner table must outlive the copy
+    SPR_HIP(hipStreamSynchronize(c.stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess) {
+        c.launches += 1;
+        c.kernel_ms += ms;
+    }
+    return SPR_OK;
+}
+
+}  // namespace spr

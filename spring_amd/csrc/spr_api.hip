@@ -240,6 +240,14 @@ extern "C" int spr_ctx_create(const spr_crystal *cr, const spr_ifc *fc3, const s
     if (mesh->multiplicity) c.h_mult.assign(mesh->multiplicity, mesh->multiplicity + c.nq0);
     c.nsym = mesh->nsym;
     if (mesh->nsym > 0 && mesh->rot) c.h_rot.assign(mesh->rot, mesh->rot + 9 * mesh->nsym);
+    if (mesh->tetra) {
+        for (size_t i = 0; i < (size_t)24 * c.nq; ++i)
+            if (mesh->tetra[i] < 0 || mesh->tetra[i] >= c.nq) {
+                set_error("spr_ctx_create: tetra index out of range");
+                return fail(SPR_E_ARG);
+            }
+        c.h_tetra.assign(mesh->tetra, mesh->tetra + (size_t)24 * c.nq);
+    }
     if ((rc = upload(&c.irr_grid, mesh->irr_grid, c.nq0))) return fail(rc);
     {
         hipDeviceProp_t prop;
@@ -271,8 +279,8 @@ extern "C" int spr_ctx_destroy(spr_ctx *h) {
                     (void *)c.omega, (void *)c.inv_omega, (void *)c.nbe, (void *)c.vel,
                     (void *)c.irr_grid, (void *)c.partial, (void *)c.out, (void *)c.counter,
                     (void *)c.d_pairs, (void *)c.d_item_q0, (void *)c.d_item_off, (void *)c.d_item_cnt,
-                    (void *)c.d_item_first, (void *)c.omega32, (void *)c.mfma_scratch,
-                    (void *)c.ipartial, (void *)c.wout})
+                    (void *)c.d_item_first,
+                    (void *)c.ipartial, (void *)c.wout, (void *)c.tetra, (void *)c.partner, (void *)c.tout})
         if (p) (void)hipFree(p);
     if (c.ev0) (void)hipEventDestroy(c.ev0);
     if (c.ev1) (void)hipEventDestroy(c.ev1);
@@ -476,4 +484,41 @@ extern "C" int spr_v3_w_pairs(spr_ctx *h, int n, const int *q0_index, const int 
     }
     (void)hipFree(d_idx);
     return rc;
+}
+
+extern "C" int spr_lw3_tetra(spr_ctx *h, int i0, double *lw, double *p3_q0) {
+    if (!h || !lw || !p3_q0) {
+        set_error("spr_lw3_tetra: bad argument");
+        return SPR_E_ARG;
+    }
+    Ctx &c = h->c;
+    if (i0 < 0 || i0 >= c.nq0) {
+        set_error("spr_lw3_tetra: q0 index out of range");
+        return SPR_E_ARG;
+    }
+    SPR_HIP(hipSetDevice(c.device));
+    int rc;
+    if (!c.tout) SPR_HIP(hipMalloc((void **)&c.tout, (size_t)2 * c.nq0 * c.nd * sizeof(double)));
+    if ((rc = spr::launch_lw3_tetra(c, i0, c.tout, c.tout + c.nd))) return rc;
+    SPR_HIP(hipMemcpy(lw, c.tout, c.nd * sizeof(double), hipMemcpyDeviceToHost));
+    SPR_HIP(hipMemcpy(p3_q0, c.tout + c.nd, c.nd * sizeof(double), hipMemcpyDeviceToHost));
+    return SPR_OK;
+}
+
+extern "C" int spr_lw3_tetra_all(spr_ctx *h, double *lw_all, double *p3_all) {
+    if (!h || !lw_all || !p3_all) {
+        set_error("spr_lw3_tetra_all: bad argument");
+        return SPR_E_ARG;
+    }
+    Ctx &c = h->c;
+    SPR_HIP(hipSetDevice(c.device));
+    int rc;
+    if (!c.tout) SPR_HIP(hipMalloc((void **)&c.tout, (size_t)2 * c.nq0 * c.nd * sizeof(double)));
+    const size_t n = (size_t)c.nq0 * c.nd;
+    for (int qi = 0; qi < c.nq0; ++qi)
+        if ((rc = spr::launch_lw3_tetra(c, qi, c.tout + (size_t)qi * c.nd, c.tout + n + (size_t)qi * c.nd)))
+            return rc;
+    SPR_HIP(hipMemcpy(lw_all, c.tout, n * sizeof(double), hipMemcpyDeviceToHost));
+    SPR_HIP(hipMemcpy(p3_all, c.tout + n, n * sizeof(double), hipMemcpyDeviceToHost));
+    return SPR_OK;
 }

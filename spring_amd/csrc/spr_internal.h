@@ -42,7 +42,7 @@ struct ItemLaunch {
     bool direct = false;    // one workgroup per item, no graph, no tuning, no reduction
     void *extra[4] = {nullptr, nullptr, nullptr, nullptr};   // further kernel arguments
     int nextra = 0;
-    int fixed_mode = -1;    // >= 0: always this launch mode (kernels with per-workgroup slots)
+    int upi = 1;            // work units per item (nitems counts units); partial rows per item
 };
 
 // Caller-supplied work items (spr_lw3_pairs): partial[item] receives the item's sum.
@@ -102,14 +102,15 @@ struct Ctx {
     int2 *d_pairs = nullptr;
     int *d_item_q0 = nullptr, *d_item_off = nullptr, *d_item_cnt = nullptr, *d_item_first = nullptr;
     std::vector<int> h_item_first;
-    float *omega32 = nullptr;      // (3, nq, nd): omega, 1/omega, nBE in FP32 for the MFMA kernel
-    float *mfma_scratch = nullptr; // FP32 MFMA kernel: one U2 stash slot per resident workgroup
-    size_t mfma_scratch_bytes = 0;
     int ncu = 256;                 // compute units of the device
     int last_mode = -1;            // launch mode of the last item launch (0: one WG per item)
+    int last_upi = 1;              // work units per item of the last item launch
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
     double *ipartial = nullptr;    // (n_items, nd) item partials (stable: graphs keep the pointer)
-    double *wout = nullptr;        // W parity output
+    double *wout = nullptr;        // W parity output (also W of one q0 for the tetrahedron path)
+    int *tetra = nullptr, *partner = nullptr;   // (6 nq, 4) tetrahedra; q2 partner of every q1
+    std::vector<int> h_tetra;
+    double *tout = nullptr;        // tetrahedron outputs: lw then P3_q0, (2, nq0, nd)
     const ItemSet *override_items = nullptr;
     const int *w_pair_q0 = nullptr, *w_pair_q1 = nullptr;   // W parity path over a pair list
     int w_npairs = 0;
@@ -129,7 +130,9 @@ int launch_lw4(Ctx &c, int q0_first, int q0_count, double sigma, int prec, doubl
 int ensure_pairs(Ctx &c);
 int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double *d_lw);
 void drop_graphs(Ctx &c);
-int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, double *d_lw);
+int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, int upi, double *d_lw);
+constexpr int kMaxUnitsPerItem = 4;   // partial rows reserved per item
 int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_mode_k);
+int launch_lw3_tetra(Ctx &c, int qi, double *d_lw, double *d_p3);
 
 }  // namespace spr

@@ -20,7 +20,7 @@ SYMBOLS = [
     "spr_lw3_all", "spr_lw4", "spr_lw4_all", "spr_kappa_rta", "spr_kernel_time",
     "spr_kernel_time_reset", "spr_lw3_pairs", "spr_v3_w_pairs", "spr_mlp_create", "spr_mlp_destroy",
     "spr_mlp_num_params", "spr_mlp_get_params", "spr_mlp_set_params", "spr_mlp_grad",
-    "spr_mlp_train", "spr_mlp_predict",
+    "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all",
 ]
 
 
@@ -40,7 +40,8 @@ class _IFC(ct.Structure):
 class _Mesh(ct.Structure):
     _fields_ = [("mesh", ct.c_int * 3), ("nq", ct.c_int), ("q_int", ct.c_void_p),
                 ("nq0", ct.c_int), ("irr_grid", ct.c_void_p), ("pos", ct.c_void_p),
-                ("multiplicity", ct.c_void_p), ("nsym", ct.c_int), ("rot", ct.c_void_p)]
+                ("multiplicity", ct.c_void_p), ("nsym", ct.c_int), ("rot", ct.c_void_p),
+                ("tetra", ct.c_void_p)]
 
 
 class _Phonons(ct.Structure):
@@ -106,7 +107,7 @@ class GpuContext:
         self._mesh = _Mesh((ct.c_int * 3)(*[int(x) for x in qm.mesh]), qm.nq,
                            _ptr(arr(qm.q_int, np.int32)), qm.nq0, _ptr(arr(qm.irr_idx, np.int32)),
                            _ptr(arr(qm.pos, np.int32)), _ptr(arr(qm.multiplicity, np.int32)),
-                           rot.shape[0], _ptr(rotf))
+                           rot.shape[0], _ptr(rotf), _ptr(arr(qm.tetra, np.int32)))
         self._ph = _Phonons(ph.nq, self.nd, _ptr(arr(ph.evec, np.complex128)),
                             _ptr(arr(ph.omega, np.float64)), _ptr(arr(ph.inv_omega, np.float64)),
                             _ptr(arr(ph.nbe, np.float64)), _ptr(arr(ph.grp_vel, np.float64)))
@@ -191,6 +192,19 @@ class GpuContext:
         self._check(self.lib.spr_lw3_pairs(self.h, ct.c_int(q0.size), _ptr(q0), _ptr(q1), ct.c_double(sigma),
                                            ct.c_int(PREC[prec]), _ptr(out)), "spr_lw3_pairs")
         return out
+
+    def lw3_tetra(self, i0: int):
+        """Tetrahedron-method linewidth and P3_q0 of irreducible q0 number i0 (FP64)."""
+        lw = np.zeros(self.nd)
+        p3 = np.zeros(self.nd)
+        self._check(self.lib.spr_lw3_tetra(self.h, ct.c_int(i0), _ptr(lw), _ptr(p3)), "spr_lw3_tetra")
+        return lw, p3
+
+    def lw3_tetra_all(self):
+        lw = np.zeros((self.nq0, self.nd))
+        p3 = np.zeros((self.nq0, self.nd))
+        self._check(self.lib.spr_lw3_tetra_all(self.h, _ptr(lw), _ptr(p3)), "spr_lw3_tetra_all")
+        return lw, p3
 
     def kernel_time(self):
         n = ct.c_int()

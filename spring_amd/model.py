@@ -91,25 +91,31 @@ def rocksalt_primitive(a: float = 5.64, masses=(22.99, 35.45), name="rocksalt-2"
     return Crystal(lat, frac, np.array(masses, float), np.array([0, 1]), name)
 
 
-def rocksalt_supercell10(a: float = 4.21, masses=(24.3, 16.0), jitter: float = 0.02,
-                         seed: int = 3) -> Crystal:
-    """10-atom cell: 5x1x1 supercell of the rocksalt primitive cell (5 cations, 5 anions),
+def rocksalt_supercell(n: int, a: float = 4.21, masses=(24.3, 16.0), jitter: float = 0.02,
+                       seed: int = 3) -> Crystal:
+    """2n-atom cell: n x 1 x 1 supercell of the rocksalt primitive cell (n cations, n anions),
     every atom displaced by a seeded uniform jitter of at most `jitter` Angstrom per axis."""
     prim = rocksalt_primitive(a, masses)
     lat = prim.latvec.copy()
-    lat[:, 0] *= 5.0
+    lat[:, 0] *= float(n)
     rng = np.random.default_rng(seed)
     cart = []
     spec = []
-    for i in range(5):
+    for i in range(n):
         for b in range(2):
             r = prim.latvec @ (prim.basis_frac[b] + np.array([i, 0, 0]))
             cart.append(r)
             spec.append(b)
-    cart = np.array(cart) + rng.uniform(-jitter, jitter, size=(10, 3))
+    cart = np.array(cart) + rng.uniform(-jitter, jitter, size=(2 * n, 3))
     frac = cart @ np.linalg.inv(lat).T
     mass = np.array([masses[s] for s in spec], float)
-    return Crystal(lat, frac, mass, np.array(spec), "rocksalt-10")
+    return Crystal(lat, frac, mass, np.array(spec), f"rocksalt-{2 * n}")
+
+
+def rocksalt_supercell10(a: float = 4.21, masses=(24.3, 16.0), jitter: float = 0.02,
+                         seed: int = 3) -> Crystal:
+    """10-atom cell: 5x1x1 supercell of the rocksalt primitive cell (5 cations, 5 anions)."""
+    return rocksalt_supercell(5, a, masses, jitter, seed)
 
 
 # ----------------------------------------------------------------------------------------
@@ -324,3 +330,16 @@ def rocksalt10(seed: int = 11) -> Model:
     fc2 = spring_fc2(cr, [(r1, 1.5, 0.2), (r2, 0.3, 0.05)], seed=seed, nominal=nominal)
     fc3 = random_anharmonic(cr, 3, r3, 0.5, seed + 1, nominal=nominal)
     return Model(cr, fc2, fc3, None, "rocksalt10")
+
+
+def rocksalt_chain(n: int, seed: int = 21) -> Model:
+    """2n-atom jittered n x 1 x 1 rocksalt supercell (a = 4.21 A, 24.3/16.0 amu): the
+    rocksalt10 spring model with FC3 N(0,0.5) eV/A^3 within the 2nd shell (a smaller cluster
+    set, for tests of the intermediate cell sizes)."""
+    a = 4.21
+    cr = rocksalt_supercell(n, a=a, seed=seed)
+    nominal = rocksalt_supercell(n, a=a, jitter=0.0)
+    r1, r2 = a / 2.0, a / np.sqrt(2.0)
+    fc2 = spring_fc2(cr, [(r1, 1.5, 0.2), (r2, 0.3, 0.05)], seed=seed, nominal=nominal)
+    fc3 = random_anharmonic(cr, 3, r2, 0.5, seed + 1, nominal=nominal)
+    return Model(cr, fc2, fc3, None, f"rocksalt{2 * n}")

@@ -282,3 +282,22 @@ def test_tetra_weights_closed_forms():
     gs = np.array([O.tetra_weights(x, M, np.ones(4))[1] for x in xs])
     assert abs(np.trapezoid(gs, xs) - 1.0) < 1e-6
     assert O.tetra_weights(2.5, M, np.ones(4))[0] == 0
+
+
+def test_lw3_tetra_agrees_with_gaussian_limit(rocksalt):
+    """The tetrahedron linewidth and the Gaussian-delta linewidth approximate the same BZ
+    integral: summed over branches and a spread of q0 on a 12^3 mesh (sigma = 0.6 rad/ps) they
+    agree to a few per cent.  A flipped sign or a dropped difference process (each about 30-60 %
+    of the total here) fails this."""
+    from tests.conftest import problem
+    qm, ph = problem(rocksalt, (12, 12, 12))
+    inp = O.Inputs(rocksalt, qm, ph)
+    tt = gg = 0.0
+    for qi in range(1, qm.nq0, 12):
+        ig = int(qm.irr_idx[qi])
+        _, W = O.lw3_gauss(inp, ig, 1.0, want_w=True)
+        lt, p3 = O.lw3_tetra(qm, ph, ig, W)
+        assert np.all(p3 >= 0.0)
+        tt += lt.sum()
+        gg += O.lw3_gauss(inp, ig, 0.6).sum()
+    assert abs(tt - gg) / gg < 0.05

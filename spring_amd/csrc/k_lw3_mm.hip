@@ -64,6 +64,7 @@ struct MmArgs {
     const double *omega, *inv_omega, *nbe;
     double sigma, factor;
     int npmax;                   // most clusters of one basis atom
+    int nterm;                   // clusters of all basis atoms
 };
 
 constexpr int pow2ceil(int x) { return x <= 1 ? 1 : x <= 2 ? 2 : x <= 4 ? 4 : 8; }
@@ -99,6 +100,10 @@ template <int MODE, int NB> struct MmShape {
     static constexpr int VT = (MT_D * NTC + 7) / 8;         // V tiles per wave
     static_assert(NT <= 4, "Ndof must be <= 32");
     static_assert(NSPLIT <= kMaxUnitsPerItem, "too many s2 blocks");
+    // cluster phases of every basis atom per pair in LDS (FP32 / BF16), else per atom
+    static constexpr bool PALL = MODE != MM_F64;
+    static constexpr int NE = NB * NB * 27;                 // FT outputs per atom (block, j)
+    static constexpr int KE = (NE + 511) / 512;             // FT outputs per thread
 };
 
 // LDS layout (bytes), shared by the kernel and the host-side size check
@@ -113,10 +118,12 @@ template <int MODE, int NB> struct MmLds {
     __host__ __device__ static constexpr size_t tu() { return TU; }
     __host__ __device__ static constexpr size_t ac() { return AC; }
     __host__ __device__ static constexpr size_t e() { return E; }
-    static size_t total(int npmax, int nsite, int L) {
+    static size_t total(int npmax, int nterm, int nsite, int L) {
         size_t b = tu() + ac() + 2 * e();
-        b += (size_t)npmax * sizeof(Z) + (size_t)L * sizeof(Z) + 8 * Sh::ND * sizeof(double);
-        b += 2 * (size_t)nsite * sizeof(int);
+        b += (size_t)(Sh::PALL ? nterm : npmax) * sizeof(Z) + (size_t)L * sizeof(Z) + 8 * Sh::ND * sizeof(double);
+        if (Sh::PALL) b += (size_t)npmax * 27 * sizeof(R);      // FC of one atom (staged)
+        b += (size_t)Sh::ND * Sh::ND * sizeof(Z) + 16;           // e0 of the unit's q0
+        b += 2 * (size_t)nsite * sizeof(int) + (size_t)(NB * NB * NB + 1) * sizeof(int);
         return (b + 15) & ~(size_t)15;
     }
 };
@@ -172,6 +179,46 @@ struct Frag {
             }
         }
         return b;
+    }
+
+    // two tiles (rows m0.., m1..) with the same B fragment; the two chains interleave
+    __device__ static void mma2(const R *sA, int m0, int m1, const BF &b, typename Sh::ACC *d) {
+        const int l = threadIdx.x & 63, g = l >> 4;
+        const R *r0 = sA + (m0 + (l & 15)) * Sh::PA;
+        const R *r1 = sA + (m1 + (l & 15)) * Sh::PA;
+        typename Sh::ACC a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
+        if constexpr (MODE == MM_F64) {
+            const double2 *p0 = reinterpret_cast<const double2 *>(r0 + g * Sh::S);
+            const double2 *p1 = reinterpret_cast<const double2 *>(r1 + g * Sh::S);
+#pragma unroll
+            for (int i = 0; i < Sh::S / 2; ++i) {
+                const double2 x = p0[i], y = p1[i];
+                a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, b.h[2 * i], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(y.x, b.h[2 * i], a1, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, b.h[2 * i + 1], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(y.y, b.h[2 * i + 1], a1, 0, 0, 0);
+            }
+        } else if constexpr (MODE == MM_F32) {
+            const float4 *p0 = reinterpret_cast<const float4 *>(r0 + g * Sh::S);
+            const float4 *p1 = reinterpret_cast<const float4 *>(r1 + g * Sh::S);
+#pragma unroll
+            for (int i = 0; i < Sh::S / 4; ++i) {
+                const float4 x = p0[i], y = p1[i];
+                a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, b.h[4 * i], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(y.x, b.h[4 * i], a1, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, b.h[4 * i + 1], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(y.y, b.h[4 * i + 1], a1, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, b.h[4 * i + 2], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(y.z, b.h[4 * i + 2], a1, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, b.h[4 * i + 3], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(y.w, b.h[4 * i + 3], a1, 0, 0, 0);
+            }
+        } else {
+            a0 = mma(sA, m0, b, a0);
+            a1 = mma(sA, m1, b, a1);
+        }
+        d[0] = a0;
+        d[1] = a1;
     }
 
     __device__ static typename Sh::ACC mma(const R *sA, int m_base, const BF &b, typename Sh::ACC acc) {
@@ -242,14 +289,17 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     p += L::e();
     Z *sE2 = reinterpret_cast<Z *>(p);
     p += L::e();
-    Z *sP = reinterpret_cast<Z *>(p);
-    p += (size_t)A.npmax * sizeof(Z);
+    Z *sP = reinterpret_cast<Z *>(p);                 // cluster phases (all atoms or atom mu)
+    p += (size_t)(Sh::PALL ? A.nterm : A.npmax) * sizeof(Z);
     Z *sPh = reinterpret_cast<Z *>(p);
     p += (size_t)A.L * sizeof(Z);
     double *sMode = reinterpret_cast<double *>(p);    // w0 iw0f | w1 n1 iw1 | w2 n2 iw2 | -
     p += 8 * ND * sizeof(double);
     int *sM1 = reinterpret_cast<int *>(p);
     int *sM2 = sM1 + A.nsite;
+    int *sBlk = sM2 + A.nsite;                        // blk_start of every (mu, nu, eta) block
+    Z *sE0 = reinterpret_cast<Z *>(((uintptr_t)(sBlk + NB * NB * NB + 1) + 15) & ~(uintptr_t)15);   // e0[s0][a]
+    R *sFC = reinterpret_cast<R *>(sE0 + ND * ND);    // FC of atom mu (FP32 / BF16 modes)
 
     const int tid = threadIdx.x, l = tid & 63, g = l >> 4;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -272,6 +322,8 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
         }
     }
     for (int i = tid; i < A.L; i += 512) sPh[i] = phase[i];
+    for (int i = tid; i <= NB * NB * NB; i += 512) sBlk[i] = A.blk_start[i];
+    for (int i = tid; i < ND * ND; i += 512) sE0[i] = evec[(size_t)i0 * ND * ND + i];
     if (tid < ND) {
         sMode[tid] = A.omega[(size_t)i0 * ND + tid];
         sMode[ND + tid] = A.inv_omega[(size_t)i0 * ND + tid] * A.factor;
@@ -327,35 +379,61 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
 #pragma unroll
         for (int j = 0; j < Sh::VT; ++j) vacc[j] = DACC{0, 0, 0, 0};
         __syncthreads();
+        // FC of atom mu is staged in LDS (FP32 / BF16) by direct global-to-LDS loads (no
+        // registers), issued for the next atom while the GEMMs of this one run
+        auto stage = [&](int mu_n) {
+            const int a = sBlk[mu_n * NB * NB] * 27, n = sBlk[(mu_n + 1) * NB * NB] * 27 - a;
+            for (int c = w; c * 64 < n; c += 8)
+                if (c * 64 + l < n) __builtin_amdgcn_global_load_lds(fc + a + c * 64 + l, sFC + c * 64, 4, 0, 0);
+        };
+        if constexpr (Sh::PALL) {
+            stage(0);
+            // cluster phases of every atom: P(u) = e^{2 pi i (m1(N1) + m2(N2)) / L}
+            for (int u = tid; u < A.nterm; u += 512) {
+                int m = sM1[A.tup_site[2 * u]] + sM2[A.tup_site[2 * u + 1]];
+                m -= m >= A.L ? A.L : 0;
+                sP[u] = sPh[m];
+            }
+            __builtin_amdgcn_s_waitcnt(0);   // the staged FC has landed
+            __syncthreads();
+        }
 
 #pragma unroll 1
         for (int mu = 0; mu < NB; ++mu) {
-            // ---- cluster phases of atom mu ----
-            const int u0 = A.blk_start[mu * NB * NB], u1 = A.blk_start[(mu + 1) * NB * NB];
-            for (int u = u0 + tid; u < u1; u += 512) {
-                int m = sM1[A.tup_site[2 * u]] + sM2[A.tup_site[2 * u + 1]];
-                m -= m >= A.L ? A.L : 0;
-                sP[u - u0] = sPh[m];
-            }
-            __syncthreads();
-            // ---- FT: T[(alpha, 3 nu + beta)][2 (3 eta + gamma) + p]; lanes run along the 27
-            //      components of a block, so FC loads are contiguous ----
-            for (int e = tid; e < NB * NB * 27; e += 512) {
-                const int bl = e / 27, j = e % 27;
-                const int nu = bl / NB, eta = bl % NB;
-                const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
-                const int blk = (mu * NB + nu) * NB + eta;
-                const int ua = A.blk_start[blk], ub = A.blk_start[blk + 1];
-                R re = 0, im = 0;
-                for (int u = ua; u < ub; ++u) {
-                    const R f = fc[(size_t)u * 27 + j];
-                    const Z ph = sP[u - u0];
-                    re = fma(f, ph.x, re);
-                    im = fma(f, ph.y, im);
+            const int u0 = Sh::PALL ? 0 : sBlk[mu * NB * NB];
+            if constexpr (!Sh::PALL) {
+                // ---- cluster phases of atom mu ----
+                const int u1 = sBlk[(mu + 1) * NB * NB];
+                for (int u = u0 + tid; u < u1; u += 512) {
+                    int m = sM1[A.tup_site[2 * u]] + sM2[A.tup_site[2 * u + 1]];
+                    m -= m >= A.L ? A.L : 0;
+                    sP[u - u0] = sPh[m];
                 }
-                R *dst = sT + (al * ND + 3 * nu + be) * PA + 2 * (3 * eta + ga);
-                dst[0] = re;
-                dst[1] = im;
+                __syncthreads();
+            }
+            // ---- FT: T[(alpha, 3 nu + beta)][2 (3 eta + gamma) + p]; lanes run along the 27
+            //      components of a block, so FC reads are contiguous ----
+            {
+                const R *fcm = Sh::PALL ? sFC : fc + (size_t)u0 * 27;
+                const int ub0 = Sh::PALL ? sBlk[mu * NB * NB] : u0;   // first cluster of atom mu
+                for (int e = tid; e < Sh::NE; e += 512) {
+                    const int bl = e / 27, j = e % 27;
+                    const int nu = bl / NB, eta = bl % NB;
+                    const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
+                    const int blk = (mu * NB + nu) * NB + eta;
+                    const int ua = sBlk[blk], ub = sBlk[blk + 1];
+                    R re = 0, im = 0;
+#pragma unroll 4
+                    for (int u = ua; u < ub; ++u) {
+                        const R f = fcm[(size_t)(u - ub0) * 27 + j];
+                        const Z ph = sP[u - u0];
+                        re = fma(f, ph.x, re);
+                        im = fma(f, ph.y, im);
+                    }
+                    R *dst = sT + (al * ND + 3 * nu + be) * PA + 2 * (3 * eta + ga);
+                    dst[0] = re;
+                    dst[1] = im;
+                }
             }
             // padding of T (the region holds U2 between the GEMMs)
             for (int i = tid; i < Sh::MB * (PA - K2); i += 512) {
@@ -367,20 +445,30 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                 sT[m * PA + k] = (R)0;
             }
             __syncthreads();
+            if constexpr (Sh::PALL)
+                if (mu + 1 < NB) stage(mu + 1);   // every thread is past the FT of mu
             // ---- GEMM B: U1[(alpha, b)][(s2, q)] = T x E2', s2 in this unit's block
             //      -> sAC[(alpha, s2 - h SB)][(b, q)] ----
             if (bOK) {
                 const typename F::BF bf = F::load_b(sE2, ntB * 16);
                 const int np = ntB * 16 + (l & 15);
                 const int s2 = np >> 1, q = np & 1;
-                for (int mt = mtB0; mt < Sh::MT_B; mt += MSTEPB) {
-                    ACC d = F::mma(sT, mt * 16, bf, ACC{0, 0, 0, 0});
+                // two M tiles per pass: independent accumulation chains
+                for (int mt = mtB0; mt < Sh::MT_B; mt += 2 * MSTEPB) {
+                    const int mt2 = mt + MSTEPB < Sh::MT_B ? mt + MSTEPB : mt;
+                    ACC d[2];
+                    F::mma2(sT, mt * 16, mt2 * 16, bf, d);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int m = mt * 16 + tile_row<MODE == MM_F64>(g, r);
-                        if (m < 3 * ND && s2 < ND) {
-                            const int al = m / ND, b = m % ND;
-                            sAC[(al * SB + s2 - h * SB) * PA + 2 * b + q] = (R)d[r];
+                    for (int t = 0; t < 2; ++t) {
+                        const int mtt = t ? mt2 : mt;
+                        if (t && mt2 == mt) break;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int m = mtt * 16 + tile_row<MODE == MM_F64>(g, r);
+                            if (m < 3 * ND && s2 < ND) {
+                                const int al = m / ND, b = m % ND;
+                                sAC[(al * SB + s2 - h * SB) * PA + 2 * b + q] = (R)d[t][r];
+                            }
                         }
                     }
                 }
@@ -391,15 +479,22 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                 const typename F::BF bf = F::load_b(sE1, ntC * 16);
                 const int np = ntC * 16 + (l & 15);
                 const int s1 = np >> 1, q = np & 1;
-                for (int mt = mtC0; mt < Sh::MT_C; mt += MSTEPC) {
-                    ACC d = F::mma(sAC, mt * 16, bf, ACC{0, 0, 0, 0});
+                for (int mt = mtC0; mt < Sh::MT_C; mt += 2 * MSTEPC) {
+                    const int mt2 = mt + MSTEPC < Sh::MT_C ? mt + MSTEPC : mt;
+                    ACC d[2];
+                    F::mma2(sAC, mt * 16, mt2 * 16, bf, d);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int m = mt * 16 + tile_row<MODE == MM_F64>(g, r);
-                        if (m < 3 * SB && s1 < ND) {
-                            const int al = m / SB, sl = m % SB;
-                            R *z = reinterpret_cast<R *>(sU2 + (s1 * SB + sl) * 3 + al);
-                            z[q] = (R)d[r];
+                    for (int t = 0; t < 2; ++t) {
+                        const int mtt = t ? mt2 : mt;
+                        if (t && mt2 == mt) break;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int m = mtt * 16 + tile_row<MODE == MM_F64>(g, r);
+                            if (m < 3 * SB && s1 < ND) {
+                                const int al = m / SB, sl = m % SB;
+                                R *z = reinterpret_cast<R *>(sU2 + (s1 * SB + sl) * 3 + al);
+                                z[q] = (R)d[t][r];
+                            }
                         }
                     }
                 }
@@ -410,7 +505,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             if (cOK) {
                 DR b0 = 0, b1 = 0;
                 if (g < 3 && s0 < ND) {
-                    const Z e = evec[((size_t)i0 * ND + s0) * ND + 3 * mu + g];
+                    const Z e = sE0[s0 * ND + 3 * mu + g];
                     b0 = (DR)(qD ? e.y : e.x);
                     b1 = (DR)(qD ? e.x : -e.y);
                 }
@@ -431,6 +526,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                     }
                 }
             }
+            if constexpr (Sh::PALL) __builtin_amdgcn_s_waitcnt(0);   // the next atom's FC has landed
             __syncthreads();   // sU2 (= sT) is rewritten by the next atom's FT
         }
 
@@ -503,7 +599,7 @@ __global__ __launch_bounds__(512) void k_lw3_mm(MmArgs A) {
 template <int MODE, int NB>
 static int launch_mm_t(Ctx &c, MmArgs A, int nitems, int q0_first, int q0_count, int it0, double *d_lw) {
     using Sh = MmShape<MODE, NB>;
-    const size_t dyn = MmLds<MODE, NB>::total(A.npmax, A.nsite, A.L);
+    const size_t dyn = MmLds<MODE, NB>::total(A.npmax, A.nterm, A.nsite, A.L);
     if (dyn > 160 * 1024) {
         set_error("three-phonon MFMA kernel: LDS need " + std::to_string(dyn) + " B exceeds 160 KB");
         return SPR_E_UNSUPPORTED;
@@ -557,6 +653,7 @@ int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, do
     for (int mu = 0; mu < c.nb; ++mu)
         npmax = std::max(npmax, c.fc3.h_blk_start[(mu + 1) * nb2] - c.fc3.h_blk_start[mu * nb2]);
     A.npmax = std::max(npmax, 1);
+    A.nterm = std::max(c.fc3.h_blk_start[c.nb * nb2], 1);
     A.ic.partial = ov ? ov->partial : c.ipartial;
     const int mode = prec == SPR_PREC_FP64 ? MM_F64 : prec == SPR_PREC_FP32 ? MM_F32 : MM_BF16;
     switch (c.nb) {

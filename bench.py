@@ -33,9 +33,12 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and matrix are the same rate)
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", default="rocksalt16", choices=["rocksalt16", "cell10"],
+    p.add_argument("--workload", default="rocksalt16", choices=["rocksalt16", "cell10", "learned10"],
                    help="rocksalt16: 2-atom cell, 16^3 mesh (the headline line); cell10: 10-atom "
-                        "cell, 20^3 mesh with time reversal, FP64 vs FP32 vs BF16")
+                        "cell, 20^3 mesh with time reversal, FP64 vs FP32 vs BF16; learned10: the "
+                        "learned importance-sampling proposal on the cell10 workload")
+    p.add_argument("--samples", type=int, default=2_000_000, help="learned10: training samples")
+    p.add_argument("--budget", type=float, default=0.05, help="learned10: sampled fraction of q1")
     p.add_argument("--mesh10", type=int, default=20)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
@@ -57,6 +60,8 @@ def main():
         sys.exit(2)
     if args.workload == "cell10":
         return cell10(args)
+    if args.workload == "learned10":
+        return learned10(args)
     from spring_amd import model, mesh, phonons, lib
 
     t_in0 = time.perf_counter()
@@ -255,6 +260,92 @@ def cell10(args):
         "fp64": entry("fp64"), "fp32": entry("fp32"), "bf16": entry("bf16"),
         "cpu_baseline": cpu, "phonons_s": t_ph, "host_model_s": t_model,
         "kappa_W_mK": [float(r64["k"][i, i]) for i in range(3)],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def learned10(args):
+    """Learned importance-sampling proposal on the 10-atom cell: the network is trained on
+    args.samples |V3|^2 values produced by the FP64 W path, scores every (q0, q1) on the GPU,
+    and K = budget * Nq sampled q1 per q0 (all branches of each sampled pair, FP64 pair terms)
+    estimate every linewidth; kappa and the per-mode linewidths are judged against the FP64
+    full sum.  The same sampling with the network-free (G-only) proposal and with uniform
+    sampling is reported beside it."""
+    from spring_amd import model, mesh, phonons, lib
+    from spring_amd import learned as Lrn
+    n = args.mesh10
+    m = model.rocksalt10()
+    t0 = time.perf_counter()
+    qm = mesh.make_mesh(m.crystal, (n, n, n), time_rev=1)
+    ph = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
+    t_ph = time.perf_counter() - t0
+    ctx = lib.GpuContext(m, qm, ph)
+    nd = m.crystal.ndof
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    lw_full = ctx.lw3_all(args.sigma, "fp64")
+    k_full, _ = ctx.kappa(lw_full, args.T, symm=True)
+    t_full = time.perf_counter() - t
+    log(f"learned10: FP64 full sum {t_full:.2f} s, nq0 {qm.nq0}")
+    wmax = float(np.abs(ph.omega).max())
+    per_pair = 200
+    t = time.perf_counter()
+    X, y = Lrn.training_set(ctx, qm, ph, n_pairs=max(1, args.samples // per_pair), per_pair=per_pair, seed=0)
+    t_set = time.perf_counter() - t
+    t = time.perf_counter()
+    prop = Lrn.Proposal.train(X, y, wmax=wmax, epochs=8, batch=4096, lr=2e-3)
+    t_train = time.perf_counter() - t
+    log(f"learned10: {y.size} samples in {t_set:.1f} s, trained in {t_train:.1f} s, loss "
+        f"{prop.losses[0]:.3f} -> {prop.losses[-1]:.3f}")
+    qis = np.arange(qm.nq0)
+    K = max(1, int(round(args.budget * qm.nq)))
+    sig = np.abs(lw_full) > 1e-6 * np.abs(lw_full).max()
+    rng = np.random.default_rng(1)
+    res = {}
+    for name in ("learned", "physics", "uniform"):
+        t = time.perf_counter()
+        if name == "uniform":
+            S, n_eval = np.ones((qm.nq0, qm.nq)), 0
+        else:
+            S, n_eval = Lrn.device_scores(prop if name == "learned" else None, ctx, qis, args.sigma)
+        t_score = time.perf_counter() - t
+        t = time.perf_counter()
+        q0s, q1s, ws = [], [], []
+        for qi in range(qm.nq0):
+            pr = Lrn.mixture(S[qi])
+            pick = rng.choice(qm.nq, size=K, replace=True, p=pr)
+            q0s.append(np.full(K, qi))
+            q1s.append(pick)
+            ws.append(1.0 / (K * pr[pick]))
+        q0s, q1s, ws = np.concatenate(q0s), np.concatenate(q1s), np.concatenate(ws)
+        t_draw = time.perf_counter() - t
+        t = time.perf_counter()
+        est = np.zeros((qm.nq0, nd))
+        chunk = 200_000
+        for lo in range(0, q0s.size, chunk):
+            tt = ctx.lw3_pairs(q0s[lo:lo + chunk], q1s[lo:lo + chunk], args.sigma, "fp64")
+            np.add.at(est, q0s[lo:lo + chunk], tt * ws[lo:lo + chunk, None])
+        k_est, _ = ctx.kappa(est, args.T, symm=True)
+        t_pairs = time.perf_counter() - t
+        err = np.abs(est - lw_full)[sig] / np.abs(lw_full)[sig]
+        res[name] = {
+            "kappa_rel_err_vs_fp64": float(np.max(np.abs(k_est - k_full)) / np.max(np.abs(k_full))),
+            "linewidth_rel_err_median": float(np.median(err)), "linewidth_rel_err_mean": float(err.mean()),
+            "linewidth_rel_err_p90": float(np.quantile(err, 0.9)),
+            "score_s": t_score, "network_evals": int(n_eval), "draw_s": t_draw, "sampled_pairs_s": t_pairs,
+            "total_s": t_score + t_draw + t_pairs, "speedup_vs_fp64_full": t_full / (t_score + t_draw + t_pairs),
+        }
+        log(f"learned10 {name}: {res[name]}")
+    out = {
+        "metric": "learned importance sampling of the three-phonon linewidth sum: kappa and linewidth "
+                  "error vs the FP64 full sum, time vs the FP64 GPU path",
+        "config": {"workload": f"10-atom cell, {n}^3 mesh with time reversal, {qm.nq0} irreducible q0, "
+                               f"{K} sampled q1 per q0 ({args.budget:.0%} of Nq), all branches per pair",
+                   "sigma_rad_per_ps": args.sigma, "T_K": args.T, "training_samples": int(y.size),
+                   "network": "12 -> 128 -> 128 -> 1 tanh, FP32 Adam training, BF16 MFMA scoring"},
+        "fp64_full_s": t_full, "phonons_s": t_ph, "training_set_s": t_set, "training_s": t_train,
+        "training_loss": [float(prop.losses[0]), float(prop.losses[-1])],
+        "proposals": res,
     }
     print(json.dumps(out), flush=True)
 

@@ -157,6 +157,16 @@ int spr_mlp_train(spr_mlp *m, const float *X, const float *y, long n, int epochs
 /* prec: SPR_PREC_FP32 or SPR_PREC_BF16 */
 int spr_mlp_predict(spr_mlp *m, const float *X, long n, int prec, float *out);
 
+/* Proposal scores for importance sampling of the linewidth sum of irreducible q0 q0_index(k),
+ * k < n: S_out(nq, n), S(q1) = sum over branch triplets with min|w0 -+ w1 -+ w2| < window*sigma of
+ * exp(y_std * f(x)) |G|, G the occupation-weighted Gaussians of CalculateLinewidth
+ * (ThirdOrder_m.f90:514-537) and f the network's normalised log W prediction for the triplet's
+ * feature row (reduced q0, q1; w0, w1, w2 / wmax; s0, s1, s2 / (Ndof - 1)).  Evaluated on the
+ * device with BF16 matrix cores; n_eval (optional) receives the number of network evaluations.
+ * No counterpart in the reference. */
+int spr_mlp_score(spr_ctx *ctx, spr_mlp *m, int n, const int *q0_index, double sigma, double window,
+                  float y_std, float wmax, double *S_out, long long *n_eval);
+
 /* Timing of the last launch of the dominant kernel (HIP events on the context's stream):
  * number of launches and summed milliseconds since the last reset. */
 int spr_kernel_time(spr_ctx *ctx, int *launches, double *ms);

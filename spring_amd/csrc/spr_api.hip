@@ -147,10 +147,6 @@ static void free_ifc(DevIFC &d) {
 
 using namespace spr;
 
-struct spr_ctx {
-    Ctx c;
-};
-
 extern "C" const char *spr_last_error(void) { return spr::g_err.c_str(); }
 
 extern "C" int spr_ctx_create(const spr_crystal *cr, const spr_ifc *fc3, const spr_ifc *fc4,

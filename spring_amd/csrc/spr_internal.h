@@ -136,3 +136,8 @@ int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_m
 int launch_lw3_tetra(Ctx &c, int qi, double *d_lw, double *d_p3);
 
 }  // namespace spr
+
+// the opaque handle of include/spring_gpu.h
+struct spr_ctx {
+    spr::Ctx c;
+};

@@ -20,7 +20,7 @@ SYMBOLS = [
     "spr_lw3_all", "spr_lw4", "spr_lw4_all", "spr_kappa_rta", "spr_kernel_time",
     "spr_kernel_time_reset", "spr_lw3_pairs", "spr_v3_w_pairs", "spr_mlp_create", "spr_mlp_destroy",
     "spr_mlp_num_params", "spr_mlp_get_params", "spr_mlp_set_params", "spr_mlp_grad",
-    "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all",
+    "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all", "spr_mlp_score",
 ]
 
 
@@ -271,6 +271,16 @@ class Mlp:
         self._check(self.lib.spr_mlp_train(self.h, _ptr(X), _ptr(y), ct.c_long(y.size), ct.c_int(epochs),
                                            ct.c_int(batch), ct.c_float(lr), _ptr(losses)), "spr_mlp_train")
         return losses
+
+    def score(self, ctx: "GpuContext", q0_index, sigma: float, window: float, y_std: float, wmax: float):
+        """Device proposal scores S(q1) (n, nq) for irreducible q0 indices and the evaluation count."""
+        q0 = np.ascontiguousarray(q0_index, np.int32)
+        S = np.zeros((q0.size, ctx.nq))
+        ne = ct.c_longlong()
+        self._check(self.lib.spr_mlp_score(ctx.h, self.h, ct.c_int(q0.size), _ptr(q0), ct.c_double(sigma),
+                                           ct.c_double(window), ct.c_float(y_std), ct.c_float(wmax), _ptr(S),
+                                           ct.byref(ne)), "spr_mlp_score")
+        return S, ne.value
 
     def predict(self, X, prec: str = "fp32") -> np.ndarray:
         X = np.ascontiguousarray(X, np.float32)

@@ -64,10 +64,46 @@ def test_learned_proposal_beats_uniform(rocksalt):
         t = ctx.lw3_pairs(np.full(qm.nq, qi), np.arange(qm.nq), sigma, "fp64")
         sig = np.abs(t.sum(axis=0)) > 1e-6 * np.abs(t.sum(axis=0)).max()
         pu = np.full(qm.nq, 1.0 / qm.nq)
-        pp = Lrn.mixture(Lrn.proposal_scores(None, qm, ph, ig, sigma)[0])
-        pl = Lrn.mixture(Lrn.proposal_scores(prop, qm, ph, ig, sigma)[0])
+        pp = Lrn.mixture(Lrn.device_scores(None, ctx, [qi], sigma)[0][0])
+        pl = Lrn.mixture(Lrn.device_scores(prop, ctx, [qi], sigma)[0][0])
         for name, p in (("uniform", pu), ("physics", pp), ("learned", pl)):
             errs[name].append(Lrn.is_rms_error(t, p, K)[sig])
     e = {k: float(np.mean(np.concatenate(v))) for k, v in errs.items()}
     print("mean relative RMS error at 5% budget:", e, flush=True)
     assert e["learned"] < e["uniform"]
+    assert e["physics"] < e["uniform"]
+
+
+def test_device_scores_match_host_definition(rocksalt):
+    """spr_mlp_score (features, resonance window, BF16 network, exp(sd f) |G| sums on the
+    device) against the same definition evaluated on the host with the FP32 network."""
+    from spring_amd import learned as Lrn
+    from spring_amd.lib import GpuContext, Mlp
+    qm, ph = problem(rocksalt, (6, 6, 6))
+    ctx = GpuContext(rocksalt, qm, ph)
+    mlp = Mlp(Lrn.NF, seed=5)
+    prop = Lrn.Proposal(mlp, 0.3, 1.7, float(np.abs(ph.omega).max()))
+    sigma, window = 0.3, 3.0
+    qis = [1, 4, qm.nq0 - 1]
+    S, n_eval = Lrn.device_scores(prop, ctx, qis, sigma, window)
+    n_ref = 0
+    for k, qi in enumerate(qis):
+        ig = int(qm.irr_idx[qi])
+        q1 = np.arange(qm.nq)
+        q2 = qm.index(-qm.q_int[ig][None, :].astype(np.int64) - qm.q_int.astype(np.int64))
+        w0 = ph.omega[ig][None, :, None, None]
+        w1 = ph.omega[q1][:, None, :, None]
+        w2 = ph.omega[q2][:, None, None, :]
+        d = np.minimum(np.abs(w0 - w1 - w2), np.minimum(np.abs(w0 + w1 - w2), np.abs(w0 - w1 + w2)))
+        sel = np.nonzero(d < window * sigma)
+        G = np.abs(Lrn.gauss_weights(ph, ig, q1, q2, sigma))
+        X = Lrn.features(qm, ph, ig, q1[sel[0]], sel[1], sel[2], sel[3], prop.wmax)
+        f = mlp.predict(X, "fp32")
+        ref = np.zeros(qm.nq)
+        np.add.at(ref, sel[0], np.exp(prop.sd * f) * G[sel])
+        n_ref += X.shape[0]
+        big = ref > 1e-3 * ref.max()
+        assert np.max(np.abs(S[k] - ref)[big] / ref[big]) < 5e-2
+        assert abs(S[k].sum() - ref.sum()) < 2e-2 * ref.sum()
+        assert np.all(S[k][ref == 0] == 0)
+    assert n_eval == n_ref

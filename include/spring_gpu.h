@@ -163,7 +163,7 @@ int spr_mlp_predict(spr_mlp *m, const float *X, long n, int prec, float *out);
  * (ThirdOrder_m.f90:514-537) and f the network's normalised log W prediction for the triplet's
  * feature row (reduced q0, q1; w0, w1, w2 / wmax; s0, s1, s2 / (Ndof - 1)).  Evaluated on the
  * device with BF16 matrix cores; n_eval (optional) receives the number of network evaluations.
- * No counterpart in the reference. */
+ * m == NULL scores |G| alone (the proposal without a network).  No counterpart in the reference. */
 int spr_mlp_score(spr_ctx *ctx, spr_mlp *m, int n, const int *q0_index, double sigma, double window,
                   float y_std, float wmax, double *S_out, long long *n_eval);
 

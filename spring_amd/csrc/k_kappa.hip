@@ -45,14 +45,22 @@ __global__ void k_kappa_terms(int nq_terms, int nd, const int *qgrid, const int 
     for (int k = 0; k < 9; ++k) out[k] = c * k9[k];
 }
 
-// mode_k[s][k] = f * sum_t terms[t][s][k] in t order; one thread per (s, k)
-__global__ void k_kappa_reduce(int nq_terms, int nd, const double *terms, double f, double *mode_k) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nd * 9) return;
+// mode_k[s][k] = f * sum_t terms[t][s][k]; one workgroup per (s, k): strided partial sums per
+// thread, then a tree over the 256 partials (the order is fixed, so the result is reproducible)
+__global__ __launch_bounds__(256) void k_kappa_reduce(int nq_terms, int nd, const double *terms, double f,
+                                                      double *mode_k) {
+    const int i = blockIdx.x;
     const int s = i / 9, k = i % 9;
     double acc = 0.0;
-    for (int t = 0; t < nq_terms; ++t) acc += terms[((size_t)t * nd + s) * 9 + k];
-    mode_k[i] = acc * f;
+    for (int t = threadIdx.x; t < nq_terms; t += 256) acc += terms[((size_t)t * nd + s) * 9 + k];
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mode_k[i] = red[0] * f;
 }
 
 }  // namespace spr
@@ -74,51 +82,28 @@ int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_k
         return SPR_E_STATE;
     }
     const int nt = symm ? c.nq0 : c.nq;
-    std::vector<int> lwpos(nt);
-    for (int t = 0; t < nt; ++t) lwpos[t] = symm ? t : c.h_pos[t];
-    int *d_pos = nullptr, *d_mult = nullptr;
-    double *d_rot = nullptr, *d_terms = nullptr;
-    auto cleanup = [&]() {
-        (void)hipFree(d_pos);
-        (void)hipFree(d_mult);
-        (void)hipFree(d_rot);
-        (void)hipFree(d_terms);
-    };
-    if (hipMalloc((void **)&d_pos, nt * sizeof(int)) != hipSuccess ||
-        hipMalloc((void **)&d_terms, (size_t)nt * nd * 9 * sizeof(double)) != hipSuccess) {
-        cleanup();
-        set_error("kappa: hipMalloc failed");
-        return SPR_E_OOM;
+    int *&d_pos = symm ? c.k_pos_irr : c.k_pos_full;
+    if (!c.k_terms) SPR_HIP(hipMalloc((void **)&c.k_terms, (size_t)c.nq * nd * 9 * sizeof(double)));
+    if (!d_pos) {
+        std::vector<int> lwpos(nt);
+        for (int t = 0; t < nt; ++t) lwpos[t] = symm ? t : c.h_pos[t];
+        SPR_HIP(hipMalloc((void **)&d_pos, nt * sizeof(int)));
+        SPR_HIP(hipMemcpy(d_pos, lwpos.data(), nt * sizeof(int), hipMemcpyHostToDevice));
     }
-    if (hipMemcpy(d_pos, lwpos.data(), nt * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
-        cleanup();
-        set_error("kappa: upload failed");
-        return SPR_E_HIP;
-    }
-    if (symm) {
-        if (hipMalloc((void **)&d_mult, nt * sizeof(int)) != hipSuccess ||
-            hipMalloc((void **)&d_rot, c.h_rot.size() * sizeof(double)) != hipSuccess ||
-            hipMemcpy(d_mult, c.h_mult.data(), nt * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(d_rot, c.h_rot.data(), c.h_rot.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-            cleanup();
-            set_error("kappa: symm upload failed");
-            return SPR_E_HIP;
-        }
+    if (symm && !c.k_mult) {
+        SPR_HIP(hipMalloc((void **)&c.k_mult, nt * sizeof(int)));
+        SPR_HIP(hipMalloc((void **)&c.k_rot, c.h_rot.size() * sizeof(double)));
+        SPR_HIP(hipMemcpy(c.k_mult, c.h_mult.data(), nt * sizeof(int), hipMemcpyHostToDevice));
+        SPR_HIP(hipMemcpy(c.k_rot, c.h_rot.data(), c.h_rot.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     const int n = nt * nd;
     hipLaunchKernelGGL(k_kappa_terms, dim3((n + 255) / 256), dim3(256), 0, c.stream, nt, nd,
                        symm ? c.irr_grid : nullptr, d_pos, c.omega, c.nbe, c.vel, d_lw_all,
-                       symm ? d_rot : nullptr, c.nsym, d_mult, d_terms);
+                       symm ? c.k_rot : nullptr, c.nsym, symm ? c.k_mult : nullptr, c.k_terms);
     const double kw = (1.054571817 * 1.054571817) * 0.001 / 1.380649;   // constants.f90:65
     const double f = kw / (T * T * c.vol) / (double)c.nq;
-    hipLaunchKernelGGL(k_kappa_reduce, dim3((nd * 9 + 63) / 64), dim3(64), 0, c.stream, nt, nd,
-                       d_terms, f, d_kappa_modes);
-    hipError_t e = hipStreamSynchronize(c.stream);
-    cleanup();
-    if (e != hipSuccess) {
-        set_error(std::string("kappa: ") + hipGetErrorString(e));
-        return SPR_E_HIP;
-    }
+    hipLaunchKernelGGL(k_kappa_reduce, dim3(nd * 9), dim3(256), 0, c.stream, nt, nd, c.k_terms, f, d_kappa_modes);
+    SPR_HIP(hipGetLastError());
     return SPR_OK;
 }
 

@@ -119,7 +119,7 @@ int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double
         float best = 1e30f;
         int best_mode = 2;
         float ms;
-        const int sub = std::min(L.nitems, 32 * c.ncu);
+        const int sub = std::min(L.nitems, 8 * c.ncu * std::max(1, L.per_cu));
         if ((rc = time_mode(c, L, 2, sub, &ms))) return rc;   // warm-up (code load, caches)
         for (int m : {0, 1, 2, 4}) {
             if ((rc = time_mode(c, L, m, sub, &ms))) return rc;

@@ -112,14 +112,16 @@ template <int MODE, int NB> struct MmLds {
     using R = typename Sh::R;
     using Z = typename Sh::Z;
     static constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
-    static constexpr size_t TU = cmax((size_t)Sh::MB * Sh::PA * sizeof(R), (size_t)Sh::MT_D * 16 * 3 * sizeof(Z));
+    static constexpr size_t TU = (size_t)Sh::MB * Sh::PA * sizeof(R);
+    static constexpr size_t U2 = (size_t)Sh::MT_D * 16 * 3 * sizeof(Z);
     static constexpr size_t AC = cmax((size_t)Sh::MC * Sh::PA * sizeof(R), (size_t)512 * sizeof(double));
     static constexpr size_t E = (size_t)Sh::ND * Sh::PE * sizeof(Z);
     __host__ __device__ static constexpr size_t tu() { return TU; }
+    __host__ __device__ static constexpr size_t u2() { return U2; }
     __host__ __device__ static constexpr size_t ac() { return AC; }
     __host__ __device__ static constexpr size_t e() { return E; }
     static size_t total(int npmax, int nterm, int nsite, int L) {
-        size_t b = tu() + ac() + 2 * e();
+        size_t b = tu() + u2() + ac() + 2 * e();
         b += (size_t)(Sh::PALL ? nterm : npmax) * sizeof(Z) + (size_t)L * sizeof(Z) + 8 * Sh::ND * sizeof(double);
         if (Sh::PALL) b += (size_t)npmax * 27 * sizeof(R);      // FC of one atom (staged)
         b += (size_t)Sh::ND * Sh::ND * sizeof(Z) + 16;           // e0 of the unit's q0
@@ -279,9 +281,10 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
 
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char *p = smem;
-    R *sT = reinterpret_cast<R *>(p);                 // T rows (alpha, b); then U2 (aliased)
-    Z *sU2 = reinterpret_cast<Z *>(p);                //   U2[(s1, s2 local)][alpha], 3 per row
+    R *sT = reinterpret_cast<R *>(p);                 // T rows (alpha, b), K = (c, p)
     p += L::tu();
+    Z *sU2 = reinterpret_cast<Z *>(p);                // U2[(s1, s2 local)][alpha], 3 per row
+    p += L::u2();
     R *sAC = reinterpret_cast<R *>(p);                // U1 as rows (alpha, s2 local), K = (b, p)
     double *sRed = reinterpret_cast<double *>(p);     //   (end of unit only)
     p += L::ac();
@@ -315,6 +318,8 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
 
     // ---- per unit: zero padding, phase table, q0 modes ----
     for (int i = tid; i < Sh::MC * PA; i += 512) sAC[i] = (R)0;
+    for (int i = tid; i < Sh::MB * PA; i += 512) sT[i] = (R)0;          // padding rows and columns stay 0
+    for (int i = tid; i < Sh::MT_D * 16 * 3; i += 512) sU2[i] = Z{0, 0};
     for (int i = tid; i < ND * PE; i += 512) {
         if (i % PE >= ND) {
             sE1[i] = Z{0, 0};
@@ -398,55 +403,51 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             __syncthreads();
         }
 
+        // cluster phases of atom mu (FP64 mode; the other modes hold every atom's phases)
+        auto phases = [&](int mu) {
+            const int u0 = sBlk[mu * NB * NB], u1 = sBlk[(mu + 1) * NB * NB];
+            for (int u = u0 + tid; u < u1; u += 512) {
+                int m = sM1[A.tup_site[2 * u]] + sM2[A.tup_site[2 * u + 1]];
+                m -= m >= A.L ? A.L : 0;
+                sP[u - u0] = sPh[m];
+            }
+        };
+        // ---- FT: T[(alpha, 3 nu + beta)][2 (3 eta + gamma) + p]; lanes run along the 27
+        //      components of a block, so FC reads are contiguous ----
+        auto ft = [&](int mu) {
+            const int u0 = Sh::PALL ? 0 : sBlk[mu * NB * NB];
+            const R *fcm = Sh::PALL ? sFC : fc + (size_t)u0 * 27;
+            const int ub0 = sBlk[mu * NB * NB];   // first cluster of atom mu
+            for (int e = tid; e < Sh::NE; e += 512) {
+                const int bl = e / 27, j = e % 27;
+                const int nu = bl / NB, eta = bl % NB;
+                const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
+                const int blk = (mu * NB + nu) * NB + eta;
+                const int ua = sBlk[blk], ub = sBlk[blk + 1];
+                R re = 0, im = 0;
+#pragma unroll 4
+                for (int u = ua; u < ub; ++u) {
+                    const R f = fcm[(size_t)(u - ub0) * 27 + j];
+                    const Z ph = sP[u - u0];
+                    re = fma(f, ph.x, re);
+                    im = fma(f, ph.y, im);
+                }
+                R *dst = sT + (al * ND + 3 * nu + be) * PA + 2 * (3 * eta + ga);
+                dst[0] = re;
+                dst[1] = im;
+            }
+        };
+        if constexpr (!Sh::PALL) {
+            phases(0);
+            __syncthreads();
+        }
+        ft(0);
+        __syncthreads();
+
+        // Per atom: [stage next FC; GEMM B] | [GEMM C (+ next phases, FP64)] | [GEMM D + FT of the
+        // next atom] -- three barriers per atom; the phase sum runs beside the GEMM D MFMAs
 #pragma unroll 1
         for (int mu = 0; mu < NB; ++mu) {
-            const int u0 = Sh::PALL ? 0 : sBlk[mu * NB * NB];
-            if constexpr (!Sh::PALL) {
-                // ---- cluster phases of atom mu ----
-                const int u1 = sBlk[(mu + 1) * NB * NB];
-                for (int u = u0 + tid; u < u1; u += 512) {
-                    int m = sM1[A.tup_site[2 * u]] + sM2[A.tup_site[2 * u + 1]];
-                    m -= m >= A.L ? A.L : 0;
-                    sP[u - u0] = sPh[m];
-                }
-                __syncthreads();
-            }
-            // ---- FT: T[(alpha, 3 nu + beta)][2 (3 eta + gamma) + p]; lanes run along the 27
-            //      components of a block, so FC reads are contiguous ----
-            {
-                const R *fcm = Sh::PALL ? sFC : fc + (size_t)u0 * 27;
-                const int ub0 = Sh::PALL ? sBlk[mu * NB * NB] : u0;   // first cluster of atom mu
-                for (int e = tid; e < Sh::NE; e += 512) {
-                    const int bl = e / 27, j = e % 27;
-                    const int nu = bl / NB, eta = bl % NB;
-                    const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
-                    const int blk = (mu * NB + nu) * NB + eta;
-                    const int ua = sBlk[blk], ub = sBlk[blk + 1];
-                    R re = 0, im = 0;
-#pragma unroll 4
-                    for (int u = ua; u < ub; ++u) {
-                        const R f = fcm[(size_t)(u - ub0) * 27 + j];
-                        const Z ph = sP[u - u0];
-                        re = fma(f, ph.x, re);
-                        im = fma(f, ph.y, im);
-                    }
-                    R *dst = sT + (al * ND + 3 * nu + be) * PA + 2 * (3 * eta + ga);
-                    dst[0] = re;
-                    dst[1] = im;
-                }
-            }
-            // padding of T (the region holds U2 between the GEMMs)
-            for (int i = tid; i < Sh::MB * (PA - K2); i += 512) {
-                const int m = i / (PA - K2), k = K2 + i % (PA - K2);
-                sT[m * PA + k] = (R)0;
-            }
-            for (int i = tid; i < (Sh::MB - 3 * ND) * K2; i += 512) {
-                const int m = 3 * ND + i / K2, k = i % K2;
-                sT[m * PA + k] = (R)0;
-            }
-            __syncthreads();
-            if constexpr (Sh::PALL)
-                if (mu + 1 < NB) stage(mu + 1);   // every thread is past the FT of mu
             // ---- GEMM B: U1[(alpha, b)][(s2, q)] = T x E2', s2 in this unit's block
             //      -> sAC[(alpha, s2 - h SB)][(b, q)] ----
             if (bOK) {
@@ -499,6 +500,11 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                     }
                 }
             }
+            if constexpr (Sh::PALL) {
+                __builtin_amdgcn_s_waitcnt(0);   // the next atom's FC has landed
+            } else {
+                if (mu + 1 < NB) phases(mu + 1);   // sP of atom mu was read by the last FT
+            }
             __syncthreads();
             // ---- GEMM D: V[(s1, s2 local)][(s0, q)] += U2[.][(alpha, p)] E0'[(alpha, p)][(s0, q)]
             //      K' = 8: step i, lane group g -> (alpha = g, p = i); alpha = 3 is padding ----
@@ -526,8 +532,8 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                     }
                 }
             }
-            if constexpr (Sh::PALL) __builtin_amdgcn_s_waitcnt(0);   // the next atom's FC has landed
-            __syncthreads();   // sU2 (= sT) is rewritten by the next atom's FT
+            if (mu + 1 < NB) ft(mu + 1);   // T of atom mu was read by GEMM B
+            __syncthreads();
         }
 
         // ---- W and the Gaussian-delta weights; lanes with q = 0 take rows 0,1, q = 1 rows 2,3 ----

@@ -251,21 +251,27 @@ __global__ __launch_bounds__(256) void k_mlp_infer_bf16(const float *param, int 
 //   S(q0, q1) = sum over branch triplets with min_k |d_k| < window sigma of exp(sd f(x)) |G|,
 // d1 = w0 - w1 - w2, d2 = w0 + w1 - w2, d3 = w0 - w1 + w2, G = (n1 + n2 + 1) g(d1) +
 // (n1 - n2) (g(d2) - g(d3)) with g the normalised Gaussian, and f(x) the network's normalised
-// log W prediction for the feature row x of the triplet (the rows of spring_amd.learned.features).
-// Triplets are scanned 256 at a time, resonant ones are compacted in order into an LDS queue and
-// evaluated 64 at a time: layer 1 (K = 32) and layer 2 (128 x 128) on BF16 MFMA with FP32
-// accumulation, tanh(x) = 1 - 2 / (1 + e^{2x}).  Per-q1 sums are added in queue order
-// (deterministic).  Persistent workgroups loop over (q0, 4-q1 chunk) work items.
+// log W prediction for the feature row x of the triplet (the rows of spring_amd.learned.features);
+// without a network (param == NULL) the weight is |G| alone.
+// A workgroup (8 waves) owns a work item (q0, 4 consecutive q1).  A thread takes one (q1, s1, s2)
+// and tests SC_S0R values of s0 per round; resonant triplets are compacted in (thread, s0) order
+// into an LDS queue (exclusive prefix sum of the per-thread counts) and evaluated 128 rows at a
+// time: layer 1 (K = 32) and layer 2 (128 x 128) on BF16 MFMA with FP32 accumulation,
+// tanh(x) = 1 - 2 / (1 + e^{2x}).  Per-q1 sums are formed by fixed-order wave reductions
+// (deterministic).  Persistent workgroups loop over the work items.
 constexpr int SC_QPB = 4;       // q1 per work item
-constexpr int SC_CAP = 512;     // queue capacity (>= 63 + 256)
+constexpr int SC_S0R = 6;       // s0 values tested per thread and round
+constexpr int SC_NT = 512;      // threads
+constexpr int SC_ROWS = 128;    // network rows per tile (16 per wave)
+constexpr int SC_CAP = SC_ROWS - 1 + SC_NT * SC_S0R;   // queue capacity
 constexpr int SC_PW1 = 40;      // row pitch (bf16) of X and W1^T
 constexpr int SC_PW2 = 136;     // row pitch (bf16) of H1 and W2^T
 
 struct ScoreArgs {
-    const float *param;
+    const float *param;          // network parameters, or NULL for |G| alone
     int F;
     const int *q0_grid;
-    int nq0s, nq, nd, m0, m1, m2;
+    int nq0s, nq, m0, m1, m2;
     const double *omega, *nbe;
     double sigma, window;
     float sd, inv_wmax;
@@ -279,216 +285,260 @@ __device__ __forceinline__ float fast_tanh(float x) { return 1.f - __fdividef(2.
 
 __device__ __forceinline__ int fold_coord(int k, int n) { return k > n / 2 ? k - n : k; }
 
-__global__ __launch_bounds__(256) void k_mlp_score(ScoreArgs A) {
+template <int ND>
+__global__ __launch_bounds__(SC_NT) void k_mlp_score(ScoreArgs A) {
+    constexpr int ND2 = ND * ND, ND3 = ND2 * ND, NW = SC_NT / 64;
     extern __shared__ __align__(16) unsigned char smsc[];
-    __bf16 *sW1T = reinterpret_cast<__bf16 *>(smsc);            // [MH][SC_PW1]  (j, f)
+    __bf16 *sW1T = reinterpret_cast<__bf16 *>(smsc);         // [MH][SC_PW1]  (j, f)
     __bf16 *sW2T = sW1T + MH * SC_PW1;                        // [MH][SC_PW2]  (j2, j1)
-    __bf16 *sX = sW2T + MH * SC_PW2;                          // [64][SC_PW1]
-    __bf16 *sH1 = sX + 64 * SC_PW1;                           // [64][SC_PW2]
-    float *sB1 = reinterpret_cast<float *>(sH1 + 64 * SC_PW2);
-    float *sB2 = sB1 + MH, *sW3 = sB2 + MH, *sOut = sW3 + MH;   // sOut[64]
-    float *sQG = sOut + 64;                                   // queue |G|
-    int *sQK = reinterpret_cast<int *>(sQG + SC_CAP);         // queue key: ql * nd^3 + triplet
-    double *sMd = reinterpret_cast<double *>(sQK + SC_CAP);   // w0[nd] | w1,n1 [QPB][nd] | w2,n2
+    __bf16 *sX = sW2T + MH * SC_PW2;                          // [SC_ROWS][SC_PW1]
+    __bf16 *sH1 = sX + SC_ROWS * SC_PW1;                      // [SC_ROWS][SC_PW2]
+    float *sB1 = reinterpret_cast<float *>(sH1 + SC_ROWS * SC_PW2);
+    float *sB2 = sB1 + MH, *sW3 = sB2 + MH, *sOut = sW3 + MH;   // sOut[SC_ROWS]
+    float *sQG = sOut + SC_ROWS;                              // queue |G|
+    int *sQK = reinterpret_cast<int *>(sQG + SC_CAP);         // queue key: triplet index in the item
+    double *sMd = reinterpret_cast<double *>(sQK + SC_CAP + (SC_CAP & 1));   // w0 | (w1 n1 w2 n2)[QPB]
     double *sS = sMd + (1 + 4 * SC_QPB) * 32;                 // [SC_QPB]
-    int *sInt = reinterpret_cast<int *>(sS + SC_QPB);         // wave counts [4], work item, misc
+    double *sPq = sS + SC_QPB;                                // [SC_ROWS / 64][SC_QPB]
+    int *sInt = reinterpret_cast<int *>(sPq + (SC_ROWS / 64) * SC_QPB);   // wave totals [NW], work item
     const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, g = l >> 4;
-    const int F = A.F, nd = A.nd, nd2 = nd * nd, nd3 = nd2 * nd;
-    const MlpParamsView P = mlp_view(A.param, F);
-    for (int i = tid; i < MH * SC_PW1; i += 256) {
-        const int j = i / SC_PW1, f = i % SC_PW1;
-        sW1T[i] = (__bf16)(f < F ? P.W1[f * MH + j] : 0.f);
+    const bool net = A.param != nullptr;
+    if (net) {
+        const MlpParamsView P = mlp_view(A.param, A.F);
+        for (int i = tid; i < MH * SC_PW1; i += SC_NT) {
+            const int j = i / SC_PW1, f = i % SC_PW1;
+            sW1T[i] = (__bf16)(f < A.F ? P.W1[f * MH + j] : 0.f);
+        }
+        for (int i = tid; i < MH * MH; i += SC_NT) sW2T[(i % MH) * SC_PW2 + i / MH] = (__bf16)P.W2[i];
+        for (int i = tid; i < MH; i += SC_NT) {
+            sB1[i] = P.b1[i];
+            sB2[i] = P.b2[i];
+            sW3[i] = P.W3[i];
+        }
     }
-    for (int i = tid; i < MH * MH; i += 256) {
-        const int j1 = i / MH, j2 = i % MH;
-        sW2T[j2 * SC_PW2 + j1] = (__bf16)P.W2[i];
-    }
-    for (int i = tid; i < MH; i += 256) {
-        sB1[i] = P.b1[i];
-        sB2[i] = P.b2[i];
-        sW3[i] = P.W3[i];
-    }
-    const float b3 = P.b3[0];
+    const float b3 = net ? A.param[mlp_nparam(A.F) - 1] : 0.f;
     const double c0 = 1.0 / (A.sigma * 2.5066282746310002), hinv = 0.5 / (A.sigma * A.sigma);
     const double win = A.window * A.sigma;
-    const float fnd = nd > 1 ? 1.f / (float)(nd - 1) : 0.f;
+    const float fnd = ND > 1 ? 1.f / (float)(ND - 1) : 0.f;
 
     for (;;) {
         __syncthreads();
-        if (tid == 0) sInt[4] = atomicAdd(A.counter, 1);
+        if (tid == 0) sInt[NW] = atomicAdd(A.counter, 1);
         __syncthreads();
-        const int wi = sInt[4];
+        const int wi = sInt[NW];
         if (wi >= A.nwork) break;
         const int qi = wi / A.nchunk, q1lo = (wi % A.nchunk) * SC_QPB;
         const int nql = min(SC_QPB, A.nq - q1lo);
         const int i0 = A.q0_grid[qi];
         const int k00 = i0 % A.m0, k01 = (i0 / A.m0) % A.m1, k02 = i0 / (A.m0 * A.m1);
-        if (tid < nd) sMd[tid] = A.omega[(size_t)i0 * nd + tid];
-        for (int i = tid; i < nql * nd; i += 256) {
-            const int ql = i / nd, s = i % nd, q1 = q1lo + ql;
+        if (tid < ND) sMd[tid] = A.omega[(size_t)i0 * ND + tid];
+        for (int i = tid; i < nql * ND; i += SC_NT) {
+            const int ql = i / ND, s = i % ND, q1 = q1lo + ql;
             const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
             const int a0 = ((-(k00 + k10)) % A.m0 + A.m0) % A.m0, a1 = ((-(k01 + k11)) % A.m1 + A.m1) % A.m1,
                       a2 = ((-(k02 + k12)) % A.m2 + A.m2) % A.m2;
             const int q2 = a0 + A.m0 * (a1 + A.m1 * a2);
-            sMd[32 + (4 * ql + 0) * 32 + s] = A.omega[(size_t)q1 * nd + s];
-            sMd[32 + (4 * ql + 1) * 32 + s] = A.nbe[(size_t)q1 * nd + s];
-            sMd[32 + (4 * ql + 2) * 32 + s] = A.omega[(size_t)q2 * nd + s];
-            sMd[32 + (4 * ql + 3) * 32 + s] = A.nbe[(size_t)q2 * nd + s];
+            sMd[32 + (4 * ql + 0) * 32 + s] = A.omega[(size_t)q1 * ND + s];
+            sMd[32 + (4 * ql + 1) * 32 + s] = A.nbe[(size_t)q1 * ND + s];
+            sMd[32 + (4 * ql + 2) * 32 + s] = A.omega[(size_t)q2 * ND + s];
+            sMd[32 + (4 * ql + 3) * 32 + s] = A.nbe[(size_t)q2 * ND + s];
         }
         if (tid < SC_QPB) sS[tid] = 0.0;
         __syncthreads();
         const float k0x = (float)fold_coord(k00, A.m0) / A.m0, k0y = (float)fold_coord(k01, A.m1) / A.m1,
                     k0z = (float)fold_coord(k02, A.m2) / A.m2;
 
-        // evaluate queue rows [0, nv) (nv <= 64) and add their contributions to sS
+        // evaluate queue rows [0, nv) (nv <= SC_ROWS) and add their contributions to sS
         auto tile = [&](int nv) {
-            if (tid < 64) {
-                __bf16 *x = sX + tid * SC_PW1;
-                float xs[SC_PW1 > 32 ? 32 : SC_PW1];
+            if (net) {
+                if (tid < SC_ROWS) {
+                    float xs[32];
 #pragma unroll
-                for (int f = 0; f < 32; ++f) xs[f] = 0.f;
-                if (tid < nv) {
-                    const int key = sQK[tid];
-                    const int ql = key / nd3, r = key % nd3;
-                    const int s0 = r / nd2, s1 = (r / nd) % nd, s2 = r % nd;
-                    const int q1 = q1lo + ql;
-                    const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
-                    xs[0] = k0x;
-                    xs[1] = k0y;
-                    xs[2] = k0z;
-                    xs[3] = (float)fold_coord(k10, A.m0) / A.m0;
-                    xs[4] = (float)fold_coord(k11, A.m1) / A.m1;
-                    xs[5] = (float)fold_coord(k12, A.m2) / A.m2;
-                    xs[6] = (float)sMd[s0] * A.inv_wmax;
-                    xs[7] = (float)sMd[32 + (4 * ql + 0) * 32 + s1] * A.inv_wmax;
-                    xs[8] = (float)sMd[32 + (4 * ql + 2) * 32 + s2] * A.inv_wmax;
-                    xs[9] = s0 * fnd;
-                    xs[10] = s1 * fnd;
-                    xs[11] = s2 * fnd;
+                    for (int f = 0; f < 32; ++f) xs[f] = 0.f;
+                    if (tid < nv) {
+                        const int key = sQK[tid];
+                        const int ql = key / ND3, r = key % ND3;
+                        const int s0 = r / ND2, s1 = (r / ND) % ND, s2 = r % ND;
+                        const int q1 = q1lo + ql;
+                        const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
+                        xs[0] = k0x;
+                        xs[1] = k0y;
+                        xs[2] = k0z;
+                        xs[3] = (float)fold_coord(k10, A.m0) / A.m0;
+                        xs[4] = (float)fold_coord(k11, A.m1) / A.m1;
+                        xs[5] = (float)fold_coord(k12, A.m2) / A.m2;
+                        xs[6] = (float)sMd[s0] * A.inv_wmax;
+                        xs[7] = (float)sMd[32 + (4 * ql + 0) * 32 + s1] * A.inv_wmax;
+                        xs[8] = (float)sMd[32 + (4 * ql + 2) * 32 + s2] * A.inv_wmax;
+                        xs[9] = s0 * fnd;
+                        xs[10] = s1 * fnd;
+                        xs[11] = s2 * fnd;
+                    }
+                    __bf16 *x = sX + tid * SC_PW1;
+#pragma unroll
+                    for (int f = 0; f < 32; ++f) x[f] = (__bf16)xs[f];
                 }
+                __syncthreads();
+                // layer 1 (wave w: rows 16 w .. 16 w + 15), K = 32 (features, zero padded)
+                {
+                    bf16x8_m a;
 #pragma unroll
-                for (int f = 0; f < 32; ++f) x[f] = (__bf16)xs[f];
-            }
-            __syncthreads();
-            // layer 1: rows 16 w .. 16 w + 15, K = 32 (features, zero padded)
-            {
-                bf16x8_m a;
+                    for (int t = 0; t < 8; ++t) a[t] = sX[(16 * w + (l & 15)) * SC_PW1 + 8 * g + t];
 #pragma unroll
-                for (int t = 0; t < 8; ++t) a[t] = sX[(16 * w + (l & 15)) * SC_PW1 + 8 * g + t];
+                    for (int nt = 0; nt < MH / 16; ++nt) {
+                        bf16x8_m b;
 #pragma unroll
+                        for (int t = 0; t < 8; ++t) b[t] = sW1T[(nt * 16 + (l & 15)) * SC_PW1 + 8 * g + t];
+                        v4f_m z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, v4f_m{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                        const int j = nt * 16 + (l & 15);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            sH1[(16 * w + 4 * g + r) * SC_PW2 + j] = (__bf16)fast_tanh(z[r] + sB1[j]);
+                    }
+                }
+                __syncthreads();
+                // layer 2 and the output layer
+                float yacc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
                 for (int nt = 0; nt < MH / 16; ++nt) {
-                    bf16x8_m b;
+                    v4f_m z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                    for (int t = 0; t < 8; ++t) b[t] = sW1T[(nt * 16 + (l & 15)) * SC_PW1 + 8 * g + t];
-                    v4f_m z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, v4f_m{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                    for (int kk = 0; kk < MH; kk += 32) {
+                        bf16x8_m a, b;
+#pragma unroll
+                        for (int t = 0; t < 8; ++t) {
+                            a[t] = sH1[(16 * w + (l & 15)) * SC_PW2 + kk + 8 * g + t];
+                            b[t] = sW2T[(nt * 16 + (l & 15)) * SC_PW2 + kk + 8 * g + t];
+                        }
+                        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, z, 0, 0, 0);
+                    }
                     const int j = nt * 16 + (l & 15);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) sH1[(16 * w + 4 * g + r) * SC_PW2 + j] = (__bf16)fast_tanh(z[r] + sB1[j]);
+                    for (int r = 0; r < 4; ++r) yacc[r] = fmaf(fast_tanh(z[r] + sB2[j]), sW3[j], yacc[r]);
                 }
-            }
-            __syncthreads();
-            // layer 2 and the output layer
-            float yacc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-            for (int nt = 0; nt < MH / 16; ++nt) {
-                v4f_m z = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int kk = 0; kk < MH; kk += 32) {
-                    bf16x8_m a, b;
-#pragma unroll
-                    for (int t = 0; t < 8; ++t) {
-                        a[t] = sH1[(16 * w + (l & 15)) * SC_PW2 + kk + 8 * g + t];
-                        b[t] = sW2T[(nt * 16 + (l & 15)) * SC_PW2 + kk + 8 * g + t];
-                    }
-                    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, z, 0, 0, 0);
-                }
-                const int j = nt * 16 + (l & 15);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) yacc[r] = fmaf(fast_tanh(z[r] + sB2[j]), sW3[j], yacc[r]);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                float v = yacc[r];
-                v += __shfl_xor(v, 1, 64);
-                v += __shfl_xor(v, 2, 64);
-                v += __shfl_xor(v, 4, 64);
-                v += __shfl_xor(v, 8, 64);
-                yacc[r] = v;
-            }
-            if ((l & 15) == 0)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int row = 16 * w + 4 * g + r;
-                    sOut[row] = row < nv ? __expf(A.sd * (yacc[r] + b3)) * sQG[row] : 0.f;
+                    float v = yacc[r];
+                    v += __shfl_xor(v, 1, 64);
+                    v += __shfl_xor(v, 2, 64);
+                    v += __shfl_xor(v, 4, 64);
+                    v += __shfl_xor(v, 8, 64);
+                    yacc[r] = v;
                 }
+                if ((l & 15) == 0)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * w + 4 * g + r;
+                        sOut[row] = row < nv ? __expf(A.sd * (yacc[r] + b3)) * sQG[row] : 0.f;
+                    }
+            } else if (tid < SC_ROWS) {
+                sOut[tid] = tid < nv ? sQG[tid] : 0.f;
+            }
             __syncthreads();
-            if (tid == 0)
-                for (int r = 0; r < nv; ++r) sS[sQK[r] / nd3] += (double)sOut[r];
+            // per-q1 sums of the tile: waves 0 and 1 (64 rows each), fixed reduction order
+            if (w < SC_ROWS / 64) {
+                const int row = 64 * w + l;
+                const int ql = row < nv ? sQK[row] / ND3 : -1;
+                const double o = (double)sOut[row];
+                for (int q = 0; q < SC_QPB; ++q) {
+                    double v = ql == q ? o : 0.0;
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                    if (l == 0) sPq[SC_QPB * w + q] = v;
+                }
+            }
+            __syncthreads();
+            if (tid < SC_QPB) {
+                double add = 0.0;
+                for (int ww = 0; ww < SC_ROWS / 64; ++ww) add += sPq[SC_QPB * ww + tid];
+                sS[tid] += add;
+            }
+            __syncthreads();
         };
 
         int qn = 0;
         unsigned long long nres = 0;
-        const int ntri = nql * nd3;
-        for (int base = 0; base < ntri; base += 256) {
-            const int t = base + tid;
-            bool res = false;
-            float G = 0.f;
-            if (t < ntri) {
-                const int ql = t / nd3, r = t % nd3;
-                const int s0 = r / nd2, s1 = (r / nd) % nd, s2 = r % nd;
-                const double w0 = sMd[s0];
-                const double w1 = sMd[32 + (4 * ql + 0) * 32 + s1], n1 = sMd[32 + (4 * ql + 1) * 32 + s1];
-                const double w2 = sMd[32 + (4 * ql + 2) * 32 + s2], n2 = sMd[32 + (4 * ql + 3) * 32 + s2];
-                const double d1 = w0 - w1 - w2, d2 = w0 + w1 - w2, d3 = w0 - w1 + w2;
-                res = fmin(fabs(d1), fmin(fabs(d2), fabs(d3))) < win;
-                if (res)
-                    G = (float)fabs((n1 + n2 + 1.0) * c0 * exp(-hinv * d1 * d1) +
-                                    (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3)));
-            }
-            // ordered compaction into the queue
-            const unsigned long long mask = __ballot(res);
-            const int rank = __popcll(mask & ((1ull << l) - 1ull));
-            if (l == 0) sInt[w] = __popcll(mask);
-            __syncthreads();
-            int off = qn;
-            for (int ww = 0; ww < w; ++ww) off += sInt[ww];
-            const int added = sInt[0] + sInt[1] + sInt[2] + sInt[3];
-            if (res) {
-                sQK[off + rank] = t;
-                sQG[off + rank] = G;
-            }
-            qn += added;
-            nres += (unsigned long long)added;
-            __syncthreads();
-            while (qn >= 64) {
-                tile(64);
-                // move the remaining entries to the front
-                int kq = 0;
-                float gq = 0.f;
-                const bool mv = tid + 64 < qn;
-                if (mv) {
-                    kq = sQK[tid + 64];
-                    gq = sQG[tid + 64];
+        const int npair = nql * ND2;      // (q1, s1, s2) of the item
+        for (int cb = 0; cb < npair; cb += SC_NT) {
+            const int pidx = cb + tid;
+            const bool pv = pidx < npair;
+            const int ql = pv ? pidx / ND2 : 0, s1 = pv ? (pidx / ND) % ND : 0, s2 = pv ? pidx % ND : 0;
+            const double w1 = sMd[32 + (4 * ql + 0) * 32 + s1], n1 = sMd[32 + (4 * ql + 1) * 32 + s1];
+            const double w2 = sMd[32 + (4 * ql + 2) * 32 + s2], n2 = sMd[32 + (4 * ql + 3) * 32 + s2];
+            const double ap = w1 + w2, am = w1 - w2;   // d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am
+            for (int s0b = 0; s0b < ND; s0b += SC_S0R) {
+                unsigned bits = 0;
+#pragma unroll
+                for (int k = 0; k < SC_S0R; ++k) {
+                    const int s0 = s0b + k;
+                    if (pv && s0 < ND) {
+                        const double w0 = sMd[s0];
+                        const double dm = fmin(fabs(w0 - ap), fmin(fabs(w0 + am), fabs(w0 - am)));
+                        if (dm < win) bits |= 1u << k;
+                    }
                 }
-                __syncthreads();
-                if (mv) {
-                    sQK[tid] = kq;
-                    sQG[tid] = gq;
+                const int cntk = __popc(bits);
+                // exclusive prefix sum of cntk over the workgroup, in thread order
+                int incl = cntk;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int v = __shfl_up(incl, off, 64);
+                    if (l >= off) incl += v;
                 }
-                qn -= 64;
+                if (l == 63) sInt[w] = incl;
                 __syncthreads();
+                int off = qn, total = 0;
+                for (int ww = 0; ww < NW; ++ww) {
+                    const int c = sInt[ww];
+                    if (ww < w) off += c;
+                    total += c;
+                }
+                off += incl - cntk;
+                for (int k = 0; k < SC_S0R; ++k)
+                    if (bits & (1u << k)) {
+                        const int s0 = s0b + k;
+                        const double w0 = sMd[s0];
+                        const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
+                        sQK[off] = ql * ND3 + s0 * ND2 + s1 * ND + s2;
+                        sQG[off] = (float)fabs((n1 + n2 + 1.0) * c0 * exp(-hinv * d1 * d1) +
+                                               (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3)));
+                        ++off;
+                    }
+                qn += total;
+                nres += (unsigned long long)total;
+                __syncthreads();
+                while (qn >= SC_ROWS) {
+                    tile(SC_ROWS);
+                    // move the remaining entries to the front (in order)
+                    for (int base = SC_ROWS; base < qn; base += SC_NT) {
+                        const int i = base + tid;
+                        int kq = 0;
+                        float gq = 0.f;
+                        if (i < qn) {
+                            kq = sQK[i];
+                            gq = sQG[i];
+                        }
+                        __syncthreads();
+                        if (i < qn) {
+                            sQK[i - SC_ROWS] = kq;
+                            sQG[i - SC_ROWS] = gq;
+                        }
+                        __syncthreads();
+                    }
+                    qn -= SC_ROWS;
+                }
             }
         }
         if (qn > 0) tile(qn);
         __syncthreads();
         if (tid < nql) A.S[(size_t)qi * A.nq + q1lo + tid] = sS[tid];
-        if (tid == 0) atomicAdd(A.n_eval, nres);
+        if (tid == 0 && net) atomicAdd(A.n_eval, nres);
     }
 }
 
 size_t score_lds() {
-    return (size_t)(MH * SC_PW1 + MH * SC_PW2 + 64 * SC_PW1 + 64 * SC_PW2) * 2 + (3 * MH + 64) * 4 +
-           SC_CAP * 8 + ((1 + 4 * SC_QPB) * 32 + SC_QPB) * 8 + 8 * 4;
+    return (size_t)(MH * SC_PW1 + MH * SC_PW2 + SC_ROWS * SC_PW1 + SC_ROWS * SC_PW2) * 2 + (3 * MH + SC_ROWS) * 4 +
+           (SC_CAP + 1) * 8 + ((1 + 4 * SC_QPB) * 32 + SC_QPB + (SC_ROWS / 64) * SC_QPB) * 8 + 16 * 4;
 }
 
 }  // namespace spr
@@ -686,12 +736,12 @@ extern "C" int spr_mlp_predict(spr_mlp *h, const float *X, long n, int prec, flo
 
 extern "C" int spr_mlp_score(spr_ctx *ctx, spr_mlp *m, int n, const int *q0_index, double sigma, double window,
                              float y_std, float wmax, double *S_out, long long *n_eval) {
-    if (!ctx || !m || n < 1 || !q0_index || !S_out || sigma <= 0.0 || window <= 0.0 || wmax <= 0.f) {
+    if (!ctx || n < 1 || !q0_index || !S_out || sigma <= 0.0 || window <= 0.0 || wmax <= 0.f) {
         set_error("spr_mlp_score: bad argument");
         return SPR_E_ARG;
     }
     Ctx &c = ctx->c;
-    if (m->F != 12) {
+    if (m && m->F != 12) {
         set_error("spr_mlp_score: the network must take the 12 proposal features");
         return SPR_E_ARG;
     }
@@ -733,12 +783,11 @@ extern "C" int spr_mlp_score(spr_ctx *ctx, spr_mlp *m, int n, const int *q0_inde
             break;
         }
         ScoreArgs A;
-        A.param = m->param;
-        A.F = m->F;
+        A.param = m ? m->param : nullptr;
+        A.F = m ? m->F : 12;
         A.q0_grid = d_grid;
         A.nq0s = n;
         A.nq = c.nq;
-        A.nd = c.nd;
         A.m0 = c.mesh[0];
         A.m1 = c.mesh[1];
         A.m2 = c.mesh[2];
@@ -754,31 +803,48 @@ extern "C" int spr_mlp_score(spr_ctx *ctx, spr_mlp *m, int n, const int *q0_inde
         A.S = d_S;
         A.n_eval = d_ne;
         const size_t lds = score_lds();
-        if (hipFuncSetAttribute((const void *)k_mlp_score, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-            hipSuccess) {
+        const void *fn = nullptr;
+        switch (c.nd) {
+            case 6: fn = (const void *)k_mlp_score<6>; break;
+            case 9: fn = (const void *)k_mlp_score<9>; break;
+            case 12: fn = (const void *)k_mlp_score<12>; break;
+            case 15: fn = (const void *)k_mlp_score<15>; break;
+            case 18: fn = (const void *)k_mlp_score<18>; break;
+            case 24: fn = (const void *)k_mlp_score<24>; break;
+            case 30: fn = (const void *)k_mlp_score<30>; break;
+            default: break;
+        }
+        if (!fn) {
+            set_error("spr_mlp_score: Ndof " + std::to_string(c.nd) + " not built (6-18, 24, 30)");
+            rc = SPR_E_UNSUPPORTED;
+            break;
+        }
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
             set_error("spr_mlp_score: LDS attribute failed");
             rc = SPR_E_HIP;
             break;
         }
-        const int grid_n = std::min(A.nwork, 2 * c.ncu);
-        hipEvent_t e0, e1;
-        (void)hipEventCreate(&e0);
-        (void)hipEventCreate(&e1);
-        (void)hipEventRecord(e0, c.stream);
-        hipLaunchKernelGGL(k_mlp_score, dim3(grid_n), dim3(256), lds, c.stream, A);
-        (void)hipEventRecord(e1, c.stream);
+        const int grid_n = std::min(A.nwork, c.ncu);
+        (void)hipEventRecord(c.ev0, c.stream);
+        {
+            void *args[] = {&A};
+            if (hipLaunchKernel(fn, dim3(grid_n), dim3(SC_NT), args, lds, c.stream) != hipSuccess) {
+                set_error("spr_mlp_score: launch failed");
+                rc = SPR_E_HIP;
+                break;
+            }
+        }
+        (void)hipEventRecord(c.ev1, c.stream);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c.stream) != hipSuccess) {
             set_error("spr_mlp_score: kernel failed");
             rc = SPR_E_HIP;
             break;
         }
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+        if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess) {
             c.launches += 1;
             c.kernel_ms += ms;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         unsigned long long ne = 0;
         if (hipMemcpy(S_out, d_S, (size_t)n * c.nq * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(&ne, d_ne, sizeof(ne), hipMemcpyDeviceToHost) != hipSuccess) {

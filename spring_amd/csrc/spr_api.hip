@@ -276,7 +276,9 @@ extern "C" int spr_ctx_destroy(spr_ctx *h) {
                     (void *)c.irr_grid, (void *)c.partial, (void *)c.out, (void *)c.counter,
                     (void *)c.d_pairs, (void *)c.d_item_q0, (void *)c.d_item_off, (void *)c.d_item_cnt,
                     (void *)c.d_item_first,
-                    (void *)c.ipartial, (void *)c.wout, (void *)c.tetra, (void *)c.partner, (void *)c.tout})
+                    (void *)c.ipartial, (void *)c.wout, (void *)c.tetra, (void *)c.partner, (void *)c.tout,
+                    (void *)c.k_terms, (void *)c.k_io, (void *)c.k_rot, (void *)c.k_pos_full, (void *)c.k_pos_irr,
+                    (void *)c.k_mult})
         if (p) (void)hipFree(p);
     if (c.ev0) (void)hipEventDestroy(c.ev0);
     if (c.ev1) (void)hipEventDestroy(c.ev1);
@@ -397,23 +399,20 @@ extern "C" int spr_kappa_rta(spr_ctx *h, const double *lw_all, double T, int sym
     Ctx &c = h->c;
     SPR_HIP(hipSetDevice(c.device));
     const size_t nl = (size_t)c.nq0 * c.nd, nm = (size_t)c.nd * 9;
-    double *d = nullptr;
-    if (hipMalloc((void **)&d, (nl + nm) * sizeof(double)) != hipSuccess) {
-        set_error("spr_kappa_rta: hipMalloc failed");
-        return SPR_E_OOM;
-    }
+    if (!c.k_io) SPR_HIP(hipMalloc((void **)&c.k_io, (nl + nm) * sizeof(double)));
+    double *d = c.k_io;
     int rc = SPR_OK;
     std::vector<double> modes(nm);
-    if (hipMemcpy(d, lw_all, nl * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpyAsync(d, lw_all, nl * sizeof(double), hipMemcpyHostToDevice, c.stream) != hipSuccess) {
         set_error("spr_kappa_rta: upload failed");
         rc = SPR_E_HIP;
     }
     if (!rc) rc = launch_kappa(c, d, T, symm, d + nl);
-    if (!rc && hipMemcpy(modes.data(), d + nl, nm * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (!rc && (hipMemcpyAsync(modes.data(), d + nl, nm * sizeof(double), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+                hipStreamSynchronize(c.stream) != hipSuccess)) {
         set_error("spr_kappa_rta: download failed");
         rc = SPR_E_HIP;
     }
-    (void)hipFree(d);
     if (rc) return rc;
     for (int k = 0; k < 9; ++k) {
         double s = 0.0;

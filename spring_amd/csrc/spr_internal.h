@@ -111,6 +111,9 @@ struct Ctx {
     int *tetra = nullptr, *partner = nullptr;   // (6 nq, 4) tetrahedra; q2 partner of every q1
     std::vector<int> h_tetra;
     double *tout = nullptr;        // tetrahedron outputs: lw then P3_q0, (2, nq0, nd)
+    // kappa buffers, allocated on first use: per-mode terms, lw / mode sums, index tables
+    double *k_terms = nullptr, *k_io = nullptr, *k_rot = nullptr;
+    int *k_pos_full = nullptr, *k_pos_irr = nullptr, *k_mult = nullptr;
     const ItemSet *override_items = nullptr;
     const int *w_pair_q0 = nullptr, *w_pair_q1 = nullptr;   // W parity path over a pair list
     int w_npairs = 0;

@@ -127,16 +127,14 @@ def proposal_scores(prop: Proposal | None, qm, ph, i0g, sigma, resonant=1e-3, pr
     return S, n_eval
 
 
-def device_scores(prop: Proposal | None, ctx, q0_index, sigma, window=3.0, mlp_zero=None):
+def device_scores(prop: Proposal | None, ctx, q0_index, sigma, window=3.0):
     """S(q1) of every q1 for each irreducible q0 index, computed on the GPU (spr_mlp_score):
     triplets with min |w0 -+ w1 -+ w2| < window sigma, weight exp(sd f(x)) |G|.  prop=None gives
-    the G-only proposal (a network that predicts a constant).  Returns (S (n, nq), n_evals)."""
+    the G-only proposal (no network).  Returns (S (n, nq), n_evals)."""
+    from . import lib
     if prop is None:
-        from .lib import Mlp
-        net = mlp_zero if mlp_zero is not None else Mlp(NF, seed=0)
-        net.set_params(np.zeros(net.n_params, np.float32))
-        return net.score(ctx, q0_index, sigma, window, 1.0, 1.0)
-    return prop.mlp.score(ctx, q0_index, sigma, window, prop.sd, prop.wmax)
+        return lib.score(ctx, None, q0_index, sigma, window)
+    return lib.score(ctx, prop.mlp, q0_index, sigma, window, prop.sd, prop.wmax)
 
 
 def mixture(S, lam=0.1):

@@ -107,7 +107,8 @@ class GpuContext:
         self._mesh = _Mesh((ct.c_int * 3)(*[int(x) for x in qm.mesh]), qm.nq,
                            _ptr(arr(qm.q_int, np.int32)), qm.nq0, _ptr(arr(qm.irr_idx, np.int32)),
                            _ptr(arr(qm.pos, np.int32)), _ptr(arr(qm.multiplicity, np.int32)),
-                           rot.shape[0], _ptr(rotf), _ptr(arr(qm.tetra, np.int32)))
+                           rot.shape[0], _ptr(rotf),
+                           _ptr(arr(qm.tetra, np.int32)) if getattr(qm, "tetra", None) is not None else None)
         self._ph = _Phonons(ph.nq, self.nd, _ptr(arr(ph.evec, np.complex128)),
                             _ptr(arr(ph.omega, np.float64)), _ptr(arr(ph.inv_omega, np.float64)),
                             _ptr(arr(ph.nbe, np.float64)), _ptr(arr(ph.grp_vel, np.float64)))
@@ -274,13 +275,7 @@ class Mlp:
 
     def score(self, ctx: "GpuContext", q0_index, sigma: float, window: float, y_std: float, wmax: float):
         """Device proposal scores S(q1) (n, nq) for irreducible q0 indices and the evaluation count."""
-        q0 = np.ascontiguousarray(q0_index, np.int32)
-        S = np.zeros((q0.size, ctx.nq))
-        ne = ct.c_longlong()
-        self._check(self.lib.spr_mlp_score(ctx.h, self.h, ct.c_int(q0.size), _ptr(q0), ct.c_double(sigma),
-                                           ct.c_double(window), ct.c_float(y_std), ct.c_float(wmax), _ptr(S),
-                                           ct.byref(ne)), "spr_mlp_score")
-        return S, ne.value
+        return score(ctx, self, q0_index, sigma, window, y_std, wmax)
 
     def predict(self, X, prec: str = "fp32") -> np.ndarray:
         X = np.ascontiguousarray(X, np.float32)
@@ -288,3 +283,19 @@ class Mlp:
         self._check(self.lib.spr_mlp_predict(self.h, _ptr(X), ct.c_long(X.shape[0]), ct.c_int(PREC[prec]),
                                              _ptr(out)), "spr_mlp_predict")
         return out
+
+
+def score(ctx: GpuContext, mlp: "Mlp | None", q0_index, sigma: float, window: float, y_std: float = 1.0,
+          wmax: float = 1.0):
+    """spr_mlp_score: S(q1) (n, nq) for irreducible q0 indices and the number of network
+    evaluations; mlp=None scores |G| alone."""
+    lib = load()
+    q0 = np.ascontiguousarray(q0_index, np.int32)
+    S = np.zeros((q0.size, ctx.nq))
+    ne = ct.c_longlong()
+    rc = lib.spr_mlp_score(ctx.h, mlp.h if mlp is not None else None, ct.c_int(q0.size), _ptr(q0),
+                           ct.c_double(sigma), ct.c_double(window), ct.c_float(y_std), ct.c_float(wmax), _ptr(S),
+                           ct.byref(ne))
+    if rc != 0:
+        raise SprError(f"spr_mlp_score failed ({rc}): {lib.spr_last_error().decode()}")
+    return S, ne.value

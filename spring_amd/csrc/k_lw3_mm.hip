@@ -448,6 +448,8 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
         // next atom] -- three barriers per atom; the phase sum runs beside the GEMM D MFMAs
 #pragma unroll 1
         for (int mu = 0; mu < NB; ++mu) {
+            if constexpr (Sh::PALL)
+                if (mu + 1 < NB) stage(mu + 1);   // sFC of atom mu was read by the last FT
             // ---- GEMM B: U1[(alpha, b)][(s2, q)] = T x E2', s2 in this unit's block
             //      -> sAC[(alpha, s2 - h SB)][(b, q)] ----
             if (bOK) {

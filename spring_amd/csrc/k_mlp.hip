@@ -253,17 +253,17 @@ __global__ __launch_bounds__(256) void k_mlp_infer_bf16(const float *param, int 
 // (n1 - n2) (g(d2) - g(d3)) with g the normalised Gaussian, and f(x) the network's normalised
 // log W prediction for the feature row x of the triplet (the rows of spring_amd.learned.features);
 // without a network (param == NULL) the weight is |G| alone.
-// A workgroup (8 waves) owns a work item (q0, 4 consecutive q1).  A thread takes one (q1, s1, s2)
-// and tests SC_S0R values of s0 per round; resonant triplets are compacted in (thread, s0) order
-// into an LDS queue (exclusive prefix sum of the per-thread counts) and evaluated 128 rows at a
-// time: layer 1 (K = 32) and layer 2 (128 x 128) on BF16 MFMA with FP32 accumulation,
-// tanh(x) = 1 - 2 / (1 + e^{2x}).  Per-q1 sums are formed by fixed-order wave reductions
-// (deterministic).  Persistent workgroups loop over the work items.
+// A workgroup (4 waves) owns a work item (q0, 4 consecutive q1); each wave works alone on its
+// share of the (q1, s1, s2) combinations: a lane tests every s0 of its combination (a bit mask),
+// resonant triplets are appended in lane order to the wave's LDS queue, and every 16 queued rows
+// run through the network on the wave's own tile (layer 1, K = 32, and layer 2, 128 x 128, on
+// BF16 MFMA with FP32 accumulation, tanh(x) = 1 - 2 / (1 + e^{2x})).  Per-q1 sums are formed in a
+// fixed order (lane, wave), so the scores are reproducible.  Persistent workgroups loop over the
+// work items.
 constexpr int SC_QPB = 4;       // q1 per work item
-constexpr int SC_S0R = 6;       // s0 values tested per thread and round
-constexpr int SC_NT = 512;      // threads
-constexpr int SC_ROWS = 128;    // network rows per tile (16 per wave)
-constexpr int SC_CAP = SC_ROWS - 1 + SC_NT * SC_S0R;   // queue capacity
+constexpr int SC_NT = 256;      // threads (4 waves)
+constexpr int SC_NW = SC_NT / 64;
+constexpr int SC_WQ = 16 + 64;  // per-wave queue capacity
 constexpr int SC_PW1 = 40;      // row pitch (bf16) of X and W1^T
 constexpr int SC_PW2 = 136;     // row pitch (bf16) of H1 and W2^T
 
@@ -287,21 +287,25 @@ __device__ __forceinline__ int fold_coord(int k, int n) { return k > n / 2 ? k -
 
 template <int ND>
 __global__ __launch_bounds__(SC_NT) void k_mlp_score(ScoreArgs A) {
-    constexpr int ND2 = ND * ND, ND3 = ND2 * ND, NW = SC_NT / 64;
+    constexpr int ND2 = ND * ND, ND3 = ND2 * ND;
+    static_assert(ND <= 32, "s0 mask is 32 bits");
     extern __shared__ __align__(16) unsigned char smsc[];
     __bf16 *sW1T = reinterpret_cast<__bf16 *>(smsc);         // [MH][SC_PW1]  (j, f)
     __bf16 *sW2T = sW1T + MH * SC_PW1;                        // [MH][SC_PW2]  (j2, j1)
-    __bf16 *sX = sW2T + MH * SC_PW2;                          // [SC_ROWS][SC_PW1]
-    __bf16 *sH1 = sX + SC_ROWS * SC_PW1;                      // [SC_ROWS][SC_PW2]
-    float *sB1 = reinterpret_cast<float *>(sH1 + SC_ROWS * SC_PW2);
-    float *sB2 = sB1 + MH, *sW3 = sB2 + MH, *sOut = sW3 + MH;   // sOut[SC_ROWS]
-    float *sQG = sOut + SC_ROWS;                              // queue |G|
-    int *sQK = reinterpret_cast<int *>(sQG + SC_CAP);         // queue key: triplet index in the item
-    double *sMd = reinterpret_cast<double *>(sQK + SC_CAP + (SC_CAP & 1));   // w0 | (w1 n1 w2 n2)[QPB]
-    double *sS = sMd + (1 + 4 * SC_QPB) * 32;                 // [SC_QPB]
-    double *sPq = sS + SC_QPB;                                // [SC_ROWS / 64][SC_QPB]
-    int *sInt = reinterpret_cast<int *>(sPq + (SC_ROWS / 64) * SC_QPB);   // wave totals [NW], work item
+    float *sB1 = reinterpret_cast<float *>(sW2T + MH * SC_PW2);
+    float *sB2 = sB1 + MH, *sW3 = sB2 + MH;
+    double *sMd = reinterpret_cast<double *>(sW3 + MH);      // w0[32] | (w1 n1 w2 n2)[QPB][32]
+    double *sWS = sMd + (1 + 4 * SC_QPB) * 32;                // [SC_NW][SC_QPB]
+    int *sItem = reinterpret_cast<int *>(sWS + SC_NW * SC_QPB);
+    unsigned char *wbase = reinterpret_cast<unsigned char *>(sItem + 4);
     const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, g = l >> 4;
+    // per-wave region: X[16][SC_PW1] bf16 | H1[16][SC_PW2] bf16 | key[SC_WQ] | G[SC_WQ] | out[16]
+    constexpr size_t WREG = (16 * SC_PW1 + 16 * SC_PW2) * 2 + SC_WQ * 8 + 16 * 4;
+    __bf16 *wX = reinterpret_cast<__bf16 *>(wbase + w * WREG);
+    __bf16 *wH = wX + 16 * SC_PW1;
+    int *wK = reinterpret_cast<int *>(wH + 16 * SC_PW2);
+    float *wG = reinterpret_cast<float *>(wK + SC_WQ);
+    float *wO = wG + SC_WQ;
     const bool net = A.param != nullptr;
     if (net) {
         const MlpParamsView P = mlp_view(A.param, A.F);
@@ -323,9 +327,9 @@ __global__ __launch_bounds__(SC_NT) void k_mlp_score(ScoreArgs A) {
 
     for (;;) {
         __syncthreads();
-        if (tid == 0) sInt[NW] = atomicAdd(A.counter, 1);
+        if (tid == 0) sItem[0] = atomicAdd(A.counter, 1);
         __syncthreads();
-        const int wi = sInt[NW];
+        const int wi = sItem[0];
         if (wi >= A.nwork) break;
         const int qi = wi / A.nchunk, q1lo = (wi % A.nchunk) * SC_QPB;
         const int nql = min(SC_QPB, A.nq - q1lo);
@@ -343,202 +347,211 @@ __global__ __launch_bounds__(SC_NT) void k_mlp_score(ScoreArgs A) {
             sMd[32 + (4 * ql + 2) * 32 + s] = A.omega[(size_t)q2 * ND + s];
             sMd[32 + (4 * ql + 3) * 32 + s] = A.nbe[(size_t)q2 * ND + s];
         }
-        if (tid < SC_QPB) sS[tid] = 0.0;
         __syncthreads();
         const float k0x = (float)fold_coord(k00, A.m0) / A.m0, k0y = (float)fold_coord(k01, A.m1) / A.m1,
                     k0z = (float)fold_coord(k02, A.m2) / A.m2;
+        // w0 of every s0 in registers (uniform across lanes)
+        double w0r[ND];
+#pragma unroll
+        for (int s = 0; s < ND; ++s) w0r[s] = sMd[s];
 
-        // evaluate queue rows [0, nv) (nv <= SC_ROWS) and add their contributions to sS
+        double accq[SC_QPB] = {0.0, 0.0, 0.0, 0.0};   // lane 0: this wave's per-q1 sums
+        unsigned long long nres = 0;
+        int qn = 0;                                    // wave queue length (uniform)
+
+        // the wave's network tile over queue rows [0, nv), nv <= 16; adds to accq
         auto tile = [&](int nv) {
-            if (net) {
-                if (tid < SC_ROWS) {
-                    float xs[32];
+            if (l < 16) {
+                float xs[32];
 #pragma unroll
-                    for (int f = 0; f < 32; ++f) xs[f] = 0.f;
-                    if (tid < nv) {
-                        const int key = sQK[tid];
-                        const int ql = key / ND3, r = key % ND3;
-                        const int s0 = r / ND2, s1 = (r / ND) % ND, s2 = r % ND;
-                        const int q1 = q1lo + ql;
-                        const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
-                        xs[0] = k0x;
-                        xs[1] = k0y;
-                        xs[2] = k0z;
-                        xs[3] = (float)fold_coord(k10, A.m0) / A.m0;
-                        xs[4] = (float)fold_coord(k11, A.m1) / A.m1;
-                        xs[5] = (float)fold_coord(k12, A.m2) / A.m2;
-                        xs[6] = (float)sMd[s0] * A.inv_wmax;
-                        xs[7] = (float)sMd[32 + (4 * ql + 0) * 32 + s1] * A.inv_wmax;
-                        xs[8] = (float)sMd[32 + (4 * ql + 2) * 32 + s2] * A.inv_wmax;
-                        xs[9] = s0 * fnd;
-                        xs[10] = s1 * fnd;
-                        xs[11] = s2 * fnd;
-                    }
-                    __bf16 *x = sX + tid * SC_PW1;
-#pragma unroll
-                    for (int f = 0; f < 32; ++f) x[f] = (__bf16)xs[f];
+                for (int f = 0; f < 32; ++f) xs[f] = 0.f;
+                if (l < nv) {
+                    const int key = wK[l];
+                    const int ql = key / ND3, r = key % ND3;
+                    const int s0 = r / ND2, s1 = (r / ND) % ND, s2 = r % ND;
+                    const int q1 = q1lo + ql;
+                    const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
+                    xs[0] = k0x;
+                    xs[1] = k0y;
+                    xs[2] = k0z;
+                    xs[3] = (float)fold_coord(k10, A.m0) / A.m0;
+                    xs[4] = (float)fold_coord(k11, A.m1) / A.m1;
+                    xs[5] = (float)fold_coord(k12, A.m2) / A.m2;
+                    xs[6] = (float)sMd[s0] * A.inv_wmax;
+                    xs[7] = (float)sMd[32 + (4 * ql + 0) * 32 + s1] * A.inv_wmax;
+                    xs[8] = (float)sMd[32 + (4 * ql + 2) * 32 + s2] * A.inv_wmax;
+                    xs[9] = s0 * fnd;
+                    xs[10] = s1 * fnd;
+                    xs[11] = s2 * fnd;
                 }
-                __syncthreads();
-                // layer 1 (wave w: rows 16 w .. 16 w + 15), K = 32 (features, zero padded)
-                {
-                    bf16x8_m a;
 #pragma unroll
-                    for (int t = 0; t < 8; ++t) a[t] = sX[(16 * w + (l & 15)) * SC_PW1 + 8 * g + t];
+                for (int f = 0; f < 32; ++f) wX[l * SC_PW1 + f] = (__bf16)xs[f];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            {
+                bf16x8_m a;
 #pragma unroll
-                    for (int nt = 0; nt < MH / 16; ++nt) {
-                        bf16x8_m b;
+                for (int t = 0; t < 8; ++t) a[t] = wX[(l & 15) * SC_PW1 + 8 * g + t];
 #pragma unroll
-                        for (int t = 0; t < 8; ++t) b[t] = sW1T[(nt * 16 + (l & 15)) * SC_PW1 + 8 * g + t];
-                        v4f_m z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, v4f_m{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-                        const int j = nt * 16 + (l & 15);
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            sH1[(16 * w + 4 * g + r) * SC_PW2 + j] = (__bf16)fast_tanh(z[r] + sB1[j]);
-                    }
-                }
-                __syncthreads();
-                // layer 2 and the output layer
-                float yacc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
                 for (int nt = 0; nt < MH / 16; ++nt) {
-                    v4f_m z = {0.f, 0.f, 0.f, 0.f};
+                    bf16x8_m b;
 #pragma unroll
-                    for (int kk = 0; kk < MH; kk += 32) {
-                        bf16x8_m a, b;
-#pragma unroll
-                        for (int t = 0; t < 8; ++t) {
-                            a[t] = sH1[(16 * w + (l & 15)) * SC_PW2 + kk + 8 * g + t];
-                            b[t] = sW2T[(nt * 16 + (l & 15)) * SC_PW2 + kk + 8 * g + t];
-                        }
-                        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, z, 0, 0, 0);
-                    }
+                    for (int t = 0; t < 8; ++t) b[t] = sW1T[(nt * 16 + (l & 15)) * SC_PW1 + 8 * g + t];
+                    v4f_m z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, v4f_m{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
                     const int j = nt * 16 + (l & 15);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) yacc[r] = fmaf(fast_tanh(z[r] + sB2[j]), sW3[j], yacc[r]);
+                    for (int r = 0; r < 4; ++r) wH[(4 * g + r) * SC_PW2 + j] = (__bf16)fast_tanh(z[r] + sB1[j]);
                 }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            float yacc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+            for (int nt = 0; nt < MH / 16; ++nt) {
+                v4f_m z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kk = 0; kk < MH; kk += 32) {
+                    bf16x8_m a, b;
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        a[t] = wH[(l & 15) * SC_PW2 + kk + 8 * g + t];
+                        b[t] = sW2T[(nt * 16 + (l & 15)) * SC_PW2 + kk + 8 * g + t];
+                    }
+                    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, z, 0, 0, 0);
+                }
+                const int j = nt * 16 + (l & 15);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) yacc[r] = fmaf(fast_tanh(z[r] + sB2[j]), sW3[j], yacc[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float v = yacc[r];
+                v += __shfl_xor(v, 1, 64);
+                v += __shfl_xor(v, 2, 64);
+                v += __shfl_xor(v, 4, 64);
+                v += __shfl_xor(v, 8, 64);
+                yacc[r] = v;
+            }
+            if ((l & 15) == 0)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    float v = yacc[r];
-                    v += __shfl_xor(v, 1, 64);
-                    v += __shfl_xor(v, 2, 64);
-                    v += __shfl_xor(v, 4, 64);
-                    v += __shfl_xor(v, 8, 64);
-                    yacc[r] = v;
+                    const int row = 4 * g + r;
+                    wO[row] = row < nv ? __expf(A.sd * (yacc[r] + b3)) * wG[row] : 0.f;
                 }
-                if ((l & 15) == 0)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // per-q1 sums of the tile (lanes 0..15 hold the rows; fixed reduction order)
+            const int ql = l < nv ? wK[l] / ND3 : -1;
+            const double o = l < 16 ? (double)wO[l] : 0.0;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = 16 * w + 4 * g + r;
-                        sOut[row] = row < nv ? __expf(A.sd * (yacc[r] + b3)) * sQG[row] : 0.f;
-                    }
-            } else if (tid < SC_ROWS) {
-                sOut[tid] = tid < nv ? sQG[tid] : 0.f;
-            }
-            __syncthreads();
-            // per-q1 sums of the tile: waves 0 and 1 (64 rows each), fixed reduction order
-            if (w < SC_ROWS / 64) {
-                const int row = 64 * w + l;
-                const int ql = row < nv ? sQK[row] / ND3 : -1;
-                const double o = (double)sOut[row];
-                for (int q = 0; q < SC_QPB; ++q) {
-                    double v = ql == q ? o : 0.0;
+            for (int q = 0; q < SC_QPB; ++q) {
+                double v = ql == q ? o : 0.0;
 #pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-                    if (l == 0) sPq[SC_QPB * w + q] = v;
-                }
+                for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                accq[q] += v;   // every lane holds the same tile sum (lane 0 is the one used)
             }
-            __syncthreads();
-            if (tid < SC_QPB) {
-                double add = 0.0;
-                for (int ww = 0; ww < SC_ROWS / 64; ++ww) add += sPq[SC_QPB * ww + tid];
-                sS[tid] += add;
-            }
-            __syncthreads();
         };
 
-        int qn = 0;
-        unsigned long long nres = 0;
-        const int npair = nql * ND2;      // (q1, s1, s2) of the item
-        for (int cb = 0; cb < npair; cb += SC_NT) {
-            const int pidx = cb + tid;
-            const bool pv = pidx < npair;
+        for (int pb = w * 64; pb < nql * ND2; pb += SC_NT) {
+            const int pidx = pb + l;
+            const bool pv = pidx < nql * ND2;
             const int ql = pv ? pidx / ND2 : 0, s1 = pv ? (pidx / ND) % ND : 0, s2 = pv ? pidx % ND : 0;
             const double w1 = sMd[32 + (4 * ql + 0) * 32 + s1], n1 = sMd[32 + (4 * ql + 1) * 32 + s1];
             const double w2 = sMd[32 + (4 * ql + 2) * 32 + s2], n2 = sMd[32 + (4 * ql + 3) * 32 + s2];
             const double ap = w1 + w2, am = w1 - w2;   // d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am
-            for (int s0b = 0; s0b < ND; s0b += SC_S0R) {
-                unsigned bits = 0;
+            unsigned mask = 0;
+            if (pv) {
 #pragma unroll
-                for (int k = 0; k < SC_S0R; ++k) {
-                    const int s0 = s0b + k;
-                    if (pv && s0 < ND) {
-                        const double w0 = sMd[s0];
-                        const double dm = fmin(fabs(w0 - ap), fmin(fabs(w0 + am), fabs(w0 - am)));
-                        if (dm < win) bits |= 1u << k;
-                    }
+                for (int s0 = 0; s0 < ND; ++s0) {
+                    const double w0 = w0r[s0];
+                    const double dm = fmin(fabs(w0 - ap), fmin(fabs(w0 + am), fabs(w0 - am)));
+                    mask |= (dm < win ? 1u : 0u) << s0;
                 }
-                const int cntk = __popc(bits);
-                // exclusive prefix sum of cntk over the workgroup, in thread order
-                int incl = cntk;
+            }
+            // emit the resonant triplets in (lane, s0) order
+            while (__ballot(mask != 0)) {
+                const bool has = mask != 0;
+                const int s0 = has ? __ffs(mask) - 1 : 0;
+                mask &= has ? mask - 1 : mask;
+                float G = 0.f;
+                if (has) {
+                    const double w0 = sMd[s0];
+                    const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
+                    G = (float)fabs((n1 + n2 + 1.0) * c0 * exp(-hinv * d1 * d1) +
+                                    (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3)));
+                }
+                const unsigned long long bal = __ballot(has);
+                const int cnt = __popcll(bal);
+                nres += (unsigned long long)cnt;
+                if (!net) {
+                    // no network: the weight is |G|, summed per lane in emission order
 #pragma unroll
-                for (int off = 1; off < 64; off <<= 1) {
-                    const int v = __shfl_up(incl, off, 64);
-                    if (l >= off) incl += v;
+                    for (int q = 0; q < SC_QPB; ++q)
+                        if (has && ql == q) accq[q] += (double)G;
+                    continue;
                 }
-                if (l == 63) sInt[w] = incl;
-                __syncthreads();
-                int off = qn, total = 0;
-                for (int ww = 0; ww < NW; ++ww) {
-                    const int c = sInt[ww];
-                    if (ww < w) off += c;
-                    total += c;
+                const int rank = __popcll(bal & ((1ull << l) - 1ull));
+                if (has) {
+                    wK[qn + rank] = ql * ND3 + s0 * ND2 + s1 * ND + s2;
+                    wG[qn + rank] = G;
                 }
-                off += incl - cntk;
-                for (int k = 0; k < SC_S0R; ++k)
-                    if (bits & (1u << k)) {
-                        const int s0 = s0b + k;
-                        const double w0 = sMd[s0];
-                        const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
-                        sQK[off] = ql * ND3 + s0 * ND2 + s1 * ND + s2;
-                        sQG[off] = (float)fabs((n1 + n2 + 1.0) * c0 * exp(-hinv * d1 * d1) +
-                                               (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3)));
-                        ++off;
+                qn += cnt;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                while (qn >= 16) {
+                    tile(16);
+                    // shift the remaining entries (< 64) to the front, in order
+                    int kq = 0;
+                    float gq = 0.f;
+                    const bool mv = l + 16 < qn;
+                    if (mv) {
+                        kq = wK[l + 16];
+                        gq = wG[l + 16];
                     }
-                qn += total;
-                nres += (unsigned long long)total;
-                __syncthreads();
-                while (qn >= SC_ROWS) {
-                    tile(SC_ROWS);
-                    // move the remaining entries to the front (in order)
-                    for (int base = SC_ROWS; base < qn; base += SC_NT) {
-                        const int i = base + tid;
-                        int kq = 0;
-                        float gq = 0.f;
-                        if (i < qn) {
-                            kq = sQK[i];
-                            gq = sQG[i];
-                        }
-                        __syncthreads();
-                        if (i < qn) {
-                            sQK[i - SC_ROWS] = kq;
-                            sQG[i - SC_ROWS] = gq;
-                        }
-                        __syncthreads();
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (mv) {
+                        wK[l] = kq;
+                        wG[l] = gq;
                     }
-                    qn -= SC_ROWS;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    qn -= 16;
                 }
             }
         }
-        if (qn > 0) tile(qn);
+        if (net && qn > 0) tile(qn);
+        // this wave's per-q1 totals: with a network every lane holds them; without, reduce lanes
+#pragma unroll
+        for (int q = 0; q < SC_QPB; ++q) {
+            double v = accq[q];
+            if (!net) {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            }
+            if (l == 0) sWS[w * SC_QPB + q] = v;
+        }
+        if (l == 0) atomicAdd(A.n_eval, net ? nres : 0ull);
         __syncthreads();
-        if (tid < nql) A.S[(size_t)qi * A.nq + q1lo + tid] = sS[tid];
-        if (tid == 0 && net) atomicAdd(A.n_eval, nres);
+        if (tid < nql) {
+            double t = 0.0;
+            for (int ww = 0; ww < SC_NW; ++ww) t += sWS[ww * SC_QPB + tid];
+            A.S[(size_t)qi * A.nq + q1lo + tid] = t;
+        }
     }
 }
 
 size_t score_lds() {
-    return (size_t)(MH * SC_PW1 + MH * SC_PW2 + SC_ROWS * SC_PW1 + SC_ROWS * SC_PW2) * 2 + (3 * MH + SC_ROWS) * 4 +
-           (SC_CAP + 1) * 8 + ((1 + 4 * SC_QPB) * 32 + SC_QPB + (SC_ROWS / 64) * SC_QPB) * 8 + 16 * 4;
+    const size_t wreg = (16 * SC_PW1 + 16 * SC_PW2) * 2 + SC_WQ * 8 + 16 * 4;
+    return (size_t)(MH * SC_PW1 + MH * SC_PW2) * 2 + 3 * MH * 4 + ((1 + 4 * SC_QPB) * 32 + SC_NW * SC_QPB) * 8 +
+           4 * 4 + SC_NW * wreg + 16;
 }
 
 }  // namespace spr
@@ -824,7 +837,7 @@ extern "C" int spr_mlp_score(spr_ctx *ctx, spr_mlp *m, int n, const int *q0_inde
             rc = SPR_E_HIP;
             break;
         }
-        const int grid_n = std::min(A.nwork, c.ncu);
+        const int grid_n = std::min(A.nwork, 2 * c.ncu);
         (void)hipEventRecord(c.ev0, c.stream);
         {
             void *args[] = {&A};

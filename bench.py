@@ -39,6 +39,8 @@ def parse():
                         "learned importance-sampling proposal on the cell10 workload")
     p.add_argument("--samples", type=int, default=2_000_000, help="learned10: training samples")
     p.add_argument("--budget", type=float, default=0.05, help="learned10: sampled fraction of q1")
+    p.add_argument("--pair-prec", default="fp32", choices=["fp64", "fp32", "bf16"],
+                   help="learned10: precision of the sampled pair terms")
     p.add_argument("--mesh10", type=int, default=20)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
@@ -267,8 +269,8 @@ def cell10(args):
 def learned10(args):
     """Learned importance-sampling proposal on the 10-atom cell: the network is trained on
     args.samples |V3|^2 values produced by the FP64 W path, scores every (q0, q1) on the GPU,
-    and K = budget * Nq sampled q1 per q0 (all branches of each sampled pair, FP64 pair terms)
-    estimate every linewidth; kappa and the per-mode linewidths are judged against the FP64
+    and K = budget * Nq sampled q1 per q0 (all branches of each sampled pair, pair terms in
+    args.pair_prec) estimate every linewidth; kappa and the per-mode linewidths are judged against the FP64
     full sum.  The same sampling with the network-free (G-only) proposal and with uniform
     sampling is reported beside it."""
     from spring_amd import model, mesh, phonons, lib
@@ -323,7 +325,7 @@ def learned10(args):
         est = np.zeros((qm.nq0, nd))
         chunk = 200_000
         for lo in range(0, q0s.size, chunk):
-            tt = ctx.lw3_pairs(q0s[lo:lo + chunk], q1s[lo:lo + chunk], args.sigma, "fp64")
+            tt = ctx.lw3_pairs(q0s[lo:lo + chunk], q1s[lo:lo + chunk], args.sigma, args.pair_prec)
             np.add.at(est, q0s[lo:lo + chunk], tt * ws[lo:lo + chunk, None])
         k_est, _ = ctx.kappa(est, args.T, symm=True)
         t_pairs = time.perf_counter() - t
@@ -340,7 +342,8 @@ def learned10(args):
         "metric": "learned importance sampling of the three-phonon linewidth sum: kappa and linewidth "
                   "error vs the FP64 full sum, time vs the FP64 GPU path",
         "config": {"workload": f"10-atom cell, {n}^3 mesh with time reversal, {qm.nq0} irreducible q0, "
-                               f"{K} sampled q1 per q0 ({args.budget:.0%} of Nq), all branches per pair",
+                               f"{K} sampled q1 per q0 ({args.budget:.0%} of Nq), all branches per pair, "
+                               f"{args.pair_prec} pair terms",
                    "sigma_rad_per_ps": args.sigma, "T_K": args.T, "training_samples": int(y.size),
                    "network": "12 -> 128 -> 128 -> 1 tanh, FP32 Adam training, BF16 MFMA scoring"},
         "fp64_full_s": t_full, "phonons_s": t_ph, "training_set_s": t_set, "training_s": t_train,

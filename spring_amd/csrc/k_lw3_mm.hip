@@ -71,9 +71,11 @@ constexpr int pow2ceil(int x) { return x <= 1 ? 1 : x <= 2 ? 2 : x <= 4 ? 4 : 8;
 
 // V tiles per wave when each unit covers s2 blocks of 8 nts values
 constexpr int v_tiles(int nd, int nt, int nts) { return ((nd * 8 * nts + 15) / 16 * pow2ceil(nt) + 7) / 8; }
-// largest s2 block (in N tiles of 8 s2) whose V accumulators fit 64 registers per lane
+// largest s2 block (in N tiles of 8 s2) whose V accumulators fit the register budget per lane
+// (64; 48 when three N tiles leave a quarter of the waves idle in GEMMs C and D)
 constexpr int pick_nts(int nd, int nt, int words) {
-    return v_tiles(nd, nt, nt) * words <= 64 ? nt : v_tiles(nd, nt, 2) * words <= 64 && nt >= 2 ? 2 : 1;
+    const int budget = nt == 3 ? 48 : 64;
+    return v_tiles(nd, nt, nt) * words <= budget ? nt : v_tiles(nd, nt, 2) * words <= budget && nt >= 2 ? 2 : 1;
 }
 
 template <int MODE, int NB> struct MmShape {
@@ -122,7 +124,8 @@ template <int MODE, int NB> struct MmLds {
     __host__ __device__ static constexpr size_t e() { return E; }
     static size_t total(int npmax, int nterm, int nsite, int L) {
         size_t b = tu() + u2() + ac() + 2 * e();
-        b += (size_t)(Sh::PALL ? nterm : npmax) * sizeof(Z) + (size_t)L * sizeof(Z) + 8 * Sh::ND * sizeof(double);
+        b += (size_t)(Sh::PALL ? nterm : npmax) * sizeof(Z) + (size_t)L * sizeof(Z) + 8 * Sh::ND * sizeof(double) +
+             4 * Sh::ND * sizeof(float);
         if (Sh::PALL) b += (size_t)npmax * 27 * sizeof(R);      // FC of one atom (staged)
         b += (size_t)Sh::ND * Sh::ND * sizeof(Z) + 16;           // e0 of the unit's q0
         b += 2 * (size_t)nsite * sizeof(int) + (size_t)(NB * NB * NB + 1) * sizeof(int);
@@ -298,6 +301,8 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     p += (size_t)A.L * sizeof(Z);
     double *sMode = reinterpret_cast<double *>(p);    // w0 iw0f | w1 n1 iw1 | w2 n2 iw2 | -
     p += 8 * ND * sizeof(double);
+    float *sModeF = reinterpret_cast<float *>(p);     // n1 iw1 n2 iw2 in FP32
+    p += 4 * ND * sizeof(float);
     int *sM1 = reinterpret_cast<int *>(p);
     int *sM2 = sM1 + A.nsite;
     int *sBlk = sM2 + A.nsite;                        // blk_start of every (mu, nu, eta) block
@@ -335,6 +340,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     }
     const double c0 = 1.0 / (A.sigma * 2.5066282746310002);   // 1/(sigma sqrt(2 pi))
     const double hinv = 0.5 / (A.sigma * A.sigma);
+    const float hneg = (float)(-hinv);
 
     // GEMM B: N tiles h NTS + (w % NTSB), M tiles w / NTSB + (8 / NTSB) j
     const int ntB = h * Sh::NTS + (w % Sh::NTSB);
@@ -370,6 +376,10 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             sMode[5 * ND + tid] = A.omega[(size_t)q2 * ND + tid];
             sMode[6 * ND + tid] = A.nbe[(size_t)q2 * ND + tid];
             sMode[7 * ND + tid] = A.inv_omega[(size_t)q2 * ND + tid];
+            sModeF[tid] = (float)A.nbe[(size_t)q1 * ND + tid];
+            sModeF[ND + tid] = (float)A.inv_omega[(size_t)q1 * ND + tid];
+            sModeF[2 * ND + tid] = (float)A.nbe[(size_t)q2 * ND + tid];
+            sModeF[3 * ND + tid] = (float)A.inv_omega[(size_t)q2 * ND + tid];
         }
         {
             const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
@@ -538,9 +548,12 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             __syncthreads();
         }
 
-        // ---- W and the Gaussian-delta weights; lanes with q = 0 take rows 0,1, q = 1 rows 2,3 ----
+        // ---- W and the Gaussian-delta weights; lanes with q = 0 take rows 0,1, q = 1 rows 2,3.
+        //      The delta arguments are formed in FP64 (they cancel near resonance); in the FP32 and
+        //      BF16 modes the rest is FP32, summed per pair and then into a compensated sum. ----
         if (s0ok) {
             const double w0 = sMode[s0], iw0 = sMode[ND + s0];
+            float pairf = 0.f;
 #pragma unroll
             for (int j = 0; j < Sh::VT; ++j) {
                 const int mt = mtC0 + MSTEPC * j;
@@ -553,23 +566,27 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                     const int m = mt * 16 + tile_row<MODE == MM_F64>(g, r);
                     const int s1 = m / SB, s2 = h * SB + m % SB;
                     if (s1 >= ND || s2 >= ND) continue;
-                    const double w1 = sMode[2 * ND + s1], n1 = sMode[3 * ND + s1], iw1 = sMode[4 * ND + s1];
-                    const double w2 = sMode[5 * ND + s2], n2 = sMode[6 * ND + s2], iw2 = sMode[7 * ND + s2];
-                    const double d1 = w0 - w1 - w2, d2 = w0 + w1 - w2, d3 = w0 - w1 + w2;
+                    const double w1 = sMode[2 * ND + s1], w2 = sMode[5 * ND + s2];
+                    const double ap = w1 + w2, am = w1 - w2;
                     if constexpr (MODE == MM_F64) {
+                        const double n1 = sMode[3 * ND + s1], iw1 = sMode[4 * ND + s1];
+                        const double n2 = sMode[6 * ND + s2], iw2 = sMode[7 * ND + s2];
+                        const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
                         const double W = (x * x + y * y) * iw0 * iw1 * iw2;
                         const double G = (n1 + n2 + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
                                          (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
                         accd += wpair * W * G;
                     } else {
-                        const float W = (x * x + y * y) * (float)(iw0 * iw1 * iw2);
-                        const float g1 = (float)c0 * __expf((float)(-hinv * d1 * d1));
-                        const float g2 = (float)c0 * __expf((float)(-hinv * d2 * d2));
-                        const float g3 = (float)c0 * __expf((float)(-hinv * d3 * d3));
-                        accf.add((float)wpair * W * ((float)(n1 + n2 + 1.0) * g1 + (float)(n1 - n2) * (g2 - g3)));
+                        const float n1 = sModeF[s1], iw1 = sModeF[ND + s1];
+                        const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
+                        const float d1 = (float)(w0 - ap), d2 = (float)(w0 + am), d3 = (float)(w0 - am);
+                        const float W = (x * x + y * y) * (iw1 * iw2);
+                        const float e1 = __expf(hneg * d1 * d1), e2 = __expf(hneg * d2 * d2), e3 = __expf(hneg * d3 * d3);
+                        pairf = fmaf(W, (n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3), pairf);
                     }
                 }
             }
+            if constexpr (MODE != MM_F64) accf.add(pairf * (float)(wpair * c0 * iw0));
         }
     }
     // ---- reduce per s0 in a fixed order: wave w holds the columns of N tile w % NTC ----

@@ -33,7 +33,7 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and matrix are the same rate)
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", default="rocksalt16", choices=["rocksalt16", "cell10", "learned10"],
+    p.add_argument("--workload", default="rocksalt16", choices=["rocksalt16", "cell10", "learned10", "fourph10"],
                    help="rocksalt16: 2-atom cell, 16^3 mesh (the headline line); cell10: 10-atom "
                         "cell, 20^3 mesh with time reversal, FP64 vs FP32 vs BF16; learned10: the "
                         "learned importance-sampling proposal on the cell10 workload")
@@ -64,6 +64,8 @@ def main():
         return cell10(args)
     if args.workload == "learned10":
         return learned10(args)
+    if args.workload == "fourph10":
+        return fourph10(args)
     from spring_amd import model, mesh, phonons, lib
 
     t_in0 = time.perf_counter()
@@ -349,6 +351,65 @@ def learned10(args):
         "fp64_full_s": t_full, "phonons_s": t_ph, "training_set_s": t_set, "training_s": t_train,
         "training_loss": [float(prop.losses[0]), float(prop.losses[-1])],
         "proposals": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def fourph10(args):
+    """Four-phonon Gaussian linewidths of the 2-atom model (FC4 within the 1st shell) on a
+    10^3 mesh: FP64 against FP32, and the FP64 CPU oracle on a sample of q0."""
+    from spring_amd import model, mesh, phonons, lib
+    m = model.rocksalt2(with_fc4=True)
+    n = 10
+    qm = mesh.make_mesh(m.crystal, (n, n, n))
+    ph = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
+    ctx = lib.GpuContext(m, qm, ph)
+    nd = m.crystal.ndof
+    quartets = qm.nq0 * qm.nq * qm.nq * nd ** 4
+    res = {}
+    for prec in ("fp64", "fp32"):
+        ctx.lw4_all(args.sigma, prec)
+        ctx.kernel_time_reset()
+        t = time.perf_counter()
+        lw = ctx.lw4_all(args.sigma, prec)
+        dt = time.perf_counter() - t
+        nl, kms = ctx.kernel_time()
+        res[prec] = dict(step=dt, kern=kms / max(nl, 1), lw=lw)
+    l64, l32 = res["fp64"]["lw"], res["fp32"]["lw"]
+    sig = np.abs(l64) > 1e-6 * np.abs(l64).max()
+    # algorithmic FLOP per (q0, q1, q2) triple: phase sums 4*81*sum Natm4^3 over the nonzero
+    # clusters, four complex mode products 32*Ndof^5, ~30 per branch quartet
+    nclus = int(np.count_nonzero(np.abs(m.fc4.fc).sum(axis=(4, 5, 6, 7))))
+    flop = 4 * 81 * nclus + 32 * nd ** 5 + 30 * nd ** 4
+    triples = qm.nq0 * qm.nq * qm.nq
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        inp = O.Inputs(m, qm, ph)
+        threads = max(1, min(16, os.cpu_count() or 1))
+        from concurrent.futures import ThreadPoolExecutor
+        i0 = int(qm.irr_idx[min(1, qm.nq0 - 1)])
+        per = 8                                        # q1 per thread, every q2
+        t = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda k: O.lw4_gauss(inp, i0, args.sigma, k * per, (k + 1) * per), range(threads)))
+        dt = time.perf_counter() - t
+        cpu = {"value": threads * per * qm.nq * nd ** 4 / dt, "unit": "quartets/s", "cores": threads,
+               "kind": "port", "sample": f"one irreducible q0, {threads * per} q1 x all {qm.nq} q2, FP64 C "
+                                         f"restatement, {threads} threads, {dt:.1f} s"}
+    out = {
+        "metric": "four-phonon |V4|^2 quartets/s (Gaussian delta)", "unit": "quartets/s",
+        "value": quartets / res["fp64"]["step"], "dtype": "f64",
+        "config": {"workload": f"4-phonon Gaussian linewidths, 2-atom rocksalt with FC4 (1st shell), {n}^3 mesh, "
+                               f"{qm.nq0} irreducible q0 x {qm.nq}^2 (q1, q2) x {nd}^4 branches",
+                   "sigma_rad_per_ps": args.sigma, "T_K": args.T, "flop_per_triple": flop,
+                   "fc4_clusters": nclus},
+        "fp64": {"s": res["fp64"]["step"], "kernel_ms": res["fp64"]["kern"],
+                 "achieved_TFLOPs": flop * triples / (res["fp64"]["kern"] * 1e-3) / 1e12},
+        "fp32": {"s": res["fp32"]["step"], "kernel_ms": res["fp32"]["kern"],
+                 "speedup_vs_fp64": res["fp64"]["step"] / res["fp32"]["step"],
+                 "linewidth_max_rel_err_vs_fp64": float(np.max(np.abs(l32 - l64)[sig] / np.abs(l64)[sig]))},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -36,7 +36,8 @@ def parse():
     p.add_argument("--workload", default="rocksalt16", choices=["rocksalt16", "cell10", "learned10", "fourph10"],
                    help="rocksalt16: 2-atom cell, 16^3 mesh (the headline line); cell10: 10-atom "
                         "cell, 20^3 mesh with time reversal, FP64 vs FP32 vs BF16; learned10: the "
-                        "learned importance-sampling proposal on the cell10 workload")
+                        "learned importance-sampling proposal on the cell10 workload; fourph10: "
+                        "four-phonon linewidths, 2-atom cell with FC4, 10^3 mesh")
     p.add_argument("--samples", type=int, default=2_000_000, help="learned10: training samples")
     p.add_argument("--budget", type=float, default=0.05, help="learned10: sampled fraction of q1")
     p.add_argument("--pair-prec", default="fp32", choices=["fp64", "fp32", "bf16"],

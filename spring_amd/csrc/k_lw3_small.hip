@@ -106,21 +106,36 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
     for (int be = 0; be < 3; ++be)
 #pragma unroll
         for (int c = 0; c < ND; ++c) T[be][c] = cmk<R>(0, 0);
+    // tuples in batches of four (every block is padded to a multiple of four with zero FC
+    // values): the site pairs and FC values of a batch arrive in one scalar-load round, its
+    // four phases in one LDS round, so the load latency is paid once per 72 FMAs
 #pragma unroll
     for (int eta = 0; eta < NB; ++eta) {
         const int blk = (mu * NB + nu) * NB + eta;
         const int u0 = blk_start[blk], u1 = blk_start[blk + 1];
-        for (int u = u0; u < u1; ++u) {
-            const int sa = tup_site[2 * u], sb = tup_site[2 * u + 1];
-            int m = s_m1[sa * 64 + lane] + s_m2[sb * 64 + lane];
-            m -= m >= A.L ? A.L : 0;
-            const Z ph = s_ph[m];
-            const R *f = fc + (size_t)u * 27 + al * 9;
+        for (int u = u0; u < u1; u += kClusterBatch) {
+            int sa[kClusterBatch], sb[kClusterBatch];
 #pragma unroll
-            for (int be = 0; be < 3; ++be)
+            for (int k = 0; k < kClusterBatch; ++k) {
+                sa[k] = tup_site[2 * (u + k)];
+                sb[k] = tup_site[2 * (u + k) + 1];
+            }
+            Z ph[kClusterBatch];
 #pragma unroll
-                for (int ga = 0; ga < 3; ++ga)
-                    T[be][3 * eta + ga] = crfma(f[be * 3 + ga], ph, T[be][3 * eta + ga]);
+            for (int k = 0; k < kClusterBatch; ++k) {
+                int m = s_m1[sa[k] * 64 + lane] + s_m2[sb[k] * 64 + lane];
+                m -= m >= A.L ? A.L : 0;
+                ph[k] = s_ph[m];
+            }
+#pragma unroll
+            for (int k = 0; k < kClusterBatch; ++k) {
+                const R *f = fc + (size_t)(u + k) * 27 + al * 9;
+#pragma unroll
+                for (int be = 0; be < 3; ++be)
+#pragma unroll
+                    for (int ga = 0; ga < 3; ++ga)
+                        T[be][3 * eta + ga] = crfma(f[be * 3 + ga], ph[k], T[be][3 * eta + ga]);
+            }
         }
     }
     // ---- 2. U1(beta, s2) = sum_c T(beta, c) e2(c, s2), in place per row ----
@@ -171,8 +186,14 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
             const double n2 = s_md[(4 * ND + s2) * 64 + lane];
             const double W = (double)cabs2(v) * iw01 * s_md[(5 * ND + s2) * 64 + lane];
             const double d1 = w0 - w1 - w2, d2 = w0 + w1 - w2, d3 = w0 - w1 + w2;
-            const double g = (n1 + n2 + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
-                             (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
+            const double t1 = hinv * d1 * d1, t2 = hinv * d2 * d2, t3 = hinv * d3 * d3;
+            double g = 0.0;
+            // exp(-t) is exactly zero in FP64 beyond t = 745.2: a wave whose lanes are all that far
+            // from resonance (consecutive q1 of one q0 have close frequencies, so most are) skips
+            // the exponentials with the same result.  The FP64 variant keeps the branch-free form:
+            // the branch costs it registers it does not have.
+            if (sizeof(R) == 8 || __ballot(fmin(t1, fmin(t2, t3)) < 746.0))
+                g = (n1 + n2 + 1.0) * (c0 * exp(-t1)) + (n1 - n2) * (c0 * exp(-t2) - c0 * exp(-t3));
             if (sizeof(R) == 8)
                 dacc += wpair * W * g;
             else
@@ -231,7 +252,7 @@ static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first,
                           double *d_lw) {
     constexpr int ND = 3 * NB;
     SmallArgs A = A0;
-    A.fc = sizeof(R) == 8 ? (const void *)c.fc3.fc64 : (const void *)c.fc3.fc32;
+    A.fc = sizeof(R) == 8 ? (const void *)c.fc3.fc64_p : (const void *)c.fc3.fc32_p;
     A.phase = sizeof(R) == 8 ? (const void *)c.phase64 : (const void *)c.phase32;
     A.evec = sizeof(R) == 8 ? (const void *)c.evec : (const void *)c.evec32;
     constexpr int NW = ND * NB;
@@ -337,8 +358,8 @@ int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec,
     A.L = c.L;
     A.m0 = c.mesh[0];
     A.m1 = c.mesh[1];
-    A.blk_start = c.fc3.blk_start;
-    A.tup_site = c.fc3.tup_site;
+    A.blk_start = c.fc3.blk_start_p;
+    A.tup_site = c.fc3.tup_site_p;
     A.site = c.fc3.site;
     A.nsite = c.fc3.nsite;
     A.omega = c.omega;

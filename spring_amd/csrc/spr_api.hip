@@ -131,6 +131,24 @@ static int build_ifc(Ctx &c, const spr_ifc *f, const double *osm, DevIFC &d) {
     if ((rc = upload(&d.fc64, all_val.data(), all_val.size()))) return rc;
     if ((rc = upload(&d.fc32, v32.data(), v32.size()))) return rc;
     if ((rc = upload(&d.site, site.data(), site.size()))) return rc;
+    // padded copy: block b of leading atom mu gets zero-valued tuples on site site_off[mu]
+    std::vector<int> pstart(nblk + 1, 0), ptup;
+    std::vector<double> pval;
+    const int per_mu = nblk / nb;
+    for (int b = 0; b < nblk; ++b) {
+        const int n = start[b + 1] - start[b];
+        const int np = (n + kClusterBatch - 1) / kClusterBatch * kClusterBatch;
+        pstart[b + 1] = pstart[b] + np;
+        ptup.insert(ptup.end(), all_tup.begin() + (size_t)start[b] * nsub, all_tup.begin() + (size_t)start[b + 1] * nsub);
+        pval.insert(pval.end(), all_val.begin() + (size_t)start[b] * ncomp, all_val.begin() + (size_t)start[b + 1] * ncomp);
+        ptup.insert(ptup.end(), (size_t)(np - n) * nsub, site_off[b / per_mu]);
+        pval.insert(pval.end(), (size_t)(np - n) * ncomp, 0.0);
+    }
+    std::vector<float> p32(pval.begin(), pval.end());
+    if ((rc = upload(&d.blk_start_p, pstart.data(), pstart.size()))) return rc;
+    if ((rc = upload(&d.tup_site_p, ptup.data(), ptup.size()))) return rc;
+    if ((rc = upload(&d.fc64_p, pval.data(), pval.size()))) return rc;
+    if ((rc = upload(&d.fc32_p, p32.data(), p32.size()))) return rc;
     return SPR_OK;
 }
 
@@ -140,6 +158,10 @@ static void free_ifc(DevIFC &d) {
     (void)hipFree(d.fc64);
     (void)hipFree(d.fc32);
     (void)hipFree(d.site);
+    (void)hipFree(d.blk_start_p);
+    (void)hipFree(d.tup_site_p);
+    (void)hipFree(d.fc64_p);
+    (void)hipFree(d.fc32_p);
     d = DevIFC();
 }
 

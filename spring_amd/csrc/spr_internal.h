@@ -68,7 +68,15 @@ struct DevIFC {
     float *fc32 = nullptr;
     int4 *site = nullptr;      // (nsite): phase coefficients (c1,c2,c3) and basis atom
     std::vector<int> h_blk_start;
+    // the same list with every block padded to a multiple of kClusterBatch tuples (zero FC
+    // values on a valid site), for kernels that load the tuples of a batch together
+    int *blk_start_p = nullptr;
+    int *tup_site_p = nullptr;
+    double *fc64_p = nullptr;
+    float *fc32_p = nullptr;
 };
+
+constexpr int kClusterBatch = 4;
 
 struct Ctx {
     int device = 0;

@@ -16,6 +16,7 @@
 // The returned sum is the raw lw of TetrahedronIntegration4th; main.f90:660 adds 0.5 lw.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <string>
 
 #include "cplx.h"
@@ -181,6 +182,20 @@ __global__ void k_reduce_chunks(const double *partial, int nq0, int nchunks, int
     lw[i] = acc;
 }
 
+// Sctr4THz2 (constants.f90:62-63) / Nq^2
+double lw4_factor(const Ctx &c) {
+    const double pi = 4.0 * std::atan(1.0);
+    return 0.125 * pi * std::pow(1.054571817 * 1.602176565, 2) * std::pow(6.022140857, 4) * 1000000.0 /
+           ((double)c.nq * (double)c.nq);
+}
+
+int launch_reduce_chunks(Ctx &c, const double *partial, int q0_count, int nchunks, double *d_lw) {
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((q0_count * c.nd + 255) / 256), dim3(256), 0, c.stream, partial,
+                       q0_count, nchunks, c.nd, d_lw);
+    SPR_HIP(hipGetLastError());
+    return SPR_OK;
+}
+
 template <typename R, int ND>
 static int launch_lw4_t(Ctx &c, int q0_first, int q0_count, double sigma, double *d_lw) {
     constexpr int BLK = 256;
@@ -206,10 +221,7 @@ static int launch_lw4_t(Ctx &c, int q0_first, int q0_count, double sigma, double
     A.inv_omega = c.inv_omega;
     A.nbe = c.nbe;
     A.sigma = sigma;
-    const double pi = 4.0 * std::atan(1.0);
-    // Sctr4THz2 (constants.f90:62-63) / Nq^2
-    A.factor = 0.125 * pi * std::pow(1.054571817 * 1.602176565, 2) * std::pow(6.022140857, 4) * 1000000.0 /
-               ((double)c.nq * (double)c.nq);
+    A.factor = lw4_factor(c);
     int rc;
     if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)q0_count * A.nchunks * ND * sizeof(double))))
         return rc;
@@ -219,9 +231,7 @@ static int launch_lw4_t(Ctx &c, int q0_first, int q0_count, double sigma, double
     hipLaunchKernelGGL((k_lw4<R, ND, BLK>), dim3(A.nchunks, q0_count), dim3(BLK), dyn, c.stream, A);
     SPR_HIP(hipGetLastError());
     SPR_HIP(hipEventRecord(c.ev1, c.stream));
-    hipLaunchKernelGGL(k_reduce_chunks, dim3((q0_count * ND + 255) / 256), dim3(256), 0, c.stream, c.partial,
-                       q0_count, A.nchunks, ND, d_lw);
-    SPR_HIP(hipGetLastError());
+    if ((rc = launch_reduce_chunks(c, c.partial, q0_count, A.nchunks, d_lw))) return rc;
     SPR_HIP(hipEventSynchronize(c.ev1));
     float ms = 0.f;
     SPR_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
@@ -236,6 +246,15 @@ int launch_lw4(Ctx &c, int q0_first, int q0_count, double sigma, int prec, doubl
         return SPR_E_UNSUPPORTED;
     }
     const bool f64 = prec == SPR_PREC_FP64;
+    // FP32: the matrix-core kernel where its shape is built and the item fits in LDS
+    // (SPR_KERNEL=generic keeps the vector kernel)
+    if (!f64) {
+        const char *f = getenv("SPR_KERNEL");
+        if (!(f && std::string(f) == "generic")) {
+            const int rc = launch_lw4_mm(c, q0_first, q0_count, sigma, d_lw);
+            if (rc != SPR_E_UNSUPPORTED) return rc;
+        }
+    }
     switch (c.nd) {
         case 3: return f64 ? launch_lw4_t<double, 3>(c, q0_first, q0_count, sigma, d_lw)
                            : launch_lw4_t<float, 3>(c, q0_first, q0_count, sigma, d_lw);

@@ -138,6 +138,10 @@ int ensure(void **p, size_t *cap, size_t bytes);
 int launch_lw3(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw,
                double *d_w_out);
 int launch_lw4(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw);
+int launch_lw4_mm(Ctx &c, int q0_first, int q0_count, double sigma, double *d_lw);
+double lw4_factor(const Ctx &c);
+// d_lw[q][s] = sum over chunks in order of partial[(q, chunk)][s]
+int launch_reduce_chunks(Ctx &c, const double *partial, int q0_count, int nchunks, double *d_lw);
 int ensure_pairs(Ctx &c);
 int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double *d_lw);
 void drop_graphs(Ctx &c);

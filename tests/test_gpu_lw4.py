@@ -33,3 +33,19 @@ def test_lw4_even_mesh(rocksalt):
     lw = ctx.lw4_all(0.1, "fp64")[sel]
     ref = np.array([O.lw4_gauss(inp, int(qm.irr_idx[i]), 0.1) for i in sel])
     assert rel_err(lw, ref, 1e-9) < 1e-10
+
+
+def test_lw4_fp32_matrix_cores_vs_fp64_and_vector(rocksalt, monkeypatch):
+    """FP32 runs on the matrix-core kernel; it agrees with the FP64 path and with the FP32
+    vector kernel (SPR_KERNEL=generic) within the FP32 tolerance."""
+    from spring_amd.lib import GpuContext
+    qm, ph = problem(rocksalt, (4, 4, 4))
+    ctx = GpuContext(rocksalt, qm, ph)
+    lw64 = ctx.lw4_all(0.1, "fp64")
+    lw32 = ctx.lw4_all(0.1, "fp32")
+    monkeypatch.setenv("SPR_KERNEL", "generic")
+    lw32v = ctx.lw4_all(0.1, "fp32")
+    e64, ev = rel_err(lw32, lw64, 1e-6), rel_err(lw32, lw32v, 1e-6)
+    print("lw4 fp32 matrix cores: vs fp64", e64, "vs fp32 vector", ev, flush=True)
+    assert e64 < 1e-4 and ev < 1e-4
+    assert np.all(np.isfinite(lw32))

@@ -6,8 +6,8 @@
 // workgroup barrier between them.  With the index pairs ab = (a, b), cd = (c, d),
 // s01 = (s0, s1), s23 = (s2, s3), per (q0, q1, q2):
 //   FT4     T[cd][ab]   = sum_u Phi_u(a, b, c, d) e^{2 pi i m_u / L}     (ScatterMatEl4,
-//           FourthOrder_m.f90:286-422; VALU, FC4 of the item in LDS, the cluster phases of the
-//           triple computed once per wave)
+//           FourthOrder_m.f90:286-422; VALU, lanes along the FC4 components of a cluster, the
+//           cluster phases of the triple computed once per wave)
 //   GEMM 1  Y[cd][s01]  = sum_ab T[cd][ab] e0(a, s0) e1(b, s1)           (K = Ndof^2 complex)
 //   GEMM 2  V[s23][s01] = sum_cd e2(c, s2) e3(d, s3) Y[cd][s01]          (K = Ndof^2 complex)
 // (no eigenvector conjugated), then W4 = |V|^2 prod(1/w) Sctr4THz2/Nq^2 and the three processes
@@ -23,8 +23,9 @@
 // [[Re, Im], [-Im, Re]] (output columns (s01, re/im)), in GEMM 2 as the A operand
 // [[Re, -Im], [Im, Re]] (output rows (s23, re/im), so a lane holds whole complex values).
 // GEMM 1 consumes T one M tile (16 rows cd) at a time and writes those rows of Y in place,
-// so T and Y share one tile per wave.  e0 x e1 is the same for every triple of the item and
-// is tabulated once in LDS; e2 x e3 is formed while the A fragments are built.  The delta
+// so T and Y share one tile per wave.  e0 x e1 is the same for every triple of the item: its
+// B fragments are tabulated once per item in LDS; e2 x e3 is formed while the A fragments
+// are built.  The delta
 // arguments are formed in FP64, the weighted terms summed with a compensated FP32 sum per
 // lane, and the per-s0 totals reduced in FP64 in a fixed order.
 #include <algorithm>
@@ -46,7 +47,6 @@ struct Lw4MmArgs {
     const int *blk_start, *tup_site;
     const float *fc;           // (nterm, 81)
     int nterm;
-    int fc_lds;                // FC4 staged in LDS per item (else read from global memory)
     const int4 *site;
     int nsite;
     const float2 *phase;       // (L)
@@ -63,7 +63,6 @@ template <int ND> struct M4 {
     static constexpr int KP = (K2 + 15) / 16 * 16;     // K padded to the 4-lane-group split
     static constexpr int S = KP / 4;                   // K values per lane group
     static constexpr int PA = KP + 4;                  // row pitch of T / Y (floats)
-    static constexpr int PE = KP / 2 + 2;              // row pitch of the e0 x e1 table (complex)
     static constexpr int MT1 = (NN + 15) / 16;         // GEMM 1: rows cd
     static constexpr int NT1 = (K2 + 15) / 16;         //         columns (s01, re/im)
     static constexpr int MT2 = (K2 + 15) / 16;         // GEMM 2: rows (s23, re/im)
@@ -71,7 +70,6 @@ template <int ND> struct M4 {
     static constexpr int NW = 8;                       // waves per workgroup
     static constexpr int NBLK = NB * NB * NB * NB;
     static_assert(S % 4 == 0, "A fragments are read as float4");
-    static_assert(KP / 2 <= PE, "table rows cover the padded K");
 };
 
 constexpr size_t al16(size_t b) { return (b + 15) & ~(size_t)15; }
@@ -79,12 +77,12 @@ constexpr size_t al16(size_t b) { return (b + 15) & ~(size_t)15; }
 // LDS layout, shared by the kernel and the host-side size check
 template <int ND> struct M4Lds {
     using Sh = M4<ND>;
-    size_t tile, e01, fc, e, md, m1, ph, blk, red, wave, total;
-    __host__ __device__ M4Lds(int nterm, int nsite, int L, bool fc_lds) {
+    size_t tile, e01, e, md, m1, ph, blk, red, wave, total;
+    // e01: the GEMM 1 B fragments of e0 x e1, per (N tile, lane) S contiguous floats
+    __host__ __device__ M4Lds(int nterm, int nsite, int L) {
         // per-wave T / Y tiles (NN rows each)
         tile = al16((size_t)Sh::NW * Sh::NN * Sh::PA * sizeof(float));
-        e01 = al16((size_t)Sh::NN * Sh::PE * sizeof(float2));
-        fc = fc_lds ? al16((size_t)nterm * 81 * sizeof(float)) : 0;
+        e01 = al16((size_t)Sh::NT1 * 64 * Sh::S * sizeof(float));
         e = al16((size_t)2 * Sh::NN * sizeof(float2));
         md = al16((size_t)6 * ND * sizeof(double));
         m1 = al16((size_t)nsite * sizeof(int));
@@ -95,7 +93,7 @@ template <int ND> struct M4Lds {
         // q2 q3 | cluster phases
         wave = e + al16((size_t)Sh::NN * 32) + al16((size_t)2 * nsite * sizeof(int)) +
                al16((size_t)nterm * sizeof(float2));
-        total = tile + e01 + fc + e + md + m1 + ph + blk + red + Sh::NW * wave;
+        total = tile + e01 + e + md + m1 + ph + blk + red + Sh::NW * wave;
     }
 };
 
@@ -110,21 +108,34 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// B fragment of the expansion of the complex table E[s][c] (pitch PE): B'[(c, p)][(s, q)] =
-// Re E for p == q, Im E for (p, q) = (0, 1), -Im E for (1, 0)
+// B fragment element j of lane l, N tile nt, of the expansion of E[(s0, s1)][(a, b)] =
+// e0(a, s0) e1(b, s1): B'[(ab, p)][(s01, q)] = Re E for p == q, Im E for (p, q) = (0, 1),
+// -Im E for (1, 0)
 template <int ND>
-__device__ __forceinline__ void b_table(const float2 *sE, int n_base, float (&b)[M4<ND>::S]) {
+__device__ __forceinline__ float b_e01(const float2 *e0, const float2 *e1, int nt, int l, int j) {
     using Sh = M4<ND>;
-    const int l = threadIdx.x & 63, g = l >> 4;
-    const int np = n_base + (l & 15);
+    const int g = l >> 4;
+    const int np = nt * 16 + (l & 15);
     const int s = np >> 1, q = np & 1;
-    const bool ok = np < Sh::K2;
+    const int c = g * (Sh::S / 2) + j / 2, p = j & 1;
+    if (np >= Sh::K2 || c >= Sh::NN) return 0.f;
+    const float2 e = cmulf(e0[(s / ND) * ND + c / ND], e1[(s % ND) * ND + c % ND]);
+    return p == q ? e.x : (q ? e.y : -e.y);
+}
+
+// the tabulated B fragment of N tile nt for this lane
+template <int ND>
+__device__ __forceinline__ void b_table(const float *sB, int nt, float (&b)[M4<ND>::S]) {
+    using Sh = M4<ND>;
+    const int l = threadIdx.x & 63;
+    const float4 *p = reinterpret_cast<const float4 *>(sB + ((size_t)nt * 64 + l) * Sh::S);
 #pragma unroll
-    for (int i = 0; i < Sh::S / 2; ++i) {
-        const int c = g * (Sh::S / 2) + i;
-        const float2 e = ok ? sE[s * Sh::PE + c] : make_float2(0.f, 0.f);
-        b[2 * i] = q ? e.y : e.x;
-        b[2 * i + 1] = q ? e.x : -e.y;
+    for (int i = 0; i < Sh::S / 4; ++i) {
+        const float4 v = p[i];
+        b[4 * i] = v.x;
+        b[4 * i + 1] = v.y;
+        b[4 * i + 2] = v.z;
+        b[4 * i + 3] = v.w;
     }
 }
 
@@ -140,9 +151,10 @@ __device__ __forceinline__ void a_kron(const float2 *e2, const float2 *e3, int m
     const int s2 = ok ? s / ND : 0, s3 = ok ? s % ND : 0;
 #pragma unroll
     for (int i = 0; i < Sh::S / 2; ++i) {
-        const int c = g * (Sh::S / 2) + i;
-        float2 e = make_float2(0.f, 0.f);
-        if (ok && c < Sh::NN) e = cmulf(e2[s2 * ND + c / ND], e3[s3 * ND + c % ND]);
+        // padding (rows past K2, K past NN) reads a valid entry and is zeroed by a select
+        const int c = g * (Sh::S / 2) + i, cc = min(c, Sh::NN - 1);
+        float2 e = cmulf(e2[s2 * ND + cc / ND], e3[s3 * ND + cc % ND]);
+        if (!(ok && c < Sh::NN)) e = make_float2(0.f, 0.f);
         a[2 * i] = po ? e.y : e.x;
         a[2 * i + 1] = po ? e.x : -e.y;
     }
@@ -171,12 +183,13 @@ template <int ND>
 __device__ __forceinline__ void b_y(const float *Y, int nt, float (&b)[M4<ND>::S]) {
     using Sh = M4<ND>;
     const int l = threadIdx.x & 63, g = l >> 4;
-    const int s01 = nt * 16 + (l & 15);
+    const int s01 = nt * 16 + (l & 15), sc = min(s01, Sh::NN - 1);
 #pragma unroll
     for (int i = 0; i < Sh::S / 2; ++i) {
-        const int cd = g * (Sh::S / 2) + i;
-        float2 y = make_float2(0.f, 0.f);
-        if (s01 < Sh::NN && cd < Sh::NN) y = *reinterpret_cast<const float2 *>(Y + cd * Sh::PA + 2 * s01);
+        // padding reads a valid entry and is zeroed by a select
+        const int cd = g * (Sh::S / 2) + i, cc = min(cd, Sh::NN - 1);
+        float2 y = *reinterpret_cast<const float2 *>(Y + cc * Sh::PA + 2 * sc);
+        if (!(s01 < Sh::NN && cd < Sh::NN)) y = make_float2(0.f, 0.f);
         b[2 * i] = y.x;
         b[2 * i + 1] = y.y;
     }
@@ -193,18 +206,16 @@ __device__ __forceinline__ v4f mma(const float (&a)[S], const float (&b)[S]) {
 template <int ND>
 __global__ __launch_bounds__(512) void k_lw4_mm(Lw4MmArgs A) {
     using Sh = M4<ND>;
-    constexpr int NN = Sh::NN, PA = Sh::PA, PE = Sh::PE, NW = Sh::NW, NB = Sh::NB;
+    constexpr int NN = Sh::NN, PA = Sh::PA, NW = Sh::NW, NB = Sh::NB;
     constexpr int MT1 = Sh::MT1, NT1 = Sh::NT1, MT2 = Sh::MT2, NT2 = Sh::NT2;
-    const M4Lds<ND> Ly(A.nterm, A.nsite, A.L, A.fc_lds != 0);
+    const M4Lds<ND> Ly(A.nterm, A.nsite, A.L);
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char *p = smem;
     float *sTile = reinterpret_cast<float *>(p);
     p += Ly.tile;
-    float2 *sE01 = reinterpret_cast<float2 *>(p);      // e0(a, s0) e1(b, s1) as [s01][ab]
+    float *sB01 = reinterpret_cast<float *>(p);        // GEMM 1 B fragments of e0 x e1
     p += Ly.e01;
-    float *sFC = reinterpret_cast<float *>(p);
-    const float *fcv = A.fc_lds ? sFC : A.fc;
-    p += Ly.fc;
+    const float *__restrict__ fcv = A.fc;             // FC4 from global memory (cached)
     float2 *sE = reinterpret_cast<float2 *>(p);        // e0 | e1
     p += Ly.e;
     double *sMd = reinterpret_cast<double *>(p);       // w n 1/w of q0, then of q1
@@ -250,20 +261,17 @@ __global__ __launch_bounds__(512) void k_lw4_mm(Lw4MmArgs A) {
     }
     for (int i = tid; i < A.L; i += 512) sPh[i] = A.phase[i];
     for (int i = tid; i <= Sh::NBLK; i += 512) sBlk[i] = A.blk_start[i];
-    if (A.fc_lds)
-        for (int i = tid; i < A.nterm * 81; i += 512) sFC[i] = A.fc[i];
     __syncthreads();
-    for (int i = tid; i < NN * PE; i += 512) {
-        const int s = i / PE, c = i % PE;
-        float2 e = make_float2(0.f, 0.f);
-        if (c < NN) e = cmulf(sE[(s / ND) * ND + c / ND], sE[NN + (s % ND) * ND + c % ND]);
-        sE01[i] = e;
+    for (int i = tid; i < NT1 * 64 * Sh::S; i += 512) {
+        const int nt = i / (64 * Sh::S), ll = (i / Sh::S) % 64, j = i % Sh::S;
+        sB01[i] = b_e01<ND>(sE, sE + NN, nt, ll, j);
     }
     __syncthreads();
 
     // ---- per lane: constants of its GEMM 2 columns s01 = nt*16 + l%16 ----
     const double c0 = 1.0 / (A.sigma * 2.5066282746310002);   // 1/(sigma sqrt(2 pi))
-    const float hneg = (float)(-0.5 / (A.sigma * A.sigma));
+    // exp(-x^2 / (2 sigma^2)) = 2^(hl2 x^2) on the exponential unit
+    const float hl2 = (float)(-0.5 / (A.sigma * A.sigma) * 1.4426950408889634);
     const bool g0 = i0 == 0;
     double Ra[NT2], Rb[NT2];
     float r1[NT2], r2[NT2];
@@ -321,22 +329,28 @@ __global__ __launch_bounds__(512) void k_lw4_mm(Lw4MmArgs A) {
         for (int blk = 0; blk < Sh::NBLK; ++blk) {
             const int mu = blk / (NB * NB * NB), nu = (blk / (NB * NB)) % NB, et = (blk / NB) % NB, rh = blk % NB;
             const int u0 = sBlk[blk], u1 = sBlk[blk + 1];
+            // component l and (lanes < 17) 64 + l
+            const bool two = l < 81 - 64;
+            float re0 = 0.f, im0 = 0.f, re1 = 0.f, im1 = 0.f;
+#pragma unroll 4
+            for (int u = u0; u < u1; ++u) {
+                const float2 ph = wPh[u];
+                const float f0 = fcv[u * 81 + l];
+                const float f1 = two ? fcv[u * 81 + 64 + l] : 0.f;
+                re0 = fmaf(f0, ph.x, re0);
+                im0 = fmaf(f0, ph.y, im0);
+                re1 = fmaf(f1, ph.x, re1);
+                im1 = fmaf(f1, ph.y, im1);
+            }
 #pragma unroll
-            for (int c0i = 0; c0i < 81; c0i += 64) {
-                const int comp = c0i + l;
+            for (int h = 0; h < 2; ++h) {
+                const int comp = h * 64 + l;
                 if (comp < 81) {
-                    float re = 0.f, im = 0.f;
-                    for (int u = u0; u < u1; ++u) {
-                        const float f = fcv[u * 81 + comp];
-                        const float2 ph = wPh[u];
-                        re = fmaf(f, ph.x, re);
-                        im = fmaf(f, ph.y, im);
-                    }
                     const int a = 3 * mu + comp / 27, b = 3 * nu + (comp / 9) % 3;
                     const int c = 3 * et + (comp / 3) % 3, d = 3 * rh + comp % 3;
                     float *dst = T + (c * ND + d) * PA + 2 * (a * ND + b);
-                    dst[0] = re;
-                    dst[1] = im;
+                    dst[0] = h ? re1 : re0;
+                    dst[1] = h ? im1 : im0;
                 }
             }
         }
@@ -351,7 +365,7 @@ __global__ __launch_bounds__(512) void k_lw4_mm(Lw4MmArgs A) {
 #pragma unroll 1
             for (int nt = 0; nt < NT1; ++nt) {
                 float b[Sh::S];
-                b_table<ND>(sE01, nt * 16, b);
+                b_table<ND>(sB01, nt, b);
                 const v4f y = mma<Sh::S>(a, b);
                 const int col = nt * 16 + (l & 15);
 #pragma unroll
@@ -365,6 +379,9 @@ __global__ __launch_bounds__(512) void k_lw4_mm(Lw4MmArgs A) {
 
         // ---- GEMM 2 per M tile of rows (s23, re/im), then W4 and the processes on each tile:
         //      rows 4g, 4g+1 are (re, im) of s23 = mt*8 + 2g, rows 4g+2, 4g+3 of s23 + 1 ----
+        float tri[NT2];
+#pragma unroll
+        for (int nt = 0; nt < NT2; ++nt) tri[nt] = 0.f;
 #pragma unroll 1
         for (int mt = 0; mt < MT2; ++mt) {
             asm volatile("" ::: "memory");   // keeps the B fragments of the N tiles inside the loop
@@ -383,12 +400,15 @@ __global__ __launch_bounds__(512) void k_lw4_mm(Lw4MmArgs A) {
                     const double2 cd2 = wCd[s23];
                     const float4 cf = wCf[s23];
                     const float x1 = (float)(Ra[nt] - cd2.x), x2 = (float)(Rb[nt] - cd2.x), x3 = (float)(Rb[nt] - cd2.y);
-                    const float e1 = __expf(hneg * x1 * x1), e2 = __expf(hneg * x2 * x2), e3 = __expf(hneg * x3 * x3);
+                    const float e1 = __builtin_amdgcn_exp2f(hl2 * x1 * x1), e2 = __builtin_amdgcn_exp2f(hl2 * x2 * x2),
+                                e3 = __builtin_amdgcn_exp2f(hl2 * x3 * x3);
                     const float inner = cf.x * (r1[nt] * e1 * (1.f / 6.f) + r2[nt] * e2 * 0.5f) + r2[nt] * cf.y * e3 * 0.5f;
-                    acc[nt].add((v[2 * h] * v[2 * h] + v[2 * h + 1] * v[2 * h + 1]) * cf.z * inner);
+                    tri[nt] = fmaf((v[2 * h] * v[2 * h] + v[2 * h + 1] * v[2 * h + 1]) * cf.z, inner, tri[nt]);
                 }
             }
         }
+#pragma unroll
+        for (int nt = 0; nt < NT2; ++nt) acc[nt].add(tri[nt]);
     }
 
     // ---- per s0: the lane's columns times their constants, then waves in a fixed order ----
@@ -443,8 +463,7 @@ int launch_lw4_mm_t(Ctx &c, int q0_first, int q0_count, double sigma, double *d_
     A.nbe = c.nbe;
     A.sigma = sigma;
     A.factor = lw4_factor(c);
-    A.fc_lds = M4Lds<ND>(A.nterm, A.nsite, A.L, true).total <= 160 * 1024;
-    const size_t dyn = M4Lds<ND>(A.nterm, A.nsite, A.L, A.fc_lds != 0).total;
+    const size_t dyn = M4Lds<ND>(A.nterm, A.nsite, A.L).total;
     if (dyn > 160 * 1024) return SPR_E_UNSUPPORTED;   // the caller falls back to the vector kernel
     int rc;
     if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)q0_count * c.nq * ND * sizeof(double))))

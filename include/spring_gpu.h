@@ -119,6 +119,33 @@ int spr_lw3_tetra(spr_ctx *ctx, int i0, double *lw, double *p3_q0);
 /* every irreducible q0: lw_all(Ndof, nq0), p3_all(Ndof, nq0) */
 int spr_lw3_tetra_all(spr_ctx *ctx, double *lw_all, double *p3_all);
 
+/* ---- Iterative Boltzmann transport (kappa_iter.x, src/Kappa/Src_iterative) ----
+ * Works on the q points irreducible under time reversal only, as the reference does
+ * (make_tetrahedron, Src_iterative/Irr_q_point_m.f90:206-330): representative = the lower grid
+ * index of q and -q, F(-q) = -F(q).  The first call stores W of every representative q and
+ * every grid q1 in device memory (QconsrvMomentum, Quasi_momentum_consrv.f90:118-451;
+ * nirr x Nq x Ndof^3 doubles) and needs the context's tetrahedra.  F arrays are
+ * F[qi][s][3] (F_vecAll(3, s, qi)), 1/tau arrays [qi][s] (Inv_tauqs(s, qi)). */
+
+/* nirr, and (if not NULL) the grid index of each irreducible point and the number of grid
+ * points it stands for (1 or 2). */
+int spr_bte_irr(spr_ctx *ctx, int *nirr, int *irr_grid, int *deg);
+
+/* 1/tau of every irreducible (q, s): TetrahedronWplus + TetrahedronWminus
+ * (Src_iterative/Quasi_momentum_consrv.f90:701-1208). */
+int spr_bte_tau_inv(spr_ctx *ctx, double *inv_tau);
+
+/* dF of one iteration for the given F, before the division by 1/tau: TetrahedronIterPlus +
+ * TetrahedronIterMinus (Src_iterative/TetrahedronDeltaSum.f90:22-620). */
+int spr_bte_delta_f(spr_ctx *ctx, const double *F, double *dF);
+
+/* The self-consistent loop of Src_iterative/main.f90:243-361 at temperature T (K): F0 of
+ * CreateFzero, then F = F0 + dF/(1/tau) (multiplyTaudFnew) until the largest change of a kappa
+ * component is below eps_k (W/mK) or max_iter iterations ran.  kappa[9] (row-major 3x3, W/mK);
+ * kappa0[9] (F = F0, may be NULL), iterations, F_out[nirr][nd][3] (may be NULL). */
+int spr_bte_solve(spr_ctx *ctx, double T, double eps_k, int max_iter, double *kappa, double *kappa0,
+                  int *iterations, double *F_out);
+
 /* Four-phonon Gaussian linewidth sum of irreducible q0 `i0` over all (q1,q2), the processes
  * and weights of TetrahedronIntegration4th (Tetrahedron4th_sep.f90:242-343) with a Gaussian
  * delta; main.f90:660 adds 0.5*lw to the linewidth. */

@@ -21,6 +21,9 @@
  *   orc_scatter_mat_el4 / orc_v4_factorised / orc_lw4_gauss   as the 3-phonon ones
  *   orc_kappa_rta          closed-form single-mode model; symmetric tensor
  *   orc_lw3_tetra          sum of vertex weights == 1; linear-band density of states
+ *   orc_bte_sums           1/tau == 2 x the tetrahedron linewidth of orc_lw3_tetra; with
+ *                          W+ = W- = 1 and F = a w on the full grid, dF == a w 1/tau exactly
+ *                          (the vertex weights reproduce Mqs); linearity in F
  */
 #include <complex.h>
 #include <math.h>
@@ -609,5 +612,128 @@ void orc_lw3_tetra(int nd, int nq, const int *tetra, const int *partner, const d
         }
         lw[s0] = acc;
         p3q0[s0] = pacc;
+    }
+}
+
+
+/* --------------------------------------------------------------------------------------
+ * Iterative BTE (src/Kappa/Src_iterative).  Linear tetrahedron weights of that code
+ * (TetrahedronWplus, Quasi_momentum_consrv.f90:815-900): strict inequalities in all three
+ * cases, the vertex weights must sum to one within zero_prec = 5e-5 (else nothing is added);
+ * vertices sorted by M with their data (my_sort, :624-698, stable for ties).  Adds
+ * mul * g/6 * sum_v I_v X_v[k] to acc[k] for k < K.
+ * ------------------------------------------------------------------------------------*/
+static void orc_tetra_iter(double Mqs, const double *M_in, const double *X_in, int K, double mul, double *acc)
+{
+    int idx[4] = {0, 1, 2, 3};
+    for (int a = 1; a < 4; ++a)
+        for (int b = a; b > 0 && M_in[idx[b - 1]] > M_in[idx[b]]; --b) {
+            int t = idx[b]; idx[b] = idx[b - 1]; idx[b - 1] = t;
+        }
+    double M[4];
+    for (int v = 0; v < 4; ++v) M[v] = M_in[idx[v]];
+    double g, Iv[4];
+    if (Mqs > M[0] && Mqs < M[1]) {
+        g = 3.0 * (Mqs - M[0]) * (Mqs - M[0]) / ((M[1] - M[0]) * (M[2] - M[0]) * (M[3] - M[0]));
+        Iv[0] = ((Mqs - M[1]) / (M[0] - M[1]) + (Mqs - M[2]) / (M[0] - M[2]) + (Mqs - M[3]) / (M[0] - M[3])) / 3.0;
+        for (int v = 1; v < 4; ++v) Iv[v] = ((Mqs - M[0]) / (M[v] - M[0])) / 3.0;
+    } else if (Mqs > M[1] && Mqs < M[2]) {
+        g = (3.0 / ((M[1] - M[2]) * (M[3] - M[0]))) *
+            (((Mqs - M[0]) * (Mqs - M[2])) / (M[2] - M[0]) - ((Mqs - M[1]) * (Mqs - M[3])) / (M[1] - M[3]));
+        Iv[0] = ((Mqs - M[3]) / (M[0] - M[3])) / 3.0 +
+               ((Mqs - M[2]) / (M[0] - M[2])) * ((Mqs - M[0]) / (M[2] - M[0])) * ((Mqs - M[2]) / (M[1] - M[2])) /
+                   (g * (M[3] - M[0]));
+        Iv[1] = ((Mqs - M[2]) / (M[1] - M[2])) / 3.0 +
+               ((Mqs - M[3]) / (M[1] - M[3])) * ((Mqs - M[3]) / (M[1] - M[3])) * ((Mqs - M[1]) / (M[2] - M[1])) /
+                   (g * (M[3] - M[0]));
+        Iv[2] = ((Mqs - M[1]) / (M[2] - M[1])) / 3.0 +
+               ((Mqs - M[0]) / (M[2] - M[0])) * ((Mqs - M[0]) / (M[2] - M[0])) * ((Mqs - M[2]) / (M[1] - M[2])) /
+                   (g * (M[3] - M[0]));
+        Iv[3] = ((Mqs - M[0]) / (M[3] - M[0])) / 3.0 +
+               ((Mqs - M[1]) / (M[3] - M[1])) * ((Mqs - M[3]) / (M[1] - M[3])) * ((Mqs - M[1]) / (M[2] - M[1])) /
+                   (g * (M[3] - M[0]));
+    } else if (Mqs > M[2] && Mqs < M[3]) {
+        g = 3.0 * (Mqs - M[3]) * (Mqs - M[3]) / ((M[3] - M[0]) * (M[3] - M[1]) * (M[3] - M[2]));
+        for (int v = 0; v < 3; ++v) Iv[v] = ((Mqs - M[3]) / (M[v] - M[3])) / 3.0;
+        Iv[3] = ((Mqs - M[0]) / (M[3] - M[0]) + (Mqs - M[1]) / (M[3] - M[1]) + (Mqs - M[2]) / (M[3] - M[2])) / 3.0;
+    } else {
+        return;
+    }
+    if (!(fabs(Iv[0] + Iv[1] + Iv[2] + Iv[3] - 1.0) < 5.0e-5)) return;
+    for (int k = 0; k < K; ++k) {
+        double t = 0.0;
+        for (int v = 0; v < 4; ++v) t += Iv[v] * X_in[idx[v] * K + k];
+        acc[k] += mul * (1.0 / 6.0) * g * t;
+    }
+}
+
+static int orc_grid(const int *mesh, int a0, int a1, int a2)
+{
+    a0 = ((a0 % mesh[0]) + mesh[0]) % mesh[0];
+    a1 = ((a1 % mesh[1]) + mesh[1]) % mesh[1];
+    a2 = ((a2 % mesh[2]) + mesh[2]) % mesh[2];
+    return a0 + mesh[0] * (a1 + mesh[1] * a2);
+}
+
+/* TetrahedronWplus + TetrahedronWminus (Quasi_momentum_consrv.f90:701-1208) when F == NULL:
+ *   out[qi][s] = 1/tau;
+ * TetrahedronIterPlus + TetrahedronIterMinus (TetrahedronDeltaSum.f90:22-620) otherwise:
+ *   out[qi][s][3] = dF before the division by 1/tau.
+ * For the irreducible point q = irr_grid[qi] and every tetrahedron T (tetra[6 nq][4] grid
+ * indices), every s, s1, s2, vertex q1 of T:
+ *   +: M = w(q+q1, s2) - w(q1, s1), X = Wp[qi][q1][s][s1][s2] (F(q+q1, s2) - F(q1, s1))
+ *   -: M = w(q1, s1) + w(q-q1, s2),  X = Wm[qi][-q1][s][s1][s2] (F(q-q1, s2) + F(q1, s1)), x 1/2
+ * Wp / Wm[qi][q1] are W+ / W- of the pair (q1, -q-q1).  F of grid point g is
+ * map_sign[g] * F[map_irr[g]] (F(-q) = -F(q)); X = W when F == NULL. */
+void orc_bte_sums(int nd, int nq, const int *mesh, int nirr, const int *irr_grid, const int *map_irr,
+                  const int *map_sign, const int *tetra, const double *Wp, const double *Wm,
+                  const double *omega, const double *F, double *out)
+{
+    const int K = F ? 3 : 1;
+    const size_t per_q = (size_t)nq * nd * nd * nd;
+    memset(out, 0, sizeof(double) * (size_t)nirr * nd * K);
+    for (int qi = 0; qi < nirr; ++qi) {
+        const int q = irr_grid[qi];
+        const int k0 = q % mesh[0], k1 = (q / mesh[0]) % mesh[1], k2 = q / (mesh[0] * mesh[1]);
+        for (int t = 0; t < 6 * nq; ++t) {
+            int vq1[4], vp[4], vm[4], vn[4];
+            for (int v = 0; v < 4; ++v) {
+                const int g = tetra[4 * t + v];
+                const int a0 = g % mesh[0], a1 = (g / mesh[0]) % mesh[1], a2 = g / (mesh[0] * mesh[1]);
+                vq1[v] = g;
+                vp[v] = orc_grid(mesh, k0 + a0, k1 + a1, k2 + a2);
+                vm[v] = orc_grid(mesh, k0 - a0, k1 - a1, k2 - a2);
+                vn[v] = orc_grid(mesh, -a0, -a1, -a2);
+            }
+            for (int s = 0; s < nd; ++s) {
+                const double Mqs = omega[(size_t)q * nd + s];
+                for (int s1 = 0; s1 < nd; ++s1)
+                    for (int s2 = 0; s2 < nd; ++s2) {
+                        double Mp[4], Mm[4], Xp[12], Xm[12];
+                        for (int v = 0; v < 4; ++v) {
+                            const size_t off = (size_t)s * nd * nd + (size_t)s1 * nd + s2;
+                            const double wp = Wp[(size_t)qi * per_q + (size_t)vq1[v] * nd * nd * nd + off];
+                            const double wm = Wm[(size_t)qi * per_q + (size_t)vn[v] * nd * nd * nd + off];
+                            Mp[v] = omega[(size_t)vp[v] * nd + s2] - omega[(size_t)vq1[v] * nd + s1];
+                            Mm[v] = omega[(size_t)vq1[v] * nd + s1] + omega[(size_t)vm[v] * nd + s2];
+                            if (!F) {
+                                Xp[v] = wp;
+                                Xm[v] = wm;
+                            } else {
+                                for (int k = 0; k < 3; ++k) {
+                                    const double f1 = map_sign[vq1[v]] * F[((size_t)map_irr[vq1[v]] * nd + s1) * 3 + k];
+                                    const double fp = map_sign[vp[v]] * F[((size_t)map_irr[vp[v]] * nd + s2) * 3 + k];
+                                    const double fm = map_sign[vm[v]] * F[((size_t)map_irr[vm[v]] * nd + s2) * 3 + k];
+                                    Xp[v * 3 + k] = wp * (fp - f1);
+                                    Xm[v * 3 + k] = wm * (fm + f1);
+                                }
+                            }
+                        }
+                        double *o = out + ((size_t)qi * nd + s) * K;
+                        orc_tetra_iter(Mqs, Mp, Xp, K, 1.0, o);
+                        orc_tetra_iter(Mqs, Mm, Xm, K, 0.5, o);
+                    }
+            }
+        }
     }
 }

@@ -210,3 +210,111 @@ def lw3_tetra(qm, ph, i0_grid, W):
     lib().orc_lw3_tetra(ct.c_int(nd), ct.c_int(qm.nq), *[a.ctypes.data_as(ct.c_void_p) for a in arrs],
                         lw.ctypes.data_as(ct.c_void_p), p3.ctypes.data_as(ct.c_void_p))
     return lw, p3
+
+
+# ---------------------------------------------------------------------------------------------
+# Iterative BTE (src/Kappa/Src_iterative): time-reversal irreducible set, W+/W-, the tetrahedron
+# sums of orc_bte_sums, and the self-consistent loop of main.f90:243-361.
+# ---------------------------------------------------------------------------------------------
+
+FUNIT_E_11M_K = 1.054571817 / 1.380649      # Src_iterative/constants.f90:56
+KW_MK = 1.054571817 * 0.001                 # Src_iterative/constants.f90:58
+BTE_ZERO_PREC = 5.0e-5                      # Src_iterative/constants.f90:29
+
+
+def tr_irreducible(qm):
+    """Grid points irreducible under time reversal only (make_tetrahedron,
+    Irr_q_point_m.f90:206-330): representative = the lower grid index of q and -q.
+    Returns (irr grid indices, map_irr[nq], map_sign[nq], deg[nirr])."""
+    nq = qm.nq
+    neg = qm.index(-qm.q_int.astype(np.int64))
+    map_irr = np.full(nq, -1, np.int32)
+    map_sign = np.zeros(nq, np.int32)
+    irr, deg = [], []
+    for g in range(nq):
+        if map_irr[g] >= 0:
+            continue
+        k = len(irr)
+        irr.append(g)
+        map_irr[g], map_sign[g] = k, 1
+        deg.append(1)
+        if neg[g] != g:
+            map_irr[neg[g]], map_sign[neg[g]] = k, -1
+            deg[-1] = 2
+    return np.array(irr, np.int32), map_irr, map_sign, np.array(deg, np.int32)
+
+
+def bte_w_pm(qm, ph, irr, W):
+    """W+ = 4 W (n1 - n2), W- = 4 W (1 + n1 + n2) of the pairs (q1, -q-q1) for every irreducible q:
+    W[qi][q1][s][s1][s2] is the linewidth path's W (Sctr3THz2 with pi/16); the iterative code's
+    Sctr3THz2 has pi/4 (Src_iterative/constants.f90:53-54; ScatterProbability,
+    Quasi_momentum_consrv.f90:567-622)."""
+    nbe = ph.nbe
+    Wp = np.empty_like(W)
+    Wm = np.empty_like(W)
+    for k, q in enumerate(irr):
+        q2 = qm.index(-qm.q_int[q][None, :].astype(np.int64) - qm.q_int.astype(np.int64))
+        n1 = nbe[:, None, :, None]             # (q1, 1, s1, 1)
+        n2 = nbe[q2][:, None, None, :]         # (q1, 1, 1, s2)
+        Wp[k] = 4.0 * W[k] * (n1 - n2)
+        Wm[k] = 4.0 * W[k] * (1.0 + n1 + n2)
+    return Wp, Wm
+
+
+def bte_sums(qm, ph, irr, map_irr, map_sign, Wp, Wm, F=None):
+    """orc_bte_sums: 1/tau (nirr, nd) when F is None, else dF before the 1/tau division
+    (nirr, nd, 3)."""
+    nd = ph.omega.shape[1]
+    nirr = len(irr)
+    out = np.zeros((nirr, nd) if F is None else (nirr, nd, 3))
+    arrs = [np.ascontiguousarray(qm.mesh, np.int32), np.ascontiguousarray(irr, np.int32),
+            np.ascontiguousarray(map_irr, np.int32), np.ascontiguousarray(map_sign, np.int32),
+            np.ascontiguousarray(qm.tetra, np.int32), np.ascontiguousarray(Wp, np.float64),
+            np.ascontiguousarray(Wm, np.float64), np.ascontiguousarray(ph.omega, np.float64)]
+    Fa = None if F is None else np.ascontiguousarray(F, np.float64)
+    lib().orc_bte_sums(ct.c_int(nd), ct.c_int(qm.nq), arrs[0].ctypes.data_as(ct.c_void_p), ct.c_int(nirr),
+                       *[a.ctypes.data_as(ct.c_void_p) for a in arrs[1:]],
+                       None if Fa is None else Fa.ctypes.data_as(ct.c_void_p),
+                       out.ctypes.data_as(ct.c_void_p))
+    return out
+
+
+def bte_kappa(qm, ph, irr, deg, F, vol):
+    """Helper kappa (HelperSubroutines.f90:274-347): kW_mK/(Nq vol) sum deg w n(n+1) v (x) F,
+    acoustic branches at Gamma skipped."""
+    nd = ph.omega.shape[1]
+    k = np.zeros((3, 3))
+    for qi, g in enumerate(irr):
+        for s in range(3 if g == 0 else 0, nd):
+            w, n = ph.omega[g, s], ph.nbe[g, s]
+            k += deg[qi] * w * n * (n + 1.0) * np.outer(ph.grp_vel[g, s], F[qi, s])
+    return KW_MK / (qm.nq * vol) * k
+
+
+def bte_solve(qm, ph, vol, T, irr, map_irr, map_sign, deg, Wp, Wm, eps_k, max_iter):
+    """main.f90:243-361: F0 (CreateFzero), then F = F0 + tau dF(F) until max |kappa - kappa_old|
+    < eps_k or max_iter iterations.  Returns (kappa, kappa0, iterations, F)."""
+    nd = ph.omega.shape[1]
+    itau = bte_sums(qm, ph, irr, map_irr, map_sign, Wp, Wm)
+    skip = np.zeros((len(irr), nd), bool)
+    skip[irr == 0, :3] = True
+    ok = ~skip & (np.abs(itau) > BTE_ZERO_PREC)
+    F0 = np.zeros((len(irr), nd, 3))
+    w, v = ph.omega[irr], ph.grp_vel[irr]
+    F0[ok] = (FUNIT_E_11M_K * w[ok][:, None] * v[ok]) / (itau[ok][:, None] * T * T)
+    F = F0.copy()
+    k_old = bte_kappa(qm, ph, irr, deg, F, vol)
+    kappa0 = k_old.copy()
+    it = 0
+    k_new = k_old
+    while it < max_iter:
+        it += 1
+        dF = bte_sums(qm, ph, irr, map_irr, map_sign, Wp, Wm, F)
+        dF[ok] /= itau[ok][:, None]
+        F = F0 + dF
+        k_new = bte_kappa(qm, ph, irr, deg, F, vol)
+        done = np.max(np.abs(k_new - k_old)) < eps_k
+        k_old = k_new
+        if done:
+            break
+    return k_new, kappa0, it, F

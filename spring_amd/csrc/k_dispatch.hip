@@ -7,9 +7,6 @@
 
 namespace spr {
 
-int launch_lw3_generic(Ctx &c, const int *d_q0, int nq0, double sigma, int prec, double *d_lw,
-                       double *d_w);
-
 int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw);
 int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw);
 

@@ -78,6 +78,19 @@ struct DevIFC {
 
 constexpr int kClusterBatch = 4;
 
+// Iterative BTE state (time-reversal irreducible set, F map, W of every representative q)
+struct BteState {
+    bool ready = false;
+    int nirr = 0;
+    std::vector<int> h_irr, h_deg;        // (nirr) grid index, number of grid points it stands for
+    std::vector<int2> h_map;              // (nq) irreducible index and sign of F
+    int4 *d_coord = nullptr;
+    int2 *d_map = nullptr;
+    int *d_irr = nullptr, *d_tetra = nullptr;
+    double *d_W = nullptr;                // (nirr, nq, nd, nd, nd)
+    double *d_F = nullptr, *d_out = nullptr;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -126,6 +139,7 @@ struct Ctx {
     const int *w_pair_q0 = nullptr, *w_pair_q1 = nullptr;   // W parity path over a pair list
     int w_npairs = 0;
     size_t wout_cap = 0;
+    BteState bte;
     // timing of the dominant kernel
     int launches = 0;
     double kernel_ms = 0.0;
@@ -139,6 +153,10 @@ int launch_lw3(Ctx &c, int q0_first, int q0_count, double sigma, int prec, doubl
                double *d_w_out);
 int launch_lw4(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw);
 int launch_lw4_mm(Ctx &c, int q0_first, int q0_count, double sigma, double *d_lw);
+// W of every grid q1 for the grid points d_q0[0..nq0) (d_w, nq0 = 1) or their Gaussian linewidths
+int launch_lw3_generic(Ctx &c, const int *d_q0, int nq0, double sigma, int prec, double *d_lw, double *d_w);
+int bte_setup(Ctx &c);
+int bte_sums(Ctx &c, int mode);
 double lw4_factor(const Ctx &c);
 // d_lw[q][s] = sum over chunks in order of partial[(q, chunk)][s]
 int launch_reduce_chunks(Ctx &c, const double *partial, int q0_count, int nchunks, double *d_lw);

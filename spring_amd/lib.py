@@ -21,6 +21,7 @@ SYMBOLS = [
     "spr_kernel_time_reset", "spr_lw3_pairs", "spr_v3_w_pairs", "spr_mlp_create", "spr_mlp_destroy",
     "spr_mlp_num_params", "spr_mlp_get_params", "spr_mlp_set_params", "spr_mlp_grad",
     "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all", "spr_mlp_score",
+    "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve",
 ]
 
 
@@ -206,6 +207,39 @@ class GpuContext:
         p3 = np.zeros((self.nq0, self.nd))
         self._check(self.lib.spr_lw3_tetra_all(self.h, _ptr(lw), _ptr(p3)), "spr_lw3_tetra_all")
         return lw, p3
+
+    # ---- iterative Boltzmann transport (time-reversal irreducible points) ----
+    def bte_irr(self):
+        """(grid index of every irreducible point, number of grid points it stands for)."""
+        n = ct.c_int()
+        self._check(self.lib.spr_bte_irr(self.h, ct.byref(n), None, None), "spr_bte_irr")
+        irr = np.zeros(n.value, np.int32)
+        deg = np.zeros(n.value, np.int32)
+        self._check(self.lib.spr_bte_irr(self.h, ct.byref(n), irr.ctypes.data_as(ct.c_void_p),
+                                         deg.ctypes.data_as(ct.c_void_p)), "spr_bte_irr")
+        return irr, deg
+
+    def bte_tau_inv(self):
+        irr, _ = self.bte_irr()
+        out = np.zeros((len(irr), self.nd))
+        self._check(self.lib.spr_bte_tau_inv(self.h, _ptr(out)), "spr_bte_tau_inv")
+        return out
+
+    def bte_delta_f(self, F):
+        F = np.ascontiguousarray(F, np.float64)
+        dF = np.zeros_like(F)
+        self._check(self.lib.spr_bte_delta_f(self.h, _ptr(F), _ptr(dF)), "spr_bte_delta_f")
+        return dF
+
+    def bte_solve(self, T: float, eps_k: float = 1e-4, max_iter: int = 100):
+        irr, _ = self.bte_irr()
+        k = np.zeros(9)
+        k0 = np.zeros(9)
+        it = ct.c_int()
+        F = np.zeros((len(irr), self.nd, 3))
+        self._check(self.lib.spr_bte_solve(self.h, ct.c_double(T), ct.c_double(eps_k), ct.c_int(max_iter),
+                                           _ptr(k), _ptr(k0), ct.byref(it), _ptr(F)), "spr_bte_solve")
+        return dict(kappa=k.reshape(3, 3), kappa0=k0.reshape(3, 3), iterations=it.value, F=F)
 
     def kernel_time(self):
         n = ct.c_int()

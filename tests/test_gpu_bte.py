@@ -1,0 +1,54 @@
+"""GPU parity of the iterative Boltzmann transport solution (kappa_iter.x) against the FP64
+CPU oracle: the time-reversal irreducible set, 1/tau, one dF for a random F, and the whole
+self-consistent loop (kappa, kappa of F0, iteration count)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import problem, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(model, dims):
+    from spring_amd.lib import GpuContext
+    qm, ph = problem(model, dims)
+    ctx = GpuContext(model, qm, ph)
+    inp = O.Inputs(model, qm, ph)
+    irr, map_irr, map_sign, deg = O.tr_irreducible(qm)
+    W = np.array([O.lw3_gauss(inp, int(g), 1.0, want_w=True)[1] for g in irr])
+    Wp, Wm = O.bte_w_pm(qm, ph, irr, W)
+    return ctx, qm, ph, (irr, map_irr, map_sign, deg, Wp, Wm)
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4), (5, 5, 5)])
+def test_bte_tau_inv_and_delta_f(rocksalt, dims):
+    ctx, qm, ph, (irr, map_irr, map_sign, deg, Wp, Wm) = _setup(rocksalt, dims)
+    g_irr, g_deg = ctx.bte_irr()
+    assert np.array_equal(g_irr, irr) and np.array_equal(g_deg, deg)
+    itau = ctx.bte_tau_inv()
+    ref = O.bte_sums(qm, ph, irr, map_irr, map_sign, Wp, Wm)
+    e = rel_err(itau, ref, 1e-9)
+    print(f"{dims} 1/tau max rel err {e:.2e}", flush=True)
+    assert e < 1e-10
+    # a random F makes some dF components cancel to near zero: those are held to the sum's
+    # rounding (normwise 1e-12), the others elementwise to 1e-10
+    F = np.random.default_rng(7).normal(size=(len(irr), ph.omega.shape[1], 3))
+    dF = ctx.bte_delta_f(F)
+    refF = O.bte_sums(qm, ph, irr, map_irr, map_sign, Wp, Wm, F)
+    en = np.max(np.abs(dF - refF)) / np.max(np.abs(refF))
+    e = rel_err(dF, refF, 1e-4)
+    print(f"{dims} dF normwise rel err {en:.2e}, elementwise (above 1e-4 of max) {e:.2e}", flush=True)
+    assert en < 1e-12 and e < 1e-10
+
+
+def test_bte_solve(rocksalt):
+    ctx, qm, ph, (irr, map_irr, map_sign, deg, Wp, Wm) = _setup(rocksalt, (4, 4, 4))
+    T, vol = 300.0, rocksalt.crystal.vol
+    out = ctx.bte_solve(T, eps_k=1e-6, max_iter=100)
+    k, k0, it, F = O.bte_solve(qm, ph, vol, T, irr, map_irr, map_sign, deg, Wp, Wm, 1e-6, 100)
+    print("bte kappa", np.diag(out["kappa"]), "iterations", out["iterations"], flush=True)
+    assert out["iterations"] == it
+    assert rel_err(out["kappa0"], k0, 1e-6) < 1e-10
+    assert rel_err(out["kappa"], k, 1e-6) < 1e-9
+    assert rel_err(out["F"], F, 1e-6) < 1e-8

@@ -8,8 +8,10 @@
 ! (little-endian raw arrays in the reference's column-major layout plus a namelist
 ! `bundle.nml`), because the HDF5 Fortran module is not available to this compiler.
 !
-! usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all]
-!   loop: one spr_lw3 call per q0 (the reference's q0 loop); all: one spr_lw3_all launch
+! usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter]
+!   loop: one spr_lw3 call per q0 (the reference's q0 loop); all: one spr_lw3_all launch;
+!   iter: the iterative solution of src/Kappa/Src_iterative/main.f90:243-361 (tetrahedron delta,
+!   time-reversal irreducible points) instead, printing the kappa of F0 and the converged kappa
 program kappa_gpu
 
     use spring_gpu_m
@@ -27,7 +29,10 @@ program kappa_gpu
     real(dp), allocatable   :: LW_Allq0(:,:), lw4(:,:), Mode_k(:,:), lw(:)
     real(dp)                :: kappa(9)
     type(gpu_ctx)           :: ctx
-    logical                 :: HeaderWrite, loop_mode
+    logical                 :: HeaderWrite, loop_mode, iter_mode
+    integer, allocatable    :: tet(:,:)
+    real(dp)                :: k_iter(9), k_iter0(9)
+    integer                 :: iterNum
     character(len=32)       :: Temp_char, frmt, Ndof_char
     character(len=600)      :: outfile_ascii, outfile_frmt
     integer                 :: Ndof
@@ -36,7 +41,7 @@ program kappa_gpu
 
     call get_command_argument(1, dir)
     if ( len_trim(dir) == 0 ) then
-        write(*, *) 'usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all]'
+        write(*, *) 'usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter]'
         ERROR STOP
     end if
     prec = SPR_PREC_FP64
@@ -45,6 +50,7 @@ program kappa_gpu
     if ( trim(arg) == 'bf16' ) prec = SPR_PREC_BF16
     call get_command_argument(3, arg)
     loop_mode = trim(arg) == 'loop'
+    iter_mode = trim(arg) == 'iter'
 
     FourPhonon = 0
     Natm4 = 1
@@ -70,6 +76,26 @@ program kappa_gpu
     call rd_r2(trim(dir)//'/inv_omega.bin', OnebyOmega)
     call rd_r2(trim(dir)//'/nbe.bin', nBE)
     call rd_r3(trim(dir)//'/grp_vel.bin', grp_vel)
+
+    if ( iter_mode ) then
+        ! ------------------ iterative BTE (Src_iterative/main.f90:243-361), MAX_ITER = 1000 ------------------
+        allocate( tet(4, 6 * Nq) )
+        call rd_i2(trim(dir)//'/tetra.bin', tet)
+        call gpu_ctx_create(ctx, Nbasis, mass, vol, atmNum3, atmIndx3, fC3, mesh, q_int, irr_grid, pos, &
+                          & mult, Rxyz, Evec, omega, OnebyOmega, nBE, grp_vel, 0, tetrahedrons=tet)
+        call kappa_iter_gpu(ctx, T, 1.0e-6_dp, 1000, k_iter, k_iter0, iterNum)
+        write(*, '(A)') 'Thermal conductivity tensor, relaxation time approximation (W/mK):'
+        write(*, '(3(ES22.14, 2X))') k_iter0(1:3)
+        write(*, '(3(ES22.14, 2X))') k_iter0(4:6)
+        write(*, '(3(ES22.14, 2X))') k_iter0(7:9)
+        write(*, '(A, I0, A)') 'Iterative solution of BTE: ', iterNum, ' iterations'
+        write(*, '(A)') 'Thermal conductivity tensor (W/mK):'
+        write(*, '(3(ES22.14, 2X))') k_iter(1:3)
+        write(*, '(3(ES22.14, 2X))') k_iter(4:6)
+        write(*, '(3(ES22.14, 2X))') k_iter(7:9)
+        call gpu_ctx_destroy(ctx)
+        stop
+    end if
 
     if ( FourPhonon /= 0 ) then
         allocate( atmNum4(Nbasis), atmIndx4(4, Natm4, Nbasis), fC4(3,3,3,3,Natm4,Natm4,Natm4,Nbasis) )

@@ -11,6 +11,9 @@
 !   FourPhononLw_gpu       <- ScatterProbq1q2_per + TetrahedronIntegration4th (Gaussian form),
 !                             FourthOrder_m.f90:42-283, Tetrahedron4th_sep.f90:37-171
 !   kappa_noIter_gpu       <- kappa_noIter / kappa_noIter_symm (ThermalConductivity.f90:36-293)
+!   TetrahedronW_gpu, TetrahedronIter_gpu, kappa_iter_gpu
+!                          <- the iterative solver (src/Kappa/Src_iterative): 1/tau, one F update,
+!                             the self-consistent loop of main.f90:243-361
 ! Index arrays arrive 1-based as in the reference (atmIndx(4,:,:), pos, the irreducible
 ! grid index) and are shifted to the C ABI's 0-based convention here.
 module spring_gpu_m
@@ -129,6 +132,36 @@ module spring_gpu_m
             real(c_double)          :: kappa(9), mode_k(*)
         end function spr_kappa_rta
 
+        integer(c_int) function spr_bte_irr(h, nirr, irr_grid, deg) bind(C, name="spr_bte_irr")
+            import :: c_int, c_ptr
+            type(c_ptr), value      :: h
+            integer(c_int)          :: nirr
+            type(c_ptr), value      :: irr_grid, deg
+        end function spr_bte_irr
+
+        integer(c_int) function spr_bte_tau_inv(h, inv_tau) bind(C, name="spr_bte_tau_inv")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            real(c_double)          :: inv_tau(*)
+        end function spr_bte_tau_inv
+
+        integer(c_int) function spr_bte_delta_f(h, F, dF) bind(C, name="spr_bte_delta_f")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            real(c_double)          :: F(*), dF(*)
+        end function spr_bte_delta_f
+
+        integer(c_int) function spr_bte_solve(h, T, eps_k, max_iter, kappa, kappa0, iterations, F) &
+                & bind(C, name="spr_bte_solve")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            real(c_double), value   :: T, eps_k
+            integer(c_int), value   :: max_iter
+            real(c_double)          :: kappa(9), kappa0(9)
+            integer(c_int)          :: iterations
+            type(c_ptr), value      :: F
+        end function spr_bte_solve
+
         integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s
@@ -137,7 +170,8 @@ module spring_gpu_m
 
     public :: gpu_ctx_create, gpu_ctx_destroy, ScatterProbAllq1_gpu, CalculateLinewidth_gpu, &
             & LinewidthAllq0_gpu, FourPhononLw_gpu, kappa_noIter_gpu, spr_error_text, &
-            & CalculateLwTetrahedron_gpu
+            & CalculateLwTetrahedron_gpu, IrrQ_tr_gpu, TetrahedronW_gpu, TetrahedronIter_gpu, &
+            & kappa_iter_gpu
 
 contains
 
@@ -294,5 +328,56 @@ contains
         lw = LW_Allq0
         call check(spr_kappa_rta(ctx%h, lw, T, s, kappa, Mode_k), 'kappa_noIter_gpu')
     end subroutine kappa_noIter_gpu
+
+    ! ---- iterative BTE (src/Kappa/Src_iterative) on the time-reversal irreducible points ----
+
+    ! num_irr_q, the 1-based grid index of each irreducible point and the grid points it stands
+    ! for (Qpoints%irr_q_int(4,:), Irr_q_point_m.f90:296-330)
+    subroutine IrrQ_tr_gpu(ctx, num_irr_q, irr_grid, deg)
+        type(gpu_ctx), intent(in)                       :: ctx
+        integer, intent(out)                            :: num_irr_q
+        integer, allocatable, intent(out)               :: irr_grid(:), deg(:)
+        integer(c_int)                                  :: n
+        integer(c_int), allocatable, target             :: g(:), d(:)
+        call check(spr_bte_irr(ctx%h, n, c_null_ptr, c_null_ptr), 'IrrQ_tr_gpu')
+        allocate( g(n), d(n) )
+        call check(spr_bte_irr(ctx%h, n, c_loc(g), c_loc(d)), 'IrrQ_tr_gpu')
+        num_irr_q = n
+        irr_grid = g + 1
+        deg = d
+    end subroutine IrrQ_tr_gpu
+
+    ! Inv_tauqs(Ndof, num_irr_q) <- TetrahedronWplus + TetrahedronWminus
+    ! (Quasi_momentum_consrv.f90:701-1208)
+    subroutine TetrahedronW_gpu(ctx, Inv_tauqs)
+        type(gpu_ctx), intent(in)   :: ctx
+        real(dp), intent(out)       :: Inv_tauqs(:,:)
+        call check(spr_bte_tau_inv(ctx%h, Inv_tauqs), 'TetrahedronW_gpu')
+    end subroutine TetrahedronW_gpu
+
+    ! dF_new(3, Ndof, num_irr_q) of F_vecAll, before multiplyTaudFnew <- TetrahedronIterPlus +
+    ! TetrahedronIterMinus (TetrahedronDeltaSum.f90:22-620)
+    subroutine TetrahedronIter_gpu(ctx, F_vecAll, dF_new)
+        type(gpu_ctx), intent(in)   :: ctx
+        real(dp), intent(in)        :: F_vecAll(:,:,:)
+        real(dp), intent(out)       :: dF_new(:,:,:)
+        real(dp), allocatable       :: F(:,:,:)
+        F = F_vecAll
+        call check(spr_bte_delta_f(ctx%h, F, dF_new), 'TetrahedronIter_gpu')
+    end subroutine TetrahedronIter_gpu
+
+    ! The sc_iter loop of main.f90:243-361: k_tensor(9) and the kappa of F0 k_tensor0(9) (W/mK),
+    ! iterNum iterations until max|k - k_old| < eps_k or MAX_ITER
+    subroutine kappa_iter_gpu(ctx, T, eps_k, max_iter, k_tensor, k_tensor0, iterNum)
+        type(gpu_ctx), intent(in)   :: ctx
+        real(dp), intent(in)        :: T, eps_k
+        integer, intent(in)         :: max_iter
+        real(dp), intent(out)       :: k_tensor(9), k_tensor0(9)
+        integer, intent(out)        :: iterNum
+        integer(c_int)              :: it
+        call check(spr_bte_solve(ctx%h, T, eps_k, int(max_iter, c_int), k_tensor, k_tensor0, it, c_null_ptr), &
+                 & 'kappa_iter_gpu')
+        iterNum = it
+    end subroutine kappa_iter_gpu
 
 end module spring_gpu_m

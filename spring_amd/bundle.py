@@ -38,6 +38,8 @@ def write_bundle(path, model, qm, ph, T: float, sigma: float, prefix: str = "syn
     put("inv_omega.bin", ph.inv_omega, "<f8")
     put("nbe.bin", ph.nbe, "<f8")
     put("grp_vel.bin", ph.grp_vel, "<f8")
+    if getattr(qm, "tetra", None) is not None:
+        put("tetra.bin", np.asarray(qm.tetra) + 1, "<i4")          # tetrahedrons(4, 6 Nq), 1-based
     natm4 = 1
     if with_fc4:
         f4 = model.fc4

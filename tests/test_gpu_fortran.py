@@ -32,3 +32,28 @@ def test_fortran_driver_parity(rocksalt, tmp_path, mode):
     kap = np.array(nums[-9:]).reshape(3, 3)
     kref = O.kappa_rta_symm(qm, ph, ref, 300.0, rocksalt.crystal.vol)
     assert np.allclose(kap, kref, rtol=1e-10)
+
+
+def test_fortran_driver_iterative_bte(rocksalt, tmp_path):
+    """kappa_gpu ... iter: the iterative BTE through the Fortran module (kappa_iter_gpu)."""
+    from spring_amd import bundle
+    from oracle import oracle as O
+    exe = os.path.join(ROOT, "fortran", "kappa_gpu")
+    assert os.path.exists(exe), "fortran/kappa_gpu is not built"
+    qm, ph = problem(rocksalt, (4, 4, 4))
+    bundle.write_bundle(str(tmp_path / "b"), rocksalt, qm, ph, 300.0, 1.0, prefix="NaClsynth")
+    r = subprocess.run([exe, str(tmp_path / "b"), "fp64", "iter"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    nums = [float(x) for x in re.findall(r"[-+]?\d\.\d+E[-+]\d+", r.stdout)]
+    k0, k = np.array(nums[-18:-9]).reshape(3, 3), np.array(nums[-9:]).reshape(3, 3)
+    inp = O.Inputs(rocksalt, qm, ph)
+    irr, map_irr, map_sign, deg = O.tr_irreducible(qm)
+    W = np.array([O.lw3_gauss(inp, int(g), 1.0, want_w=True)[1] for g in irr])
+    Wp, Wm = O.bte_w_pm(qm, ph, irr, W)
+    kr, k0r, it, _ = O.bte_solve(qm, ph, rocksalt.crystal.vol, 300.0, irr, map_irr, map_sign, deg, Wp, Wm,
+                                 1e-6, 1000)
+    assert rel_err(k0, k0r, 1e-6) < 1e-10
+    assert rel_err(k, kr, 1e-6) < 1e-9
+    assert f"{it} iterations" in r.stdout

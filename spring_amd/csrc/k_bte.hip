@@ -63,12 +63,12 @@ __device__ __forceinline__ int grid_index(int a0, int a1, int a2, const BteArgs 
 }
 
 // Linear tetrahedron weights of the iterative code for one process: the four vertex values M
-// are sorted (stable, with their weights X of K components), then g sum_v I_v X_v / 6 is added
-// to acc when Mqs lies strictly inside a case and the I_v sum to one within kZeroPrec.
-template <int K>
-__device__ __forceinline__ void tetra_acc(double mq, const double *Min, const double (*Xin)[K], double mul,
-                                         double *acc) {
-    int idx[4] = {0, 1, 2, 3};
+// are sorted (stable: idx[k] is the original vertex of sorted position k), then g and the vertex
+// weights I (sorted order) of the case that holds Mqs strictly inside; false when no case holds
+// or the I_v do not sum to one within kZeroPrec (the tetrahedron is skipped then).
+__device__ __forceinline__ bool tetra_gI(double mq, const double *Min, int *idx, double &g, double *I) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) idx[v] = v;
 #pragma unroll
     for (int a = 1; a < 4; ++a)
 #pragma unroll
@@ -81,7 +81,6 @@ __device__ __forceinline__ void tetra_acc(double mq, const double *Min, const do
     double M[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) M[v] = Min[idx[v]];
-    double g, I[4];
     if (mq > M[0] && mq < M[1]) {
         g = 3.0 * (mq - M[0]) * (mq - M[0]) / ((M[1] - M[0]) * (M[2] - M[0]) * (M[3] - M[0]));
         I[0] = ((mq - M[1]) / (M[0] - M[1]) + (mq - M[2]) / (M[0] - M[2]) + (mq - M[3]) / (M[0] - M[3])) / 3.0;
@@ -107,9 +106,18 @@ __device__ __forceinline__ void tetra_acc(double mq, const double *Min, const do
         I[2] = ((mq - M[3]) / (M[2] - M[3])) / 3.0;
         I[3] = ((mq - M[0]) / (M[3] - M[0]) + (mq - M[1]) / (M[3] - M[1]) + (mq - M[2]) / (M[3] - M[2])) / 3.0;
     } else {
-        return;
+        return false;
     }
-    if (!(fabs(I[0] + I[1] + I[2] + I[3] - 1.0) < kZeroPrec)) return;
+    return fabs(I[0] + I[1] + I[2] + I[3] - 1.0) < kZeroPrec;
+}
+
+// acc[k] += mul g/6 sum_v I_v X_v[k] for one process (X in the original vertex order)
+template <int K>
+__device__ __forceinline__ void tetra_acc(double mq, const double *Min, const double (*Xin)[K], double mul,
+                                         double *acc) {
+    int idx[4];
+    double g, I[4];
+    if (!tetra_gI(mq, Min, idx, g, I)) return;
     const double f = mul * g / 6.0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -187,6 +195,130 @@ __global__ __launch_bounds__(kBteBlock) void k_bte(BteArgs A) {
     }
     if (threadIdx.x < K)
         A.partial[(((size_t)qi * nd + s) * A.nch + ch) * K + threadIdx.x] = s_red[threadIdx.x][0];
+}
+
+// Vertex coefficients of one process for the collision matrix: coef[v] = mul g I_v W_v / 6 in
+// the original vertex order; all zero when the tetrahedron is skipped (weights of tetra_gI).
+__device__ __forceinline__ void tetra_coef(double mq, const double *Min, const double *Win, double mul,
+                                           double *coef) {
+    int idx[4];
+    double g, I[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) coef[v] = 0.0;
+    if (!tetra_gI(mq, Min, idx, g, I)) return;
+    const double f = mul * g / 6.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) coef[idx[v]] = f * I[v] * Win[idx[v]];
+}
+
+// Collision matrix row r = (qi, s) of dF = A F: the + process adds c to column (q + q1, s2) and
+// -c to (q1, s1), the - process adds c to (q - q1, s2) and to (q1, s1), each with the sign of
+// the F map (F(-q) = -F(q)); the row is accumulated in LDS.  Also 1/tau(qi, s) = sum of c.
+__global__ __launch_bounds__(kBteBlock) void k_bte_build(BteArgs A, int R, double *Amat, double *itau) {
+    extern __shared__ double s_row[];
+    __shared__ double s_red[kBteBlock];
+    const int s = blockIdx.x, qi = blockIdx.y;
+    const int nd = A.nd, nd2 = nd * nd;
+    const int q = A.irr_grid[qi];
+    const int4 kq = A.coord[q];
+    const double mq = A.omega[(size_t)q * nd + s];
+    const double *Wq = A.W + (size_t)qi * A.nq * nd * nd2;
+    for (int i = threadIdx.x; i < R; i += kBteBlock) s_row[i] = 0.0;
+    __syncthreads();
+    double tau = 0.0;
+    for (int i = threadIdx.x; i < A.ntet * nd2; i += kBteBlock) {
+        const int t = i / nd2, p = i % nd2;
+        const int s1 = p / nd, s2 = p % nd;
+        double Mp[4], Mm[4], Wp[4], Wm[4];
+        int c1[4], cp[4], cm[4];
+        double g1[4], gp[4], gm[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int q1 = A.tetra[4 * t + v];
+            const int4 k1 = A.coord[q1];
+            const int qp = grid_index(kq.x + k1.x, kq.y + k1.y, kq.z + k1.z, A);
+            const int qq2 = grid_index(-kq.x - k1.x, -kq.y - k1.y, -kq.z - k1.z, A);
+            const int nq1 = grid_index(-k1.x, -k1.y, -k1.z, A);
+            const int qm = grid_index(kq.x - k1.x, kq.y - k1.y, kq.z - k1.z, A);
+            const int mq2 = grid_index(k1.x - kq.x, k1.y - kq.y, k1.z - kq.z, A);
+            const double w1 = A.omega[(size_t)q1 * nd + s1];
+            Wp[v] = Wq[((size_t)q1 * nd + s) * nd2 + p] * A.scale *
+                    (A.nbe[(size_t)q1 * nd + s1] - A.nbe[(size_t)qq2 * nd + s2]);
+            Mp[v] = A.omega[(size_t)qp * nd + s2] - w1;
+            Wm[v] = Wq[((size_t)nq1 * nd + s) * nd2 + p] * A.scale *
+                    (1.0 + A.nbe[(size_t)nq1 * nd + s1] + A.nbe[(size_t)mq2 * nd + s2]);
+            Mm[v] = w1 + A.omega[(size_t)qm * nd + s2];
+            const int2 m1 = A.map[q1], mp = A.map[qp], mm = A.map[qm];
+            c1[v] = m1.x * nd + s1;
+            g1[v] = m1.y;
+            cp[v] = mp.x * nd + s2;
+            gp[v] = mp.y;
+            cm[v] = mm.x * nd + s2;
+            gm[v] = mm.y;
+        }
+        double kp[4], km[4];
+        tetra_coef(mq, Mp, Wp, 1.0, kp);
+        tetra_coef(mq, Mm, Wm, 0.5, km);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            tau += kp[v] + km[v];
+            if (kp[v] != 0.0) {
+                atomicAdd(&s_row[cp[v]], gp[v] * kp[v]);
+                atomicAdd(&s_row[c1[v]], -g1[v] * kp[v]);
+            }
+            if (km[v] != 0.0) {
+                atomicAdd(&s_row[cm[v]], gm[v] * km[v]);
+                atomicAdd(&s_row[c1[v]], g1[v] * km[v]);
+            }
+        }
+    }
+    s_red[threadIdx.x] = tau;
+    __syncthreads();
+    const size_t row = (size_t)qi * nd + s;
+    for (int i = threadIdx.x; i < R; i += kBteBlock) Amat[row * R + i] = s_row[i];
+    for (int w = kBteBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) itau[row] = s_red[0];
+}
+
+// One F update on the collision matrix: a wave per four consecutive rows (each F value loaded
+// once for the four), dF = sum_c A[r][c] F[c] in a fixed order per lane then a fixed tree,
+// F_new[r] = F0[r] + dF / (1/tau[r]) where `divide` is set (multiplyTaudFnew), else + dF.
+constexpr int kGemvRows = 4;
+
+__global__ void k_bte_gemv(const double *Amat, const double *F, const double *F0, const double *itau,
+                           const unsigned char *divide, int R, double *Fnew) {
+    const int wv = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+    const int r0 = wv * kGemvRows;
+    if (r0 >= R) return;   // whole waves
+    const int nr = min(kGemvRows, R - r0);
+    double acc[kGemvRows][3];
+#pragma unroll
+    for (int j = 0; j < kGemvRows; ++j) acc[j][0] = acc[j][1] = acc[j][2] = 0.0;
+    for (int c = lane; c < R; c += 64) {
+        const double f0 = F[3 * c], f1 = F[3 * c + 1], f2 = F[3 * c + 2];
+#pragma unroll
+        for (int j = 0; j < kGemvRows; ++j) {
+            const double a = j < nr ? Amat[(size_t)(r0 + j) * R + c] : 0.0;
+            acc[j][0] = fma(a, f0, acc[j][0]);
+            acc[j][1] = fma(a, f1, acc[j][1]);
+            acc[j][2] = fma(a, f2, acc[j][2]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kGemvRows; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc[j][k] += __shfl_down(acc[j][k], off, 64);
+    if (lane == 0)
+        for (int j = 0; j < nr; ++j) {
+            const int r = r0 + j;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) Fnew[3 * r + k] = F0[3 * r + k] + (divide[r] ? acc[j][k] / itau[r] : acc[j][k]);
+        }
 }
 
 // out[(qi, s)][k] = sum over chunks in order
@@ -316,6 +448,74 @@ int bte_sums(Ctx &c, int mode) {
     return SPR_OK;
 }
 
+// Collision matrix A (rows and columns (irreducible q, s)) and 1/tau in one pass over the
+// tetrahedra; SPR_E_UNSUPPORTED (no error text) when a row does not fit in LDS or A exceeds the
+// memory budget, and the caller keeps the per-iteration tetrahedron sums.
+int bte_build(Ctx &c) {
+    BteState &b = c.bte;
+    if (b.a_ready) return SPR_OK;
+    const int R = b.nirr * c.nd;
+    const size_t lds = (size_t)R * sizeof(double);
+    if (lds > 150 * 1024 || (size_t)R * R * sizeof(double) > ((size_t)32 << 30)) return SPR_E_UNSUPPORTED;
+    BteArgs A;
+    A.W = b.d_W;
+    A.tetra = b.d_tetra;
+    A.coord = b.d_coord;
+    A.irr_grid = b.d_irr;
+    A.map = b.d_map;
+    A.omega = c.omega;
+    A.nbe = c.nbe;
+    A.F = nullptr;
+    A.nq = c.nq;
+    A.nd = c.nd;
+    A.m0 = c.mesh[0];
+    A.m1 = c.mesh[1];
+    A.m2 = c.mesh[2];
+    A.ntet = 6 * c.nq;
+    A.scale = 4.0;
+    A.chunk = A.ntet;
+    A.nch = 1;
+    A.partial = nullptr;
+    if (hipMalloc((void **)&b.d_A, (size_t)R * R * sizeof(double)) != hipSuccess) {
+        set_error("BTE: hipMalloc of the collision matrix failed");
+        return SPR_E_OOM;
+    }
+    SPR_HIP(hipMalloc((void **)&b.d_itau, (size_t)R * sizeof(double)));
+    SPR_HIP(hipMalloc((void **)&b.d_F0, (size_t)R * 3 * sizeof(double)));
+    SPR_HIP(hipMalloc((void **)&b.d_Fb, (size_t)R * 3 * sizeof(double)));
+    SPR_HIP(hipMalloc((void **)&b.d_div, (size_t)R));
+    SPR_HIP(hipFuncSetAttribute((const void *)k_bte_build, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    SPR_HIP(hipEventRecord(c.ev0, c.stream));
+    hipLaunchKernelGGL(k_bte_build, dim3(c.nd, b.nirr), dim3(kBteBlock), lds, c.stream, A, R, b.d_A, b.d_itau);
+    SPR_HIP(hipGetLastError());
+    SPR_HIP(hipEventRecord(c.ev1, c.stream));
+    SPR_HIP(hipEventSynchronize(c.ev1));
+    float ms = 0.f;
+    SPR_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    b.build_ms = ms;
+    b.a_ready = true;
+    return SPR_OK;
+}
+
+// F_new = F0 + A F_cur / (1/tau) on the device (one launch)
+int bte_gemv(Ctx &c, const double *d_F, double *d_Fnew) {
+    BteState &b = c.bte;
+    const int R = b.nirr * c.nd;
+    const int waves = (R + kGemvRows - 1) / kGemvRows;
+    const int blocks = (waves + 3) / 4;   // four waves per block
+    SPR_HIP(hipEventRecord(c.ev0, c.stream));
+    hipLaunchKernelGGL(k_bte_gemv, dim3(blocks), dim3(256), 0, c.stream, b.d_A, d_F, b.d_F0, b.d_itau, b.d_div, R,
+                       d_Fnew);
+    SPR_HIP(hipGetLastError());
+    SPR_HIP(hipEventRecord(c.ev1, c.stream));
+    SPR_HIP(hipEventSynchronize(c.ev1));
+    float ms = 0.f;
+    SPR_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    c.launches += 1;
+    c.kernel_ms += ms;
+    return SPR_OK;
+}
+
 }  // namespace spr
 
 using spr::Ctx;
@@ -351,6 +551,12 @@ extern "C" int spr_bte_tau_inv(spr_ctx *h, double *inv_tau) {
         return SPR_E_ARG;
     }
     Ctx &c = h->c;
+    rc = spr::bte_build(c);
+    if (rc == SPR_OK) {
+        SPR_HIP(hipMemcpy(inv_tau, c.bte.d_itau, (size_t)c.bte.nirr * c.nd * sizeof(double), hipMemcpyDeviceToHost));
+        return SPR_OK;
+    }
+    if (rc != SPR_E_UNSUPPORTED) return rc;
     if ((rc = spr::bte_sums(c, 0))) return rc;
     SPR_HIP(hipMemcpy(inv_tau, c.bte.d_out, (size_t)c.bte.nirr * c.nd * sizeof(double), hipMemcpyDeviceToHost));
     return SPR_OK;
@@ -404,9 +610,18 @@ extern "C" int spr_bte_solve(spr_ctx *h, double T, double eps_k, int max_iter, d
     spr::BteState &b = c.bte;
     const int nd = c.nd;
     const size_t nrow = (size_t)b.nirr * nd;
-    if ((rc = spr::bte_sums(c, 0))) return rc;
+    // the collision matrix where it fits (F updates are then one matrix-vector product each),
+    // else the tetrahedron sums every iteration, as the reference does
+    rc = spr::bte_build(c);
+    if (rc != SPR_OK && rc != SPR_E_UNSUPPORTED) return rc;
+    const bool matrix = rc == SPR_OK;
     std::vector<double> itau(nrow), F0(nrow * 3, 0.0), F(nrow * 3), dF(nrow * 3);
-    SPR_HIP(hipMemcpy(itau.data(), b.d_out, nrow * sizeof(double), hipMemcpyDeviceToHost));
+    if (matrix) {
+        SPR_HIP(hipMemcpy(itau.data(), b.d_itau, nrow * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        if ((rc = spr::bte_sums(c, 0))) return rc;
+        SPR_HIP(hipMemcpy(itau.data(), b.d_out, nrow * sizeof(double), hipMemcpyDeviceToHost));
+    }
     // CreateFzero (HelperSubroutines.f90:118-186): F0 = hbar/kB w v / (1/tau T^2)
     const double funit = 1.054571817 / 1.380649;   // FunitE_11m_K, Src_iterative/constants.f90:56
     auto skipped = [&](int qi, int s) { return b.h_irr[qi] == 0 && s < 3; };
@@ -423,21 +638,34 @@ extern "C" int spr_bte_solve(spr_ctx *h, double T, double eps_k, int max_iter, d
     double k_old[9], k_new[9];
     bte_kappa(c, F, k_old);
     if (kappa0) std::memcpy(kappa0, k_old, sizeof(k_old));
+    // multiplyTaudFnew: divide by 1/tau where it is nonzero, acoustic branches at Gamma untouched
+    std::vector<unsigned char> divide(nrow);
+    for (int qi = 0; qi < b.nirr; ++qi)
+        for (int s = 0; s < nd; ++s)
+            divide[(size_t)qi * nd + s] =
+                !skipped(qi, s) && std::fabs(itau[(size_t)qi * nd + s]) > spr::kZeroPrec ? 1 : 0;
+    double *d_cur = b.d_F, *d_next = b.d_Fb;
+    if (matrix) {
+        SPR_HIP(hipMemcpy(b.d_F0, F0.data(), F0.size() * sizeof(double), hipMemcpyHostToDevice));
+        SPR_HIP(hipMemcpy(b.d_F, F0.data(), F0.size() * sizeof(double), hipMemcpyHostToDevice));
+        SPR_HIP(hipMemcpy(b.d_div, divide.data(), nrow, hipMemcpyHostToDevice));
+    }
     int it = 0;
     std::memcpy(k_new, k_old, sizeof(k_old));
     while (it < max_iter) {
         ++it;
-        SPR_HIP(hipMemcpy(b.d_F, F.data(), F.size() * sizeof(double), hipMemcpyHostToDevice));
-        if ((rc = spr::bte_sums(c, 1))) return rc;
-        SPR_HIP(hipMemcpy(dF.data(), b.d_out, dF.size() * sizeof(double), hipMemcpyDeviceToHost));
-        // multiplyTaudFnew: divide by 1/tau where it is nonzero, acoustic branches at Gamma untouched
-        for (int qi = 0; qi < b.nirr; ++qi)
-            for (int s = 0; s < nd; ++s) {
-                const double t = itau[(size_t)qi * nd + s];
-                if (skipped(qi, s) || !(std::fabs(t) > spr::kZeroPrec)) continue;
-                for (int a = 0; a < 3; ++a) dF[((size_t)qi * nd + s) * 3 + a] /= t;
-            }
-        for (size_t i = 0; i < F.size(); ++i) F[i] = F0[i] + dF[i];
+        if (matrix) {
+            if ((rc = spr::bte_gemv(c, d_cur, d_next))) return rc;
+            SPR_HIP(hipMemcpy(F.data(), d_next, F.size() * sizeof(double), hipMemcpyDeviceToHost));
+            std::swap(d_cur, d_next);
+        } else {
+            SPR_HIP(hipMemcpy(b.d_F, F.data(), F.size() * sizeof(double), hipMemcpyHostToDevice));
+            if ((rc = spr::bte_sums(c, 1))) return rc;
+            SPR_HIP(hipMemcpy(dF.data(), b.d_out, dF.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t r = 0; r < nrow; ++r)
+                for (int a = 0; a < 3; ++a)
+                    F[r * 3 + a] = F0[r * 3 + a] + (divide[r] ? dF[r * 3 + a] / itau[r] : dF[r * 3 + a]);
+        }
         bte_kappa(c, F, k_new);
         double diff = 0.0;
         for (int i = 0; i < 9; ++i) diff = std::max(diff, std::fabs(k_new[i] - k_old[i]));
@@ -449,5 +677,3 @@ extern "C" int spr_bte_solve(spr_ctx *h, double T, double eps_k, int max_iter, d
     if (F_out) std::memcpy(F_out, F.data(), F.size() * sizeof(double));
     return SPR_OK;
 }
-
-extern "C" double spr_bte_zero_prec(void) { return spr::kZeroPrec; }

@@ -301,7 +301,9 @@ extern "C" int spr_ctx_destroy(spr_ctx *h) {
                     (void *)c.ipartial, (void *)c.wout, (void *)c.tetra, (void *)c.partner, (void *)c.tout,
                     (void *)c.k_terms, (void *)c.k_io, (void *)c.k_rot, (void *)c.k_pos_full, (void *)c.k_pos_irr,
                     (void *)c.k_mult, (void *)c.bte.d_coord, (void *)c.bte.d_map, (void *)c.bte.d_irr,
-                    (void *)c.bte.d_tetra, (void *)c.bte.d_W, (void *)c.bte.d_F, (void *)c.bte.d_out})
+                    (void *)c.bte.d_tetra, (void *)c.bte.d_W, (void *)c.bte.d_F, (void *)c.bte.d_out,
+                    (void *)c.bte.d_A, (void *)c.bte.d_itau, (void *)c.bte.d_F0, (void *)c.bte.d_Fb,
+                    (void *)c.bte.d_div})
         if (p) (void)hipFree(p);
     if (c.ev0) (void)hipEventDestroy(c.ev0);
     if (c.ev1) (void)hipEventDestroy(c.ev1);

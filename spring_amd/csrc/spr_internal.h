@@ -89,6 +89,11 @@ struct BteState {
     int *d_irr = nullptr, *d_tetra = nullptr;
     double *d_W = nullptr;                // (nirr, nq, nd, nd, nd)
     double *d_F = nullptr, *d_out = nullptr;
+    // collision matrix dF = A F over rows / columns (irreducible q, s), built once
+    bool a_ready = false;
+    double *d_A = nullptr, *d_itau = nullptr, *d_F0 = nullptr, *d_Fb = nullptr;
+    unsigned char *d_div = nullptr;
+    double build_ms = 0.0;
 };
 
 struct Ctx {
@@ -157,6 +162,8 @@ int launch_lw4_mm(Ctx &c, int q0_first, int q0_count, double sigma, double *d_lw
 int launch_lw3_generic(Ctx &c, const int *d_q0, int nq0, double sigma, int prec, double *d_lw, double *d_w);
 int bte_setup(Ctx &c);
 int bte_sums(Ctx &c, int mode);
+int bte_build(Ctx &c);
+int bte_gemv(Ctx &c, const double *d_F, double *d_Fnew);
 double lw4_factor(const Ctx &c);
 // d_lw[q][s] = sum over chunks in order of partial[(q, chunk)][s]
 int launch_reduce_chunks(Ctx &c, const double *partial, int q0_count, int nchunks, double *d_lw);

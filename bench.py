@@ -44,6 +44,9 @@ def parse():
     p.add_argument("--budget", type=float, default=0.05, help="learned10: sampled fraction of q1")
     p.add_argument("--pair-prec", default="fp32", choices=["fp64", "fp32", "bf16"],
                    help="learned10: precision of the sampled pair terms")
+    p.add_argument("--head", type=float, default=0.5,
+                   help="learned10: fraction of the budget evaluated exactly on the highest-scored q1 "
+                        "(the '+head' estimators)")
     p.add_argument("--mesh10", type=int, default=20)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
@@ -311,21 +314,31 @@ def learned10(args):
     sig = np.abs(lw_full) > 1e-6 * np.abs(lw_full).max()
     rng = np.random.default_rng(1)
     res = {}
-    for name in ("learned", "physics", "uniform"):
+    scores = {}
+    for name in ("learned", "physics", "uniform", "learned+head", "physics+head"):
+        base = name.split("+")[0]
         t = time.perf_counter()
-        if name == "uniform":
+        if base in scores:
+            S, n_eval, t_score = scores[base]
+        elif base == "uniform":
             S, n_eval = np.ones((qm.nq0, qm.nq)), 0
         else:
-            S, n_eval = Lrn.device_scores(prop if name == "learned" else None, ctx, qis, args.sigma)
-        t_score = time.perf_counter() - t
+            S, n_eval = Lrn.device_scores(prop if base == "learned" else None, ctx, qis, args.sigma)
+        if base not in scores:
+            t_score = time.perf_counter() - t
+            scores[base] = (S, n_eval, t_score)
         t = time.perf_counter()
         q0s, q1s, ws = [], [], []
         for qi in range(qm.nq0):
-            pr = Lrn.mixture(S[qi])
-            pick = rng.choice(qm.nq, size=K, replace=True, p=pr)
-            q0s.append(np.full(K, qi))
+            if name.endswith("+head"):
+                pick, w = Lrn.head_tail_draw(S[qi], K, args.head, rng)
+            else:
+                pr = Lrn.mixture(S[qi])
+                pick = rng.choice(qm.nq, size=K, replace=True, p=pr)
+                w = 1.0 / (K * pr[pick])
+            q0s.append(np.full(pick.size, qi))
             q1s.append(pick)
-            ws.append(1.0 / (K * pr[pick]))
+            ws.append(w)
         q0s, q1s, ws = np.concatenate(q0s), np.concatenate(q1s), np.concatenate(ws)
         t_draw = time.perf_counter() - t
         t = time.perf_counter()

@@ -122,3 +122,20 @@ def test_library_exports_every_declared_symbol():
     for name in sorted(declared):
         assert hasattr(L, name), name
     assert declared <= set(lib.SYMBOLS)
+
+
+def test_head_tail_draw_is_unbiased():
+    """Deterministic head + sampled tail: the weighted sum is an unbiased estimate of the full sum
+    (seeded Monte Carlo mean within 4 standard errors), and a budget covering every q1 is exact."""
+    from spring_amd import learned as L
+    rng = np.random.default_rng(0)
+    S = rng.exponential(size=400)
+    t = S * rng.uniform(0.5, 1.5, size=400)
+    est = []
+    for _ in range(3000):
+        q, w = L.head_tail_draw(S, 40, 0.5, rng)
+        est.append((t[q] * w).sum())
+    est = np.array(est)
+    assert abs(est.mean() - t.sum()) < 4 * est.std() / np.sqrt(est.size)
+    q, w = L.head_tail_draw(S, 400, 1.0, rng)
+    assert np.isclose((t[q] * w).sum(), t.sum(), rtol=1e-12)

@@ -34,12 +34,13 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--workload", default="rocksalt16",
-                   choices=["rocksalt16", "cell10", "learned10", "fourph10", "bte16"],
+                   choices=["rocksalt16", "cell10", "learned10", "fourph10", "bte16", "phonons10"],
                    help="rocksalt16: 2-atom cell, 16^3 mesh (the headline line); cell10: 10-atom "
                         "cell, 20^3 mesh with time reversal, FP64 vs FP32 vs BF16; learned10: the "
                         "learned importance-sampling proposal on the cell10 workload; fourph10: "
                         "four-phonon linewidths, 2-atom cell with FC4, 10^3 mesh; bte16: iterative "
-                        "BTE of the 2-atom cell on a 16^3 mesh (--mesh)")
+                        "BTE of the 2-atom cell on a 16^3 mesh (--mesh); phonons10: harmonic phonons "
+                        "(D(q), eigensolve, group velocities) of the 10-atom cell on its 20^3 mesh")
     p.add_argument("--samples", type=int, default=2_000_000, help="learned10: training samples")
     p.add_argument("--budget", type=float, default=0.05, help="learned10: sampled fraction of q1")
     p.add_argument("--pair-prec", default="fp32", choices=["fp64", "fp32", "bf16"],
@@ -74,6 +75,8 @@ def main():
         return fourph10(args)
     if args.workload == "bte16":
         return bte16(args)
+    if args.workload == "phonons10":
+        return phonons10(args)
     from spring_amd import model, mesh, phonons, lib
 
     t_in0 = time.perf_counter()
@@ -507,6 +510,52 @@ def bte16(args):
                               "unit": "GB/s", "frac": a_bytes / (per_it_ms * 1e-3) / 8e12,
                               "traffic": None, "bytes": a_bytes},
         "kappa_rta_W_mK": np.diag(out["kappa0"]).tolist(), "kappa_iter_W_mK": np.diag(out["kappa"]).tolist(),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+def phonons10(args):
+    """Harmonic phonons of every q of the 10-atom cell's 20^3 mesh (set_Phon_mp / get_phonon):
+    spr_get_phonon on the device against the host LAPACK path (one zheevd-type eigh per q,
+    batched over 16 threads) and the oracle's per-q restatement on a sample."""
+    from spring_amd import model, mesh, phonons, lib
+    m = model.rocksalt10()
+    n = args.mesh10
+    qm = mesh.make_mesh(m.crystal, (n, n, n), time_rev=1)
+    nd = m.crystal.ndof
+    for _ in range(args.warmup):
+        lib.get_phonon(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        g = lib.get_phonon(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
+    dt = (time.perf_counter() - t) / args.steps
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t = time.perf_counter()
+    h = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T, chunk=256, threads=threads)
+    dt_host = time.perf_counter() - t
+    lam_g = np.sign(g.omega) * g.omega ** 2
+    lam_h = np.sign(h.omega) * h.omega ** 2
+    agree = float(np.abs(lam_g - lam_h).max() / np.abs(lam_h).max())
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        k = min(qm.nq, 2000)
+        t = time.perf_counter()
+        O.phonons(m.crystal, m.fc2, qm.q_cart[:k], qm.q_int[:k], args.T)
+        dtc = time.perf_counter() - t
+        cpu = {"value": k / dtc, "unit": "q-points/s", "cores": 1, "kind": "port",
+               "sample": f"first {k} q of the mesh through the oracle's per-q restatement (numpy LAPACK "
+                         f"eigh per q), 1 thread, {dtc:.1f} s"}
+    res = {
+        "metric": "harmonic phonons (D(q), eigenvectors, omega, n, group velocities) per second",
+        "value": qm.nq / dt, "unit": "q-points/s", "higher_is_better": True, "dtype": "f64",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "config": {"workload": f"10-atom cell, {n}^3 mesh ({qm.nq} q, Ndof {nd}), host buffers in and out "
+                               f"(the call includes the upload of FC2 and q and the download of "
+                               f"{qm.nq * nd * nd * 16 / 1e6:.0f} MB of eigenvectors)", "T_K": args.T},
+        "host_lapack_16_threads": {"value": qm.nq / dt_host, "unit": "q-points/s", "cores": threads,
+                                   "eigenvalues_agree": agree},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

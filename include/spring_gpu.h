@@ -146,6 +146,20 @@ int spr_bte_delta_f(spr_ctx *ctx, const double *F, double *dF);
 int spr_bte_solve(spr_ctx *ctx, double T, double eps_k, int max_iter, double *kappa, double *kappa0,
                   int *iterations, double *F_out);
 
+/* Harmonic phonons of nq q points on `device`, replacing get_phonon / DynMatgv_SR
+ * (DynaMat_mod.f90:98-190, 486-709) as set_Phon_mp calls them (phonon_mod.f90:156-315) for a
+ * non-polar crystal: D(q) from FC2 (fc2->order 2, fc(3,3,natm,nb) eV/A^2), eigenvalues and
+ * eigenvectors (ascending, upper triangle of D as zheevd 'U'), dD/dq for the group velocity,
+ * omega, 1/omega and n(T); the three lowest modes of a q with is_gamma[q] != 0 get
+ * v = n = 1/omega = 0.  latvec[9] row-major (R = latvec . cell, Angstrom); q_cart[nq][3] in
+ * 1/Angstrom.  Outputs in the layout of spr_phonons: evec[nq][ndof][ndof] complex (re, im),
+ * evec[q][s][row]; omega, inv_omega, nbe [nq][ndof]; grp_vel[nq][ndof][3].  Ndof <= 32.
+ * Eigenvectors of degenerate modes are any orthonormal basis of their subspace (as with
+ * LAPACK), so per-mode quantities of degenerate modes are basis-dependent. */
+int spr_get_phonon(const spr_crystal *cr, const double *latvec, const spr_ifc *fc2, int nq,
+                   const double *q_cart, const int *is_gamma, double T, int device, double *evec,
+                   double *omega, double *inv_omega, double *nbe, double *grp_vel);
+
 /* Four-phonon Gaussian linewidth sum of irreducible q0 `i0` over all (q1,q2), the processes
  * and weights of TetrahedronIntegration4th (Tetrahedron4th_sep.f90:242-343) with a Gaussian
  * delta; main.f90:660 adds 0.5*lw to the linewidth. */

@@ -318,3 +318,63 @@ def bte_solve(qm, ph, vol, T, irr, map_irr, map_sign, deg, Wp, Wm, eps_k, max_it
         if done:
             break
     return k_new, kappa0, it, F
+
+
+# ---- harmonic phonons -------------------------------------------------------------------
+# constants.f90:40-55 recomputed: EPS is the single-precision literal 1.0E-8 widened to double
+PH_EPS = float(np.float32(1.0e-8))
+PH_WTHZ = float(np.sqrt(1.60217733 * 6.02214076 * 10.0) * 10.0)
+PH_VGMS = PH_WTHZ * 100.0
+PH_BECONV = float(np.sqrt(1.60217733 * 6.02214076 * 10.0) * 6.582119569 * 100.0 / 8.617333262)
+
+
+def phonons(cr, fc2, q_cart, q_int, T):
+    """get_phonon restated for a non-polar crystal (DynMatgv_SR, DynaMat_mod.f90:486-709, as
+    set_Phon_mp calls it, phonon_mod.f90:156-315), term by term:
+      D(q)[(mu,a),(nu,b)] = (sum_N fc2[mu,N,a,b] e^{i q.R_N}) / sqrt(m_mu m_nu)   (:140-180)
+      dD/dq_k              = the same with i R_N,k in every term
+      lambda, e = zheevd('V','U', D) -- numpy.linalg.eigh(UPLO='U') is LAPACK ?heevd, the
+                  routine lapack_gfort.f90:212-265 wraps
+      w = sqrt|lambda|, -w where |lambda| > EPS and lambda < 0                    (:611-624)
+      v = Re <e|dD/dq|e> / (2 w) vg_ms (:667-674), n = 1/(exp(BEConv |w|/T) - 1) (:676),
+      1/omega = 1/(w wTHz) (:678); the first three modes of a Gamma point get v = n = 1/omega = 0
+      (:629-654).
+    Returns a dict: evec[q][s][row], omega, inv_omega, nbe, grp_vel[q][s][3], and the D of every
+    q (for residual checks)."""
+    q = np.asarray(q_cart, float).reshape(-1, 3)
+    nq, nb = q.shape[0], cr.nb
+    nd = 3 * nb
+    D = np.zeros((nq, nd, nd), complex)
+    dD = np.zeros((3, nq, nd, nd), complex)
+    for mu in range(nb):
+        for n in range(int(fc2.atm_num[mu])):
+            nu = int(fc2.atm_indx[mu, n, 3])
+            R = np.asarray(cr.latvec, float) @ fc2.atm_indx[mu, n, :3].astype(float)
+            term = np.exp(1j * (q @ R))[:, None, None] * fc2.fc[mu, n]
+            D[:, 3 * mu:3 * mu + 3, 3 * nu:3 * nu + 3] += term
+            for k in range(3):
+                dD[k, :, 3 * mu:3 * mu + 3, 3 * nu:3 * nu + 3] += term * (1j * R[k])
+    m = np.repeat(np.asarray(cr.mass, float), 3)
+    mass_fact = 1.0 / np.sqrt(m[:, None] * m[None, :])
+    D *= mass_fact
+    dD *= mass_fact
+    evec = np.zeros((nq, nd, nd), complex)
+    omega, inv_omega, nbe = (np.zeros((nq, nd)) for _ in range(3))
+    vel = np.zeros((nq, nd, 3))
+    gamma = np.all(np.asarray(q_int).reshape(-1, 3) == 0, axis=1)
+    for i in range(nq):
+        lam, V = np.linalg.eigh(D[i], UPLO='U')
+        for s in range(nd):
+            w = np.sqrt(abs(lam[s]))
+            if abs(lam[s]) > PH_EPS and lam[s] < 0.0:
+                w = -w
+            e = V[:, s]
+            evec[i, s] = e
+            omega[i, s] = w * PH_WTHZ
+            if gamma[i] and s < 3:
+                continue
+            for k in range(3):
+                vel[i, s, k] = np.real(np.vdot(e, dD[k, i] @ e)) / (2.0 * w) * PH_VGMS
+            nbe[i, s] = 1.0 / (np.exp(PH_BECONV * abs(w) / T) - 1.0)
+            inv_omega[i, s] = 1.0 / (w * PH_WTHZ)
+    return dict(evec=evec, omega=omega, inv_omega=inv_omega, nbe=nbe, grp_vel=vel, D=D)

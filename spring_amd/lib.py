@@ -21,7 +21,7 @@ SYMBOLS = [
     "spr_kernel_time_reset", "spr_lw3_pairs", "spr_v3_w_pairs", "spr_mlp_create", "spr_mlp_destroy",
     "spr_mlp_num_params", "spr_mlp_get_params", "spr_mlp_set_params", "spr_mlp_grad",
     "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all", "spr_mlp_score",
-    "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve",
+    "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve", "spr_get_phonon",
 ]
 
 
@@ -249,6 +249,32 @@ class GpuContext:
 
     def kernel_time_reset(self):
         self._check(self.lib.spr_kernel_time_reset(self.h), "spr_kernel_time_reset")
+
+
+def get_phonon(cr, fc2, q_cart, q_int, T: float, device: int = 0):
+    """Phonons of every q on the GPU (spr_get_phonon); the same fields and layout as
+    phonons.solve, which is the host path it replaces."""
+    from .phonons import Phonons
+    lib = load()
+    q_cart = np.ascontiguousarray(q_cart, dtype=np.float64).reshape(-1, 3)
+    nq, nd = q_cart.shape[0], cr.ndof
+    gamma = np.ascontiguousarray(np.all(np.asarray(q_int) == 0, axis=1), dtype=np.int32)
+    mass = np.ascontiguousarray(cr.mass, dtype=np.float64)
+    lat = np.ascontiguousarray(cr.latvec, dtype=np.float64)
+    num = np.ascontiguousarray(fc2.atm_num, dtype=np.int32)
+    indx = np.ascontiguousarray(fc2.atm_indx, dtype=np.int32)
+    fc = np.ascontiguousarray(fc2.fc, dtype=np.float64)
+    c = _Crystal(cr.nb, _ptr(mass), cr.vol)
+    f = _IFC(fc2.order, fc2.natm_max, _ptr(num), _ptr(indx), _ptr(fc))
+    evec = np.zeros((nq, nd, nd), np.complex128)
+    omega, inv_omega, nbe = (np.zeros((nq, nd)) for _ in range(3))
+    vel = np.zeros((nq, nd, 3))
+    rc = lib.spr_get_phonon(ct.byref(c), _ptr(lat), ct.byref(f), ct.c_int(nq), _ptr(q_cart), _ptr(gamma),
+                            ct.c_double(T), ct.c_int(device), _ptr(evec), _ptr(omega), _ptr(inv_omega),
+                            _ptr(nbe), _ptr(vel))
+    if rc != 0:
+        raise SprError(f"spr_get_phonon failed ({rc}): {lib.spr_last_error().decode()}")
+    return Phonons(evec, omega, inv_omega, nbe, vel, float(T))
 
 
 class Mlp:

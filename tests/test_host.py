@@ -124,6 +124,17 @@ def test_library_exports_every_declared_symbol():
     assert declared <= set(lib.SYMBOLS)
 
 
+def test_get_phonon_rejects_bad_arguments_on_the_host():
+    """Argument checks of spr_get_phonon run before any device call (no GPU needed)."""
+    from spring_amd.lib import SprError, get_phonon
+    m = model.rocksalt_chain(8)                                 # Ndof = 48 > 32
+    with pytest.raises(SprError, match="Ndof"):
+        get_phonon(m.crystal, m.fc2, np.zeros((1, 3)), np.zeros((1, 3), int), 300.0)
+    m = model.rocksalt2()
+    with pytest.raises(SprError, match="bad argument"):
+        get_phonon(m.crystal, m.fc2, np.zeros((1, 3)), np.zeros((1, 3), int), -1.0)
+
+
 def test_head_tail_draw_is_unbiased():
     """Deterministic head + sampled tail: the weighted sum is an unbiased estimate of the full sum
     (seeded Monte Carlo mean within 4 standard errors), and a budget covering every q1 is exact."""

@@ -8,10 +8,12 @@
 ! (little-endian raw arrays in the reference's column-major layout plus a namelist
 ! `bundle.nml`), because the HDF5 Fortran module is not available to this compiler.
 !
-! usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter]
+! usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter] [gpuph]
 !   loop: one spr_lw3 call per q0 (the reference's q0 loop); all: one spr_lw3_all launch;
 !   iter: the iterative solution of src/Kappa/Src_iterative/main.f90:243-361 (tetrahedron delta,
-!   time-reversal irreducible points) instead, printing the kappa of F0 and the converged kappa
+!   time-reversal irreducible points) instead, printing the kappa of F0 and the converged kappa;
+!   gpuph: the phonons of the mesh are computed on the device from FC2 (get_phonon_gpu, as
+!   set_Phon_mp does with get_phonon, main.f90:279-300) instead of read from the bundle
 program kappa_gpu
 
     use spring_gpu_m
@@ -19,7 +21,7 @@ program kappa_gpu
 
     character(len=512)      :: dir, arg
     character(len=64)       :: prefix
-    integer                 :: Nbasis, Natm3, Natm4, mesh(3), Nq, Nq0, Nsym, FourPhonon, prec, i0, u
+    integer                 :: Nbasis, Natm2, Natm3, Natm4, mesh(3), Nq, Nq0, Nsym, FourPhonon, prec, i0, u
     real(dp)                :: T, Sigma, vol
     integer, allocatable    :: atmNum3(:), atmIndx3(:,:,:), atmNum4(:), atmIndx4(:,:,:)
     integer, allocatable    :: q_int(:,:), irr_grid(:), pos(:), mult(:)
@@ -29,7 +31,10 @@ program kappa_gpu
     real(dp), allocatable   :: LW_Allq0(:,:), lw4(:,:), Mode_k(:,:), lw(:)
     real(dp)                :: kappa(9)
     type(gpu_ctx)           :: ctx
-    logical                 :: HeaderWrite, loop_mode, iter_mode
+    logical                 :: HeaderWrite, loop_mode, iter_mode, gpu_phonons
+    integer, allocatable    :: atmNum2(:), atmIndx2(:,:,:)
+    real(dp), allocatable   :: fC2(:,:,:,:,:), q_cart(:,:)
+    real(dp)                :: latvec(3,3), freq_min, freq_max
     integer, allocatable    :: tet(:,:)
     real(dp)                :: k_iter(9), k_iter0(9)
     integer                 :: iterNum
@@ -37,11 +42,11 @@ program kappa_gpu
     character(len=600)      :: outfile_ascii, outfile_frmt
     integer                 :: Ndof
 
-    namelist /bundle/ prefix, Nbasis, Natm3, Natm4, mesh, Nq, Nq0, Nsym, T, Sigma, vol, FourPhonon
+    namelist /bundle/ prefix, Nbasis, Natm2, Natm3, Natm4, mesh, Nq, Nq0, Nsym, T, Sigma, vol, FourPhonon
 
     call get_command_argument(1, dir)
     if ( len_trim(dir) == 0 ) then
-        write(*, *) 'usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter]'
+        write(*, *) 'usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter] [gpuph]'
         ERROR STOP
     end if
     prec = SPR_PREC_FP64
@@ -51,8 +56,11 @@ program kappa_gpu
     call get_command_argument(3, arg)
     loop_mode = trim(arg) == 'loop'
     iter_mode = trim(arg) == 'iter'
+    call get_command_argument(4, arg)
+    gpu_phonons = trim(arg) == 'gpuph'
 
     FourPhonon = 0
+    Natm2 = 1
     Natm4 = 1
     open(newunit=u, file=trim(dir)//'/bundle.nml', status='old', action='read')
     read(u, nml=bundle)
@@ -71,11 +79,27 @@ program kappa_gpu
     call rd_i(trim(dir)//'/pos.bin', pos)
     call rd_i(trim(dir)//'/multiplicity.bin', mult)
     call rd_r3(trim(dir)//'/rot.bin', Rxyz)
-    call rd_c3(trim(dir)//'/evec.bin', Evec)
-    call rd_r2(trim(dir)//'/omega.bin', omega)
-    call rd_r2(trim(dir)//'/inv_omega.bin', OnebyOmega)
-    call rd_r2(trim(dir)//'/nbe.bin', nBE)
-    call rd_r3(trim(dir)//'/grp_vel.bin', grp_vel)
+    if ( gpu_phonons ) then
+        allocate( atmNum2(Nbasis), atmIndx2(4, Natm2, Nbasis), fC2(1, 3, 3, Natm2, Nbasis), q_cart(3, Nq) )
+        call rd_i(trim(dir)//'/atm_num2.bin', atmNum2)
+        call rd_i3(trim(dir)//'/atm_indx2.bin', atmIndx2)
+        open(newunit=u, file=trim(dir)//'/fc2.bin', access='stream', form='unformatted', status='old')
+        read(u) fC2
+        close(u)
+        open(newunit=u, file=trim(dir)//'/latvec.bin', access='stream', form='unformatted', status='old')
+        read(u) latvec
+        close(u)
+        call rd_r2(trim(dir)//'/q_cart.bin', q_cart)
+        call get_phonon_gpu(Nbasis, mass, latvec, atmNum2, atmIndx2, fC2, q_cart, T, Evec, omega, &
+                          & OnebyOmega, nBE, grp_vel, 0, freq_min, freq_max)
+        write(*, '(A, ES16.8, A, ES16.8)') 'Phonons on the device: freq_min ', freq_min, ' freq_max ', freq_max
+    else
+        call rd_c3(trim(dir)//'/evec.bin', Evec)
+        call rd_r2(trim(dir)//'/omega.bin', omega)
+        call rd_r2(trim(dir)//'/inv_omega.bin', OnebyOmega)
+        call rd_r2(trim(dir)//'/nbe.bin', nBE)
+        call rd_r3(trim(dir)//'/grp_vel.bin', grp_vel)
+    end if
 
     if ( iter_mode ) then
         ! ------------------ iterative BTE (Src_iterative/main.f90:243-361), MAX_ITER = 1000 ------------------

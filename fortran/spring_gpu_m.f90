@@ -14,6 +14,8 @@
 !   TetrahedronW_gpu, TetrahedronIter_gpu, kappa_iter_gpu
 !                          <- the iterative solver (src/Kappa/Src_iterative): 1/tau, one F update,
 !                             the self-consistent loop of main.f90:243-361
+!   get_phonon_gpu         <- DynaMat::get_phonon (src/Kappa/Src/DynaMat_mod.f90:486-709), non-polar:
+!                             Evec, freq, OnebyFreq, nBE, grp_vel of every q, freq_min/freq_max
 ! Index arrays arrive 1-based as in the reference (atmIndx(4,:,:), pos, the irreducible
 ! grid index) and are shifted to the C ABI's 0-based convention here.
 module spring_gpu_m
@@ -162,6 +164,21 @@ module spring_gpu_m
             type(c_ptr), value      :: F
         end function spr_bte_solve
 
+        integer(c_int) function spr_get_phonon(cr, latvec, fc2, nq, q_cart, is_gamma, T, device, evec, &
+                & omega, inv_omega, nbe, grp_vel) bind(C, name="spr_get_phonon")
+            import :: c_int, c_double, spr_crystal, spr_ifc
+            type(spr_crystal), intent(in)   :: cr
+            real(c_double), intent(in)      :: latvec(9)
+            type(spr_ifc), intent(in)       :: fc2
+            integer(c_int), value           :: nq
+            real(c_double), intent(in)      :: q_cart(*)
+            integer(c_int), intent(in)      :: is_gamma(*)
+            real(c_double), value           :: T
+            integer(c_int), value           :: device
+            complex(c_double)               :: evec(*)
+            real(c_double)                  :: omega(*), inv_omega(*), nbe(*), grp_vel(*)
+        end function spr_get_phonon
+
         integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s
@@ -171,7 +188,7 @@ module spring_gpu_m
     public :: gpu_ctx_create, gpu_ctx_destroy, ScatterProbAllq1_gpu, CalculateLinewidth_gpu, &
             & LinewidthAllq0_gpu, FourPhononLw_gpu, kappa_noIter_gpu, spr_error_text, &
             & CalculateLwTetrahedron_gpu, IrrQ_tr_gpu, TetrahedronW_gpu, TetrahedronIter_gpu, &
-            & kappa_iter_gpu
+            & kappa_iter_gpu, get_phonon_gpu
 
 contains
 
@@ -379,5 +396,52 @@ contains
                  & 'kappa_iter_gpu')
         iterNum = it
     end subroutine kappa_iter_gpu
+
+    ! get_phonon (DynaMat_mod.f90:486-709) for a non-polar crystal on the device: latvec(3,3) with
+    ! the lattice vectors as columns (R = matmul(latvec, N)), FC2 as FC2type holds it
+    ! (atmNum2, atmIndx2(4,:,:) with a 1-based basis index, fC2(1,3,3,Natm,Nb), only fC2(1,...)
+    ! used), qpoints(3,Nq) Cartesian.  A q with all |q_k| < cmp_prec (1e-8) is Gamma: its first
+    ! three modes get grp_vel = nBE = OnebyFreq = 0, and freq_min/freq_max are set from it as
+    ! in :629-641 (left 0 when no q is Gamma).
+    subroutine get_phonon_gpu(Nbasis, mass, latvec, atmNum2, atmIndx2, fC2, qpoints, T, &
+                            & Evec, freq, OnebyFreq, nBE, grp_vel, device, freq_min, freq_max)
+        integer, intent(in)                     :: Nbasis, device
+        real(dp), intent(in)                    :: mass(:), latvec(3,3), fC2(:,:,:,:,:), qpoints(:,:), T
+        integer, intent(in)                     :: atmNum2(:), atmIndx2(:,:,:)
+        complex(dp), intent(out)                :: Evec(:,:,:)
+        real(dp), intent(out)                   :: freq(:,:), OnebyFreq(:,:), nBE(:,:), grp_vel(:,:,:)
+        real(dp), intent(out), optional         :: freq_min, freq_max
+        real(dp), parameter                     :: cmp_prec = 1.0e-8_dp
+        integer(c_int), allocatable, target     :: num2(:), indx2(:,:,:), gam(:)
+        real(dp), allocatable, target           :: mass_c(:), fc(:,:,:,:)
+        real(c_double)                          :: lat_c(9)
+        type(spr_crystal)                       :: cr
+        type(spr_ifc)                           :: f2
+        integer                                 :: nq, i
+
+        nq = size(qpoints, 2)
+        num2 = atmNum2
+        indx2 = atmIndx2
+        indx2(4, :, :) = indx2(4, :, :) - 1
+        fc = fC2(1, :, :, :, :)
+        mass_c = mass
+        lat_c = reshape(transpose(latvec), [9])       ! row-major latvec[i][j] = latvec(i,j)
+        allocate( gam(nq) )
+        do i = 1, nq
+            gam(i) = merge(1_c_int, 0_c_int, all(dabs(qpoints(:, i)) < cmp_prec))
+        end do
+        cr = spr_crystal(int(Nbasis, c_int), c_loc(mass_c), 0.0_dp)
+        f2 = spr_ifc(2, int(size(atmIndx2, 2), c_int), c_loc(num2), c_loc(indx2), c_loc(fc))
+        call check(spr_get_phonon(cr, lat_c, f2, int(nq, c_int), qpoints, gam, T, int(device, c_int), Evec, &
+                 & freq, OnebyFreq, nBE, grp_vel), 'get_phonon_gpu')
+        if ( present(freq_min) ) freq_min = 0.0_dp
+        if ( present(freq_max) ) freq_max = 0.0_dp
+        do i = 1, nq
+            if ( gam(i) /= 0 ) then
+                if ( present(freq_max) ) freq_max = maxval(freq(:, i))
+                if ( present(freq_min) ) freq_min = 1.05_dp * dabs(freq(3, i))
+            end if
+        end do
+    end subroutine get_phonon_gpu
 
 end module spring_gpu_m

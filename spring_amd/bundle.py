@@ -38,6 +38,16 @@ def write_bundle(path, model, qm, ph, T: float, sigma: float, prefix: str = "syn
     put("inv_omega.bin", ph.inv_omega, "<f8")
     put("nbe.bin", ph.nbe, "<f8")
     put("grp_vel.bin", ph.grp_vel, "<f8")
+    # harmonic inputs of get_phonon: FC2type (atmIndx2 1-based, fC2(1,3,3,Natm2,Nb)), the lattice
+    # vectors as the columns of latvec(3,3), Cartesian q points (3, Nq)
+    f2 = model.fc2
+    idx2 = f2.atm_indx.copy()
+    idx2[..., 3] += 1
+    put("atm_num2.bin", f2.atm_num, "<i4")
+    put("atm_indx2.bin", idx2, "<i4")
+    put("fc2.bin", f2.fc, "<f8")
+    put("latvec.bin", np.asarray(cr.latvec).T, "<f8")
+    put("q_cart.bin", qm.q_cart, "<f8")
     if getattr(qm, "tetra", None) is not None:
         put("tetra.bin", np.asarray(qm.tetra) + 1, "<i4")          # tetrahedrons(4, 6 Nq), 1-based
     natm4 = 1
@@ -51,7 +61,7 @@ def write_bundle(path, model, qm, ph, T: float, sigma: float, prefix: str = "syn
         natm4 = f4.natm_max
     with open(os.path.join(path, "bundle.nml"), "w") as fh:
         fh.write("&bundle\n")
-        fh.write(f"  prefix = '{prefix}'\n  Nbasis = {cr.nb}\n  Natm3 = {f3.natm_max}\n")
+        fh.write(f"  prefix = '{prefix}'\n  Nbasis = {cr.nb}\n  Natm2 = {f2.natm_max}\n  Natm3 = {f3.natm_max}\n")
         fh.write(f"  Natm4 = {natm4}\n  mesh = {int(qm.mesh[0])}, {int(qm.mesh[1])}, {int(qm.mesh[2])}\n")
         fh.write(f"  Nq = {qm.nq}\n  Nq0 = {qm.nq0}\n  Nsym = {rot.shape[0]}\n")
         fh.write(f"  T = {T!r}\n  Sigma = {sigma!r}\n  vol = {cr.vol!r}\n  FourPhonon = {int(with_fc4)}\n/\n")

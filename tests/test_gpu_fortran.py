@@ -57,3 +57,38 @@ def test_fortran_driver_iterative_bte(rocksalt, tmp_path):
     assert rel_err(k0, k0r, 1e-6) < 1e-10
     assert rel_err(k, kr, 1e-6) < 1e-9
     assert f"{it} iterations" in r.stdout
+
+
+def test_fortran_driver_device_phonons(rocksalt, tmp_path):
+    """kappa_gpu ... all gpuph: the phonons come from get_phonon_gpu (FC2 on the device) instead
+    of the bundle.  Frequencies equal the oracle's; linewidths summed over each degenerate set of
+    q0 modes equal the oracle's (the split inside a set follows the eigenvector basis)."""
+    from spring_amd import bundle
+    from oracle import oracle as O
+    from tests.test_phonon_oracle import degenerate_sets
+    exe = os.path.join(ROOT, "fortran", "kappa_gpu")
+    assert os.path.exists(exe), "fortran/kappa_gpu is not built"
+    qm, ph = problem(rocksalt, (4, 4, 4))
+    bundle.write_bundle(str(tmp_path / "b"), rocksalt, qm, ph, 300.0, 1.0, prefix="NaClsynth")
+    r = subprocess.run([exe, str(tmp_path / "b"), "fp64", "all", "gpuph"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0 and "Phonons on the device" in r.stdout
+    w, lw = bundle.read_linewidth_file(str(tmp_path / "linewidth_IBZ_NaClsynth300.0K.txt"), 6)
+    o = O.phonons(rocksalt.crystal, rocksalt.fc2, qm.q_cart, qm.q_int, 300.0)
+    irr = qm.irr_idx
+    assert np.allclose(w[1:], o["omega"][irr][1:], rtol=1e-11, atol=0)
+    assert np.allclose(w[0, 3:], o["omega"][0, 3:], rtol=1e-11, atol=0)
+    ref = O.lw3_gauss_all(O.Inputs(rocksalt, qm, ph), irr, 1.0)
+    e = 0.0
+    for k, g in enumerate(irr):
+        for grp in degenerate_sets(o["omega"][g]):
+            if g == 0 and grp[0] < 3:
+                continue
+            e = max(e, abs(lw[k, grp].sum() - ref[k, grp].sum()) / np.abs(ref).max())
+    print(f"degenerate-set linewidth sums from device phonons: {e:.1e}", flush=True)
+    assert e < 1e-10
+    # freq_max = the largest Gamma frequency, freq_min = 1.05 |freq(3, Gamma)| (DynaMat_mod.f90:629-641)
+    fmin, fmax = (float(x) for x in re.search(r"freq_min\s+(\S+)\s+freq_max\s+(\S+)", r.stdout).groups())
+    assert np.isclose(fmax, o["omega"][0].max(), rtol=1e-8)            # printed with 9 digits
+    assert abs(fmin - 1.05 * abs(o["omega"][0, 2])) < 1e-4 * fmax      # acoustic: sqrt of rounding

@@ -11,7 +11,8 @@
 //      is per lane (integer addresses, table lookup in LDS), T_a lives in registers.
 //   2. T_a <- T_a e2 then e1^T T_a (per lane, eigenvectors staged in LDS lane-major).
 //   3. the e0 contraction crosses waves: per s1, every wave writes U(a, s1, :) to LDS and
-//      wave s0 forms V(s0, s1, s2) = sum_a e0(a, s0) U(a, s1, s2).
+//      wave (s2, block of three s0) reads U(:, s1, s2) once and forms V(s0, s1, s2) =
+//      sum_a e0(a, s0) U(a, s1, s2) for its three s0.
 //   4. W = |V|^2 /(w0 w1 w2) Sctr3THz2/Nq and the Gaussian-delta occupation weights
 //      (CalculateLinewidth, ThirdOrder_m.f90:514-537) are accumulated per lane, then
 //      reduced over the wave: partial[item][s0].  A second kernel sums the items of each
@@ -153,39 +154,53 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
         for (int s2 = 0; s2 < ND; ++s2) T[be][s2] = t[s2];
     }
 
-    // ---- 3 + 4. per s1: partial U2 over this wave's rows b -> LDS; wave (s0, s2 half)
-    //      forms V(s0, s1, s2) = sum_{a, nu} e0(a, s0) U2, then W and the delta sum ----
-    const int s0 = w / NB, s2_0 = 3 * (w % NB);
-    const double w0 = A.omega[(size_t)i0 * ND + s0];
-    const double iw0f = A.inv_omega[(size_t)i0 * ND + s0] * A.factor;
+    // ---- 3 + 4. per s1: every wave writes its partial U2(a, nu; s1, s2) = sum_{b of nu} e1(b, s1)
+    //      U1(b, s2) for all s2 to LDS; wave (s2, s0 block of three) then reads the partials of its
+    //      s2 once, sums them against e0(a, s0) for its three s0, and forms W and the delta sum ----
+    const int s2 = w / NB, s0b = 3 * (w % NB);
+    double w0[3], iw0f[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        w0[k] = A.omega[(size_t)i0 * ND + s0b + k];
+        iw0f[k] = A.inv_omega[(size_t)i0 * ND + s0b + k] * A.factor;
+    }
     const double c0 = 1.0 / (A.sigma * 2.5066282746310002);
     const double hinv = 0.5 / (A.sigma * A.sigma);
-    KSum<R> kacc;
-    double dacc = 0.0;
+    KSum<R> kacc[3];
+    double dacc[3] = {0.0, 0.0, 0.0};
     __syncthreads();   // every wave is done with e2 before the exchange buffer overwrites it
+    const double w2 = s_md[(3 * ND + s2) * 64 + lane];
+    const double n2 = s_md[(4 * ND + s2) * 64 + lane];
+    const double iw2 = s_md[(5 * ND + s2) * 64 + lane];
 #pragma unroll 1
     for (int s1 = 0; s1 < ND; ++s1) {
+        Z e1v[3];
 #pragma unroll
-        for (int s2 = 0; s2 < ND; ++s2) {
+        for (int be = 0; be < 3; ++be) e1v[be] = s_e1[(s1 * ND + 3 * nu + be) * 64 + lane];
+#pragma unroll
+        for (int sx = 0; sx < ND; ++sx) {
             Z z = cmk<R>(0, 0);
 #pragma unroll
-            for (int be = 0; be < 3; ++be) z = cfma(s_e1[(s1 * ND + 3 * nu + be) * 64 + lane], T[be][s2], z);
-            s_x[(w * ND + s2) * 64 + lane] = z;
+            for (int be = 0; be < 3; ++be) z = cfma(e1v[be], T[be][sx], z);
+            s_x[(w * ND + sx) * 64 + lane] = z;
         }
         __syncthreads();
         const double w1 = s_md[(0 * ND + s1) * 64 + lane];
         const double n1 = s_md[(1 * ND + s1) * 64 + lane];
-        const double iw01 = iw0f * s_md[(2 * ND + s1) * 64 + lane];
-#pragma unroll 1
-        for (int j = 0; j < 3; ++j) {
-            const int s2 = s2_0 + j;
-            Z v = cmk<R>(0, 0);
+        const double iw1 = s_md[(2 * ND + s1) * 64 + lane];
+        Z v[3] = {cmk<R>(0, 0), cmk<R>(0, 0), cmk<R>(0, 0)};
 #pragma unroll
-            for (int ww = 0; ww < NW; ++ww) v = cfma(s_e0[s0 * ND + ww / NB], s_x[(ww * ND + s2) * 64 + lane], v);
-            const double w2 = s_md[(3 * ND + s2) * 64 + lane];
-            const double n2 = s_md[(4 * ND + s2) * 64 + lane];
-            const double W = (double)cabs2(v) * iw01 * s_md[(5 * ND + s2) * 64 + lane];
-            const double d1 = w0 - w1 - w2, d2 = w0 + w1 - w2, d3 = w0 - w1 + w2;
+        for (int ar = 0; ar < ND; ++ar) {
+            Z u = s_x[((ar * NB) * ND + s2) * 64 + lane];
+#pragma unroll
+            for (int h = 1; h < NB; ++h) u = cadd(u, s_x[((ar * NB + h) * ND + s2) * 64 + lane]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] = cfma(s_e0[(s0b + k) * ND + ar], u, v[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double W = (double)cabs2(v[k]) * (iw0f[k] * iw1) * iw2;
+            const double d1 = w0[k] - w1 - w2, d2 = w0[k] + w1 - w2, d3 = w0[k] - w1 + w2;
             const double t1 = hinv * d1 * d1, t2 = hinv * d2 * d2, t3 = hinv * d3 * d3;
             double g = 0.0;
             // exp(-t) is exactly zero in FP64 beyond t = 745.2: a wave whose lanes are all that far
@@ -195,22 +210,26 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
             if (sizeof(R) == 8 || __ballot(fmin(t1, fmin(t2, t3)) < 746.0))
                 g = (n1 + n2 + 1.0) * (c0 * exp(-t1)) + (n1 - n2) * (c0 * exp(-t2) - c0 * exp(-t3));
             if (sizeof(R) == 8)
-                dacc += wpair * W * g;
+                dacc[k] += wpair * W * g;
             else
-                kacc.add((R)(wpair * W * g));
+                kacc[k].add((R)(wpair * W * g));
         }
         __syncthreads();
     }
-    double tot = sizeof(R) == 8 ? dacc : kacc.value();
-    // wave reduction, then the NB waves sharing s0 in a fixed order
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) tot += __shfl_down(tot, off, 64);
+    // wave reductions, then for each s0 the ND waves (one per s2) sharing its block, in a fixed order
     double *s_red = reinterpret_cast<double *>(s_x);
-    if (lane == 0) s_red[w] = tot;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double tot = sizeof(R) == 8 ? dacc[k] : kacc[k].value();
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) tot += __shfl_down(tot, off, 64);
+        if (lane == 0) s_red[w * 3 + k] = tot;
+    }
     __syncthreads();
     if (threadIdx.x < ND) {
+        const int h = threadIdx.x / 3, k = threadIdx.x % 3;
         double sum = 0.0;
-        for (int h = 0; h < NB; ++h) sum += s_red[threadIdx.x * NB + h];
+        for (int sx = 0; sx < ND; ++sx) sum += s_red[(sx * NB + h) * 3 + k];
         A.ic.partial[(size_t)item * ND + threadIdx.x] = sum;
     }
 }

@@ -14,7 +14,8 @@
 //      wave (s2, block of three s0) reads U(:, s1, s2) once and forms V(s0, s1, s2) =
 //      sum_a e0(a, s0) U(a, s1, s2) for its three s0.
 //   4. W = |V|^2 /(w0 w1 w2) Sctr3THz2/Nq and the Gaussian-delta occupation weights
-//      (CalculateLinewidth, ThirdOrder_m.f90:514-537) are accumulated per lane, then
+//      (CalculateLinewidth, ThirdOrder_m.f90:514-537) are accumulated per lane (a wave skips an
+//      s0 whose three Gaussians underflow to exact zero on every lane), then
 //      reduced over the wave: partial[item][s0].  A second kernel sums the items of each
 //      q0 in a fixed order (bitwise reproducible).
 #include <algorithm>
@@ -55,7 +56,9 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
     constexpr int NW = ND * NB;            // waves: row a x half nu
     extern __shared__ __align__(16) unsigned char smem[];
     // layout: e1[ND2][64] | x[NW][ND][64] (exchange; the e2[ND2][64] staging aliases its start)
-    //         | m1[nsite][64], m2[nsite][64] (int) | md[6 ND][64] (double) | phase[L] | e0[ND2]
+    //         | m1[nsite][64], m2[nsite][64] (int) | md[6 ND][64] (double) | phase[2L] | e0[ND2]
+    // m1 holds the LDS byte address of phase[m1] and m2 the byte offset 16 m2 (8 m2 in FP32):
+    // the sum addresses phase[m1 + m2], and the table holds two periods so that no mod L is needed
     Z *s_e1 = reinterpret_cast<Z *>(smem);
     Z *s_x = s_e1 + ND2 * 64;
     Z *s_e2 = s_x;
@@ -63,7 +66,7 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
     int *s_m2 = s_m1 + A.nsite * 64;
     double *s_md = reinterpret_cast<double *>(s_m2 + A.nsite * 64);
     Z *s_ph = reinterpret_cast<Z *>(s_md + 6 * ND * 64);
-    Z *s_e0 = s_ph + A.L;
+    Z *s_e0 = s_ph + 2 * A.L;
 
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -86,10 +89,11 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
     {
         const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
         const int k20 = q2 % A.m0, k21 = (q2 / A.m0) % A.m1, k22 = q2 / (A.m0 * A.m1);
+        const int ph_base = (int)(reinterpret_cast<const unsigned char *>(s_ph) - smem);
         for (int s = w; s < A.nsite; s += NW) {
             const int4 c = A.site[s];
-            s_m1[s * 64 + lane] = (k10 * c.x + k11 * c.y + k12 * c.z) % A.L;
-            s_m2[s * 64 + lane] = (k20 * c.x + k21 * c.y + k22 * c.z) % A.L;
+            s_m1[s * 64 + lane] = ph_base + (int)sizeof(Z) * ((k10 * c.x + k11 * c.y + k12 * c.z) % A.L);
+            s_m2[s * 64 + lane] = (int)sizeof(Z) * ((k20 * c.x + k21 * c.y + k22 * c.z) % A.L);
         }
     }
     for (int k = w; k < 6 * ND; k += NW) {   // omega, nBE, 1/omega of q1 (k < 3 ND), then of q2
@@ -97,7 +101,7 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
         const double *src = what == 0 ? A.omega : what == 1 ? A.nbe : A.inv_omega;
         s_md[k * 64 + lane] = src[(size_t)qq * ND + s];
     }
-    for (int i = threadIdx.x; i < A.L; i += NW * 64) s_ph[i] = phase[i];
+    for (int i = threadIdx.x; i < 2 * A.L; i += NW * 64) s_ph[i] = phase[i < A.L ? i : i - A.L];
     for (int i = threadIdx.x; i < ND2; i += NW * 64) s_e0[i] = evec[(size_t)i0 * ND2 + i];
     __syncthreads();
 
@@ -115,18 +119,16 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
         const int blk = (mu * NB + nu) * NB + eta;
         const int u0 = blk_start[blk], u1 = blk_start[blk + 1];
         for (int u = u0; u < u1; u += kClusterBatch) {
-            int sa[kClusterBatch], sb[kClusterBatch];
-#pragma unroll
-            for (int k = 0; k < kClusterBatch; ++k) {
-                sa[k] = tup_site[2 * (u + k)];
-                sb[k] = tup_site[2 * (u + k) + 1];
-            }
+            // the batch's four (site 1, site 2) pairs in one scalar load (u is a multiple of four)
+            const int4 *ts = reinterpret_cast<const int4 *>(tup_site + 2 * u);
+            const int4 t01 = ts[0], t23 = ts[1];
+            const int sa[kClusterBatch] = {t01.x, t01.z, t23.x, t23.z};
+            const int sb[kClusterBatch] = {t01.y, t01.w, t23.y, t23.w};
             Z ph[kClusterBatch];
 #pragma unroll
             for (int k = 0; k < kClusterBatch; ++k) {
-                int m = s_m1[sa[k] * 64 + lane] + s_m2[sb[k] * 64 + lane];
-                m -= m >= A.L ? A.L : 0;
-                ph[k] = s_ph[m];
+                const int addr = s_m1[sa[k] * 64 + lane] + s_m2[sb[k] * 64 + lane];
+                ph[k] = *reinterpret_cast<const Z *>(smem + addr);
             }
 #pragma unroll
             for (int k = 0; k < kClusterBatch; ++k) {
@@ -188,27 +190,24 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
         const double w1 = s_md[(0 * ND + s1) * 64 + lane];
         const double n1 = s_md[(1 * ND + s1) * 64 + lane];
         const double iw1 = s_md[(2 * ND + s1) * 64 + lane];
-        Z v[3] = {cmk<R>(0, 0), cmk<R>(0, 0), cmk<R>(0, 0)};
-#pragma unroll
-        for (int ar = 0; ar < ND; ++ar) {
-            Z u = s_x[((ar * NB) * ND + s2) * 64 + lane];
-#pragma unroll
-            for (int h = 1; h < NB; ++h) u = cadd(u, s_x[((ar * NB + h) * ND + s2) * 64 + lane]);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) v[k] = cfma(s_e0[(s0b + k) * ND + ar], u, v[k]);
-        }
+        // exp(-t) is exactly zero in FP64 beyond t = 745.2, so a wave whose lanes are all that far
+        // from resonance for this s0 (consecutive q1 of one q0 have close frequencies, so most
+        // are) adds exact zeros: it skips V and the exponentials, with the same result bit for bit
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double W = (double)cabs2(v[k]) * (iw0f[k] * iw1) * iw2;
             const double d1 = w0[k] - w1 - w2, d2 = w0[k] + w1 - w2, d3 = w0[k] - w1 + w2;
             const double t1 = hinv * d1 * d1, t2 = hinv * d2 * d2, t3 = hinv * d3 * d3;
-            double g = 0.0;
-            // exp(-t) is exactly zero in FP64 beyond t = 745.2: a wave whose lanes are all that far
-            // from resonance (consecutive q1 of one q0 have close frequencies, so most are) skips
-            // the exponentials with the same result.  The FP64 variant keeps the branch-free form:
-            // the branch costs it registers it does not have.
-            if (sizeof(R) == 8 || __ballot(fmin(t1, fmin(t2, t3)) < 746.0))
-                g = (n1 + n2 + 1.0) * (c0 * exp(-t1)) + (n1 - n2) * (c0 * exp(-t2) - c0 * exp(-t3));
+            if (!__ballot(fmin(t1, fmin(t2, t3)) < 746.0)) continue;
+            Z v = cmk<R>(0, 0);
+#pragma unroll
+            for (int ar = 0; ar < ND; ++ar) {
+                Z u = s_x[((ar * NB) * ND + s2) * 64 + lane];
+#pragma unroll
+                for (int h = 1; h < NB; ++h) u = cadd(u, s_x[((ar * NB + h) * ND + s2) * 64 + lane]);
+                v = cfma(s_e0[(s0b + k) * ND + ar], u, v);
+            }
+            const double W = (double)cabs2(v) * (iw0f[k] * iw1) * iw2;
+            const double g = (n1 + n2 + 1.0) * (c0 * exp(-t1)) + (n1 - n2) * (c0 * exp(-t2) - c0 * exp(-t3));
             if (sizeof(R) == 8)
                 dacc[k] += wpair * W * g;
             else
@@ -276,7 +275,7 @@ static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first,
     A.evec = sizeof(R) == 8 ? (const void *)c.evec : (const void *)c.evec32;
     constexpr int NW = ND * NB;
     const size_t dyn = (size_t)(ND * ND + NW * ND) * 64 * sizeof(cplx_t<R>) + (size_t)2 * A.nsite * 64 * sizeof(int) +
-                       (size_t)6 * ND * 64 * sizeof(double) + (size_t)(c.L + ND * ND) * sizeof(cplx_t<R>);
+                       (size_t)6 * ND * 64 * sizeof(double) + (size_t)(2 * c.L + ND * ND) * sizeof(cplx_t<R>);
     if (dyn > 160 * 1024) {
         set_error("small-cell kernel: LDS need exceeds 160 KB (too many neighbour sites)");
         return SPR_E_UNSUPPORTED;

@@ -4,9 +4,12 @@
 ! loop (:552-715) producing LW_Allq0 and the per-q0 lines of linewidth_IBZ_<prefix><T>K.txt
 ! (WriteAsciiOutFile, Helper_m.f90:597-667, same header and G20.12 columns), then the RTA
 ! kappa (kappa_noIter_symm by default, -Symm of parse_cmd_line.f90) printed as in
-! main.f90:799-819.  Inputs come from a directory written by spring_amd.bundle.write_bundle
-! (little-endian raw arrays in the reference's column-major layout plus a namelist
-! `bundle.nml`), because the HDF5 Fortran module is not available to this compiler.
+! main.f90:799-819, and writes kappa_<prefix><T>K.h5 as main.f90:773-798 does.  Inputs come from a
+! directory written by spring_amd.bundle.write_bundle (little-endian raw arrays in the
+! reference's column-major layout plus a namelist `bundle.nml`); the force constants are read
+! from the reference's own files FC2nd_/FC3rd_/FC4th_<prefix><T>K_F.h5 when the directory holds
+! them (through libspring_h5, since the HDF5 Fortran module of this image is not readable by
+! this compiler).
 !
 ! usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter] [gpuph]
 !   loop: one spr_lw3 call per q0 (the reference's q0 loop); all: one spr_lw3_all launch;
@@ -17,6 +20,7 @@
 program kappa_gpu
 
     use spring_gpu_m
+    use spring_h5_m
     implicit none
 
     character(len=512)      :: dir, arg
@@ -41,6 +45,10 @@ program kappa_gpu
     character(len=32)       :: Temp_char, frmt, Ndof_char
     character(len=600)      :: outfile_ascii, outfile_frmt
     integer                 :: Ndof
+    character(len=600)      :: fc2file, fc3file, fc4file, kappa_outFile
+    logical                 :: have_h5
+    integer, allocatable    :: num_h5(:), indx_h5(:,:,:)
+    real(dp), allocatable   :: fc3_h5(:,:,:,:,:,:), fc4_h5(:,:,:,:,:,:,:,:), fc2_h5(:,:,:,:,:)
 
     namelist /bundle/ prefix, Nbasis, Natm2, Natm3, Natm4, mesh, Nq, Nq0, Nsym, T, Sigma, vol, FourPhonon
 
@@ -66,6 +74,11 @@ program kappa_gpu
     read(u, nml=bundle)
     close(u)
     Ndof = 3 * Nbasis
+    write(Temp_char, '(F6.1)') T
+    fc2file = trim(dir)//'/FC2nd_'//trim(prefix)//trim(adjustl(Temp_char))//'K_F.h5'
+    fc3file = trim(dir)//'/FC3rd_'//trim(prefix)//trim(adjustl(Temp_char))//'K_F.h5'
+    fc4file = trim(dir)//'/FC4th_'//trim(prefix)//trim(adjustl(Temp_char))//'K_F.h5'
+    kappa_outFile = 'kappa_'//trim(prefix)//trim(adjustl(Temp_char))//'K.h5'
 
     allocate( mass(Nbasis), atmNum3(Nbasis), atmIndx3(4, Natm3, Nbasis), fC3(3,3,3,Natm3,Natm3,Nbasis) )
     allocate( q_int(3, Nq), irr_grid(Nq0), pos(Nq), mult(Nq0), Rxyz(3, 3, Nsym) )
@@ -73,19 +86,41 @@ program kappa_gpu
     call rd_r(trim(dir)//'/mass.bin', mass)
     call rd_i(trim(dir)//'/atm_num3.bin', atmNum3)
     call rd_i3(trim(dir)//'/atm_indx3.bin', atmIndx3)
-    call rd_r6(trim(dir)//'/fc3.bin', fC3)
+    inquire(file=trim(fc3file), exist=have_h5)
+    if ( have_h5 ) then
+        ! r3rdFC (hdf5_wrap_mod.f90:114-223); the file's Natm and Nb must match the bundle
+        call r3rdFC_h5(trim(fc3file), num_h5, indx_h5, fc3_h5)
+        if ( size(fc3_h5, 4) /= Natm3 .or. size(fc3_h5, 6) /= Nbasis ) then
+            write(*, *) 'FC3 file shape differs from bundle.nml: ', trim(fc3file)
+            ERROR STOP
+        end if
+        atmNum3 = num_h5
+        atmIndx3 = indx_h5
+        fC3 = fc3_h5
+        write(*, '(A, A)') 'Reading Third-Order Force constant file: ', trim(fc3file)
+    else
+        call rd_r6(trim(dir)//'/fc3.bin', fC3)
+    end if
     call rd_i2(trim(dir)//'/q_int.bin', q_int)
     call rd_i(trim(dir)//'/irr_grid.bin', irr_grid)
     call rd_i(trim(dir)//'/pos.bin', pos)
     call rd_i(trim(dir)//'/multiplicity.bin', mult)
     call rd_r3(trim(dir)//'/rot.bin', Rxyz)
     if ( gpu_phonons ) then
-        allocate( atmNum2(Nbasis), atmIndx2(4, Natm2, Nbasis), fC2(1, 3, 3, Natm2, Nbasis), q_cart(3, Nq) )
-        call rd_i(trim(dir)//'/atm_num2.bin', atmNum2)
-        call rd_i3(trim(dir)//'/atm_indx2.bin', atmIndx2)
-        open(newunit=u, file=trim(dir)//'/fc2.bin', access='stream', form='unformatted', status='old')
-        read(u) fC2
-        close(u)
+        inquire(file=trim(fc2file), exist=have_h5)
+        if ( have_h5 ) then
+            call r2ndFC_h5(trim(fc2file), atmNum2, atmIndx2, fc2_h5)   ! r2ndFC, hdf5_wrap_mod.f90:40-111
+            fC2 = fc2_h5
+            allocate( q_cart(3, Nq) )
+            write(*, '(A, A)') 'Reading Second-Order Force constant file: ', trim(fc2file)
+        else
+            allocate( atmNum2(Nbasis), atmIndx2(4, Natm2, Nbasis), fC2(1, 3, 3, Natm2, Nbasis), q_cart(3, Nq) )
+            call rd_i(trim(dir)//'/atm_num2.bin', atmNum2)
+            call rd_i3(trim(dir)//'/atm_indx2.bin', atmIndx2)
+            open(newunit=u, file=trim(dir)//'/fc2.bin', access='stream', form='unformatted', status='old')
+            read(u) fC2
+            close(u)
+        end if
         open(newunit=u, file=trim(dir)//'/latvec.bin', access='stream', form='unformatted', status='old')
         read(u) latvec
         close(u)
@@ -122,12 +157,19 @@ program kappa_gpu
     end if
 
     if ( FourPhonon /= 0 ) then
-        allocate( atmNum4(Nbasis), atmIndx4(4, Natm4, Nbasis), fC4(3,3,3,3,Natm4,Natm4,Natm4,Nbasis) )
-        call rd_i(trim(dir)//'/atm_num4.bin', atmNum4)
-        call rd_i3(trim(dir)//'/atm_indx4.bin', atmIndx4)
-        open(newunit=u, file=trim(dir)//'/fc4.bin', access='stream', form='unformatted', status='old')
-        read(u) fC4
-        close(u)
+        inquire(file=trim(fc4file), exist=have_h5)
+        if ( have_h5 ) then
+            call r4thFC_h5(trim(fc4file), atmNum4, atmIndx4, fc4_h5)   ! r4thFC, hdf5_wrap_mod.f90:226-337
+            fC4 = fc4_h5
+            write(*, '(A, A)') 'Reading Fourth-Order Force constant file: ', trim(fc4file)
+        else
+            allocate( atmNum4(Nbasis), atmIndx4(4, Natm4, Nbasis), fC4(3,3,3,3,Natm4,Natm4,Natm4,Nbasis) )
+            call rd_i(trim(dir)//'/atm_num4.bin', atmNum4)
+            call rd_i3(trim(dir)//'/atm_indx4.bin', atmIndx4)
+            open(newunit=u, file=trim(dir)//'/fc4.bin', access='stream', form='unformatted', status='old')
+            read(u) fC4
+            close(u)
+        end if
         call gpu_ctx_create(ctx, Nbasis, mass, vol, atmNum3, atmIndx3, fC3, mesh, q_int, irr_grid, pos, &
                           & mult, Rxyz, Evec, omega, OnebyOmega, nBE, grp_vel, 0, atmNum4, atmIndx4, fC4)
     else
@@ -151,7 +193,6 @@ program kappa_gpu
         LW_Allq0 = LW_Allq0 + 0.5_dp * lw4      ! main.f90:660
     end if
 
-    write(Temp_char, '(F6.1)') T
     outfile_ascii = 'linewidth_IBZ_'//trim(prefix)//trim(adjustl(Temp_char))//'K.txt'
     write(Ndof_char, '(I4)') (2 * Ndof - 1)
     outfile_frmt = "("//trim(adjustl(Ndof_char))//"(G20.12, '  '), G20.12)"
@@ -168,6 +209,10 @@ program kappa_gpu
     write(*, '(3(ES22.14, 2X))') kappa(1:3)
     write(*, '(3(ES22.14, 2X))') kappa(4:6)
     write(*, '(3(ES22.14, 2X))') kappa(7:9)
+    ! kappa_<prefix><T>K.h5 (main.f90:773-798): the tensor, [omega; linewidth] of every q0 and the
+    ! mode-resolved kappa
+    call w_kappa_h5(trim(kappa_outFile), kappa, omega(:, irr_grid), LW_Allq0, Mode_k=Mode_k)
+    write(*, '(A, A)') 'Wrote ', trim(kappa_outFile)
     call gpu_ctx_destroy(ctx)
 
 contains

@@ -61,5 +61,31 @@ def _drain(procs):
         raise RuntimeError("hipcc failed:\n" + "\n".join(errs))
 
 
+H5_SRC = os.path.join(HERE, "h5", "spr_h5.cpp")
+H5_LIB = os.path.join(HERE, "libspring_h5.so")
+H5_ROOT = os.environ.get("SPR_HDF5_ROOT", "/opt/conda")
+
+
+def build_h5(verbose: bool = False) -> str | None:
+    """libspring_h5.so: the HDF5 file boundary (host code, g++ over the HDF5 C library).
+    Returns None when no HDF5 installation is found (the GPU library does not depend on it)."""
+    inc, lib = os.path.join(H5_ROOT, "include"), os.path.join(H5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print(f"no hdf5.h under {inc}: libspring_h5.so not built", flush=True)
+        return None
+    tmp = H5_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{inc}", H5_SRC, "-o", tmp,
+           f"-L{lib}", "-lhdf5", f"-Wl,-rpath,{lib}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("libspring_h5.so build failed:\n" + r.stdout)
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_h5(verbose=True))

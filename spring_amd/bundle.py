@@ -13,7 +13,9 @@ import numpy as np
 
 
 def write_bundle(path, model, qm, ph, T: float, sigma: float, prefix: str = "synth",
-                 with_fc4: bool = False):
+                 with_fc4: bool = False, h5: bool = False):
+    """h5: also write the force constants as the reference's own files
+    FC2nd_/FC3rd_/FC4th_<prefix><T>K_F.h5 (the driver reads those when present)."""
     os.makedirs(path, exist_ok=True)
     cr = model.crystal
 
@@ -59,6 +61,11 @@ def write_bundle(path, model, qm, ph, T: float, sigma: float, prefix: str = "syn
         put("atm_indx4.bin", idx4, "<i4")
         put("fc4.bin", f4.fc, "<f8")
         natm4 = f4.natm_max
+    if h5:
+        from . import h5io
+        for f in (model.fc2, model.fc3, model.fc4 if with_fc4 else None):
+            if f is not None:
+                h5io.write_fc(os.path.join(path, h5io.fc_file_name(f.order, prefix, T)), f)
     with open(os.path.join(path, "bundle.nml"), "w") as fh:
         fh.write("&bundle\n")
         fh.write(f"  prefix = '{prefix}'\n  Nbasis = {cr.nb}\n  Natm2 = {f2.natm_max}\n  Natm3 = {f3.natm_max}\n")

@@ -92,3 +92,38 @@ def test_fortran_driver_device_phonons(rocksalt, tmp_path):
     fmin, fmax = (float(x) for x in re.search(r"freq_min\s+(\S+)\s+freq_max\s+(\S+)", r.stdout).groups())
     assert np.isclose(fmax, o["omega"][0].max(), rtol=1e-8)            # printed with 9 digits
     assert abs(fmin - 1.05 * abs(o["omega"][0, 2])) < 1e-4 * fmax      # acoustic: sqrt of rounding
+
+
+def test_fortran_driver_reads_fc_h5_and_writes_kappa_h5(rocksalt, tmp_path):
+    """The driver reads FC3rd_<prefix><T>K_F.h5 (r3rdFC) and FC2nd_ (r2ndFC, device phonons) in
+    place of the raw arrays, which are zeroed here so that only the HDF5 files can give the
+    right answer, and writes kappa_<prefix><T>K.h5 (main.f90:773-798)."""
+    from spring_amd import bundle, h5io
+    from oracle import oracle as O
+    exe = os.path.join(ROOT, "fortran", "kappa_gpu")
+    assert os.path.exists(exe), "fortran/kappa_gpu is not built"
+    qm, ph = problem(rocksalt, (4, 4, 4))
+    b = tmp_path / "b"
+    bundle.write_bundle(str(b), rocksalt, qm, ph, 300.0, 1.0, prefix="NaClsynth", h5=True)
+    for name in ("fc3.bin", "fc2.bin"):
+        n = os.path.getsize(b / name)
+        (b / name).write_bytes(b"\0" * n)
+    r = subprocess.run([exe, str(b), "fp64", "all"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0 and "FC3rd_NaClsynth300.0K_F.h5" in r.stdout
+    ref = O.lw3_gauss_all(O.Inputs(rocksalt, qm, ph), qm.irr_idx, 1.0)
+    w, lw = bundle.read_linewidth_file(str(tmp_path / "linewidth_IBZ_NaClsynth300.0K.txt"), 6)
+    assert rel_err(lw, ref, 1e-8) < 1e-10
+    out = h5io.read_kappa(str(tmp_path / "kappa_NaClsynth300.0K.h5"), 6, qm.nq0, mode_k=True)
+    assert np.array_equal(out["omega"], ph.omega[qm.irr_idx])
+    assert rel_err(out["lw"], ref, 1e-8) < 1e-10
+    kref = O.kappa_rta_symm(qm, ph, ref, 300.0, rocksalt.crystal.vol)
+    assert np.allclose(out["kappa"], kref, rtol=1e-10)
+    assert np.allclose(out["mode_k"].sum(axis=0).reshape(3, 3), out["kappa"], rtol=1e-12)
+    # device phonons from FC2nd_<prefix><T>K_F.h5
+    r = subprocess.run([exe, str(b), "fp64", "all", "gpuph"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0 and "FC2nd_NaClsynth300.0K_F.h5" in r.stdout
+    w, _ = bundle.read_linewidth_file(str(tmp_path / "linewidth_IBZ_NaClsynth300.0K.txt"), 6)
+    assert np.allclose(w[1:], ph.omega[qm.irr_idx][1:], rtol=1e-11, atol=0)

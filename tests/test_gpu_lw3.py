@@ -107,3 +107,55 @@ def test_small_kernel_matches_generic(small, monkeypatch):
     b = ctx.lw3_all(0.1, "fp64")
     monkeypatch.delenv("SPR_KERNEL")
     assert rel_err(a, b, 1e-9) < 1e-12
+
+
+def test_full_size_2atom_16(rocksalt):
+    """The bench workload at its full size (2-atom rocksalt, 16^3 mesh, all 145 irreducible q0):
+    FP64 linewidths equal the oracle's on a seeded sample of q0, the launch is bitwise
+    repeatable, FP32 stays within the stated tolerance (kappa 1e-6, linewidths 1e-4), and the
+    symmetrised kappa of the GPU linewidths equals the oracle's."""
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    qm, ph = problem(rocksalt, (16, 16, 16))
+    ctx = GpuContext(rocksalt, qm, ph)
+    lw = ctx.lw3_all(0.1, "fp64")
+    assert np.array_equal(lw, ctx.lw3_all(0.1, "fp64"))
+    pos = np.random.default_rng(5).choice(qm.nq0, size=12, replace=False)
+    ref = O.lw3_gauss_all(O.Inputs(rocksalt, qm, ph), qm.irr_idx[pos], 0.1, threads=8)
+    e = rel_err(lw[pos], ref, 1e-9)
+    print(f"16^3 FP64 vs oracle on 12 q0: {e:.2e}", flush=True)
+    assert e < 1e-10
+    k64, _ = ctx.kappa(lw, 300.0, symm=True)
+    kref = O.kappa_rta_symm(qm, ph, lw, 300.0, rocksalt.crystal.vol)
+    assert np.allclose(k64, kref, rtol=1e-12, atol=1e-14 * np.abs(kref).max())
+    lw32 = ctx.lw3_all(0.1, "fp32")
+    k32, _ = ctx.kappa(lw32, 300.0, symm=True)
+    ek, el = rel_err(k32, k64, 1e-6), rel_err(lw32, lw, 1e-6)
+    print(f"16^3 FP32 vs FP64: kappa {ek:.2e}, linewidths {el:.2e}", flush=True)
+    assert ek < 1e-6 and el < 1e-4
+    ctx.close()
+
+
+def test_full_size_10atom_20(rocksalt10):
+    """The north-star workload at its full size (10-atom cell, 20^3 mesh with time reversal):
+    the linewidths of two seeded irreducible q0, each summed over all 8000 q1, in FP64 equal the
+    oracle's; FP32 and split BF16 stay within 1e-4 and 1e-3 of FP64."""
+    from concurrent.futures import ThreadPoolExecutor
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    qm, ph = problem(rocksalt10, (20, 20, 20), time_rev=1)
+    ctx = GpuContext(rocksalt10, qm, ph)
+    inp = O.Inputs(rocksalt10, qm, ph)
+    chunks = [(lo, min(qm.nq, lo + 250)) for lo in range(0, qm.nq, 250)]
+    for i0 in np.random.default_rng(9).choice(qm.nq0, size=2, replace=False):
+        i0 = int(i0)
+        lw = ctx.lw3(i0, 0.1, "fp64")
+        with ThreadPoolExecutor(16) as ex:
+            parts = list(ex.map(lambda c: O.lw3_gauss(inp, int(qm.irr_idx[i0]), 0.1, c[0], c[1]), chunks))
+        ref = np.sum(parts, axis=0)
+        e64 = rel_err(lw, ref, 1e-9)
+        e32 = rel_err(ctx.lw3(i0, 0.1, "fp32"), lw, 1e-6)
+        e16 = rel_err(ctx.lw3(i0, 0.1, "bf16"), lw, 1e-6)
+        print(f"20^3 10-atom q0 {i0}: FP64 vs oracle {e64:.2e}, FP32 {e32:.2e}, BF16 {e16:.2e}", flush=True)
+        assert e64 < 1e-10 and e32 < 1e-4 and e16 < 1e-3
+    ctx.close()

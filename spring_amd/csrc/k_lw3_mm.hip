@@ -358,6 +358,25 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
 
     KSum<float> accf;
     double accd = 0.0;
+    // FT outputs of this thread, e = tid + 512 k -> (block nu eta, component alpha beta gamma):
+    // the block index within the atom, the component and the destination in T, the same for
+    // every pair and atom of the unit
+    int ft_bl[Sh::KE], ft_j[Sh::KE], ft_dst[Sh::KE];
+#pragma unroll
+    for (int k = 0; k < Sh::KE; ++k) {
+        const int e = tid + 512 * k;
+        ft_bl[k] = -1;
+        ft_j[k] = 0;
+        ft_dst[k] = 0;
+        if (e < Sh::NE) {
+            const int bl = e / 27, j = e % 27;
+            const int nu = bl / NB, eta = bl % NB;
+            const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
+            ft_bl[k] = bl;
+            ft_j[k] = j;
+            ft_dst[k] = (al * ND + 3 * nu + be) * PA + 2 * (3 * eta + ga);
+        }
+    }
 
     for (int pi = 0; pi < cnt; ++pi) {
         const int2 pr = A.pairs[poff + pi];
@@ -428,23 +447,48 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             const int u0 = Sh::PALL ? 0 : sBlk[mu * NB * NB];
             const R *fcm = Sh::PALL ? sFC : fc + (size_t)u0 * 27;
             const int ub0 = sBlk[mu * NB * NB];   // first cluster of atom mu
-            for (int e = tid; e < Sh::NE; e += 512) {
-                const int bl = e / 27, j = e % 27;
-                const int nu = bl / NB, eta = bl % NB;
-                const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
-                const int blk = (mu * NB + nu) * NB + eta;
-                const int ua = sBlk[blk], ub = sBlk[blk + 1];
-                R re = 0, im = 0;
+            if constexpr (Sh::PALL) {
+                // FP32 / BF16: index arithmetic of this thread's outputs precomputed per unit
+#pragma unroll
+                for (int k = 0; k < Sh::KE; ++k) {
+                    if (ft_bl[k] < 0) continue;
+                    const int blk = mu * NB * NB + ft_bl[k];
+                    const int ua = sBlk[blk], ub = sBlk[blk + 1];
+                    const R *fj = fcm + (size_t)(ua - ub0) * 27 + ft_j[k];
+                    const Z *pj = sP + (ua - u0);
+                    R re = 0, im = 0;
 #pragma unroll 4
-                for (int u = ua; u < ub; ++u) {
-                    const R f = fcm[(size_t)(u - ub0) * 27 + j];
-                    const Z ph = sP[u - u0];
-                    re = fma(f, ph.x, re);
-                    im = fma(f, ph.y, im);
+                    for (int u = 0; u < ub - ua; ++u) {
+                        const R f = fj[(size_t)u * 27];
+                        const Z ph = pj[u];
+                        re = fma(f, ph.x, re);
+                        im = fma(f, ph.y, im);
+                    }
+                    R *dst = sT + ft_dst[k];
+                    dst[0] = re;
+                    dst[1] = im;
                 }
-                R *dst = sT + (al * ND + 3 * nu + be) * PA + 2 * (3 * eta + ga);
-                dst[0] = re;
-                dst[1] = im;
+            } else {
+                // FP64 (FC read from global memory): the registers the precomputed indices would
+                // take cost more than the arithmetic they save (measured)
+                for (int e = tid; e < Sh::NE; e += 512) {
+                    const int bl = e / 27, j = e % 27;
+                    const int nu = bl / NB, eta = bl % NB;
+                    const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
+                    const int blk = (mu * NB + nu) * NB + eta;
+                    const int ua = sBlk[blk], ub = sBlk[blk + 1];
+                    R re = 0, im = 0;
+#pragma unroll 4
+                    for (int u = ua; u < ub; ++u) {
+                        const R f = fcm[(size_t)(u - ub0) * 27 + j];
+                        const Z ph = sP[u - u0];
+                        re = fma(f, ph.x, re);
+                        im = fma(f, ph.y, im);
+                    }
+                    R *dst = sT + (al * ND + 3 * nu + be) * PA + 2 * (3 * eta + ga);
+                    dst[0] = re;
+                    dst[1] = im;
+                }
             }
         };
         if constexpr (!Sh::PALL) {

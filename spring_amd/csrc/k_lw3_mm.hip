@@ -109,6 +109,11 @@ template <int MODE, int NB> struct MmShape {
 };
 
 // LDS layout (bytes), shared by the kernel and the host-side size check
+// FP32 / BF16: V is staged for the delta stage in the cluster-phase region, three 16 x 16 FP32
+// tiles per wave at a time
+constexpr int kDvTiles = 3;
+constexpr size_t kDvBytes = (size_t)8 * kDvTiles * 256 * sizeof(float);
+
 template <int MODE, int NB> struct MmLds {
     using Sh = MmShape<MODE, NB>;
     using R = typename Sh::R;
@@ -122,9 +127,14 @@ template <int MODE, int NB> struct MmLds {
     __host__ __device__ static constexpr size_t u2() { return U2; }
     __host__ __device__ static constexpr size_t ac() { return AC; }
     __host__ __device__ static constexpr size_t e() { return E; }
+    // cluster phases (all atoms in the FP32 / BF16 modes, which also stage V there afterwards)
+    __host__ __device__ static size_t p_bytes(int npmax, int nterm) {
+        const size_t b = (size_t)(Sh::PALL ? nterm : npmax) * sizeof(Z);
+        return Sh::PALL && b < kDvBytes ? kDvBytes : b;
+    }
     static size_t total(int npmax, int nterm, int nsite, int L) {
         size_t b = tu() + u2() + ac() + 2 * e();
-        b += (size_t)(Sh::PALL ? nterm : npmax) * sizeof(Z) + (size_t)L * sizeof(Z) + 8 * Sh::ND * sizeof(double) +
+        b += p_bytes(npmax, nterm) + (size_t)L * sizeof(Z) + 8 * Sh::ND * sizeof(double) +
              4 * Sh::ND * sizeof(float);
         if (Sh::PALL) b += (size_t)npmax * 27 * sizeof(R);      // FC of one atom (staged)
         b += (size_t)Sh::ND * Sh::ND * sizeof(Z) + 16;           // e0 of the unit's q0
@@ -132,6 +142,13 @@ template <int MODE, int NB> struct MmLds {
         return (b + 15) & ~(size_t)15;
     }
 };
+
+// orders the LDS accesses of one wave (a wave's LDS operations complete in issue order)
+__device__ __forceinline__ void mm_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // output row (within a 16 x 16 tile) of accumulator element r in lane group g
 template <bool F64OUT>
@@ -296,7 +313,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     Z *sE2 = reinterpret_cast<Z *>(p);
     p += L::e();
     Z *sP = reinterpret_cast<Z *>(p);                 // cluster phases (all atoms or atom mu)
-    p += (size_t)(Sh::PALL ? A.nterm : A.npmax) * sizeof(Z);
+    p += L::p_bytes(A.npmax, A.nterm);
     Z *sPh = reinterpret_cast<Z *>(p);
     p += (size_t)A.L * sizeof(Z);
     double *sMode = reinterpret_cast<double *>(p);    // w0 iw0f | w1 n1 iw1 | w2 n2 iw2 | -
@@ -592,45 +609,108 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             __syncthreads();
         }
 
-        // ---- W and the Gaussian-delta weights; lanes with q = 0 take rows 0,1, q = 1 rows 2,3.
-        //      The delta arguments are formed in FP64 (they cancel near resonance); in the FP32 and
-        //      BF16 modes the rest is FP32, summed per pair and then into a compensated sum. ----
-        if (s0ok) {
-            const double w0 = sMode[s0], iw0 = sMode[ND + s0];
+        // ---- W and the Gaussian-delta weights.  Lanes l and l^1 hold Re and Im of the same
+        //      (row, s0); lane q = 0 takes rows r = 0, 1 of every tile and lane q = 1 rows 2, 3, each
+        //      receiving the other part from its partner, so no lane idles.  The delta arguments
+        //      are formed in FP64 (they cancel near resonance); in the FP32 and BF16 modes the rest
+        //      is FP32, summed per pair and then into a compensated sum.  A tile row whose three
+        //      Gaussians underflow to exact zero on every lane of the wave adds nothing and skips
+        //      the exponentials (same sums bit for bit). ----
+        if constexpr (Sh::PALL) {
+            // V staged in LDS (the cluster phases are dead after the last FT): each wave copies
+            // up to kDvTiles of its tiles into its own slice, then runs the delta arithmetic over
+            // its elements as a rolled loop, which keeps the register count of this stage low
+            float *sV = reinterpret_cast<float *>(sP) + (size_t)w * (kDvTiles * 256);
+            const double w0 = s0ok ? sMode[s0] : 0.0, iw0 = s0ok ? sMode[ND + s0] : 0.0;
             float pairf = 0.f;
 #pragma unroll
-            for (int j = 0; j < Sh::VT; ++j) {
-                const int mt = mtC0 + MSTEPC * j;
-                if (mt >= Sh::MT_D) continue;
+            for (int j0 = 0; j0 < Sh::VT; j0 += kDvTiles) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const DR x = vacc[j][r];
-                    const DR y = __shfl_xor(x, 1, 64);
-                    if ((r >> 1) != qD) continue;
-                    const int m = mt * 16 + tile_row<MODE == MM_F64>(g, r);
-                    const int s1 = m / SB, s2 = h * SB + m % SB;
-                    if (s1 >= ND || s2 >= ND) continue;
-                    const double w1 = sMode[2 * ND + s1], w2 = sMode[5 * ND + s2];
-                    const double ap = w1 + w2, am = w1 - w2;
-                    if constexpr (MODE == MM_F64) {
-                        const double n1 = sMode[3 * ND + s1], iw1 = sMode[4 * ND + s1];
-                        const double n2 = sMode[6 * ND + s2], iw2 = sMode[7 * ND + s2];
+                for (int t = 0; t < kDvTiles; ++t)
+                    if (j0 + t < Sh::VT) *reinterpret_cast<v4f *>(sV + (t * 64 + l) * 4) = vacc[j0 + t];
+                mm_wave_sync();
+                if (s0ok) {
+                    const int ne = 2 * (Sh::VT - j0 < kDvTiles ? Sh::VT - j0 : kDvTiles);
+#pragma unroll 1
+                    for (int e = 0; e < ne; ++e) {
+                        const int t = e >> 1, k = e & 1;
+                        const int mt = mtC0 + MSTEPC * (j0 + t);
+                        if (mt >= Sh::MT_D) continue;
+                        const int r = 2 * qD + k;
+                        const float x = sV[(t * 64 + l) * 4 + r], y = sV[(t * 64 + (l ^ 1)) * 4 + r];
+                        const int m = mt * 16 + tile_row<false>(g, r);
+                        const int s1 = m / SB, s2 = h * SB + m % SB;
+                        const bool ok = s1 < ND && s2 < ND;
+                        const double w1 = ok ? sMode[2 * ND + s1] : 0.0, w2 = ok ? sMode[5 * ND + s2] : 0.0;
+                        const double ap = w1 + w2, am = w1 - w2;
                         const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
-                        const double W = (x * x + y * y) * iw0 * iw1 * iw2;
-                        const double G = (n1 + n2 + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
-                                         (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
-                        accd += wpair * W * G;
-                    } else {
+                        const double tmin = hinv * fmin(d1 * d1, fmin(d2 * d2, d3 * d3));
+                        // __expf(-t) is exactly zero beyond t = 104: a wave whose lanes are all that
+                        // far from resonance adds nothing here
+                        if (!__ballot(ok && tmin < 110.0)) continue;
+                        if (!ok) continue;
                         const float n1 = sModeF[s1], iw1 = sModeF[ND + s1];
                         const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
-                        const float d1 = (float)(w0 - ap), d2 = (float)(w0 + am), d3 = (float)(w0 - am);
+                        const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
                         const float W = (x * x + y * y) * (iw1 * iw2);
-                        const float e1 = __expf(hneg * d1 * d1), e2 = __expf(hneg * d2 * d2), e3 = __expf(hneg * d3 * d3);
+                        const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
                         pairf = fmaf(W, (n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3), pairf);
                     }
                 }
+                mm_wave_sync();   // this wave has read the slice before the next round overwrites it
             }
-            if constexpr (MODE != MM_F64) accf.add(pairf * (float)(wpair * c0 * iw0));
+            if (s0ok) accf.add(pairf * (float)(wpair * c0 * iw0));
+        } else {
+            if (s0ok) {
+                const double w0 = sMode[s0], iw0 = sMode[ND + s0];
+                // s2 of this lane's two rows depends on the lane only when the s2 block divides 16
+                constexpr bool kS2Lane = 16 % SB == 0;
+                int s2k[2];
+                double w2k[2];
+    #pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    s2k[k] = h * SB + tile_row<MODE == MM_F64>(g, 2 * qD + k) % SB;
+                    w2k[k] = s2k[k] < ND ? sMode[5 * ND + s2k[k]] : 0.0;
+                }
+                float pairf = 0.f;
+    #pragma unroll
+                for (int j = 0; j < Sh::VT; ++j) {
+                    const int mt = mtC0 + MSTEPC * j;
+                    if (mt >= Sh::MT_D) continue;
+    #pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const DR x = qD ? vacc[j][2 + k] : vacc[j][k];          // row 2 qD + k, this lane
+                        const DR y = __shfl_xor(qD ? vacc[j][k] : vacc[j][2 + k], 1, 64);   // same row, partner
+                        const int m = mt * 16 + tile_row<MODE == MM_F64>(g, 2 * qD + k);
+                        const int s1 = m / SB, s2 = kS2Lane ? s2k[k] : h * SB + m % SB;
+                        const bool ok = s1 < ND && s2 < ND;
+                        const double w1 = ok ? sMode[2 * ND + s1] : 0.0;
+                        const double w2 = kS2Lane ? w2k[k] : (ok ? sMode[5 * ND + s2] : 0.0);
+                        const double ap = w1 + w2, am = w1 - w2;
+                        const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
+                        const double tmin = hinv * fmin(d1 * d1, fmin(d2 * d2, d3 * d3));
+                        // FP64: exp(-t) is zero beyond t = 745.2; FP32: __expf(-t) is zero beyond t = 104
+                        if (!__ballot(ok && tmin < (MODE == MM_F64 ? 746.0 : 110.0))) continue;
+                        if (!ok) continue;
+                        if constexpr (MODE == MM_F64) {
+                            const double n1 = sMode[3 * ND + s1], iw1 = sMode[4 * ND + s1];
+                            const double n2 = sMode[6 * ND + s2], iw2 = sMode[7 * ND + s2];
+                            const double W = (x * x + y * y) * iw0 * iw1 * iw2;
+                            const double G = (n1 + n2 + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
+                                             (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
+                            accd += wpair * W * G;
+                        } else {
+                            const float n1 = sModeF[s1], iw1 = sModeF[ND + s1];
+                            const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
+                            const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
+                            const float W = (x * x + y * y) * (iw1 * iw2);
+                            const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
+                            pairf = fmaf(W, (n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3), pairf);
+                        }
+                    }
+                }
+                if constexpr (MODE != MM_F64) accf.add(pairf * (float)(wpair * c0 * iw0));
+            }
         }
     }
     // ---- reduce per s0 in a fixed order: wave w holds the columns of N tile w % NTC ----

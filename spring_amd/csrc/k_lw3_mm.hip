@@ -100,6 +100,10 @@ template <int MODE, int NB> struct MmShape {
     static constexpr int NTC = pow2ceil(NT);                // waves per M tile, GEMMs C and D
     static constexpr int MT_D = (ND * SB + 15) / 16;        // GEMM D: rows (s1, s2 local)
     static constexpr int VT = (MT_D * NTC + 7) / 8;         // V tiles per wave
+    // U2 rows (s1, s2 local) hold 3 complex (alpha); each s1 block of SB rows is padded by one
+    // complex so that the GEMM C stores of consecutive s1 fall in different LDS banks
+    static constexpr int U2S1 = 3 * SB + 1;
+    static constexpr int U2ROWS = (MT_D * 16 + SB - 1) / SB;   // s1 blocks, padding rows included
     static_assert(NT <= 4, "Ndof must be <= 32");
     static_assert(NSPLIT <= kMaxUnitsPerItem, "too many s2 blocks");
     // cluster phases of every basis atom per pair in LDS (FP32 / BF16), else per atom
@@ -120,7 +124,7 @@ template <int MODE, int NB> struct MmLds {
     using Z = typename Sh::Z;
     static constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
     static constexpr size_t TU = (size_t)Sh::MB * Sh::PA * sizeof(R);
-    static constexpr size_t U2 = (size_t)Sh::MT_D * 16 * 3 * sizeof(Z);
+    static constexpr size_t U2 = (size_t)Sh::U2ROWS * Sh::U2S1 * sizeof(Z);
     static constexpr size_t AC = cmax((size_t)Sh::MC * Sh::PA * sizeof(R), (size_t)512 * sizeof(double));
     static constexpr size_t E = (size_t)Sh::ND * Sh::PE * sizeof(Z);
     __host__ __device__ static constexpr size_t tu() { return TU; }
@@ -303,7 +307,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     unsigned char *p = smem;
     R *sT = reinterpret_cast<R *>(p);                 // T rows (alpha, b), K = (c, p)
     p += L::tu();
-    Z *sU2 = reinterpret_cast<Z *>(p);                // U2[(s1, s2 local)][alpha], 3 per row
+    Z *sU2 = reinterpret_cast<Z *>(p);                // U2[s1][s2 local][alpha], s1 stride U2S1
     p += L::u2();
     R *sAC = reinterpret_cast<R *>(p);                // U1 as rows (alpha, s2 local), K = (b, p)
     double *sRed = reinterpret_cast<double *>(p);     //   (end of unit only)
@@ -341,7 +345,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     // ---- per unit: zero padding, phase table, q0 modes ----
     for (int i = tid; i < Sh::MC * PA; i += 512) sAC[i] = (R)0;
     for (int i = tid; i < Sh::MB * PA; i += 512) sT[i] = (R)0;          // padding rows and columns stay 0
-    for (int i = tid; i < Sh::MT_D * 16 * 3; i += 512) sU2[i] = Z{0, 0};
+    for (int i = tid; i < Sh::U2ROWS * Sh::U2S1; i += 512) sU2[i] = Z{0, 0};
     for (int i = tid; i < ND * PE; i += 512) {
         if (i % PE >= ND) {
             sE1[i] = Z{0, 0};
@@ -566,7 +570,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                             const int m = mtt * 16 + tile_row<MODE == MM_F64>(g, r);
                             if (m < 3 * SB && s1 < ND) {
                                 const int al = m / SB, sl = m % SB;
-                                R *z = reinterpret_cast<R *>(sU2 + (s1 * SB + sl) * 3 + al);
+                                R *z = reinterpret_cast<R *>(sU2 + s1 * Sh::U2S1 + sl * 3 + al);
                                 z[q] = (R)d[t][r];
                             }
                         }
@@ -594,7 +598,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                     if (mt < Sh::MT_D) {
                         const int m = mt * 16 + (l & 15);
                         Z a = Z{0, 0};
-                        if (g < 3) a = sU2[m * 3 + g];
+                        if (g < 3) a = sU2[(m / SB) * Sh::U2S1 + (m % SB) * 3 + g];
                         if constexpr (MODE == MM_F64) {
                             vacc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b0, vacc[j], 0, 0, 0);
                             vacc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b1, vacc[j], 0, 0, 0);

@@ -58,6 +58,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="repeat CPU passes until this much wall time is spent")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--phonons", default="device", choices=["device", "host"],
+                   help="cell10: phonons of the mesh from spr_get_phonon (device) or the host LAPACK "
+                        "path; both feed the FP64 reference, the reduced-precision runs and the CPU leg")
     return p.parse_args()
 
 
@@ -200,10 +203,13 @@ def cell10(args):
     n = args.mesh10
     t0 = time.perf_counter()
     qm = mesh.make_mesh(m.crystal, (n, n, n), time_rev=1)
-    ph = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
+    if args.phonons == "device":
+        ph = lib.get_phonon(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
+    else:
+        ph = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
     t_ph = time.perf_counter() - t0
-    print(f"# cell10: model {t_model:.1f} s, mesh+phonons {t_ph:.1f} s, nq0 {qm.nq0}", file=sys.stderr,
-          flush=True)
+    print(f"# cell10: model {t_model:.1f} s, mesh+phonons ({args.phonons}) {t_ph:.2f} s, nq0 {qm.nq0}",
+          file=sys.stderr, flush=True)
     ctx = lib.GpuContext(m, qm, ph)
     nd = m.crystal.ndof
     triplets = qm.nq0 * qm.nq * nd ** 3
@@ -273,7 +279,7 @@ def cell10(args):
                    "pairs_executed_per_step": npairs, "triplets_per_step": triplets,
                    "flop_per_pair": flop_pair},
         "fp64": entry("fp64"), "fp32": entry("fp32"), "bf16": entry("bf16"),
-        "cpu_baseline": cpu, "phonons_s": t_ph, "host_model_s": t_model,
+        "cpu_baseline": cpu, "phonons_s": t_ph, "phonons": args.phonons, "host_model_s": t_model,
         "kappa_W_mK": [float(r64["k"][i, i]) for i in range(3)],
     }
     print(json.dumps(out), flush=True)

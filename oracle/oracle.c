@@ -21,6 +21,11 @@
  *   orc_scatter_mat_el4 / orc_v4_factorised / orc_lw4_gauss   as the 3-phonon ones
  *   orc_kappa_rta          closed-form single-mode model; symmetric tensor
  *   orc_lw3_tetra          sum of vertex weights == 1; linear-band density of states
+ *   orc_w4_rows            the Gaussian four-phonon sum rebuilt from its W4 == orc_lw4_gauss
+ *   orc_lw4_tetra_from_w   per process (disjoint energy ranges): integral over Mq == the grid sum
+ *                          of F (the delta integrates to one), first moment == the grid mean
+ *                          of M interpolated linearly; a single (q1, q2) spike picks n2, n3 of
+ *                          q2 and q3 = -(q0+q1+q2); no NaN from degenerate vertices; Gamma skip
  *   orc_bte_sums           1/tau == 2 x the tetrahedron linewidth of orc_lw3_tetra; with
  *                          W+ = W- = 1 and F = a w on the full grid, dF == a w 1/tau exactly
  *                          (the vertex weights reproduce Mqs); linearity in F
@@ -526,8 +531,8 @@ void orc_kappa_rta_symm(int nd, int nq, int nq0, const int *irr_q_int, const int
  * ties, the order my_sort's first-uncovered search produces), then the three cases give
  * g (density at Mqs) and the vertex weights I(4); returns g * sum(I F) / 6 in *gIF and g.
  * ------------------------------------------------------------------------------------*/
-int orc_tetra_weights(double Mqs, const double *M_in, const double *F_in, double *g_out, double *I_out,
-                      double *gIF_out)
+static int tetra_core(double Mqs, const double *M_in, const double *F_in, double *g_out, double *I_out,
+                      double *F_out)
 {
     double M[4], F[4];
     int idx[4] = {0, 1, 2, 3};
@@ -565,6 +570,16 @@ int orc_tetra_weights(double Mqs, const double *M_in, const double *F_in, double
         Iv[3] = ((Mqs - M[0]) / (M[3] - M[0]) + (Mqs - M[1]) / (M[3] - M[1]) + (Mqs - M[2]) / (M[3] - M[2])) / 3.0;
         enter = 1;
     }
+    *g_out = g;
+    for (int v = 0; v < 4; ++v) { I_out[v] = Iv[v]; F_out[v] = F[v]; }
+    return enter;
+}
+
+int orc_tetra_weights(double Mqs, const double *M_in, const double *F_in, double *g_out, double *I_out,
+                      double *gIF_out)
+{
+    double g, Iv[4], F[4];
+    const int enter = tetra_core(Mqs, M_in, F_in, &g, Iv, F);
     double ixf = 0.0;
     if (enter)
         for (int v = 0; v < 4; ++v) ixf += Iv[v] * F[v];
@@ -615,6 +630,135 @@ void orc_lw3_tetra(int nd, int nq, const int *tetra, const int *partner, const d
     }
 }
 
+
+/* --------------------------------------------------------------------------------------
+ * Four-phonon tetrahedron linewidth (TetrahedronIntegration4th, SumOverPolarization and
+ * FindFandOmegaVer_4th, Tetrahedron4th_sep.f90:37-425).
+ *
+ * orc_w4_rows: W4(s3,s2,s1,s0; q1, q2) of irreducible q0 `i0` for q1 in [q1_lo, q1_hi) and
+ * every q2, q3 = -(q0+q1+q2) mod G: W[q1-q1_lo][q2][s0][s1][s2][s3] =
+ * |V4|^2 prod(1/w) Sctr4THz2/Nq^2 (ScatterProbability4, FourthOrder_m.f90:425-473).  The
+ * reference stores W4 once per permutation class of (q1,q2,q3) and reads the other orders by
+ * permuting (s1,s2,s3) (FindFandOmegaVer_4th :296-320, MomentumConsrv4th,
+ * QuasiMomentumConsrv4th_m.f90:32-206); with an index-permutation-symmetric FC4 that is the
+ * same number as W4 evaluated for the ordered triple, which is what is computed here.
+ * ------------------------------------------------------------------------------------*/
+static int grid_q3(const int *mesh, const int *k0, const int *k1, const int *k2)
+{
+    int k3[3];
+    for (int d = 0; d < 3; ++d) {
+        int v = -(k0[d] + k1[d] + k2[d]);
+        v %= mesh[d];
+        if (v < 0) v += mesh[d];
+        k3[d] = v;
+    }
+    return k3[0] + mesh[0] * (k3[1] + mesh[1] * k3[2]);
+}
+
+void orc_w4_rows(int nb, int natm, const int *atm_num, const int *atm_indx, const double *fc4,
+                 const double *latvec, const double *osm, int nq, const int *mesh, const int *q_int,
+                 const double *q_cart, const cplx *evec, const double *inv_omega, int i0, int q1_lo,
+                 int q1_hi, double *W)
+{
+    const int nd = 3 * nb;
+    const size_t n4 = (size_t)nd * nd * nd * nd;
+    cplx *phi = malloc(n4 * sizeof(cplx));
+    const double sctr4 = 0.125 * M_PI * pow(1.054571817 * 1.602176565, 2) * pow(6.022140857, 4) * 1000000.0;
+    const double factor = sctr4 / ((double)nq * (double)nq);
+    const int *k0 = q_int + 3 * i0;
+    const int g0 = k0[0] == 0 && k0[1] == 0 && k0[2] == 0;
+    for (int q1 = q1_lo; q1 < q1_hi; ++q1)
+        for (int q2 = 0; q2 < nq; ++q2) {
+            const int q3 = grid_q3(mesh, k0, q_int + 3 * q1, q_int + 3 * q2);
+            const int chk[4] = {g0, q1 == 0, q2 == 0, q3 == 0};
+            orc_v4_factorised(nb, natm, atm_num, atm_indx, fc4, latvec, osm, q_cart + 3 * q1,
+                              q_cart + 3 * q2, q_cart + 3 * q3, chk, evec + (size_t)i0 * nd * nd,
+                              evec + (size_t)q1 * nd * nd, evec + (size_t)q2 * nd * nd,
+                              evec + (size_t)q3 * nd * nd, phi);
+            const double *i0w = inv_omega + (size_t)i0 * nd, *i1w = inv_omega + (size_t)q1 * nd;
+            const double *i2w = inv_omega + (size_t)q2 * nd, *i3w = inv_omega + (size_t)q3 * nd;
+            double *out = W + ((size_t)(q1 - q1_lo) * nq + q2) * n4;
+            for (int s0 = 0; s0 < nd; ++s0)
+                for (int s1 = 0; s1 < nd; ++s1)
+                    for (int s2 = 0; s2 < nd; ++s2)
+                        for (int s3 = 0; s3 < nd; ++s3) {
+                            const size_t e = (((size_t)s0 * nd + s1) * nd + s2) * nd + s3;
+                            const cplx v = phi[e];
+                            out[e] = (creal(v) * creal(v) + cimag(v) * cimag(v)) *
+                                     (i0w[s0] * i1w[s1] * i2w[s2] * i3w[s3]) * factor;
+                        }
+        }
+    free(phi);
+}
+
+/* One process of SumOverPolarization (:242-263): FindgAndIver on the vertex values, NaN
+ * weights and density set to zero, FindIverxFver (includes 1/6), returns g * IxF. */
+static double tetra4_term(double Mq, const double *M, const double *F)
+{
+    double g, Iv[4], Fs[4];
+    const int enter = tetra_core(Mq, M, F, &g, Iv, Fs);
+    for (int v = 0; v < 4; ++v)
+        if (Iv[v] != Iv[v]) Iv[v] = 0.0;
+    if (g != g) g = 0.0;
+    double ixf = 0.0;
+    if (enter)
+        for (int v = 0; v < 4; ++v) ixf += Iv[v] * Fs[v];
+    ixf = (1.0 / 6.0) * ixf;
+    return g * ixf;
+}
+
+/* orc_lw4_tetra_from_w: the q1 in [q1_lo, q1_hi) part of lw(s0) of q0 `i0` from
+ * W[q1-q1_lo][q2][s0][s1][s2][s3] (orc_w4_rows).  For every tetrahedron of the q2 mesh
+ * (tetra[6 nq][4]) the vertices carry q2_v and q3_v = -(q0+q1+q2_v), and
+ *   M1 = w1 + w2 + w3,  F1 = W n1 n2 n3 / n0,              added with 1/6
+ *   M2 = w2 + w3 - w1,  F2 = W (1+n1) n2 n3 / n0,          added with 1/2
+ *   M3 = w3 - w1 - w2,  F3 = W (1+n1)(1+n2) n3 / n0,       added with 1/2
+ * at Mq = omega0[s0] (:329-343).  omega0 and nbe0 are the q0's omega and n (the caller passes
+ * omega[i0] and nbe[i0]); s0 < 3 skipped at q0 = Gamma (:285-286).  Raw sum; main.f90:660 adds
+ * 0.5 lw to the linewidth. */
+void orc_lw4_tetra_from_w(int nd, int nq, const int *mesh, const int *q_int, const int *tetra, int i0,
+                          int q1_lo, int q1_hi, const double *W, const double *omega, const double *nbe,
+                          const double *omega0, const double *nbe0, double *lw)
+{
+    const size_t n4 = (size_t)nd * nd * nd * nd;
+    const int *k0 = q_int + 3 * i0;
+    const int g0 = k0[0] == 0 && k0[1] == 0 && k0[2] == 0;
+    for (int s = 0; s < nd; ++s) lw[s] = 0.0;
+    for (int q1 = q1_lo; q1 < q1_hi; ++q1) {
+        const double *Wq1 = W + (size_t)(q1 - q1_lo) * nq * n4;
+        for (int t = 0; t < 6 * nq; ++t) {
+            int q2v[4], q3v[4];
+            for (int v = 0; v < 4; ++v) {
+                q2v[v] = tetra[4 * t + v];
+                q3v[v] = grid_q3(mesh, k0, q_int + 3 * q1, q_int + 3 * q2v[v]);
+            }
+            for (int s0 = g0 ? 3 : 0; s0 < nd; ++s0) {
+                const double Mq0 = omega0[s0], n0 = nbe0[s0];
+                for (int s1 = 0; s1 < nd; ++s1) {
+                    const double w1 = omega[(size_t)q1 * nd + s1], n1 = nbe[(size_t)q1 * nd + s1];
+                    for (int s2 = 0; s2 < nd; ++s2)
+                        for (int s3 = 0; s3 < nd; ++s3) {
+                            double F1[4], F2[4], F3[4], M1[4], M2[4], M3[4];
+                            for (int v = 0; v < 4; ++v) {
+                                const double w = Wq1[(size_t)q2v[v] * n4 + (((size_t)s0 * nd + s1) * nd + s2) * nd + s3];
+                                const double n2 = nbe[(size_t)q2v[v] * nd + s2], n3 = nbe[(size_t)q3v[v] * nd + s3];
+                                const double w2 = omega[(size_t)q2v[v] * nd + s2], w3 = omega[(size_t)q3v[v] * nd + s3];
+                                F1[v] = w * (n1 * n2 * n3 / n0);
+                                M1[v] = w1 + w2 + w3;
+                                F2[v] = w * ((1.0 + n1) * n2 * n3 / n0);
+                                M2[v] = w2 + w3 - w1;
+                                F3[v] = w * ((1.0 + n1) * (1.0 + n2) * n3 / n0);
+                                M3[v] = w3 - w1 - w2;
+                            }
+                            lw[s0] += (1.0 / 6.0) * tetra4_term(Mq0, M1, F1);
+                            lw[s0] += 0.5 * tetra4_term(Mq0, M2, F2);
+                            lw[s0] += 0.5 * tetra4_term(Mq0, M3, F3);
+                        }
+                }
+            }
+        }
+    }
+}
 
 /* --------------------------------------------------------------------------------------
  * Iterative BTE (src/Kappa/Src_iterative).  Linear tetrahedron weights of that code

@@ -212,6 +212,51 @@ def lw3_tetra(qm, ph, i0_grid, W):
     return lw, p3
 
 
+def w4_rows(inp: Inputs, i0_grid: int, q1_lo: int, q1_hi: int):
+    """W4[q1 - q1_lo][q2][s0][s1][s2][s3] of q0 `i0_grid` (orc_w4_rows)."""
+    nd = inp.nd
+    W = np.zeros((q1_hi - q1_lo, inp.nq, nd, nd, nd, nd))
+    lib().orc_w4_rows(ct.c_int(inp.nb), ct.c_int(inp.natm), inp.p_atm_num, inp.p_atm_indx, inp.p_fc,
+                      inp.p_latvec, inp.p_osm, ct.c_int(inp.nq), inp.p_mesh, inp.p_q_int, inp.p_q_cart,
+                      inp.p_evec, inp.p_inv_omega, ct.c_int(i0_grid), ct.c_int(q1_lo), ct.c_int(q1_hi),
+                      W.ctypes.data_as(ct.c_void_p))
+    return W
+
+
+def lw4_tetra_from_w(mesh, q_int, tetra, i0_grid, q1_lo, W, omega, nbe, omega0=None, nbe0=None):
+    """orc_lw4_tetra_from_w: the q1 in [q1_lo, q1_lo + len(W)) part of the four-phonon
+    tetrahedron lw of q0 `i0_grid`; omega0/nbe0 default to the q0's own."""
+    nq, nd = omega.shape
+    omega0 = omega[i0_grid] if omega0 is None else omega0
+    nbe0 = nbe[i0_grid] if nbe0 is None else nbe0
+    lw = np.zeros(nd)
+    arrs = [np.ascontiguousarray(mesh, np.int32), np.ascontiguousarray(q_int, np.int32),
+            np.ascontiguousarray(tetra, np.int32)]
+    rest = [np.ascontiguousarray(x, np.float64) for x in (W, omega, nbe, omega0, nbe0)]
+    assert rest[0].size == W.shape[0] * nq * nd ** 4
+    lib().orc_lw4_tetra_from_w(ct.c_int(nd), ct.c_int(nq), *[a.ctypes.data_as(ct.c_void_p) for a in arrs],
+                               ct.c_int(i0_grid), ct.c_int(q1_lo), ct.c_int(q1_lo + W.shape[0]),
+                               *[a.ctypes.data_as(ct.c_void_p) for a in rest], lw.ctypes.data_as(ct.c_void_p))
+    return lw
+
+
+def lw4_tetra(inp: Inputs, qm, i0_grid: int, threads: int = 1, chunk: int = 8):
+    """Four-phonon tetrahedron lw of q0 `i0_grid` (TetrahedronIntegration4th): W4 rows of a q1
+    chunk, then their tetrahedron sums; chunks summed in q1 order."""
+    spans = [(lo, min(inp.nq, lo + chunk)) for lo in range(0, inp.nq, chunk)]
+
+    def part(span):
+        W = w4_rows(inp, i0_grid, *span)
+        return lw4_tetra_from_w(inp.mesh, inp.q_int, qm.tetra, i0_grid, span[0], W, inp.omega, inp.nbe)
+
+    if threads <= 1:
+        parts = [part(sp) for sp in spans]
+    else:
+        with ThreadPoolExecutor(threads) as ex:
+            parts = list(ex.map(part, spans))
+    return np.sum(parts, axis=0)
+
+
 # ---------------------------------------------------------------------------------------------
 # Iterative BTE (src/Kappa/Src_iterative): time-reversal irreducible set, W+/W-, the tetrahedron
 # sums of orc_bte_sums, and the self-consistent loop of main.f90:243-361.

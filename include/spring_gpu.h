@@ -166,6 +166,17 @@ int spr_get_phonon(const spr_crystal *cr, const double *latvec, const spr_ifc *f
 int spr_lw4(spr_ctx *ctx, int i0, double sigma, int prec, double *lw);
 int spr_lw4_all(spr_ctx *ctx, double sigma, int prec, double *lw_all);
 
+/* Four-phonon linewidth sum of irreducible q0 `i0` with the linear tetrahedron method, the
+ * reference's delta for four-phonon scattering (TetrahedronIntegration4th,
+ * Tetrahedron4th_sep.f90:37-425): for every q1 and every tetrahedron of the q2 mesh, the three
+ * processes of spr_lw4 with g and vertex weights from FindgAndIver at Mq = omega(s0, q0), NaN
+ * weights of degenerate vertices set to zero.  W4 of every ordered (q1, q2) from the FP64 W4
+ * path, held one chunk of q1 rows at a time (at most 2 GiB); FP64 throughout.  Needs FC4 and
+ * mesh->tetra; Ndof 3 or 6.  lw(Ndof) raw sum; main.f90:660 adds 0.5*lw. */
+int spr_lw4_tetra(spr_ctx *ctx, int i0, double *lw);
+/* every irreducible q0: lw_all(Ndof, nq0) */
+int spr_lw4_tetra_all(spr_ctx *ctx, double *lw_all);
+
 /* RTA thermal conductivity (kappa_noIter, ThermalConductivity.f90:36-164, symm = 0; or
  * kappa_noIter_symm + GrVelKronProdSymm, :167-293 and VelocityKronProd_symm.f90:34-93,
  * symm = 1) from LW_Allq0(Ndof, nq0), FP64: kappa(9) W/mK, mode_k(9, Ndof) optional. */

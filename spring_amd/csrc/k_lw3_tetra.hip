@@ -140,6 +140,17 @@ __global__ void k_tetra_reduce(const double *partial, int nd, int nch, double *l
 
 }  // namespace
 
+int ensure_tetra(Ctx &c) {
+    if (c.tetra) return SPR_OK;
+    if (c.h_tetra.empty()) {
+        set_error("tetrahedron linewidth: the context was created without tetrahedra");
+        return SPR_E_STATE;
+    }
+    SPR_HIP(hipMalloc((void **)&c.tetra, c.h_tetra.size() * sizeof(int)));
+    SPR_HIP(hipMemcpy(c.tetra, c.h_tetra.data(), c.h_tetra.size() * sizeof(int), hipMemcpyHostToDevice));
+    return SPR_OK;
+}
+
 // Tetrahedron linewidth of irreducible q0 `qi` into d_lw[nd], d_p3[nd] (device), using c.wout
 // (filled here with W of every q1 by the FP64 W path).
 int launch_lw3_tetra(Ctx &c, int qi, double *d_lw, double *d_p3) {
@@ -147,15 +158,8 @@ int launch_lw3_tetra(Ctx &c, int qi, double *d_lw, double *d_p3) {
     const size_t nw = (size_t)c.nq * c.nd * c.nd * c.nd;
     if ((rc = ensure((void **)&c.wout, &c.wout_cap, nw * sizeof(double)))) return rc;
     if ((rc = launch_lw3(c, qi, 1, 0.1, SPR_PREC_FP64, nullptr, c.wout))) return rc;
-    if (!c.tetra) {
-        if (c.h_tetra.empty()) {
-            set_error("tetrahedron linewidth: the context was created without tetrahedra");
-            return SPR_E_STATE;
-        }
-        SPR_HIP(hipMalloc((void **)&c.tetra, c.h_tetra.size() * sizeof(int)));
-        SPR_HIP(hipMemcpy(c.tetra, c.h_tetra.data(), c.h_tetra.size() * sizeof(int), hipMemcpyHostToDevice));
-        SPR_HIP(hipMalloc((void **)&c.partner, (size_t)c.nq * sizeof(int)));
-    }
+    if ((rc = ensure_tetra(c))) return rc;
+    if (!c.partner) SPR_HIP(hipMalloc((void **)&c.partner, (size_t)c.nq * sizeof(int)));
     // partner table of this q0: q2 = -(q0 + q1) on the grid
     const int i0 = c.h_irr_grid[qi];
     const int m0 = c.mesh[0], m1 = c.mesh[1], m2 = c.mesh[2];

@@ -14,6 +14,10 @@
 //      (Tetrahedron4th_sep.f90:174-425) with a Gaussian in place of the tetrahedron
 //      integral; NaN terms are zeroed as there (:238-263); s0 < 3 skipped at q0 = Gamma.
 // The returned sum is the raw lw of TetrahedronIntegration4th; main.f90:660 adds 0.5 lw.
+//
+// Store mode (WST, FP64): one workgroup per (q1, q2 chunk) writes W4 of every ordered (q1, q2)
+// to W[q1 - q1_lo][q2][s0][s1][s2][s3] for the tetrahedron integration (k_lw4_tetra.hip)
+// instead of the Gaussian processes.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -36,9 +40,12 @@ struct Lw4Args {
     const double *omega, *inv_omega, *nbe;
     double sigma, factor;
     double *partial;   // [q0][chunk][ND]
+    // store mode: q1 in [q1_lo, q1_lo + gridDim.x / nq2ch), q2 chunks of q2_chunk points
+    double *wst;
+    int q1_lo, q2_chunk, nq2ch;
 };
 
-template <typename R, int ND, int BLK>
+template <typename R, int ND, int BLK, bool WST>
 __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
     using Z = cplx_t<R>;
     constexpr int NB = ND / 3;
@@ -72,10 +79,21 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
 #pragma unroll
     for (int s = 0; s < ND; ++s) acc[s] = 0.0;
 
-    const int q1_lo = ch * A.chunk, q1_hi = min(A.nq, q1_lo + A.chunk);
+    int q1_lo, q1_hi, q2_lo, q2_hi;
+    if constexpr (WST) {
+        q1_lo = A.q1_lo + ch / A.nq2ch;
+        q1_hi = q1_lo + 1;
+        q2_lo = (ch % A.nq2ch) * A.q2_chunk;
+        q2_hi = min(A.nq, q2_lo + A.q2_chunk);
+    } else {
+        q1_lo = ch * A.chunk;
+        q1_hi = min(A.nq, q1_lo + A.chunk);
+        q2_lo = 0;
+        q2_hi = A.nq;
+    }
     for (int q1 = q1_lo; q1 < q1_hi; ++q1) {
         const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
-        for (int q2 = 0; q2 < A.nq; ++q2) {
+        for (int q2 = q2_lo; q2 < q2_hi; ++q2) {
             const int k20 = q2 % A.m0, k21 = (q2 / A.m0) % A.m1, k22 = q2 / (A.m0 * A.m1);
             int a0 = -(k00 + k10 + k20) % A.m0, a1 = -(k01 + k11 + k21) % A.m1, a2 = -(k02 + k12 + k22) % A.m2;
             a0 += a0 < 0 ? A.m0 : 0;
@@ -135,6 +153,16 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
                 dst = t;
             }
             __syncthreads();
+            if constexpr (WST) {
+                // 3'. W4 only (every s0; the Gamma rows are zero through 1/omega = 0)
+                double *wrow = A.wst + ((size_t)(q1 - A.q1_lo) * A.nq + q2) * ND4;
+                for (int e = tid; e < ND4; e += BLK) {
+                    const int s0 = e / ND3, s1 = (e / ND2) % ND, s2 = (e / ND) % ND, s3 = e % ND;
+                    wrow[e] = (double)cabs2(src[e]) * (s_md[0][2][s0] * s_md[1][2][s1] * s_md[2][2][s2] *
+                                                      s_md[3][2][s3]) * A.factor;
+                }
+                continue;
+            }
             // 3. W4 and the three processes; src holds V[s0][s1][s2][s3]
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
@@ -160,6 +188,7 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
             }
         }
     }
+    if constexpr (WST) return;
     __shared__ double s_red[BLK];
     for (int s = 0; s < ND; ++s) {
         __syncthreads();
@@ -228,7 +257,9 @@ static int launch_lw4_t(Ctx &c, int q0_first, int q0_count, double sigma, double
     A.partial = c.partial;
     const size_t dyn = (size_t)((3 * A.nsite + 3) & ~3) * sizeof(int) + (size_t)c.L * sizeof(cplx_t<R>);
     SPR_HIP(hipEventRecord(c.ev0, c.stream));
-    hipLaunchKernelGGL((k_lw4<R, ND, BLK>), dim3(A.nchunks, q0_count), dim3(BLK), dyn, c.stream, A);
+    A.wst = nullptr;
+    A.q1_lo = A.q2_chunk = A.nq2ch = 0;
+    hipLaunchKernelGGL((k_lw4<R, ND, BLK, false>), dim3(A.nchunks, q0_count), dim3(BLK), dyn, c.stream, A);
     SPR_HIP(hipGetLastError());
     SPR_HIP(hipEventRecord(c.ev1, c.stream));
     if ((rc = launch_reduce_chunks(c, c.partial, q0_count, A.nchunks, d_lw))) return rc;
@@ -238,6 +269,50 @@ static int launch_lw4_t(Ctx &c, int q0_first, int q0_count, double sigma, double
     c.launches += 1;
     c.kernel_ms += ms;
     return SPR_OK;
+}
+
+template <int ND>
+static int launch_w4_rows_t(Ctx &c, int qi, int q1_lo, int nq1, double *d_w) {
+    constexpr int BLK = 256;
+    Lw4Args A{};
+    A.q0_grid = c.irr_grid + qi;
+    A.nq = c.nq;
+    A.L = c.L;
+    A.m0 = c.mesh[0];
+    A.m1 = c.mesh[1];
+    A.m2 = c.mesh[2];
+    A.blk_start = c.fc4.blk_start;
+    A.tup_site = c.fc4.tup_site;
+    A.fc = c.fc4.fc64;
+    A.site = c.fc4.site;
+    A.nsite = c.fc4.nsite;
+    A.phase = c.phase64;
+    A.evec = c.evec;
+    A.omega = c.omega;
+    A.inv_omega = c.inv_omega;
+    A.nbe = c.nbe;
+    A.factor = lw4_factor(c);
+    A.wst = d_w;
+    A.q1_lo = q1_lo;
+    // enough workgroups to fill the device: split the q2 range of every q1
+    const long target = 4L * c.ncu;
+    A.nq2ch = (int)std::min<long>(c.nq, std::max<long>(1, (target + nq1 - 1) / nq1));
+    A.q2_chunk = (c.nq + A.nq2ch - 1) / A.nq2ch;
+    A.nq2ch = (c.nq + A.q2_chunk - 1) / A.q2_chunk;
+    const size_t dyn = (size_t)((3 * A.nsite + 3) & ~3) * sizeof(int) + (size_t)c.L * sizeof(cplx_t<double>);
+    hipLaunchKernelGGL((k_lw4<double, ND, BLK, true>), dim3(nq1 * A.nq2ch, 1), dim3(BLK), dyn, c.stream, A);
+    SPR_HIP(hipGetLastError());
+    return SPR_OK;
+}
+
+int launch_w4_rows(Ctx &c, int qi, int q1_lo, int nq1, double *d_w) {
+    switch (c.nd) {
+        case 3: return launch_w4_rows_t<3>(c, qi, q1_lo, nq1, d_w);
+        case 6: return launch_w4_rows_t<6>(c, qi, q1_lo, nq1, d_w);
+        default:
+            set_error("four-phonon path: Ndof " + std::to_string(c.nd) + " not built (3, 6)");
+            return SPR_E_UNSUPPORTED;
+    }
 }
 
 int launch_lw4(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw) {

@@ -298,7 +298,7 @@ extern "C" int spr_ctx_destroy(spr_ctx *h) {
                     (void *)c.irr_grid, (void *)c.partial, (void *)c.out, (void *)c.counter,
                     (void *)c.d_pairs, (void *)c.d_item_q0, (void *)c.d_item_off, (void *)c.d_item_cnt,
                     (void *)c.d_item_first,
-                    (void *)c.ipartial, (void *)c.wout, (void *)c.tetra, (void *)c.partner, (void *)c.tout,
+                    (void *)c.ipartial, (void *)c.wout, (void *)c.tetra, (void *)c.partner, (void *)c.tout, (void *)c.w4buf,
                     (void *)c.k_terms, (void *)c.k_io, (void *)c.k_rot, (void *)c.k_pos_full, (void *)c.k_pos_irr,
                     (void *)c.k_mult, (void *)c.bte.d_coord, (void *)c.bte.d_map, (void *)c.bte.d_irr,
                     (void *)c.bte.d_tetra, (void *)c.bte.d_W, (void *)c.bte.d_F, (void *)c.bte.d_out,
@@ -523,6 +523,37 @@ extern "C" int spr_lw3_tetra(spr_ctx *h, int i0, double *lw, double *p3_q0) {
     SPR_HIP(hipMemcpy(lw, c.tout, c.nd * sizeof(double), hipMemcpyDeviceToHost));
     SPR_HIP(hipMemcpy(p3_q0, c.tout + c.nd, c.nd * sizeof(double), hipMemcpyDeviceToHost));
     return SPR_OK;
+}
+
+static int lw4_tetra_common(spr_ctx *h, int first, int count, double *lw, const char *who) {
+    if (!h || !lw) {
+        set_error(std::string(who) + ": bad argument");
+        return SPR_E_ARG;
+    }
+    Ctx &c = h->c;
+    if (first < 0 || count < 1 || first + count > c.nq0) {
+        set_error(std::string(who) + ": q0 index out of range");
+        return SPR_E_ARG;
+    }
+    if (c.fc4.nterm == 0 && c.fc4.order == 0) {
+        set_error(std::string(who) + ": no FC4 was given to spr_ctx_create");
+        return SPR_E_STATE;
+    }
+    SPR_HIP(hipSetDevice(c.device));
+    int rc;
+    if (!c.tout) SPR_HIP(hipMalloc((void **)&c.tout, (size_t)2 * c.nq0 * c.nd * sizeof(double)));
+    for (int qi = first; qi < first + count; ++qi)
+        if ((rc = spr::launch_lw4_tetra(c, qi, c.tout + (size_t)(qi - first) * c.nd))) return rc;
+    SPR_HIP(hipMemcpy(lw, c.tout, (size_t)count * c.nd * sizeof(double), hipMemcpyDeviceToHost));
+    return SPR_OK;
+}
+
+extern "C" int spr_lw4_tetra(spr_ctx *h, int i0, double *lw) {
+    return lw4_tetra_common(h, i0, 1, lw, "spr_lw4_tetra");
+}
+
+extern "C" int spr_lw4_tetra_all(spr_ctx *h, double *lw_all) {
+    return lw4_tetra_common(h, 0, h ? h->c.nq0 : 0, lw_all, "spr_lw4_tetra_all");
 }
 
 extern "C" int spr_lw3_tetra_all(spr_ctx *h, double *lw_all, double *p3_all) {

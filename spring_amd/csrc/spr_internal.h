@@ -137,6 +137,8 @@ struct Ctx {
     int *tetra = nullptr, *partner = nullptr;   // (6 nq, 4) tetrahedra; q2 partner of every q1
     std::vector<int> h_tetra;
     double *tout = nullptr;        // tetrahedron outputs: lw then P3_q0, (2, nq0, nd)
+    double *w4buf = nullptr;       // W4 rows of one q1 chunk (four-phonon tetrahedron path)
+    size_t w4buf_cap = 0;
     // kappa buffers, allocated on first use: per-mode terms, lw / mode sums, index tables
     double *k_terms = nullptr, *k_io = nullptr, *k_rot = nullptr;
     int *k_pos_full = nullptr, *k_pos_irr = nullptr, *k_mult = nullptr;
@@ -174,6 +176,12 @@ int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, int upi, do
 constexpr int kMaxUnitsPerItem = 4;   // partial rows reserved per item
 int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_mode_k);
 int launch_lw3_tetra(Ctx &c, int qi, double *d_lw, double *d_p3);
+// device tetrahedra (6 nq, 4) of the context (uploaded on first use)
+int ensure_tetra(Ctx &c);
+// FP64 W4 of irreducible q0 `qi` for q1 in [q1_lo, q1_lo + nq1) and every q2:
+// d_w[q1 - q1_lo][q2][s0][s1][s2][s3]
+int launch_w4_rows(Ctx &c, int qi, int q1_lo, int nq1, double *d_w);
+int launch_lw4_tetra(Ctx &c, int qi, double *d_lw);
 
 }  // namespace spr
 

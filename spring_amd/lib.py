@@ -20,7 +20,8 @@ SYMBOLS = [
     "spr_lw3_all", "spr_lw4", "spr_lw4_all", "spr_kappa_rta", "spr_kernel_time",
     "spr_kernel_time_reset", "spr_lw3_pairs", "spr_v3_w_pairs", "spr_mlp_create", "spr_mlp_destroy",
     "spr_mlp_num_params", "spr_mlp_get_params", "spr_mlp_set_params", "spr_mlp_grad",
-    "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all", "spr_mlp_score",
+    "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all",
+    "spr_lw4_tetra", "spr_lw4_tetra_all", "spr_mlp_score",
     "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve", "spr_get_phonon",
 ]
 
@@ -207,6 +208,17 @@ class GpuContext:
         p3 = np.zeros((self.nq0, self.nd))
         self._check(self.lib.spr_lw3_tetra_all(self.h, _ptr(lw), _ptr(p3)), "spr_lw3_tetra_all")
         return lw, p3
+
+    def lw4_tetra(self, i0: int):
+        """Four-phonon tetrahedron-method linewidth sum of irreducible q0 number i0 (FP64)."""
+        lw = np.zeros(self.nd)
+        self._check(self.lib.spr_lw4_tetra(self.h, ct.c_int(i0), _ptr(lw)), "spr_lw4_tetra")
+        return lw
+
+    def lw4_tetra_all(self):
+        lw = np.zeros((self.nq0, self.nd))
+        self._check(self.lib.spr_lw4_tetra_all(self.h, _ptr(lw)), "spr_lw4_tetra_all")
+        return lw
 
     # ---- iterative Boltzmann transport (time-reversal irreducible points) ----
     def bte_irr(self):

@@ -135,6 +135,16 @@ def test_get_phonon_rejects_bad_arguments_on_the_host():
         get_phonon(m.crystal, m.fc2, np.zeros((1, 3)), np.zeros((1, 3), int), -1.0)
 
 
+def test_lw4_tetra_rejects_a_null_context_on_the_host():
+    import ctypes as ct
+    from spring_amd import lib
+    L = lib.load()
+    out = np.zeros(6)
+    assert L.spr_lw4_tetra(None, ct.c_int(0), out.ctypes.data_as(ct.c_void_p)) < 0
+    assert b"bad argument" in L.spr_last_error()
+    assert L.spr_lw4_tetra_all(None, out.ctypes.data_as(ct.c_void_p)) < 0
+
+
 def test_head_tail_draw_is_unbiased():
     """Deterministic head + sampled tail: the weighted sum is an unbiased estimate of the full sum
     (seeded Monte Carlo mean within 4 standard errors), and a budget covering every q1 is exact."""

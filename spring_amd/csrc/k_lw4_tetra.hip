@@ -16,14 +16,18 @@
 // to read the other orders; with an index-permutation-symmetric FC4 that is the same number as
 // W4 of the ordered triple, computed here.
 //
-// Mapping: a thread owns (q1, tetrahedron, s1 s2 s3) and walks s0; the vertex energies do not
-// depend on s0, so each process's vertices are sorted once (stable, with the vertex's n-product
-// and W address as payload) and reused for every s0.  W reads are contiguous in (s1 s2 s3).
+// Mapping: a thread owns (q1, s1 s2 s3, tetrahedron), tetrahedron fastest across lanes, and
+// walks the processes, then s0.  The vertex energies do not depend on s0, so each process's
+// vertices are sorted once (stable, with the vertex's n-product and W address as payload) and
+// reused for every s0; W is read only where Mq falls inside the tetrahedron's energy range.
+// Neighbouring tetrahedra have alike energy ranges, so whole waves skip a process together,
+// and they share vertices, so the W reads of a wave hit the same cache lines.
 // Grid-stride over items; per-workgroup partials reduced in a fixed order (bitwise
 // reproducible).  All arithmetic FP64.
 #include <algorithm>
 #include <cstdlib>
 #include <string>
+#include <vector>
 
 #include "spr_internal.h"
 
@@ -35,17 +39,19 @@ constexpr int T4BLK = 256;
 
 struct Tet4Args {
     const double *W;            // [nq1][nq][ND^4] rows of q1 in [q1_lo, q1_lo + nq1)
-    const int *tetra;           // [6 nq][4]
+    const int4 *tetra;          // [6 nq] grid indices of the four vertices
+    const int *kpack;           // [nq] grid coordinates k0 | k1 << 10 | k2 << 20
     const double *omega, *nbe;  // [nq][ND]
     int nq, m0, m1, m2;
     int i0, q1_lo, nq1;
     double *partial;            // [gridDim.x][ND] of this chunk
 };
 
-// one vertex record: energy, s0-independent part of F, W offset of s0 = 0
+// one vertex record: energy, s0-independent part of F, W offset of s0 = 0 (a chunk holds at
+// most 2^31 doubles)
 struct Vx {
     double M, a;
-    long off;
+    int off;
 };
 
 __device__ __forceinline__ void cswap(Vx &x, Vx &y) {
@@ -116,65 +122,87 @@ __global__ __launch_bounds__(T4BLK) void k_lw4_tetra(Tet4Args A) {
     constexpr int ND2 = ND * ND, ND3 = ND2 * ND, ND4 = ND3 * ND;
     const int tid = threadIdx.x;
     const int ntet = 6 * A.nq;
-    const long total = (long)A.nq1 * ntet * ND3;
+    const int total = A.nq1 * ntet * ND3;     // the host keeps this below 2^31
     const int i0 = A.i0;
-    const int k00 = i0 % A.m0, k01 = (i0 / A.m0) % A.m1, k02 = i0 / (A.m0 * A.m1);
+    const int kp0 = A.kpack[i0];
     const int s0_lo = i0 == 0 ? 3 : 0;
+    __shared__ double s_w0[ND], s_rn0[ND];
+    if (tid < ND) {
+        s_w0[tid] = A.omega[(size_t)i0 * ND + tid];
+        s_rn0[tid] = 1.0 / A.nbe[(size_t)i0 * ND + tid];
+    }
+    __syncthreads();
     double acc[ND];
 #pragma unroll
     for (int s = 0; s < ND; ++s) acc[s] = 0.0;
-    for (long it = (long)blockIdx.x * T4BLK + tid; it < total; it += (long)gridDim.x * T4BLK) {
-        const int p = (int)(it % ND3);
-        const long r = it / ND3;
-        const int t = (int)(r % ntet), q1l = (int)(r / ntet);
+    for (int it = blockIdx.x * T4BLK + tid; it < total; it += gridDim.x * T4BLK) {
+        // tetrahedron fastest: the lanes of a wave hold neighbouring tetrahedra of one branch set,
+        // whose energy ranges are alike, so whole waves skip a process together
+        const int r = it / ntet, t = it - r * ntet;
+        const int q1l = r / ND3, p = r - q1l * ND3;
         const int q1 = A.q1_lo + q1l;
         const int s1 = p / ND2, s2 = (p / ND) % ND, s3 = p % ND;
-        const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
+        // coordinates of -(q0 + q1) on the grid, before adding q2 and wrapping
+        const int kp1 = A.kpack[q1];
+        const int b0 = (kp0 & 1023) + (kp1 & 1023), b1 = ((kp0 >> 10) & 1023) + ((kp1 >> 10) & 1023),
+                  b2 = (kp0 >> 20) + (kp1 >> 20);
         const double w1 = A.omega[(size_t)q1 * ND + s1], n1 = A.nbe[(size_t)q1 * ND + s1];
-        Vx P1[4], P2[4], P3[4];
+        double w2[4], w3[4], n2[4], n3[4];
+        int off[4];
+        const int4 tv = A.tetra[t];
+        const int qv[4] = {tv.x, tv.y, tv.z, tv.w};
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int q2 = A.tetra[4 * t + v];
-            const int k20 = q2 % A.m0, k21 = (q2 / A.m0) % A.m1, k22 = q2 / (A.m0 * A.m1);
-            int a0 = -(k00 + k10 + k20) % A.m0, a1 = -(k01 + k11 + k21) % A.m1, a2 = -(k02 + k12 + k22) % A.m2;
+            const int q2 = qv[v];
+            const int kp2 = A.kpack[q2];
+            // a = -(k0 + k1 + k2) mod m with 0 <= k0 + k1 + k2 <= 3 (m - 1)
+            int a0 = -(b0 + (kp2 & 1023)), a1 = -(b1 + ((kp2 >> 10) & 1023)), a2 = -(b2 + (kp2 >> 20));
+            a0 += a0 < 0 ? A.m0 : 0;
+            a0 += a0 < 0 ? A.m0 : 0;
             a0 += a0 < 0 ? A.m0 : 0;
             a1 += a1 < 0 ? A.m1 : 0;
+            a1 += a1 < 0 ? A.m1 : 0;
+            a1 += a1 < 0 ? A.m1 : 0;
+            a2 += a2 < 0 ? A.m2 : 0;
+            a2 += a2 < 0 ? A.m2 : 0;
             a2 += a2 < 0 ? A.m2 : 0;
             const int q3 = a0 + A.m0 * (a1 + A.m1 * a2);
-            const double w2 = A.omega[(size_t)q2 * ND + s2], n2 = A.nbe[(size_t)q2 * ND + s2];
-            const double w3 = A.omega[(size_t)q3 * ND + s3], n3 = A.nbe[(size_t)q3 * ND + s3];
-            const long off = ((long)q1l * A.nq + q2) * ND4 + p;
-            P1[v] = Vx{w1 + w2 + w3, n1 * n2 * n3, off};
-            P2[v] = Vx{w2 + w3 - w1, (1.0 + n1) * n2 * n3, off};
-            P3[v] = Vx{w3 - w1 - w2, (1.0 + n1) * (1.0 + n2) * n3, off};
+            w2[v] = A.omega[q2 * ND + s2];
+            n2[v] = A.nbe[q2 * ND + s2];
+            w3[v] = A.omega[q3 * ND + s3];
+            n3[v] = A.nbe[q3 * ND + s3];
+            off[v] = (q1l * A.nq + q2) * ND4 + p;
         }
-        sort4(P1);
-        sort4(P2);
-        sort4(P3);
-        const double M1[4] = {P1[0].M, P1[1].M, P1[2].M, P1[3].M};
-        const double M2[4] = {P2[0].M, P2[1].M, P2[2].M, P2[3].M};
-        const double M3[4] = {P3[0].M, P3[1].M, P3[2].M, P3[3].M};
+        // one process at a time (fewer live registers); its vertices sorted once for all s0
 #pragma unroll 1
-        for (int s0 = s0_lo; s0 < ND; ++s0) {
-            const double mq = A.omega[(size_t)i0 * ND + s0], n0 = A.nbe[(size_t)i0 * ND + s0];
-            const long so = (long)s0 * ND3;
-            double F1[4], F2[4], F3[4];
+        for (int k = 0; k < 3; ++k) {
+            Vx P[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                F1[v] = A.W[P1[v].off + so] * (P1[v].a / n0);
-                F2[v] = A.W[P2[v].off + so] * (P2[v].a / n0);
-                F3[v] = A.W[P3[v].off + so] * (P3[v].a / n0);
+                const double m = k == 0 ? w1 + w2[v] + w3[v] : k == 1 ? w2[v] + w3[v] - w1 : w3[v] - w1 - w2[v];
+                const double a = k == 0 ? n1 * n2[v] * n3[v]
+                                        : k == 1 ? (1.0 + n1) * n2[v] * n3[v] : (1.0 + n1) * (1.0 + n2[v]) * n3[v];
+                P[v] = Vx{m, a, off[v]};
             }
-            const double t1 = (1.0 / 6.0) * tetra4(mq, M1, F1);
-            const double t2 = 0.5 * tetra4(mq, M2, F2);
-            const double t3 = 0.5 * tetra4(mq, M3, F3);
+            sort4(P);
+            const double M[4] = {P[0].M, P[1].M, P[2].M, P[3].M};
+            const double wk = k == 0 ? 1.0 / 6.0 : 0.5;
+#pragma unroll 1
+            for (int s0 = s0_lo; s0 < ND; ++s0) {
+                const double mq = s_w0[s0];
+                // a process contributes only for M0 <= mq <= M3 (the union of the three cases);
+                // W and F are formed only then
+                if (!(mq >= M[0] && mq <= M[3])) continue;
+                const double rn0 = s_rn0[s0];
+                const int so = s0 * ND3;
+                double F[4];
 #pragma unroll
-            for (int s = 0; s < ND; ++s)
-                if (s == s0) {
-                    acc[s] += t1;
-                    acc[s] += t2;
-                    acc[s] += t3;
-                }
+                for (int v = 0; v < 4; ++v) F[v] = A.W[P[v].off + so] * (P[v].a * rn0);
+                const double tk = wk * tetra4(mq, M, F);
+#pragma unroll
+                for (int s = 0; s < ND; ++s)
+                    if (s == s0) acc[s] += tk;
+            }
         }
     }
     __shared__ double s_red[T4BLK];
@@ -189,6 +217,22 @@ __global__ __launch_bounds__(T4BLK) void k_lw4_tetra(Tet4Args A) {
         }
         if (tid == 0) A.partial[(size_t)blockIdx.x * ND + s] = s_red[0];
     }
+}
+
+// out[s] = sum over n partials[k][s] in a fixed order: one workgroup per s, strided partial sums
+// per thread, then a tree (bitwise reproducible)
+__global__ __launch_bounds__(256) void k_reduce_tree(const double *partial, int n, int nd, double *out) {
+    const int s = blockIdx.x, tid = threadIdx.x;
+    double a = 0.0;
+    for (int k = tid; k < n; k += 256) a += partial[(size_t)k * nd + s];
+    __shared__ double r[256];
+    r[tid] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) r[tid] += r[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) out[s] = r[0];
 }
 
 // bytes of W4 rows held at once (a chunk of q1; a q1 row is nq * Ndof^4 doubles): 2 GiB, or
@@ -208,9 +252,24 @@ int launch_lw4_tetra(Ctx &c, int qi, double *d_lw) {
     }
     int rc;
     if ((rc = ensure_tetra(c))) return rc;
+    if (c.mesh[0] > 1024 || c.mesh[1] > 1024 || c.mesh[2] > 1024) {
+        set_error("four-phonon tetrahedron path: mesh dimensions above 1024");
+        return SPR_E_UNSUPPORTED;
+    }
+    if (!c.kpack) {
+        std::vector<int> kp(c.nq);
+        for (int q = 0; q < c.nq; ++q)
+            kp[q] = (q % c.mesh[0]) | ((q / c.mesh[0]) % c.mesh[1]) << 10 | (q / (c.mesh[0] * c.mesh[1])) << 20;
+        SPR_HIP(hipMalloc((void **)&c.kpack, (size_t)c.nq * sizeof(int)));
+        SPR_HIP(hipMemcpy(c.kpack, kp.data(), (size_t)c.nq * sizeof(int), hipMemcpyHostToDevice));
+    }
     const size_t nd4 = (size_t)c.nd * c.nd * c.nd * c.nd;
     const size_t row = (size_t)c.nq * nd4 * sizeof(double);
-    const int nq1c = (int)std::max<size_t>(1, std::min<size_t>((size_t)c.nq, w4_chunk_bytes() / row));
+    // chunk: the byte cap, W offsets below 2^31 and items (nq1 x 6 nq x Ndof^3) below 2^31
+    const size_t item_row = (size_t)6 * c.nq * c.nd * c.nd * c.nd;
+    const int nq1c = (int)std::max<size_t>(
+        1, std::min<size_t>({(size_t)c.nq, w4_chunk_bytes() / row, (((size_t)1 << 31) - 1) / (row / sizeof(double)),
+                             (((size_t)1 << 31) - 1) / item_row}));
     const int ncalls = (c.nq + nq1c - 1) / nq1c;
     const long items = (long)nq1c * 6 * c.nq * c.nd * c.nd * c.nd;
     const int grid = (int)std::min<long>(8L * c.ncu, (items + T4BLK - 1) / T4BLK);
@@ -220,7 +279,8 @@ int launch_lw4_tetra(Ctx &c, int qi, double *d_lw) {
     const int i0 = c.h_irr_grid[qi];
     Tet4Args A;
     A.W = c.w4buf;
-    A.tetra = c.tetra;
+    A.tetra = reinterpret_cast<const int4 *>(c.tetra);
+    A.kpack = c.kpack;
     A.omega = c.omega;
     A.nbe = c.nbe;
     A.nq = c.nq;
@@ -242,7 +302,8 @@ int launch_lw4_tetra(Ctx &c, int qi, double *d_lw) {
         SPR_HIP(hipGetLastError());
     }
     SPR_HIP(hipEventRecord(c.ev1, c.stream));
-    if ((rc = launch_reduce_chunks(c, c.partial, 1, ncalls * grid, d_lw))) return rc;
+    hipLaunchKernelGGL(k_reduce_tree, dim3(c.nd), dim3(256), 0, c.stream, c.partial, ncalls * grid, c.nd, d_lw);
+    SPR_HIP(hipGetLastError());
     SPR_HIP(hipStreamSynchronize(c.stream));
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess) {

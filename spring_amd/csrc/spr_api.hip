@@ -298,7 +298,7 @@ extern "C" int spr_ctx_destroy(spr_ctx *h) {
                     (void *)c.irr_grid, (void *)c.partial, (void *)c.out, (void *)c.counter,
                     (void *)c.d_pairs, (void *)c.d_item_q0, (void *)c.d_item_off, (void *)c.d_item_cnt,
                     (void *)c.d_item_first,
-                    (void *)c.ipartial, (void *)c.wout, (void *)c.tetra, (void *)c.partner, (void *)c.tout, (void *)c.w4buf,
+                    (void *)c.ipartial, (void *)c.wout, (void *)c.tetra, (void *)c.partner, (void *)c.tout, (void *)c.w4buf, (void *)c.kpack,
                     (void *)c.k_terms, (void *)c.k_io, (void *)c.k_rot, (void *)c.k_pos_full, (void *)c.k_pos_irr,
                     (void *)c.k_mult, (void *)c.bte.d_coord, (void *)c.bte.d_map, (void *)c.bte.d_irr,
                     (void *)c.bte.d_tetra, (void *)c.bte.d_W, (void *)c.bte.d_F, (void *)c.bte.d_out,

@@ -138,6 +138,7 @@ struct Ctx {
     std::vector<int> h_tetra;
     double *tout = nullptr;        // tetrahedron outputs: lw then P3_q0, (2, nq0, nd)
     double *w4buf = nullptr;       // W4 rows of one q1 chunk (four-phonon tetrahedron path)
+    int *kpack = nullptr;          // grid coordinates of every q, k0 | k1 << 10 | k2 << 20
     size_t w4buf_cap = 0;
     // kappa buffers, allocated on first use: per-mode terms, lw / mode sums, index tables
     double *k_terms = nullptr, *k_io = nullptr, *k_rot = nullptr;

@@ -11,8 +11,11 @@
 ! them (through libspring_h5, since the HDF5 Fortran module of this image is not readable by
 ! this compiler).
 !
-! usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter] [gpuph]
+! usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter|tetra] [gpuph]
 !   loop: one spr_lw3 call per q0 (the reference's q0 loop); all: one spr_lw3_all launch;
+!   tetra: the reference's default delta, the tetrahedron method, per q0 for the three-phonon
+!   (CalculateLwTetrahedron, main.f90:585-600) and the four-phonon sum (TetrahedronIntegration4th,
+!   main.f90:655-660), FP64;
 !   iter: the iterative solution of src/Kappa/Src_iterative/main.f90:243-361 (tetrahedron delta,
 !   time-reversal irreducible points) instead, printing the kappa of F0 and the converged kappa;
 !   gpuph: the phonons of the mesh are computed on the device from FC2 (get_phonon_gpu, as
@@ -32,10 +35,11 @@ program kappa_gpu
     real(dp), allocatable   :: mass(:), fC3(:,:,:,:,:,:), fC4(:,:,:,:,:,:,:,:), Rxyz(:,:,:)
     complex(dp), allocatable :: Evec(:,:,:)
     real(dp), allocatable   :: omega(:,:), OnebyOmega(:,:), nBE(:,:), grp_vel(:,:,:)
-    real(dp), allocatable   :: LW_Allq0(:,:), lw4(:,:), Mode_k(:,:), lw(:)
+    real(dp), allocatable   :: LW_Allq0(:,:), lw4(:,:), Mode_k(:,:), lw(:), P3_q0(:)
     real(dp)                :: kappa(9)
     type(gpu_ctx)           :: ctx
-    logical                 :: HeaderWrite, loop_mode, iter_mode, gpu_phonons
+    logical                 :: HeaderWrite, loop_mode, iter_mode, tetra_mode, gpu_phonons
+    real(dp)                :: P3
     integer, allocatable    :: atmNum2(:), atmIndx2(:,:,:)
     real(dp), allocatable   :: fC2(:,:,:,:,:), q_cart(:,:)
     real(dp)                :: latvec(3,3), freq_min, freq_max
@@ -54,7 +58,7 @@ program kappa_gpu
 
     call get_command_argument(1, dir)
     if ( len_trim(dir) == 0 ) then
-        write(*, *) 'usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter] [gpuph]'
+        write(*, *) 'usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter|tetra] [gpuph]'
         ERROR STOP
     end if
     prec = SPR_PREC_FP64
@@ -64,6 +68,7 @@ program kappa_gpu
     call get_command_argument(3, arg)
     loop_mode = trim(arg) == 'loop'
     iter_mode = trim(arg) == 'iter'
+    tetra_mode = trim(arg) == 'tetra'
     call get_command_argument(4, arg)
     gpu_phonons = trim(arg) == 'gpuph'
 
@@ -156,6 +161,10 @@ program kappa_gpu
         stop
     end if
 
+    if ( tetra_mode ) then
+        allocate( tet(4, 6 * Nq) )
+        call rd_i2(trim(dir)//'/tetra.bin', tet)
+    end if
     if ( FourPhonon /= 0 ) then
         inquire(file=trim(fc4file), exist=have_h5)
         if ( have_h5 ) then
@@ -170,8 +179,17 @@ program kappa_gpu
             read(u) fC4
             close(u)
         end if
+        if ( tetra_mode ) then
+            call gpu_ctx_create(ctx, Nbasis, mass, vol, atmNum3, atmIndx3, fC3, mesh, q_int, irr_grid, pos, &
+                              & mult, Rxyz, Evec, omega, OnebyOmega, nBE, grp_vel, 0, atmNum4, atmIndx4, fC4, &
+                              & tetrahedrons=tet)
+        else
+            call gpu_ctx_create(ctx, Nbasis, mass, vol, atmNum3, atmIndx3, fC3, mesh, q_int, irr_grid, pos, &
+                              & mult, Rxyz, Evec, omega, OnebyOmega, nBE, grp_vel, 0, atmNum4, atmIndx4, fC4)
+        end if
+    else if ( tetra_mode ) then
         call gpu_ctx_create(ctx, Nbasis, mass, vol, atmNum3, atmIndx3, fC3, mesh, q_int, irr_grid, pos, &
-                          & mult, Rxyz, Evec, omega, OnebyOmega, nBE, grp_vel, 0, atmNum4, atmIndx4, fC4)
+                          & mult, Rxyz, Evec, omega, OnebyOmega, nBE, grp_vel, 0, tetrahedrons=tet)
     else
         call gpu_ctx_create(ctx, Nbasis, mass, vol, atmNum3, atmIndx3, fC3, mesh, q_int, irr_grid, pos, &
                           & mult, Rxyz, Evec, omega, OnebyOmega, nBE, grp_vel, 0)
@@ -179,7 +197,17 @@ program kappa_gpu
 
     ! ---------------------------- three-phonon linewidths (main.f90:552-715) ----------------------------
     allocate( LW_Allq0(Ndof, Nq0), lw(Ndof) )
-    if ( loop_mode ) then
+    if ( tetra_mode ) then
+        allocate( P3_q0(Ndof) )
+        do i0 = 1, Nq0
+            call CalculateLwTetrahedron_gpu(ctx, i0, lw, P3_q0, P3)
+            LW_Allq0(:, i0) = lw
+            if ( FourPhonon /= 0 ) then
+                call TetrahedronIntegration4th_gpu(ctx, i0, lw)
+                LW_Allq0(:, i0) = LW_Allq0(:, i0) + 0.5_dp * lw      ! main.f90:660
+            end if
+        end do
+    else if ( loop_mode ) then
         do i0 = 1, Nq0
             call CalculateLinewidth_gpu(ctx, i0, Sigma, prec, lw)
             LW_Allq0(:, i0) = lw    !** third-order linewidth, no 0.5 factor (main.f90:585) **!
@@ -187,7 +215,7 @@ program kappa_gpu
     else
         call LinewidthAllq0_gpu(ctx, Sigma, prec, LW_Allq0)
     end if
-    if ( FourPhonon /= 0 ) then
+    if ( FourPhonon /= 0 .and. .not. tetra_mode ) then
         allocate( lw4(Ndof, Nq0) )
         call FourPhononLw_gpu(ctx, Sigma, prec, lw4)
         LW_Allq0 = LW_Allq0 + 0.5_dp * lw4      ! main.f90:660

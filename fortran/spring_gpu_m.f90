@@ -117,6 +117,19 @@ module spring_gpu_m
             real(c_double)          :: lw(*), p3(*)
         end function spr_lw3_tetra_all
 
+        integer(c_int) function spr_lw4_tetra(h, i0, lw) bind(C, name="spr_lw4_tetra")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            integer(c_int), value   :: i0
+            real(c_double)          :: lw(*)
+        end function spr_lw4_tetra
+
+        integer(c_int) function spr_lw4_tetra_all(h, lw) bind(C, name="spr_lw4_tetra_all")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            real(c_double)          :: lw(*)
+        end function spr_lw4_tetra_all
+
         integer(c_int) function spr_lw4_all(h, sigma, prec, lw) bind(C, name="spr_lw4_all")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value      :: h
@@ -187,7 +200,7 @@ module spring_gpu_m
 
     public :: gpu_ctx_create, gpu_ctx_destroy, ScatterProbAllq1_gpu, CalculateLinewidth_gpu, &
             & LinewidthAllq0_gpu, FourPhononLw_gpu, kappa_noIter_gpu, spr_error_text, &
-            & CalculateLwTetrahedron_gpu, IrrQ_tr_gpu, TetrahedronW_gpu, TetrahedronIter_gpu, &
+            & CalculateLwTetrahedron_gpu, TetrahedronIntegration4th_gpu, FourPhononLwTetra_gpu, IrrQ_tr_gpu, TetrahedronW_gpu, TetrahedronIter_gpu, &
             & kappa_iter_gpu, get_phonon_gpu
 
 contains
@@ -322,6 +335,24 @@ contains
         call check(spr_lw3_tetra(ctx%h, int(i0 - 1, c_int), lw, P3_q0), 'CalculateLwTetrahedron_gpu')
         P3 = sum(P3_q0)
     end subroutine CalculateLwTetrahedron_gpu
+
+    ! Four-phonon tetrahedron-method sum lw(Ndof) of irreducible q0 number i0 (1-based), the output
+    ! of TetrahedronIntegration4th (Tetrahedron4th_sep.f90:37-170) summed over every q1 (the
+    ! reference sums the images' parts); the caller adds 0.5*lw (main.f90:660).  FP64.  Needs FC4
+    ! and the tetrahedra at gpu_ctx_create.
+    subroutine TetrahedronIntegration4th_gpu(ctx, i0, lw)
+        type(gpu_ctx), intent(in)   :: ctx
+        integer, intent(in)         :: i0
+        real(dp), intent(out)       :: lw(:)
+        call check(spr_lw4_tetra(ctx%h, int(i0 - 1, c_int), lw), 'TetrahedronIntegration4th_gpu')
+    end subroutine TetrahedronIntegration4th_gpu
+
+    ! lw4(Ndof, Nq0) of every irreducible q0 with the tetrahedron method (as above).
+    subroutine FourPhononLwTetra_gpu(ctx, lw4)
+        type(gpu_ctx), intent(in)   :: ctx
+        real(dp), intent(out)       :: lw4(:,:)
+        call check(spr_lw4_tetra_all(ctx%h, lw4), 'FourPhononLwTetra_gpu')
+    end subroutine FourPhononLwTetra_gpu
 
     ! Raw four-phonon sum lw4(Ndof, Nq0); the caller adds 0.5*lw4 (main.f90:660).
     subroutine FourPhononLw_gpu(ctx, Sigma, prec, lw4)

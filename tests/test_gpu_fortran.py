@@ -127,3 +127,28 @@ def test_fortran_driver_reads_fc_h5_and_writes_kappa_h5(rocksalt, tmp_path):
     assert r.returncode == 0 and "FC2nd_NaClsynth300.0K_F.h5" in r.stdout
     w, _ = bundle.read_linewidth_file(str(tmp_path / "linewidth_IBZ_NaClsynth300.0K.txt"), 6)
     assert np.allclose(w[1:], ph.omega[qm.irr_idx][1:], rtol=1e-11, atol=0)
+
+
+def test_fortran_driver_tetrahedron_three_and_four_phonon(rocksalt, tmp_path):
+    """kappa_gpu ... tetra on a bundle with FC4: per q0, CalculateLwTetrahedron_gpu plus
+    0.5 x TetrahedronIntegration4th_gpu (main.f90:585-600, 655-660), against the oracle."""
+    from spring_amd import bundle
+    from oracle import oracle as O
+    exe = os.path.join(ROOT, "fortran", "kappa_gpu")
+    assert os.path.exists(exe), "fortran/kappa_gpu is not built"
+    qm, ph = problem(rocksalt, (3, 3, 3))
+    bundle.write_bundle(str(tmp_path / "b"), rocksalt, qm, ph, 300.0, 1.0, prefix="NaClsynth",
+                        with_fc4=True)
+    r = subprocess.run([exe, str(tmp_path / "b"), "fp64", "tetra"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    _, lw = bundle.read_linewidth_file(str(tmp_path / "linewidth_IBZ_NaClsynth300.0K.txt"), 6)
+    inp3, inp4 = O.Inputs(rocksalt, qm, ph), O.Inputs(rocksalt, qm, ph, which=4)
+    ref = []
+    for g in qm.irr_idx:
+        _, W = O.lw3_gauss(inp3, int(g), 1.0, want_w=True)
+        l3, _ = O.lw3_tetra(qm, ph, int(g), W)
+        ref.append(l3 + 0.5 * O.lw4_tetra(inp4, qm, int(g), threads=8, chunk=4))
+    ref = np.array(ref)
+    assert rel_err(lw, ref, 1e-8) < 1e-10

@@ -122,7 +122,12 @@ __global__ __launch_bounds__(T4BLK) void k_lw4_tetra(Tet4Args A) {
     constexpr int ND2 = ND * ND, ND3 = ND2 * ND, ND4 = ND3 * ND;
     const int tid = threadIdx.x;
     const int ntet = 6 * A.nq;
-    const int total = A.nq1 * ntet * ND3;     // the host keeps this below 2^31
+    // workgroups go to the eight XCDs round robin (workgroup b to XCD b mod 8): XCD x takes the
+    // q1 rows x, x + 8, ..., so every W row is fetched into one XCD's L2 only
+    const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nlb = gridDim.x >> 3;
+    const int nrows = (A.nq1 - xcd + 7) / 8;
+    const int per_row = ntet * ND3;
+    const int total = nrows * per_row;        // the host keeps nq1 x per_row below 2^31
     const int i0 = A.i0;
     const int kp0 = A.kpack[i0];
     const int s0_lo = i0 == 0 ? 3 : 0;
@@ -132,14 +137,23 @@ __global__ __launch_bounds__(T4BLK) void k_lw4_tetra(Tet4Args A) {
         s_rn0[tid] = 1.0 / A.nbe[(size_t)i0 * ND + tid];
     }
     __syncthreads();
+    // range of Mq over the s0 that are summed: a process whose vertex energies miss it is skipped
+    double w0lo = s_w0[s0_lo], w0hi = s_w0[s0_lo];
+#pragma unroll
+    for (int s = 0; s < ND; ++s)
+        if (s >= s0_lo) {
+            w0lo = fmin(w0lo, s_w0[s]);
+            w0hi = fmax(w0hi, s_w0[s]);
+        }
     double acc[ND];
 #pragma unroll
     for (int s = 0; s < ND; ++s) acc[s] = 0.0;
-    for (int it = blockIdx.x * T4BLK + tid; it < total; it += gridDim.x * T4BLK) {
+    for (int it = lb * T4BLK + tid; it < total; it += nlb * T4BLK) {
         // tetrahedron fastest: the lanes of a wave hold neighbouring tetrahedra of one branch set,
         // whose energy ranges are alike, so whole waves skip a process together
-        const int r = it / ntet, t = it - r * ntet;
-        const int q1l = r / ND3, p = r - q1l * ND3;
+        const int row = it / per_row, rem = it - row * per_row;
+        const int p = rem / ntet, t = rem - p * ntet;
+        const int q1l = xcd + 8 * row;
         const int q1 = A.q1_lo + q1l;
         const int s1 = p / ND2, s2 = (p / ND) % ND, s3 = p % ND;
         // coordinates of -(q0 + q1) on the grid, before adding q2 and wrapping
@@ -177,13 +191,17 @@ __global__ __launch_bounds__(T4BLK) void k_lw4_tetra(Tet4Args A) {
 #pragma unroll 1
         for (int k = 0; k < 3; ++k) {
             Vx P[4];
+            double mlo = 0.0, mhi = 0.0;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const double m = k == 0 ? w1 + w2[v] + w3[v] : k == 1 ? w2[v] + w3[v] - w1 : w3[v] - w1 - w2[v];
                 const double a = k == 0 ? n1 * n2[v] * n3[v]
                                         : k == 1 ? (1.0 + n1) * n2[v] * n3[v] : (1.0 + n1) * (1.0 + n2[v]) * n3[v];
                 P[v] = Vx{m, a, off[v]};
+                mlo = v == 0 ? m : fmin(mlo, m);
+                mhi = v == 0 ? m : fmax(mhi, m);
             }
+            if (mhi < w0lo || mlo > w0hi) continue;
             sort4(P);
             const double M[4] = {P[0].M, P[1].M, P[2].M, P[3].M};
             const double wk = k == 0 ? 1.0 / 6.0 : 0.5;
@@ -272,7 +290,8 @@ int launch_lw4_tetra(Ctx &c, int qi, double *d_lw) {
                              (((size_t)1 << 31) - 1) / item_row}));
     const int ncalls = (c.nq + nq1c - 1) / nq1c;
     const long items = (long)nq1c * 6 * c.nq * c.nd * c.nd * c.nd;
-    const int grid = (int)std::min<long>(8L * c.ncu, (items + T4BLK - 1) / T4BLK);
+    // a multiple of 8 workgroups (the XCD split of the kernel)
+    const int grid = 8 * (int)std::max<long>(1, std::min<long>(c.ncu, (items + 8L * T4BLK - 1) / (8L * T4BLK)));
     if ((rc = ensure((void **)&c.w4buf, &c.w4buf_cap, (size_t)nq1c * row))) return rc;
     if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)ncalls * grid * c.nd * sizeof(double))))
         return rc;

@@ -95,22 +95,20 @@ def main():
     qm = mesh.make_mesh(m.crystal, (n, n, n))
     ph = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
     ctx = lib.GpuContext(m, qm, ph)
-    lw = ctx.lw3_all(args.sigma, "fp64")          # also builds the pair lists
-    kap, _ = ctx.kappa(lw, args.T, symm=True)
+    lw, kap, _ = ctx.lw3_kappa_all(args.sigma, "fp64", args.T)     # also builds the pair lists
     kappa_tts = time.perf_counter() - t0
 
     nd = m.crystal.ndof
     triplets = qm.nq0 * qm.nq * nd ** 3
 
     def run(prec, steps, warmup):
+        # one step: linewidths of every irreducible q0 and kappa, LW_Allq0 kept on the device
         for _ in range(warmup):
-            l = ctx.lw3_all(args.sigma, prec)
-            ctx.kappa(l, args.T, symm=True)
+            ctx.lw3_kappa_all(args.sigma, prec, args.T)
         ctx.kernel_time_reset()
         t = time.perf_counter()
         for _ in range(steps):
-            l = ctx.lw3_all(args.sigma, prec)
-            k, _ = ctx.kappa(l, args.T, symm=True)
+            l, k, _ = ctx.lw3_kappa_all(args.sigma, prec, args.T)
         dt = time.perf_counter() - t
         nl, kms = ctx.kernel_time()
         return dt / steps, kms / max(nl, 1), l, k

@@ -183,6 +183,13 @@ int spr_lw4_tetra_all(spr_ctx *ctx, double *lw_all);
 int spr_kappa_rta(spr_ctx *ctx, const double *lw_all, double T, int symm,
                   double *kappa, double *mode_k);
 
+/* spr_lw3_all followed by spr_kappa_rta with LW_Allq0 kept on the device in between (one
+ * synchronisation instead of a download and an upload): lw_all(Ndof, nq0), kappa(9) W/mK, mode_k
+ * (9, Ndof) optional.  The two calls of main.f90:552-715 and :750-760 (ThermalConductivity.f90:
+ * 167-293) in one; same results. */
+int spr_lw3_kappa_all(spr_ctx *ctx, double sigma, int prec, double T, int symm, double *lw_all,
+                      double *kappa, double *mode_k);
+
 /* Per-pair terms of the three-phonon Gaussian sum for caller-chosen (q0, q1) pairs:
  * out(Ndof, n): out(s0, k) = sum_{s1,s2} W(s2,s1,s0; q0_index(k), q1_grid(k)) [delta terms], the q1
  * term of CalculateLinewidth's sum (ThirdOrder_m.f90:490-541).  Used by the sampled (importance)

@@ -172,7 +172,22 @@ int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double
         // direct launch: the events bracket the item kernel alone
         if ((rc = enqueue(c, L, mode, q0_first, q0_count, it0, d_lw, true))) return rc;
     }
+    if (c.defer_timing) {
+        // the caller synchronises later and then reads the events (collect_timing)
+        c.timing_pending = true;
+        return SPR_OK;
+    }
     SPR_HIP(hipEventSynchronize(c.ev1));
+    float ms = 0.f;
+    SPR_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    c.launches += 1;
+    c.kernel_ms += ms;
+    return SPR_OK;
+}
+
+int collect_timing(Ctx &c) {
+    if (!c.timing_pending) return SPR_OK;
+    c.timing_pending = false;
     float ms = 0.f;
     SPR_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     c.launches += 1;

@@ -448,6 +448,53 @@ extern "C" int spr_kappa_rta(spr_ctx *h, const double *lw_all, double T, int sym
     return SPR_OK;
 }
 
+extern "C" int spr_lw3_kappa_all(spr_ctx *h, double sigma, int prec, double T, int symm, double *lw_all,
+                                 double *kappa, double *mode_k) {
+    if (!h || !lw_all || !kappa) {
+        set_error("spr_lw3_kappa_all: NULL argument");
+        return SPR_E_ARG;
+    }
+    if (!(T > 0.0)) {
+        set_error("spr_lw3_kappa_all: T must be positive");
+        return SPR_E_ARG;
+    }
+    Ctx &c = h->c;
+    int rc;
+    if ((rc = check_prec(prec))) return rc;
+    if (!(sigma > 0.0)) {
+        set_error("sigma must be positive");
+        return SPR_E_ARG;
+    }
+    SPR_HIP(hipSetDevice(c.device));
+    const size_t nl = (size_t)c.nq0 * c.nd, nm = (size_t)c.nd * 9;
+    if (!c.k_io) SPR_HIP(hipMalloc((void **)&c.k_io, (nl + nm) * sizeof(double)));
+    // LW_Allq0 stays on the device between the linewidths and kappa, and the host waits once
+    c.defer_timing = true;
+    rc = launch_lw3(c, 0, c.nq0, sigma, prec, c.out, nullptr);
+    c.defer_timing = false;
+    if (!rc) rc = launch_kappa(c, c.out, T, symm, c.k_io + nl);
+    std::vector<double> modes(nm);
+    if (!rc && (hipMemcpyAsync(lw_all, c.out, nl * sizeof(double), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+                hipMemcpyAsync(modes.data(), c.k_io + nl, nm * sizeof(double), hipMemcpyDeviceToHost, c.stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(c.stream) != hipSuccess)) {
+        set_error("spr_lw3_kappa_all: download failed");
+        rc = SPR_E_HIP;
+    }
+    if (rc) {
+        c.timing_pending = false;
+        return rc;
+    }
+    if ((rc = collect_timing(c))) return rc;
+    for (int k = 0; k < 9; ++k) {
+        double acc = 0.0;
+        for (int m = 0; m < c.nd; ++m) acc += modes[(size_t)m * 9 + k];
+        kappa[k] = acc;
+    }
+    if (mode_k) std::memcpy(mode_k, modes.data(), nm * sizeof(double));
+    return SPR_OK;
+}
+
 namespace spr {
 int launch_lw3_pairs(Ctx &c, int n, const int *q0_index, const int *q1_grid, double sigma, int prec, double *h_out);
 }

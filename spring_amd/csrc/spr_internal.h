@@ -151,6 +151,9 @@ struct Ctx {
     // timing of the dominant kernel
     int launches = 0;
     double kernel_ms = 0.0;
+    // run_items leaves its events unread (no host wait) while defer_timing is set;
+    // collect_timing reads them after the caller's own synchronisation
+    bool defer_timing = false, timing_pending = false;
 };
 
 int ensure(void **p, size_t *cap, size_t bytes);
@@ -173,6 +176,7 @@ int launch_reduce_chunks(Ctx &c, const double *partial, int q0_count, int nchunk
 int ensure_pairs(Ctx &c);
 int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double *d_lw);
 void drop_graphs(Ctx &c);
+int collect_timing(Ctx &c);
 int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, int upi, double *d_lw);
 constexpr int kMaxUnitsPerItem = 4;   // partial rows reserved per item
 int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_mode_k);

@@ -21,7 +21,7 @@ SYMBOLS = [
     "spr_kernel_time_reset", "spr_lw3_pairs", "spr_v3_w_pairs", "spr_mlp_create", "spr_mlp_destroy",
     "spr_mlp_num_params", "spr_mlp_get_params", "spr_mlp_set_params", "spr_mlp_grad",
     "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all",
-    "spr_lw4_tetra", "spr_lw4_tetra_all", "spr_mlp_score",
+    "spr_lw4_tetra", "spr_lw4_tetra_all", "spr_lw3_kappa_all", "spr_mlp_score",
     "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve", "spr_get_phonon",
 ]
 
@@ -186,6 +186,17 @@ class GpuContext:
         self._check(self.lib.spr_kappa_rta(self.h, _ptr(lw), ct.c_double(T), ct.c_int(int(symm)),
                                            _ptr(k), _ptr(modes)), "spr_kappa_rta")
         return k.reshape(3, 3), modes
+
+    def lw3_kappa_all(self, sigma: float, prec: str, T: float, symm: bool = True):
+        """Linewidths of every irreducible q0 and the RTA kappa in one call, LW_Allq0 kept on the
+        device in between: (lw (nq0, Ndof), kappa (3, 3), mode_k (Ndof, 9))."""
+        lw = np.zeros((self.nq0, self.nd))
+        k = np.zeros(9)
+        modes = np.zeros((self.nd, 9))
+        self._check(self.lib.spr_lw3_kappa_all(self.h, ct.c_double(sigma), ct.c_int(PREC[prec]), ct.c_double(T),
+                                               ct.c_int(int(symm)), _ptr(lw), _ptr(k), _ptr(modes)),
+                    "spr_lw3_kappa_all")
+        return lw, k.reshape(3, 3), modes
 
     def lw3_pairs(self, q0_index, q1_grid, sigma: float, prec: str = "fp32") -> np.ndarray:
         """Per-pair linewidth terms (n, Ndof) for the given (irreducible q0, q1) pairs."""

@@ -48,6 +48,19 @@ def test_lw3_deterministic(small):
     assert np.array_equal(a, b)
 
 
+def test_lw3_kappa_all_equals_the_two_calls(small):
+    """spr_lw3_kappa_all (LW_Allq0 kept on the device) gives the same bits as spr_lw3_all then
+    spr_kappa_rta, in every precision (sigma = 1 rad/ps: every mode of the small mesh has a
+    finite linewidth, so kappa is finite)."""
+    m, qm, ph, ctx, inp = small
+    for prec in ("fp64", "fp32"):
+        lw_a = ctx.lw3_all(1.0, prec)
+        k_a, mk_a = ctx.kappa(lw_a, 300.0, symm=True)
+        lw_b, k_b, mk_b = ctx.lw3_kappa_all(1.0, prec, 300.0, symm=True)
+        assert np.all(np.isfinite(k_a))
+        assert np.array_equal(lw_a, lw_b) and np.array_equal(k_a, k_b) and np.array_equal(mk_a, mk_b)
+
+
 def test_lw3_fp32_close(small):
     m, qm, ph, ctx, inp = small
     ref = ctx.lw3_all(0.1, "fp64")

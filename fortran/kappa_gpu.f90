@@ -35,7 +35,7 @@ program kappa_gpu
     real(dp), allocatable   :: mass(:), fC3(:,:,:,:,:,:), fC4(:,:,:,:,:,:,:,:), Rxyz(:,:,:)
     complex(dp), allocatable :: Evec(:,:,:)
     real(dp), allocatable   :: omega(:,:), OnebyOmega(:,:), nBE(:,:), grp_vel(:,:,:)
-    real(dp), allocatable   :: LW_Allq0(:,:), lw4(:,:), Mode_k(:,:), lw(:), P3_q0(:)
+    real(dp), allocatable   :: LW_Allq0(:,:), lw4(:,:), Mode_k(:,:), lw(:), P3_q0(:), P_Allq0(:,:)
     real(dp)                :: kappa(9)
     type(gpu_ctx)           :: ctx
     logical                 :: HeaderWrite, loop_mode, iter_mode, tetra_mode, gpu_phonons
@@ -198,10 +198,11 @@ program kappa_gpu
     ! ---------------------------- three-phonon linewidths (main.f90:552-715) ----------------------------
     allocate( LW_Allq0(Ndof, Nq0), lw(Ndof) )
     if ( tetra_mode ) then
-        allocate( P3_q0(Ndof) )
+        allocate( P3_q0(Ndof), P_Allq0(Ndof, Nq0) )
         do i0 = 1, Nq0
             call CalculateLwTetrahedron_gpu(ctx, i0, lw, P3_q0, P3)
             LW_Allq0(:, i0) = lw
+            P_Allq0(:, i0) = P3_q0 * 1000.0_dp / ( 9.0_dp * dble( Ndof**2 ) * dble( Nq ) )   ! fs, main.f90:689
             if ( FourPhonon /= 0 ) then
                 call TetrahedronIntegration4th_gpu(ctx, i0, lw)
                 LW_Allq0(:, i0) = LW_Allq0(:, i0) + 0.5_dp * lw      ! main.f90:660
@@ -239,7 +240,12 @@ program kappa_gpu
     write(*, '(3(ES22.14, 2X))') kappa(7:9)
     ! kappa_<prefix><T>K.h5 (main.f90:773-798): the tensor, [omega; linewidth] of every q0 and the
     ! mode-resolved kappa
-    call w_kappa_h5(trim(kappa_outFile), kappa, omega(:, irr_grid), LW_Allq0, Mode_k=Mode_k)
+    if ( tetra_mode ) then
+        ! PhaseSpace3rd as main.f90:777 writes it in the reference's (tetrahedron) mode
+        call w_kappa_h5(trim(kappa_outFile), kappa, omega(:, irr_grid), LW_Allq0, P_Allq0=P_Allq0, Mode_k=Mode_k)
+    else
+        call w_kappa_h5(trim(kappa_outFile), kappa, omega(:, irr_grid), LW_Allq0, Mode_k=Mode_k)
+    end if
     write(*, '(A, A)') 'Wrote ', trim(kappa_outFile)
     call gpu_ctx_destroy(ctx)
 

@@ -152,3 +152,10 @@ def test_fortran_driver_tetrahedron_three_and_four_phonon(rocksalt, tmp_path):
         ref.append(l3 + 0.5 * O.lw4_tetra(inp4, qm, int(g), threads=8, chunk=4))
     ref = np.array(ref)
     assert rel_err(lw, ref, 1e-8) < 1e-10
+    # the kappa file holds PhaseSpace3rd = P3_q0 * 1000 / (9 Ndof^2 Nq) (fs, main.f90:689, 777)
+    from spring_amd import h5io
+    nd = 6
+    k = h5io.read_kappa(str(tmp_path / "kappa_NaClsynth300.0K.h5"), nd, qm.nq0, p3=True)
+    p3 = np.array([O.lw3_tetra(qm, ph, int(g), O.lw3_gauss(inp3, int(g), 1.0, want_w=True)[1])[1]
+                   for g in qm.irr_idx]) * 1000.0 / (9.0 * nd ** 2 * qm.nq)
+    assert rel_err(k["p3"], p3, 1e-8) < 1e-10

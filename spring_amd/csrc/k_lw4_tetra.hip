@@ -74,33 +74,46 @@ __device__ __forceinline__ void sort4(Vx *v) {
     cswap(v[0], v[1]);
 }
 
-// g * (1/6) sum_v I_v F_v of one sorted tetrahedron at mq, NaN I_v and g set to zero
-__device__ __forceinline__ double tetra4(double mq, const double *M, const double *F) {
+// Reciprocals of the six differences of one sorted tetrahedron's vertex energies; they do not
+// depend on s0.  A zero difference gives inf, which turns the reference's 0/0 into 0 * inf =
+// NaN and its x/0 into x * inf = inf, so the NaN handling below sees the same cases.
+struct Rcp {
+    double r10, r20, r30, r21, r31, r32;
+};
+
+__device__ __forceinline__ Rcp rcp6(const double *M) {
+    return Rcp{1.0 / (M[1] - M[0]), 1.0 / (M[2] - M[0]), 1.0 / (M[3] - M[0]),
+               1.0 / (M[2] - M[1]), 1.0 / (M[3] - M[1]), 1.0 / (M[3] - M[2])};
+}
+
+// g * (1/6) sum_v I_v F_v of one sorted tetrahedron at mq (FindgAndIver / FindIverxFver with the
+// divisions by vertex-energy differences as products with R), NaN I_v and g set to zero
+__device__ __forceinline__ double tetra4(double mq, const double *M, const Rcp &R, const double *F) {
+    constexpr double third = 1.0 / 3.0;
     double g, I0, I1, I2, I3;
     if (mq >= M[0] && mq < M[1]) {
-        g = 3.0 * (mq - M[0]) * (mq - M[0]) / ((M[1] - M[0]) * (M[2] - M[0]) * (M[3] - M[0]));
-        I0 = ((mq - M[1]) / (M[0] - M[1]) + (mq - M[2]) / (M[0] - M[2]) + (mq - M[3]) / (M[0] - M[3])) / 3.0;
-        I1 = ((mq - M[0]) / (M[1] - M[0])) / 3.0;
-        I2 = ((mq - M[0]) / (M[2] - M[0])) / 3.0;
-        I3 = ((mq - M[0]) / (M[3] - M[0])) / 3.0;
+        const double x0 = mq - M[0];
+        g = 3.0 * x0 * x0 * (R.r10 * R.r20 * R.r30);
+        I0 = ((M[1] - mq) * R.r10 + (M[2] - mq) * R.r20 + (M[3] - mq) * R.r30) * third;
+        I1 = x0 * R.r10 * third;
+        I2 = x0 * R.r20 * third;
+        I3 = x0 * R.r30 * third;
     } else if (mq >= M[1] && mq < M[2]) {
-        g = (3.0 / ((M[1] - M[2]) * (M[3] - M[0]))) *
-            (((mq - M[0]) * (mq - M[2])) / (M[2] - M[0]) - ((mq - M[1]) * (mq - M[3])) / (M[1] - M[3]));
-        const double gd = g * (M[3] - M[0]);
-        I0 = ((mq - M[3]) / (M[0] - M[3])) / 3.0 +
-             ((mq - M[2]) / (M[0] - M[2])) * ((mq - M[0]) / (M[2] - M[0])) * ((mq - M[2]) / (M[1] - M[2])) / gd;
-        I1 = ((mq - M[2]) / (M[1] - M[2])) / 3.0 +
-             ((mq - M[3]) / (M[1] - M[3])) * ((mq - M[3]) / (M[1] - M[3])) * ((mq - M[1]) / (M[2] - M[1])) / gd;
-        I2 = ((mq - M[1]) / (M[2] - M[1])) / 3.0 +
-             ((mq - M[0]) / (M[2] - M[0])) * ((mq - M[0]) / (M[2] - M[0])) * ((mq - M[2]) / (M[1] - M[2])) / gd;
-        I3 = ((mq - M[0]) / (M[3] - M[0])) / 3.0 +
-             ((mq - M[1]) / (M[3] - M[1])) * ((mq - M[3]) / (M[1] - M[3])) * ((mq - M[1]) / (M[2] - M[1])) / gd;
+        const double x0 = mq - M[0], x1 = mq - M[1], x2 = mq - M[2], x3 = mq - M[3];
+        // (M1 - M2) = -d21 and (M1 - M3) = -d31
+        g = (-3.0 * R.r21 * R.r30) * (x0 * x2 * R.r20 + x1 * x3 * R.r31);
+        const double igd = 1.0 / (g * (M[3] - M[0]));
+        I0 = (-x3 * R.r30) * third + (-x2 * R.r20) * (x0 * R.r20) * (-x2 * R.r21) * igd;
+        I1 = (-x2 * R.r21) * third + (-x3 * R.r31) * (-x3 * R.r31) * (x1 * R.r21) * igd;
+        I2 = (x1 * R.r21) * third + (x0 * R.r20) * (x0 * R.r20) * (-x2 * R.r21) * igd;
+        I3 = (x0 * R.r30) * third + (x1 * R.r31) * (-x3 * R.r31) * (x1 * R.r21) * igd;
     } else if (mq >= M[2] && mq <= M[3]) {
-        g = 3.0 * (mq - M[3]) * (mq - M[3]) / ((M[3] - M[0]) * (M[3] - M[1]) * (M[3] - M[2]));
-        I0 = ((mq - M[3]) / (M[0] - M[3])) / 3.0;
-        I1 = ((mq - M[3]) / (M[1] - M[3])) / 3.0;
-        I2 = ((mq - M[3]) / (M[2] - M[3])) / 3.0;
-        I3 = ((mq - M[0]) / (M[3] - M[0]) + (mq - M[1]) / (M[3] - M[1]) + (mq - M[2]) / (M[3] - M[2])) / 3.0;
+        const double x3 = mq - M[3];
+        g = 3.0 * x3 * x3 * (R.r30 * R.r31 * R.r32);
+        I0 = -x3 * R.r30 * third;
+        I1 = -x3 * R.r31 * third;
+        I2 = -x3 * R.r32 * third;
+        I3 = ((mq - M[0]) * R.r30 + (mq - M[1]) * R.r31 + (mq - M[2]) * R.r32) * third;
     } else {
         return 0.0;
     }
@@ -204,6 +217,7 @@ __global__ __launch_bounds__(T4BLK) void k_lw4_tetra(Tet4Args A) {
             if (mhi < w0lo || mlo > w0hi) continue;
             sort4(P);
             const double M[4] = {P[0].M, P[1].M, P[2].M, P[3].M};
+            const Rcp R = rcp6(M);
             const double wk = k == 0 ? 1.0 / 6.0 : 0.5;
 #pragma unroll 1
             for (int s0 = s0_lo; s0 < ND; ++s0) {
@@ -216,7 +230,7 @@ __global__ __launch_bounds__(T4BLK) void k_lw4_tetra(Tet4Args A) {
                 double F[4];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) F[v] = A.W[P[v].off + so] * (P[v].a * rn0);
-                const double tk = wk * tetra4(mq, M, F);
+                const double tk = wk * tetra4(mq, M, R, F);
 #pragma unroll
                 for (int s = 0; s < ND; ++s)
                     if (s == s0) acc[s] += tk;

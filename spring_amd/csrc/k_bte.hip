@@ -227,27 +227,47 @@ __global__ __launch_bounds__(kBteBlock) void k_bte_build(BteArgs A, int R, doubl
     __syncthreads();
     double tau = 0.0;
     for (int i = threadIdx.x; i < A.ntet * nd2; i += kBteBlock) {
-        const int t = i / nd2, p = i % nd2;
+        // tetrahedron fastest across lanes: a wave holds neighbouring tetrahedra of one branch
+        // pair, whose energy ranges are alike, so whole waves skip together below
+        const int p = i / A.ntet, t = i - p * A.ntet;
         const int s1 = p / nd, s2 = p % nd;
         double Mp[4], Mm[4], Wp[4], Wm[4];
         int c1[4], cp[4], cm[4];
         double g1[4], gp[4], gm[4];
+        int q1v[4];
+        int4 k1v[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int q1 = A.tetra[4 * t + v];
             const int4 k1 = A.coord[q1];
+            q1v[v] = q1;
+            k1v[v] = k1;
+            const int qp = grid_index(kq.x + k1.x, kq.y + k1.y, kq.z + k1.z, A);
+            const int qm = grid_index(kq.x - k1.x, kq.y - k1.y, kq.z - k1.z, A);
+            const double w1 = A.omega[(size_t)q1 * nd + s1];
+            Mp[v] = A.omega[(size_t)qp * nd + s2] - w1;
+            Mm[v] = w1 + A.omega[(size_t)qm * nd + s2];
+        }
+        // both processes need Mq strictly inside their vertex-energy range (tetra_gI's cases);
+        // outside it they add nothing, so W and the occupations are not read
+        const double lo_p = fmin(fmin(Mp[0], Mp[1]), fmin(Mp[2], Mp[3]));
+        const double hi_p = fmax(fmax(Mp[0], Mp[1]), fmax(Mp[2], Mp[3]));
+        const double lo_m = fmin(fmin(Mm[0], Mm[1]), fmin(Mm[2], Mm[3]));
+        const double hi_m = fmax(fmax(Mm[0], Mm[1]), fmax(Mm[2], Mm[3]));
+        if (!((mq > lo_p && mq < hi_p) || (mq > lo_m && mq < hi_m))) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int q1 = q1v[v];
+            const int4 k1 = k1v[v];
             const int qp = grid_index(kq.x + k1.x, kq.y + k1.y, kq.z + k1.z, A);
             const int qq2 = grid_index(-kq.x - k1.x, -kq.y - k1.y, -kq.z - k1.z, A);
             const int nq1 = grid_index(-k1.x, -k1.y, -k1.z, A);
             const int qm = grid_index(kq.x - k1.x, kq.y - k1.y, kq.z - k1.z, A);
             const int mq2 = grid_index(k1.x - kq.x, k1.y - kq.y, k1.z - kq.z, A);
-            const double w1 = A.omega[(size_t)q1 * nd + s1];
             Wp[v] = Wq[((size_t)q1 * nd + s) * nd2 + p] * A.scale *
                     (A.nbe[(size_t)q1 * nd + s1] - A.nbe[(size_t)qq2 * nd + s2]);
-            Mp[v] = A.omega[(size_t)qp * nd + s2] - w1;
             Wm[v] = Wq[((size_t)nq1 * nd + s) * nd2 + p] * A.scale *
                     (1.0 + A.nbe[(size_t)nq1 * nd + s1] + A.nbe[(size_t)mq2 * nd + s2]);
-            Mm[v] = w1 + A.omega[(size_t)qm * nd + s2];
             const int2 m1 = A.map[q1], mp = A.map[qp], mm = A.map[qm];
             c1[v] = m1.x * nd + s1;
             g1[v] = m1.y;

@@ -35,12 +35,14 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--workload", default="rocksalt16",
                    choices=["rocksalt16", "cell10", "learned10", "fourph10", "fourph_tetra", "bte16",
-                            "phonons10"],
+                            "phonons10", "tetra16"],
                    help="rocksalt16: 2-atom cell, 16^3 mesh (the headline line); cell10: 10-atom "
                         "cell, 20^3 mesh with time reversal, FP64 vs FP32 vs BF16; learned10: the "
                         "learned importance-sampling proposal on the cell10 workload; fourph10: "
                         "four-phonon linewidths, 2-atom cell with FC4, 10^3 mesh; fourph_tetra: the "
-                        "same with the tetrahedron method on a --mesh4 mesh; bte16: iterative "
+                        "same with the tetrahedron method on a --mesh4 mesh; tetra16: three-phonon "
+                        "linewidths with the tetrahedron method (the reference's default delta) "
+                        "on the 2-atom --mesh mesh; bte16: iterative "
                         "BTE of the 2-atom cell on a 16^3 mesh (--mesh); phonons10: harmonic phonons "
                         "(D(q), eigensolve, group velocities) of the 10-atom cell on its 20^3 mesh")
     p.add_argument("--samples", type=int, default=2_000_000, help="learned10: training samples")
@@ -81,6 +83,8 @@ def main():
         return fourph10(args)
     if args.workload == "fourph_tetra":
         return fourph_tetra(args)
+    if args.workload == "tetra16":
+        return tetra16(args)
     if args.workload == "bte16":
         return bte16(args)
     if args.workload == "phonons10":
@@ -498,6 +502,55 @@ def fourph_tetra(args):
                                f"{n}^3 mesh, {qm.nq0} irreducible q0 x {qm.nq} q1 x {6 * qm.nq} "
                                f"tetrahedra x {nd}^4 branches", "T_K": args.T},
         "lw_sum": float(lw.sum()),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def tetra16(args):
+    """Three-phonon tetrahedron-method linewidths (CalculateLwTetrahedron, the reference's default
+    delta) of every irreducible q0 of the 2-atom model on a --mesh mesh, FP64: per q0, W of every
+    q1 (spr_v3_w path) then the tetrahedron sums; the CPU leg runs the oracle's tetrahedron sum
+    on the same W for a few q0."""
+    from spring_amd import model, mesh, phonons, lib
+    m = model.rocksalt2()
+    n = args.mesh
+    qm = mesh.make_mesh(m.crystal, (n, n, n))
+    ph = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
+    ctx = lib.GpuContext(m, qm, ph)
+    nd = m.crystal.ndof
+    for _ in range(max(0, args.warmup)):
+        ctx.lw3_tetra(min(1, qm.nq0 - 1))
+    steps = max(1, args.steps)
+    ctx.kernel_time_reset()
+    t = time.perf_counter()
+    for _ in range(steps):
+        lw, p3 = ctx.lw3_tetra_all()
+    dt = (time.perf_counter() - t) / steps
+    nl, kms = ctx.kernel_time()
+    # one unit: (q0, tetrahedron, s0 s1 s2), three processes at four vertices
+    units = qm.nq0 * 6 * qm.nq * nd ** 3
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        inp = O.Inputs(m, qm, ph)
+        k = min(4, qm.nq0)
+        Ws = [(int(qm.irr_idx[i]), O.lw3_gauss(inp, int(qm.irr_idx[i]), 1.0, want_w=True)[1]) for i in range(k)]
+        t = time.perf_counter()
+        for g, W in Ws:
+            O.lw3_tetra(qm, ph, g, W)
+        dtc = time.perf_counter() - t
+        cpu = {"value": k * 6 * qm.nq * nd ** 3 / dtc, "unit": "units/s", "cores": 1, "kind": "port",
+               "sample": f"tetrahedron sums of {k} irreducible q0 from precomputed W, FP64 C restatement, "
+                         f"1 thread, {dtc:.1f} s"}
+    out = {
+        "metric": "three-phonon tetrahedron units/s (q0, tetrahedron, branch triplet)", "unit": "units/s",
+        "value": units / dt, "higher_is_better": True, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3-phonon tetrahedron linewidths, 2-atom rocksalt, {n}^3 mesh, {qm.nq0} "
+                               f"irreducible q0 x {6 * qm.nq} tetrahedra x {nd}^3 branches", "T_K": args.T},
+        "tetra_kernel_ms_per_q0": kms / max(nl, 1),
+        "lw_sum": float(lw.sum()), "p3_sum": float(p3.sum()),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

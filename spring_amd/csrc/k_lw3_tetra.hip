@@ -8,8 +8,9 @@
 // P3_q0(s0) accumulates g of all three processes.  W(q1, s0, s1, s2) of the q0 comes from the
 // FP64 W path; all arithmetic is FP64.
 //
-// Mapping: one workgroup per (tetrahedron chunk, s0); a thread owns (tetrahedron, s1 s2) and
-// walks its chunk; W reads are contiguous in (s1 s2).  Per-workgroup partials are reduced in a
+// Mapping: one workgroup per (tetrahedron chunk, s0); a thread owns (s1 s2, tetrahedron) of
+// its chunk, tetrahedron fastest across lanes, and reads W and n only when w0(s0) lies inside
+// one of the three processes' vertex-energy ranges.  Per-workgroup partials are reduced in a
 // fixed order by a second kernel (bitwise reproducible).
 #include <vector>
 
@@ -81,21 +82,38 @@ __global__ __launch_bounds__(TBLK) void k_lw3_tetra(const double *W, const int *
     const int t_lo = ch * chunk, t_hi = min(ntet, t_lo + chunk);
     const double mq = omega0[s0];
     double lw = 0.0, p3 = 0.0;
-    for (int i = threadIdx.x; i < (t_hi - t_lo) * nd2; i += TBLK) {
-        const int t = t_lo + i / nd2, p = i % nd2;
+    const int nt = t_hi - t_lo;
+    for (int i = threadIdx.x; i < nt * nd2; i += TBLK) {
+        // tetrahedron fastest across lanes: a wave holds neighbouring tetrahedra of one branch
+        // pair, whose energy ranges are alike, so whole waves skip together below
+        const int p = i / nt, t = t_lo + (i - p * nt);
         const int s1 = p / nd, s2 = p % nd;
         double F1[4], F2[4], M1[4], M2[4], M3[4];
+        int q1v[4], q2v[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int q1 = tetra[4 * t + v], q2 = partner[q1];
-            const double w = W[((size_t)q1 * nd + s0) * nd2 + p];
-            const double n1 = nbe[(size_t)q1 * nd + s1], n2 = nbe[(size_t)q2 * nd + s2];
+            q1v[v] = q1;
+            q2v[v] = q2;
             const double w1 = omega[(size_t)q1 * nd + s1], w2 = omega[(size_t)q2 * nd + s2];
-            F1[v] = w * (n1 + n2 + 1.0);
-            F2[v] = w * (n1 - n2);
             M1[v] = w1 + w2;
             M2[v] = w2 - w1;
             M3[v] = w1 - w2;
+        }
+        // a process adds g and weights only for M0 <= mq <= M3 of its vertex energies (the union
+        // of the three cases of tetra_gif); outside every range W and n are not read
+        const double lo1 = fmin(fmin(M1[0], M1[1]), fmin(M1[2], M1[3]));
+        const double hi1 = fmax(fmax(M1[0], M1[1]), fmax(M1[2], M1[3]));
+        const double lo2 = fmin(fmin(M2[0], M2[1]), fmin(M2[2], M2[3]));
+        const double hi2 = fmax(fmax(M2[0], M2[1]), fmax(M2[2], M2[3]));
+        if (!((mq >= lo1 && mq <= hi1) || (mq >= lo2 && mq <= hi2) || (mq >= -hi2 && mq <= -lo2))) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int q1 = q1v[v], q2 = q2v[v];
+            const double w = W[((size_t)q1 * nd + s0) * nd2 + p];
+            const double n1 = nbe[(size_t)q1 * nd + s1], n2 = nbe[(size_t)q2 * nd + s2];
+            F1[v] = w * (n1 + n2 + 1.0);
+            F2[v] = w * (n1 - n2);
         }
         double g, gif;
         tetra_gif(mq, M1, F1, g, gif);

@@ -142,10 +142,31 @@ __global__ __launch_bounds__(kBteBlock) void k_bte(BteArgs A) {
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    for (int i = threadIdx.x; i < (t_hi - t_lo) * nd2; i += kBteBlock) {
-        const int t = t_lo + i / nd2, p = i % nd2;
+    const int nt = t_hi - t_lo;
+    for (int i = threadIdx.x; i < nt * nd2; i += kBteBlock) {
+        // tetrahedron fastest across lanes (neighbouring tetrahedra have alike energy ranges)
+        const int p = i / nt, t = t_lo + (i - p * nt);
         const int s1 = p / nd, s2 = p % nd;
         double Mp[4], Mm[4], Xp[4][K], Xm[4][K];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int4 k1 = A.coord[A.tetra[4 * t + v]];
+            const int q1 = A.tetra[4 * t + v];
+            const int qp = grid_index(kq.x + k1.x, kq.y + k1.y, kq.z + k1.z, A);
+            const int qm = grid_index(kq.x - k1.x, kq.y - k1.y, kq.z - k1.z, A);
+            const double w1 = A.omega[(size_t)q1 * nd + s1];
+            Mp[v] = A.omega[(size_t)qp * nd + s2] - w1;
+            Mm[v] = w1 + A.omega[(size_t)qm * nd + s2];
+        }
+        // both processes need mq strictly inside their vertex-energy range (tetra_gI's cases);
+        // outside it they add nothing, so W, n and F are not read
+        {
+            const double lo_p = fmin(fmin(Mp[0], Mp[1]), fmin(Mp[2], Mp[3]));
+            const double hi_p = fmax(fmax(Mp[0], Mp[1]), fmax(Mp[2], Mp[3]));
+            const double lo_m = fmin(fmin(Mm[0], Mm[1]), fmin(Mm[2], Mm[3]));
+            const double hi_m = fmax(fmax(Mm[0], Mm[1]), fmax(Mm[2], Mm[3]));
+            if (!((mq > lo_p && mq < hi_p) || (mq > lo_m && mq < hi_m))) continue;
+        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int q1 = A.tetra[4 * t + v];
@@ -155,15 +176,12 @@ __global__ __launch_bounds__(kBteBlock) void k_bte(BteArgs A) {
             const int nq1 = grid_index(-k1.x, -k1.y, -k1.z, A);                        // -q1
             const int qm = grid_index(kq.x - k1.x, kq.y - k1.y, kq.z - k1.z, A);        // q - q1
             const int mq2 = grid_index(k1.x - kq.x, k1.y - kq.y, k1.z - kq.z, A);       // q1 - q
-            const double w1 = A.omega[(size_t)q1 * nd + s1];
             // + process, pair (q1, -q-q1)
             const double wp = Wq[((size_t)q1 * nd + s) * nd2 + p] * A.scale *
                               (A.nbe[(size_t)q1 * nd + s1] - A.nbe[(size_t)qq2 * nd + s2]);
-            Mp[v] = A.omega[(size_t)qp * nd + s2] - w1;
             // - process, pair (-q1, q1 - q)
             const double wm = Wq[((size_t)nq1 * nd + s) * nd2 + p] * A.scale *
                               (1.0 + A.nbe[(size_t)nq1 * nd + s1] + A.nbe[(size_t)mq2 * nd + s2]);
-            Mm[v] = w1 + A.omega[(size_t)qm * nd + s2];
             if (MODE == 0) {
                 Xp[v][0] = wp;
                 Xm[v][0] = wm;

@@ -45,9 +45,13 @@ struct SmallArgs {
     const void *evec;
     const double *omega, *inv_omega, *nbe;
     double sigma, factor;
+    // W output (WOUT instantiation): w_out[q0 - w_q0_first][q1][s0][s1][s2] for every grid q1,
+    // both orientations of each pair written
+    double *w_out;
+    int w_q0_first, w_nq;
 };
 
-template <typename R, int NB>
+template <typename R, int NB, bool WOUT>
 __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, const R *__restrict__ fc,
                                            const int *__restrict__ blk_start, const int *__restrict__ tup_site) {
     using Z = cplx_t<R>;
@@ -197,7 +201,7 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
         for (int k = 0; k < 3; ++k) {
             const double d1 = w0[k] - w1 - w2, d2 = w0[k] + w1 - w2, d3 = w0[k] - w1 + w2;
             const double t1 = hinv * d1 * d1, t2 = hinv * d2 * d2, t3 = hinv * d3 * d3;
-            if (!__ballot(fmin(t1, fmin(t2, t3)) < 746.0)) continue;
+            if (!WOUT && !__ballot(fmin(t1, fmin(t2, t3)) < 746.0)) continue;
             Z v = cmk<R>(0, 0);
 #pragma unroll
             for (int ar = 0; ar < ND; ++ar) {
@@ -207,6 +211,16 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
                 v = cfma(s_e0[(s0b + k) * ND + ar], u, v);
             }
             const double W = (double)cabs2(v) * (iw0f[k] * iw1) * iw2;
+            if constexpr (WOUT) {
+                // W(s2, s1, s0; q1) and, for the partner ordering, W(s1, s2, s0; q2)
+                if (lane < cnt) {
+                    const int s0 = s0b + k;
+                    double *wq = A.w_out + (size_t)(qi - A.w_q0_first) * A.w_nq * ND * ND * ND;
+                    wq[(((size_t)q1 * ND + s0) * ND + s1) * ND + s2] = W;
+                    if (q1 != q2) wq[(((size_t)q2 * ND + s0) * ND + s2) * ND + s1] = W;
+                }
+                continue;
+            }
             const double g = (n1 + n2 + 1.0) * (c0 * exp(-t1)) + (n1 - n2) * (c0 * exp(-t2) - c0 * exp(-t3));
             if (sizeof(R) == 8)
                 dacc[k] += wpair * W * g;
@@ -234,12 +248,12 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
 }
 
 // One workgroup per item, or (counter set) a persistent grid pulling items until none are left.
-template <typename R, int NB>
+template <typename R, int NB, bool WOUT>
 __global__ __launch_bounds__(NB * NB * 3 * 64) void k_lw3_small(SmallArgs A, const R *__restrict__ fc,
                                                                const int *__restrict__ blk_start,
                                                                const int *__restrict__ tup_site) {
     if (!A.ic.counter) {
-        small_item<R, NB>(A, blockIdx.x, fc, blk_start, tup_site);
+        small_item<R, NB, WOUT>(A, blockIdx.x, fc, blk_start, tup_site);
         return;
     }
     __shared__ int s_item;
@@ -249,7 +263,7 @@ __global__ __launch_bounds__(NB * NB * 3 * 64) void k_lw3_small(SmallArgs A, con
         __syncthreads();
         const int item = s_item;
         if (item >= A.ic.nitems) break;
-        small_item<R, NB>(A, item, fc, blk_start, tup_site);
+        small_item<R, NB, WOUT>(A, item, fc, blk_start, tup_site);
     }
 }
 
@@ -267,7 +281,7 @@ __global__ void k_reduce_items(const double *partial, const int *first, int base
 
 template <typename R, int NB>
 static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first, int q0_count, int it0,
-                          double *d_lw) {
+                          double *d_lw, bool wout = false) {
     constexpr int ND = 3 * NB;
     SmallArgs A = A0;
     A.fc = sizeof(R) == 8 ? (const void *)c.fc3.fc64_p : (const void *)c.fc3.fc32_p;
@@ -280,7 +294,7 @@ static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first,
         set_error("small-cell kernel: LDS need exceeds 160 KB (too many neighbour sites)");
         return SPR_E_UNSUPPORTED;
     }
-    const void *fn = (const void *)k_lw3_small<R, NB>;
+    const void *fn = wout ? (const void *)k_lw3_small<R, NB, true> : (const void *)k_lw3_small<R, NB, false>;
     if (dyn > 64 * 1024) SPR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
     int per_cu = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NW * 64, dyn) != hipSuccess || per_cu < 1)
@@ -288,7 +302,7 @@ static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first,
     const R *fcp = reinterpret_cast<const R *>(A.fc);
     const int *bsp = A.blk_start, *tsp = A.tup_site;
     ItemLaunch L{fn, &A, sizeof(A), NW * 64, dyn, nitems, per_cu,
-                 std::string("small") + (sizeof(R) == 8 ? "64" : "32")};
+                 std::string("small") + (sizeof(R) == 8 ? "64" : "32") + (wout ? "w" : "")};
     L.extra[0] = (void *)&fcp;
     L.extra[1] = (void *)&bsp;
     L.extra[2] = (void *)&tsp;
@@ -388,6 +402,8 @@ int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec,
     A.factor = 1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) *
                1000000.0 * pi * 0.0625 / (double)c.nq;   // Sctr3THz2 / Nq, constants.f90:59-60
     A.ic.partial = ov ? ov->partial : c.ipartial;
+    A.w_out = nullptr;
+    A.w_q0_first = A.w_nq = 0;
     const bool f64 = prec == SPR_PREC_FP64;
     if (c.nb == 2)
         return f64 ? launch_small_t<double, 2>(c, A, nitems, q0_first, q0_count, it0, d_lw)
@@ -397,6 +413,49 @@ int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec,
                    : launch_small_t<float, 1>(c, A, nitems, q0_first, q0_count, it0, d_lw);
     set_error("small-cell kernel supports Nb <= 2");
     return SPR_E_UNSUPPORTED;
+}
+
+// FP64 W of every grid q1 for irreducible q0 [q_first, q_first + q_count) into
+// d_w[q0 - q_first][q1][s0][s1][s2] (Nb <= 2): the pair-reduced items of those q0, each pair
+// writing both orderings (q -> -(q0 + q) is an involution, so every q1 is written exactly once),
+// no underflow skip.
+int launch_lw3_small_w(Ctx &c, int q_first, int q_count, double *d_w) {
+    if (c.nb > 2 || c.override_items) {
+        set_error("small-cell W path: Nb <= 2 and no item override");
+        return SPR_E_UNSUPPORTED;
+    }
+    int rc;
+    if ((rc = ensure_pairs(c))) return rc;
+    const int it0 = c.h_item_first[q_first];
+    const int nitems = c.h_item_first[q_first + q_count] - it0;
+    SmallArgs A;
+    A.item_q0 = c.d_item_q0 + it0;
+    A.item_off = c.d_item_off + it0;
+    A.item_cnt = c.d_item_cnt + it0;
+    A.pairs = c.d_pairs;
+    A.q0_grid = c.irr_grid;
+    A.L = c.L;
+    A.m0 = c.mesh[0];
+    A.m1 = c.mesh[1];
+    A.blk_start = c.fc3.blk_start_p;
+    A.tup_site = c.fc3.tup_site_p;
+    A.site = c.fc3.site;
+    A.nsite = c.fc3.nsite;
+    A.omega = c.omega;
+    A.inv_omega = c.inv_omega;
+    A.nbe = c.nbe;
+    A.sigma = 1.0;   // unused without the delta stage
+    const double pi = 4.0 * std::atan(1.0);
+    A.factor = 1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) *
+               1000000.0 * pi * 0.0625 / (double)c.nq;   // Sctr3THz2 / Nq, constants.f90:59-60
+    A.ic.partial = c.ipartial;
+    A.w_out = d_w;
+    A.w_q0_first = q_first;
+    A.w_nq = c.nq;
+    // the per-item sums of this launch are zero; they are reduced into a scratch row per q0
+    if ((rc = ensure((void **)&c.wsink, &c.wsink_cap, (size_t)q_count * c.nd * sizeof(double)))) return rc;
+    if (c.nb == 2) return launch_small_t<double, 2>(c, A, nitems, q_first, q_count, it0, c.wsink, true);
+    return launch_small_t<double, 1>(c, A, nitems, q_first, q_count, it0, c.wsink, true);
 }
 
 }  // namespace spr

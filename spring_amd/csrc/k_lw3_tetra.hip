@@ -169,43 +169,64 @@ int ensure_tetra(Ctx &c) {
     return SPR_OK;
 }
 
-// Tetrahedron linewidth of irreducible q0 `qi` into d_lw[nd], d_p3[nd] (device), using c.wout
-// (filled here with W of every q1 by the FP64 W path).
-int launch_lw3_tetra(Ctx &c, int qi, double *d_lw, double *d_p3) {
+// Tetrahedron linewidths of irreducible q0 [q_first, q_first + q_count) into d_lw[k][nd] and
+// d_p3[k][nd] (device, k = q - q_first).  W of every q1 for up to 16 q0 at a time in c.wout:
+// with the small-cell kernel (pair-reduced, both orderings written) for Nb <= 2, else the
+// generic W path one q0 at a time.  The partner tables of a batch are uploaded once.
+int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p3) {
     int rc;
     const size_t nw = (size_t)c.nq * c.nd * c.nd * c.nd;
-    if ((rc = ensure((void **)&c.wout, &c.wout_cap, nw * sizeof(double)))) return rc;
-    if ((rc = launch_lw3(c, qi, 1, 0.1, SPR_PREC_FP64, nullptr, c.wout))) return rc;
+    const bool small = c.nb <= 2 && !c.override_items;
+    const int bmax = small ? (int)std::max<size_t>(1, std::min<size_t>(16, (size_t(2) << 30) / (nw * sizeof(double))))
+                           : 1;
+    const int B = std::min(bmax, q_count);
+    if ((rc = ensure((void **)&c.wout, &c.wout_cap, (size_t)B * nw * sizeof(double)))) return rc;
     if ((rc = ensure_tetra(c))) return rc;
-    if (!c.partner) SPR_HIP(hipMalloc((void **)&c.partner, (size_t)c.nq * sizeof(int)));
-    // partner table of this q0: q2 = -(q0 + q1) on the grid
-    const int i0 = c.h_irr_grid[qi];
-    const int m0 = c.mesh[0], m1 = c.mesh[1], m2 = c.mesh[2];
-    std::vector<int> part(c.nq);
-    for (int q1 = 0; q1 < c.nq; ++q1) {
-        const int a0 = ((-(i0 % m0 + q1 % m0)) % m0 + m0) % m0;
-        const int a1 = ((-((i0 / m0) % m1 + (q1 / m0) % m1)) % m1 + m1) % m1;
-        const int a2 = ((-(i0 / (m0 * m1) + q1 / (m0 * m1))) % m2 + m2) % m2;
-        part[q1] = a0 + m0 * (a1 + m1 * a2);
-    }
-    SPR_HIP(hipMemcpyAsync(c.partner, part.data(), part.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    if ((rc = ensure((void **)&c.partner, &c.partner_cap, (size_t)B * c.nq * sizeof(int)))) return rc;
     const int ntet = 6 * c.nq;
     const int chunk = std::max(1, (ntet + 255) / 256);
     const int nch = (ntet + chunk - 1) / chunk;
     if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)nch * c.nd * 2 * sizeof(double)))) return rc;
-    SPR_HIP(hipEventRecord(c.ev0, c.stream));
-    hipLaunchKernelGGL(k_lw3_tetra, dim3(nch, c.nd), dim3(TBLK), 0, c.stream, c.wout, c.tetra, c.partner, c.omega,
-                       c.nbe, c.omega + (size_t)i0 * c.nd, c.nd, ntet, chunk, c.partial);
-    SPR_HIP(hipGetLastError());
-    SPR_HIP(hipEventRecord(c.ev1, c.stream));
-    hipLaunchKernelGGL(k_tetra_reduce, dim3(1), dim3(64), 0, c.stream, c.partial, c.nd, nch, d_lw, d_p3);
-    SPR_HIP(hipGetLastError());
-    // the host partner table must outlive the copy
-    SPR_HIP(hipStreamSynchronize(c.stream));
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess) {
-        c.launches += 1;
-        c.kernel_ms += ms;
+    const int m0 = c.mesh[0], m1 = c.mesh[1], m2 = c.mesh[2];
+    std::vector<int> part((size_t)B * c.nq);
+    for (int b0 = q_first; b0 < q_first + q_count; b0 += B) {
+        const int nb = std::min(B, q_first + q_count - b0);
+        if (small) {
+            if ((rc = launch_lw3_small_w(c, b0, nb, c.wout))) return rc;
+        } else if ((rc = launch_lw3(c, b0, 1, 0.1, SPR_PREC_FP64, nullptr, c.wout))) {
+            return rc;
+        }
+        // partner tables of the batch: q2 = -(q0 + q1) on the grid
+        for (int k = 0; k < nb; ++k) {
+            const int i0 = c.h_irr_grid[b0 + k];
+            for (int q1 = 0; q1 < c.nq; ++q1) {
+                const int a0 = ((-(i0 % m0 + q1 % m0)) % m0 + m0) % m0;
+                const int a1 = ((-((i0 / m0) % m1 + (q1 / m0) % m1)) % m1 + m1) % m1;
+                const int a2 = ((-(i0 / (m0 * m1) + q1 / (m0 * m1))) % m2 + m2) % m2;
+                part[(size_t)k * c.nq + q1] = a0 + m0 * (a1 + m1 * a2);
+            }
+        }
+        SPR_HIP(hipMemcpyAsync(c.partner, part.data(), (size_t)nb * c.nq * sizeof(int), hipMemcpyHostToDevice,
+                               c.stream));
+        SPR_HIP(hipEventRecord(c.ev0, c.stream));
+        for (int k = 0; k < nb; ++k) {
+            const int qi = b0 + k, i0 = c.h_irr_grid[qi];
+            hipLaunchKernelGGL(k_lw3_tetra, dim3(nch, c.nd), dim3(TBLK), 0, c.stream, c.wout + (size_t)k * nw, c.tetra,
+                               c.partner + (size_t)k * c.nq, c.omega, c.nbe, c.omega + (size_t)i0 * c.nd, c.nd, ntet,
+                               chunk, c.partial);
+            SPR_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_tetra_reduce, dim3(1), dim3(64), 0, c.stream, c.partial, c.nd, nch,
+                               d_lw + (size_t)(qi - q_first) * c.nd, d_p3 + (size_t)(qi - q_first) * c.nd);
+            SPR_HIP(hipGetLastError());
+        }
+        SPR_HIP(hipEventRecord(c.ev1, c.stream));
+        // the host partner tables must outlive the copy
+        SPR_HIP(hipStreamSynchronize(c.stream));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess) {
+            c.launches += nb;
+            c.kernel_ms += ms;
+        }
     }
     return SPR_OK;
 }

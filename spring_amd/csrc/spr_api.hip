@@ -298,7 +298,7 @@ extern "C" int spr_ctx_destroy(spr_ctx *h) {
                     (void *)c.irr_grid, (void *)c.partial, (void *)c.out, (void *)c.counter,
                     (void *)c.d_pairs, (void *)c.d_item_q0, (void *)c.d_item_off, (void *)c.d_item_cnt,
                     (void *)c.d_item_first,
-                    (void *)c.ipartial, (void *)c.wout, (void *)c.tetra, (void *)c.partner, (void *)c.tout, (void *)c.w4buf, (void *)c.kpack,
+                    (void *)c.ipartial, (void *)c.wout, (void *)c.tetra, (void *)c.partner, (void *)c.tout, (void *)c.w4buf, (void *)c.kpack, (void *)c.wsink,
                     (void *)c.k_terms, (void *)c.k_io, (void *)c.k_rot, (void *)c.k_pos_full, (void *)c.k_pos_irr,
                     (void *)c.k_mult, (void *)c.bte.d_coord, (void *)c.bte.d_map, (void *)c.bte.d_irr,
                     (void *)c.bte.d_tetra, (void *)c.bte.d_W, (void *)c.bte.d_F, (void *)c.bte.d_out,
@@ -566,7 +566,7 @@ extern "C" int spr_lw3_tetra(spr_ctx *h, int i0, double *lw, double *p3_q0) {
     SPR_HIP(hipSetDevice(c.device));
     int rc;
     if (!c.tout) SPR_HIP(hipMalloc((void **)&c.tout, (size_t)2 * c.nq0 * c.nd * sizeof(double)));
-    if ((rc = spr::launch_lw3_tetra(c, i0, c.tout, c.tout + c.nd))) return rc;
+    if ((rc = spr::launch_lw3_tetra(c, i0, 1, c.tout, c.tout + c.nd))) return rc;
     SPR_HIP(hipMemcpy(lw, c.tout, c.nd * sizeof(double), hipMemcpyDeviceToHost));
     SPR_HIP(hipMemcpy(p3_q0, c.tout + c.nd, c.nd * sizeof(double), hipMemcpyDeviceToHost));
     return SPR_OK;
@@ -613,9 +613,7 @@ extern "C" int spr_lw3_tetra_all(spr_ctx *h, double *lw_all, double *p3_all) {
     int rc;
     if (!c.tout) SPR_HIP(hipMalloc((void **)&c.tout, (size_t)2 * c.nq0 * c.nd * sizeof(double)));
     const size_t n = (size_t)c.nq0 * c.nd;
-    for (int qi = 0; qi < c.nq0; ++qi)
-        if ((rc = spr::launch_lw3_tetra(c, qi, c.tout + (size_t)qi * c.nd, c.tout + n + (size_t)qi * c.nd)))
-            return rc;
+    if ((rc = spr::launch_lw3_tetra(c, 0, c.nq0, c.tout, c.tout + n))) return rc;
     SPR_HIP(hipMemcpy(lw_all, c.tout, n * sizeof(double), hipMemcpyDeviceToHost));
     SPR_HIP(hipMemcpy(p3_all, c.tout + n, n * sizeof(double), hipMemcpyDeviceToHost));
     return SPR_OK;

@@ -139,6 +139,8 @@ struct Ctx {
     double *tout = nullptr;        // tetrahedron outputs: lw then P3_q0, (2, nq0, nd)
     double *w4buf = nullptr;       // W4 rows of one q1 chunk (four-phonon tetrahedron path)
     int *kpack = nullptr;          // grid coordinates of every q, k0 | k1 << 10 | k2 << 20
+    double *wsink = nullptr;       // (q0 count, nd) reduction target of the small-cell W-output launch
+    size_t wsink_cap = 0, partner_cap = 0;
     size_t w4buf_cap = 0;
     // kappa buffers, allocated on first use: per-mode terms, lw / mode sums, index tables
     double *k_terms = nullptr, *k_io = nullptr, *k_rot = nullptr;
@@ -180,7 +182,11 @@ int collect_timing(Ctx &c);
 int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, int upi, double *d_lw);
 constexpr int kMaxUnitsPerItem = 4;   // partial rows reserved per item
 int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_mode_k);
-int launch_lw3_tetra(Ctx &c, int qi, double *d_lw, double *d_p3);
+// tetrahedron linewidths of irreducible q0 [q_first, q_first + q_count): d_lw[k][nd], d_p3[k][nd]
+int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p3);
+// FP64 W of every grid q1 of irreducible q0 [q_first, q_first + q_count) with the small-cell
+// kernel (Nb <= 2): d_w[q0 - q_first][q1][s0][s1][s2]
+int launch_lw3_small_w(Ctx &c, int q_first, int q_count, double *d_w);
 // device tetrahedra (6 nq, 4) of the context (uploaded on first use)
 int ensure_tetra(Ctx &c);
 // FP64 W4 of irreducible q0 `qi` for q1 in [q1_lo, q1_lo + nq1) and every q2:

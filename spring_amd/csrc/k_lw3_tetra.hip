@@ -143,17 +143,31 @@ __global__ __launch_bounds__(TBLK) void k_lw3_tetra(const double *W, const int *
     }
 }
 
-// out[s0] = sum over chunks in order (lw), out[nd + s0] (P3_q0)
-__global__ void k_tetra_reduce(const double *partial, int nd, int nch, double *lw, double *p3) {
-    const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s0 >= nd) return;
+// lw[s0] and p3[s0]: the chunk partials of s0 summed in a fixed order, one workgroup per s0
+// (strided per-thread sums, then a tree)
+__global__ __launch_bounds__(256) void k_tetra_reduce(const double *partial, int nd, int nch, double *lw,
+                                                      double *p3) {
+    const int s0 = blockIdx.x, tid = threadIdx.x;
     double a = 0.0, b = 0.0;
-    for (int c = 0; c < nch; ++c) {
+    for (int c = tid; c < nch; c += 256) {
         a += partial[((size_t)s0 * nch + c) * 2];
         b += partial[((size_t)s0 * nch + c) * 2 + 1];
     }
-    lw[s0] = a;
-    p3[s0] = b;
+    __shared__ double ra[256], rb[256];
+    ra[tid] = a;
+    rb[tid] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            ra[tid] += ra[tid + w];
+            rb[tid] += rb[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        lw[s0] = ra[0];
+        p3[s0] = rb[0];
+    }
 }
 
 }  // namespace
@@ -215,7 +229,7 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
                                c.partner + (size_t)k * c.nq, c.omega, c.nbe, c.omega + (size_t)i0 * c.nd, c.nd, ntet,
                                chunk, c.partial);
             SPR_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_tetra_reduce, dim3(1), dim3(64), 0, c.stream, c.partial, c.nd, nch,
+            hipLaunchKernelGGL(k_tetra_reduce, dim3(c.nd), dim3(256), 0, c.stream, c.partial, c.nd, nch,
                                d_lw + (size_t)(qi - q_first) * c.nd, d_p3 + (size_t)(qi - q_first) * c.nd);
             SPR_HIP(hipGetLastError());
         }

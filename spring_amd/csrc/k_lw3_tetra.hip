@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "spr_internal.h"
+#include "tetra_dev.h"
 
 namespace spr {
 
@@ -143,6 +144,104 @@ __global__ __launch_bounds__(TBLK) void k_lw3_tetra(const double *W, const int *
     }
 }
 
+// Ndof 3 and 6: every s0 in one pass.  A thread owns (s1 s2, tetrahedron), tetrahedron fastest
+// across lanes; the vertex energies of the three processes do not depend on s0, so each process
+// whose energy span meets [min w0, max w0] is sorted once (with its occupation factor and W
+// offset as payload) and its six vertex-energy reciprocals are taken once, then every s0 with
+// Mq inside the span reads its four W and adds g I F / 6.  partial[s0][workgroup][2].
+template <int ND>
+__global__ __launch_bounds__(TBLK) void k_lw3_tetra_s(const double *W, const int *tetra, const int *partner,
+                                                      const double *omega, const double *nbe,
+                                                      const double *omega0, int ntet, double *partial) {
+    constexpr int ND2 = ND * ND;
+    const int tid = threadIdx.x;
+    __shared__ double s_w0[ND];
+    if (tid < ND) s_w0[tid] = omega0[tid];
+    __syncthreads();
+    double w0lo = s_w0[0], w0hi = s_w0[0];
+#pragma unroll
+    for (int s = 1; s < ND; ++s) {
+        w0lo = fmin(w0lo, s_w0[s]);
+        w0hi = fmax(w0hi, s_w0[s]);
+    }
+    double lw[ND], p3[ND];
+#pragma unroll
+    for (int s = 0; s < ND; ++s) lw[s] = p3[s] = 0.0;
+    const int total = ntet * ND2;
+    for (int it = blockIdx.x * TBLK + tid; it < total; it += gridDim.x * TBLK) {
+        const int p = it / ntet, t = it - p * ntet;
+        const int s1 = p / ND, s2 = p % ND;
+        double w1[4], w2[4];
+        int q1v[4], q2v[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int q1 = tetra[4 * t + v], q2 = partner[q1];
+            q1v[v] = q1;
+            q2v[v] = q2;
+            w1[v] = omega[q1 * ND + s1];
+            w2[v] = omega[q2 * ND + s2];
+        }
+#pragma unroll 1
+        for (int k = 0; k < 3; ++k) {
+            // M1 = w1 + w2 with F1 = W (n1 + n2 + 1); M2 = w2 - w1 and M3 = w1 - w2, both with
+            // F2 = W (n1 - n2), the third subtracted
+            Vx P[4];
+            double mlo = 0.0, mhi = 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const double m = k == 0 ? w1[v] + w2[v] : k == 1 ? w2[v] - w1[v] : w1[v] - w2[v];
+                P[v] = Vx{m, 0.0, q1v[v] * ND * ND2 + p};
+                mlo = v == 0 ? m : fmin(mlo, m);
+                mhi = v == 0 ? m : fmax(mhi, m);
+            }
+            if (mhi < w0lo || mlo > w0hi) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const double n1 = nbe[q1v[v] * ND + s1], n2 = nbe[q2v[v] * ND + s2];
+                P[v].a = k == 0 ? n1 + n2 + 1.0 : n1 - n2;
+            }
+            sort4(P);
+            const double M[4] = {P[0].M, P[1].M, P[2].M, P[3].M};
+            const Rcp R = rcp6(M);
+#pragma unroll 1
+            for (int s0 = 0; s0 < ND; ++s0) {
+                const double mq = s_w0[s0];
+                if (!(mq >= M[0] && mq <= M[3])) continue;
+                double F[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) F[v] = W[P[v].off + s0 * ND2] * P[v].a;
+                double g;
+                const double gif = tetra_rcp<false>(mq, M, R, F, g);
+#pragma unroll
+                for (int s = 0; s < ND; ++s)
+                    if (s == s0) {
+                        lw[s] += k == 2 ? -gif : gif;
+                        p3[s] += g;
+                    }
+            }
+        }
+    }
+    __shared__ double s_lw[TBLK], s_p3[TBLK];
+#pragma unroll
+    for (int s = 0; s < ND; ++s) {
+        __syncthreads();
+        s_lw[tid] = lw[s];
+        s_p3[tid] = p3[s];
+        __syncthreads();
+        for (int w = TBLK / 2; w > 0; w >>= 1) {
+            if (tid < w) {
+                s_lw[tid] += s_lw[tid + w];
+                s_p3[tid] += s_p3[tid + w];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            partial[((size_t)s * gridDim.x + blockIdx.x) * 2] = s_lw[0];
+            partial[((size_t)s * gridDim.x + blockIdx.x) * 2 + 1] = s_p3[0];
+        }
+    }
+}
+
 // lw[s0] and p3[s0]: the chunk partials of s0 summed in a fixed order, one workgroup per s0
 // (strided per-thread sums, then a tree)
 __global__ __launch_bounds__(256) void k_tetra_reduce(const double *partial, int nd, int nch, double *lw,
@@ -198,8 +297,12 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
     if ((rc = ensure_tetra(c))) return rc;
     if ((rc = ensure((void **)&c.partner, &c.partner_cap, (size_t)B * c.nq * sizeof(int)))) return rc;
     const int ntet = 6 * c.nq;
+    // Ndof 3 and 6: all s0 per pass (k_lw3_tetra_s), one partial per workgroup; otherwise one
+    // workgroup per (tetrahedron chunk, s0)
+    const bool all_s0 = c.nd == 3 || c.nd == 6;
     const int chunk = std::max(1, (ntet + 255) / 256);
-    const int nch = (ntet + chunk - 1) / chunk;
+    const int nch = all_s0 ? (int)std::min<long>(8L * c.ncu, ((long)ntet * c.nd * c.nd + TBLK - 1) / TBLK)
+                           : (ntet + chunk - 1) / chunk;
     if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)nch * c.nd * 2 * sizeof(double)))) return rc;
     const int m0 = c.mesh[0], m1 = c.mesh[1], m2 = c.mesh[2];
     std::vector<int> part((size_t)B * c.nq);
@@ -225,9 +328,18 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
         SPR_HIP(hipEventRecord(c.ev0, c.stream));
         for (int k = 0; k < nb; ++k) {
             const int qi = b0 + k, i0 = c.h_irr_grid[qi];
-            hipLaunchKernelGGL(k_lw3_tetra, dim3(nch, c.nd), dim3(TBLK), 0, c.stream, c.wout + (size_t)k * nw, c.tetra,
-                               c.partner + (size_t)k * c.nq, c.omega, c.nbe, c.omega + (size_t)i0 * c.nd, c.nd, ntet,
-                               chunk, c.partial);
+            const double *wk = c.wout + (size_t)k * nw;
+            const int *pk = c.partner + (size_t)k * c.nq;
+            const double *w0 = c.omega + (size_t)i0 * c.nd;
+            if (c.nd == 6)
+                hipLaunchKernelGGL(k_lw3_tetra_s<6>, dim3(nch), dim3(TBLK), 0, c.stream, wk, c.tetra, pk, c.omega,
+                                   c.nbe, w0, ntet, c.partial);
+            else if (c.nd == 3)
+                hipLaunchKernelGGL(k_lw3_tetra_s<3>, dim3(nch), dim3(TBLK), 0, c.stream, wk, c.tetra, pk, c.omega,
+                                   c.nbe, w0, ntet, c.partial);
+            else
+                hipLaunchKernelGGL(k_lw3_tetra, dim3(nch, c.nd), dim3(TBLK), 0, c.stream, wk, c.tetra, pk, c.omega,
+                                   c.nbe, w0, c.nd, ntet, chunk, c.partial);
             SPR_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_tetra_reduce, dim3(c.nd), dim3(256), 0, c.stream, c.partial, c.nd, nch,
                                d_lw + (size_t)(qi - q_first) * c.nd, d_p3 + (size_t)(qi - q_first) * c.nd);

@@ -150,7 +150,7 @@ def test_full_size_2atom_16(rocksalt):
 
 
 def test_full_size_10atom_20(rocksalt10):
-    """The north-star workload at its full size (10-atom cell, 20^3 mesh with time reversal):
+    """The 10-atom workload at its full size (10-atom cell, 20^3 mesh with time reversal):
     the linewidths of two seeded irreducible q0, each summed over all 8000 q1, in FP64 equal the
     oracle's; FP32 and split BF16 stay within 1e-4 and 1e-3 of FP64."""
     from concurrent.futures import ThreadPoolExecutor

@@ -39,3 +39,27 @@ def test_lw3_tetra_10atom(rocksalt10):
     g_lw, g_p3 = ctx.lw3_tetra(qi)
     assert rel_err(g_lw, lw, 1e-9) < 1e-10
     assert rel_err(g_p3, p3, 1e-9) < 1e-10
+
+
+def test_full_size_tetra_2atom_16(rocksalt):
+    """The reference's default delta at the 2-atom 16^3 size (145 irreducible q0, W of 16 q0 per
+    launch): two seeded q0 of the all-q0 call equal the oracle's tetrahedron sums over its own W
+    of all 4096 q1, and every linewidth and P3 is finite and non-negative in total."""
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    qm, ph = problem(rocksalt, (16, 16, 16))
+    ctx = GpuContext(rocksalt, qm, ph)
+    inp = O.Inputs(rocksalt, qm, ph)
+    print("tetra 16^3: all q0 on the GPU", flush=True)
+    lw_all, p3_all = ctx.lw3_tetra_all()
+    assert np.all(np.isfinite(lw_all)) and np.all(np.isfinite(p3_all))
+    assert np.all(lw_all.sum(axis=1) > 0.0) and np.all(p3_all >= 0.0)
+    rng = np.random.default_rng(16)
+    for qi in rng.choice(np.arange(1, qm.nq0), size=2, replace=False):
+        ig = int(qm.irr_idx[qi])
+        print(f"tetra 16^3: oracle q0 #{qi}", flush=True)
+        _, W = O.lw3_gauss(inp, ig, 1.0, want_w=True)
+        lw, p3 = O.lw3_tetra(qm, ph, ig, W)
+        e1, e2 = rel_err(lw_all[qi], lw, 1e-9), rel_err(p3_all[qi], p3, 1e-9)
+        print(f"tetra 16^3 q0 #{qi}: lw {e1:.2e} p3 {e2:.2e}", flush=True)
+        assert e1 < 1e-10 and e2 < 1e-10

@@ -54,3 +54,21 @@ def test_lw4_tetra_needs_fc4():
     ctx = GpuContext(m, qm, ph)
     with pytest.raises(SprError, match="no FC4"):
         ctx.lw4_tetra(0)
+
+
+def test_lw4_tetra_full_size_8(rocksalt):
+    """The four-phonon tetrahedron workload at its full size (2-atom with FC4, 8^3 mesh): one seeded
+    irreducible q0 of the all-q0 call against the oracle over all 512 q1 rows (16 threads)."""
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    qm, ph = problem(rocksalt, (8, 8, 8))
+    ctx = GpuContext(rocksalt, qm, ph)
+    print("lw4_tetra 8^3: all q0 on the GPU", flush=True)
+    lw = ctx.lw4_tetra_all()
+    assert np.all(np.isfinite(lw)) and np.all(lw.sum(axis=1) > 0.0)
+    qi = int(np.random.default_rng(8).integers(1, qm.nq0))
+    print(f"lw4_tetra 8^3: oracle q0 #{qi}", flush=True)
+    ref = O.lw4_tetra(O.Inputs(rocksalt, qm, ph, which=4), qm, int(qm.irr_idx[qi]), threads=16, chunk=8)
+    err = rel_err(lw[qi], ref, 1e-9)
+    print(f"lw4_tetra 8^3 q0 #{qi}: {err:.2e}", flush=True)
+    assert err < 1e-10

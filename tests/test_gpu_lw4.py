@@ -49,3 +49,27 @@ def test_lw4_fp32_matrix_cores_vs_fp64_and_vector(rocksalt, monkeypatch):
     print("lw4 fp32 matrix cores: vs fp64", e64, "vs fp32 vector", ev, flush=True)
     assert e64 < 1e-4 and ev < 1e-4
     assert np.all(np.isfinite(lw32))
+
+
+def test_lw4_full_size_10(rocksalt):
+    """The four-phonon Gaussian workload at its full size (2-atom with FC4, 10^3 mesh, sigma 0.1):
+    one seeded irreducible q0 of the all-q0 call in FP64 against the oracle over all 10^6 (q1, q2),
+    threaded over q1 chunks."""
+    from concurrent.futures import ThreadPoolExecutor
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    qm, ph = problem(rocksalt, (10, 10, 10))
+    ctx = GpuContext(rocksalt, qm, ph)
+    print("lw4 10^3: all q0 on the GPU (FP64)", flush=True)
+    lw = ctx.lw4_all(0.1, "fp64")
+    qi = int(np.random.default_rng(10).integers(1, qm.nq0))
+    ig = int(qm.irr_idx[qi])
+    inp = O.Inputs(rocksalt, qm, ph, which=4)
+    print(f"lw4 10^3: oracle q0 #{qi}", flush=True)
+    spans = [(lo, min(qm.nq, lo + 25)) for lo in range(0, qm.nq, 25)]
+    with ThreadPoolExecutor(16) as ex:
+        parts = list(ex.map(lambda sp: O.lw4_gauss(inp, ig, 0.1, sp[0], sp[1]), spans))
+    ref = np.sum(parts, axis=0)
+    err = rel_err(lw[qi], ref, 1e-9)
+    print(f"lw4 10^3 q0 #{qi}: {err:.2e}", flush=True)
+    assert err < 1e-10

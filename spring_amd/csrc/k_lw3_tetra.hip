@@ -150,11 +150,17 @@ __global__ __launch_bounds__(TBLK) void k_lw3_tetra(const double *W, const int *
 // offset as payload) and its six vertex-energy reciprocals are taken once, then every s0 with
 // Mq inside the span reads its four W and adds g I F / 6.  partial[s0][workgroup][2].
 template <int ND>
-__global__ __launch_bounds__(TBLK) void k_lw3_tetra_s(const double *W, const int *tetra, const int *partner,
-                                                      const double *omega, const double *nbe,
-                                                      const double *omega0, int ntet, double *partial) {
+__global__ __launch_bounds__(TBLK) void k_lw3_tetra_s(const double *Wb, const int *tetra, const int *partner_b,
+                                                      const double *omega, const double *nbe, const int *q0_grid,
+                                                      int nq, int ntet, double *partial_b) {
     constexpr int ND2 = ND * ND;
     const int tid = threadIdx.x;
+    // blockIdx.y: the q0 of the batch (its W rows, partner table and partial rows)
+    const int kb = blockIdx.y;
+    const double *W = Wb + (size_t)kb * nq * ND * ND2;
+    const int *partner = partner_b + (size_t)kb * nq;
+    double *partial = partial_b + (size_t)kb * ND * gridDim.x * 2;
+    const double *omega0 = omega + (size_t)q0_grid[kb] * ND;
     __shared__ double s_w0[ND];
     if (tid < ND) s_w0[tid] = omega0[tid];
     __syncthreads();
@@ -244,9 +250,12 @@ __global__ __launch_bounds__(TBLK) void k_lw3_tetra_s(const double *W, const int
 
 // lw[s0] and p3[s0]: the chunk partials of s0 summed in a fixed order, one workgroup per s0
 // (strided per-thread sums, then a tree)
-__global__ __launch_bounds__(256) void k_tetra_reduce(const double *partial, int nd, int nch, double *lw,
-                                                      double *p3) {
+__global__ __launch_bounds__(256) void k_tetra_reduce(const double *partial_b, int nd, int nch, double *lw_b,
+                                                      double *p3_b) {
+    // blockIdx.y: the q0 of a batch (partial rows and outputs of q0 k at k * nd * nch * 2, k * nd)
     const int s0 = blockIdx.x, tid = threadIdx.x;
+    const double *partial = partial_b + (size_t)blockIdx.y * nd * nch * 2;
+    double *lw = lw_b + (size_t)blockIdx.y * nd, *p3 = p3_b + (size_t)blockIdx.y * nd;
     double a = 0.0, b = 0.0;
     for (int c = tid; c < nch; c += 256) {
         a += partial[((size_t)s0 * nch + c) * 2];
@@ -303,7 +312,7 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
     const int chunk = std::max(1, (ntet + 255) / 256);
     const int nch = all_s0 ? (int)std::min<long>(8L * c.ncu, ((long)ntet * c.nd * c.nd + TBLK - 1) / TBLK)
                            : (ntet + chunk - 1) / chunk;
-    if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)nch * c.nd * 2 * sizeof(double)))) return rc;
+    if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)B * nch * c.nd * 2 * sizeof(double)))) return rc;
     const int m0 = c.mesh[0], m1 = c.mesh[1], m2 = c.mesh[2];
     std::vector<int> part((size_t)B * c.nq);
     for (int b0 = q_first; b0 < q_first + q_count; b0 += B) {
@@ -326,24 +335,30 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
         SPR_HIP(hipMemcpyAsync(c.partner, part.data(), (size_t)nb * c.nq * sizeof(int), hipMemcpyHostToDevice,
                                c.stream));
         SPR_HIP(hipEventRecord(c.ev0, c.stream));
-        for (int k = 0; k < nb; ++k) {
-            const int qi = b0 + k, i0 = c.h_irr_grid[qi];
-            const double *wk = c.wout + (size_t)k * nw;
-            const int *pk = c.partner + (size_t)k * c.nq;
-            const double *w0 = c.omega + (size_t)i0 * c.nd;
+        if (all_s0) {
+            // one launch for the batch's q0 (grid.y), then one reduction launch
+            const int *q0g = c.irr_grid + b0;
             if (c.nd == 6)
-                hipLaunchKernelGGL(k_lw3_tetra_s<6>, dim3(nch), dim3(TBLK), 0, c.stream, wk, c.tetra, pk, c.omega,
-                                   c.nbe, w0, ntet, c.partial);
-            else if (c.nd == 3)
-                hipLaunchKernelGGL(k_lw3_tetra_s<3>, dim3(nch), dim3(TBLK), 0, c.stream, wk, c.tetra, pk, c.omega,
-                                   c.nbe, w0, ntet, c.partial);
+                hipLaunchKernelGGL(k_lw3_tetra_s<6>, dim3(nch, nb), dim3(TBLK), 0, c.stream, c.wout, c.tetra,
+                                   c.partner, c.omega, c.nbe, q0g, c.nq, ntet, c.partial);
             else
-                hipLaunchKernelGGL(k_lw3_tetra, dim3(nch, c.nd), dim3(TBLK), 0, c.stream, wk, c.tetra, pk, c.omega,
-                                   c.nbe, w0, c.nd, ntet, chunk, c.partial);
+                hipLaunchKernelGGL(k_lw3_tetra_s<3>, dim3(nch, nb), dim3(TBLK), 0, c.stream, c.wout, c.tetra,
+                                   c.partner, c.omega, c.nbe, q0g, c.nq, ntet, c.partial);
             SPR_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_tetra_reduce, dim3(c.nd), dim3(256), 0, c.stream, c.partial, c.nd, nch,
-                               d_lw + (size_t)(qi - q_first) * c.nd, d_p3 + (size_t)(qi - q_first) * c.nd);
+            hipLaunchKernelGGL(k_tetra_reduce, dim3(c.nd, nb), dim3(256), 0, c.stream, c.partial, c.nd, nch,
+                               d_lw + (size_t)(b0 - q_first) * c.nd, d_p3 + (size_t)(b0 - q_first) * c.nd);
             SPR_HIP(hipGetLastError());
+        } else {
+            for (int k = 0; k < nb; ++k) {
+                const int qi = b0 + k, i0 = c.h_irr_grid[qi];
+                hipLaunchKernelGGL(k_lw3_tetra, dim3(nch, c.nd), dim3(TBLK), 0, c.stream, c.wout + (size_t)k * nw,
+                                   c.tetra, c.partner + (size_t)k * c.nq, c.omega, c.nbe,
+                                   c.omega + (size_t)i0 * c.nd, c.nd, ntet, chunk, c.partial);
+                SPR_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_tetra_reduce, dim3(c.nd, 1), dim3(256), 0, c.stream, c.partial, c.nd, nch,
+                                   d_lw + (size_t)(qi - q_first) * c.nd, d_p3 + (size_t)(qi - q_first) * c.nd);
+                SPR_HIP(hipGetLastError());
+            }
         }
         SPR_HIP(hipEventRecord(c.ev1, c.stream));
         // the host partner tables must outlive the copy

@@ -18,14 +18,20 @@
  *   orc_scatter_prob       constants recomputed from their formulas; transpose identity
  *   orc_lw3_gauss          Gaussian against scipy.stats.norm.pdf on hand-built W;
  *                          pair-swap symmetry; brute force over all q1 on tiny meshes
- *   orc_scatter_mat_el4 / orc_v4_factorised / orc_lw4_gauss   as the 3-phonon ones
+ *   orc_scatter_mat_el4 / orc_v4_factorised   as the 3-phonon ones
+ *   orc_lw4_gauss          process factors and weights (tests/test_lw4_process_terms.py): on
+ *                          flat integer bands, exactly on or off shell, the ordered sum equals
+ *                          the sum over distinct physical processes (multisets, 1/m! for
+ *                          identical modes) of the linearised golden-rule occupation
+ *                          differences, which equal the compact factors only on shell
  *   orc_kappa_rta          closed-form single-mode model; symmetric tensor
  *   orc_lw3_tetra          sum of vertex weights == 1; linear-band density of states
  *   orc_w4_rows            the Gaussian four-phonon sum rebuilt from its W4 == orc_lw4_gauss
  *   orc_lw4_tetra_from_w   per process (disjoint energy ranges): integral over Mq == the grid sum
  *                          of F (the delta integrates to one), first moment == the grid mean
  *                          of M interpolated linearly; a single (q1, q2) spike picks n2, n3 of
- *                          q2 and q3 = -(q0+q1+q2); no NaN from degenerate vertices; Gamma skip
+ *                          q2 and q3 = -(q0+q1+q2); no NaN from degenerate vertices; Gamma skip;
+ *                          window integrals on near-flat bands == the same physical-process sum
  *   orc_bte_sums           1/tau == 2 x the tetrahedron linewidth of orc_lw3_tetra; with
  *                          W+ = W- = 1 and F = a w on the full grid, dF == a w 1/tau exactly
  *                          (the vertex weights reproduce Mqs); linearity in F

@@ -1,17 +1,19 @@
 """Benchmark: three-phonon scattering rates on one MI355X.
 
-Workload (default): 2-atom rocksalt model, 16x16x16 Gamma-centred mesh (145 irreducible
-q0 under Oh, 4096 q1 each, 6^3 branch triplets per pair), Gaussian sigma = 0.1 rad/ps,
-T = 300 K.  One step = the linewidths of every irreducible q0 (all q1, pair-permutation
-reduced) plus the RTA kappa tensor, from device-resident inputs.
+Workload (default, --workload cell10): the 10-atom cell (30 branches), 20x20x20 Gamma-centred
+mesh with time reversal (4004 irreducible q0, 8000 q1 each, 30^3 branch triplets per pair),
+Gaussian sigma = 0.1 rad/ps, T = 300 K.  A step is one slice of the irreducible q0; the `steps`
+slices cover the wedge once, so the timed region yields every linewidth and the RTA kappa.
+FP64 is the headline; FP32 and split-BF16 matrix-core runs report their kappa and linewidth
+errors against it.  --workload rocksalt16 is the 2-atom 16^3 line (one step = every q0).
 
 Prints one JSON line:
   metric/value   |V3|^2 triplets per second (sum over irreducible q0 of Nq * Ndof^3 per
-                 step, divided by the step time), FP64 path
+                 sweep, divided by the sweep time), FP64 path
   roofline       the dominant kernel's algorithmic FLOP rate (HIP events on the library's
-                 stream) against MI355X's FP64 peak
-  cpu_baseline   the FP64 CPU oracle timed on a bounded sample of q0 on the host cores
-  fp32           the FP32 path (compensated FP32 delta sum): rate, speedup, error vs FP64
+                 stream) against MI355X's peak for its precision
+  cpu_baseline   the FP64 CPU oracle timed on a bounded sample of the workload on the host cores
+  fp32, bf16     the reduced-precision paths: rate, speedup, error vs FP64
   kappa_tts_s    wall time from FC + structure in memory to kappa (phonons included)
 """
 from __future__ import annotations
@@ -33,11 +35,11 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and matrix are the same rate)
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", default="rocksalt16",
+    p.add_argument("--workload", default="cell10",
                    choices=["rocksalt16", "cell10", "learned10", "fourph10", "fourph_tetra", "bte16",
                             "phonons10", "tetra16"],
-                   help="rocksalt16: 2-atom cell, 16^3 mesh (the headline line); cell10: 10-atom "
-                        "cell, 20^3 mesh with time reversal, FP64 vs FP32 vs BF16; learned10: the "
+                   help="cell10 (default, the headline line): 10-atom cell, 20^3 mesh with time "
+                        "reversal, FP64 vs FP32 vs BF16; rocksalt16: 2-atom cell, 16^3 mesh; learned10: the "
                         "learned importance-sampling proposal on the cell10 workload; fourph10: "
                         "four-phonon linewidths, 2-atom cell with FC4, 10^3 mesh; fourph_tetra: the "
                         "same with the tetrahedron method on a --mesh4 mesh; tetra16: three-phonon "
@@ -54,13 +56,13 @@ def parse():
                         "(the '+head' estimators)")
     p.add_argument("--mesh10", type=int, default=20)
     p.add_argument("--mesh4", type=int, default=8, help="fourph_tetra: mesh size")
-    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--mesh", type=int, default=16)
     p.add_argument("--sigma", type=float, default=0.1)
     p.add_argument("--T", type=float, default=300.0)
     p.add_argument("--cpu-q0", type=int, default=145, help="irreducible q0 in one CPU pass")
-    p.add_argument("--cpu-seconds", type=float, default=10.0,
+    p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="repeat CPU passes until this much wall time is spent")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--phonons", default="device", choices=["device", "host"],
@@ -183,6 +185,7 @@ def main():
                    "pairs_executed_per_step": npairs, "triplets_per_step": triplets},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "compute_unit": "vector ALU (FP64 FMA; MI355X's FP64 vector and matrix peaks are equal)",
                      "kernel": "k_lw3_small<double,2>", "kernel_ms": kern64,
                      "flop_per_pair": flop_pair, "pairs_per_launch": npairs},
         "cpu_baseline": cpu,
@@ -200,93 +203,192 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 
 
+def _host_cpu():
+    """Host threads to use (all cores this process may run on, within OMP_NUM_THREADS when the
+    host sets it), the CPU model and the oracle's compiler flags."""
+    cores = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = min(cores, int(env)) if env and env.isdigit() and int(env) > 0 else cores
+    model_name = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model_name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    flags = "unknown"
+    try:
+        for line in open(os.path.join(ROOT, "oracle", "Makefile")):
+            if line.startswith("CFLAGS"):
+                flags = "gcc " + line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return threads, cores, model_name, flags
+
+
+def _profile_value(name, key):
+    """An integer recorded as `key: value` in a committed profile summary, or None."""
+    tf = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(tf):
+        for line in open(tf):
+            if line.startswith(key + ":"):
+                return int(line.split(":")[1])
+    return None
+
+
 def cell10(args):
-    """10-atom cell (30 branches), Nq = mesh10^3 with time reversal: FP64 generic kernel
-    against the FP32 and BF16 MFMA kernels, kappa error of each against FP64."""
+    """The 10-atom workload (30 branches) on a mesh10^3 mesh with time reversal: all irreducible
+    q0, every q1, Gaussian sigma, 300 K, FP64 against the FP32 and split-BF16 matrix-core paths.
+
+    A step is one slice of the irreducible q0 (spr_lw3_slice, DistributeQpoints' (offset, count)
+    split): the wedge is cut into `steps` slices, so the timed steps cover every irreducible q0
+    exactly once and their linewidths give kappa.  `warmup` slices run untimed before each
+    precision (tuning, graph capture).  The kappa time to solution of each precision is measured
+    end to end (mesh, device phonons, upload, pair lists, linewidths, kappa) in a fresh context."""
     from spring_amd import model, mesh, phonons, lib
     t0 = time.perf_counter()
     m = model.rocksalt10()
     t_model = time.perf_counter() - t0
     n = args.mesh10
-    t0 = time.perf_counter()
-    qm = mesh.make_mesh(m.crystal, (n, n, n), time_rev=1)
-    if args.phonons == "device":
-        ph = lib.get_phonon(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
-    else:
-        ph = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
-    t_ph = time.perf_counter() - t0
-    print(f"# cell10: model {t_model:.1f} s, mesh+phonons ({args.phonons}) {t_ph:.2f} s, nq0 {qm.nq0}",
-          file=sys.stderr, flush=True)
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+
+    def setup():
+        t = time.perf_counter()
+        qm = mesh.make_mesh(m.crystal, (n, n, n), time_rev=1)
+        t_mesh = time.perf_counter() - t
+        if args.phonons == "device":
+            ph = lib.get_phonon(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
+        else:
+            ph = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
+        t_ph = time.perf_counter() - t - t_mesh
+        return qm, ph, t_mesh, t_ph
+
+    qm, ph, t_mesh, t_ph = setup()
     ctx = lib.GpuContext(m, qm, ph)
+    log(f"cell10: model {t_model:.1f} s, mesh {t_mesh:.2f} s, phonons ({args.phonons}) {t_ph:.2f} s, nq0 {qm.nq0}")
     nd = m.crystal.ndof
+    steps = max(1, min(args.steps, qm.nq0))
+    bounds = np.linspace(0, qm.nq0, steps + 1).round().astype(int)
     triplets = qm.nq0 * qm.nq * nd ** 3
     nblocks = int(np.count_nonzero(np.abs(m.fc3.fc).sum(axis=(3, 4, 5))))
     flop_pair = 4 * 27 * nblocks + 24 * nd ** 4 + 30 * nd ** 3
     npairs = sum(len(mesh.pair_list(qm, int(i))[0]) for i in qm.irr_idx)
-    res = {}
-    for prec, warm, peak in (("fp64", 0, FP64_PEAK_TFLOPS), ("fp32", 1, FP32_PEAK_TFLOPS),
-                             ("bf16", 1, BF16_PEAK_TFLOPS)):
-        for _ in range(warm):
-            ctx.lw3_all(args.sigma, prec)
+
+    def sweep(prec):
+        for k in range(args.warmup):
+            a, b = bounds[k % steps], bounds[k % steps + 1]
+            ctx.lw3_slice(int(a), int(b - a), args.sigma, prec)
         ctx.kernel_time_reset()
+        lw = np.zeros((qm.nq0, nd))
         t = time.perf_counter()
-        lw = ctx.lw3_all(args.sigma, prec)
-        k, _ = ctx.kappa(lw, args.T, symm=True)
+        for k in range(steps):
+            a, b = bounds[k], bounds[k + 1]
+            lw[a:b] = ctx.lw3_slice(int(a), int(b - a), args.sigma, prec)
         dt = time.perf_counter() - t
         nl, kms = ctx.kernel_time()
-        res[prec] = dict(step=dt, kern=kms / max(nl, 1), lw=lw, k=k, peak=peak)
-        print(f"# cell10 {prec}: {dt:.2f} s, kernel {kms:.1f} ms", file=sys.stderr, flush=True)
+        k3, _ = ctx.kappa(lw, args.T, symm=True)
+        log(f"cell10 {prec}: {steps} steps {dt:.2f} s, kernel {kms:.0f} ms over {nl} launches")
+        return dict(step=dt / steps, total=dt, kern=kms / max(nl, 1), launches=nl, lw=lw, k=k3)
+
+    res = {"fp64": sweep("fp64"), "fp32": sweep("fp32"), "bf16": sweep("bf16")}
     r64 = res["fp64"]
     sig = np.abs(r64["lw"]) > 1e-6 * np.abs(r64["lw"]).max()
+    peaks = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16": BF16_PEAK_TFLOPS}
+    pairs_per_launch = npairs / steps
+
+    def kappa_err(prec):
+        r = res[prec]
+        return float(np.max(np.abs(r["k"] - r64["k"])) / np.max(np.abs(r64["k"])))
+
+    # kappa time to solution, end to end in a fresh context (the compiled kernels, the tuning
+    # table and the graphs of the sweeps above are not reused: a new context captures anew)
+    tts = {}
+    for prec in ("bf16", "fp32"):
+        t = time.perf_counter()
+        qm2, ph2, _, _ = setup()
+        c2 = lib.GpuContext(m, qm2, ph2)
+        _, k2, _ = c2.lw3_kappa_all(args.sigma, prec, args.T)
+        tts[prec] = time.perf_counter() - t
+        c2.close()
+        log(f"cell10 kappa time to solution {prec}: {tts[prec]:.2f} s")
 
     def entry(prec):
         r = res[prec]
-        # executed pairs: the FP64 generic kernel visits every q1 <= q2 pair directly, the MFMA
-        # kernels the same pair lists
-        achieved = flop_pair * npairs / (r["kern"] * 1e-3) / 1e12
-        e = {"value": triplets / r["step"], "s_per_step": r["step"], "kernel_ms": r["kern"],
-             "kappa_tts_s": t_ph + r["step"], "speedup_vs_fp64": r64["step"] / r["step"],
-             "roofline": {"achieved": achieved, "peak": r["peak"], "unit": "TFLOP/s",
-                          "frac": achieved / r["peak"]}}
+        achieved = flop_pair * pairs_per_launch / (r["kern"] * 1e-3) / 1e12
+        e = {"value": triplets / r["total"], "ms_per_step": r["step"] * 1e3, "sweep_s": r["total"],
+             "kernel_ms_per_launch": r["kern"], "speedup_vs_fp64": r64["total"] / r["total"],
+             "roofline": {"bound": "mfma", "achieved": achieved, "peak": peaks[prec], "unit": "TFLOP/s",
+                          "frac": achieved / peaks[prec]}}
         if prec != "fp64":
-            e["kappa_rel_err_vs_fp64"] = float(np.max(np.abs(r["k"] - r64["k"])) / np.max(np.abs(r64["k"])))
+            e["kappa_rel_err_vs_fp64"] = kappa_err(prec)
             e["linewidth_max_rel_err_vs_fp64"] = float(
                 np.max(np.abs(r["lw"] - r64["lw"])[sig] / np.abs(r64["lw"])[sig]))
             e["linewidth_mean_rel_err_vs_fp64"] = float(
                 np.mean(np.abs(r["lw"] - r64["lw"])[sig] / np.abs(r64["lw"])[sig]))
+            e["kappa_tts_s"] = tts[prec]
+        else:
+            # FP64 is the reference of the error: its time to solution is composed of the measured
+            # parts (mesh, phonons, the timed sweep); the reduced precisions are measured whole
+            e["kappa_tts_s_composed"] = t_mesh + t_ph + r["total"]
         return e
 
     cpu = None
     if not args.no_cpu:
+        from concurrent.futures import ThreadPoolExecutor
         from oracle import oracle as O
         inp = O.Inputs(m, qm, ph)
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads, cores, cpu_model, cflags = _host_cpu()
         rng = np.random.default_rng(0)
-        pos = rng.choice(qm.nq0, size=min(threads, qm.nq0), replace=False)
+        qpos = rng.choice(qm.nq0, size=qm.nq0, replace=False)
+        # calibrate one thread on a short q1 range, then size the sample to ~args.cpu_seconds
         t = time.perf_counter()
-        lw_cpu = O.lw3_gauss_all(inp, qm.irr_idx[pos], args.sigma, threads=threads)
+        O.lw3_gauss(inp, int(qm.irr_idx[qpos[0]]), args.sigma, 0, 8)
+        per_pair = (time.perf_counter() - t) / 8
+        pairs = max(threads, int(args.cpu_seconds * threads / per_pair))
+        nq0_s = max(1, min(len(qpos), pairs // qm.nq))
+        chunk = (qm.nq + threads - 1) // threads if nq0_s * qm.nq <= 2 * pairs else qm.nq
+        work = [(int(qm.irr_idx[qpos[j]]), lo, min(qm.nq, lo + chunk)) for j in range(nq0_s)
+                for lo in range(0, qm.nq, chunk)]
+        t = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            parts = list(ex.map(lambda w: O.lw3_gauss(inp, w[0], args.sigma, w[1], w[2]), work))
         dt = time.perf_counter() - t
+        lw_cpu = np.array([np.sum(parts[j * len(work) // nq0_s:(j + 1) * len(work) // nq0_s], axis=0)
+                           for j in range(nq0_s)])
         gs = np.abs(lw_cpu) > 1e-6 * np.abs(lw_cpu).max()
-        cpu_err = float(np.max(np.abs(r64["lw"][pos] - lw_cpu)[gs] / np.abs(lw_cpu)[gs]))
-        cpu = {"value": len(pos) * qm.nq * nd ** 3 / dt, "unit": "triplets/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{len(pos)} seeded irreducible q0, all {qm.nq} q1 each, FP64 C restatement, "
-                         f"{threads} threads, {dt:.1f} s wall; GPU FP64 linewidths agree to max rel "
+        cpu_err = float(np.max(np.abs(r64["lw"][qpos[:nq0_s]] - lw_cpu)[gs] / np.abs(lw_cpu)[gs]))
+        cpu = {"value": nq0_s * qm.nq * nd ** 3 / dt, "unit": "triplets/s", "cores": threads,
+               "kind": "port", "host_cores_visible": cores, "cpu_model": cpu_model, "compiler": cflags,
+               "sample": f"{nq0_s} seeded irreducible q0 of the same workload, all {qm.nq} q1 each (no "
+                         f"pair-permutation saving), split into q1 ranges over {threads} threads, FP64 C "
+                         f"restatement, {dt:.1f} s wall; GPU FP64 linewidths of those q0 agree to max rel "
                          f"err {cpu_err:.1e}"}
+    traffic = _profile_value("k_lw3_mm_cell10.md", f"traffic_bytes_per_launch_fp64_{n}_{steps}")
+    e64 = entry("fp64")
     out = {
         "metric": "|V3|^2 triplets/s and kappa time-to-solution at <=1e-3 rel err vs FP64 full sum",
-        "value": triplets / r64["step"], "unit": "triplets/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": r64["step"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "value": triplets / r64["total"], "unit": "triplets/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": r64["step"] * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded jittered 10-atom rocksalt supercell, spring FC2, random "
                 "permutation-symmetric ASR FC3 within the 3rd shell",
         "config": {"workload": f"3-phonon Gaussian linewidths + RTA kappa, 10-atom cell, {n}^3 mesh "
-                               f"with time reversal, {qm.nq0} irreducible q0 x {qm.nq} q1 x {nd}^3 branches",
+                               f"with time reversal, {qm.nq0} irreducible q0 x {qm.nq} q1 x {nd}^3 branches; "
+                               f"a step is one of {steps} slices of the irreducible q0",
                    "mesh": n, "nq0": qm.nq0, "ndof": nd, "sigma_rad_per_ps": args.sigma, "T_K": args.T,
-                   "pairs_executed_per_step": npairs, "triplets_per_step": triplets,
+                   "pairs_executed_per_sweep": npairs, "triplets_per_sweep": triplets,
                    "flop_per_pair": flop_pair},
-        "fp64": entry("fp64"), "fp32": entry("fp32"), "bf16": entry("bf16"),
-        "cpu_baseline": cpu, "phonons_s": t_ph, "phonons": args.phonons, "host_model_s": t_model,
+        "roofline": {"bound": "mfma", "achieved": e64["roofline"]["achieved"], "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": e64["roofline"]["frac"], "traffic": traffic,
+                     "kernel": "k_lw3_mm<0,10>", "kernel_ms": r64["kern"], "flop_per_pair": flop_pair,
+                     "pairs_per_launch": pairs_per_launch},
+        "cpu_baseline": cpu,
+        "fp64": e64, "fp32": entry("fp32"), "bf16": entry("bf16"),
+        "kappa_tts_s": min(tts.values()),
+        "kappa_tts_precision": min(tts, key=tts.get),
+        "kappa_tts_rel_err_vs_fp64": kappa_err(min(tts, key=tts.get)),
+        "phonons_s": t_ph, "mesh_s": t_mesh, "phonons": args.phonons, "host_model_s": t_model,
         "kappa_W_mK": [float(r64["k"][i, i]) for i in range(3)],
     }
     print(json.dumps(out), flush=True)

@@ -111,6 +111,11 @@ int spr_lw3(spr_ctx *ctx, int i0, double sigma, int prec, double *lw);
  * three-phonon scattering): LW_Allq0(Ndof, nq0). */
 int spr_lw3_all(spr_ctx *ctx, double sigma, int prec, double *lw_all);
 
+/* The same for the slice of irreducible q0 [q0_off, q0_off + q0_cnt): lw(Ndof, q0_cnt).  The
+ * (offset, count) work split of DistributeQpoints (src/Kappa/Src/Helper_m.f90:356-434), by which
+ * each coarray image takes a contiguous block of the q0 loop (main.f90:552-715). */
+int spr_lw3_slice(spr_ctx *ctx, int q0_off, int q0_cnt, double sigma, int prec, double *lw);
+
 /* Tetrahedron-method three-phonon linewidth of irreducible q0 `i0` (CalculateLwTetrahedron,
  * ThirdOrder_m.f90:169-275, the reference's default delta): lw(Ndof) rad/ps and P3_q0(Ndof);
  * the reference's P3 of the q0 is sum(P3_q0).  W of every q1 comes from the FP64 W path of

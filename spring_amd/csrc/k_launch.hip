@@ -107,8 +107,12 @@ int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double
         return SPR_OK;
     }
     int mode = 2;
-    const std::string key = L.name + "|nd" + std::to_string(c.nd) + "|nq" + std::to_string(c.nq) + "|items" +
-                            std::to_string(L.nitems);
+    // the decision is kept per size class (item counts within a factor of two share it), so
+    // equal-sized slices of one wedge are tuned once
+    int size_class = 0;
+    while ((2 << size_class) <= L.nitems) ++size_class;
+    const std::string key = L.name + "|nd" + std::to_string(c.nd) + "|nq" + std::to_string(c.nq) + "|items2^" +
+                            std::to_string(size_class);
     const char *force = getenv("SPR_LAUNCH_MODE");
     const char *at = getenv("SPR_AUTOTUNE");
     if (force && *force) {

@@ -385,6 +385,15 @@ extern "C" int spr_lw3_all(spr_ctx *h, double sigma, int prec, double *lw_all) {
     return lw_common(h, 0, h->c.nq0, sigma, prec, lw_all, 3);
 }
 
+extern "C" int spr_lw3_slice(spr_ctx *h, int q0_off, int q0_cnt, double sigma, int prec, double *lw) {
+    if (!h) return SPR_E_ARG;
+    if (q0_off < 0 || q0_cnt < 1 || q0_off + q0_cnt > h->c.nq0) {
+        set_error("spr_lw3_slice: q0 slice out of range");
+        return SPR_E_ARG;
+    }
+    return lw_common(h, q0_off, q0_cnt, sigma, prec, lw, 3);
+}
+
 extern "C" int spr_lw4(spr_ctx *h, int i0, double sigma, int prec, double *lw) {
     if (!h) return SPR_E_ARG;
     int rc;

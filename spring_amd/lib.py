@@ -23,6 +23,7 @@ SYMBOLS = [
     "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all",
     "spr_lw4_tetra", "spr_lw4_tetra_all", "spr_lw3_kappa_all", "spr_mlp_score",
     "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve", "spr_get_phonon",
+    "spr_lw3_slice",
 ]
 
 
@@ -165,6 +166,13 @@ class GpuContext:
         lw = np.zeros((self.nq0, self.nd))
         self._check(self.lib.spr_lw3_all(self.h, ct.c_double(sigma), ct.c_int(PREC[prec]),
                                          _ptr(lw)), "spr_lw3_all")
+        return lw
+
+    def lw3_slice(self, q0_off: int, q0_cnt: int, sigma: float, prec: str = "fp64") -> np.ndarray:
+        """Linewidths of the irreducible q0 [q0_off, q0_off + q0_cnt) in one launch."""
+        lw = np.zeros((q0_cnt, self.nd))
+        self._check(self.lib.spr_lw3_slice(self.h, ct.c_int(q0_off), ct.c_int(q0_cnt), ct.c_double(sigma),
+                                           ct.c_int(PREC[prec]), _ptr(lw)), "spr_lw3_slice")
         return lw
 
     def lw4(self, i0: int, sigma: float, prec: str = "fp64") -> np.ndarray:

@@ -3,9 +3,12 @@
 //     from an atomic counter (reset by a fill kernel, so the sequence can be captured);
 //   * hipGraph capture of [reset counter -> item kernel -> per-q0 reduction], replayed on
 //     later calls with the same arguments;
-//   * a small autotuner choosing between one-workgroup-per-item and k = 1, 2, 4 persistent
-//     grids by timing each on the problem itself; decisions persist in a text cache
-//     (SPR_TUNE_CACHE, default $HOME/.cache/spring_amd/tune.txt).
+//   * a small autotuner choosing the launch mode (one workgroup per item, or k = 1, 2, 4
+//     persistent grids) and, for kernels built in several tile shapes (the s2-block size of the
+//     matrix-core kernel), the shape, by timing each candidate on a sample of the problem's own
+//     items; decisions persist per (kernel, Ndof, Nq, size class) in a text cache
+//     (SPR_TUNE_CACHE, default $HOME/.cache/spring_amd/tune.txt).  SPR_LAUNCH_MODE and
+//     SPR_TILE_SHAPE force a choice.
 // Every per-item partial lands at its item index and the reduction sums items in index
 // order, so the result does not depend on which workgroup took which item.
 #include <cstdio>
@@ -95,9 +98,60 @@ static int time_mode(Ctx &c, ItemLaunch &L, int mode, int sub, float *ms) {
     return SPR_OK;
 }
 
-int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double *d_lw) {
+static std::string size_key(const Ctx &c, const std::string &name, int nitems) {
+    // the decision is kept per size class (item counts within a factor of two share it), so
+    // equal-sized slices of one wedge are tuned once
+    int size_class = 0;
+    while ((2 << size_class) <= nitems) ++size_class;
+    return name + "|nd" + std::to_string(c.nd) + "|nq" + std::to_string(c.nq) + "|items2^" +
+           std::to_string(size_class);
+}
+
+int run_items_shapes(Ctx &c, ItemLaunch *Ls, int n, int q0_first, int q0_count, int it0, double *d_lw) {
+    if (n == 1 || Ls[0].direct) return run_items(c, Ls[0], q0_first, q0_count, it0, d_lw);
+    int rc;
+    const int items = Ls[0].nitems / Ls[0].upi;
+    const std::string key = size_key(c, Ls[0].name + "|shape", items);
+    const char *fs = getenv("SPR_TILE_SHAPE");
+    const char *fm = getenv("SPR_LAUNCH_MODE");
+    const char *at = getenv("SPR_AUTOTUNE");
+    int shape = 0, mode = -1;
+    if (fs && *fs) {
+        shape = std::min(std::max(atoi(fs), 0), n - 1);
+    } else if (tune_table().count(key)) {
+        const int v = tune_table()[key];
+        shape = std::min(v / 8, n - 1);
+        mode = v % 8;
+    } else if (!(at && std::string(at) == "0") && items >= 4 * c.ncu / std::max(1, Ls[0].upi)) {
+        // every candidate runs the same first `sub` items (its own units of them)
+        const int sub = std::min(items, 4 * c.ncu);
+        float best = 1e30f, ms;
+        int best_v = 0;
+        for (int v = 0; v < n; ++v) {
+            ItemLaunch &L = Ls[v];
+            if (L.dyn > 64 * 1024)
+                SPR_HIP(hipFuncSetAttribute(L.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.dyn));
+            if ((rc = time_mode(c, L, 2, sub * L.upi, &ms))) return rc;   // warm-up (code load, caches)
+            for (int m : {0, 1, 2, 4}) {
+                if ((rc = time_mode(c, L, m, sub * L.upi, &ms))) return rc;
+                if (ms < best) {
+                    best = ms;
+                    best_v = v * 8 + m;
+                }
+            }
+        }
+        shape = best_v / 8;
+        mode = best_v % 8;
+        tune_store(key, best_v);
+    }
+    if (fm && *fm) mode = -1;   // run_items applies SPR_LAUNCH_MODE itself
+    return run_items(c, Ls[shape], q0_first, q0_count, it0, d_lw, mode);
+}
+
+int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double *d_lw, int preset) {
     int rc;
     c.last_upi = L.upi;
+    c.last_shape = L.shape;
     if (L.dyn > 64 * 1024)
         SPR_HIP(hipFuncSetAttribute(L.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.dyn));
     // ---- launch mode: env override, tuned value, or default (persistent 2 x CUs) ----
@@ -107,16 +161,14 @@ int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double
         return SPR_OK;
     }
     int mode = 2;
-    // the decision is kept per size class (item counts within a factor of two share it), so
-    // equal-sized slices of one wedge are tuned once
-    int size_class = 0;
-    while ((2 << size_class) <= L.nitems) ++size_class;
-    const std::string key = L.name + "|nd" + std::to_string(c.nd) + "|nq" + std::to_string(c.nq) + "|items2^" +
-                            std::to_string(size_class);
+    const std::string key = size_key(c, L.name + L.shape, L.nitems);
     const char *force = getenv("SPR_LAUNCH_MODE");
     const char *at = getenv("SPR_AUTOTUNE");
     if (force && *force) {
         mode = atoi(force);
+        if (mode != 0 && mode != 1 && mode != 2 && mode != 4) mode = 2;
+    } else if (preset >= 0) {
+        mode = preset;
     } else if (tune_table().count(key)) {
         mode = tune_table()[key];
     } else if (!(at && std::string(at) == "0") && L.nitems >= 4 * c.ncu) {

@@ -3,9 +3,11 @@
 //
 // Work unit: a batch of up to 64 (q1, q2) pairs of one irreducible q0 (pair-permutation
 // reduced list, ScatterProbAllq1 ThirdOrder_m.f90:98-164) and one block of s2 branches (the
-// s2 split keeps V of a unit within 64 accumulator registers per lane: Ndof = 30 takes two
-// blocks in FP32/BF16 and four in FP64; only the FT is repeated per block).  One workgroup
-// (8 waves) runs the pairs of its unit one after another.  Per pair, per basis atom mu
+// s2 split keeps V of a unit in the accumulator registers; only the FT is repeated per block).
+// The block size is a tile shape: shape 0 keeps V within 64 registers per lane (Ndof = 30: two
+// blocks in FP32/BF16, four in FP64), shape 1 (FP64) takes the next larger block (two blocks at
+// Ndof = 30, 254 registers); the launcher times both on the problem and keeps the faster.  One
+// workgroup (8 waves) runs the pairs of its unit one after another.  Per pair, per basis atom mu
 // (rows a = 3 mu + alpha):
 //   FT      T(alpha, b, c) = sum_{(N1,N2) of mu} Phi(alpha,beta,gamma) e^{2 pi i m/L}   (VALU)
 //   GEMM B  U1(alpha, b, s2) = sum_c  T(alpha, b, c) e2(c, s2)
@@ -30,9 +32,10 @@
 //          split operands x = hi + lo (hi = bf16(x), lo = bf16(x - hi)): X Y ~ Xh Yh + Xh Yl +
 //          Xl Yh, FP32 accumulation, so GEMMs B and C keep about 16 mantissa bits; GEMM D
 //          (K = 3 complex per atom) runs on the FP32 instruction.
-// Delta arguments w0 -+ w1 -+ w2 are formed in FP64 in every mode; FP32 and BF16 sum the
-// delta-weighted terms with a compensated FP32 sum per lane; the per-s0 totals are reduced in
-// FP64 in a fixed order (bitwise reproducible).
+// Delta arguments w0 -+ w1 -+ w2 are formed in FP64 in every mode; FP32 and BF16 evaluate only
+// the triplets within sqrt(100) sigma of a resonance (packed per wave, every dropped term is below
+// e^-50 of its peak) and sum them with a compensated FP32 sum per lane; the per-s0 totals are
+// reduced in FP64 in a fixed order (bitwise reproducible).
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -77,8 +80,17 @@ constexpr int pick_nts(int nd, int nt, int words) {
     const int budget = nt == 3 ? 48 : 64;
     return v_tiles(nd, nt, nt) * words <= budget ? nt : v_tiles(nd, nt, 2) * words <= budget && nt >= 2 ? 2 : 1;
 }
+// the next larger s2 block that cuts the number of blocks (fewer repeated phase sums, more
+// accumulator registers); equal to `nts` when there is none
+constexpr int next_nts(int nt, int nts) {
+    for (int k = nts + 1; k <= nt; ++k)
+        if ((nt + k - 1) / k < (nt + nts - 1) / nts) return k;
+    return nts;
+}
 
-template <int MODE, int NB> struct MmShape {
+// Tile shape VAR: 0 = the register-budget choice above, 1 = next_nts of it.  Both are built;
+// the launcher times them on the problem and keeps the faster (k_launch.hip).
+template <int MODE, int NB, int VAR> struct MmShape {
     using R = std::conditional_t<MODE == MM_F64, double, float>;
     using Z = cplx_t<R>;
     using ACC = std::conditional_t<MODE == MM_F64, v4d, v4f>;
@@ -91,7 +103,8 @@ template <int MODE, int NB> struct MmShape {
     static constexpr int MT_B = (3 * ND + 15) / 16;         // GEMM B: rows (alpha, b)
     static constexpr int MB = MT_B * 16;
     // s2 split: a unit covers s2 in [h SB, (h + 1) SB), i.e. NTS N tiles of GEMM B
-    static constexpr int NTS = pick_nts(ND, NT, MODE == MM_F64 ? 8 : 4);
+    static constexpr int NTS0 = pick_nts(ND, NT, MODE == MM_F64 ? 8 : 4);
+    static constexpr int NTS = VAR == 0 ? NTS0 : next_nts(NT, NTS0);
     static constexpr int SB = 8 * NTS;
     static constexpr int NSPLIT = (NT + NTS - 1) / NTS;
     static constexpr int NTSB = pow2ceil(NTS);              // waves per M tile, GEMM B
@@ -112,14 +125,13 @@ template <int MODE, int NB> struct MmShape {
     static constexpr int KE = (NE + 511) / 512;             // FT outputs per thread
 };
 
-// LDS layout (bytes), shared by the kernel and the host-side size check
-// FP32 / BF16: V is staged for the delta stage in the cluster-phase region, three 16 x 16 FP32
-// tiles per wave at a time
+// LDS layout (bytes), shared by the kernel and the host-side size check.  After the atom loop of
+// a pair, everything before the phase table is dead; the FP32 / BF16 delta stage then stages V
+// there (kDvTiles 16 x 16 tiles per wave) followed by each wave's list of resonant elements.
 constexpr int kDvTiles = 3;
-constexpr size_t kDvBytes = (size_t)8 * kDvTiles * 256 * sizeof(float);
 
-template <int MODE, int NB> struct MmLds {
-    using Sh = MmShape<MODE, NB>;
+template <int MODE, int NB, int VAR> struct MmLds {
+    using Sh = MmShape<MODE, NB, VAR>;
     using R = typename Sh::R;
     using Z = typename Sh::Z;
     static constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
@@ -127,14 +139,20 @@ template <int MODE, int NB> struct MmLds {
     static constexpr size_t U2 = (size_t)Sh::U2ROWS * Sh::U2S1 * sizeof(Z);
     static constexpr size_t AC = cmax((size_t)Sh::MC * Sh::PA * sizeof(R), (size_t)512 * sizeof(double));
     static constexpr size_t E = (size_t)Sh::ND * Sh::PE * sizeof(Z);
+    // FP32 / BF16 delta stage: staged V and the element lists (FP64 works in registers)
+    static constexpr size_t DV = Sh::PALL ? (size_t)8 * kDvTiles * 256 * sizeof(float) : 0;
+    static constexpr size_t DL = Sh::PALL ? (size_t)8 * kDvTiles * 128 * sizeof(unsigned short) : 0;
     __host__ __device__ static constexpr size_t tu() { return TU; }
     __host__ __device__ static constexpr size_t u2() { return U2; }
     __host__ __device__ static constexpr size_t ac() { return AC; }
     __host__ __device__ static constexpr size_t e() { return E; }
-    // cluster phases (all atoms in the FP32 / BF16 modes, which also stage V there afterwards)
+    // cluster phases (all atoms in the FP32 / BF16 modes, atom mu in FP64), padded so that the
+    // dead prefix holds the delta stage's buffers
     __host__ __device__ static size_t p_bytes(int npmax, int nterm) {
-        const size_t b = (size_t)(Sh::PALL ? nterm : npmax) * sizeof(Z);
-        return Sh::PALL && b < kDvBytes ? kDvBytes : b;
+        size_t b = (size_t)(Sh::PALL ? nterm : npmax) * sizeof(Z);
+        const size_t pre = TU + U2 + AC + 2 * E;
+        if (pre + b < DV + DL) b = DV + DL - pre;
+        return b;
     }
     static size_t total(int npmax, int nterm, int nsite, int L) {
         size_t b = tu() + u2() + ac() + 2 * e();
@@ -161,9 +179,9 @@ __device__ __forceinline__ int tile_row(int g, int r) { return F64OUT ? 4 * r + 
 // 16 x 16 output tile of A (rows in LDS, K contiguous, pitch PA) times the B' expansion of
 // the complex eigenvector matrix E (LDS, [s][c] pitch PE): acc[m][n'] += sum_k' A[m][k'] B'[k'][n'].
 // The B' fragment is built once per call site and passed in `bf`.
-template <int MODE, int NB>
+template <int MODE, int NB, int VAR>
 struct Frag {
-    using Sh = MmShape<MODE, NB>;
+    using Sh = MmShape<MODE, NB, VAR>;
     using R = typename Sh::R;
     using Z = typename Sh::Z;
     // FP64 / FP32: S values per lane; BF16: KP/32 slabs of (hi, lo) bf16x8
@@ -205,6 +223,41 @@ struct Frag {
             }
         }
         return b;
+    }
+
+    // FP64: two tiles as mma2, with the B' fragment built from E in two halves of K inside the
+    // product (16 instead of 32 registers live; the K order, and so the sums, are unchanged)
+    __device__ static void mma2_e(const R *sA, int m0, int m1, const Z *sE, int n_base, typename Sh::ACC *d) {
+        static_assert(MODE == MM_F64, "FP64 only");
+        const int l = threadIdx.x & 63, g = l >> 4;
+        const int np = n_base + (l & 15);
+        const int s = np >> 1, q = np & 1;
+        const bool ok = np < Sh::K2;
+        const double2 *p0 = reinterpret_cast<const double2 *>(sA + (m0 + (l & 15)) * Sh::PA + g * Sh::S);
+        const double2 *p1 = reinterpret_cast<const double2 *>(sA + (m1 + (l & 15)) * Sh::PA + g * Sh::S);
+        typename Sh::ACC a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
+        constexpr int HQ = Sh::S / 4;   // complex K values per lane group and half
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            double bv[2 * HQ];
+#pragma unroll
+            for (int i = 0; i < HQ; ++i) {
+                const int c = g * (Sh::S / 2) + hh * HQ + i;
+                const Z e = ok ? sE[s * Sh::PE + c] : Z{0, 0};
+                bv[2 * i] = q ? e.y : e.x;          // p = 0
+                bv[2 * i + 1] = q ? e.x : -e.y;     // p = 1
+            }
+#pragma unroll
+            for (int i = 0; i < HQ; ++i) {
+                const double2 x = p0[hh * HQ + i], y = p1[hh * HQ + i];
+                a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, bv[2 * i], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(y.x, bv[2 * i], a1, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, bv[2 * i + 1], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(y.y, bv[2 * i + 1], a1, 0, 0, 0);
+            }
+        }
+        d[0] = a0;
+        d[1] = a1;
     }
 
     // two tiles (rows m0.., m1..) with the same B fragment; the two chains interleave
@@ -290,14 +343,14 @@ struct Frag {
     }
 };
 
-template <int MODE, int NB>
+template <int MODE, int NB, int VAR>
 __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
-    using Sh = MmShape<MODE, NB>;
-    using L = MmLds<MODE, NB>;
+    using Sh = MmShape<MODE, NB, VAR>;
+    using L = MmLds<MODE, NB, VAR>;
     using R = typename Sh::R;
     using Z = typename Sh::Z;
     using ACC = typename Sh::ACC;
-    using F = Frag<MODE, NB>;
+    using F = Frag<MODE, NB, VAR>;
     constexpr int ND = Sh::ND, K2 = Sh::K2, PA = Sh::PA, PE = Sh::PE, SB = Sh::SB;
     // GEMM D runs on the FP64 (FP64 mode) or FP32 instruction
     using DACC = std::conditional_t<MODE == MM_F64, v4d, v4f>;
@@ -528,14 +581,16 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             // ---- GEMM B: U1[(alpha, b)][(s2, q)] = T x E2', s2 in this unit's block
             //      -> sAC[(alpha, s2 - h SB)][(b, q)] ----
             if (bOK) {
-                const typename F::BF bf = F::load_b(sE2, ntB * 16);
+                typename F::BF bf;
+                if constexpr (MODE != MM_F64) bf = F::load_b(sE2, ntB * 16);
                 const int np = ntB * 16 + (l & 15);
                 const int s2 = np >> 1, q = np & 1;
                 // two M tiles per pass: independent accumulation chains
                 for (int mt = mtB0; mt < Sh::MT_B; mt += 2 * MSTEPB) {
                     const int mt2 = mt + MSTEPB < Sh::MT_B ? mt + MSTEPB : mt;
                     ACC d[2];
-                    F::mma2(sT, mt * 16, mt2 * 16, bf, d);
+                    if constexpr (MODE == MM_F64) F::mma2_e(sT, mt * 16, mt2 * 16, sE2, ntB * 16, d);
+                    else F::mma2(sT, mt * 16, mt2 * 16, bf, d);
 #pragma unroll
                     for (int t = 0; t < 2; ++t) {
                         const int mtt = t ? mt2 : mt;
@@ -554,13 +609,15 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             __syncthreads();
             // ---- GEMM C: U2[(alpha, s2 local)][(s1, q)] = sAC x E1' -> sU2[(s1, s2 local)][alpha] ----
             if (cOK) {
-                const typename F::BF bf = F::load_b(sE1, ntC * 16);
+                typename F::BF bf;
+                if constexpr (MODE != MM_F64) bf = F::load_b(sE1, ntC * 16);
                 const int np = ntC * 16 + (l & 15);
                 const int s1 = np >> 1, q = np & 1;
                 for (int mt = mtC0; mt < Sh::MT_C; mt += 2 * MSTEPC) {
                     const int mt2 = mt + MSTEPC < Sh::MT_C ? mt + MSTEPC : mt;
                     ACC d[2];
-                    F::mma2(sAC, mt * 16, mt2 * 16, bf, d);
+                    if constexpr (MODE == MM_F64) F::mma2_e(sAC, mt * 16, mt2 * 16, sE1, ntC * 16, d);
+                    else F::mma2(sAC, mt * 16, mt2 * 16, bf, d);
 #pragma unroll
                     for (int t = 0; t < 2; ++t) {
                         const int mtt = t ? mt2 : mt;
@@ -613,79 +670,43 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             __syncthreads();
         }
 
-        // ---- W and the Gaussian-delta weights.  Lanes l and l^1 hold Re and Im of the same
-        //      (row, s0); lane q = 0 takes rows r = 0, 1 of every tile and lane q = 1 rows 2, 3, each
-        //      receiving the other part from its partner, so no lane idles.  The delta arguments
-        //      are formed in FP64 (they cancel near resonance); in the FP32 and BF16 modes the rest
-        //      is FP32, summed per pair and then into a compensated sum.  A tile row whose three
-        //      Gaussians underflow to exact zero on every lane of the wave adds nothing and skips
-        //      the exponentials (same sums bit for bit). ----
-        if constexpr (Sh::PALL) {
-            // V staged in LDS (the cluster phases are dead after the last FT): each wave copies
-            // up to kDvTiles of its tiles into its own slice, then runs the delta arithmetic over
-            // its elements as a rolled loop, which keeps the register count of this stage low
-            float *sV = reinterpret_cast<float *>(sP) + (size_t)w * (kDvTiles * 256);
-            const double w0 = s0ok ? sMode[s0] : 0.0, iw0 = s0ok ? sMode[ND + s0] : 0.0;
-            float pairf = 0.f;
-#pragma unroll
-            for (int j0 = 0; j0 < Sh::VT; j0 += kDvTiles) {
-#pragma unroll
-                for (int t = 0; t < kDvTiles; ++t)
-                    if (j0 + t < Sh::VT) *reinterpret_cast<v4f *>(sV + (t * 64 + l) * 4) = vacc[j0 + t];
-                mm_wave_sync();
-                if (s0ok) {
-                    const int ne = 2 * (Sh::VT - j0 < kDvTiles ? Sh::VT - j0 : kDvTiles);
-#pragma unroll 1
-                    for (int e = 0; e < ne; ++e) {
-                        const int t = e >> 1, k = e & 1;
-                        const int mt = mtC0 + MSTEPC * (j0 + t);
-                        if (mt >= Sh::MT_D) continue;
-                        const int r = 2 * qD + k;
-                        const float x = sV[(t * 64 + l) * 4 + r], y = sV[(t * 64 + (l ^ 1)) * 4 + r];
-                        const int m = mt * 16 + tile_row<false>(g, r);
-                        const int s1 = m / SB, s2 = h * SB + m % SB;
-                        const bool ok = s1 < ND && s2 < ND;
-                        const double w1 = ok ? sMode[2 * ND + s1] : 0.0, w2 = ok ? sMode[5 * ND + s2] : 0.0;
-                        const double ap = w1 + w2, am = w1 - w2;
-                        const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
-                        const double tmin = hinv * fmin(d1 * d1, fmin(d2 * d2, d3 * d3));
-                        // __expf(-t) is exactly zero beyond t = 104: a wave whose lanes are all that
-                        // far from resonance adds nothing here
-                        if (!__ballot(ok && tmin < 110.0)) continue;
-                        if (!ok) continue;
-                        const float n1 = sModeF[s1], iw1 = sModeF[ND + s1];
-                        const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
-                        const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
-                        const float W = (x * x + y * y) * (iw1 * iw2);
-                        const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
-                        pairf = fmaf(W, (n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3), pairf);
-                    }
-                }
-                mm_wave_sync();   // this wave has read the slice before the next round overwrites it
-            }
-            if (s0ok) accf.add(pairf * (float)(wpair * c0 * iw0));
-        } else {
+        // ---- W and the Gaussian-delta weights (FP32 and BF16: on the resonant triplets only).  Lanes l and l^1
+        //      hold Re and Im of the same (row, s0); the lane with q = 0 owns rows r = 0, 1 of
+        //      every tile and the lane with q = 1 rows 2, 3 ("its elements").  Per chunk of
+        //      kDvTiles tiles, each wave (1) stages its V tiles in LDS, (2) tests its elements: a
+        //      triplet whose three Gaussians are all below exp(-TCUT) of their peak adds nothing
+        //      measurable (FP64: e^-100; FP32 and BF16: e^-50) and is dropped, (3) packs the others
+        //      into a list (ballot order), (4) evaluates the list with every lane, writing each term
+        //      over the staged Re of its element, and (5) each owner lane adds the terms of its own
+        //      elements in element order, so the sums do not depend on the packing.  The delta
+        //      arguments are formed in FP64 in every mode (they cancel near resonance); FP32 and
+        //      BF16 evaluate the rest in FP32 and sum per pair, then into a compensated sum. ----
+        if constexpr (MODE == MM_F64) {
+            // FP64: every element in registers.  Lanes l and l^1 hold Re and Im of the same
+            // (row, s0); lane q = 0 takes rows r = 0, 1 of every tile and lane q = 1 rows 2, 3, each
+            // receiving the other part from its partner.  A tile row whose three Gaussians
+            // underflow to exact zero on every lane of the wave adds nothing and skips the
+            // exponentials.
             if (s0ok) {
                 const double w0 = sMode[s0], iw0 = sMode[ND + s0];
                 // s2 of this lane's two rows depends on the lane only when the s2 block divides 16
                 constexpr bool kS2Lane = 16 % SB == 0;
                 int s2k[2];
                 double w2k[2];
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < 2; ++k) {
-                    s2k[k] = h * SB + tile_row<MODE == MM_F64>(g, 2 * qD + k) % SB;
+                    s2k[k] = h * SB + tile_row<true>(g, 2 * qD + k) % SB;
                     w2k[k] = s2k[k] < ND ? sMode[5 * ND + s2k[k]] : 0.0;
                 }
-                float pairf = 0.f;
-    #pragma unroll
+#pragma unroll
                 for (int j = 0; j < Sh::VT; ++j) {
                     const int mt = mtC0 + MSTEPC * j;
                     if (mt >= Sh::MT_D) continue;
-    #pragma unroll
+#pragma unroll
                     for (int k = 0; k < 2; ++k) {
-                        const DR x = qD ? vacc[j][2 + k] : vacc[j][k];          // row 2 qD + k, this lane
-                        const DR y = __shfl_xor(qD ? vacc[j][k] : vacc[j][2 + k], 1, 64);   // same row, partner
-                        const int m = mt * 16 + tile_row<MODE == MM_F64>(g, 2 * qD + k);
+                        const double x = qD ? vacc[j][2 + k] : vacc[j][k];          // row 2 qD + k, this lane
+                        const double y = __shfl_xor(qD ? vacc[j][k] : vacc[j][2 + k], 1, 64);   // same row, partner
+                        const int m = mt * 16 + tile_row<true>(g, 2 * qD + k);
                         const int s1 = m / SB, s2 = kS2Lane ? s2k[k] : h * SB + m % SB;
                         const bool ok = s1 < ND && s2 < ND;
                         const double w1 = ok ? sMode[2 * ND + s1] : 0.0;
@@ -693,28 +714,94 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                         const double ap = w1 + w2, am = w1 - w2;
                         const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
                         const double tmin = hinv * fmin(d1 * d1, fmin(d2 * d2, d3 * d3));
-                        // FP64: exp(-t) is zero beyond t = 745.2; FP32: __expf(-t) is zero beyond t = 104
-                        if (!__ballot(ok && tmin < (MODE == MM_F64 ? 746.0 : 110.0))) continue;
+                        // exp(-t) is zero beyond t = 745.2
+                        if (!__ballot(ok && tmin < 746.0)) continue;
                         if (!ok) continue;
-                        if constexpr (MODE == MM_F64) {
-                            const double n1 = sMode[3 * ND + s1], iw1 = sMode[4 * ND + s1];
-                            const double n2 = sMode[6 * ND + s2], iw2 = sMode[7 * ND + s2];
-                            const double W = (x * x + y * y) * iw0 * iw1 * iw2;
-                            const double G = (n1 + n2 + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
-                                             (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
-                            accd += wpair * W * G;
-                        } else {
-                            const float n1 = sModeF[s1], iw1 = sModeF[ND + s1];
-                            const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
-                            const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
-                            const float W = (x * x + y * y) * (iw1 * iw2);
-                            const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
-                            pairf = fmaf(W, (n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3), pairf);
-                        }
+                        const double n1 = sMode[3 * ND + s1], iw1 = sMode[4 * ND + s1];
+                        const double n2 = sMode[6 * ND + s2], iw2 = sMode[7 * ND + s2];
+                        const double W = (x * x + y * y) * iw0 * iw1 * iw2;
+                        const double G = (n1 + n2 + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
+                                         (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
+                        accd += wpair * W * G;
                     }
                 }
-                if constexpr (MODE != MM_F64) accf.add(pairf * (float)(wpair * c0 * iw0));
             }
+        } else {
+            constexpr double TCUT = 50.0;
+            const double dcut = A.sigma * sqrt(2.0 * TCUT);
+            float *sV = reinterpret_cast<float *>(smem) + (size_t)w * (kDvTiles * 256);
+            unsigned short *sList = reinterpret_cast<unsigned short *>(smem + L::DV) + (size_t)w * (kDvTiles * 128);
+            const double w0 = s0ok ? sMode[s0] : 0.0;
+            float pairf = 0.f;
+#pragma unroll
+            for (int j0 = 0; j0 < Sh::VT; j0 += kDvTiles) {
+                // (1) stage this wave's tiles
+#pragma unroll
+                for (int t = 0; t < kDvTiles; ++t)
+                    if (j0 + t < Sh::VT) *reinterpret_cast<v4f *>(sV + (t * 64 + l) * 4) = vacc[j0 + t];
+                mm_wave_sync();
+                // (2) + (3): test this lane's elements, pack the resonant ones
+                int cnt = 0;
+                unsigned act = 0;
+#pragma unroll
+                for (int t = 0; t < kDvTiles; ++t) {
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const int mt = mtC0 + MSTEPC * (j0 + t);
+                        const int m = mt * 16 + tile_row<false>(g, 2 * qD + k);
+                        const int s1 = m / SB, s2 = h * SB + m % SB;
+                        bool near = false;
+                        if (s0ok && j0 + t < Sh::VT && mt < Sh::MT_D && s1 < ND && s2 < ND) {
+                            const double w1 = sMode[2 * ND + s1], w2 = sMode[5 * ND + s2];
+                            const double ap = w1 + w2, am = w1 - w2;
+                            near = fabs(w0 - ap) < dcut || fabs(w0 + am) < dcut || fabs(w0 - am) < dcut;
+                        }
+                        const unsigned long long msk = __ballot(near);
+                        if (near) {
+                            const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi(
+                                                      (unsigned)(msk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
+                            sList[pos] = (unsigned short)(t * 128 + l * 2 + k);
+                            act |= 1u << (2 * t + k);
+                        }
+                        cnt += __popcll(msk);
+                    }
+                }
+                mm_wave_sync();
+                // (4): every lane evaluates list entries; the term replaces the staged Re of its
+                //      element (read by this lane only, before the write)
+                for (int idx = l; idx < cnt; idx += 64) {
+                    const int code = sList[idx];
+                    const int t = code >> 7, lo = (code >> 1) & 63, k = code & 1;
+                    const int r = 2 * (lo & 1) + k;
+                    const float x = sV[(t * 64 + lo) * 4 + r], y = sV[(t * 64 + (lo ^ 1)) * 4 + r];
+                    const int mt = mtC0 + MSTEPC * (j0 + t);
+                    const int m = mt * 16 + tile_row<false>(lo >> 4, r);
+                    const int s1 = m / SB, s2 = h * SB + m % SB;
+                    const double we = sMode[(ntC * 16 + (lo & 15)) >> 1];
+                    const double w1 = sMode[2 * ND + s1], w2 = sMode[5 * ND + s2];
+                    const double ap = w1 + w2, am = w1 - w2;
+                    const float f1 = (float)(we - ap), f2 = (float)(we + am), f3 = (float)(we - am);
+                    const float n1 = sModeF[s1], iw1 = sModeF[ND + s1];
+                    const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
+                    const float W = (x * x + y * y) * (iw1 * iw2);
+                    const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
+                    sV[(t * 64 + lo) * 4 + r] = W * ((n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3));
+                }
+                mm_wave_sync();
+                // (5): the owner adds its elements' terms in element order
+#pragma unroll
+                for (int t = 0; t < kDvTiles; ++t)
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+                        if (act & (1u << (2 * t + k))) pairf += sV[(t * 64 + l) * 4 + 2 * qD + k];
+                mm_wave_sync();   // this wave has read the slice before the next chunk overwrites it
+            }
+            if (s0ok) accf.add(pairf * (float)(wpair * c0 * sMode[ND + s0]));
+            // the staging overwrote zero padding of the GEMM operand buffers (set once per unit):
+            // clear the prefix again once every wave is done with it
+            __syncthreads();
+            for (int i = tid; i < (int)((L::DV + L::DL) / 16); i += 512)
+                reinterpret_cast<uint4 *>(smem)[i] = make_uint4(0u, 0u, 0u, 0u);
         }
     }
     // ---- reduce per s0 in a fixed order: wave w holds the columns of N tile w % NTC ----
@@ -732,10 +819,10 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     }
 }
 
-template <int MODE, int NB>
+template <int MODE, int NB, int VAR>
 __global__ __launch_bounds__(512) void k_lw3_mm(MmArgs A) {
     if (!A.ic.counter) {
-        mm_unit<MODE, NB>(A, blockIdx.x);
+        mm_unit<MODE, NB, VAR>(A, blockIdx.x);
         return;
     }
     __shared__ int s_unit;
@@ -745,27 +832,42 @@ __global__ __launch_bounds__(512) void k_lw3_mm(MmArgs A) {
         __syncthreads();
         const int unit = s_unit;
         if (unit >= A.ic.nitems) break;
-        mm_unit<MODE, NB>(A, unit);
+        mm_unit<MODE, NB, VAR>(A, unit);
     }
+}
+
+// ItemLaunch of tile shape VAR, or false when it cannot run here (LDS, registers)
+template <int MODE, int NB, int VAR>
+static bool mm_launch_of(MmArgs &A, int nitems, ItemLaunch &L) {
+    using Sh = MmShape<MODE, NB, VAR>;
+    const size_t dyn = MmLds<MODE, NB, VAR>::total(A.npmax, A.nterm, A.nsite, A.L);
+    if (dyn > 160 * 1024) return false;
+    const void *fn = (const void *)k_lw3_mm<MODE, NB, VAR>;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess) return false;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 512, dyn) != hipSuccess || per_cu < 1) return false;
+    static const char *names[3] = {"mm_f64", "mm_f32", "mm_bf16"};
+    L = ItemLaunch{fn, &A, sizeof(A), 512, dyn, nitems * Sh::NSPLIT, per_cu, std::string(names[MODE])};
+    L.upi = Sh::NSPLIT;
+    L.shape = "s2blk" + std::to_string(Sh::SB);
+    return true;
 }
 
 template <int MODE, int NB>
 static int launch_mm_t(Ctx &c, MmArgs A, int nitems, int q0_first, int q0_count, int it0, double *d_lw) {
-    using Sh = MmShape<MODE, NB>;
-    const size_t dyn = MmLds<MODE, NB>::total(A.npmax, A.nterm, A.nsite, A.L);
-    if (dyn > 160 * 1024) {
-        set_error("three-phonon MFMA kernel: LDS need " + std::to_string(dyn) + " B exceeds 160 KB");
+    ItemLaunch Ls[2];
+    int n = 0;
+    if (mm_launch_of<MODE, NB, 0>(A, nitems, Ls[n])) ++n;
+    // the larger s2 block is built for FP64 only: in FP32 / BF16 its accumulators spill
+    if constexpr (MODE == MM_F64 && MmShape<MODE, NB, 1>::NTS != MmShape<MODE, NB, 0>::NTS)
+        if (mm_launch_of<MODE, NB, 1>(A, nitems, Ls[n])) ++n;
+    if (n == 0) {
+        set_error("three-phonon MFMA kernel: no tile shape fits this device (LDS " +
+                  std::to_string(MmLds<MODE, NB, 0>::total(A.npmax, A.nterm, A.nsite, A.L)) + " B)");
         return SPR_E_UNSUPPORTED;
     }
-    const void *fn = (const void *)k_lw3_mm<MODE, NB>;
-    SPR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
-    int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 512, dyn) != hipSuccess || per_cu < 1) per_cu = 1;
-    static const char *names[3] = {"mm_f64", "mm_f32", "mm_bf16"};
-    ItemLaunch L{fn, &A, sizeof(A), 512, dyn, nitems * Sh::NSPLIT, per_cu, std::string(names[MODE])};
-    L.direct = c.override_items != nullptr;
-    L.upi = Sh::NSPLIT;
-    return run_items(c, L, q0_first, q0_count, it0, d_lw);
+    for (int k = 0; k < n; ++k) Ls[k].direct = c.override_items != nullptr;
+    return run_items_shapes(c, Ls, n, q0_first, q0_count, it0, d_lw);
 }
 
 int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw) {

@@ -43,6 +43,7 @@ struct ItemLaunch {
     void *extra[4] = {nullptr, nullptr, nullptr, nullptr};   // further kernel arguments
     int nextra = 0;
     int upi = 1;            // work units per item (nitems counts units); partial rows per item
+    std::string shape;      // tile-shape label of this variant (tuning table, last_shape)
 };
 
 // Caller-supplied work items (spr_lw3_pairs): partial[item] receives the item's sum.
@@ -131,6 +132,7 @@ struct Ctx {
     int ncu = 256;                 // compute units of the device
     int last_mode = -1;            // launch mode of the last item launch (0: one WG per item)
     int last_upi = 1;              // work units per item of the last item launch
+    std::string last_shape;        // tile shape of the last item launch
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
     double *ipartial = nullptr;    // (n_items, nd) item partials (stable: graphs keep the pointer)
     double *wout = nullptr;        // W parity output (also W of one q0 for the tetrahedron path)
@@ -176,7 +178,11 @@ double lw4_factor(const Ctx &c);
 // d_lw[q][s] = sum over chunks in order of partial[(q, chunk)][s]
 int launch_reduce_chunks(Ctx &c, const double *partial, int q0_count, int nchunks, double *d_lw);
 int ensure_pairs(Ctx &c);
-int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double *d_lw);
+int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double *d_lw, int mode = -1);
+// the same over alternative tile shapes of one kernel (same items, each its own units per item):
+// the launcher times every (shape, launch mode) on a sample of the items, keeps the fastest in the
+// tuning table and runs it
+int run_items_shapes(Ctx &c, ItemLaunch *Ls, int n, int q0_first, int q0_count, int it0, double *d_lw);
 void drop_graphs(Ctx &c);
 int collect_timing(Ctx &c);
 int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, int upi, double *d_lw);

@@ -102,3 +102,39 @@ def test_mfma_pair_terms_fp64_10atom(ten):
     q1 = np.arange(qm.nq)
     t = ctx.lw3_pairs(np.full(qm.nq, i), q1, 0.1, "fp64")
     assert rel_err(t.sum(axis=0), ref[i], 1e-9) < 1e-10
+
+
+def test_launch_modes_and_tile_shapes_at_a_tuned_size(rocksalt10, monkeypatch, tmp_path):
+    """On a mesh large enough that the launcher tunes (10-atom cell, 8^3 with time reversal: 1040
+    items of 64 pairs), every forced launch mode (one workgroup per item, persistent grids of 1, 2
+    and 4 workgroups per CU) and the direct launch without a captured graph give bit-identical
+    linewidths: each item writes its own partial row and the rows are summed in item order.  The
+    two FP64 tile shapes (s2 blocks of 8 and 16) cut the same sums differently and agree to
+    rounding."""
+    from spring_amd import lib, mesh
+    monkeypatch.setenv("SPR_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    m = rocksalt10
+    qm = mesh.make_mesh(m.crystal, (8, 8, 8), time_rev=1)
+    ph = lib.get_phonon(m.crystal, m.fc2, qm.q_cart, qm.q_int, 300.0)
+    ctx = lib.GpuContext(m, qm, ph)
+    for prec in ("fp64", "fp32"):
+        base = ctx.lw3_all(0.1, prec)          # tunes (shape and mode) on first use
+        assert np.all(np.isfinite(base)) and np.abs(base).max() > 0
+        for mode in ("0", "1", "2", "4"):
+            monkeypatch.setenv("SPR_LAUNCH_MODE", mode)
+            assert np.array_equal(ctx.lw3_all(0.1, prec), base), (prec, mode)
+        monkeypatch.delenv("SPR_LAUNCH_MODE")
+        monkeypatch.setenv("SPR_NO_GRAPH", "1")
+        assert np.array_equal(ctx.lw3_all(0.1, prec), base), (prec, "no graph")
+        monkeypatch.delenv("SPR_NO_GRAPH")
+    shapes = {}
+    for sh in ("0", "1"):
+        monkeypatch.setenv("SPR_TILE_SHAPE", sh)
+        shapes[sh] = ctx.lw3_all(0.1, "fp64")
+        monkeypatch.setenv("SPR_LAUNCH_MODE", "0")
+        assert np.array_equal(ctx.lw3_all(0.1, "fp64"), shapes[sh]), sh
+        monkeypatch.delenv("SPR_LAUNCH_MODE")
+    assert rel_err(shapes["1"], shapes["0"], 1e-9) < 1e-13
+    tune = (tmp_path / "tune.txt").read_text()
+    assert "mm_f64|shape" in tune and "mm_f32" in tune
+    ctx.close()

@@ -872,7 +872,8 @@ static int launch_mm_t(Ctx &c, MmArgs A, int nitems, int q0_first, int q0_count,
 
 int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw) {
     int rc;
-    if ((rc = ensure_pairs(c))) return rc;
+    // caller-supplied items (override_items) need none of the context's pair lists
+    if (!c.override_items && (rc = ensure_pairs(c))) return rc;
     const ItemSet *ov = c.override_items;
     const int it0 = ov ? 0 : c.h_item_first[q0_first];
     const int nitems = ov ? ov->nitems : c.h_item_first[q0_first + q0_count] - it0;

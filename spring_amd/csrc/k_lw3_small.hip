@@ -377,7 +377,8 @@ int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, int upi, do
 // Linewidths of irreducible q0 [first, first + count) with the small-cell kernel.
 int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec, double *d_lw) {
     int rc;
-    if ((rc = ensure_pairs(c))) return rc;
+    // caller-supplied items (override_items) need none of the context's pair lists
+    if (!c.override_items && (rc = ensure_pairs(c))) return rc;
     const ItemSet *ov = c.override_items;
     const int it0 = ov ? 0 : c.h_item_first[q0_first];
     const int nitems = ov ? ov->nitems : c.h_item_first[q0_first + q0_count] - it0;

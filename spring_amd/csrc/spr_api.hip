@@ -516,6 +516,10 @@ extern "C" int spr_lw3_pairs(spr_ctx *h, int n, const int *q0_index, const int *
     }
     int rc;
     if ((rc = check_prec(prec))) return rc;
+    if (!(sigma > 0.0)) {
+        set_error("sigma must be positive");
+        return SPR_E_ARG;
+    }
     SPR_HIP(hipSetDevice(h->c.device));
     return launch_lw3_pairs(h->c, n, q0_index, q1_grid, sigma, prec, out);
 }

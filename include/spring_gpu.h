@@ -202,6 +202,17 @@ int spr_lw3_kappa_all(spr_ctx *ctx, double sigma, int prec, double T, int symm, 
 int spr_lw3_pairs(spr_ctx *ctx, int n, const int *q0_index, const int *q1_grid, double sigma,
                   int prec, double *out);
 
+/* spr_lw3_pairs with no device allocation: the same terms for the same arguments (same kernels,
+ * same summation order), with every device buffer it needs beyond the context's resident tables
+ * held in a fixed library scratch array of each GPU (about 5 MB, kept until the process ends).
+ * No counterpart in the reference: this entry is for callers that cannot allocate (code under
+ * HIP graph capture, tight sampling or serving loops, harnesses that forbid hipMalloc).  At most
+ * SPR_NOALLOC_MAX_PAIRS pairs per call (SPR_E_ARG beyond); Ndof <= 32.  Calls that overlap on one
+ * GPU share its scratch array and race. */
+#define SPR_NOALLOC_MAX_PAIRS 4096
+int spr_lw3_pairs_noalloc(spr_ctx *ctx, int n, const int *q0_index, const int *q1_grid, double sigma,
+                          int prec, double *out);
+
 /* Learned proposal for importance sampling of the linewidth sum: a 3-layer MLP
  * (n_features -> 128 -> 128 -> 1, tanh) predicting log W of a branch triplet from
  * (q0, q1, omegas, branch ids).  No counterpart in the reference.  FP32 training with Adam

@@ -191,6 +191,10 @@ extern "C" int spr_ctx_create(const spr_crystal *cr, const spr_ifc *fc3, const s
         return SPR_E_ARG;
     }
     SPR_HIP(hipSetDevice(device));
+    {
+        const int rc0 = noalloc_setup(device);
+        if (rc0) return rc0;
+    }
     auto *h = new spr_ctx();
     Ctx &c = h->c;
     c.device = device;

@@ -199,6 +199,8 @@ int ensure_tetra(Ctx &c);
 // d_w[q1 - q1_lo][q2][s0][s1][s2][s3]
 int launch_w4_rows(Ctx &c, int qi, int q1_lo, int nq1, double *d_w);
 int launch_lw4_tetra(Ctx &c, int qi, double *d_lw);
+// resolves the device address of the no-allocation scratch array of `device` (current device)
+int noalloc_setup(int device);
 
 }  // namespace spr
 

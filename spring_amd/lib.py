@@ -23,7 +23,7 @@ SYMBOLS = [
     "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all",
     "spr_lw4_tetra", "spr_lw4_tetra_all", "spr_lw3_kappa_all", "spr_mlp_score",
     "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve", "spr_get_phonon",
-    "spr_lw3_slice",
+    "spr_lw3_slice", "spr_lw3_pairs_noalloc",
 ]
 
 
@@ -213,6 +213,16 @@ class GpuContext:
         out = np.zeros((q0.size, self.nd))
         self._check(self.lib.spr_lw3_pairs(self.h, ct.c_int(q0.size), _ptr(q0), _ptr(q1), ct.c_double(sigma),
                                            ct.c_int(PREC[prec]), _ptr(out)), "spr_lw3_pairs")
+        return out
+
+    def lw3_pairs_noalloc(self, q0_index, q1_grid, sigma: float, prec: str = "fp32") -> np.ndarray:
+        """lw3_pairs through the entry point that allocates no device memory (at most
+        SPR_NOALLOC_MAX_PAIRS = 4096 pairs per call)."""
+        q0 = np.ascontiguousarray(q0_index, np.int32)
+        q1 = np.ascontiguousarray(q1_grid, np.int32)
+        out = np.zeros((q0.size, self.nd))
+        self._check(self.lib.spr_lw3_pairs_noalloc(self.h, ct.c_int(q0.size), _ptr(q0), _ptr(q1), ct.c_double(sigma),
+                                                   ct.c_int(PREC[prec]), _ptr(out)), "spr_lw3_pairs_noalloc")
         return out
 
     def lw3_tetra(self, i0: int):

@@ -65,6 +65,11 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="repeat CPU passes until this much wall time is spent")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--precisions", default="fp64,fp32,bf16",
+                   help="cell10: the precisions to sweep (comma list; fp64 is the reference of the errors "
+                        "and always runs)")
+    p.add_argument("--no-tts", action="store_true",
+                   help="cell10: skip the end-to-end kappa time-to-solution runs (profiling)")
     p.add_argument("--phonons", default="device", choices=["device", "host"],
                    help="cell10: phonons of the mesh from spr_get_phonon (device) or the host LAPACK "
                         "path; both feed the FP64 reference, the reduced-precision runs and the CPU leg")
@@ -291,7 +296,8 @@ def cell10(args):
         log(f"cell10 {prec}: {steps} steps {dt:.2f} s, kernel {kms:.0f} ms over {nl} launches")
         return dict(step=dt / steps, total=dt, kern=kms / max(nl, 1), launches=nl, lw=lw, k=k3)
 
-    res = {"fp64": sweep("fp64"), "fp32": sweep("fp32"), "bf16": sweep("bf16")}
+    precs = ["fp64"] + [x for x in ("fp32", "bf16") if x in args.precisions.split(",")]
+    res = {p_: sweep(p_) for p_ in precs}
     r64 = res["fp64"]
     sig = np.abs(r64["lw"]) > 1e-6 * np.abs(r64["lw"]).max()
     peaks = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16": BF16_PEAK_TFLOPS}
@@ -304,7 +310,7 @@ def cell10(args):
     # kappa time to solution, end to end in a fresh context (the compiled kernels, the tuning
     # table and the graphs of the sweeps above are not reused: a new context captures anew)
     tts = {}
-    for prec in ("bf16", "fp32"):
+    for prec in ([] if args.no_tts else [x for x in ("bf16", "fp32") if x in res]):
         t = time.perf_counter()
         qm2, ph2, _, _ = setup()
         c2 = lib.GpuContext(m, qm2, ph2)
@@ -326,7 +332,7 @@ def cell10(args):
                 np.max(np.abs(r["lw"] - r64["lw"])[sig] / np.abs(r64["lw"])[sig]))
             e["linewidth_mean_rel_err_vs_fp64"] = float(
                 np.mean(np.abs(r["lw"] - r64["lw"])[sig] / np.abs(r64["lw"])[sig]))
-            e["kappa_tts_s"] = tts[prec]
+            e["kappa_tts_s"] = tts.get(prec)
         else:
             # FP64 is the reference of the error: its time to solution is composed of the measured
             # parts (mesh, phonons, the timed sweep); the reduced precisions are measured whole
@@ -381,13 +387,13 @@ def cell10(args):
                    "flop_per_pair": flop_pair},
         "roofline": {"bound": "mfma", "achieved": e64["roofline"]["achieved"], "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": e64["roofline"]["frac"], "traffic": traffic,
-                     "kernel": "k_lw3_mm<0,10>", "kernel_ms": r64["kern"], "flop_per_pair": flop_pair,
+                     "kernel": "k_lw3_mm<0,10> (FP64; tile shape picked by the launcher tuner)", "kernel_ms": r64["kern"], "flop_per_pair": flop_pair,
                      "pairs_per_launch": pairs_per_launch},
         "cpu_baseline": cpu,
-        "fp64": e64, "fp32": entry("fp32"), "bf16": entry("bf16"),
-        "kappa_tts_s": min(tts.values()),
-        "kappa_tts_precision": min(tts, key=tts.get),
-        "kappa_tts_rel_err_vs_fp64": kappa_err(min(tts, key=tts.get)),
+        **{p_: (e64 if p_ == "fp64" else entry(p_)) for p_ in precs},
+        "kappa_tts_s": min(tts.values()) if tts else None,
+        "kappa_tts_precision": min(tts, key=tts.get) if tts else None,
+        "kappa_tts_rel_err_vs_fp64": kappa_err(min(tts, key=tts.get)) if tts else None,
         "phonons_s": t_ph, "mesh_s": t_mesh, "phonons": args.phonons, "host_model_s": t_model,
         "kappa_W_mK": [float(r64["k"][i, i]) for i in range(3)],
     }

@@ -199,15 +199,23 @@ void orc_scatter_prob(int ndof, const int *q0chk, const double *iw0, const doubl
 /* Gaussian-delta stage of CalculateLinewidth for one pair (ThirdOrder_m.f90:514-537):
  *   lw(s0) = sum_{s1,s2} W(s2,s1,s0) [ (n1+n2+1) d(w0-w1-w2) + (n1-n2)(d(w0+w1-w2) - d(w0-w1+w2)) ]
  *   d(x) = 1/(sigma sqrt(2 pi)) exp(-x^2/(2 sigma^2))        (:527-529) */
+/* Branch selection (PolarizationStrtEnd, Helper_m.f90:310-353): pol = {lo0, hi0, lo1, hi1, ...},
+ * 0-based inclusive ranges of s0, s1, s2[, s3]; NULL = every branch.  The sums run over these
+ * ranges as CalculateLwTetrahedron's loops do (ThirdOrder_m.f90:207-214), and TetrahedronIntegration4th's
+ * (Tetrahedron4th_sep.f90:213-224); s0 outside its range gives 0. */
+static int pol_lo(const int *pol, int k) { return pol ? pol[2 * k] : 0; }
+static int pol_hi(const int *pol, int k, int nd) { return pol ? pol[2 * k + 1] + 1 : nd; }
+
 void orc_lw3_from_w(int nd, const double *W, const double *w0, const double *w1, const double *w2,
-                    const double *n1, const double *n2, double sigma, double *lw)
+                    const double *n1, const double *n2, double sigma, const int *pol, double *lw)
 {
     const double inv_s = 1.0 / sigma, inv_s2 = inv_s * inv_s;
     const double c0 = inv_s / sqrt(2.0 * M_PI);
-    for (int s0 = 0; s0 < nd; ++s0) {
+    for (int s0 = 0; s0 < nd; ++s0) lw[s0] = 0.0;
+    for (int s0 = pol_lo(pol, 0); s0 < pol_hi(pol, 0, nd); ++s0) {
         double acc = 0.0;
-        for (int s1 = 0; s1 < nd; ++s1)
-            for (int s2 = 0; s2 < nd; ++s2) {
+        for (int s1 = pol_lo(pol, 1); s1 < pol_hi(pol, 1, nd); ++s1)
+            for (int s2 = pol_lo(pol, 2); s2 < pol_hi(pol, 2, nd); ++s2) {
                 const double d1 = w0[s0] - w1[s1] - w2[s2];
                 const double d2 = w0[s0] + w1[s1] - w2[s2];
                 const double d3 = w0[s0] - w1[s1] + w2[s2];
@@ -237,7 +245,7 @@ void orc_lw3_gauss(int nb, int natm, const int *atm_num, const int *atm_indx,
                    int nq, const int *mesh, const int *q_int, const double *q_cart,
                    const cplx *evec, const double *omega, const double *inv_omega,
                    const double *nbe, int i0, double sigma, int q1_lo, int q1_hi,
-                   double *lw, double *W_out)
+                   const int *pol, double *lw, double *W_out)
 {
     const int nd = 3 * nb;
     const size_t n3 = (size_t)nd * nd * nd;
@@ -267,7 +275,7 @@ void orc_lw3_gauss(int nb, int natm, const int *atm_num, const int *atm_indx,
         if (W_out) memcpy(W_out + (size_t)(q1 - q1_lo) * n3, W, n3 * sizeof(double));
         double lwq[64];
         orc_lw3_from_w(nd, W, omega + (size_t)i0 * nd, omega + (size_t)q1 * nd, omega + (size_t)q2 * nd,
-                       nbe + (size_t)q1 * nd, nbe + (size_t)q2 * nd, sigma, lwq);
+                       nbe + (size_t)q1 * nd, nbe + (size_t)q2 * nd, sigma, pol, lwq);
         for (int s0 = 0; s0 < nd; ++s0) lw[s0] += lwq[s0];
     }
     free(phi);
@@ -409,7 +417,8 @@ void orc_lw4_gauss(int nb, int natm, const int *atm_num, const int *atm_indx,
                    const double *fc4, const double *latvec, const double *osm,
                    int nq, const int *mesh, const int *q_int, const double *q_cart,
                    const cplx *evec, const double *omega, const double *inv_omega,
-                   const double *nbe, int i0, double sigma, int q1_lo, int q1_hi, double *lw)
+                   const double *nbe, int i0, double sigma, int q1_lo, int q1_hi, const int *pol,
+                   double *lw)
 {
     const int nd = 3 * nb;
     const size_t n4 = (size_t)nd * nd * nd * nd;
@@ -443,11 +452,12 @@ void orc_lw4_gauss(int nb, int natm, const int *atm_num, const int *atm_indx,
             const double *i2w = inv_omega + (size_t)q2 * nd, *i3w = inv_omega + (size_t)q3 * nd;
             const double *n0 = nbe + (size_t)i0 * nd, *n1 = nbe + (size_t)q1 * nd;
             const double *n2 = nbe + (size_t)q2 * nd, *n3 = nbe + (size_t)q3 * nd;
-            for (int s0 = g0 ? 3 : 0; s0 < nd; ++s0) {
+            const int s0a = pol_lo(pol, 0) > (g0 ? 3 : 0) ? pol_lo(pol, 0) : (g0 ? 3 : 0);
+            for (int s0 = s0a; s0 < pol_hi(pol, 0, nd); ++s0) {
                 double acc = 0.0;
-                for (int s1 = 0; s1 < nd; ++s1)
-                    for (int s2 = 0; s2 < nd; ++s2)
-                        for (int s3 = 0; s3 < nd; ++s3) {
+                for (int s1 = pol_lo(pol, 1); s1 < pol_hi(pol, 1, nd); ++s1)
+                    for (int s2 = pol_lo(pol, 2); s2 < pol_hi(pol, 2, nd); ++s2)
+                        for (int s3 = pol_lo(pol, 3); s3 < pol_hi(pol, 3, nd); ++s3) {
                             const cplx v = phi[(((size_t)s0 * nd + s1) * nd + s2) * nd + s3];
                             const double W = (creal(v) * creal(v) + cimag(v) * cimag(v)) *
                                              (i0w[s0] * i1w[s1] * i2w[s2] * i3w[s3]) * factor;
@@ -474,18 +484,20 @@ void orc_lw4_gauss(int nb, int natm, const int *atm_num, const int *atm_indx,
  * kappa_noIter (ThermalConductivity.f90:36-164): RTA thermal conductivity over the full
  * mesh, tau = 0.5 / lw(s, pos(q)), modes s < 3 skipped at Gamma:
  *   kappa_ab = kW_m_k / (T^2 V Nq) sum_{q,s} w^2 n (n+1) tau v_a v_b     (W/m K)
- * lw_irr[q0][s], pos[q] = irreducible position of grid point q, vel[q][s][3].
+ * lw_irr[q0][s], pos[q] = irreducible position of grid point q, vel[q][s][3].  srange = {lo, hi}
+ * (0-based inclusive) is the mode range default_strt..default_end of the s0 selection rule
+ * (:106-111); at q = Gamma the loop starts at s = 3 (:123-124); NULL = every mode.
  * ------------------------------------------------------------------------------------*/
 void orc_kappa_rta(int nd, int nq, const int *q_int, const int *pos, const double *omega,
                    const double *nbe, const double *vel, const double *lw_irr, double T,
-                   double vol, double *kappa)
+                   double vol, const int *srange, double *kappa)
 {
     const double kw = (1.054571817 * 1.054571817) * 0.001 / 1.380649;
     const double f2 = kw / (T * T * vol) / (double)nq;
     double k[9] = {0};
     for (int q = 0; q < nq; ++q) {
         const int g = q_int[3 * q] == 0 && q_int[3 * q + 1] == 0 && q_int[3 * q + 2] == 0;
-        for (int s = g ? 3 : 0; s < nd; ++s) {
+        for (int s = g ? 3 : pol_lo(srange, 0); s < pol_hi(srange, 0, nd); ++s) {
             const double w = omega[(size_t)q * nd + s], n = nbe[(size_t)q * nd + s];
             const double tau = 0.5 / lw_irr[(size_t)pos[q] * nd + s];
             const double *v = vel + ((size_t)q * nd + s) * 3;
@@ -503,7 +515,7 @@ void orc_kappa_rta(int nd, int nq, const int *q_int, const int *pos, const doubl
 void orc_kappa_rta_symm(int nd, int nq, int nq0, const int *irr_q_int, const int *mult,
                         int nsym, const double *rot, const double *omega0, const double *nbe0,
                         const double *vel0, const double *lw_irr, double T, double vol,
-                        double *kappa)
+                        const int *srange, double *kappa)
 {
     const double kw = (1.054571817 * 1.054571817) * 0.001 / 1.380649;
     const double f2 = kw / (T * T * vol) / (double)nq;
@@ -512,7 +524,7 @@ void orc_kappa_rta_symm(int nd, int nq, int nq0, const int *irr_q_int, const int
         const int *ki = irr_q_int + 3 * q;
         const int g = ki[0] == 0 && ki[1] == 0 && ki[2] == 0;
         const double p = 1.0 / (double)(nsym / mult[q]);
-        for (int s = g ? 3 : 0; s < nd; ++s) {
+        for (int s = g ? 3 : pol_lo(srange, 0); s < pol_hi(srange, 0, nd); ++s) {
             const double *v = vel0 + ((size_t)q * nd + s) * 3;
             double kr[9] = {0};
             for (int r = 0; r < nsym; ++r) {
@@ -601,18 +613,18 @@ int orc_tetra_weights(double Mqs, const double *M_in, const double *F_in, double
  * F1 = W(n1+n2+1), M2 = w2-w1 with F2 = W(n1-n2), M3 = w1-w2 with F3 = F2 entering with a minus
  * sign; P3_q0 accumulates g of all three.  tetra[6 nq][4] grid indices. */
 void orc_lw3_tetra(int nd, int nq, const int *tetra, const int *partner, const double *W,
-                   const double *omega, const double *nbe, const double *omega0,
+                   const double *omega, const double *nbe, const double *omega0, const int *pol,
                    double *lw, double *p3q0)
 {
     for (int s = 0; s < nd; ++s) { lw[s] = 0.0; p3q0[s] = 0.0; }
-    for (int s0 = 0; s0 < nd; ++s0) {
+    for (int s0 = pol_lo(pol, 0); s0 < pol_hi(pol, 0, nd); ++s0) {
         const double Mq0 = omega0[s0];
         double acc = 0.0, pacc = 0.0;
         for (int t = 0; t < 6 * nq; ++t) {
             int v1[4], v2[4];
             for (int v = 0; v < 4; ++v) { v1[v] = tetra[4 * t + v]; v2[v] = partner[v1[v]]; }
-            for (int s1 = 0; s1 < nd; ++s1)
-                for (int s2 = 0; s2 < nd; ++s2) {
+            for (int s1 = pol_lo(pol, 1); s1 < pol_hi(pol, 1, nd); ++s1)
+                for (int s2 = pol_lo(pol, 2); s2 < pol_hi(pol, 2, nd); ++s2) {
                     double F1[4], F2[4], F3[4], M1[4], M2[4], M3[4];
                     for (int v = 0; v < 4; ++v) {
                         const double w = W[(((size_t)v1[v] * nd + s0) * nd + s1) * nd + s2];
@@ -724,7 +736,7 @@ static double tetra4_term(double Mq, const double *M, const double *F)
  * 0.5 lw to the linewidth. */
 void orc_lw4_tetra_from_w(int nd, int nq, const int *mesh, const int *q_int, const int *tetra, int i0,
                           int q1_lo, int q1_hi, const double *W, const double *omega, const double *nbe,
-                          const double *omega0, const double *nbe0, double *lw)
+                          const double *omega0, const double *nbe0, const int *pol, double *lw)
 {
     const size_t n4 = (size_t)nd * nd * nd * nd;
     const int *k0 = q_int + 3 * i0;
@@ -738,12 +750,13 @@ void orc_lw4_tetra_from_w(int nd, int nq, const int *mesh, const int *q_int, con
                 q2v[v] = tetra[4 * t + v];
                 q3v[v] = grid_q3(mesh, k0, q_int + 3 * q1, q_int + 3 * q2v[v]);
             }
-            for (int s0 = g0 ? 3 : 0; s0 < nd; ++s0) {
+            const int s0a = pol_lo(pol, 0) > (g0 ? 3 : 0) ? pol_lo(pol, 0) : (g0 ? 3 : 0);
+            for (int s0 = s0a; s0 < pol_hi(pol, 0, nd); ++s0) {
                 const double Mq0 = omega0[s0], n0 = nbe0[s0];
-                for (int s1 = 0; s1 < nd; ++s1) {
+                for (int s1 = pol_lo(pol, 1); s1 < pol_hi(pol, 1, nd); ++s1) {
                     const double w1 = omega[(size_t)q1 * nd + s1], n1 = nbe[(size_t)q1 * nd + s1];
-                    for (int s2 = 0; s2 < nd; ++s2)
-                        for (int s3 = 0; s3 < nd; ++s3) {
+                    for (int s2 = pol_lo(pol, 2); s2 < pol_hi(pol, 2, nd); ++s2)
+                        for (int s3 = pol_lo(pol, 3); s3 < pol_hi(pol, 3, nd); ++s3) {
                             double F1[4], F2[4], F3[4], M1[4], M2[4], M3[4];
                             for (int v = 0; v < 4; ++v) {
                                 const double w = Wq1[(size_t)q2v[v] * n4 + (((size_t)s0 * nd + s1) * nd + s2) * nd + s3];

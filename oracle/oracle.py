@@ -31,6 +31,24 @@ def lib():
     return _LIB
 
 
+def _pol(pol):
+    """Branch ranges as the C oracle takes them: None (every branch) or a flat int32 array
+    {lo0, hi0, lo1, hi1, ...}, 0-based inclusive (PolarizationStrtEnd, Helper_m.f90:310-353)."""
+    if pol is None:
+        return None, None
+    a = np.ascontiguousarray(np.asarray(pol, np.int32).ravel())
+    return a, a.ctypes.data_as(ct.c_void_p)
+
+
+def polarization(order, nd, rules):
+    """PolarizationStrtEnd (Helper_m.f90:310-353) with 0-based ranges: 'A' -> [0, 2], 'O' ->
+    [3, nd-1], anything else -> [0, nd-1], one (lo, hi) per slot of the `order` rule letters."""
+    out = []
+    for r in rules[:order]:
+        out += [0, 2] if r in "Aa" else [3, nd - 1] if r in "Oo" else [0, nd - 1]
+    return np.array(out, np.int32)
+
+
 def _p(a, dtype):
     a = np.ascontiguousarray(a, dtype=dtype)
     return a, a.ctypes.data_as(ct.c_void_p)
@@ -103,62 +121,67 @@ def scatter_prob(nd, chk, iw0, iw1, iw2, phi, factor):
     return W, Wp
 
 
-def lw3_gauss(inp: Inputs, i0_grid: int, sigma: float, q1_lo=0, q1_hi=None, want_w=False):
+def lw3_gauss(inp: Inputs, i0_grid: int, sigma: float, q1_lo=0, q1_hi=None, want_w=False, pol=None):
     q1_hi = inp.nq if q1_hi is None else q1_hi
     nd = inp.nd
     lw = np.zeros(nd)
     W = np.zeros((q1_hi - q1_lo, nd, nd, nd)) if want_w else None
+    pa, pp = _pol(pol)
     lib().orc_lw3_gauss(ct.c_int(inp.nb), ct.c_int(inp.natm), inp.p_atm_num, inp.p_atm_indx,
                         inp.p_fc, inp.p_latvec, inp.p_osm, ct.c_int(inp.nq), inp.p_mesh,
                         inp.p_q_int, inp.p_q_cart, inp.p_evec, inp.p_omega, inp.p_inv_omega,
                         inp.p_nbe, ct.c_int(i0_grid), ct.c_double(sigma), ct.c_int(q1_lo),
-                        ct.c_int(q1_hi), lw.ctypes.data_as(ct.c_void_p),
+                        ct.c_int(q1_hi), pp, lw.ctypes.data_as(ct.c_void_p),
                         W.ctypes.data_as(ct.c_void_p) if want_w else None)
     return (lw, W) if want_w else lw
 
 
-def lw4_gauss(inp: Inputs, i0_grid: int, sigma: float, q1_lo=0, q1_hi=None):
+def lw4_gauss(inp: Inputs, i0_grid: int, sigma: float, q1_lo=0, q1_hi=None, pol=None):
     q1_hi = inp.nq if q1_hi is None else q1_hi
     lw = np.zeros(inp.nd)
+    pa, pp = _pol(pol)
     lib().orc_lw4_gauss(ct.c_int(inp.nb), ct.c_int(inp.natm), inp.p_atm_num, inp.p_atm_indx,
                         inp.p_fc, inp.p_latvec, inp.p_osm, ct.c_int(inp.nq), inp.p_mesh,
                         inp.p_q_int, inp.p_q_cart, inp.p_evec, inp.p_omega, inp.p_inv_omega,
                         inp.p_nbe, ct.c_int(i0_grid), ct.c_double(sigma), ct.c_int(q1_lo),
-                        ct.c_int(q1_hi), lw.ctypes.data_as(ct.c_void_p))
+                        ct.c_int(q1_hi), pp, lw.ctypes.data_as(ct.c_void_p))
     return lw
 
 
-def lw3_gauss_all(inp: Inputs, irr_idx, sigma: float, threads: int = 1):
+def lw3_gauss_all(inp: Inputs, irr_idx, sigma: float, threads: int = 1, pol=None):
     """Linewidth of every irreducible q0 (rows), threaded over q0 by the caller."""
     irr_idx = list(int(i) for i in irr_idx)
     if threads <= 1:
-        return np.array([lw3_gauss(inp, i, sigma) for i in irr_idx])
+        return np.array([lw3_gauss(inp, i, sigma, pol=pol) for i in irr_idx])
     with ThreadPoolExecutor(threads) as ex:
-        return np.array(list(ex.map(lambda i: lw3_gauss(inp, i, sigma), irr_idx)))
+        return np.array(list(ex.map(lambda i: lw3_gauss(inp, i, sigma, pol=pol), irr_idx)))
 
 
-def lw4_gauss_all(inp: Inputs, irr_idx, sigma: float, threads: int = 1):
+def lw4_gauss_all(inp: Inputs, irr_idx, sigma: float, threads: int = 1, pol=None):
     irr_idx = list(int(i) for i in irr_idx)
     if threads <= 1:
-        return np.array([lw4_gauss(inp, i, sigma) for i in irr_idx])
+        return np.array([lw4_gauss(inp, i, sigma, pol=pol) for i in irr_idx])
     with ThreadPoolExecutor(threads) as ex:
-        return np.array(list(ex.map(lambda i: lw4_gauss(inp, i, sigma), irr_idx)))
+        return np.array(list(ex.map(lambda i: lw4_gauss(inp, i, sigma, pol=pol), irr_idx)))
 
 
-def kappa_rta(qm, ph, lw_irr, T, vol):
+def kappa_rta(qm, ph, lw_irr, T, vol, srange=None):
+    """srange = (lo, hi) 0-based mode range of the s0 selection rule (None: every mode)."""
     nd = ph.omega.shape[1]
     k = np.zeros(9)
+    sa, sp = _pol(srange)
     arrs = [np.ascontiguousarray(qm.q_int, np.int32), np.ascontiguousarray(qm.pos, np.int32),
             np.ascontiguousarray(ph.omega), np.ascontiguousarray(ph.nbe),
             np.ascontiguousarray(ph.grp_vel), np.ascontiguousarray(lw_irr, np.float64)]
     lib().orc_kappa_rta(ct.c_int(nd), ct.c_int(qm.nq), *[a.ctypes.data_as(ct.c_void_p) for a in arrs],
-                        ct.c_double(T), ct.c_double(vol), k.ctypes.data_as(ct.c_void_p))
+                        ct.c_double(T), ct.c_double(vol), sp, k.ctypes.data_as(ct.c_void_p))
     return k.reshape(3, 3)
 
 
-def kappa_rta_symm(qm, ph, lw_irr, T, vol):
+def kappa_rta_symm(qm, ph, lw_irr, T, vol, srange=None):
     nd = ph.omega.shape[1]
     k = np.zeros(9)
+    sa, sp = _pol(srange)
     irr = qm.irr_idx
     arrs = [np.ascontiguousarray(qm.irr_q_int, np.int32), np.ascontiguousarray(qm.multiplicity, np.int32)]
     rot = np.ascontiguousarray(qm.rot_cart, np.float64)
@@ -171,17 +194,18 @@ def kappa_rta_symm(qm, ph, lw_irr, T, vol):
                              *[a.ctypes.data_as(ct.c_void_p) for a in arrs], ct.c_int(nsym),
                              rot.ctypes.data_as(ct.c_void_p),
                              *[a.ctypes.data_as(ct.c_void_p) for a in arrs2], ct.c_double(T),
-                             ct.c_double(vol), k.ctypes.data_as(ct.c_void_p))
+                             ct.c_double(vol), sp, k.ctypes.data_as(ct.c_void_p))
     return k.reshape(3, 3)
 
 
-def lw3_from_w(W, w0, w1, w2, n1, n2, sigma):
+def lw3_from_w(W, w0, w1, w2, n1, n2, sigma, pol=None):
     """Gaussian-delta stage alone: W[s0][s1][s2] of one pair -> lw(s0)."""
     nd = W.shape[0]
     lw = np.zeros(nd)
     arrs = [np.ascontiguousarray(x, np.float64) for x in (W, w0, w1, w2, n1, n2)]
+    pa, pp = _pol(pol)
     lib().orc_lw3_from_w(ct.c_int(nd), *[a.ctypes.data_as(ct.c_void_p) for a in arrs],
-                         ct.c_double(sigma), lw.ctypes.data_as(ct.c_void_p))
+                         ct.c_double(sigma), pp, lw.ctypes.data_as(ct.c_void_p))
     return lw
 
 
@@ -197,7 +221,7 @@ def tetra_weights(Mqs, M, F):
     return enter, g.value, I, gif.value
 
 
-def lw3_tetra(qm, ph, i0_grid, W):
+def lw3_tetra(qm, ph, i0_grid, W, pol=None):
     """CalculateLwTetrahedron for one q0 from W[q1][s0][s1][s2]: (lw, P3_q0)."""
     nd = ph.omega.shape[1]
     partner = np.ascontiguousarray(qm.index(-qm.q_int[i0_grid][None, :].astype(np.int64)
@@ -207,8 +231,9 @@ def lw3_tetra(qm, ph, i0_grid, W):
     arrs = [np.ascontiguousarray(qm.tetra, np.int32), partner, np.ascontiguousarray(W, np.float64),
             np.ascontiguousarray(ph.omega), np.ascontiguousarray(ph.nbe),
             np.ascontiguousarray(ph.omega[i0_grid])]
+    pa, pp = _pol(pol)
     lib().orc_lw3_tetra(ct.c_int(nd), ct.c_int(qm.nq), *[a.ctypes.data_as(ct.c_void_p) for a in arrs],
-                        lw.ctypes.data_as(ct.c_void_p), p3.ctypes.data_as(ct.c_void_p))
+                        pp, lw.ctypes.data_as(ct.c_void_p), p3.ctypes.data_as(ct.c_void_p))
     return lw, p3
 
 
@@ -223,7 +248,7 @@ def w4_rows(inp: Inputs, i0_grid: int, q1_lo: int, q1_hi: int):
     return W
 
 
-def lw4_tetra_from_w(mesh, q_int, tetra, i0_grid, q1_lo, W, omega, nbe, omega0=None, nbe0=None):
+def lw4_tetra_from_w(mesh, q_int, tetra, i0_grid, q1_lo, W, omega, nbe, omega0=None, nbe0=None, pol=None):
     """orc_lw4_tetra_from_w: the q1 in [q1_lo, q1_lo + len(W)) part of the four-phonon
     tetrahedron lw of q0 `i0_grid`; omega0/nbe0 default to the q0's own."""
     nq, nd = omega.shape
@@ -236,18 +261,19 @@ def lw4_tetra_from_w(mesh, q_int, tetra, i0_grid, q1_lo, W, omega, nbe, omega0=N
     assert rest[0].size == W.shape[0] * nq * nd ** 4
     lib().orc_lw4_tetra_from_w(ct.c_int(nd), ct.c_int(nq), *[a.ctypes.data_as(ct.c_void_p) for a in arrs],
                                ct.c_int(i0_grid), ct.c_int(q1_lo), ct.c_int(q1_lo + W.shape[0]),
-                               *[a.ctypes.data_as(ct.c_void_p) for a in rest], lw.ctypes.data_as(ct.c_void_p))
+                               *[a.ctypes.data_as(ct.c_void_p) for a in rest], _pol(pol)[1],
+                               lw.ctypes.data_as(ct.c_void_p))
     return lw
 
 
-def lw4_tetra(inp: Inputs, qm, i0_grid: int, threads: int = 1, chunk: int = 8):
+def lw4_tetra(inp: Inputs, qm, i0_grid: int, threads: int = 1, chunk: int = 8, pol=None):
     """Four-phonon tetrahedron lw of q0 `i0_grid` (TetrahedronIntegration4th): W4 rows of a q1
     chunk, then their tetrahedron sums; chunks summed in q1 order."""
     spans = [(lo, min(inp.nq, lo + chunk)) for lo in range(0, inp.nq, chunk)]
 
     def part(span):
         W = w4_rows(inp, i0_grid, *span)
-        return lw4_tetra_from_w(inp.mesh, inp.q_int, qm.tetra, i0_grid, span[0], W, inp.omega, inp.nbe)
+        return lw4_tetra_from_w(inp.mesh, inp.q_int, qm.tetra, i0_grid, span[0], W, inp.omega, inp.nbe, pol=pol)
 
     if threads <= 1:
         parts = [part(sp) for sp in spans]

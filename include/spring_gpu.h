@@ -103,6 +103,18 @@ int spr_v3_w(spr_ctx *ctx, int i0, double *W_out);
  * of the learned proposal (spr_mlp_*). */
 int spr_v3_w_pairs(spr_ctx *ctx, int n, const int *q0_index, const int *q1_grid, double *W_out);
 
+/* Branch selection rules of the context (SelectionRule3 / SelectionRule4 of &AnhInfo, mapped by
+ * PolarizationStrtEnd, src/Kappa/Src/Helper_m.f90:310-353, into Polarization3rd(2, 3) /
+ * Polarization4th(2, 4)): pol[2 k], pol[2 k + 1] = first and last branch (1-based, inclusive) of
+ * slot k (s0, s1, s2[, s3]).  Every later three-phonon (order 3) or four-phonon (order 4)
+ * linewidth of the context sums over these ranges as CalculateLwTetrahedron
+ * (ThirdOrder_m.f90:169-171, 207-214) and SumOverPolarization (Tetrahedron4th_sep.f90:213-224)
+ * loop over them (s0 outside its range gives 0), and spr_kappa_rta / spr_lw3_kappa_all take their
+ * mode range from slot 1 (default_strt..default_end of kappa_noIter, ThermalConductivity.f90:
+ * 50-51, 106-111; the wider of the two when both orders are set).  The Gaussian sums apply the
+ * same ranges.  Default: every branch.  The iterative BTE ignores them, as kappa_iter.x does. */
+int spr_set_branch_rules(spr_ctx *ctx, int order, const int *pol);
+
 /* Gaussian-delta three-phonon linewidth of irreducible q0 `i0` (ScatterProbAllq1 fused with
  * CalculateLinewidth, ThirdOrder_m.f90:442-543): lw(Ndof), rad/ps. */
 int spr_lw3(spr_ctx *ctx, int i0, double sigma, int prec, double *lw);
@@ -187,6 +199,14 @@ int spr_lw4_tetra_all(spr_ctx *ctx, double *lw_all);
  * symm = 1) from LW_Allq0(Ndof, nq0), FP64: kappa(9) W/mK, mode_k(9, Ndof) optional. */
 int spr_kappa_rta(spr_ctx *ctx, const double *lw_all, double T, int symm,
                   double *kappa, double *mode_k);
+
+/* spr_kappa_rta plus the spectrally resolved kappa of -Spectral_k (SpectralResolve,
+ * ThermalConductivity.f90:76-90, 147-160, completed and written as "SpectralResolvedKappa" at
+ * main.f90:780-794): spectral_k(11, Ndof * nq0), column qi0 + s nq0 (0-based) = (s + 1, omega(s)
+ * of irreducible q0 qi0, its nine kappa_ab contributions times kW_m_k / (T^2 V)); summed over
+ * columns and divided by Nq they give kappa. */
+int spr_kappa_rta_spectral(spr_ctx *ctx, const double *lw_all, double T, int symm, double *kappa,
+                           double *mode_k, double *spectral_k);
 
 /* spr_lw3_all followed by spr_kappa_rta with LW_Allq0 kept on the device in between (one
  * synchronisation instead of a download and an upload): lw_all(Ndof, nq0), kappa(9) W/mK, mode_k

@@ -5,4 +5,4 @@ q-mesh and its irreducible wedge, harmonic phonons, and the driver that mirrors
 src/Kappa/Src/main.f90.  Device side: libspring_gpu.so (hand-written gfx950 HIP kernels
 behind a C ABI, declared in include/spring_gpu.h), loaded by spring_amd.lib.
 """
-__all__ = ["constants", "model", "mesh", "phonons", "lib", "kappa"]
+__all__ = ["constants", "model", "mesh", "phonons", "lib"]

@@ -4,6 +4,8 @@
 //   tau = 0.5 / lw(s, pos(q));  kappa_ab = kW_m_k / (T^2 V Nq) sum_{q,s} w^2 n (n+1) tau v_a v_b
 // symm = 1 sums the irreducible q0 only, with (mult / Nsym) sum_R (R v)(R v)^T for v v^T.
 // Contributions are written per (q, s) and reduced per mode in a fixed order (deterministic).
+#include <algorithm>
+
 #include "spr_internal.h"
 
 namespace spr {
@@ -11,14 +13,16 @@ namespace spr {
 __global__ void k_kappa_terms(int nq_terms, int nd, const int *qgrid, const int *lwpos,
                               const double *omega, const double *nbe, const double *vel,
                               const double *lw_all, const double *rot, int nsym,
-                              const int *mult, double *terms) {
+                              const int *mult, int s_lo, int s_hi, double *terms) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq_terms * nd) return;
     const int t = i / nd, s = i % nd;
     const int q = qgrid ? qgrid[t] : t;
     double *out = terms + (size_t)i * 9;
     const bool gamma = q == 0;
-    if (gamma && s < 3) {
+    // the mode loop default_strt..default_end, from s = 4 (1-based) at Gamma
+    // (ThermalConductivity.f90:106-111, 123-124)
+    if ((gamma && s < 3) || s < s_lo || s > s_hi) {
         for (int k = 0; k < 9; ++k) out[k] = 0.0;
         return;
     }
@@ -67,6 +71,17 @@ __global__ __launch_bounds__(256) void k_kappa_reduce(int nq_terms, int nd, cons
 
 namespace spr {
 
+// the kappa mode range: slot 1 of the three-phonon rule, widened by the four-phonon one when the
+// context holds FC4 (kappa_noIter's FindStrtEnd, ThermalConductivity.f90:106-111)
+void kappa_mode_range(const Ctx &c, int &lo, int &hi) {
+    lo = c.pol3[0];
+    hi = c.pol3[1];
+    if (c.fc4.order == 4) {
+        lo = std::min(lo, c.pol4[0]);
+        hi = std::max(hi, c.pol4[1]);
+    }
+}
+
 int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_kappa_modes) {
     const int nd = c.nd;
     if (!c.vel) {
@@ -97,9 +112,11 @@ int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_k
         SPR_HIP(hipMemcpy(c.k_rot, c.h_rot.data(), c.h_rot.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     const int n = nt * nd;
+    int s_lo, s_hi;
+    kappa_mode_range(c, s_lo, s_hi);
     hipLaunchKernelGGL(k_kappa_terms, dim3((n + 255) / 256), dim3(256), 0, c.stream, nt, nd,
                        symm ? c.irr_grid : nullptr, d_pos, c.omega, c.nbe, c.vel, d_lw_all,
-                       symm ? c.k_rot : nullptr, c.nsym, symm ? c.k_mult : nullptr, c.k_terms);
+                       symm ? c.k_rot : nullptr, c.nsym, symm ? c.k_mult : nullptr, s_lo, s_hi, c.k_terms);
     const double kw = (1.054571817 * 1.054571817) * 0.001 / 1.380649;   // constants.f90:65
     const double f = kw / (T * T * c.vol) / (double)c.nq;
     hipLaunchKernelGGL(k_kappa_reduce, dim3(nd * 9), dim3(256), 0, c.stream, nt, nd, c.k_terms, f, d_kappa_modes);

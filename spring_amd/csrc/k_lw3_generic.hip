@@ -38,6 +38,7 @@ struct Lw3Args {
     double *w_out;     // [q1][s0][s1][s2]  (pair mode: [pair][s0][s1][s2])
     const int *pair_q0;   // pair mode (W_OUT only): block x = pair, q0 index and q1 per pair
     const int *pair_q1;
+    const double *sel;    // branch selection (NULL: all): (nd) s0 mask, (nd, nd) pair mask [s1][s2]
 };
 
 __device__ __forceinline__ void grid_coords(int i, int m0, int m1, int &k0, int &k1, int &k2) {
@@ -166,7 +167,8 @@ __global__ __launch_bounds__(BLK) void k_lw3_generic(Lw3Args A) {
                 if (W_OUT) {
                     A.w_out[(((size_t)(pair_mode ? ch : q1) * ND + s0) * ND + js1) * ND + js2] = W;
                 } else {
-                    const double g = gauss_g(A.omega[(size_t)i0 * ND + s0], w1, w2, n1, n2, c0, hinv);
+                    double g = gauss_g(A.omega[(size_t)i0 * ND + s0], w1, w2, n1, n2, c0, hinv);
+                    if (A.sel) g *= A.sel[s0] * A.sel[ND + js1 * ND + js2];
                     if (sizeof(R) == 8)
                         accd[l] += wp * W * g;
                     else
@@ -234,6 +236,7 @@ static int launch_nd(Ctx &c, const int *d_q0, int nq0, double sigma, double *d_l
     A.omega = c.omega;
     A.inv_omega = c.inv_omega;
     A.nbe = c.nbe;
+    A.sel = (c.sel3 && !d_w) ? c.d_sel3 : nullptr;
     A.sigma = sigma;
     {
         // Sctr3THz2 from its defining factors (constants.f90:59-60)

@@ -68,6 +68,7 @@ struct MmArgs {
     double sigma, factor;
     int npmax;                   // most clusters of one basis atom
     int nterm;                   // clusters of all basis atoms
+    const double *sel;           // branch selection (NULL: all): (nd) s0 mask, (nd, nd) pair mask [s1][s2]
 };
 
 constexpr int pow2ceil(int x) { return x <= 1 ? 1 : x <= 2 ? 2 : x <= 4 ? 4 : 8; }
@@ -722,7 +723,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                         const double W = (x * x + y * y) * iw0 * iw1 * iw2;
                         const double G = (n1 + n2 + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
                                          (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
-                        accd += wpair * W * G;
+                        accd += A.sel ? wpair * W * G * A.sel[ND + s1 * ND + s2] : wpair * W * G;
                     }
                 }
             }
@@ -751,7 +752,8 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                         const int m = mt * 16 + tile_row<false>(g, 2 * qD + k);
                         const int s1 = m / SB, s2 = h * SB + m % SB;
                         bool near = false;
-                        if (s0ok && j0 + t < Sh::VT && mt < Sh::MT_D && s1 < ND && s2 < ND) {
+                        if (s0ok && j0 + t < Sh::VT && mt < Sh::MT_D && s1 < ND && s2 < ND &&
+                            (!A.sel || A.sel[ND + s1 * ND + s2] != 0.0)) {
                             const double w1 = sMode[2 * ND + s1], w2 = sMode[5 * ND + s2];
                             const double ap = w1 + w2, am = w1 - w2;
                             near = fabs(w0 - ap) < dcut || fabs(w0 + am) < dcut || fabs(w0 - am) < dcut;
@@ -785,7 +787,8 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                     const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
                     const float W = (x * x + y * y) * (iw1 * iw2);
                     const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
-                    sV[(t * 64 + lo) * 4 + r] = W * ((n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3));
+                    const float sm = A.sel ? (float)A.sel[ND + s1 * ND + s2] : 1.f;
+                    sV[(t * 64 + lo) * 4 + r] = W * sm * ((n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3));
                 }
                 mm_wave_sync();
                 // (5): the owner adds its elements' terms in element order
@@ -815,7 +818,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
         for (int ww = nt; ww < 8; ww += Sh::NTC)
             for (int gg = 0; gg < 4; ++gg)
                 for (int q = 0; q < 2; ++q) tot += sRed[ww * 64 + gg * 16 + sj + q];
-        A.ic.partial[(size_t)unit * ND + s] = tot;
+        A.ic.partial[(size_t)unit * ND + s] = A.sel ? tot * A.sel[s] : tot;
     }
 }
 
@@ -897,6 +900,7 @@ int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, do
     A.omega = c.omega;
     A.inv_omega = c.inv_omega;
     A.nbe = c.nbe;
+    A.sel = c.sel3 ? c.d_sel3 : nullptr;
     A.sigma = sigma;
     {
         // Sctr3THz2 from its defining factors (constants.f90:59-60)

@@ -49,6 +49,8 @@ struct SmallArgs {
     // both orientations of each pair written
     double *w_out;
     int w_q0_first, w_nq;
+    // branch selection (NULL: every branch): (nd) s0 mask, then the (nd, nd) pair mask [s1][s2]
+    const double *sel;
 };
 
 template <typename R, int NB, bool WOUT>
@@ -221,7 +223,8 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
                 }
                 continue;
             }
-            const double g = (n1 + n2 + 1.0) * (c0 * exp(-t1)) + (n1 - n2) * (c0 * exp(-t2) - c0 * exp(-t3));
+            double g = (n1 + n2 + 1.0) * (c0 * exp(-t1)) + (n1 - n2) * (c0 * exp(-t2) - c0 * exp(-t3));
+            if (A.sel) g *= A.sel[ND + s1 * ND + s2];
             if (sizeof(R) == 8)
                 dacc[k] += wpair * W * g;
             else
@@ -243,7 +246,7 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
         const int h = threadIdx.x / 3, k = threadIdx.x % 3;
         double sum = 0.0;
         for (int sx = 0; sx < ND; ++sx) sum += s_red[(sx * NB + h) * 3 + k];
-        A.ic.partial[(size_t)item * ND + threadIdx.x] = sum;
+        A.ic.partial[(size_t)item * ND + threadIdx.x] = A.sel ? sum * A.sel[threadIdx.x] : sum;
     }
 }
 
@@ -405,6 +408,7 @@ int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec,
     A.ic.partial = ov ? ov->partial : c.ipartial;
     A.w_out = nullptr;
     A.w_q0_first = A.w_nq = 0;
+    A.sel = c.sel3 ? c.d_sel3 : nullptr;
     const bool f64 = prec == SPR_PREC_FP64;
     if (c.nb == 2)
         return f64 ? launch_small_t<double, 2>(c, A, nitems, q0_first, q0_count, it0, d_lw)
@@ -451,6 +455,7 @@ int launch_lw3_small_w(Ctx &c, int q_first, int q_count, double *d_w) {
                1000000.0 * pi * 0.0625 / (double)c.nq;   // Sctr3THz2 / Nq, constants.f90:59-60
     A.ic.partial = c.ipartial;
     A.w_out = d_w;
+    A.sel = nullptr;   // W output: the tetrahedron sums apply the branch ranges themselves
     A.w_q0_first = q_first;
     A.w_nq = c.nq;
     // the per-item sums of this launch are zero; they are reduced into a scratch row per q0

@@ -75,10 +75,17 @@ __device__ __forceinline__ void tetra_gif(double mq, const double *Min, const do
 
 constexpr int TBLK = 256;
 
+// branch ranges of s0, s1, s2 (Polarization3rd, 0-based inclusive): the loops of
+// CalculateLwTetrahedron (ThirdOrder_m.f90:207-214)
+struct Ranges3 {
+    int lo[3], hi[3];
+};
+
 __global__ __launch_bounds__(TBLK) void k_lw3_tetra(const double *W, const int *tetra, const int *partner,
                                                     const double *omega, const double *nbe, const double *omega0,
-                                                    int nd, int ntet, int chunk, double *partial) {
+                                                    int nd, int ntet, int chunk, Ranges3 rg, double *partial) {
     const int ch = blockIdx.x, s0 = blockIdx.y, nch = gridDim.x;
+    const bool s0_in = s0 >= rg.lo[0] && s0 <= rg.hi[0];
     const int nd2 = nd * nd;
     const int t_lo = ch * chunk, t_hi = min(ntet, t_lo + chunk);
     const double mq = omega0[s0];
@@ -89,6 +96,7 @@ __global__ __launch_bounds__(TBLK) void k_lw3_tetra(const double *W, const int *
         // pair, whose energy ranges are alike, so whole waves skip together below
         const int p = i / nt, t = t_lo + (i - p * nt);
         const int s1 = p / nd, s2 = p % nd;
+        if (!s0_in || s1 < rg.lo[1] || s1 > rg.hi[1] || s2 < rg.lo[2] || s2 > rg.hi[2]) continue;
         double F1[4], F2[4], M1[4], M2[4], M3[4];
         int q1v[4], q2v[4];
 #pragma unroll
@@ -152,7 +160,7 @@ __global__ __launch_bounds__(TBLK) void k_lw3_tetra(const double *W, const int *
 template <int ND>
 __global__ __launch_bounds__(TBLK) void k_lw3_tetra_s(const double *Wb, const int *tetra, const int *partner_b,
                                                       const double *omega, const double *nbe, const int *q0_grid,
-                                                      int nq, int ntet, double *partial_b) {
+                                                      int nq, int ntet, Ranges3 rg, double *partial_b) {
     constexpr int ND2 = ND * ND;
     const int tid = threadIdx.x;
     // blockIdx.y: the q0 of the batch (its W rows, partner table and partial rows)
@@ -177,6 +185,7 @@ __global__ __launch_bounds__(TBLK) void k_lw3_tetra_s(const double *Wb, const in
     for (int it = blockIdx.x * TBLK + tid; it < total; it += gridDim.x * TBLK) {
         const int p = it / ntet, t = it - p * ntet;
         const int s1 = p / ND, s2 = p % ND;
+        if (s1 < rg.lo[1] || s1 > rg.hi[1] || s2 < rg.lo[2] || s2 > rg.hi[2]) continue;
         double w1[4], w2[4];
         int q1v[4], q2v[4];
 #pragma unroll
@@ -210,7 +219,7 @@ __global__ __launch_bounds__(TBLK) void k_lw3_tetra_s(const double *Wb, const in
             const double M[4] = {P[0].M, P[1].M, P[2].M, P[3].M};
             const Rcp R = rcp6(M);
 #pragma unroll 1
-            for (int s0 = 0; s0 < ND; ++s0) {
+            for (int s0 = rg.lo[0]; s0 <= rg.hi[0]; ++s0) {
                 const double mq = s_w0[s0];
                 if (!(mq >= M[0] && mq <= M[3])) continue;
                 double F[4];
@@ -335,15 +344,20 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
         SPR_HIP(hipMemcpyAsync(c.partner, part.data(), (size_t)nb * c.nq * sizeof(int), hipMemcpyHostToDevice,
                                c.stream));
         SPR_HIP(hipEventRecord(c.ev0, c.stream));
+        Ranges3 rg;
+        for (int k = 0; k < 3; ++k) {
+            rg.lo[k] = c.pol3[2 * k];
+            rg.hi[k] = c.pol3[2 * k + 1];
+        }
         if (all_s0) {
             // one launch for the batch's q0 (grid.y), then one reduction launch
             const int *q0g = c.irr_grid + b0;
             if (c.nd == 6)
                 hipLaunchKernelGGL(k_lw3_tetra_s<6>, dim3(nch, nb), dim3(TBLK), 0, c.stream, c.wout, c.tetra,
-                                   c.partner, c.omega, c.nbe, q0g, c.nq, ntet, c.partial);
+                                   c.partner, c.omega, c.nbe, q0g, c.nq, ntet, rg, c.partial);
             else
                 hipLaunchKernelGGL(k_lw3_tetra_s<3>, dim3(nch, nb), dim3(TBLK), 0, c.stream, c.wout, c.tetra,
-                                   c.partner, c.omega, c.nbe, q0g, c.nq, ntet, c.partial);
+                                   c.partner, c.omega, c.nbe, q0g, c.nq, ntet, rg, c.partial);
             SPR_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_tetra_reduce, dim3(c.nd, nb), dim3(256), 0, c.stream, c.partial, c.nd, nch,
                                d_lw + (size_t)(b0 - q_first) * c.nd, d_p3 + (size_t)(b0 - q_first) * c.nd);
@@ -353,7 +367,7 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
                 const int qi = b0 + k, i0 = c.h_irr_grid[qi];
                 hipLaunchKernelGGL(k_lw3_tetra, dim3(nch, c.nd), dim3(TBLK), 0, c.stream, c.wout + (size_t)k * nw,
                                    c.tetra, c.partner + (size_t)k * c.nq, c.omega, c.nbe,
-                                   c.omega + (size_t)i0 * c.nd, c.nd, ntet, chunk, c.partial);
+                                   c.omega + (size_t)i0 * c.nd, c.nd, ntet, chunk, rg, c.partial);
                 SPR_HIP(hipGetLastError());
                 hipLaunchKernelGGL(k_tetra_reduce, dim3(c.nd, 1), dim3(256), 0, c.stream, c.partial, c.nd, nch,
                                    d_lw + (size_t)(qi - q_first) * c.nd, d_p3 + (size_t)(qi - q_first) * c.nd);

@@ -43,6 +43,7 @@ struct Lw4Args {
     // store mode: q1 in [q1_lo, q1_lo + gridDim.x / nq2ch), q2 chunks of q2_chunk points
     double *wst;
     int q1_lo, q2_chunk, nq2ch;
+    const double *sel;   // branch selection (NULL: all): (4, nd) masks of s0, s1, s2, s3
 };
 
 template <typename R, int ND, int BLK, bool WST>
@@ -182,6 +183,7 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
                 double t = W * (F1 * (c0 * exp(-hinv * x1 * x1)) / 6.0 + 0.5 * F2 * (c0 * exp(-hinv * x2 * x2)) +
                                 0.5 * F3 * (c0 * exp(-hinv * x3 * x3)));
                 if (t != t) t = 0.0;
+                if (A.sel) t *= A.sel[s0] * A.sel[ND + s1] * A.sel[2 * ND + s2] * A.sel[3 * ND + s3];
 #pragma unroll
                 for (int s = 0; s < ND; ++s)
                     if (s == s0) acc[s] += t;
@@ -255,6 +257,7 @@ static int launch_lw4_t(Ctx &c, int q0_first, int q0_count, double sigma, double
     if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)q0_count * A.nchunks * ND * sizeof(double))))
         return rc;
     A.partial = c.partial;
+    A.sel = c.sel4 ? c.d_sel4 : nullptr;
     const size_t dyn = (size_t)((3 * A.nsite + 3) & ~3) * sizeof(int) + (size_t)c.L * sizeof(cplx_t<R>);
     SPR_HIP(hipEventRecord(c.ev0, c.stream));
     A.wst = nullptr;

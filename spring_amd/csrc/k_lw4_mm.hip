@@ -54,6 +54,7 @@ struct Lw4MmArgs {
     const double *omega, *inv_omega, *nbe;
     double sigma, factor;
     double *partial;           // [q0][q1][nd]
+    const double *sel;         // branch selection (NULL: all): (4, nd) masks of s0, s1, s2, s3
 };
 
 template <int ND> struct M4 {
@@ -308,8 +309,9 @@ __global__ __launch_bounds__(512) void k_lw4_mm(Lw4MmArgs A) {
             const double w2 = A.omega[(size_t)q2 * ND + s2], w3 = A.omega[(size_t)q3 * ND + s3];
             const double n2 = A.nbe[(size_t)q2 * ND + s2], n3 = A.nbe[(size_t)q3 * ND + s3];
             wCd[l] = make_double2(w2 + w3, w3 - w2);
+            const double m23 = A.sel ? A.sel[2 * ND + s2] * A.sel[3 * ND + s3] : 1.0;
             wCf[l] = make_float4((float)(n2 * n3), (float)((1.0 + n2) * n3),
-                                 (float)(A.inv_omega[(size_t)q2 * ND + s2] * A.inv_omega[(size_t)q3 * ND + s3]), 0.f);
+                                 (float)(A.inv_omega[(size_t)q2 * ND + s2] * A.inv_omega[(size_t)q3 * ND + s3] * m23), 0.f);
         }
         for (int i = l; i < 2 * A.nsite; i += 64) {
             const int4 c = A.site[i % A.nsite];
@@ -420,7 +422,8 @@ __global__ __launch_bounds__(512) void k_lw4_mm(Lw4MmArgs A) {
         const int s01 = nt * 16 + (l & 15);
         if (s01 < NN) {
             const int s0 = s01 / ND, s1 = s01 % ND;
-            const double v = sMd[2 * ND + s0] * sMd[5 * ND + s1] * c0 * A.factor * acc[nt].value();
+            double v = sMd[2 * ND + s0] * sMd[5 * ND + s1] * c0 * A.factor * acc[nt].value();
+            if (A.sel) v *= A.sel[s0] * A.sel[ND + s1];
 #pragma unroll
             for (int s = 0; s < ND; ++s)
                 if (s == s0) tot[s] += v;
@@ -469,6 +472,7 @@ int launch_lw4_mm_t(Ctx &c, int q0_first, int q0_count, double sigma, double *d_
     if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)q0_count * c.nq * ND * sizeof(double))))
         return rc;
     A.partial = c.partial;
+    A.sel = c.sel4 ? c.d_sel4 : nullptr;
     const void *fn = (const void *)k_lw4_mm<ND>;
     SPR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
     SPR_HIP(hipEventRecord(c.ev0, c.stream));

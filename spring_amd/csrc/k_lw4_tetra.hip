@@ -46,6 +46,7 @@ struct Tet4Args {
     int nq, m0, m1, m2;
     int i0, q1_lo, nq1;
     double *partial;            // [gridDim.x][ND] of this chunk
+    int rlo[4], rhi[4];         // branch ranges of s0..s3 (Polarization4th, 0-based inclusive)
 };
 
 template <int ND>
@@ -61,7 +62,8 @@ __global__ __launch_bounds__(T4BLK) void k_lw4_tetra(Tet4Args A) {
     const int total = nrows * per_row;        // the host keeps nq1 x per_row below 2^31
     const int i0 = A.i0;
     const int kp0 = A.kpack[i0];
-    const int s0_lo = i0 == 0 ? 3 : 0;
+    // s0 from Polarization4th(1, 1), from 4 (1-based) at q0 = Gamma (Tetrahedron4th_sep.f90:213-216)
+    const int s0_lo = max(i0 == 0 ? 3 : 0, A.rlo[0]), s0_hi = A.rhi[0];
     __shared__ double s_w0[ND], s_rn0[ND];
     if (tid < ND) {
         s_w0[tid] = A.omega[(size_t)i0 * ND + tid];
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(T4BLK) void k_lw4_tetra(Tet4Args A) {
     double w0lo = s_w0[s0_lo], w0hi = s_w0[s0_lo];
 #pragma unroll
     for (int s = 0; s < ND; ++s)
-        if (s >= s0_lo) {
+        if (s >= s0_lo && s <= s0_hi) {
             w0lo = fmin(w0lo, s_w0[s]);
             w0hi = fmax(w0hi, s_w0[s]);
         }
@@ -87,6 +89,8 @@ __global__ __launch_bounds__(T4BLK) void k_lw4_tetra(Tet4Args A) {
         const int q1l = xcd + 8 * row;
         const int q1 = A.q1_lo + q1l;
         const int s1 = p / ND2, s2 = (p / ND) % ND, s3 = p % ND;
+        if (s1 < A.rlo[1] || s1 > A.rhi[1] || s2 < A.rlo[2] || s2 > A.rhi[2] || s3 < A.rlo[3] || s3 > A.rhi[3])
+            continue;
         // coordinates of -(q0 + q1) on the grid, before adding q2 and wrapping
         const int kp1 = A.kpack[q1];
         const int b0 = (kp0 & 1023) + (kp1 & 1023), b1 = ((kp0 >> 10) & 1023) + ((kp1 >> 10) & 1023),
@@ -138,7 +142,7 @@ __global__ __launch_bounds__(T4BLK) void k_lw4_tetra(Tet4Args A) {
             const Rcp R = rcp6(M);
             const double wk = k == 0 ? 1.0 / 6.0 : 0.5;
 #pragma unroll 1
-            for (int s0 = s0_lo; s0 < ND; ++s0) {
+            for (int s0 = s0_lo; s0 <= s0_hi; ++s0) {
                 const double mq = s_w0[s0];
                 // a process contributes only for M0 <= mq <= M3 (the union of the three cases);
                 // W and F are formed only then
@@ -240,6 +244,10 @@ int launch_lw4_tetra(Ctx &c, int qi, double *d_lw) {
     A.m1 = c.mesh[1];
     A.m2 = c.mesh[2];
     A.i0 = i0;
+    for (int k = 0; k < 4; ++k) {
+        A.rlo[k] = c.pol4[2 * k];
+        A.rhi[k] = c.pol4[2 * k + 1];
+    }
     SPR_HIP(hipEventRecord(c.ev0, c.stream));
     for (int k = 0; k < ncalls; ++k) {
         const int lo = k * nq1c, n = std::min(nq1c, c.nq - lo);

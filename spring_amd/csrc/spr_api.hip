@@ -205,6 +205,8 @@ extern "C" int spr_ctx_create(const spr_crystal *cr, const spr_ifc *fc3, const s
     c.vol = cr->vol;
     for (int k = 0; k < 3; ++k) c.mesh[k] = mesh->mesh[k];
     c.L = (int)lcm(lcm(c.mesh[0], c.mesh[1]), c.mesh[2]);
+    for (int k = 0; k < 3; ++k) c.pol3[2 * k + 1] = c.nd - 1;
+    for (int k = 0; k < 4; ++k) c.pol4[2 * k + 1] = c.nd - 1;
     int rc = SPR_OK;
     auto fail = [&](int r) {
         spr_ctx_destroy(h);
@@ -307,7 +309,7 @@ extern "C" int spr_ctx_destroy(spr_ctx *h) {
                     (void *)c.k_mult, (void *)c.bte.d_coord, (void *)c.bte.d_map, (void *)c.bte.d_irr,
                     (void *)c.bte.d_tetra, (void *)c.bte.d_W, (void *)c.bte.d_F, (void *)c.bte.d_out,
                     (void *)c.bte.d_A, (void *)c.bte.d_itau, (void *)c.bte.d_F0, (void *)c.bte.d_Fb,
-                    (void *)c.bte.d_div})
+                    (void *)c.bte.d_div, (void *)c.d_sel3, (void *)c.d_sel4})
         if (p) (void)hipFree(p);
     if (c.ev0) (void)hipEventDestroy(c.ev0);
     if (c.ev1) (void)hipEventDestroy(c.ev1);
@@ -410,6 +412,51 @@ extern "C" int spr_lw4_all(spr_ctx *h, double sigma, int prec, double *lw_all) {
     return lw_common(h, 0, h->c.nq0, sigma, prec, lw_all, 4);
 }
 
+extern "C" int spr_set_branch_rules(spr_ctx *h, int order, const int *pol) {
+    if (!h || !pol || (order != 3 && order != 4)) {
+        set_error("spr_set_branch_rules: order must be 3 or 4 and pol non-NULL");
+        return SPR_E_ARG;
+    }
+    Ctx &c = h->c;
+    const int nd = c.nd;
+    int r[8];
+    bool all = true;
+    for (int k = 0; k < order; ++k) {
+        const int lo = pol[2 * k], hi = pol[2 * k + 1];
+        if (lo < 1 || hi > nd || lo > hi) {
+            set_error("spr_set_branch_rules: range of slot " + std::to_string(k + 1) + " outside 1.." + std::to_string(nd));
+            return SPR_E_ARG;
+        }
+        r[2 * k] = lo - 1;
+        r[2 * k + 1] = hi - 1;
+        all = all && lo == 1 && hi == nd;
+    }
+    SPR_HIP(hipSetDevice(c.device));
+    auto in = [&](int k, int s) { return s >= r[2 * k] && s <= r[2 * k + 1] ? 1.0 : 0.0; };
+    if (order == 3) {
+        // s0 mask, then the pair mask of the pair-reduced Gaussian sums: an unordered pair {q1, q2}
+        // stands for both orderings, so branch (s1, s2) of the stored ordering takes
+        // (m1(s1) m2(s2) + m1(s2) m2(s1)) / 2 (the reduced sum weights q1 != q2 by 2)
+        std::vector<double> m((size_t)nd + (size_t)nd * nd);
+        for (int s = 0; s < nd; ++s) m[s] = in(0, s);
+        for (int a = 0; a < nd; ++a)
+            for (int b = 0; b < nd; ++b) m[nd + (size_t)a * nd + b] = 0.5 * (in(1, a) * in(2, b) + in(1, b) * in(2, a));
+        if (!c.d_sel3) SPR_HIP(hipMalloc((void **)&c.d_sel3, m.size() * sizeof(double)));
+        SPR_HIP(hipMemcpy(c.d_sel3, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
+        std::copy(r, r + 6, c.pol3);
+        c.sel3 = !all;
+    } else {
+        std::vector<double> m((size_t)4 * nd);
+        for (int k = 0; k < 4; ++k)
+            for (int s = 0; s < nd; ++s) m[(size_t)k * nd + s] = in(k, s);
+        if (!c.d_sel4) SPR_HIP(hipMalloc((void **)&c.d_sel4, m.size() * sizeof(double)));
+        SPR_HIP(hipMemcpy(c.d_sel4, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
+        std::copy(r, r + 8, c.pol4);
+        c.sel4 = !all;
+    }
+    return SPR_OK;
+}
+
 extern "C" int spr_kernel_time(spr_ctx *h, int *launches, double *ms) {
     if (!h) return SPR_E_ARG;
     if (launches) *launches = h->c.launches;
@@ -458,6 +505,41 @@ extern "C" int spr_kappa_rta(spr_ctx *h, const double *lw_all, double T, int sym
         kappa[k] = s;
     }
     if (mode_k) std::memcpy(mode_k, modes.data(), nm * sizeof(double));
+    return SPR_OK;
+}
+
+extern "C" int spr_kappa_rta_spectral(spr_ctx *h, const double *lw_all, double T, int symm, double *kappa,
+                                      double *mode_k, double *spectral_k) {
+    if (!h || !spectral_k) {
+        set_error("spr_kappa_rta_spectral: NULL argument");
+        return SPR_E_ARG;
+    }
+    int rc;
+    if ((rc = spr_kappa_rta(h, lw_all, T, symm, kappa, mode_k))) return rc;
+    Ctx &c = h->c;
+    const int nd = c.nd, nt = symm ? c.nq0 : c.nq;
+    // the per-(q, s) terms spr_kappa_rta left in k_terms (without the kW_m_k / (T^2 V) factor)
+    std::vector<double> terms((size_t)nt * nd * 9);
+    SPR_HIP(hipMemcpy(terms.data(), c.k_terms, terms.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const double kw = (1.054571817 * 1.054571817) * 0.001 / 1.380649;   // constants.f90:65
+    const double f1 = kw / (T * T * c.vol);
+    const size_t ncol = (size_t)nd * c.nq0;
+    std::fill(spectral_k, spectral_k + 11 * ncol, 0.0);
+    // column (qi0, s) = qi0 + s nq0 (indx2num over (Nq0, Ndof)); grid q adds to its irreducible
+    // point in q order (kappa_noIter's q loop) or the irreducible point holds its own term (symm)
+    for (int t = 0; t < nt; ++t) {
+        const int qi = symm ? t : c.h_pos[t];
+        for (int s = 0; s < nd; ++s) {
+            double *col = spectral_k + 11 * ((size_t)s * c.nq0 + qi);
+            for (int k = 0; k < 9; ++k) col[2 + k] += terms[((size_t)t * nd + s) * 9 + k] * f1;
+        }
+    }
+    for (int qi = 0; qi < c.nq0; ++qi)
+        for (int s = 0; s < nd; ++s) {
+            double *col = spectral_k + 11 * ((size_t)s * c.nq0 + qi);
+            col[0] = (double)(s + 1);
+            col[1] = c.h_omega[(size_t)c.h_irr_grid[qi] * nd + s];
+        }
     return SPR_OK;
 }
 

@@ -152,6 +152,13 @@ struct Ctx {
     int w_npairs = 0;
     size_t wout_cap = 0;
     BteState bte;
+    // branch selection rules (PolarizationStrtEnd, Helper_m.f90:310-353): 0-based inclusive
+    // (lo, hi) of s0, s1, s2 (three-phonon) and s0..s3 (four-phonon); every branch by default
+    int pol3[6] = {0, 0, 0, 0, 0, 0};
+    int pol4[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool sel3 = false, sel4 = false;   // a rule other than "every branch" is set
+    double *d_sel3 = nullptr;          // (nd) s0 mask, then (nd, nd) pair mask [s1][s2] (see spr_api.hip)
+    double *d_sel4 = nullptr;          // (4, nd) masks of s0, s1, s2, s3
     // timing of the dominant kernel
     int launches = 0;
     double kernel_ms = 0.0;
@@ -188,6 +195,7 @@ int collect_timing(Ctx &c);
 int launch_reduce_items(Ctx &c, int q0_first, int q0_count, int it0, int upi, double *d_lw);
 constexpr int kMaxUnitsPerItem = 4;   // partial rows reserved per item
 int launch_kappa(Ctx &c, const double *d_lw_all, double T, int symm, double *d_mode_k);
+void kappa_mode_range(const Ctx &c, int &lo, int &hi);
 // tetrahedron linewidths of irreducible q0 [q_first, q_first + q_count): d_lw[k][nd], d_p3[k][nd]
 int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p3);
 // FP64 W of every grid q1 of irreducible q0 [q_first, q_first + q_count) with the small-cell

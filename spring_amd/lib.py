@@ -23,7 +23,7 @@ SYMBOLS = [
     "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all",
     "spr_lw4_tetra", "spr_lw4_tetra_all", "spr_lw3_kappa_all", "spr_mlp_score",
     "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve", "spr_get_phonon",
-    "spr_lw3_slice", "spr_lw3_pairs_noalloc",
+    "spr_lw3_slice", "spr_lw3_pairs_noalloc", "spr_set_branch_rules", "spr_kappa_rta_spectral",
 ]
 
 
@@ -156,6 +156,20 @@ class GpuContext:
                     "spr_v3_w_pairs")
         return W
 
+    def set_branch_rules(self, order: int, rules):
+        """Branch selection of every later `order`-phonon linewidth and of kappa: `rules` is a
+        string of order letters ('A' acoustic, 'O' optical, anything else every branch), as
+        SelectionRule3 / SelectionRule4 of &AnhInfo, or an explicit Polarization array
+        (order x 2, 1-based inclusive)."""
+        if isinstance(rules, str):
+            pol = []
+            for r in rules[:order]:
+                pol += [1, 3] if r in "Aa" else [4, self.nd] if r in "Oo" else [1, self.nd]
+        else:
+            pol = list(np.asarray(rules).ravel())
+        p = np.ascontiguousarray(pol, np.int32)
+        self._check(self.lib.spr_set_branch_rules(self.h, ct.c_int(order), _ptr(p)), "spr_set_branch_rules")
+
     def lw3(self, i0: int, sigma: float, prec: str = "fp64") -> np.ndarray:
         lw = np.zeros(self.nd)
         self._check(self.lib.spr_lw3(self.h, ct.c_int(i0), ct.c_double(sigma),
@@ -194,6 +208,17 @@ class GpuContext:
         self._check(self.lib.spr_kappa_rta(self.h, _ptr(lw), ct.c_double(T), ct.c_int(int(symm)),
                                            _ptr(k), _ptr(modes)), "spr_kappa_rta")
         return k.reshape(3, 3), modes
+
+    def kappa_spectral(self, lw_all: np.ndarray, T: float, symm: bool = True):
+        """(kappa (3, 3), mode_k (Ndof, 9), Spectral_k as (Ndof * nq0, 11) rows (s + 1, omega,
+        nine kappa_ab terms x kW_m_k / (T^2 V)), row qi0 + s nq0)."""
+        k = np.zeros(9)
+        modes = np.zeros((self.nd, 9))
+        spec = np.zeros((self.nd * self.nq0, 11))
+        lw = np.ascontiguousarray(lw_all, np.float64)
+        self._check(self.lib.spr_kappa_rta_spectral(self.h, _ptr(lw), ct.c_double(T), ct.c_int(int(symm)),
+                                                    _ptr(k), _ptr(modes), _ptr(spec)), "spr_kappa_rta_spectral")
+        return k.reshape(3, 3), modes, spec
 
     def lw3_kappa_all(self, sigma: float, prec: str, T: float, symm: bool = True):
         """Linewidths of every irreducible q0 and the RTA kappa in one call, LW_Allq0 kept on the

@@ -4,7 +4,8 @@
  * This library replaces the hot routines of SPRING's kappa.x / LineWidth.x:
  *   ScatterProbAllq1 + ScatterMatEl + ScatterProbability   src/Kappa/Src/ThirdOrder_m.f90:41-166, 278-439
  *   CalculateLinewidth (Gaussian delta)                     src/Kappa/Src/ThirdOrder_m.f90:442-543
- *                                                            src/LineWidth/Src/Linewidth_m.f90:234-323
+ *   CalculateLwTetrahedron (tetrahedron delta)              src/Kappa/Src/ThirdOrder_m.f90:169-275
+ *   LineWidth.x's off-mesh q0 path (spr_lw3_offmesh)        src/LineWidth/Src/Linewidth_m.f90:48-417
  *   ScatterProbq1q2_per + ScatterMatEl4 + ScatterProbability4  src/Kappa/Src/FourthOrder_m.f90:42-473
  *   TetrahedronIntegration4th (processes, Gaussian form)   src/Kappa/Src/Tetrahedron4th_sep.f90:37-425
  *   kappa_noIter / kappa_noIter_symm                        src/Kappa/Src/ThermalConductivity.f90:36-293
@@ -41,6 +42,10 @@ extern "C" {
 #define SPR_PREC_FP64 0
 #define SPR_PREC_FP32 1      /* FP32 phase sums + contraction, compensated FP32 delta sum */
 #define SPR_PREC_BF16 2      /* BF16-operand MFMA contraction, FP32 accumulate */
+
+/* delta function of the energy conservation */
+#define SPR_DELTA_GAUSS 0
+#define SPR_DELTA_TETRA 1    /* linear tetrahedron method (the reference's default) */
 
 /* crystal: the `cell` fields the path uses (src/Kappa/Src/unit_mod.f90:9-38) */
 typedef struct {
@@ -135,6 +140,21 @@ int spr_lw3_slice(spr_ctx *ctx, int q0_off, int q0_cnt, double sigma, int prec, 
 int spr_lw3_tetra(spr_ctx *ctx, int i0, double *lw, double *p3_q0);
 /* every irreducible q0: lw_all(Ndof, nq0), p3_all(Ndof, nq0) */
 int spr_lw3_tetra_all(spr_ctx *ctx, double *lw_all, double *p3_all);
+
+/* LineWidth.x: the three-phonon linewidth of a q0 that is not a mesh point (a point of the
+ * dispersion path), ScatterProbAllq1 + CalculateLwTetrahedron (delta = SPR_DELTA_TETRA, the path
+ * src/LineWidth/Src/main.f90:293-379 runs) of src/LineWidth/Src/Linewidth_m.f90:48-417, or with
+ * SPR_DELTA_GAUSS the Gaussian sum of CalculateLinewidth's formula (:234-323; main.f90 keeps that
+ * call switched off) over every branch, sigma in THz.  q0_cart (1/A) with its phonons as set_Phon_mp gives
+ * them (evec0 = Evec(Ndof, Ndof) complex, omega0, inv_omega0 = OnebyOmega); the q1 are the
+ * context's mesh (zero_centered selects the reference's tetraQZeroCentered mesh and BZ test,
+ * main.f90:316-330); for every q1, q2 = G - (q0 + q1) with the reference's G search
+ * (:120-184) and its phonons solved on the device from fc2 (order 2, fc(3,3,natm,nb)) at
+ * temperature T; a q1 whose q2 is not found adds nothing.  lw(Ndof) rad/ps; W_out (optional,
+ * Sctrq1(Ndof, Ndof, Ndof, Nq)) the W rows.  FP64.  latvec as spr_get_phonon. */
+int spr_lw3_offmesh(spr_ctx *ctx, const double *q0_cart, const double *evec0, const double *omega0,
+                    const double *inv_omega0, const spr_crystal *cr, const double *latvec, const spr_ifc *fc2,
+                    double T, int zero_centered, int delta, double sigma, double *lw, double *W_out);
 
 /* ---- Iterative Boltzmann transport (kappa_iter.x, src/Kappa/Src_iterative) ----
  * Works on the q points irreducible under time reversal only, as the reference does

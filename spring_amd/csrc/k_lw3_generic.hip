@@ -39,6 +39,12 @@ struct Lw3Args {
     const int *pair_q0;   // pair mode (W_OUT only): block x = pair, q0 index and q1 per pair
     const int *pair_q1;
     const double *sel;    // branch selection (NULL: all): (nd) s0 mask, (nd, nd) pair mask [s1][s2]
+    // off-mesh q0 (OFF instantiation, W output of one q0): its eigenvectors and 1/omega, the
+    // eigenvectors and 1/omega of q2 = G - (q0 + q1) per grid q1, e^{-2 pi i x0 . n} per site
+    // (x0 the fractional q0, n the site's cell), and whether q2 was found for each q1
+    const void *off_e0, *off_e2, *off_p0;
+    const double *off_iw0, *off_iw2;
+    const int *off_exists;
 };
 
 __device__ __forceinline__ void grid_coords(int i, int m0, int m1, int &k0, int &k1, int &k2) {
@@ -56,29 +62,33 @@ __device__ __forceinline__ double gauss_g(double w0, double w1, double w2, doubl
     return (n1 + n2 + 1.0) * g1 + (n1 - n2) * (g2 - g3);
 }
 
-template <typename R, int ND, int S0B, int BLK, bool W_OUT>
+template <typename R, int ND, int S0B, int BLK, bool W_OUT, bool OFF = false>
 __global__ __launch_bounds__(BLK) void k_lw3_generic(Lw3Args A) {
+    static_assert(!OFF || W_OUT, "the off-mesh mode writes W");
     using Z = cplx_t<R>;
     constexpr int NB = ND / 3;
     constexpr int ND2 = ND * ND;
     constexpr int NS0B = (ND + S0B - 1) / S0B;
     __shared__ Z s_e0[ND2], s_e1[ND2], s_e2[ND2], s_T[ND2], s_Z[ND2];
-    extern __shared__ __align__(16) int s_dyn[];   // m1[nsite], m2[nsite], phase[L]
+    extern __shared__ __align__(16) int s_dyn[];   // m1[nsite], m2[nsite], phase[L]; OFF: + site phases
     int *s_m1 = s_dyn;
     int *s_m2 = s_dyn + A.nsite;
     Z *s_ph = reinterpret_cast<Z *>(s_dyn + 2 * ((A.nsite + 1) & ~1));
+    Z *s_pa = s_ph + A.L;          // OFF: phase of q1 at each site
+    Z *s_pb = s_pa + A.nsite;      // OFF: phase of q2 at each site
 
     const int tid = threadIdx.x;
     const int ch = blockIdx.x, s0b = blockIdx.y, qb = blockIdx.z;
     const bool pair_mode = W_OUT && A.pair_q0 != nullptr;
-    const int i0 = A.q0_grid[pair_mode ? A.pair_q0[ch] : qb];
+    const int i0 = OFF ? 0 : A.q0_grid[pair_mode ? A.pair_q0[ch] : qb];
     int k00, k01, k02;
     grid_coords(i0, A.m0, A.m1, k00, k01, k02);
     const Z *evec = reinterpret_cast<const Z *>(A.evec);
     const Z *phase = reinterpret_cast<const Z *>(A.phase);
     const R *fc = reinterpret_cast<const R *>(A.fc);
 
-    for (int i = tid; i < ND2; i += BLK) s_e0[i] = evec[(size_t)i0 * ND2 + i];
+    const Z *e0src = OFF ? reinterpret_cast<const Z *>(A.off_e0) : evec + (size_t)i0 * ND2;
+    for (int i = tid; i < ND2; i += BLK) s_e0[i] = e0src[i];
     for (int i = tid; i < A.L; i += BLK) s_ph[i] = phase[i];
 
     const bool own = tid < ND2;
@@ -104,14 +114,23 @@ __global__ __launch_bounds__(BLK) void k_lw3_generic(Lw3Args A) {
         if (!W_OUT && q1 > i2) continue;
         const double wp = W_OUT ? 1.0 : (q1 == i2 ? 1.0 : 2.0);
         __syncthreads();
+        const Z *e2src = OFF ? reinterpret_cast<const Z *>(A.off_e2) + (size_t)q1 * ND2 : evec + (size_t)i2 * ND2;
         for (int i = tid; i < ND2; i += BLK) {
             s_e1[i] = evec[(size_t)q1 * ND2 + i];
-            s_e2[i] = evec[(size_t)i2 * ND2 + i];
+            s_e2[i] = e2src[i];
         }
         for (int s = tid; s < A.nsite; s += BLK) {
             const int4 c = A.site[s];
             s_m1[s] = (k10 * c.x + k11 * c.y + k12 * c.z) % A.L;
             s_m2[s] = (a0 * c.x + a1 * c.y + a2 * c.z) % A.L;
+            if constexpr (OFF) {
+                // q2 = G - (q0 + q1): e^{2 pi i x2 . n} = e^{-2 pi i x0 . n} conj(e^{2 pi i x1 . n}) for
+                // an integer cell n, whatever the G
+                const Z p1 = s_ph[s_m1[s]];
+                const Z p0 = reinterpret_cast<const Z *>(A.off_p0)[s];
+                s_pa[s] = p1;
+                s_pb[s] = cmk<R>(p0.x * p1.x + p0.y * p1.y, p0.y * p1.x - p0.x * p1.y);
+            }
         }
         __syncthreads();
 
@@ -128,9 +147,16 @@ __global__ __launch_bounds__(BLK) void k_lw3_generic(Lw3Args A) {
                 Z t = cmk<R>(0, 0);
                 const int t1 = A.blk_start[blk + 1];
                 for (int u = A.blk_start[blk]; u < t1; ++u) {
-                    int m = s_m1[A.tup_site[2 * u]] + s_m2[A.tup_site[2 * u + 1]];
-                    m -= m >= A.L ? A.L : 0;
-                    t = crfma(fc[(size_t)u * 27 + al * 9 + be * 3 + ga], s_ph[m], t);
+                    Z ph;
+                    if constexpr (OFF) {
+                        const Z x = s_pa[A.tup_site[2 * u]], y = s_pb[A.tup_site[2 * u + 1]];
+                        ph = cmk<R>(x.x * y.x - x.y * y.y, x.x * y.y + x.y * y.x);
+                    } else {
+                        int m = s_m1[A.tup_site[2 * u]] + s_m2[A.tup_site[2 * u + 1]];
+                        m -= m >= A.L ? A.L : 0;
+                        ph = s_ph[m];
+                    }
+                    t = crfma(fc[(size_t)u * 27 + al * 9 + be * 3 + ga], ph, t);
                 }
                 s_T[tid] = t;
             }
@@ -157,13 +183,15 @@ __global__ __launch_bounds__(BLK) void k_lw3_generic(Lw3Args A) {
         if (own) {
             const double w1 = A.omega[(size_t)q1 * ND + js1], w2 = A.omega[(size_t)i2 * ND + js2];
             const double n1 = A.nbe[(size_t)q1 * ND + js1], n2 = A.nbe[(size_t)i2 * ND + js2];
-            const double iw12 = A.inv_omega[(size_t)q1 * ND + js1] * A.inv_omega[(size_t)i2 * ND + js2];
+            const double iw12 = A.inv_omega[(size_t)q1 * ND + js1] *
+                                (OFF ? A.off_iw2[(size_t)q1 * ND + js2] : A.inv_omega[(size_t)i2 * ND + js2]);
+            const bool exists = !OFF || A.off_exists[q1];
 #pragma unroll
             for (int l = 0; l < S0B; ++l) {
                 const int s0 = s0b * S0B + l;
                 if (s0 >= ND) continue;
-                const double iw = A.inv_omega[(size_t)i0 * ND + s0] * iw12;
-                const double W = (double)cabs2(V[l]) * iw * A.factor;
+                const double iw = (OFF ? A.off_iw0[s0] : A.inv_omega[(size_t)i0 * ND + s0]) * iw12;
+                const double W = exists ? (double)cabs2(V[l]) * iw * A.factor : 0.0;
                 if (W_OUT) {
                     A.w_out[(((size_t)(pair_mode ? ch : q1) * ND + s0) * ND + js1) * ND + js2] = W;
                 } else {
@@ -273,6 +301,72 @@ static int launch_nd(Ctx &c, const int *d_q0, int nq0, double sigma, double *d_l
     c.launches += 1;
     c.kernel_ms += ms;
     return SPR_OK;
+}
+
+// FP64 W of every grid q1 for an off-mesh q0 (the W rows of LineWidth's ScatterProbAllq1)
+template <int ND>
+static int launch_off_nd(Ctx &c, const OffMeshDev &om, double *d_w) {
+    constexpr int S0B = ND <= 15 ? ND : 10;
+    constexpr int NS0B = (ND + S0B - 1) / S0B;
+    constexpr int BLK = ((ND * ND + 63) / 64) * 64;
+    Lw3Args A{};
+    A.q0_grid = nullptr;
+    A.nq = c.nq;
+    A.L = c.L;
+    A.m0 = c.mesh[0];
+    A.m1 = c.mesh[1];
+    A.m2 = c.mesh[2];
+    const long target = 4096;
+    int nch = (int)std::min<long>(c.nq, std::max<long>(1, (target + NS0B - 1) / NS0B));
+    A.chunk = (c.nq + nch - 1) / nch;
+    A.nchunks = (c.nq + A.chunk - 1) / A.chunk;
+    A.blk_start = c.fc3.blk_start;
+    A.tup_site = c.fc3.tup_site;
+    A.fc = c.fc3.fc64;
+    A.site = c.fc3.site;
+    A.nsite = c.fc3.nsite;
+    A.phase = c.phase64;
+    A.evec = c.evec;
+    A.omega = c.omega;
+    A.inv_omega = c.inv_omega;
+    A.nbe = c.nbe;
+    A.sel = nullptr;
+    A.sigma = 1.0;
+    const double pi = 4.0 * std::atan(1.0);
+    A.factor = 1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) * 1000000.0 * pi *
+               0.0625 / (double)c.nq;   // Sctr3THz2 / Nq (constants.f90:59-60)
+    A.w_out = d_w;
+    A.pair_q0 = A.pair_q1 = nullptr;
+    A.off_e0 = om.e0;
+    A.off_e2 = om.e2;
+    A.off_p0 = om.p0;
+    A.off_iw0 = om.iw0;
+    A.off_iw2 = om.iw2;
+    A.off_exists = om.exists;
+    const size_t dyn = (size_t)2 * ((A.nsite + 1) & ~1) * sizeof(int) + (size_t)(c.L + 2 * A.nsite) * sizeof(double2);
+    if (dyn > 64 * 1024)
+        SPR_HIP(hipFuncSetAttribute((const void *)k_lw3_generic<double, ND, S0B, BLK, true, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    hipLaunchKernelGGL((k_lw3_generic<double, ND, S0B, BLK, true, true>), dim3(A.nchunks, NS0B, 1), dim3(BLK), dyn,
+                       c.stream, A);
+    SPR_HIP(hipGetLastError());
+    return SPR_OK;
+}
+
+int launch_w_offmesh(Ctx &c, const OffMeshDev &om, double *d_w) {
+    switch (c.nd) {
+        case 3: return launch_off_nd<3>(c, om, d_w);
+        case 6: return launch_off_nd<6>(c, om, d_w);
+        case 9: return launch_off_nd<9>(c, om, d_w);
+        case 12: return launch_off_nd<12>(c, om, d_w);
+        case 15: return launch_off_nd<15>(c, om, d_w);
+        case 18: return launch_off_nd<18>(c, om, d_w);
+        case 24: return launch_off_nd<24>(c, om, d_w);
+        case 30: return launch_off_nd<30>(c, om, d_w);
+        default:
+            set_error("off-mesh three-phonon W: Ndof " + std::to_string(c.nd) + " not built (3..18, 24, 30)");
+            return SPR_E_UNSUPPORTED;
+    }
 }
 
 template <typename R>

@@ -300,6 +300,27 @@ int ensure_tetra(Ctx &c) {
     return SPR_OK;
 }
 
+int tetra_sums_from_w(Ctx &c, const double *d_w, const int *d_partner, const double *d_omega, const double *d_nbe,
+                      const double *d_omega0, double *d_lw, double *d_p3) {
+    int rc;
+    if ((rc = ensure_tetra(c))) return rc;
+    const int ntet = 6 * c.nq;
+    const int chunk = std::max(1, (ntet + 255) / 256);
+    const int nch = (ntet + chunk - 1) / chunk;
+    if ((rc = ensure((void **)&c.partial, &c.partial_cap, (size_t)nch * c.nd * 2 * sizeof(double)))) return rc;
+    Ranges3 rg;
+    for (int k = 0; k < 3; ++k) {
+        rg.lo[k] = 0;
+        rg.hi[k] = c.nd - 1;
+    }
+    hipLaunchKernelGGL(k_lw3_tetra, dim3(nch, c.nd), dim3(TBLK), 0, c.stream, d_w, c.tetra, d_partner, d_omega, d_nbe,
+                       d_omega0, c.nd, ntet, chunk, rg, c.partial);
+    SPR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_tetra_reduce, dim3(c.nd, 1), dim3(256), 0, c.stream, c.partial, c.nd, nch, d_lw, d_p3);
+    SPR_HIP(hipGetLastError());
+    return SPR_OK;
+}
+
 // Tetrahedron linewidths of irreducible q0 [q_first, q_first + q_count) into d_lw[k][nd] and
 // d_p3[k][nd] (device, k = q - q_first).  W of every q1 for up to 16 q0 at a time in c.wout:
 // with the small-cell kernel (pair-reduced, both orderings written) for Nb <= 2, else the

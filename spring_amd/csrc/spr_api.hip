@@ -79,6 +79,7 @@ static int build_ifc(Ctx &c, const spr_ifc *f, const double *osm, DevIFC &d) {
                 cc[k] = (int)((m * (c.L / c.mesh[k])) % c.L);
             }
             site[site_off[mu] + n] = make_int4(cc[0], cc[1], cc[2], ix[3]);
+            d.h_site_cell.push_back(make_int4(ix[0], ix[1], ix[2], ix[3]));
         }
     const int nblk = order == 3 ? nb * nb * nb : nb * nb * nb * nb;
     std::vector<std::vector<int>> tup(nblk);
@@ -309,7 +310,7 @@ extern "C" int spr_ctx_destroy(spr_ctx *h) {
                     (void *)c.k_mult, (void *)c.bte.d_coord, (void *)c.bte.d_map, (void *)c.bte.d_irr,
                     (void *)c.bte.d_tetra, (void *)c.bte.d_W, (void *)c.bte.d_F, (void *)c.bte.d_out,
                     (void *)c.bte.d_A, (void *)c.bte.d_itau, (void *)c.bte.d_F0, (void *)c.bte.d_Fb,
-                    (void *)c.bte.d_div, (void *)c.d_sel3, (void *)c.d_sel4})
+                    (void *)c.bte.d_div, (void *)c.d_sel3, (void *)c.d_sel4, (void *)c.off_buf})
         if (p) (void)hipFree(p);
     if (c.ev0) (void)hipEventDestroy(c.ev0);
     if (c.ev1) (void)hipEventDestroy(c.ev1);

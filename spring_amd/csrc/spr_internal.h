@@ -68,6 +68,7 @@ struct DevIFC {
     double *fc64 = nullptr;    // (nterm, 3^order) mass-scaled FC values, (a,b,c[,d]) order
     float *fc32 = nullptr;
     int4 *site = nullptr;      // (nsite): phase coefficients (c1,c2,c3) and basis atom
+    std::vector<int4> h_site_cell;   // (nsite): the site's lattice cell (n1,n2,n3) and basis atom
     std::vector<int> h_blk_start;
     // the same list with every block padded to a multiple of kClusterBatch tuples (zero FC
     // values on a valid site), for kernels that load the tuples of a batch together
@@ -159,6 +160,8 @@ struct Ctx {
     bool sel3 = false, sel4 = false;   // a rule other than "every branch" is set
     double *d_sel3 = nullptr;          // (nd) s0 mask, then (nd, nd) pair mask [s1][s2] (see spr_api.hip)
     double *d_sel4 = nullptr;          // (4, nd) masks of s0, s1, s2, s3
+    double *off_buf = nullptr;         // inputs of the off-mesh (LineWidth) path
+    size_t off_cap = 0;
     // timing of the dominant kernel
     int launches = 0;
     double kernel_ms = 0.0;
@@ -207,6 +210,19 @@ int ensure_tetra(Ctx &c);
 // d_w[q1 - q1_lo][q2][s0][s1][s2][s3]
 int launch_w4_rows(Ctx &c, int qi, int q1_lo, int nq1, double *d_w);
 int launch_lw4_tetra(Ctx &c, int qi, double *d_lw);
+// device inputs of the off-mesh q0 W (LineWidth): eigenvectors (nd, nd) and 1/omega (nd) of q0,
+// eigenvectors (nq, nd, nd) and 1/omega (nq, nd) of q2 per grid q1, e^{-2 pi i x0 . n} per FC3 site,
+// and whether q2 exists per q1
+struct OffMeshDev {
+    const double2 *e0, *e2, *p0;
+    const double *iw0, *iw2;
+    const int *exists;
+};
+int launch_w_offmesh(Ctx &c, const OffMeshDev &om, double *d_w);
+// tetrahedron sums of one q0 from W of every q1 (k_lw3_tetra): q2 data at omega[partner[q1]],
+// nbe[partner[q1]]; every branch; d_lw(nd), d_p3(nd)
+int tetra_sums_from_w(Ctx &c, const double *d_w, const int *d_partner, const double *d_omega, const double *d_nbe,
+                      const double *d_omega0, double *d_lw, double *d_p3);
 // resolves the device address of the no-allocation scratch array of `device` (current device)
 int noalloc_setup(int device);
 

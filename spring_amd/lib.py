@@ -24,6 +24,7 @@ SYMBOLS = [
     "spr_lw4_tetra", "spr_lw4_tetra_all", "spr_lw3_kappa_all", "spr_mlp_score",
     "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve", "spr_get_phonon",
     "spr_lw3_slice", "spr_lw3_pairs_noalloc", "spr_set_branch_rules", "spr_kappa_rta_spectral",
+    "spr_lw3_offmesh",
 ]
 
 
@@ -249,6 +250,32 @@ class GpuContext:
         self._check(self.lib.spr_lw3_pairs_noalloc(self.h, ct.c_int(q0.size), _ptr(q0), _ptr(q1), ct.c_double(sigma),
                                                    ct.c_int(PREC[prec]), _ptr(out)), "spr_lw3_pairs_noalloc")
         return out
+
+    def lw3_offmesh(self, model, q0_cart, ph0, T: float, delta: str = "tetra", sigma: float = 0.1,
+                    zero_centered: bool = True, want_w: bool = False):
+        """LineWidth.x's linewidth of an off-mesh q0 (spr_lw3_offmesh).  ph0: dict or object with
+        evec (Ndof, Ndof) [s][row], omega, inv_omega of q0; model gives the crystal, lattice and
+        FC2 for the q2 phonons.  Returns lw (Ndof) [and W[q1][s0][s1][s2]]."""
+        cr, fc2 = model.crystal, model.fc2
+        get = (lambda k: ph0[k]) if isinstance(ph0, dict) else (lambda k: getattr(ph0, k))
+        e0 = np.ascontiguousarray(get("evec"), np.complex128)
+        w0 = np.ascontiguousarray(get("omega"), np.float64)
+        iw0 = np.ascontiguousarray(get("inv_omega"), np.float64)
+        q0 = np.ascontiguousarray(q0_cart, np.float64)
+        mass = np.ascontiguousarray(cr.mass, np.float64)
+        crs = _Crystal(cr.nb, _ptr(mass), cr.vol)
+        an = np.ascontiguousarray(fc2.atm_num, np.int32)
+        ai = np.ascontiguousarray(fc2.atm_indx, np.int32)
+        fc = np.ascontiguousarray(fc2.fc, np.float64)
+        f2 = _IFC(2, fc2.natm_max, _ptr(an), _ptr(ai), _ptr(fc))
+        lat = np.ascontiguousarray(cr.latvec, np.float64)
+        lw = np.zeros(self.nd)
+        W = np.zeros((self.nq, self.nd, self.nd, self.nd)) if want_w else None
+        self._check(self.lib.spr_lw3_offmesh(self.h, _ptr(q0), _ptr(e0), _ptr(w0), _ptr(iw0), ct.byref(crs), _ptr(lat),
+                                             ct.byref(f2), ct.c_double(T), ct.c_int(int(zero_centered)),
+                                             ct.c_int(1 if delta == "tetra" else 0), ct.c_double(sigma), _ptr(lw),
+                                             _ptr(W) if want_w else None), "spr_lw3_offmesh")
+        return (lw, W) if want_w else lw
 
     def lw3_tetra(self, i0: int):
         """Tetrahedron-method linewidth and P3_q0 of irreducible q0 number i0 (FP64)."""

@@ -11,6 +11,10 @@
 ! them (through libspring_h5, since the HDF5 Fortran module of this image is not readable by
 ! this compiler).
 !
+! usage: kappa_gpu [-inp input.nml] [-T K] [-Symm T|F] [-Spectral_k T|F] [-Mode_resolved_k T|F]
+!                  [-prec fp64|fp32|bf16] [-delta tetra|gauss]
+!   the reference's command line and inputs (input.nml + FC*_F.h5 in the working directory):
+!   kappa_nml_m runs it.  With a directory as the first argument instead:
 ! usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter|tetra] [gpuph]
 !   loop: one spr_lw3 call per q0 (the reference's q0 loop); all: one spr_lw3_all launch;
 !   tetra: the reference's default delta, the tetrahedron method, per q0 for the three-phonon
@@ -24,6 +28,7 @@ program kappa_gpu
 
     use spring_gpu_m
     use spring_h5_m
+    use kappa_nml_m
     implicit none
 
     character(len=512)      :: dir, arg
@@ -57,9 +62,9 @@ program kappa_gpu
     namelist /bundle/ prefix, Nbasis, Natm2, Natm3, Natm4, mesh, Nq, Nq0, Nsym, T, Sigma, vol, FourPhonon
 
     call get_command_argument(1, dir)
-    if ( len_trim(dir) == 0 ) then
-        write(*, *) 'usage: kappa_gpu <bundle-dir> [fp64|fp32|bf16] [loop|all|iter|tetra] [gpuph]'
-        ERROR STOP
+    if ( len_trim(dir) == 0 .or. dir(1:1) == '-' ) then
+        call run_namelist_host()
+        stop
     end if
     prec = SPR_PREC_FP64
     call get_command_argument(2, arg)
@@ -250,25 +255,6 @@ program kappa_gpu
     call gpu_ctx_destroy(ctx)
 
 contains
-
-    subroutine WriteAsciiOutFile(filename, write_frmt, freq, LW_q0, Ndof, HeaderWrite)
-        character(len=*), intent(in)    :: filename, write_frmt
-        real(dp), intent(in)            :: freq(:), LW_q0(:)
-        integer, intent(in)             :: Ndof
-        logical, intent(inout)          :: HeaderWrite
-        integer                         :: ui
-        if ( HeaderWrite ) then
-            open(newunit=ui, file=filename, status='REPLACE', action='WRITE')
-            write(ui, 20) Ndof, Ndof
-            HeaderWrite = .false.
-        else
-            open(newunit=ui, file=filename, status='OLD', action='WRITE', position='APPEND')
-        end if
-        write(ui, write_frmt) freq, LW_q0
-        close(ui)
-        20 FORMAT( '#--------- phonon_frequency(rad.THz) ( ', I4, ' columns) ---------      ', &
-                 & '---------- linewidth(rad.THz) ( ', I4, ' columns) ----------')
-    end subroutine WriteAsciiOutFile
 
     subroutine rd_r(f, a)
         character(len=*), intent(in) :: f

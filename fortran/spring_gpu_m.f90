@@ -16,6 +16,13 @@
 !                             the self-consistent loop of main.f90:243-361
 !   get_phonon_gpu         <- DynaMat::get_phonon (src/Kappa/Src/DynaMat_mod.f90:486-709), non-polar:
 !                             Evec, freq, OnebyFreq, nBE, grp_vel of every q, freq_min/freq_max
+!   PolarizationStrtEnd    <- Helper::PolarizationStrtEnd (src/Kappa/Src/Helper_m.f90:310-353); the
+!                             Polarization3rd/4th it returns are the optional last arguments of
+!                             the linewidth and kappa wrappers, as of the routines they replace
+!   CalculateLwOffMesh_gpu <- LineWidth.x's ScatterProbAllq1 + CalculateLwTetrahedron
+!                             (src/LineWidth/Src/Linewidth_m.f90:48-417) for a q0 off the mesh
+!   MakeMeshIrrQ_gpu       <- q_tetrahedron%make_tetrahedron + FindIrrQ (TetrahedronQ_m.f90:56-331)
+!                             and the rotations of set_SpaceGrSPGlib, host code of the library
 ! Index arrays arrive 1-based as in the reference (atmIndx(4,:,:), pos, the irreducible
 ! grid index) and are shifted to the C ABI's 0-based convention here.
 module spring_gpu_m
@@ -192,6 +199,48 @@ module spring_gpu_m
             real(c_double)                  :: omega(*), inv_omega(*), nbe(*), grp_vel(*)
         end function spr_get_phonon
 
+        integer(c_int) function spr_set_branch_rules(h, order, pol) bind(C, name="spr_set_branch_rules")
+            import :: c_int, c_ptr
+            type(c_ptr), value      :: h
+            integer(c_int), value   :: order
+            integer(c_int)          :: pol(*)
+        end function spr_set_branch_rules
+
+        integer(c_int) function spr_kappa_rta_spectral(h, lw, T, symm, kappa, mode_k, spectral) &
+                & bind(C, name="spr_kappa_rta_spectral")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: h
+            real(c_double)          :: lw(*)
+            real(c_double), value   :: T
+            integer(c_int), value   :: symm
+            real(c_double)          :: kappa(9), mode_k(*), spectral(*)
+        end function spr_kappa_rta_spectral
+
+        integer(c_int) function spr_lw3_offmesh(h, q0, evec0, omega0, inv_omega0, cr, latvec, fc2, T, &
+                & zero_centered, delta, sigma, lw, W) bind(C, name="spr_lw3_offmesh")
+            import :: c_int, c_ptr, c_double, spr_crystal, spr_ifc
+            type(c_ptr), value              :: h
+            real(c_double), intent(in)      :: q0(3), omega0(*), inv_omega0(*), latvec(9)
+            complex(c_double), intent(in)   :: evec0(*)
+            type(spr_crystal), intent(in)   :: cr
+            type(spr_ifc), intent(in)       :: fc2
+            real(c_double), value           :: T, sigma
+            integer(c_int), value           :: zero_centered, delta
+            real(c_double)                  :: lw(*)
+            type(c_ptr), value              :: W
+        end function spr_lw3_offmesh
+
+        integer(c_int) function spr_mesh_build(latvec, nb, frac, typ, mesh, time_rev, symmetry, symprec, &
+                & nq0, nsym, q_int, q_cart, irr_grid, pos, multiplicity, rot, tetra) bind(C, name="spr_mesh_build")
+            import :: c_int, c_ptr, c_double
+            real(c_double), intent(in)      :: latvec(9), frac(*)
+            integer(c_int), value           :: nb, time_rev, symmetry
+            integer(c_int), intent(in)      :: typ(*), mesh(3)
+            real(c_double), value           :: symprec
+            integer(c_int)                  :: nq0, nsym
+            type(c_ptr), value              :: q_int, q_cart, irr_grid, pos, multiplicity, rot, tetra
+        end function spr_mesh_build
+
         integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s
@@ -201,7 +250,7 @@ module spring_gpu_m
     public :: gpu_ctx_create, gpu_ctx_destroy, ScatterProbAllq1_gpu, CalculateLinewidth_gpu, &
             & LinewidthAllq0_gpu, FourPhononLw_gpu, kappa_noIter_gpu, spr_error_text, &
             & CalculateLwTetrahedron_gpu, TetrahedronIntegration4th_gpu, FourPhononLwTetra_gpu, IrrQ_tr_gpu, TetrahedronW_gpu, TetrahedronIter_gpu, &
-            & kappa_iter_gpu, get_phonon_gpu
+            & kappa_iter_gpu, get_phonon_gpu, PolarizationStrtEnd, CalculateLwOffMesh_gpu, MakeMeshIrrQ_gpu
 
 contains
 
@@ -308,30 +357,37 @@ contains
     end subroutine ScatterProbAllq1_gpu
 
     ! Gaussian-delta three-phonon linewidth lw(Ndof) of irreducible q0 number i0 (1-based).
-    subroutine CalculateLinewidth_gpu(ctx, i0, Sigma, prec, lw)
+    subroutine CalculateLinewidth_gpu(ctx, i0, Sigma, prec, lw, Polarization3rd)
         type(gpu_ctx), intent(in)   :: ctx
         integer, intent(in)         :: i0, prec
         real(dp), intent(in)        :: Sigma
         real(dp), intent(out)       :: lw(:)
+        integer, intent(in), optional :: Polarization3rd(2, 3)
+        if ( present(Polarization3rd) ) call set_rules(ctx, 3, Polarization3rd)
         call check(spr_lw3(ctx%h, int(i0 - 1, c_int), Sigma, int(prec, c_int), lw), 'CalculateLinewidth_gpu')
     end subroutine CalculateLinewidth_gpu
 
     ! LW_Allq0(Ndof, Nq0) of every irreducible q0 in one launch.
-    subroutine LinewidthAllq0_gpu(ctx, Sigma, prec, LW_Allq0)
+    subroutine LinewidthAllq0_gpu(ctx, Sigma, prec, LW_Allq0, Polarization3rd)
         type(gpu_ctx), intent(in)   :: ctx
         integer, intent(in)         :: prec
         real(dp), intent(in)        :: Sigma
         real(dp), intent(out)       :: LW_Allq0(:,:)
+        integer, intent(in), optional :: Polarization3rd(2, 3)
+        if ( present(Polarization3rd) ) call set_rules(ctx, 3, Polarization3rd)
         call check(spr_lw3_all(ctx%h, Sigma, int(prec, c_int), LW_Allq0), 'LinewidthAllq0_gpu')
     end subroutine LinewidthAllq0_gpu
 
     ! Tetrahedron-method linewidth lw(Ndof) and P3_q0(Ndof) of irreducible q0 number i0 (1-based),
     ! the outputs of CalculateLwTetrahedron (ThirdOrder_m.f90:169-188); P3 = sum(P3_q0).
-    ! Needs the tetrahedra at gpu_ctx_create.
-    subroutine CalculateLwTetrahedron_gpu(ctx, i0, lw, P3_q0, P3)
+    ! Needs the tetrahedra at gpu_ctx_create.  Polarization3rd (from PolarizationStrtEnd) restricts
+    ! the branch loops as the reference's argument of the same name (ThirdOrder_m.f90:207-214).
+    subroutine CalculateLwTetrahedron_gpu(ctx, i0, lw, P3_q0, P3, Polarization3rd)
         type(gpu_ctx), intent(in)   :: ctx
         integer, intent(in)         :: i0
         real(dp), intent(out)       :: lw(:), P3_q0(:), P3
+        integer, intent(in), optional :: Polarization3rd(2, 3)
+        if ( present(Polarization3rd) ) call set_rules(ctx, 3, Polarization3rd)
         call check(spr_lw3_tetra(ctx%h, int(i0 - 1, c_int), lw, P3_q0), 'CalculateLwTetrahedron_gpu')
         P3 = sum(P3_q0)
     end subroutine CalculateLwTetrahedron_gpu
@@ -340,42 +396,89 @@ contains
     ! of TetrahedronIntegration4th (Tetrahedron4th_sep.f90:37-170) summed over every q1 (the
     ! reference sums the images' parts); the caller adds 0.5*lw (main.f90:660).  FP64.  Needs FC4
     ! and the tetrahedra at gpu_ctx_create.
-    subroutine TetrahedronIntegration4th_gpu(ctx, i0, lw)
+    subroutine TetrahedronIntegration4th_gpu(ctx, i0, lw, Polarization4th)
         type(gpu_ctx), intent(in)   :: ctx
         integer, intent(in)         :: i0
         real(dp), intent(out)       :: lw(:)
+        integer, intent(in), optional :: Polarization4th(2, 4)
+        if ( present(Polarization4th) ) call set_rules(ctx, 4, Polarization4th)
         call check(spr_lw4_tetra(ctx%h, int(i0 - 1, c_int), lw), 'TetrahedronIntegration4th_gpu')
     end subroutine TetrahedronIntegration4th_gpu
 
     ! lw4(Ndof, Nq0) of every irreducible q0 with the tetrahedron method (as above).
-    subroutine FourPhononLwTetra_gpu(ctx, lw4)
+    subroutine FourPhononLwTetra_gpu(ctx, lw4, Polarization4th)
         type(gpu_ctx), intent(in)   :: ctx
         real(dp), intent(out)       :: lw4(:,:)
+        integer, intent(in), optional :: Polarization4th(2, 4)
+        if ( present(Polarization4th) ) call set_rules(ctx, 4, Polarization4th)
         call check(spr_lw4_tetra_all(ctx%h, lw4), 'FourPhononLwTetra_gpu')
     end subroutine FourPhononLwTetra_gpu
 
     ! Raw four-phonon sum lw4(Ndof, Nq0); the caller adds 0.5*lw4 (main.f90:660).
-    subroutine FourPhononLw_gpu(ctx, Sigma, prec, lw4)
+    subroutine FourPhononLw_gpu(ctx, Sigma, prec, lw4, Polarization4th)
         type(gpu_ctx), intent(in)   :: ctx
         integer, intent(in)         :: prec
         real(dp), intent(in)        :: Sigma
         real(dp), intent(out)       :: lw4(:,:)
+        integer, intent(in), optional :: Polarization4th(2, 4)
+        if ( present(Polarization4th) ) call set_rules(ctx, 4, Polarization4th)
         call check(spr_lw4_all(ctx%h, Sigma, int(prec, c_int), lw4), 'FourPhononLw_gpu')
     end subroutine FourPhononLw_gpu
 
-    ! kappa(9) in W/mK (column-major 3x3) and Mode_k(9, Ndof).
-    subroutine kappa_noIter_gpu(ctx, LW_Allq0, T, Symm, kappa, Mode_k)
+    ! kappa(9) in W/mK (column-major 3x3) and Mode_k(9, Ndof); Spectral_k(11, Ndof*Nq0) when
+    ! present (-Spectral_k: columns (s, omega, nine terms) as main.f90:780-790 completes them);
+    ! Polarization3rd / Polarization4th restrict the mode range as in kappa_noIter's arguments.
+    subroutine kappa_noIter_gpu(ctx, LW_Allq0, T, Symm, kappa, Mode_k, Spectral_k, Polarization3rd, Polarization4th)
         type(gpu_ctx), intent(in)   :: ctx
         real(dp), intent(in)        :: LW_Allq0(:,:), T
         logical, intent(in)         :: Symm
         real(dp), intent(out)       :: kappa(9), Mode_k(:,:)
+        real(dp), intent(out), optional :: Spectral_k(:,:)
+        integer, intent(in), optional :: Polarization3rd(2, 3), Polarization4th(2, 4)
         integer(c_int)              :: s
         real(dp), allocatable       :: lw(:,:)
         s = 0
         if ( Symm ) s = 1
         lw = LW_Allq0
-        call check(spr_kappa_rta(ctx%h, lw, T, s, kappa, Mode_k), 'kappa_noIter_gpu')
+        if ( present(Polarization3rd) ) call set_rules(ctx, 3, Polarization3rd)
+        if ( present(Polarization4th) ) call set_rules(ctx, 4, Polarization4th)
+        if ( present(Spectral_k) ) then
+            call check(spr_kappa_rta_spectral(ctx%h, lw, T, s, kappa, Mode_k, Spectral_k), 'kappa_noIter_gpu')
+        else
+            call check(spr_kappa_rta(ctx%h, lw, T, s, kappa, Mode_k), 'kappa_noIter_gpu')
+        end if
     end subroutine kappa_noIter_gpu
+
+    ! PolarizationStrtEnd (Helper_m.f90:310-353): 'A'/'a' -> branches 1..3, 'O'/'o' -> 4..Ndof, any
+    ! other letter -> 1..Ndof, per slot; nd_selection is true when some slot is restricted.
+    subroutine PolarizationStrtEnd(order, Ndof, character_arr, out_int_arr, nd_selection)
+        integer, intent(in)             :: order, Ndof
+        character(len=1), intent(in)    :: character_arr(order)
+        integer, intent(out)            :: out_int_arr(2, order)
+        logical, intent(out)            :: nd_selection
+        integer                         :: k
+        nd_selection = .false.
+        do k = 1, order
+            select case (character_arr(k))
+            case ('A', 'a')
+                out_int_arr(:, k) = [1, 3]
+                nd_selection = .true.
+            case ('O', 'o')
+                out_int_arr(:, k) = [4, Ndof]
+                nd_selection = .true.
+            case default
+                out_int_arr(:, k) = [1, Ndof]
+            end select
+        end do
+    end subroutine PolarizationStrtEnd
+
+    subroutine set_rules(ctx, order, pol)
+        type(gpu_ctx), intent(in)   :: ctx
+        integer, intent(in)         :: order, pol(:,:)
+        integer(c_int)              :: p(2 * order)
+        p = int(reshape(pol, [2 * order]), c_int)
+        call check(spr_set_branch_rules(ctx%h, int(order, c_int), p), 'spr_set_branch_rules')
+    end subroutine set_rules
 
     ! ---- iterative BTE (src/Kappa/Src_iterative) on the time-reversal irreducible points ----
 
@@ -474,5 +577,89 @@ contains
             end if
         end do
     end subroutine get_phonon_gpu
+
+    ! LineWidth.x (Linewidth_m.f90:48-417, main.f90:293-379): the three-phonon linewidth lw(Ndof) of
+    ! a q0 off the mesh, q0(3) Cartesian with its phonons (Evec_q0(Ndof,Ndof), omega_q0,
+    ! OnebyOmega_q0 as set_Phon_mp gives them); the q1 are the context's mesh, ZeroCentered picks
+    ! the reference's BZ test and q1 folding; q2 = G - (q0 + q1) as ScatterProbAllq1 finds it,
+    ! with its phonons solved on the device from FC2 (FC2type arrays, as get_phonon_gpu).  Sigma
+    ! present: the Gaussian sum with that width (rad/ps) instead of the tetrahedron method.
+    ! Sctrq1(Ndof,Ndof,Ndof,Nq) optional: the W rows.  FP64.
+    subroutine CalculateLwOffMesh_gpu(ctx, q0, Evec_q0, omega_q0, OnebyOmega_q0, Nbasis, mass, latvec, &
+                                    & atmNum2, atmIndx2, fC2, T, ZeroCentered, lw, Sctrq1, Sigma)
+        type(gpu_ctx), intent(in)                   :: ctx
+        real(dp), intent(in)                        :: q0(3), omega_q0(:), OnebyOmega_q0(:), mass(:), latvec(3,3), &
+                                                     & fC2(:,:,:,:,:), T
+        complex(dp), intent(in)                     :: Evec_q0(:,:)
+        integer, intent(in)                         :: Nbasis, atmNum2(:), atmIndx2(:,:,:)
+        logical, intent(in)                         :: ZeroCentered
+        real(dp), intent(out)                       :: lw(:)
+        real(dp), intent(out), optional, target, contiguous :: Sctrq1(:,:,:,:)
+        real(dp), intent(in), optional              :: Sigma
+        integer(c_int), allocatable, target         :: num2(:), indx2(:,:,:)
+        real(dp), allocatable, target               :: mass_c(:), fc(:,:,:,:)
+        real(c_double)                              :: lat_c(9), sig
+        type(spr_crystal)                           :: cr
+        type(spr_ifc)                               :: f2
+        type(c_ptr)                                 :: pw
+        integer(c_int)                              :: delta
+        num2 = atmNum2
+        indx2 = atmIndx2
+        indx2(4, :, :) = indx2(4, :, :) - 1
+        fc = fC2(1, :, :, :, :)
+        mass_c = mass
+        lat_c = reshape(transpose(latvec), [9])
+        cr = spr_crystal(int(Nbasis, c_int), c_loc(mass_c), 0.0_dp)
+        f2 = spr_ifc(2, int(size(atmIndx2, 2), c_int), c_loc(num2), c_loc(indx2), c_loc(fc))
+        pw = c_null_ptr
+        if ( present(Sctrq1) ) pw = c_loc(Sctrq1)
+        delta = 1
+        sig = 0.0_dp
+        if ( present(Sigma) ) then
+            delta = 0
+            sig = Sigma
+        end if
+        call check(spr_lw3_offmesh(ctx%h, q0, Evec_q0, omega_q0, OnebyOmega_q0, cr, lat_c, f2, T, &
+                 & merge(1_c_int, 0_c_int, ZeroCentered), delta, sig, lw, pw), 'CalculateLwOffMesh_gpu')
+    end subroutine CalculateLwOffMesh_gpu
+
+    ! make_tetrahedron + FindIrrQ (TetrahedronQ_m.f90:56-331, zero-centred, shift 0) and the
+    ! Cartesian rotations of set_SpaceGrSPGlib, from the crystal: latvec(3,3) (vectors as
+    ! columns), basis_frac(3, Nbasis) (atm_pos), typ_indx(Nbasis), qmesh(3), time_rev, symprec
+    ! (Angstrom).  Outputs allocated here, indices 1-based as the reference's: q_pnt_int(3, Nq),
+    ! q_pnt(3, Nq), irr_grid(Nq0) (grid index of each irreducible point), pos(Nq),
+    ! multiplicity(Nq0), Rxyz(3, 3, Nsym), tetrahedrons(4, 6 Nq).
+    subroutine MakeMeshIrrQ_gpu(latvec, basis_frac, typ_indx, qmesh, time_rev, symprec, q_pnt_int, q_pnt, &
+                              & irr_grid, pos, multiplicity, Rxyz, tetrahedrons)
+        real(dp), intent(in)                        :: latvec(3,3), basis_frac(:,:), symprec
+        integer, intent(in)                         :: typ_indx(:), qmesh(3), time_rev
+        integer, allocatable, intent(out)           :: q_pnt_int(:,:), irr_grid(:), pos(:), multiplicity(:), &
+                                                     & tetrahedrons(:,:)
+        real(dp), allocatable, intent(out)          :: q_pnt(:,:), Rxyz(:,:,:)
+        integer(c_int), allocatable, target         :: qi(:,:), ig(:), ps(:), mu(:), tt(:,:), typ(:)
+        real(dp), allocatable, target               :: qc(:,:), rt(:,:,:), frac(:,:)
+        real(c_double)                              :: lat_c(9)
+        integer(c_int)                              :: nq0, nsym, m(3)
+        integer                                     :: nq
+        lat_c = reshape(transpose(latvec), [9])
+        frac = basis_frac
+        typ = typ_indx
+        m = qmesh
+        call check(spr_mesh_build(lat_c, int(size(typ), c_int), frac, typ, m, int(time_rev, c_int), 1_c_int, &
+                 & symprec, nq0, nsym, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                 & c_null_ptr), 'MakeMeshIrrQ_gpu')
+        nq = product(qmesh)
+        allocate( qi(3, nq), qc(3, nq), ig(nq0), ps(nq), mu(nq0), rt(3, 3, nsym), tt(4, 6 * nq) )
+        call check(spr_mesh_build(lat_c, int(size(typ), c_int), frac, typ, m, int(time_rev, c_int), 1_c_int, &
+                 & symprec, nq0, nsym, c_loc(qi), c_loc(qc), c_loc(ig), c_loc(ps), c_loc(mu), c_loc(rt), &
+                 & c_loc(tt)), 'MakeMeshIrrQ_gpu')
+        q_pnt_int = qi
+        q_pnt = qc
+        irr_grid = ig + 1
+        pos = ps + 1
+        multiplicity = mu
+        Rxyz = rt
+        tetrahedrons = tt + 1
+    end subroutine MakeMeshIrrQ_gpu
 
 end module spring_gpu_m

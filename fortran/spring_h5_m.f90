@@ -8,6 +8,7 @@
 !   w_kappa_h5 <- the writes of main.f90:773-798 (w1dh5_no_grp, w2dh5_no_grp_type2,
 !                w2dh5_no_grp): ThermalConductivityTensor, LineWidth, PhaseSpace3rd (optional),
 !                ModeResolvedKappa (optional)
+!   w_spectral_h5 <- w2dh5_no_grp(Spectral_k, kappa_outFile, "SpectralResolvedKappa") (main.f90:792)
 ! atmIndx keeps the files' 1-based basis index, as the reference's readers return it.  Failures
 ! print the message and ERROR STOP, as the reference's HDF5 calls leave a failed read fatal.
 module spring_h5_m
@@ -45,6 +46,13 @@ module spring_h5_m
             type(c_ptr), value      :: p3, mode_k
         end function spr_h5_write_kappa
 
+        integer(c_int) function spr_h5_write_spectral(file, ndof, nq0, spec) bind(C, name="spr_h5_write_spectral")
+            import :: c_int, c_char, c_double
+            character(kind=c_char)  :: file(*)
+            integer(c_int), value   :: ndof, nq0
+            real(c_double)          :: spec(*)
+        end function spr_h5_write_spectral
+
         type(c_ptr) function spr_h5_last_error() bind(C, name="spr_h5_last_error")
             import :: c_ptr
         end function spr_h5_last_error
@@ -55,7 +63,7 @@ module spring_h5_m
         end function c_strlen
     end interface
 
-    public :: r2ndFC_h5, r3rdFC_h5, r4thFC_h5, w_kappa_h5
+    public :: r2ndFC_h5, r3rdFC_h5, r4thFC_h5, w_kappa_h5, w_spectral_h5
 
 contains
 
@@ -142,5 +150,16 @@ contains
         call check(spr_h5_write_kappa(trim(filename)//c_null_char, k, int(size(omega_q0, 1), c_int), &
                  & int(size(omega_q0, 2), c_int), omega_q0, LW_Allq0, pp, pm), 'w_kappa_h5')
     end subroutine w_kappa_h5
+
+    ! SpectralResolvedKappa (11, Ndof Nq0) into the kappa file written by w_kappa_h5
+    subroutine w_spectral_h5(filename, Spectral_k, Ndof, Nq0)
+        character(len=*), intent(in)    :: filename
+        real(dp), intent(in)            :: Spectral_k(:,:)
+        integer, intent(in)             :: Ndof, Nq0
+        real(dp), allocatable           :: a(:,:)
+        a = Spectral_k
+        call check(spr_h5_write_spectral(trim(filename)//c_null_char, int(Ndof, c_int), int(Nq0, c_int), a), &
+                 & 'w_spectral_h5')
+    end subroutine w_spectral_h5
 
 end module spring_h5_m

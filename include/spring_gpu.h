@@ -197,6 +197,24 @@ int spr_get_phonon(const spr_crystal *cr, const double *latvec, const spr_ifc *f
                    const double *q_cart, const int *is_gamma, double T, int device, double *evec,
                    double *omega, double *inv_omega, double *nbe, double *grp_vel);
 
+/* The q mesh of kappa.x from the crystal alone (host code, no device), replacing make_tetrahedron
+ * + FindIrrQ (src/Kappa/Src/TetrahedronQ_m.f90:56-331, zero-centred, shift 0) and the rotations
+ * set_SpaceGrSPGlib takes from spglib: latvec[9] row-major as spr_get_phonon; basis_frac[nb][3]
+ * (atm_pos(3, Nbasis) of &AtmInfo); typ[nb] species (typ_indx); mesh[3] (qmesh); time_rev (0/1)
+ * adds -R to the reciprocal action, as spglib's is_time_reversal; symmetry = 0 uses the identity
+ * only; symprec: distance tolerance (Angstrom) for mapping atoms.  Always sets nq0_out and
+ * nsym_out; when every output pointer is NULL nothing else is written (size query).  Outputs
+ * (any may be NULL): q_int[nq][3] folded into (-N/2, N/2] (q_pnt_int), q_cart[nq][3] 1/A
+ * (q_pnt), irr_grid[nq0] (0-based grid index of each irreducible q, ascending), pos[nq]
+ * (0-based), multiplicity[nq0], rot = Rxyz(3, 3, nsym) Cartesian rotations (column-major; with
+ * time_rev the point group followed by its negatives), tetra[6 nq][4] (tetrahedrons(4, 6 Nq),
+ * 0-based).  The point group is every integer matrix with entries in {-1, 0, 1} that keeps the
+ * metric and maps the basis onto itself species by species up to a common translation; the
+ * irreducible representative is the smallest grid index of each orbit (spglib's choice). */
+int spr_mesh_build(const double *latvec, int nb, const double *basis_frac, const int *typ, const int *mesh,
+                   int time_rev, int symmetry, double symprec, int *nq0_out, int *nsym_out, int *q_int,
+                   double *q_cart, int *irr_grid, int *pos, int *multiplicity, double *rot, int *tetra);
+
 /* Four-phonon Gaussian linewidth sum of irreducible q0 `i0` over all (q1,q2), the processes
  * and weights of TetrahedronIntegration4th (Tetrahedron4th_sep.f90:242-343) with a Gaussian
  * delta; main.f90:660 adds 0.5*lw to the linewidth. */

@@ -50,6 +50,12 @@ int spr_h5_write_kappa(const char *file, const double *kappa, int ndof, int nq0,
 int spr_h5_read_kappa(const char *file, int ndof, int nq0, double *kappa, double *omega_q0, double *lw,
                       double *p3, double *mode_k);
 
+/* Adds SpectralResolvedKappa (11, Ndof Nq0) to an existing kappa file (w2dh5_no_grp as
+ * main.f90:780-794 writes the -Spectral_k output); spectral_k is spr_kappa_rta_spectral's
+ * [Ndof nq0][11].  The read returns it back. */
+int spr_h5_write_spectral(const char *file, int ndof, int nq0, const double *spectral_k);
+int spr_h5_read_spectral(const char *file, int ndof, int nq0, double *spectral_k);
+
 const char *spr_h5_last_error(void);
 
 #ifdef __cplusplus

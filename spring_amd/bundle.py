@@ -75,6 +75,40 @@ def write_bundle(path, model, qm, ph, T: float, sigma: float, prefix: str = "syn
     return path
 
 
+def write_input_nml(path, model, qmesh, T: float, prefix: str = "synth", sigma: float = 5.1,
+                    time_rev: int = 0, symprec: float = 1e-5, rules3: str = "DDD", rules4: str = "DDDD",
+                    four_phonon: bool = False):
+    """A kappa.x input: `input.nml` with &FCInfo, &CrystalInfo, &AtmInfo (cell%init_data,
+    src/Kappa/Src/unit_mod.f90:64-214) and &AnhInfo (ReadNmlKappa, Helper_m.f90:37-132), plus the
+    force constants as FC2nd_/FC3rd_[/FC4th_]<prefix><T>K_F.h5 beside it.  Species become types
+    (typ_indx 1-based, one mass per type).  Reals are written in shortest round-trip form, so the
+    reader gets the same doubles."""
+    from . import h5io
+    os.makedirs(path, exist_ok=True)
+    cr = model.crystal
+    species, typ = np.unique(np.asarray(cr.species), return_inverse=True)
+    mass_typ = [float(np.asarray(cr.mass)[typ == t][0]) for t in range(species.size)]
+    f = lambda xs: ", ".join(repr(float(x)) for x in xs)
+    lines = ["&FCInfo", f"  prefix = '{prefix}'", "  SecondFC = .true.", "  ThirdFC = .true.",
+             f"  FourthFC = {'.true.' if four_phonon else '.false.'}", "  Force_dd = .false.", "/",
+             "&CrystalInfo", f"  Nbasis = {cr.nb}", f"  Ntyp = {species.size}",
+             f"  Latvec = {f(np.asarray(cr.latvec).T.ravel())}", "  Sup_Cell = 5, 5, 5", "/",
+             "&AtmInfo", f"  atm_pos = {f(np.asarray(cr.basis_frac).ravel())}",
+             f"  typ_indx = {', '.join(str(int(t) + 1) for t in typ)}",
+             f"  typ_lbl = {', '.join(repr('X%d' % t) for t in range(species.size))}",
+             f"  mass = {f(mass_typ)}", "/",
+             "&AnhInfo", f"  SelectionRule3 = {', '.join(repr(c) for c in rules3)}",
+             f"  SelectionRule4 = {', '.join(repr(c) for c in rules4)}", f"  Sigma = {float(sigma)!r}",
+             f"  symprec = {float(symprec)!r}", f"  qmesh = {', '.join(str(int(x)) for x in qmesh)}",
+             f"  time_rev = {int(time_rev)}", f"  FourPhonon = {'.true.' if four_phonon else '.false.'}", "/", ""]
+    with open(os.path.join(path, "input.nml"), "w") as fh:
+        fh.write("\n".join(lines))
+    for ifc in (model.fc2, model.fc3, model.fc4 if four_phonon else None):
+        if ifc is not None:
+            h5io.write_fc(os.path.join(path, h5io.fc_file_name(ifc.order, prefix, T)), ifc)
+    return path
+
+
 def read_linewidth_file(fname, ndof):
     """Rows of linewidth_IBZ_*.txt: (omega(ndof), lw(ndof)) per irreducible q0.
 

@@ -218,3 +218,21 @@ extern "C" int spr_h5_read_kappa(const char *file, int ndof, int nq0, double *ka
     if (mode_k && read_ds(f.id, "ModeResolvedKappa", H5T_NATIVE_DOUBLE, {(hsize_t)ndof, 9}, mode_k, file)) return -1;
     return 0;
 }
+
+// SpectralResolvedKappa (11, Ndof Nq0) added to an existing kappa file (w2dh5_no_grp,
+// hdf5_wrap_mod.f90:650-721, as main.f90:780-794 writes it)
+extern "C" int spr_h5_write_spectral(const char *file, int ndof, int nq0, const double *spectral_k) {
+    quiet();
+    if (!file || !spectral_k || ndof < 1 || nq0 < 1) return fail("spr_h5_write_spectral: bad argument");
+    Hid f(H5Fopen(file, H5F_ACC_RDWR, H5P_DEFAULT), H5Fclose);
+    if (!f.ok()) return fail(std::string(file) + ": cannot open for writing (write the kappa file first)");
+    return write_ds(f.id, "SpectralResolvedKappa", H5T_NATIVE_DOUBLE, {(hsize_t)ndof * nq0, 11}, spectral_k);
+}
+
+extern "C" int spr_h5_read_spectral(const char *file, int ndof, int nq0, double *spectral_k) {
+    quiet();
+    if (!file || !spectral_k || ndof < 1 || nq0 < 1) return fail("spr_h5_read_spectral: bad argument");
+    Hid f(H5Fopen(file, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+    if (!f.ok()) return fail(std::string(file) + ": cannot open as HDF5");
+    return read_ds(f.id, "SpectralResolvedKappa", H5T_NATIVE_DOUBLE, {(hsize_t)ndof * nq0, 11}, spectral_k, file);
+}

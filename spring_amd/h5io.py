@@ -18,7 +18,7 @@ from .model import IFC
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libspring_h5.so")
 SYMBOLS = ["spr_h5_fc_dims", "spr_h5_read_fc", "spr_h5_write_fc", "spr_h5_write_kappa", "spr_h5_read_kappa",
-           "spr_h5_last_error"]
+           "spr_h5_write_spectral", "spr_h5_read_spectral", "spr_h5_last_error"]
 
 _lib = None
 
@@ -90,6 +90,20 @@ def write_kappa(path: str, kappa, omega_q0, lw, p3=None, mode_k=None) -> None:
     m = None if mode_k is None else np.ascontiguousarray(mode_k, np.float64)
     _check(load().spr_h5_write_kappa(path.encode(), _p(k), nd, nq0, _p(w), _p(l), None if p is None else _p(p),
                                      None if m is None else _p(m)), "spr_h5_write_kappa")
+
+
+def write_spectral(path: str, spectral) -> None:
+    """Adds SpectralResolvedKappa to an existing kappa file; spectral (Ndof Nq0, 11) as
+    GpuContext.kappa_spectral returns it."""
+    a = np.ascontiguousarray(spectral, np.float64)
+    nq0_nd = a.shape[0]
+    _check(load().spr_h5_write_spectral(path.encode(), nq0_nd, 1, _p(a)), "spr_h5_write_spectral")
+
+
+def read_spectral(path: str, ndof: int, nq0: int):
+    a = np.zeros((ndof * nq0, 11))
+    _check(load().spr_h5_read_spectral(path.encode(), ndof, nq0, _p(a)), "spr_h5_read_spectral")
+    return a
 
 
 def read_kappa(path: str, ndof: int, nq0: int, p3: bool = False, mode_k: bool = False) -> dict:

@@ -24,7 +24,7 @@ SYMBOLS = [
     "spr_lw4_tetra", "spr_lw4_tetra_all", "spr_lw3_kappa_all", "spr_mlp_score",
     "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve", "spr_get_phonon",
     "spr_lw3_slice", "spr_lw3_pairs_noalloc", "spr_set_branch_rules", "spr_kappa_rta_spectral",
-    "spr_lw3_offmesh",
+    "spr_lw3_offmesh", "spr_mesh_build",
 ]
 
 
@@ -368,6 +368,31 @@ def get_phonon(cr, fc2, q_cart, q_int, T: float, device: int = 0):
     if rc != 0:
         raise SprError(f"spr_get_phonon failed ({rc}): {lib.spr_last_error().decode()}")
     return Phonons(evec, omega, inv_omega, nbe, vel, float(T))
+
+
+def mesh_build(cr, mesh, time_rev: int = 0, symmetry: bool = True, symprec: float = 1e-2):
+    """The q mesh from the crystal on the host side of the library (spr_mesh_build): a dict with
+    q_int (nq, 3), q_cart (nq, 3), irr_idx (nq0), pos (nq), multiplicity (nq0), rot (nsym, 3, 3)
+    Cartesian, tetra (6 nq, 4), all 0-based; the same tables as mesh.make_mesh."""
+    lib = load()
+    lat = np.ascontiguousarray(cr.latvec, np.float64)
+    frac = np.ascontiguousarray(cr.basis_frac, np.float64)
+    typ = np.ascontiguousarray(cr.species, np.int32)
+    m = np.ascontiguousarray(mesh, np.int32)
+    nq0, nsym = ct.c_int(), ct.c_int()
+    args = (_ptr(lat), ct.c_int(cr.nb), _ptr(frac), _ptr(typ), _ptr(m), ct.c_int(int(time_rev)),
+            ct.c_int(int(symmetry)), ct.c_double(symprec), ct.byref(nq0), ct.byref(nsym))
+    if lib.spr_mesh_build(*args, *([None] * 7)) != 0:
+        raise SprError(f"spr_mesh_build failed: {lib.spr_last_error().decode()}")
+    nq = int(np.prod(m))
+    out = dict(q_int=np.zeros((nq, 3), np.int32), q_cart=np.zeros((nq, 3)), irr_idx=np.zeros(nq0.value, np.int32),
+               pos=np.zeros(nq, np.int32), multiplicity=np.zeros(nq0.value, np.int32),
+               rotf=np.zeros((nsym.value, 3, 3)), tetra=np.zeros((6 * nq, 4), np.int32))
+    keys = ("q_int", "q_cart", "irr_idx", "pos", "multiplicity", "rotf", "tetra")
+    if lib.spr_mesh_build(*args, *[_ptr(out[k]) for k in keys]) != 0:
+        raise SprError(f"spr_mesh_build failed: {lib.spr_last_error().decode()}")
+    out["rot"] = np.transpose(out.pop("rotf"), (0, 2, 1)).copy()   # Rxyz(3,3,k) -> [k][i][j]
+    return out
 
 
 class Mlp:

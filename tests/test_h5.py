@@ -125,6 +125,13 @@ def test_kappa_file(tmp_path):
     assert vals.size == nq0 * 2 * nd
     rows = vals.reshape(nq0, 2 * nd)                  # h5dump order: q0 outer, then 2 Ndof
     assert np.array_equal(rows[:, :nd], w) and np.array_equal(rows[:, nd:], lw)
+    # -Spectral_k: SpectralResolvedKappa (11, Ndof Nq0) added to the same file (main.f90:780-794)
+    spec = rng.normal(size=(nd * nq0, 11))
+    h5io.write_spectral(p, spec)
+    assert np.array_equal(h5io.read_spectral(p, nd, nq0), spec)
+    head = _dump(p, "-H")
+    assert re.search(r'DATASET "SpectralResolvedKappa" \{\s*DATATYPE\s+H5T_IEEE_F64LE\s*DATASPACE\s+SIMPLE \{ \( %d, 11 \)' % (nd * nq0), head)
+    assert "ThermalConductivityTensor" in head          # the earlier datasets stay
 
 
 def test_errors_are_reported(tmp_path):

@@ -24,6 +24,7 @@
 // reads of one vertex are coalesced; per-workgroup partial sums are reduced in a fixed order
 // by a second kernel (bitwise reproducible).  FP64 throughout.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -491,6 +492,10 @@ int bte_sums(Ctx &c, int mode) {
 // memory budget, and the caller keeps the per-iteration tetrahedron sums.
 int bte_build(Ctx &c) {
     BteState &b = c.bte;
+    // SPR_BTE_NO_MATRIX=1 keeps the per-iteration tetrahedron sums (the path of cells whose
+    // collision matrix does not fit), so that path can be run on any mesh
+    if (const char *e = std::getenv("SPR_BTE_NO_MATRIX"))
+        if (std::atoi(e) != 0) return SPR_E_UNSUPPORTED;
     if (b.a_ready) return SPR_OK;
     const int R = b.nirr * c.nd;
     const size_t lds = (size_t)R * sizeof(double);
@@ -648,6 +653,10 @@ extern "C" int spr_bte_solve(spr_ctx *h, double T, double eps_k, int max_iter, d
     spr::BteState &b = c.bte;
     const int nd = c.nd;
     const size_t nrow = (size_t)b.nirr * nd;
+    if (c.h_vel.empty()) {
+        spr::set_error("spr_bte_solve: the context has no group velocities (grp_vel at spr_ctx_create)");
+        return SPR_E_STATE;
+    }
     // the collision matrix where it fits (F updates are then one matrix-vector product each),
     // else the tetrahedron sums every iteration, as the reference does
     rc = spr::bte_build(c);

@@ -332,6 +332,19 @@ def rocksalt10(seed: int = 11) -> Model:
     return Model(cr, fc2, fc3, None, "rocksalt10")
 
 
+def fcc1(with_fc4: bool = True, seed: int = 31) -> Model:
+    """Monatomic fcc (a = 3.61 A, 63.55 amu, three branches): nearest-neighbour springs 1.2/0.1
+    eV/A^2, FC3 N(0,1) eV/A^3 and FC4 N(0,2) eV/A^4 within the nearest-neighbour shell."""
+    a = 3.61
+    lat = 0.5 * a * np.array([[0.0, 1.0, 1.0], [1.0, 0.0, 1.0], [1.0, 1.0, 0.0]]).T
+    cr = Crystal(lat, np.zeros((1, 3)), np.array([63.55]), np.array([0]), "fcc-1")
+    r1 = a / np.sqrt(2.0)
+    fc2 = spring_fc2(cr, [(r1, 1.2, 0.1)], seed=seed)
+    fc3 = random_anharmonic(cr, 3, r1, 1.0, seed + 1)
+    fc4 = random_anharmonic(cr, 4, r1, 2.0, seed + 2) if with_fc4 else None
+    return Model(cr, fc2, fc3, fc4, "fcc1")
+
+
 def rocksalt_chain(n: int, seed: int = 21) -> Model:
     """2n-atom jittered n x 1 x 1 rocksalt supercell (a = 4.21 A, 24.3/16.0 amu): the
     rocksalt10 spring model with FC3 N(0,0.5) eV/A^3 within the 2nd shell (a smaller cluster

@@ -48,7 +48,42 @@ def test_bte_solve(rocksalt):
     out = ctx.bte_solve(T, eps_k=1e-6, max_iter=100)
     k, k0, it, F = O.bte_solve(qm, ph, vol, T, irr, map_irr, map_sign, deg, Wp, Wm, 1e-6, 100)
     print("bte kappa", np.diag(out["kappa"]), "iterations", out["iterations"], flush=True)
-    assert out["iterations"] == it
+    _check_solve(out, k, k0, it, F)
+
+
+def _check_solve(out, k, k0, it, F):
+    # the collision matrix is summed with LDS atomics, so its last bits (and, when the eps_k test
+    # is met by a hair, the iteration count) can vary between runs; then kappa is the neighbouring
+    # iterate, which differs from the oracle's by less than eps_k
+    assert abs(out["iterations"] - it) <= 1
     assert rel_err(out["kappa0"], k0, 1e-6) < 1e-10
-    assert rel_err(out["kappa"], k, 1e-6) < 1e-9
-    assert rel_err(out["F"], F, 1e-6) < 1e-8
+    if out["iterations"] == it:
+        assert rel_err(out["kappa"], k, 1e-6) < 1e-9
+        assert rel_err(out["F"], F, 1e-6) < 1e-8
+    else:
+        assert np.abs(out["kappa"] - k).max() < 1e-6
+
+
+def test_bte_fallback_path(rocksalt, monkeypatch):
+    """SPR_BTE_NO_MATRIX=1: 1/tau from the tetrahedron sums (k_bte<0>) and the solve with the
+    per-iteration sums and the host F update -- the path of meshes whose collision matrix does
+    not fit -- against the oracle and against the matrix path."""
+    ctx_m, qm, ph, (irr, map_irr, map_sign, deg, Wp, Wm) = _setup(rocksalt, (4, 4, 4))
+    T, vol = 300.0, rocksalt.crystal.vol
+    itau_m = ctx_m.bte_tau_inv()
+    out_m = ctx_m.bte_solve(T, eps_k=1e-6, max_iter=100)
+    ctx_m.close()
+    monkeypatch.setenv("SPR_BTE_NO_MATRIX", "1")
+    from spring_amd.lib import GpuContext
+    ctx = GpuContext(rocksalt, qm, ph)
+    itau = ctx.bte_tau_inv()
+    ref = O.bte_sums(qm, ph, irr, map_irr, map_sign, Wp, Wm)
+    assert rel_err(itau, ref, 1e-9) < 1e-10
+    assert rel_err(itau, itau_m, 1e-9) < 1e-12
+    out = ctx.bte_solve(T, eps_k=1e-6, max_iter=100)
+    k, k0, it, F = O.bte_solve(qm, ph, vol, T, irr, map_irr, map_sign, deg, Wp, Wm, 1e-6, 100)
+    print("fallback kappa", np.diag(out["kappa"]), "iterations", out["iterations"], flush=True)
+    _check_solve(out, k, k0, it, F)
+    if out["iterations"] == out_m["iterations"]:
+        assert rel_err(out["kappa"], out_m["kappa"], 1e-6) < 1e-9
+    ctx.close()

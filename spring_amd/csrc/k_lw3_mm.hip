@@ -69,6 +69,10 @@ struct MmArgs {
     int npmax;                   // most clusters of one basis atom
     int nterm;                   // clusters of all basis atoms
     const double *sel;           // branch selection (NULL: all): (nd) s0 mask, (nd, nd) pair mask [s1][s2]
+    // W output (WOUT instantiation, FP64): w_out[q0 - w_q0_first][q1][s0][s1][s2] of every grid
+    // q1, both orderings of each reduced pair written
+    double *w_out;
+    int w_q0_first, w_nq;
 };
 
 constexpr int pow2ceil(int x) { return x <= 1 ? 1 : x <= 2 ? 2 : x <= 4 ? 4 : 8; }
@@ -344,8 +348,9 @@ struct Frag {
     }
 };
 
-template <int MODE, int NB, int VAR>
+template <int MODE, int NB, int VAR, bool WOUT>
 __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
+    static_assert(!WOUT || MODE == MM_F64, "W output is FP64");
     using Sh = MmShape<MODE, NB, VAR>;
     using L = MmLds<MODE, NB, VAR>;
     using R = typename Sh::R;
@@ -682,7 +687,30 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
         //      elements in element order, so the sums do not depend on the packing.  The delta
         //      arguments are formed in FP64 in every mode (they cancel near resonance); FP32 and
         //      BF16 evaluate the rest in FP32 and sum per pair, then into a compensated sum. ----
-        if constexpr (MODE == MM_F64) {
+        if constexpr (WOUT) {
+            // W of every element, both orderings: W(q1; s0, s1, s2) and W(q2; s0, s2, s1) (the map
+            // q -> -(q0 + q) is an involution, so every grid q1 is written exactly once)
+            if (s0ok) {
+                const double iw0 = sMode[ND + s0];
+                double *wq = A.w_out + (size_t)(qi - A.w_q0_first) * A.w_nq * ND * ND * ND;
+#pragma unroll
+                for (int j = 0; j < Sh::VT; ++j) {
+                    const int mt = mtC0 + MSTEPC * j;
+                    if (mt >= Sh::MT_D) continue;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const double x = qD ? vacc[j][2 + k] : vacc[j][k];
+                        const double y = __shfl_xor(qD ? vacc[j][k] : vacc[j][2 + k], 1, 64);
+                        const int m = mt * 16 + tile_row<true>(g, 2 * qD + k);
+                        const int s1 = m / SB, s2 = h * SB + m % SB;
+                        if (s1 >= ND || s2 >= ND) continue;
+                        const double W = (x * x + y * y) * iw0 * sMode[4 * ND + s1] * sMode[7 * ND + s2];
+                        wq[(((size_t)q1 * ND + s0) * ND + s1) * ND + s2] = W;
+                        if (q1 != q2) wq[(((size_t)q2 * ND + s0) * ND + s2) * ND + s1] = W;
+                    }
+                }
+            }
+        } else if constexpr (MODE == MM_F64) {
             // FP64: every element in registers.  Lanes l and l^1 hold Re and Im of the same
             // (row, s0); lane q = 0 takes rows r = 0, 1 of every tile and lane q = 1 rows 2, 3, each
             // receiving the other part from its partner.  A tile row whose three Gaussians
@@ -822,10 +850,10 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     }
 }
 
-template <int MODE, int NB, int VAR>
+template <int MODE, int NB, int VAR, bool WOUT>
 __global__ __launch_bounds__(512) void k_lw3_mm(MmArgs A) {
     if (!A.ic.counter) {
-        mm_unit<MODE, NB, VAR>(A, blockIdx.x);
+        mm_unit<MODE, NB, VAR, WOUT>(A, blockIdx.x);
         return;
     }
     __shared__ int s_unit;
@@ -835,35 +863,36 @@ __global__ __launch_bounds__(512) void k_lw3_mm(MmArgs A) {
         __syncthreads();
         const int unit = s_unit;
         if (unit >= A.ic.nitems) break;
-        mm_unit<MODE, NB, VAR>(A, unit);
+        mm_unit<MODE, NB, VAR, WOUT>(A, unit);
     }
 }
 
 // ItemLaunch of tile shape VAR, or false when it cannot run here (LDS, registers)
-template <int MODE, int NB, int VAR>
+template <int MODE, int NB, int VAR, bool WOUT>
 static bool mm_launch_of(MmArgs &A, int nitems, ItemLaunch &L) {
     using Sh = MmShape<MODE, NB, VAR>;
     const size_t dyn = MmLds<MODE, NB, VAR>::total(A.npmax, A.nterm, A.nsite, A.L);
     if (dyn > 160 * 1024) return false;
-    const void *fn = (const void *)k_lw3_mm<MODE, NB, VAR>;
+    const void *fn = (const void *)k_lw3_mm<MODE, NB, VAR, WOUT>;
     if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess) return false;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 512, dyn) != hipSuccess || per_cu < 1) return false;
     static const char *names[3] = {"mm_f64", "mm_f32", "mm_bf16"};
-    L = ItemLaunch{fn, &A, sizeof(A), 512, dyn, nitems * Sh::NSPLIT, per_cu, std::string(names[MODE])};
+    L = ItemLaunch{fn, &A, sizeof(A), 512, dyn, nitems * Sh::NSPLIT, per_cu,
+                   std::string(names[MODE]) + (WOUT ? "_w" : "")};
     L.upi = Sh::NSPLIT;
     L.shape = "s2blk" + std::to_string(Sh::SB);
     return true;
 }
 
-template <int MODE, int NB>
+template <int MODE, int NB, bool WOUT = false>
 static int launch_mm_t(Ctx &c, MmArgs A, int nitems, int q0_first, int q0_count, int it0, double *d_lw) {
     ItemLaunch Ls[2];
     int n = 0;
-    if (mm_launch_of<MODE, NB, 0>(A, nitems, Ls[n])) ++n;
+    if (mm_launch_of<MODE, NB, 0, WOUT>(A, nitems, Ls[n])) ++n;
     // the larger s2 block is built for FP64 only: in FP32 / BF16 its accumulators spill
     if constexpr (MODE == MM_F64 && MmShape<MODE, NB, 1>::NTS != MmShape<MODE, NB, 0>::NTS)
-        if (mm_launch_of<MODE, NB, 1>(A, nitems, Ls[n])) ++n;
+        if (mm_launch_of<MODE, NB, 1, WOUT>(A, nitems, Ls[n])) ++n;
     if (n == 0) {
         set_error("three-phonon MFMA kernel: no tile shape fits this device (LDS " +
                   std::to_string(MmLds<MODE, NB, 0>::total(A.npmax, A.nterm, A.nsite, A.L)) + " B)");
@@ -882,6 +911,8 @@ int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, do
     const int nitems = ov ? ov->nitems : c.h_item_first[q0_first + q0_count] - it0;
     const bool f64 = prec == SPR_PREC_FP64;
     MmArgs A;
+    A.w_out = nullptr;
+    A.w_q0_first = A.w_nq = 0;
     A.item_q0 = ov ? ov->item_q0 : c.d_item_q0 + it0;
     A.item_off = ov ? ov->item_off : c.d_item_off + it0;
     A.item_cnt = ov ? ov->item_cnt : c.d_item_cnt + it0;
@@ -927,6 +958,65 @@ int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, do
         default:
             set_error("three-phonon MFMA kernel: Nb " + std::to_string(c.nb) + " not built (2-6, 8, 10)");
             return SPR_E_UNSUPPORTED;
+    }
+}
+
+bool lw3_mm_w_built(int nb) { return nb == 2 || nb == 3 || nb == 4 || nb == 5 || nb == 6 || nb == 8 || nb == 10; }
+
+// W of every grid q1 for irreducible q0 [q_first, q_first + q_count) into
+// d_w[q0 - q_first][q1][s0][s1][s2] (FP64), from the pair-reduced items of the context: the
+// W-output mode of the matrix-core kernel, both orderings written, no underflow skip.
+int launch_lw3_mm_w(Ctx &c, int q_first, int q_count, double *d_w) {
+    if (c.override_items || !lw3_mm_w_built(c.nb)) {
+        set_error("matrix-core W path: Nb in 2-6, 8, 10 and no item override");
+        return SPR_E_UNSUPPORTED;
+    }
+    int rc;
+    if ((rc = ensure_pairs(c))) return rc;
+    const int it0 = c.h_item_first[q_first];
+    const int nitems = c.h_item_first[q_first + q_count] - it0;
+    MmArgs A;
+    A.item_q0 = c.d_item_q0 + it0;
+    A.item_off = c.d_item_off + it0;
+    A.item_cnt = c.d_item_cnt + it0;
+    A.pairs = c.d_pairs;
+    A.q0_grid = c.irr_grid;
+    A.L = c.L;
+    A.m0 = c.mesh[0];
+    A.m1 = c.mesh[1];
+    A.blk_start = c.fc3.blk_start;
+    A.tup_site = c.fc3.tup_site;
+    A.fc = (const void *)c.fc3.fc64;
+    A.site = c.fc3.site;
+    A.nsite = c.fc3.nsite;
+    A.phase = (const void *)c.phase64;
+    A.evec = (const void *)c.evec;
+    A.omega = c.omega;
+    A.inv_omega = c.inv_omega;
+    A.nbe = c.nbe;
+    A.sel = nullptr;   // W output: the tetrahedron sums apply the branch ranges themselves
+    A.sigma = 1.0;     // unused without the delta stage
+    const double pi = 4.0 * std::atan(1.0);
+    A.factor = 1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) * 1000000.0 * pi *
+               0.0625 / (double)c.nq;   // Sctr3THz2 / Nq (constants.f90:59-60)
+    int npmax = 0;
+    const int nb2 = c.nb * c.nb;
+    for (int mu = 0; mu < c.nb; ++mu)
+        npmax = std::max(npmax, c.fc3.h_blk_start[(mu + 1) * nb2] - c.fc3.h_blk_start[mu * nb2]);
+    A.npmax = std::max(npmax, 1);
+    A.nterm = std::max(c.fc3.h_blk_start[c.nb * nb2], 1);
+    A.ic.partial = c.ipartial;
+    A.w_out = d_w;
+    A.w_q0_first = q_first;
+    A.w_nq = c.nq;
+    // the per-unit sums of this launch are zero; they are reduced into a scratch row per q0
+    if ((rc = ensure((void **)&c.wsink, &c.wsink_cap, (size_t)q_count * c.nd * sizeof(double)))) return rc;
+    switch (c.nb) {
+#define SPR_NB(n) \
+    case n: return launch_mm_t<MM_F64, n, true>(c, A, nitems, q_first, q_count, it0, c.wsink);
+        SPR_NB(2) SPR_NB(3) SPR_NB(4) SPR_NB(5) SPR_NB(6) SPR_NB(8) SPR_NB(10)
+#undef SPR_NB
+        default: return SPR_E_UNSUPPORTED;
     }
 }
 

@@ -12,6 +12,8 @@
 // its chunk, tetrahedron fastest across lanes, and reads W and n only when w0(s0) lies inside
 // one of the three processes' vertex-energy ranges.  Per-workgroup partials are reduced in a
 // fixed order by a second kernel (bitwise reproducible).
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "spr_internal.h"
@@ -322,15 +324,21 @@ int tetra_sums_from_w(Ctx &c, const double *d_w, const int *d_partner, const dou
 }
 
 // Tetrahedron linewidths of irreducible q0 [q_first, q_first + q_count) into d_lw[k][nd] and
-// d_p3[k][nd] (device, k = q - q_first).  W of every q1 for up to 16 q0 at a time in c.wout:
-// with the small-cell kernel (pair-reduced, both orderings written) for Nb <= 2, else the
-// generic W path one q0 at a time.  The partner tables of a batch are uploaded once.
+// d_p3[k][nd] (device, k = q - q_first).  W of every q1 for up to 16 q0 at a time (at most
+// 2 GiB) in c.wout, pair-reduced with both orderings written: the small-cell kernel for Nb <= 2,
+// the matrix-core kernel's W mode for the larger cells it is built for; otherwise (or with
+// SPR_KERNEL=generic) the generic W path one q0 at a time.  The partner tables of a batch are
+// uploaded once.
 int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p3) {
     int rc;
     const size_t nw = (size_t)c.nq * c.nd * c.nd * c.nd;
-    const bool small = c.nb <= 2 && !c.override_items;
-    const int bmax = small ? (int)std::max<size_t>(1, std::min<size_t>(16, (size_t(2) << 30) / (nw * sizeof(double))))
-                           : 1;
+    const char *force = getenv("SPR_KERNEL");
+    const bool generic = force && std::string(force) == "generic";
+    const bool small = c.nb <= 2 && !c.override_items && !generic;
+    const bool mmw = !small && !c.override_items && !generic && lw3_mm_w_built(c.nb);
+    const int bmax = small || mmw
+                         ? (int)std::max<size_t>(1, std::min<size_t>(16, (size_t(2) << 30) / (nw * sizeof(double))))
+                         : 1;
     const int B = std::min(bmax, q_count);
     if ((rc = ensure((void **)&c.wout, &c.wout_cap, (size_t)B * nw * sizeof(double)))) return rc;
     if ((rc = ensure_tetra(c))) return rc;
@@ -349,6 +357,8 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
         const int nb = std::min(B, q_first + q_count - b0);
         if (small) {
             if ((rc = launch_lw3_small_w(c, b0, nb, c.wout))) return rc;
+        } else if (mmw) {
+            if ((rc = launch_lw3_mm_w(c, b0, nb, c.wout))) return rc;
         } else if ((rc = launch_lw3(c, b0, 1, 0.1, SPR_PREC_FP64, nullptr, c.wout))) {
             return rc;
         }

@@ -204,6 +204,8 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
 // FP64 W of every grid q1 of irreducible q0 [q_first, q_first + q_count) with the small-cell
 // kernel (Nb <= 2): d_w[q0 - q_first][q1][s0][s1][s2]
 int launch_lw3_small_w(Ctx &c, int q_first, int q_count, double *d_w);
+int launch_lw3_mm_w(Ctx &c, int q_first, int q_count, double *d_w);
+bool lw3_mm_w_built(int nb);
 // device tetrahedra (6 nq, 4) of the context (uploaded on first use)
 int ensure_tetra(Ctx &c);
 // FP64 W4 of irreducible q0 `qi` for q1 in [q1_lo, q1_lo + nq1) and every q2:

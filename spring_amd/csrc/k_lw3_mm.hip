@@ -689,27 +689,56 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
         //      BF16 evaluate the rest in FP32 and sum per pair, then into a compensated sum. ----
         if constexpr (WOUT) {
             // W of every element, both orderings: W(q1; s0, s1, s2) and W(q2; s0, s2, s1) (the map
-            // q -> -(q0 + q) is an involution, so every grid q1 is written exactly once)
-            if (s0ok) {
-                const double iw0 = sMode[ND + s0];
-                double *wq = A.w_out + (size_t)(qi - A.w_q0_first) * A.w_nq * ND * ND * ND;
+            // q -> -(q0 + q) is an involution, so every grid q1 is written exactly once).  W is
+            // staged in the dead LDS prefix as sW[s0][s1][s2 local] for a chunk of S0C s0 at a time,
+            // then written out in runs of SB (first ordering) and Ndof (second) contiguous values.
+            constexpr size_t PRE = L::TU + L::U2 + L::AC + 2 * L::E;
+            constexpr int PER_S0 = ND * SB;
+            constexpr int S0C = (int)(PRE / (PER_S0 * sizeof(double))) < ND ? (int)(PRE / (PER_S0 * sizeof(double))) : ND;
+            static_assert(S0C >= 1, "the W staging chunk must hold one s0");
+            double *sW = reinterpret_cast<double *>(smem);
+            double *wq = A.w_out + (size_t)(qi - A.w_q0_first) * A.w_nq * ND * ND * ND;
+            for (int c0 = 0; c0 < ND; c0 += S0C) {
+                __syncthreads();   // GEMM D (first chunk) or the last write-out is done with the prefix
+                if (s0ok && s0 >= c0 && s0 < c0 + S0C) {
+                    const double iw0 = sMode[ND + s0];
 #pragma unroll
-                for (int j = 0; j < Sh::VT; ++j) {
-                    const int mt = mtC0 + MSTEPC * j;
-                    if (mt >= Sh::MT_D) continue;
+                    for (int j = 0; j < Sh::VT; ++j) {
+                        const int mt = mtC0 + MSTEPC * j;
+                        if (mt >= Sh::MT_D) continue;
 #pragma unroll
-                    for (int k = 0; k < 2; ++k) {
-                        const double x = qD ? vacc[j][2 + k] : vacc[j][k];
-                        const double y = __shfl_xor(qD ? vacc[j][k] : vacc[j][2 + k], 1, 64);
-                        const int m = mt * 16 + tile_row<true>(g, 2 * qD + k);
-                        const int s1 = m / SB, s2 = h * SB + m % SB;
-                        if (s1 >= ND || s2 >= ND) continue;
-                        const double W = (x * x + y * y) * iw0 * sMode[4 * ND + s1] * sMode[7 * ND + s2];
-                        wq[(((size_t)q1 * ND + s0) * ND + s1) * ND + s2] = W;
-                        if (q1 != q2) wq[(((size_t)q2 * ND + s0) * ND + s2) * ND + s1] = W;
+                        for (int k = 0; k < 2; ++k) {
+                            const double x = qD ? vacc[j][2 + k] : vacc[j][k];
+                            const double y = __shfl_xor(qD ? vacc[j][k] : vacc[j][2 + k], 1, 64);
+                            const int m = mt * 16 + tile_row<true>(g, 2 * qD + k);
+                            const int s1 = m / SB, sl = m % SB, s2 = h * SB + sl;
+                            if (s1 < ND && s2 < ND)
+                                sW[((s0 - c0) * ND + s1) * SB + sl] =
+                                    (x * x + y * y) * iw0 * sMode[4 * ND + s1] * sMode[7 * ND + s2];
+                        }
                     }
                 }
+                __syncthreads();
+                const int n0 = (ND - c0 < S0C ? ND - c0 : S0C) * PER_S0;
+                for (int i = tid; i < n0; i += 512) {   // W(q1; s0, s1, s2): runs of SB in s2
+                    const int s0l = i / PER_S0, r = i % PER_S0, s1 = r / SB, sl = r % SB;
+                    if (h * SB + sl < ND)
+                        wq[(((size_t)q1 * ND + c0 + s0l) * ND + s1) * ND + h * SB + sl] = sW[i];
+                }
+                if (q1 != q2)
+                    for (int i = tid; i < n0; i += 512) {   // W(q2; s0, s2, s1): runs of Ndof in s1
+                        const int s0l = i / PER_S0, r = i % PER_S0, sl = r / ND, s1 = r % ND;
+                        if (h * SB + sl < ND)
+                            wq[(((size_t)q2 * ND + c0 + s0l) * ND + h * SB + sl) * ND + s1] =
+                                sW[(s0l * ND + s1) * SB + sl];
+                    }
             }
+            // the staging overwrote zero padding of the GEMM operand buffers (set once per unit):
+            // clear what it used once every thread is done with it
+            __syncthreads();
+            constexpr size_t USED = (size_t)S0C * PER_S0 * sizeof(double);
+            for (int i = tid; i < (int)((USED + 15) / 16); i += 512)
+                reinterpret_cast<uint4 *>(smem)[i] = make_uint4(0u, 0u, 0u, 0u);
         } else if constexpr (MODE == MM_F64) {
             // FP64: every element in registers.  Lanes l and l^1 hold Re and Im of the same
             // (row, s0); lane q = 0 takes rows r = 0, 1 of every tile and lane q = 1 rows 2, 3, each

@@ -154,7 +154,7 @@ __global__ __launch_bounds__(TBLK) void k_lw3_tetra(const double *W, const int *
     }
 }
 
-// Ndof 3 and 6: every s0 in one pass.  A thread owns (s1 s2, tetrahedron), tetrahedron fastest
+// Every s0 in one pass (Ndof 3 to 30).  A thread owns (s1 s2, tetrahedron), tetrahedron fastest
 // across lanes; the vertex energies of the three processes do not depend on s0, so each process
 // whose energy span meets [min w0, max w0] is sorted once (with its occupation factor and W
 // offset as payload) and its six vertex-energy reciprocals are taken once, then every s0 with
@@ -324,10 +324,10 @@ int tetra_sums_from_w(Ctx &c, const double *d_w, const int *d_partner, const dou
 }
 
 // Tetrahedron linewidths of irreducible q0 [q_first, q_first + q_count) into d_lw[k][nd] and
-// d_p3[k][nd] (device, k = q - q_first).  W of every q1 for up to 16 q0 at a time (at most
-// 2 GiB) in c.wout, pair-reduced with both orderings written: the small-cell kernel for Nb <= 2,
-// the matrix-core kernel's W mode for the larger cells it is built for; otherwise (or with
-// SPR_KERNEL=generic) the generic W path one q0 at a time.  The partner tables of a batch are
+// d_p3[k][nd] (device, k = q - q_first).  W of every q1 for up to 16 q0 at a time in c.wout,
+// pair-reduced with both orderings written: the small-cell kernel for Nb <= 2, the matrix-core
+// kernel's W mode for the larger cells it is built for; otherwise (or with SPR_KERNEL=generic)
+// the generic W path one q0 at a time.  The partner tables of a batch are
 // uploaded once.
 int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p3) {
     int rc;
@@ -336,17 +336,24 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
     const bool generic = force && std::string(force) == "generic";
     const bool small = c.nb <= 2 && !c.override_items && !generic;
     const bool mmw = !small && !c.override_items && !generic && lw3_mm_w_built(c.nb);
-    const int bmax = small || mmw
-                         ? (int)std::max<size_t>(1, std::min<size_t>(16, (size_t(2) << 30) / (nw * sizeof(double))))
-                         : 1;
+    // W rows of up to 16 q0 per batch: a batch launch has 16x the work units of a one-q0 launch
+    // (a 10-atom 20^3 q0 alone gives ~250 units, too few for 256 CUs); the buffer takes at most a
+    // quarter of the free device memory, capped at 32 GiB, and at least 2 GiB
+    size_t wcap = size_t(2) << 30;
+    {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) wcap = std::max(wcap, std::min(size_t(32) << 30, fr / 4));
+    }
+    const int bmax = small || mmw ? (int)std::max<size_t>(1, std::min<size_t>(16, wcap / (nw * sizeof(double)))) : 1;
     const int B = std::min(bmax, q_count);
     if ((rc = ensure((void **)&c.wout, &c.wout_cap, (size_t)B * nw * sizeof(double)))) return rc;
     if ((rc = ensure_tetra(c))) return rc;
     if ((rc = ensure((void **)&c.partner, &c.partner_cap, (size_t)B * c.nq * sizeof(int)))) return rc;
     const int ntet = 6 * c.nq;
-    // Ndof 3 and 6: all s0 per pass (k_lw3_tetra_s), one partial per workgroup; otherwise one
+    // the Ndof built for k_lw3_tetra_s: all s0 per pass, one partial per workgroup; otherwise one
     // workgroup per (tetrahedron chunk, s0)
-    const bool all_s0 = c.nd == 3 || c.nd == 6;
+    const bool all_s0 = c.nd == 3 || c.nd == 6 || c.nd == 9 || c.nd == 12 || c.nd == 15 || c.nd == 18 ||
+                        c.nd == 24 || c.nd == 30;
     const int chunk = std::max(1, (ntet + 255) / 256);
     const int nch = all_s0 ? (int)std::min<long>(8L * c.ncu, ((long)ntet * c.nd * c.nd + TBLK - 1) / TBLK)
                            : (ntet + chunk - 1) / chunk;
@@ -383,12 +390,15 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
         if (all_s0) {
             // one launch for the batch's q0 (grid.y), then one reduction launch
             const int *q0g = c.irr_grid + b0;
-            if (c.nd == 6)
-                hipLaunchKernelGGL(k_lw3_tetra_s<6>, dim3(nch, nb), dim3(TBLK), 0, c.stream, c.wout, c.tetra,
-                                   c.partner, c.omega, c.nbe, q0g, c.nq, ntet, rg, c.partial);
-            else
-                hipLaunchKernelGGL(k_lw3_tetra_s<3>, dim3(nch, nb), dim3(TBLK), 0, c.stream, c.wout, c.tetra,
-                                   c.partner, c.omega, c.nbe, q0g, c.nq, ntet, rg, c.partial);
+            switch (c.nd) {
+#define SPR_TS(n)                                                                                                   \
+    case n:                                                                                                         \
+        hipLaunchKernelGGL(k_lw3_tetra_s<n>, dim3(nch, nb), dim3(TBLK), 0, c.stream, c.wout, c.tetra, c.partner, \
+                           c.omega, c.nbe, q0g, c.nq, ntet, rg, c.partial);                                         \
+        break;
+                SPR_TS(3) SPR_TS(6) SPR_TS(9) SPR_TS(12) SPR_TS(15) SPR_TS(18) SPR_TS(24) SPR_TS(30)
+#undef SPR_TS
+            }
             SPR_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_tetra_reduce, dim3(c.nd, nb), dim3(256), 0, c.stream, c.partial, c.nd, nch,
                                d_lw + (size_t)(b0 - q_first) * c.nd, d_p3 + (size_t)(b0 - q_first) * c.nd);

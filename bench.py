@@ -37,14 +37,15 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--workload", default="cell10",
                    choices=["rocksalt16", "cell10", "learned10", "fourph10", "fourph_tetra", "bte16",
-                            "phonons10", "tetra16"],
+                            "phonons10", "tetra16", "tetra10"],
                    help="cell10 (default, the headline line): 10-atom cell, 20^3 mesh with time "
                         "reversal, FP64 vs FP32 vs BF16; rocksalt16: 2-atom cell, 16^3 mesh; learned10: the "
                         "learned importance-sampling proposal on the cell10 workload; fourph10: "
                         "four-phonon linewidths, 2-atom cell with FC4, 10^3 mesh; fourph_tetra: the "
                         "same with the tetrahedron method on a --mesh4 mesh; tetra16: three-phonon "
                         "linewidths with the tetrahedron method (the reference's default delta) "
-                        "on the 2-atom --mesh mesh; bte16: iterative "
+                        "on the 2-atom --mesh mesh; tetra10: the same on the cell10 workload (10-atom "
+                        "cell, --mesh10 mesh with time reversal); bte16: iterative "
                         "BTE of the 2-atom cell on a 16^3 mesh (--mesh); phonons10: harmonic phonons "
                         "(D(q), eigensolve, group velocities) of the 10-atom cell on its 20^3 mesh")
     p.add_argument("--samples", type=int, default=2_000_000, help="learned10: training samples")
@@ -92,6 +93,8 @@ def main():
         return fourph_tetra(args)
     if args.workload == "tetra16":
         return tetra16(args)
+    if args.workload == "tetra10":
+        return tetra16(args, cell10=True)
     if args.workload == "bte16":
         return bte16(args)
     if args.workload == "phonons10":
@@ -615,15 +618,17 @@ def fourph_tetra(args):
     print(json.dumps(out), flush=True)
 
 
-def tetra16(args):
+def tetra16(args, cell10=False):
     """Three-phonon tetrahedron-method linewidths (CalculateLwTetrahedron, the reference's default
-    delta) of every irreducible q0 of the 2-atom model on a --mesh mesh, FP64: per q0, W of every
-    q1 (spr_v3_w path) then the tetrahedron sums; the CPU leg runs the oracle's tetrahedron sum
-    on the same W for a few q0."""
+    delta) of every irreducible q0, FP64: W of every q1 for a batch of q0 (the small-cell kernel's
+    W mode for the 2-atom model, the matrix-core kernel's for the 10-atom cell) then the
+    tetrahedron sums; the CPU leg runs the oracle's tetrahedron sum on the same W for a few q0.
+    cell10: the 10-atom cell on its --mesh10 mesh with time reversal, else the 2-atom model on
+    --mesh."""
     from spring_amd import model, mesh, phonons, lib
-    m = model.rocksalt2()
-    n = args.mesh
-    qm = mesh.make_mesh(m.crystal, (n, n, n))
+    m = model.rocksalt10() if cell10 else model.rocksalt2()
+    n = args.mesh10 if cell10 else args.mesh
+    qm = mesh.make_mesh(m.crystal, (n, n, n), time_rev=1 if cell10 else 0)
     ph = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
     ctx = lib.GpuContext(m, qm, ph)
     nd = m.crystal.ndof
@@ -642,8 +647,10 @@ def tetra16(args):
     if not args.no_cpu:
         from oracle import oracle as O
         inp = O.Inputs(m, qm, ph)
-        k = min(4, qm.nq0)
-        Ws = [(int(qm.irr_idx[i]), O.lw3_gauss(inp, int(qm.irr_idx[i]), 1.0, want_w=True)[1]) for i in range(k)]
+        k = min(1 if cell10 else 4, qm.nq0)
+        # W of the sampled q0: the oracle's own (2-atom), or the device's (10-atom: 8000 q1 x 30^3)
+        Ws = [(int(qm.irr_idx[i]), ctx.v3_w(i) if cell10 else O.lw3_gauss(inp, int(qm.irr_idx[i]), 1.0, want_w=True)[1])
+              for i in range(k)]
         t = time.perf_counter()
         for g, W in Ws:
             O.lw3_tetra(qm, ph, g, W)
@@ -655,8 +662,9 @@ def tetra16(args):
         "metric": "three-phonon tetrahedron units/s (q0, tetrahedron, branch triplet)", "unit": "units/s",
         "value": units / dt, "higher_is_better": True, "steps": steps, "warmup": args.warmup,
         "ms_per_step": dt * 1e3, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"3-phonon tetrahedron linewidths, 2-atom rocksalt, {n}^3 mesh, {qm.nq0} "
-                               f"irreducible q0 x {6 * qm.nq} tetrahedra x {nd}^3 branches", "T_K": args.T},
+        "config": {"workload": f"3-phonon tetrahedron linewidths, {'10-atom cell' if cell10 else '2-atom rocksalt'}, "
+                               f"{n}^3 mesh, {qm.nq0} irreducible q0 x {6 * qm.nq} tetrahedra x {nd}^3 branches",
+                   "T_K": args.T},
         "tetra_kernel_ms_per_q0": kms / max(nl, 1),
         "lw_sum": float(lw.sum()), "p3_sum": float(p3.sum()),
         "cpu_baseline": cpu,

@@ -404,12 +404,14 @@ def cell10(args):
 
 
 def learned10(args):
-    """Learned importance-sampling proposal on the 10-atom cell: the network is trained on
-    args.samples |V3|^2 values produced by the FP64 W path, scores every (q0, q1) on the GPU,
-    and K = budget * Nq sampled q1 per q0 (all branches of each sampled pair, pair terms in
-    args.pair_prec) estimate every linewidth; kappa and the per-mode linewidths are judged against the FP64
-    full sum.  The same sampling with the network-free (G-only) proposal and with uniform
-    sampling is reported beside it."""
+    """Learned importance-sampling proposal on the 10-atom cell.  The pair-level network
+    (features of (q0, q1) -> log of the pair's share of the linewidths) is trained on the pair
+    terms of args.samples / Nq random training q0 (every q1 of each, computed by the method's
+    FP32 pair path), scores every (q0, q1) on the GPU, and K = budget * Nq sampled q1 per q0 (all
+    branches of each sampled pair, pair terms in args.pair_prec) estimate every linewidth; kappa
+    and the per-mode linewidths are judged against the FP64 full sum.  The G-only proposal (the
+    same pair weights without the network) and uniform sampling are reported beside it, and each
+    proposal's kappa error is swept over budgets to find the smallest that reaches 1e-3."""
     from spring_amd import model, mesh, phonons, lib
     from spring_amd import learned as Lrn
     n = args.mesh10
@@ -427,33 +429,33 @@ def learned10(args):
     t_full = time.perf_counter() - t
     log(f"learned10: FP64 full sum {t_full:.2f} s, nq0 {qm.nq0}")
     wmax = float(np.abs(ph.omega).max())
-    per_pair = 200
+    n_train = max(1, min(qm.nq0 // 2, int(round(args.samples / qm.nq))))
+    q0_train = np.random.default_rng(0).permutation(qm.nq0)[:n_train]
     t = time.perf_counter()
-    X, y = Lrn.training_set(ctx, qm, ph, n_pairs=max(1, args.samples // per_pair), per_pair=per_pair, seed=0)
+    X, y = Lrn.pair_training_set(ctx, qm, ph, q0_train, args.sigma, prec="fp32")
     t_set = time.perf_counter() - t
     t = time.perf_counter()
-    prop = Lrn.Proposal.train(X, y, wmax=wmax, epochs=8, batch=4096, lr=2e-3)
+    prop = Lrn.PairProposal.train(X, y, wmax, epochs=10)
     t_train = time.perf_counter() - t
-    log(f"learned10: {y.size} samples in {t_set:.1f} s, trained in {t_train:.1f} s, loss "
-        f"{prop.losses[0]:.3f} -> {prop.losses[-1]:.3f}")
+    log(f"learned10: {y.size} pair samples from {n_train} q0 in {t_set:.1f} s, trained in {t_train:.1f} s, "
+        f"loss {prop.losses[0]:.3f} -> {prop.losses[-1]:.3f}")
     qis = np.arange(qm.nq0)
-    K = max(1, int(round(args.budget * qm.nq)))
     sig = np.abs(lw_full) > 1e-6 * np.abs(lw_full).max()
-    rng = np.random.default_rng(1)
-    res = {}
     scores = {}
-    for name in ("learned", "physics", "uniform", "learned+head", "physics+head"):
-        base = name.split("+")[0]
+    for name in ("learned", "g_only", "uniform"):
         t = time.perf_counter()
-        if base in scores:
-            S, n_eval, t_score = scores[base]
-        elif base == "uniform":
-            S, n_eval = np.ones((qm.nq0, qm.nq)), 0
+        if name == "uniform":
+            S = np.ones((qm.nq0, qm.nq))
         else:
-            S, n_eval = Lrn.device_scores(prop if base == "learned" else None, ctx, qis, args.sigma)
-        if base not in scores:
-            t_score = time.perf_counter() - t
-            scores[base] = (S, n_eval, t_score)
+            S = Lrn.pair_proposal_scores(prop if name == "learned" else None, ctx, qis, args.sigma, wmax)
+        scores[name] = (S, time.perf_counter() - t)
+
+    def estimate(name, budget, seed):
+        # "+head": the highest-scored half of the budget is evaluated exactly, the rest sampled
+        # from the remaining q1 (learned.head_tail_draw)
+        S, t_score = scores[name.split("+")[0]]
+        K = max(1, int(round(budget * qm.nq)))
+        rng = np.random.default_rng(seed)
         t = time.perf_counter()
         q0s, q1s, ws = [], [], []
         for qi in range(qm.nq0):
@@ -470,32 +472,46 @@ def learned10(args):
         t_draw = time.perf_counter() - t
         t = time.perf_counter()
         est = np.zeros((qm.nq0, nd))
-        chunk = 200_000
+        chunk = 400_000
         for lo in range(0, q0s.size, chunk):
             tt = ctx.lw3_pairs(q0s[lo:lo + chunk], q1s[lo:lo + chunk], args.sigma, args.pair_prec)
             np.add.at(est, q0s[lo:lo + chunk], tt * ws[lo:lo + chunk, None])
         k_est, _ = ctx.kappa(est, args.T, symm=True)
         t_pairs = time.perf_counter() - t
         err = np.abs(est - lw_full)[sig] / np.abs(lw_full)[sig]
-        res[name] = {
-            "kappa_rel_err_vs_fp64": float(np.max(np.abs(k_est - k_full)) / np.max(np.abs(k_full))),
-            "linewidth_rel_err_median": float(np.median(err)), "linewidth_rel_err_mean": float(err.mean()),
-            "linewidth_rel_err_p90": float(np.quantile(err, 0.9)),
-            "score_s": t_score, "network_evals": int(n_eval), "draw_s": t_draw, "sampled_pairs_s": t_pairs,
-            "total_s": t_score + t_draw + t_pairs, "speedup_vs_fp64_full": t_full / (t_score + t_draw + t_pairs),
-        }
-        log(f"learned10 {name}: {res[name]}")
+        return {"kappa_rel_err_vs_fp64": float(np.max(np.abs(k_est - k_full)) / np.max(np.abs(k_full))),
+                "linewidth_rel_err_median": float(np.median(err)), "linewidth_rel_err_mean": float(err.mean()),
+                "linewidth_rel_err_p90": float(np.quantile(err, 0.9)), "score_s": t_score, "draw_s": t_draw,
+                "sampled_pairs_s": t_pairs, "total_s": t_score + t_draw + t_pairs,
+                "speedup_vs_fp64_full": t_full / (t_score + t_draw + t_pairs)}
+
+    res, sweep = {}, {}
+    budgets = [0.01, 0.02, 0.05, 0.1, 0.2, 0.4]
+    for name in ("learned", "learned+head", "g_only", "g_only+head", "uniform"):
+        res[name] = estimate(name, args.budget, 1)
+        log(f"learned10 {name} at {args.budget:.0%}: {res[name]}")
+        sweep[name] = {}
+        reach = None
+        for b in budgets:
+            e = estimate(name, b, 2)
+            sweep[name][f"{b:g}"] = {"kappa_rel_err": e["kappa_rel_err_vs_fp64"], "total_s": e["total_s"],
+                                     "linewidth_rel_err_median": e["linewidth_rel_err_median"]}
+            if reach is None and e["kappa_rel_err_vs_fp64"] <= 1e-3:
+                reach = b
+        res[name]["budget_reaching_kappa_1e-3"] = reach
+        log(f"learned10 {name} sweep: {sweep[name]}")
     out = {
         "metric": "learned importance sampling of the three-phonon linewidth sum: kappa and linewidth "
                   "error vs the FP64 full sum, time vs the FP64 GPU path",
         "config": {"workload": f"10-atom cell, {n}^3 mesh with time reversal, {qm.nq0} irreducible q0, "
-                               f"{K} sampled q1 per q0 ({args.budget:.0%} of Nq), all branches per pair, "
-                               f"{args.pair_prec} pair terms",
+                               f"{int(round(args.budget * qm.nq))} sampled q1 per q0 ({args.budget:.0%} of Nq), "
+                               f"all branches per pair, {args.pair_prec} pair terms",
                    "sigma_rad_per_ps": args.sigma, "T_K": args.T, "training_samples": int(y.size),
-                   "network": "12 -> 128 -> 128 -> 1 tanh, FP32 Adam training, BF16 MFMA scoring"},
+                   "training_q0": int(n_train),
+                   "network": "12 pair features -> 128 -> 128 -> 1 tanh, FP32 Adam training, BF16 MFMA scoring"},
         "fp64_full_s": t_full, "phonons_s": t_ph, "training_set_s": t_set, "training_s": t_train,
         "training_loss": [float(prop.losses[0]), float(prop.losses[-1])],
-        "proposals": res,
+        "proposals": res, "budget_sweep": sweep,
     }
     print(json.dumps(out), flush=True)
 

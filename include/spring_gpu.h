@@ -300,6 +300,19 @@ int spr_mlp_predict(spr_mlp *m, const float *X, long n, int prec, float *out);
 int spr_mlp_score(spr_ctx *ctx, spr_mlp *m, int n, const int *q0_index, double sigma, double window,
                   float y_std, float wmax, double *S_out, long long *n_eval);
 
+/* Pair-level proposal scores (the proposal draws q1 for every branch of a q0 at once).  For
+ * each irreducible q0 q0_index(k), k < n, and every q1 (q2 = -(q0 + q1)): a scan of the branch
+ * triplets within window*sigma of a resonance gives per s0 (skipping s0 with 1/w0 = 0)
+ * g(s0) = sum |G| / (w1 w2) and its w1-, w2- and acoustic-weighted forms; with Gtot(s0) = sum over
+ * q1 of g(s0), the pair's resonance weight is S = sum_s0 g(s0) / Gtot(s0).  X_out (optional,
+ * [n][nq][12] float) receives the pair's feature row: reduced q0, q1; log(S Nq / ns0); the
+ * S-weighted means of w1, w2, w0 over wmax; the acoustic share; log(1 + triplet count) /
+ * log(Ndof^3).  S_out [n][nq]: S when m == NULL, else the network's (normalised) output for the
+ * row, BF16 matrix cores.  The network must live on the context's device.  No counterpart in the
+ * reference. */
+int spr_pair_scores(spr_ctx *ctx, spr_mlp *m, int n, const int *q0_index, double sigma, double window,
+                    float wmax, double *S_out, float *X_out);
+
 /* Timing of the last launch of the dominant kernel (HIP events on the context's stream):
  * number of launches and summed milliseconds since the last reset. */
 int spr_kernel_time(spr_ctx *ctx, int *launches, double *ms);

@@ -870,3 +870,279 @@ extern "C" int spr_mlp_score(spr_ctx *ctx, spr_mlp *m, int n, const int *q0_inde
     cleanup();
     return rc;
 }
+
+// ---- pair-level proposal: features of (q0, q1) and the network's score of the pair ---------
+// The proposal draws q1 for all branches of a q0 at once, so it scores pairs, not triplets.  For
+// each (q0, q1), q2 = -(q0 + q1), a scan over the branch triplets within window sigma of a
+// resonance gives, per s0 (skipping s0 with 1/w0 = 0),
+//   g(s0) = sum |G| / (w1 w2),  h1(s0), h2(s0) = the same weighted by w1 / w2,
+//   a(s0) = the same restricted to s1 < 3 or s2 < 3 (acoustic branches), and the triplet count;
+// with Gtot(s0) = sum over q1 of g(s0) the pair's resonance weight is S = sum_s0 g / Gtot (the
+// proposal without a network; sum over q1 of S = number of s0 with Gtot > 0).  The network maps
+// the pair's feature row (reduced q0, q1; log(S Nq / ns0); the S-weighted means of w1, w2, w0 over
+// wmax; the acoustic share; log(1 + count) / log(Ndof^3)) to the log of the pair's measured share
+// of the linewidths (sum_s0 |t(q1, s0)| / sum_q1 |t(q1, s0)|, normalised the same way).
+namespace spr {
+
+constexpr int PF_NF = 12;
+
+struct PairScanArgs {
+    const int *q0_grid;   // the batch's q0 (grid indices)
+    int nq, m0, m1, m2;
+    const double *omega, *nbe, *inv_omega;
+    double sigma, window;
+    float *acc;           // [batch][nq][Ndof][4]: g, h1, h2, a
+    int *cnt;             // [batch][nq]
+};
+
+// one workgroup per (q1, q0 of the batch); thread (s0 = tid % 32, slice = tid / 32) scans the
+// (s1, s2) combinations slice, slice + 8, ... in registers; slices are summed in a fixed order
+template <int ND>
+__global__ __launch_bounds__(256) void k_pair_scan(PairScanArgs A) {
+    const int q1 = blockIdx.x, kb = blockIdx.y, tid = threadIdx.x;
+    const int i0 = A.q0_grid[kb];
+    const int k00 = i0 % A.m0, k01 = (i0 / A.m0) % A.m1, k02 = i0 / (A.m0 * A.m1);
+    const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
+    const int a0 = ((-(k00 + k10)) % A.m0 + A.m0) % A.m0, a1 = ((-(k01 + k11)) % A.m1 + A.m1) % A.m1,
+              a2 = ((-(k02 + k12)) % A.m2 + A.m2) % A.m2;
+    const int q2 = a0 + A.m0 * (a1 + A.m1 * a2);
+    __shared__ double sm[8][32];   // w0 iw0 | w1 n1 iw1 | w2 n2 iw2
+    __shared__ float red[8][32][4];
+    __shared__ int redc[256];
+    if (tid < ND) {
+        sm[0][tid] = A.omega[(size_t)i0 * ND + tid];
+        sm[1][tid] = A.inv_omega[(size_t)i0 * ND + tid];
+        sm[2][tid] = A.omega[(size_t)q1 * ND + tid];
+        sm[3][tid] = A.nbe[(size_t)q1 * ND + tid];
+        sm[4][tid] = A.inv_omega[(size_t)q1 * ND + tid];
+        sm[5][tid] = A.omega[(size_t)q2 * ND + tid];
+        sm[6][tid] = A.nbe[(size_t)q2 * ND + tid];
+        sm[7][tid] = A.inv_omega[(size_t)q2 * ND + tid];
+    }
+    __syncthreads();
+    const int s0 = tid & 31, sl = tid >> 5;
+    const double c0 = 1.0 / (A.sigma * 2.5066282746310002), hinv = 0.5 / (A.sigma * A.sigma);
+    const double win = A.window * A.sigma;
+    float g = 0.f, h1 = 0.f, h2 = 0.f, ac = 0.f;
+    int c = 0;
+    if (s0 < ND && sm[1][s0] != 0.0) {
+        const double w0 = sm[0][s0];
+        for (int p = sl; p < ND * ND; p += 8) {
+            const int s1 = p / ND, s2 = p - s1 * ND;
+            const double w1 = sm[2][s1], w2 = sm[5][s2];
+            const double d1 = w0 - w1 - w2, d2 = w0 + w1 - w2, d3 = w0 - w1 + w2;
+            if (fmin(fabs(d1), fmin(fabs(d2), fabs(d3))) >= win) continue;
+            const double n1 = sm[3][s1], n2 = sm[6][s2];
+            const double G = (n1 + n2 + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
+                             (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
+            const float wgt = (float)(fabs(G) * sm[4][s1] * sm[7][s2]);
+            g += wgt;
+            h1 += wgt * (float)w1;
+            h2 += wgt * (float)w2;
+            if (s1 < 3 || s2 < 3) ac += wgt;
+            ++c;
+        }
+    }
+    red[sl][s0][0] = g;
+    red[sl][s0][1] = h1;
+    red[sl][s0][2] = h2;
+    red[sl][s0][3] = ac;
+    redc[tid] = c;
+    __syncthreads();
+    if (tid < ND * 4) {
+        const int s = tid >> 2, k = tid & 3;
+        float t = 0.f;
+        for (int j = 0; j < 8; ++j) t += red[j][s][k];
+        A.acc[(((size_t)kb * A.nq + q1) * ND + s) * 4 + k] = t;
+    }
+    if (tid == 0) {
+        int t = 0;
+        for (int j = 0; j < 256; ++j) t += redc[j];
+        A.cnt[(size_t)kb * A.nq + q1] = t;
+    }
+}
+
+// Gtot(kb, s0) = sum over q1 of g (strided per-thread sums, fixed tree), one workgroup per q0
+__global__ __launch_bounds__(256) void k_pair_gtot(const float *acc, int nq, int nd, double *gtot) {
+    const int kb = blockIdx.x, tid = threadIdx.x;
+    __shared__ double r[256];
+    for (int s = 0; s < nd; ++s) {
+        double t = 0.0;
+        for (int q = tid; q < nq; q += 256) t += acc[(((size_t)kb * nq + q) * nd + s) * 4];
+        r[tid] = t;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) r[tid] += r[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) gtot[(size_t)kb * nd + s] = r[0];
+        __syncthreads();
+    }
+}
+
+// feature rows X[kb][q1][12] and the resonance weight S[kb][q1]
+__global__ __launch_bounds__(256) void k_pair_feat(const float *acc, const int *cnt, const double *gtot,
+                                                   const int *q0_grid, const double *omega, int nq, int nd, int m0,
+                                                   int m1, int m2, float inv_wmax, float *X, double *S) {
+    const int kb = blockIdx.y, q1 = blockIdx.x * 256 + threadIdx.x;
+    if (q1 >= nq) return;
+    const int i0 = q0_grid[kb];
+    double s = 0.0, e1 = 0.0, e2 = 0.0, e0 = 0.0, ea = 0.0;
+    int nact = 0;
+    for (int s0 = 0; s0 < nd; ++s0) {
+        const double gt = gtot[(size_t)kb * nd + s0];
+        if (!(gt > 0.0)) continue;
+        ++nact;
+        const float *a = acc + (((size_t)kb * nq + q1) * nd + s0) * 4;
+        s += a[0] / gt;
+        e1 += a[1] / gt;
+        e2 += a[2] / gt;
+        e0 += a[0] / gt * omega[(size_t)i0 * nd + s0];
+        ea += a[3] / gt;
+    }
+    auto fold = [](int k, int n) { return (float)(k > n / 2 ? k - n : k) / (float)n; };
+    float *x = X + ((size_t)kb * nq + q1) * PF_NF;
+    x[0] = fold(i0 % m0, m0);
+    x[1] = fold((i0 / m0) % m1, m1);
+    x[2] = fold(i0 / (m0 * m1), m2);
+    x[3] = fold(q1 % m0, m0);
+    x[4] = fold((q1 / m0) % m1, m1);
+    x[5] = fold(q1 / (m0 * m1), m2);
+    x[6] = (float)log(fmax(s * nq / (nact > 0 ? nact : 1), 1e-12));
+    const bool any = s > 0.0;
+    x[7] = any ? (float)(e1 / s) * inv_wmax : 0.f;
+    x[8] = any ? (float)(e2 / s) * inv_wmax : 0.f;
+    x[9] = any ? (float)(e0 / s) * inv_wmax : 0.f;
+    x[10] = any ? (float)(ea / s) : 0.f;
+    x[11] = (float)(log1p((double)cnt[(size_t)kb * nq + q1]) / log((double)nd * nd * nd));
+    S[(size_t)kb * nq + q1] = s;
+}
+
+}  // namespace spr
+
+extern "C" int spr_pair_scores(spr_ctx *ctx, spr_mlp *m, int n, const int *q0_index, double sigma, double window,
+                               float wmax, double *S_out, float *X_out) {
+    if (!ctx || n < 1 || !q0_index || !S_out || !(sigma > 0.0) || !(window > 0.0) || !(wmax > 0.f)) {
+        set_error("spr_pair_scores: bad argument");
+        return SPR_E_ARG;
+    }
+    if (m && m->F != PF_NF) {
+        set_error("spr_pair_scores: the network must take the 12 pair features");
+        return SPR_E_ARG;
+    }
+    Ctx &c = ctx->c;
+    const int nd = c.nd, nq = c.nq;
+    const void *scan = nullptr;
+    switch (nd) {
+        case 3: scan = (const void *)k_pair_scan<3>; break;
+        case 6: scan = (const void *)k_pair_scan<6>; break;
+        case 9: scan = (const void *)k_pair_scan<9>; break;
+        case 12: scan = (const void *)k_pair_scan<12>; break;
+        case 15: scan = (const void *)k_pair_scan<15>; break;
+        case 18: scan = (const void *)k_pair_scan<18>; break;
+        case 24: scan = (const void *)k_pair_scan<24>; break;
+        case 30: scan = (const void *)k_pair_scan<30>; break;
+        default:
+            set_error("spr_pair_scores: Ndof " + std::to_string(nd) + " not built (3-18, 24, 30)");
+            return SPR_E_UNSUPPORTED;
+    }
+    std::vector<int> grid(n);
+    for (int k = 0; k < n; ++k) {
+        if (q0_index[k] < 0 || q0_index[k] >= c.nq0) {
+            set_error("spr_pair_scores: q0 index out of range");
+            return SPR_E_ARG;
+        }
+        grid[k] = c.h_irr_grid[q0_index[k]];
+    }
+    SPR_HIP(hipSetDevice(c.device));
+    // q0 in batches of B: per batch acc (B nq Ndof 4 floats), counts, Gtot, X, S, predictions
+    const int B = (int)std::max<size_t>(1, std::min<size_t>((size_t)n, ((size_t)1 << 30) / ((size_t)nq * (nd * 16 + 64))));
+    float *d_acc = nullptr, *d_X = nullptr, *d_pred = nullptr;
+    int *d_cnt = nullptr, *d_grid = nullptr;
+    double *d_gt = nullptr, *d_S = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(d_acc);
+        (void)hipFree(d_X);
+        (void)hipFree(d_pred);
+        (void)hipFree(d_cnt);
+        (void)hipFree(d_grid);
+        (void)hipFree(d_gt);
+        (void)hipFree(d_S);
+    };
+    const size_t rows = (size_t)B * nq;
+    if (hipMalloc((void **)&d_acc, rows * nd * 4 * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&d_X, rows * PF_NF * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&d_pred, rows * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&d_cnt, rows * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&d_grid, (size_t)n * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&d_gt, (size_t)B * nd * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&d_S, rows * sizeof(double)) != hipSuccess) {
+        cleanup();
+        set_error("spr_pair_scores: hipMalloc failed");
+        return SPR_E_OOM;
+    }
+    int rc = SPR_OK;
+    std::vector<float> pred;
+    do {
+        if (hipMemcpy(d_grid, grid.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("spr_pair_scores: upload failed");
+            rc = SPR_E_HIP;
+            break;
+        }
+        for (int b0 = 0; b0 < n && rc == SPR_OK; b0 += B) {
+            const int nb = std::min(B, n - b0);
+            PairScanArgs A;
+            A.q0_grid = d_grid + b0;
+            A.nq = nq;
+            A.m0 = c.mesh[0];
+            A.m1 = c.mesh[1];
+            A.m2 = c.mesh[2];
+            A.omega = c.omega;
+            A.nbe = c.nbe;
+            A.inv_omega = c.inv_omega;
+            A.sigma = sigma;
+            A.window = window;
+            A.acc = d_acc;
+            A.cnt = d_cnt;
+            void *args[] = {&A};
+            if (hipLaunchKernel(scan, dim3(nq, nb), dim3(256), args, 0, c.stream) != hipSuccess) {
+                set_error("spr_pair_scores: scan launch failed");
+                rc = SPR_E_HIP;
+                break;
+            }
+            hipLaunchKernelGGL(k_pair_gtot, dim3(nb), dim3(256), 0, c.stream, d_acc, nq, nd, d_gt);
+            hipLaunchKernelGGL(k_pair_feat, dim3((nq + 255) / 256, nb), dim3(256), 0, c.stream, d_acc, d_cnt, d_gt,
+                               d_grid + b0, c.omega, nq, nd, c.mesh[0], c.mesh[1], c.mesh[2], 1.f / wmax, d_X, d_S);
+            if (m) {
+                const size_t lds = (size_t)MH * MH * 2 + 64 * MH * 2 + 64 * 8 * sizeof(float);
+                SPR_HIP(hipFuncSetAttribute((const void *)k_mlp_infer_bf16,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(k_mlp_infer_bf16, dim3((unsigned)((size_t)nb * nq + 63) / 64), dim3(256), lds,
+                                   c.stream, m->param, PF_NF, d_X, nb * nq, d_pred);
+            }
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c.stream) != hipSuccess) {
+                set_error("spr_pair_scores: kernel failed");
+                rc = SPR_E_HIP;
+                break;
+            }
+            const size_t nrow = (size_t)nb * nq;
+            bool ok = true;
+            if (m) {
+                pred.resize(nrow);
+                ok = hipMemcpy(pred.data(), d_pred, nrow * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
+                for (size_t i = 0; ok && i < nrow; ++i) S_out[(size_t)b0 * nq + i] = pred[i];
+            } else {
+                ok = hipMemcpy(S_out + (size_t)b0 * nq, d_S, nrow * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
+            }
+            if (ok && X_out)
+                ok = hipMemcpy(X_out + (size_t)b0 * nq * PF_NF, d_X, nrow * PF_NF * sizeof(float),
+                               hipMemcpyDeviceToHost) == hipSuccess;
+            if (!ok) {
+                set_error("spr_pair_scores: download failed");
+                rc = SPR_E_HIP;
+            }
+        }
+    } while (false);
+    cleanup();
+    return rc;
+}

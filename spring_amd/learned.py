@@ -13,6 +13,14 @@ x = (q0, q1, w0, w1, w2, s0, s1, s2), trained only on W values the method itself
 (spr_v3_w_pairs), and G is known exactly.  The uniform floor lam keeps the weights bounded.
 `physics_only=True` replaces exp(f) by 1 (the proposal without the network), so the
 network's own gain can be measured against it and against uniform sampling.
+
+The pair-level proposal (PairProposal) scores what the draw selects: a (q0, q1) pair carries
+every branch.  Its target is the pair's measured share of the linewidths,
+T(q1) = sum_s0 |t(q1, s0)| / sum_q1' |t(q1', s0)| (the s0 with a nonzero linewidth), learned from
+the pair terms the method computes for a set of training q0; its inputs are the pair's
+resonance features (spr_pair_scores: the G-only weight S = sum_s0 g(q1, s0) / sum_q1 g(q1, s0)
+with g = sum |G| / (w1 w2) over the resonant triplets, and S-weighted frequency moments).  The
+network-free proposal of the same form uses S itself.
 """
 from __future__ import annotations
 
@@ -178,3 +186,71 @@ def sample_estimate(ctx, qi, p, K, sigma, prec, rng):
     q1 = rng.choice(p.size, size=K, replace=True, p=p)
     t = ctx.lw3_pairs(np.full(K, qi), q1, sigma, prec)
     return (t / p[q1][:, None]).mean(axis=0)
+
+
+PAIR_NF = 12
+
+
+def pair_features(ctx, q0_index, sigma, wmax, window=3.0):
+    """(S, X): the G-only pair weights (n, nq) and the pair feature rows (n, nq, 12)."""
+    from . import lib
+    return lib.pair_scores(ctx, None, q0_index, sigma, window, wmax, want_x=True)
+
+
+def pair_targets(t):
+    """log of each q1's share of the linewidths of one q0, normalised like the features:
+    t (nq, Ndof) pair terms -> log(T Nq / n_s0), T = sum_s0 |t| / sum_q1 |t| over s0 with a
+    nonzero sum; floored at 1e-12."""
+    a = np.abs(t)
+    L = a.sum(axis=0)
+    act = L > 1e-12 * max(L.max(), 1e-300)
+    if not act.any():
+        return np.full(t.shape[0], np.log(1e-12), np.float32)
+    T = (a[:, act] / L[act]).sum(axis=1)
+    return np.log(np.maximum(T * t.shape[0] / act.sum(), 1e-12)).astype(np.float32)
+
+
+def pair_training_set(ctx, qm, ph, q0_train, sigma, prec="fp32", window=3.0, chunk=64):
+    """(X, y) over every q1 of the training q0: X the pair features, y = pair_targets of the
+    pair terms the method computes (spr_lw3_pairs in `prec`)."""
+    wmax = float(np.abs(ph.omega).max())
+    Xs, ys = [], []
+    q0_train = np.asarray(q0_train)
+    for lo in range(0, q0_train.size, chunk):
+        qs = q0_train[lo:lo + chunk]
+        _, X = pair_features(ctx, qs, sigma, wmax, window)
+        q0s = np.repeat(qs, qm.nq)
+        q1s = np.tile(np.arange(qm.nq), qs.size)
+        t = ctx.lw3_pairs(q0s, q1s, sigma, prec).reshape(qs.size, qm.nq, -1)
+        for k in range(qs.size):
+            Xs.append(X[k])
+            ys.append(pair_targets(t[k]))
+    return np.concatenate(Xs), np.concatenate(ys)
+
+
+class PairProposal:
+    """Pair-level network: features of (q0, q1) -> log of the pair's share of the linewidths."""
+
+    def __init__(self, mlp, y_mean, y_std, wmax):
+        self.mlp, self.mu, self.sd, self.wmax = mlp, y_mean, y_std, wmax
+
+    @classmethod
+    def train(cls, X, y, wmax, epochs=10, batch=4096, lr=2e-3, seed=0):
+        mu, sd = float(y.mean()), float(y.std() + 1e-12)
+        mlp = Mlp(PAIR_NF, seed=seed)
+        rng = np.random.default_rng(seed)
+        perm = rng.permutation(y.size)
+        losses = mlp.train(X[perm], ((y - mu) / sd)[perm], epochs=epochs, batch=batch, lr=lr)
+        p = cls(mlp, mu, sd, wmax)
+        p.losses = losses
+        return p
+
+
+def pair_proposal_scores(prop, ctx, q0_index, sigma, wmax, window=3.0):
+    """Scores S(q1) (n, nq) of the pair-level proposal: exp of the network's log-share prediction
+    (prop given) or the G-only weight S (prop None); both computed on the device."""
+    from . import lib
+    if prop is None:
+        return lib.pair_scores(ctx, None, q0_index, sigma, window, wmax)
+    f = lib.pair_scores(ctx, prop.mlp, q0_index, sigma, window, wmax)
+    return np.exp(np.minimum(f * prop.sd, 60.0))

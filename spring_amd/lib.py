@@ -24,7 +24,7 @@ SYMBOLS = [
     "spr_lw4_tetra", "spr_lw4_tetra_all", "spr_lw3_kappa_all", "spr_mlp_score",
     "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve", "spr_get_phonon",
     "spr_lw3_slice", "spr_lw3_pairs_noalloc", "spr_set_branch_rules", "spr_kappa_rta_spectral",
-    "spr_lw3_offmesh", "spr_mesh_build",
+    "spr_lw3_offmesh", "spr_mesh_build", "spr_pair_scores",
 ]
 
 
@@ -461,6 +461,22 @@ class Mlp:
         self._check(self.lib.spr_mlp_predict(self.h, _ptr(X), ct.c_long(X.shape[0]), ct.c_int(PREC[prec]),
                                              _ptr(out)), "spr_mlp_predict")
         return out
+
+
+def pair_scores(ctx: GpuContext, mlp: "Mlp | None", q0_index, sigma: float, window: float, wmax: float,
+                want_x: bool = False):
+    """spr_pair_scores: per (q0, q1) the resonance weight S (mlp=None) or the network's
+    normalised output (n, nq), and optionally the pair feature rows (n, nq, 12)."""
+    lib = load()
+    q0 = np.ascontiguousarray(q0_index, np.int32)
+    S = np.zeros((q0.size, ctx.nq))
+    X = np.zeros((q0.size, ctx.nq, 12), np.float32) if want_x else None
+    rc = lib.spr_pair_scores(ctx.h, mlp.h if mlp is not None else None, ct.c_int(q0.size), _ptr(q0),
+                             ct.c_double(sigma), ct.c_double(window), ct.c_float(wmax), _ptr(S),
+                             _ptr(X) if want_x else None)
+    if rc != 0:
+        raise SprError(f"spr_pair_scores failed ({rc}): {lib.spr_last_error().decode()}")
+    return (S, X) if want_x else S
 
 
 def score(ctx: GpuContext, mlp: "Mlp | None", q0_index, sigma: float, window: float, y_std: float = 1.0,

@@ -107,3 +107,96 @@ def test_device_scores_match_host_definition(rocksalt):
         assert abs(S[k].sum() - ref.sum()) < 2e-2 * ref.sum()
         assert np.all(S[k][ref == 0] == 0)
     assert n_eval == n_ref
+
+
+def _host_pair_features(qm, ph, ig, sigma, window, wmax):
+    """spr_pair_scores' definition evaluated on the host (FP64) for one q0: (S, X)."""
+    from spring_amd import learned as Lrn
+    nd = ph.omega.shape[1]
+    q1 = np.arange(qm.nq)
+    q2 = qm.index(-qm.q_int[ig][None, :].astype(np.int64) - qm.q_int.astype(np.int64))
+    w0 = ph.omega[ig][None, :, None, None]
+    w1 = ph.omega[q1][:, None, :, None]
+    w2 = ph.omega[q2][:, None, None, :]
+    d = np.minimum(np.abs(w0 - w1 - w2), np.minimum(np.abs(w0 + w1 - w2), np.abs(w0 - w1 + w2)))
+    live = (ph.inv_omega[ig] != 0.0)[None, :, None, None]
+    inw = (d < window * sigma) & live
+    wgt = np.abs(Lrn.gauss_weights(ph, ig, q1, q2, sigma)) * ph.inv_omega[q1][:, None, :, None] * \
+        ph.inv_omega[q2][:, None, None, :] * inw
+    g = wgt.sum(axis=(2, 3))
+    h1 = (wgt * w1).sum(axis=(2, 3))
+    h2 = (wgt * w2).sum(axis=(2, 3))
+    ac = np.arange(nd) < 3
+    a = (wgt * (ac[None, None, :, None] | ac[None, None, None, :])).sum(axis=(2, 3))
+    cnt = inw.sum(axis=(1, 2, 3))
+    gt = g.sum(axis=0)
+    act = gt > 0
+    S = (g[:, act] / gt[act]).sum(axis=1)
+    fold = lambda k, n: np.where(k > n // 2, k - n, k) / n
+    X = np.zeros((qm.nq, 12))
+    k0 = np.array([ig % qm.mesh[0], (ig // qm.mesh[0]) % qm.mesh[1], ig // (qm.mesh[0] * qm.mesh[1])])
+    X[:, 0:3] = fold(k0, qm.mesh)[None, :]
+    X[:, 3:6] = fold(np.stack([q1 % qm.mesh[0], (q1 // qm.mesh[0]) % qm.mesh[1], q1 // (qm.mesh[0] * qm.mesh[1])], 1),
+                     qm.mesh[None, :])
+    X[:, 6] = np.log(np.maximum(S * qm.nq / max(act.sum(), 1), 1e-12))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        X[:, 7] = np.where(S > 0, (h1[:, act] / gt[act]).sum(1) / S / wmax, 0.0)
+        X[:, 8] = np.where(S > 0, (h2[:, act] / gt[act]).sum(1) / S / wmax, 0.0)
+        X[:, 9] = np.where(S > 0, (g[:, act] / gt[act] * ph.omega[ig][act]).sum(1) / S / wmax, 0.0)
+        X[:, 10] = np.where(S > 0, (a[:, act] / gt[act]).sum(1) / S, 0.0)
+    X[:, 11] = np.log1p(cnt) / np.log(nd ** 3)
+    return S, X
+
+
+def test_pair_features_match_host_definition(rocksalt):
+    """spr_pair_scores' resonance weights and feature rows against the host definition."""
+    from spring_amd import learned as Lrn
+    from spring_amd.lib import GpuContext
+    qm, ph = problem(rocksalt, (5, 5, 5))
+    ctx = GpuContext(rocksalt, qm, ph)
+    wmax = float(np.abs(ph.omega).max())
+    qis = [0, 3, qm.nq0 - 1]
+    S, X = Lrn.pair_features(ctx, qis, 0.3, wmax, window=3.0)
+    assert S[1].sum() > 0 and S[2].sum() > 0
+    for k, qi in enumerate(qis):
+        Sr, Xr = _host_pair_features(qm, ph, int(qm.irr_idx[qi]), 0.3, 3.0, wmax)
+        assert np.allclose(S[k], Sr, rtol=1e-5, atol=1e-7 * Sr.max()), qi
+        assert abs(S[k].sum() - Sr.sum()) <= 1e-5 * Sr.sum()
+        assert np.allclose(X[k], Xr, rtol=1e-4, atol=1e-4), qi
+    ctx.close()
+
+
+def test_pair_proposal_beats_g_only_10atom(rocksalt10):
+    """The pair-level learned proposal on the 10-atom cell (10^3 mesh with time reversal, 504
+    irreducible q0 -- the largest mesh this test's budget allows; the 20^3 form is the bench's
+    learned10 line): trained on every (q0, q1) pair term of 120 random q0 (120,000 samples the
+    method computes), it is judged on 40 other q0 by the exact relative RMS error of the 5 %
+    sampled estimate, (sum t^2/p - (sum t)^2)/K, against the G-only proposal and uniform
+    sampling at the same budget."""
+    from spring_amd import learned as Lrn
+    from spring_amd.lib import GpuContext
+    qm, ph = problem(rocksalt10, (10, 10, 10), time_rev=1)
+    ctx = GpuContext(rocksalt10, qm, ph)
+    sigma = 0.1
+    wmax = float(np.abs(ph.omega).max())
+    perm = np.random.default_rng(0).permutation(qm.nq0)
+    train, held = perm[:120], perm[120:160]
+    X, y = Lrn.pair_training_set(ctx, qm, ph, train, sigma, prec="fp32")
+    prop = Lrn.PairProposal.train(X, y, wmax, epochs=10)
+    print("pair proposal: samples", y.size, "loss", prop.losses[0], prop.losses[-1], flush=True)
+    K = int(0.05 * qm.nq)
+    Sg = Lrn.pair_proposal_scores(None, ctx, held, sigma, wmax)
+    Sl = Lrn.pair_proposal_scores(prop, ctx, held, sigma, wmax)
+    errs = {"uniform": [], "g_only": [], "learned": []}
+    for k, qi in enumerate(held):
+        t = ctx.lw3_pairs(np.full(qm.nq, qi), np.arange(qm.nq), sigma, "fp64")
+        L = np.abs(t.sum(axis=0))
+        sig = L > 1e-6 * L.max()
+        for name, p in (("uniform", np.full(qm.nq, 1.0 / qm.nq)), ("g_only", Lrn.mixture(Sg[k])),
+                        ("learned", Lrn.mixture(Sl[k]))):
+            errs[name].append(Lrn.is_rms_error(t, p, K)[sig])
+    e = {k: float(np.mean(np.concatenate(v))) for k, v in errs.items()}
+    print("mean relative RMS error at 5% budget:", e, flush=True)
+    assert e["learned"] < 0.9 * e["g_only"]
+    assert e["learned"] < 0.9 * e["uniform"]
+    ctx.close()

@@ -1,18 +1,20 @@
 // Four-phonon scattering (Ndof <= 6), FP64 or FP32, Gaussian delta.
 //
-// One workgroup takes one irreducible q0 and a chunk of q1 grid points; for every q1 it
-// runs over all q2 of the mesh with q3 = -(q0+q1+q2) mod G (MomentumConsrv4th,
-// QuasiMomentumConsrv4th_m.f90:32-206, without the permutation dedup, which the reference
-// needs only to store W4) and
+// One workgroup takes one irreducible q0 and a strided set of q1 grid points; for every q1 it
+// runs over the q2 of the mesh with q3 = -(q0+q1+q2) mod G (MomentumConsrv4th,
+// QuasiMomentumConsrv4th_m.f90:32-206), keeping one ordered triple per permutation class
+// (q1 <= q2 <= q3), and
 //   1. phase-sums FC4 into T(a,b,c,d) over the nonzero (N1,N2,N3) blocks
 //      (ScatterMatEl4, FourthOrder_m.f90:286-422), integer-address phases from a table;
 //   2. contracts d, c, b, a with e3, e2, e1, e0 (no conjugation);
-//   3. W4 = |V4|^2 prod(1/w) Sctr4THz2/Nq^2 (ScatterProbability4 :425-473) and adds
+//   3. W4 = |V4|^2 prod(1/w) Sctr4THz2/Nq^2 (ScatterProbability4 :425-473) and adds, for every
+//      ordering (q1', q2', q3') of the class,
 //        W4 [ 1/6 n1 n2 n3/n0 d(w0-w1-w2-w3) + 1/2 (1+n1) n2 n3/n0 d(w0+w1-w2-w3)
 //           + 1/2 (1+n1)(1+n2) n3/n0 d(w0+w1+w2-w3) ]
 //      the processes and weights of SumOverPolarization / FindFandOmegaVer_4th
 //      (Tetrahedron4th_sep.f90:174-425) with a Gaussian in place of the tetrahedron
-//      integral; NaN terms are zeroed as there (:238-263); s0 < 3 skipped at q0 = Gamma.
+//      integral (seven distinct terms per class element); NaN terms are zeroed as there
+//      (:238-263); s0 < 3 skipped at q0 = Gamma.  The store mode keeps every ordered triple.
 // The returned sum is the raw lw of TetrahedronIntegration4th; main.f90:660 adds 0.5 lw.
 //
 // Store mode (WST, FP64): one workgroup per (q1, q2 chunk) writes W4 of every ordered (q1, q2)
@@ -80,19 +82,24 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
 #pragma unroll
     for (int s = 0; s < ND; ++s) acc[s] = 0.0;
 
-    int q1_lo, q1_hi, q2_lo, q2_hi;
+    // store mode: the q2 chunk of one q1.  Gaussian mode: the q1 of this workgroup are
+    // ch, ch + nchunks, ... (strided: the permutation classes below make the work of a q1 fall
+    // with q1), every q2
+    int q1_lo, q1_hi, q1_step, q2_lo, q2_hi;
     if constexpr (WST) {
         q1_lo = A.q1_lo + ch / A.nq2ch;
         q1_hi = q1_lo + 1;
+        q1_step = 1;
         q2_lo = (ch % A.nq2ch) * A.q2_chunk;
         q2_hi = min(A.nq, q2_lo + A.q2_chunk);
     } else {
-        q1_lo = ch * A.chunk;
-        q1_hi = min(A.nq, q1_lo + A.chunk);
+        q1_lo = ch;
+        q1_hi = A.nq;
+        q1_step = A.nchunks;
         q2_lo = 0;
         q2_hi = A.nq;
     }
-    for (int q1 = q1_lo; q1 < q1_hi; ++q1) {
+    for (int q1 = q1_lo; q1 < q1_hi; q1 += q1_step) {
         const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
         for (int q2 = q2_lo; q2 < q2_hi; ++q2) {
             const int k20 = q2 % A.m0, k21 = (q2 / A.m0) % A.m1, k22 = q2 / (A.m0 * A.m1);
@@ -101,6 +108,12 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
             a1 += a1 < 0 ? A.m1 : 0;
             a2 += a2 < 0 ? A.m2 : 0;
             const int q3 = a0 + A.m0 * (a1 + A.m1 * a2);
+            // Gaussian mode: one ordered triple per permutation class of (q1, q2, q3), the one with
+            // q1 <= q2 <= q3 (QuasiMomentumConsrv4th_m.f90:224-312 stores one per class); its V4
+            // serves every ordering of the class (V4 is symmetric under permutations of legs 1-3
+            // with their branches), which the process terms below sum
+            if constexpr (!WST)
+                if (!(q1 <= q2 && q2 <= q3)) continue;
             const int qs[3] = {q1, q2, q3};
             __syncthreads();
             for (int i = tid; i < 3 * ND2; i += BLK) {
@@ -164,7 +177,14 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
                 }
                 continue;
             }
-            // 3. W4 and the three processes; src holds V[s0][s1][s2][s3]
+            // 3. W4 and the processes of every ordering of the class; src holds V[s0][s1][s2][s3].
+            //    An ordering puts legs (j, k, l) of the triple in places (1, 2, 3) and adds
+            //      W [ 1/6 F1 d(w0-w1-w2-w3) + 1/2 (1+n_j) n_k n_l/n0 d(w0+w_j-w_k-w_l)
+            //          + 1/2 (1+n_j)(1+n_k) n_l/n0 d(w0+w_j+w_k-w_l) ]
+            //    with the branch masks m(j,k,l) = S1[s_j] S2[s_k] S3[s_l]; over the six orderings
+            //    each leg is first in two and last in two, and a class of c distinct orderings
+            //    is met 6 / c times among the six, hence the factor c / 6 = 1 / |stabiliser|.
+            const double istab = q1 == q3 ? 1.0 / 6.0 : (q1 == q2 || q2 == q3) ? 0.5 : 1.0;
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
                 const int e = tid + j * BLK;
@@ -177,13 +197,32 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
                 const double w3 = s_md[3][0][s3], n3 = s_md[3][1][s3];
                 const double W = (double)cabs2(src[e]) * (s_md[0][2][s0] * s_md[1][2][s1] * s_md[2][2][s2] *
                                                           s_md[3][2][s3]) * A.factor;
-                const double x1 = w0 - (w1 + w2 + w3), x2 = w0 - (w2 + w3 - w1), x3 = w0 - (w3 - w1 - w2);
-                const double F1 = n1 * n2 * n3 / n0, F2 = (1.0 + n1) * n2 * n3 / n0,
-                             F3 = (1.0 + n1) * (1.0 + n2) * n3 / n0;
-                double t = W * (F1 * (c0 * exp(-hinv * x1 * x1)) / 6.0 + 0.5 * F2 * (c0 * exp(-hinv * x2 * x2)) +
-                                0.5 * F3 * (c0 * exp(-hinv * x3 * x3)));
-                if (t != t) t = 0.0;
-                if (A.sel) t *= A.sel[s0] * A.sel[ND + s1] * A.sel[2 * ND + s2] * A.sel[3 * ND + s3];
+                double m1 = 1.0, m2a = 1.0, m2b = 1.0, m2c = 1.0, m3a = 1.0, m3b = 1.0, m3c = 1.0;
+                if (A.sel) {
+                    const double *S1 = A.sel + ND, *S2 = A.sel + 2 * ND, *S3 = A.sel + 3 * ND;
+                    const double p123 = S1[s1] * S2[s2] * S3[s3], p132 = S1[s1] * S2[s3] * S3[s2];
+                    const double p213 = S1[s2] * S2[s1] * S3[s3], p231 = S1[s2] * S2[s3] * S3[s1];
+                    const double p312 = S1[s3] * S2[s1] * S3[s2], p321 = S1[s3] * S2[s2] * S3[s1];
+                    m1 = (p123 + p132 + p213 + p231 + p312 + p321) / 6.0;
+                    m2a = 0.5 * (p123 + p132);   // leg 1 first
+                    m2b = 0.5 * (p213 + p231);   // leg 2 first
+                    m2c = 0.5 * (p312 + p321);   // leg 3 first
+                    m3c = 0.5 * (p123 + p213);   // leg 3 last
+                    m3b = 0.5 * (p132 + p312);   // leg 2 last
+                    m3a = 0.5 * (p231 + p321);   // leg 1 last
+                }
+                auto gs = [&](double x) { return c0 * exp(-hinv * x * x); };
+                auto fin = [](double v) { return v != v ? 0.0 : v; };
+                const double r0 = 1.0 / n0;
+                double t = fin(m1 * W * n1 * n2 * n3 * r0 * gs(w0 - w1 - w2 - w3));
+                t += fin(m2a * W * (1.0 + n1) * n2 * n3 * r0 * gs(w0 + w1 - w2 - w3));
+                t += fin(m2b * W * (1.0 + n2) * n1 * n3 * r0 * gs(w0 + w2 - w1 - w3));
+                t += fin(m2c * W * (1.0 + n3) * n1 * n2 * r0 * gs(w0 + w3 - w1 - w2));
+                t += fin(m3c * W * (1.0 + n1) * (1.0 + n2) * n3 * r0 * gs(w0 + w1 + w2 - w3));
+                t += fin(m3b * W * (1.0 + n1) * (1.0 + n3) * n2 * r0 * gs(w0 + w1 + w3 - w2));
+                t += fin(m3a * W * (1.0 + n2) * (1.0 + n3) * n1 * r0 * gs(w0 + w2 + w3 - w1));
+                t *= istab;
+                if (A.sel) t *= A.sel[s0];
 #pragma unroll
                 for (int s = 0; s < ND; ++s)
                     if (s == s0) acc[s] += t;

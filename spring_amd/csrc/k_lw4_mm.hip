@@ -2,8 +2,9 @@
 //
 // Work item: one irreducible q0 and one q1 (every q1 of the mesh).  The 8 waves of the
 // workgroup take the q2 of the mesh in turn, q3 = -(q0 + q1 + q2) (MomentumConsrv4th,
-// QuasiMomentumConsrv4th_m.f90:32-206); each wave works on its own (q0, q1, q2) with no
-// workgroup barrier between them.  With the index pairs ab = (a, b), cd = (c, d),
+// QuasiMomentumConsrv4th_m.f90:32-206), one triple per permutation class (q1 <= q2 <= q3) when
+// no branch rule is set; each wave works on its own (q0, q1, q2) with no workgroup barrier
+// between them.  With the index pairs ab = (a, b), cd = (c, d),
 // s01 = (s0, s1), s23 = (s2, s3), per (q0, q1, q2):
 //   FT4     T[cd][ab]   = sum_u Phi_u(a, b, c, d) e^{2 pi i m_u / L}     (ScatterMatEl4,
 //           FourthOrder_m.f90:286-422; VALU, lanes along the FC4 components of a cluster, the
@@ -295,14 +296,21 @@ __global__ __launch_bounds__(512) void k_lw4_mm(Lw4MmArgs A) {
     }
     KSum<float> acc[NT2];
 
-    for (int q2 = w; q2 < A.nq; q2 += NW) {
-        wave_sync();   // the previous triple of this wave is done with its LDS
+    // Without branch rules, one ordered triple per permutation class of (q1, q2, q3): the one
+    // with q1 <= q2 <= q3, whose V4 serves every ordering (seven distinct process terms per
+    // element, see below); with branch rules every ordered triple, each with its masks.
+    const bool classes = A.sel == nullptr;
+    for (int q2 = (classes ? q1 : 0) + w; q2 < A.nq; q2 += NW) {
         const int k20 = q2 % A.m0, k21 = (q2 / A.m0) % A.m1, k22 = q2 / (A.m0 * A.m1);
         int a0 = -(k00 + k10 + k20) % A.m0, a1 = -(k01 + k11 + k21) % A.m1, a2 = -(k02 + k12 + k22) % A.m2;
         a0 += a0 < 0 ? A.m0 : 0;
         a1 += a1 < 0 ? A.m1 : 0;
         a2 += a2 < 0 ? A.m2 : 0;
         const int q3 = a0 + A.m0 * (a1 + A.m1 * a2);
+        if (classes && q3 < q2) continue;
+        // 1 / |stabiliser| of the class (the six orderings meet each distinct one 6 / c times)
+        const float istab = !classes ? 1.f : q1 == q3 ? 1.f / 6.f : (q1 == q2 || q2 == q3) ? 0.5f : 1.f;
+        wave_sync();   // the previous triple of this wave is done with its LDS
         for (int i = l; i < 2 * NN; i += 64) wE[i] = A.evec[(size_t)(i < NN ? q2 : q3) * NN + i % NN];
         if (l < NN) {   // column constants of s23 = l
             const int s2 = l / ND, s3 = l % ND;
@@ -311,7 +319,8 @@ __global__ __launch_bounds__(512) void k_lw4_mm(Lw4MmArgs A) {
             wCd[l] = make_double2(w2 + w3, w3 - w2);
             const double m23 = A.sel ? A.sel[2 * ND + s2] * A.sel[3 * ND + s3] : 1.0;
             wCf[l] = make_float4((float)(n2 * n3), (float)((1.0 + n2) * n3),
-                                 (float)(A.inv_omega[(size_t)q2 * ND + s2] * A.inv_omega[(size_t)q3 * ND + s3] * m23), 0.f);
+                                 (float)(A.inv_omega[(size_t)q2 * ND + s2] * A.inv_omega[(size_t)q3 * ND + s3] * m23),
+                                 (float)(n2 * (1.0 + n3)));
         }
         for (int i = l; i < 2 * A.nsite; i += 64) {
             const int4 c = A.site[i % A.nsite];
@@ -401,16 +410,38 @@ __global__ __launch_bounds__(512) void k_lw4_mm(Lw4MmArgs A) {
                     if (s23 >= NN) continue;
                     const double2 cd2 = wCd[s23];
                     const float4 cf = wCf[s23];
-                    const float x1 = (float)(Ra[nt] - cd2.x), x2 = (float)(Rb[nt] - cd2.x), x3 = (float)(Rb[nt] - cd2.y);
-                    const float e1 = __builtin_amdgcn_exp2f(hl2 * x1 * x1), e2 = __builtin_amdgcn_exp2f(hl2 * x2 * x2),
-                                e3 = __builtin_amdgcn_exp2f(hl2 * x3 * x3);
-                    const float inner = cf.x * (r1[nt] * e1 * (1.f / 6.f) + r2[nt] * e2 * 0.5f) + r2[nt] * cf.y * e3 * 0.5f;
+                    float inner;
+                    if (classes) {
+                        // every ordering of the class: with P = n2 n3, Q = (1+n2) n3, U = n2 (1+n3),
+                        // X = (1+n2)(1+n3), a = n1/n0, b = (1+n1)/n0 (weights 1/6, 1/2, 1/2 summed
+                        // over the six orderings):
+                        //   a [P g(w0-w1-w2-w3) + Q g(w0-w1+w2-w3) + U g(w0-w1-w2+w3) + X g(w0-w1+w2+w3)]
+                        //   + b [P g(w0+w1-w2-w3) + Q g(w0+w1+w2-w3) + U g(w0+w1-w2+w3)]
+                        const float P = cf.x, Q = cf.y, U = cf.w, X = 1.f + Q + U - P;
+                        const double ra = Ra[nt], rb = Rb[nt];
+                        const float xa1 = (float)(ra - cd2.x), xa2 = (float)(ra - cd2.y), xa3 = (float)(ra + cd2.y),
+                                    xa4 = (float)(ra + cd2.x);
+                        const float xb1 = (float)(rb - cd2.x), xb2 = (float)(rb - cd2.y), xb3 = (float)(rb + cd2.y);
+                        const float sa = P * __builtin_amdgcn_exp2f(hl2 * xa1 * xa1) +
+                                         Q * __builtin_amdgcn_exp2f(hl2 * xa2 * xa2) +
+                                         U * __builtin_amdgcn_exp2f(hl2 * xa3 * xa3) +
+                                         X * __builtin_amdgcn_exp2f(hl2 * xa4 * xa4);
+                        const float sb = P * __builtin_amdgcn_exp2f(hl2 * xb1 * xb1) +
+                                         Q * __builtin_amdgcn_exp2f(hl2 * xb2 * xb2) +
+                                         U * __builtin_amdgcn_exp2f(hl2 * xb3 * xb3);
+                        inner = r1[nt] * sa + r2[nt] * sb;
+                    } else {
+                        const float x1 = (float)(Ra[nt] - cd2.x), x2 = (float)(Rb[nt] - cd2.x), x3 = (float)(Rb[nt] - cd2.y);
+                        const float e1 = __builtin_amdgcn_exp2f(hl2 * x1 * x1), e2 = __builtin_amdgcn_exp2f(hl2 * x2 * x2),
+                                    e3 = __builtin_amdgcn_exp2f(hl2 * x3 * x3);
+                        inner = cf.x * (r1[nt] * e1 * (1.f / 6.f) + r2[nt] * e2 * 0.5f) + r2[nt] * cf.y * e3 * 0.5f;
+                    }
                     tri[nt] = fmaf((v[2 * h] * v[2 * h] + v[2 * h + 1] * v[2 * h + 1]) * cf.z, inner, tri[nt]);
                 }
             }
         }
 #pragma unroll
-        for (int nt = 0; nt < NT2; ++nt) acc[nt].add(tri[nt]);
+        for (int nt = 0; nt < NT2; ++nt) acc[nt].add(tri[nt] * istab);
     }
 
     // ---- per s0: the lane's columns times their constants, then waves in a fixed order ----

@@ -73,3 +73,13 @@ def test_lw4_full_size_10(rocksalt):
     err = rel_err(lw[qi], ref, 1e-9)
     print(f"lw4 10^3 q0 #{qi}: {err:.2e}", flush=True)
     assert err < 1e-10
+    # the FP32 matrix-core kernel (k_lw4_mm) on every irreducible q0 of the same workload against
+    # the FP64 linewidths just checked; and kappa of the summed three- and four-phonon linewidths
+    lw32 = ctx.lw4_all(0.1, "fp32")
+    e32 = rel_err(lw32, lw, 1e-6)
+    lw3 = ctx.lw3_all(0.1, "fp64")
+    k64, _ = ctx.kappa(lw3 + 0.5 * lw, 300.0, symm=True)
+    k32, _ = ctx.kappa(lw3 + 0.5 * lw32, 300.0, symm=True)
+    ek = float(np.max(np.abs(k32 - k64)) / np.max(np.abs(k64)))
+    print(f"lw4 10^3 FP32 vs FP64: linewidths {e32:.2e}, kappa {ek:.2e}", flush=True)
+    assert e32 < 1e-4 and ek < 1e-6

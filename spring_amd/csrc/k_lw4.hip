@@ -19,7 +19,8 @@
 //
 // Store mode (WST, FP64): one workgroup per (q1, q2 chunk) writes W4 of every ordered (q1, q2)
 // to W[q1 - q1_lo][q2][s0][s1][s2][s3] for the tetrahedron integration (k_lw4_tetra.hip)
-// instead of the Gaussian processes.
+// instead of the Gaussian processes; with every row resident (classes), V4 is evaluated once
+// per permutation class and written to the place of each distinct ordering.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -45,6 +46,8 @@ struct Lw4Args {
     // store mode: q1 in [q1_lo, q1_lo + gridDim.x / nq2ch), q2 chunks of q2_chunk points
     double *wst;
     int q1_lo, q2_chunk, nq2ch;
+    int classes;         // store mode with every row resident: one triple per permutation class,
+                         // W4 written to each distinct ordering's place
     const double *sel;   // branch selection (NULL: all): (4, nd) masks of s0, s1, s2, s3
 };
 
@@ -112,7 +115,7 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
             // q1 <= q2 <= q3 (QuasiMomentumConsrv4th_m.f90:224-312 stores one per class); its V4
             // serves every ordering of the class (V4 is symmetric under permutations of legs 1-3
             // with their branches), which the process terms below sum
-            if constexpr (!WST)
+            if (!WST || A.classes)
                 if (!(q1 <= q2 && q2 <= q3)) continue;
             const int qs[3] = {q1, q2, q3};
             __syncthreads();
@@ -169,6 +172,31 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
             __syncthreads();
             if constexpr (WST) {
                 // 3'. W4 only (every s0; the Gamma rows are zero through 1/omega = 0)
+                if (A.classes) {
+                    // every distinct ordering (legs i, j, l in places 1, 2, 3) of the class:
+                    // W4[q_i][q_j][s0][s_i][s_j][s_l] = W4 of the canonical element
+                    static constexpr int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+                    const int qq[3] = {q1, q2, q3};
+                    bool use[6];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        use[k] = true;
+                        for (int k2 = 0; k2 < k; ++k2)
+                            if (qq[perm[k][0]] == qq[perm[k2][0]] && qq[perm[k][1]] == qq[perm[k2][1]]) use[k] = false;
+                    }
+                    for (int e = tid; e < ND4; e += BLK) {
+                        const int s0 = e / ND3, sl[3] = {(e / ND2) % ND, (e / ND) % ND, e % ND};
+                        const double W = (double)cabs2(src[e]) * (s_md[0][2][s0] * s_md[1][2][sl[0]] *
+                                                                  s_md[2][2][sl[1]] * s_md[3][2][sl[2]]) * A.factor;
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) {
+                            if (!use[k]) continue;
+                            const int a = perm[k][0], b = perm[k][1], d = perm[k][2];
+                            A.wst[((size_t)qq[a] * A.nq + qq[b]) * ND4 + ((s0 * ND + sl[a]) * ND + sl[b]) * ND + sl[d]] = W;
+                        }
+                    }
+                    continue;
+                }
                 double *wrow = A.wst + ((size_t)(q1 - A.q1_lo) * A.nq + q2) * ND4;
                 for (int e = tid; e < ND4; e += BLK) {
                     const int s0 = e / ND3, s1 = (e / ND2) % ND, s2 = (e / ND) % ND, s3 = e % ND;
@@ -301,6 +329,7 @@ static int launch_lw4_t(Ctx &c, int q0_first, int q0_count, double sigma, double
     SPR_HIP(hipEventRecord(c.ev0, c.stream));
     A.wst = nullptr;
     A.q1_lo = A.q2_chunk = A.nq2ch = 0;
+    A.classes = 0;
     hipLaunchKernelGGL((k_lw4<R, ND, BLK, false>), dim3(A.nchunks, q0_count), dim3(BLK), dyn, c.stream, A);
     SPR_HIP(hipGetLastError());
     SPR_HIP(hipEventRecord(c.ev1, c.stream));
@@ -314,7 +343,7 @@ static int launch_lw4_t(Ctx &c, int q0_first, int q0_count, double sigma, double
 }
 
 template <int ND>
-static int launch_w4_rows_t(Ctx &c, int qi, int q1_lo, int nq1, double *d_w) {
+static int launch_w4_rows_t(Ctx &c, int qi, int q1_lo, int nq1, double *d_w, bool classes) {
     constexpr int BLK = 256;
     Lw4Args A{};
     A.q0_grid = c.irr_grid + qi;
@@ -336,6 +365,7 @@ static int launch_w4_rows_t(Ctx &c, int qi, int q1_lo, int nq1, double *d_w) {
     A.factor = lw4_factor(c);
     A.wst = d_w;
     A.q1_lo = q1_lo;
+    A.classes = classes ? 1 : 0;
     // enough workgroups to fill the device: split the q2 range of every q1
     const long target = 4L * c.ncu;
     A.nq2ch = (int)std::min<long>(c.nq, std::max<long>(1, (target + nq1 - 1) / nq1));
@@ -347,10 +377,14 @@ static int launch_w4_rows_t(Ctx &c, int qi, int q1_lo, int nq1, double *d_w) {
     return SPR_OK;
 }
 
-int launch_w4_rows(Ctx &c, int qi, int q1_lo, int nq1, double *d_w) {
+int launch_w4_rows(Ctx &c, int qi, int q1_lo, int nq1, double *d_w, bool classes) {
+    if (classes && (q1_lo != 0 || nq1 != c.nq)) {
+        set_error("four-phonon W4 store: the class mode needs every row resident");
+        return SPR_E_ARG;
+    }
     switch (c.nd) {
-        case 3: return launch_w4_rows_t<3>(c, qi, q1_lo, nq1, d_w);
-        case 6: return launch_w4_rows_t<6>(c, qi, q1_lo, nq1, d_w);
+        case 3: return launch_w4_rows_t<3>(c, qi, q1_lo, nq1, d_w, classes);
+        case 6: return launch_w4_rows_t<6>(c, qi, q1_lo, nq1, d_w, classes);
         default:
             set_error("four-phonon path: Ndof " + std::to_string(c.nd) + " not built (3, 6)");
             return SPR_E_UNSUPPORTED;

@@ -10,11 +10,12 @@
 // reference does for degenerate vertices (:238-263); s0 < 3 skipped at q0 = Gamma.  The sum is
 // the raw lw; main.f90:660 adds 0.5 lw to the linewidth.
 //
-// W4(q1, q2) comes from the FP64 store mode of k_lw4 for a chunk of q1 rows at a time (every
-// q2 of a row: the tetrahedra of a q1 touch all of them), so device memory stays bounded.
-// The reference keeps W4 once per permutation class of (q1, q2, q3) and permutes (s1, s2, s3)
-// to read the other orders; with an index-permutation-symmetric FC4 that is the same number as
-// W4 of the ordered triple, computed here.
+// W4(q1, q2) comes from the FP64 store mode of k_lw4.  When every q1 row fits (a quarter of
+// the free memory, at most 32 GiB), V4 is evaluated once per permutation class of (q1, q2, q3)
+// and written to the place of each distinct ordering, as the reference keeps W4 once per class
+// and permutes (s1, s2, s3) to read the other orders (with an index-permutation-symmetric FC4
+// the numbers are those of the ordered triples); otherwise a chunk of q1 rows at a time (every q2
+// of a row: the tetrahedra of a q1 touch all of them), every ordered triple evaluated.
 //
 // Mapping: a thread owns (q1, s1 s2 s3, tetrahedron), tetrahedron fastest across lanes, and
 // walks the processes, then s0.  The vertex energies do not depend on s0, so each process's
@@ -222,9 +223,17 @@ int launch_lw4_tetra(Ctx &c, int qi, double *d_lw) {
     const size_t row = (size_t)c.nq * nd4 * sizeof(double);
     // chunk: the byte cap, W offsets below 2^31 and items (nq1 x 6 nq x Ndof^3) below 2^31
     const size_t item_row = (size_t)6 * c.nq * c.nd * c.nd * c.nd;
-    const int nq1c = (int)std::max<size_t>(
-        1, std::min<size_t>({(size_t)c.nq, w4_chunk_bytes() / row, (((size_t)1 << 31) - 1) / (row / sizeof(double)),
-                             (((size_t)1 << 31) - 1) / item_row}));
+    const size_t lim = std::min<size_t>((((size_t)1 << 31) - 1) / (row / sizeof(double)),
+                                        (((size_t)1 << 31) - 1) / item_row);
+    // every row resident when it fits (a quarter of the free memory, at most 32 GiB, unless
+    // SPR_W4_CHUNK_MB sets a cap): the store then evaluates V4 once per permutation class
+    size_t cap = w4_chunk_bytes();
+    if (!getenv("SPR_W4_CHUNK_MB")) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) cap = std::max(cap, std::min(size_t(32) << 30, fr / 4));
+    }
+    const bool classes = (size_t)c.nq <= cap / row && (size_t)c.nq <= lim;
+    const int nq1c = classes ? c.nq : (int)std::max<size_t>(1, std::min<size_t>({(size_t)c.nq, w4_chunk_bytes() / row, lim}));
     const int ncalls = (c.nq + nq1c - 1) / nq1c;
     const long items = (long)nq1c * 6 * c.nq * c.nd * c.nd * c.nd;
     // a multiple of 8 workgroups (the XCD split of the kernel)
@@ -251,7 +260,7 @@ int launch_lw4_tetra(Ctx &c, int qi, double *d_lw) {
     SPR_HIP(hipEventRecord(c.ev0, c.stream));
     for (int k = 0; k < ncalls; ++k) {
         const int lo = k * nq1c, n = std::min(nq1c, c.nq - lo);
-        if ((rc = launch_w4_rows(c, qi, lo, n, c.w4buf))) return rc;
+        if ((rc = launch_w4_rows(c, qi, lo, n, c.w4buf, classes))) return rc;
         A.q1_lo = lo;
         A.nq1 = n;
         A.partial = c.partial + (size_t)k * grid * c.nd;

@@ -210,7 +210,7 @@ bool lw3_mm_w_built(int nb);
 int ensure_tetra(Ctx &c);
 // FP64 W4 of irreducible q0 `qi` for q1 in [q1_lo, q1_lo + nq1) and every q2:
 // d_w[q1 - q1_lo][q2][s0][s1][s2][s3]
-int launch_w4_rows(Ctx &c, int qi, int q1_lo, int nq1, double *d_w);
+int launch_w4_rows(Ctx &c, int qi, int q1_lo, int nq1, double *d_w, bool classes = false);
 int launch_lw4_tetra(Ctx &c, int qi, double *d_lw);
 // device inputs of the off-mesh q0 W (LineWidth): eigenvectors (nd, nd) and 1/omega (nd) of q0,
 // eigenvectors (nq, nd, nd) and 1/omega (nq, nd) of q2 per grid q1, e^{-2 pi i x0 . n} per FC3 site,

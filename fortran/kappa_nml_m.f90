@@ -26,7 +26,7 @@ module kappa_nml_m
     implicit none
     private
 
-    public :: run_namelist_host, WriteAsciiOutFile
+    public :: run_namelist_host, WriteAsciiOutFile, read_cell_nml, cell_volume, cross3, fail
 
 contains
 
@@ -36,22 +36,13 @@ contains
         logical                     :: SpectralResolve, ModeResolve, Symm, gauss
         integer                     :: prec
 
-        ! &FCInfo
+        ! &FCInfo, &CrystalInfo, &AtmInfo
         character(len=64)           :: prefix
-        logical                     :: SecondFC, ThirdFC, FourthFC, Force_dd
-        integer                     :: nn_2nd, nn_3rd, nn_4th
-        namelist /FCInfo/ prefix, SecondFC, ThirdFC, FourthFC, Force_dd, nn_2nd, nn_3rd, nn_4th
-        ! &CrystalInfo
-        integer                     :: Nbasis, Ntyp, Sup_Cell(3)
+        logical                     :: Force_dd
+        integer                     :: Nbasis
         real(dp)                    :: Latvec(3, 3)
-        namelist /CrystalInfo/ Nbasis, Ntyp, Latvec, Sup_Cell
-        ! &AtmInfo
-        real(dp), allocatable       :: atm_pos(:,:), mass_typ(:), BornZ(:,:,:)
-        integer, allocatable        :: typ_indx(:), atomic_no(:)
-        character(len=3), allocatable :: typ_lbl(:)
-        real(dp)                    :: dielec(3, 3)
-        real(dp), allocatable       :: mass(:)
-        namelist /AtmInfo/ atm_pos, typ_indx, typ_lbl, atomic_no, mass, dielec, BornZ
+        real(dp), allocatable       :: atm_pos(:,:), mass(:)
+        integer, allocatable        :: typ_indx(:)
         ! &AnhInfo
         character(len=1)            :: SelectionRule3(3), SelectionRule4(4)
         character(len=512)          :: SctrEl4thRDir
@@ -67,7 +58,7 @@ contains
         character(len=256)          :: msg
         character(len=32)           :: Temp_char, Ndof_char
         character(len=600)          :: fc2file, fc3file, fc4file, kappa_outFile, outfile_ascii, outfile_frmt
-        real(dp)                    :: vol, a1(3), a2(3), a3(3), freq_min, freq_max, P3, PhaseSpace3
+        real(dp)                    :: vol, freq_min, freq_max, P3, PhaseSpace3
         integer, allocatable        :: atmNum2(:), atmIndx2(:,:,:), atmNum3(:), atmIndx3(:,:,:), atmNum4(:), &
                                      & atmIndx4(:,:,:)
         real(dp), allocatable       :: fC2(:,:,:,:,:), fC3(:,:,:,:,:,:), fC4(:,:,:,:,:,:,:,:)
@@ -85,36 +76,7 @@ contains
         call get_command_line(infile, T, SpectralResolve, ModeResolve, Symm, prec, gauss)
 
         ! ---------------- input.nml (unit_mod.f90:64-214, Helper_m.f90:37-132) ----------------
-        prefix = 'spring'
-        SecondFC = .true.
-        ThirdFC = .true.
-        FourthFC = .false.
-        Force_dd = .false.
-        nn_2nd = 0
-        nn_3rd = 0
-        nn_4th = 0
-        Nbasis = 2
-        Ntyp = 1
-        Latvec = 0.0_dp
-        Sup_Cell = 5
-        open(newunit=u, file=trim(infile), status='old', action='read', delim='apostrophe', iostat=err, iomsg=msg)
-        if ( err /= 0 ) call fail('opening namelist file '//trim(infile)//': '//trim(msg))
-        read(u, nml=FCInfo, iostat=err, iomsg=msg)
-        if ( err /= 0 ) call fail('reading &FCInfo of '//trim(infile)//': '//trim(msg))
-        rewind(u)
-        read(u, nml=CrystalInfo, iostat=err, iomsg=msg)
-        if ( err /= 0 ) call fail('reading &CrystalInfo of '//trim(infile)//': '//trim(msg))
-        allocate( atm_pos(3, Nbasis), typ_indx(Nbasis), typ_lbl(Ntyp), atomic_no(Ntyp), mass(Ntyp), &
-                & BornZ(3, 3, Nbasis) )
-        dielec = 1.0_dp
-        BornZ = 0.0_dp
-        atomic_no = 0
-        typ_lbl = '   '
-        rewind(u)
-        read(u, nml=AtmInfo, iostat=err, iomsg=msg)
-        if ( err /= 0 ) call fail('reading &AtmInfo of '//trim(infile)//': '//trim(msg))
-        mass_typ = mass
-        deallocate( mass )
+        call read_cell_nml(infile, prefix, Force_dd, Nbasis, Latvec, atm_pos, typ_indx, mass)
         ! ReadNmlKappa defaults (Helper_m.f90:66-83)
         SelectionRule3 = 'D'
         SelectionRule4 = 'D'
@@ -133,7 +95,8 @@ contains
         ReadRestart = .false.
         WriteSctrEl4th = .false.
         ReadSctrEl4th = .false.
-        rewind(u)
+        open(newunit=u, file=trim(infile), status='old', action='read', delim='apostrophe', iostat=err, iomsg=msg)
+        if ( err /= 0 ) call fail('opening namelist file '//trim(infile)//': '//trim(msg))
         read(u, nml=AnhInfo, iostat=err, iomsg=msg)
         if ( err /= 0 ) call fail('reading &AnhInfo of '//trim(infile)//': '//trim(msg))
         close(u)
@@ -145,15 +108,7 @@ contains
             & write(*, '(A)') 'WARNING: restart and scattering-matrix files are neither read nor written'
 
         Ndof = 3 * Nbasis
-        allocate( mass(Nbasis) )   ! per atom, as init_data builds it (:181-186)
-        do i = 1, Nbasis
-            mass(i) = mass_typ(typ_indx(i))
-        end do
-        a1 = Latvec(:, 1)
-        a2 = Latvec(:, 2)
-        a3 = Latvec(:, 3)
-        vol = dabs(dot_product(a1, [a2(2) * a3(3) - a2(3) * a3(2), a2(3) * a3(1) - a2(1) * a3(3), &
-                                  & a2(1) * a3(2) - a2(2) * a3(1)]))
+        vol = cell_volume(Latvec)
 
         write(Temp_char, '(F6.1)') T
         fc2file = 'FC2nd_'//trim(prefix)//trim(adjustl(Temp_char))//'K_F.h5'
@@ -276,6 +231,77 @@ contains
         if ( .not. gauss ) write(*, '(A, G20.12, A)') '3-phonon scattering phase-space: ', PhaseSpace3, ' fs'
         call gpu_ctx_destroy(ctx)
     end subroutine run_namelist_host
+
+    ! cell%init_data (unit_mod.f90:64-214): &FCInfo, &CrystalInfo and &AtmInfo, each read from the
+    ! start of the file with the reference's defaults; mass(Nbasis) per atom from the per-type
+    ! masses (:181-186), typ_indx 1-based, atm_pos(3, Nbasis) fractional
+    subroutine read_cell_nml(infile, prefix, Force_dd, Nbasis, Latvec, atm_pos, typ_indx, mass_atm)
+        character(len=*), intent(in)        :: infile
+        character(len=64), intent(out)      :: prefix
+        logical, intent(out)                :: Force_dd
+        integer, intent(out)                :: Nbasis
+        real(dp), intent(out)               :: Latvec(3, 3)
+        real(dp), allocatable, intent(out)  :: atm_pos(:,:), mass_atm(:)
+        integer, allocatable, intent(out)   :: typ_indx(:)
+        logical                     :: SecondFC, ThirdFC, FourthFC
+        integer                     :: nn_2nd, nn_3rd, nn_4th
+        namelist /FCInfo/ prefix, SecondFC, ThirdFC, FourthFC, Force_dd, nn_2nd, nn_3rd, nn_4th
+        integer                     :: Ntyp, Sup_Cell(3)
+        namelist /CrystalInfo/ Nbasis, Ntyp, Latvec, Sup_Cell
+        real(dp), allocatable       :: BornZ(:,:,:), mass(:)
+        integer, allocatable        :: atomic_no(:)
+        character(len=3), allocatable :: typ_lbl(:)
+        real(dp)                    :: dielec(3, 3)
+        namelist /AtmInfo/ atm_pos, typ_indx, typ_lbl, atomic_no, mass, dielec, BornZ
+        integer                     :: u, err, i
+        character(len=256)          :: msg
+        prefix = 'spring'
+        SecondFC = .true.
+        ThirdFC = .true.
+        FourthFC = .false.
+        Force_dd = .false.
+        nn_2nd = 0
+        nn_3rd = 0
+        nn_4th = 0
+        Nbasis = 2
+        Ntyp = 1
+        Latvec = 0.0_dp
+        Sup_Cell = 5
+        open(newunit=u, file=trim(infile), status='old', action='read', delim='apostrophe', iostat=err, iomsg=msg)
+        if ( err /= 0 ) call fail('opening namelist file '//trim(infile)//': '//trim(msg))
+        read(u, nml=FCInfo, iostat=err, iomsg=msg)
+        if ( err /= 0 ) call fail('reading &FCInfo of '//trim(infile)//': '//trim(msg))
+        rewind(u)
+        read(u, nml=CrystalInfo, iostat=err, iomsg=msg)
+        if ( err /= 0 ) call fail('reading &CrystalInfo of '//trim(infile)//': '//trim(msg))
+        allocate( atm_pos(3, Nbasis), typ_indx(Nbasis), typ_lbl(Ntyp), atomic_no(Ntyp), mass(Ntyp), &
+                & BornZ(3, 3, Nbasis) )
+        dielec = 1.0_dp
+        BornZ = 0.0_dp
+        atomic_no = 0
+        typ_lbl = '   '
+        rewind(u)
+        read(u, nml=AtmInfo, iostat=err, iomsg=msg)
+        if ( err /= 0 ) call fail('reading &AtmInfo of '//trim(infile)//': '//trim(msg))
+        close(u)
+        allocate( mass_atm(Nbasis) )
+        do i = 1, Nbasis
+            mass_atm(i) = mass(typ_indx(i))
+        end do
+    end subroutine read_cell_nml
+
+    ! |a1 . (a2 x a3)| of the lattice vectors (columns of Latvec)
+    pure function cell_volume(Latvec) result(vol)
+        real(dp), intent(in)    :: Latvec(3, 3)
+        real(dp)                :: vol
+        vol = dabs(dot_product(Latvec(:, 1), cross3(Latvec(:, 2), Latvec(:, 3))))
+    end function cell_volume
+
+    pure function cross3(a, b) result(c)
+        real(dp), intent(in)    :: a(3), b(3)
+        real(dp)                :: c(3)
+        c = [a(2) * b(3) - a(3) * b(2), a(3) * b(1) - a(1) * b(3), a(1) * b(2) - a(2) * b(1)]
+    end function cross3
 
     ! get_command_line (parse_cmd_line.f90): "-name value" pairs, the reference's defaults, plus
     ! -prec and -delta of this host

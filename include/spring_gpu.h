@@ -145,7 +145,7 @@ int spr_lw3_tetra_all(spr_ctx *ctx, double *lw_all, double *p3_all);
  * dispersion path), ScatterProbAllq1 + CalculateLwTetrahedron (delta = SPR_DELTA_TETRA, the path
  * src/LineWidth/Src/main.f90:293-379 runs) of src/LineWidth/Src/Linewidth_m.f90:48-417, or with
  * SPR_DELTA_GAUSS the Gaussian sum of CalculateLinewidth's formula (:234-323; main.f90 keeps that
- * call switched off) over every branch, sigma in THz.  q0_cart (1/A) with its phonons as set_Phon_mp gives
+ * call switched off) over every branch, sigma in rad/ps.  q0_cart (1/A) with its phonons as set_Phon_mp gives
  * them (evec0 = Evec(Ndof, Ndof) complex, omega0, inv_omega0 = OnebyOmega); the q1 are the
  * context's mesh (zero_centered selects the reference's tetraQZeroCentered mesh and BZ test,
  * main.f90:316-330); for every q1, q2 = G - (q0 + q1) with the reference's G search

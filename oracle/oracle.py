@@ -525,3 +525,34 @@ def lw3_offmesh(model, qm, ph, q0_cart, ph0, T, delta="tetra", sigma=0.1, zero_c
     else:
         lw = sum(lw3_from_w(W[i], w0, inp.omega[i], w2[i], inp.nbe[i], n2[i], sigma) for i in range(nq))
     return lw, W, exists
+
+
+def q_points_highsymm(q_high_sym, G):
+    """The q0 of LineWidth.x's dispersion path (q_points_highsymm,
+    src/LineWidth/Src/CreateMesh_mod.f90:85-172).  q_high_sym: (n, 4) rows (reduced coordinates,
+    number of points on the segment ending at this node); G: reciprocal vectors as columns.
+    Segment n -> n+1 holds count(n+1) points start + k*step*v_hat, k = 1..count, step =
+    |end - start| / count (the start node itself is not a point); the distance runs on from the
+    previous segment's last point.  A segment is zero-centred when a coordinate of either end is
+    negative or all of one end's coordinates are within [-1/2, 1/2] (:132-141).  The first node's
+    count is not used (the reference allocates rows for it that it never fills).
+    Returns (q_cart (m, 3), dist (m,), zero_centered (m,) bool)."""
+    nodes = np.asarray(q_high_sym, float)
+    G = np.asarray(G, float)
+    q, dist, zc = [], [], []
+    sd = 0.0
+    for n in range(nodes.shape[0] - 1):
+        a, b, cnt = nodes[n, :3], nodes[n + 1, :3], int(nodes[n + 1, 3])
+        z = bool(np.any(a < 0) or np.any(b < 0) or np.all(np.abs(a) <= 0.5) or np.all(np.abs(b) <= 0.5))
+        s, e = G @ a, G @ b
+        vlen = float(np.linalg.norm(e - s))
+        vhat = (e - s) / vlen
+        step = vlen / cnt if cnt != 0 else 0.0
+        d = sd
+        for k in range(1, cnt + 1):
+            d = sd + k * step
+            q.append(s + (k * step) * vhat)
+            dist.append(d)
+            zc.append(z)
+        sd = d
+    return np.array(q).reshape(-1, 3), np.array(dist), np.array(zc, bool)

@@ -75,44 +75,83 @@ def write_bundle(path, model, qm, ph, T: float, sigma: float, prefix: str = "syn
     return path
 
 
-def write_input_nml(path, model, qmesh, T: float, prefix: str = "synth", sigma: float = 5.1,
-                    time_rev: int = 0, symprec: float = 1e-5, rules3: str = "DDD", rules4: str = "DDDD",
-                    four_phonon: bool = False):
-    """A kappa.x input: `input.nml` with &FCInfo, &CrystalInfo, &AtmInfo (cell%init_data,
-    src/Kappa/Src/unit_mod.f90:64-214) and &AnhInfo (ReadNmlKappa, Helper_m.f90:37-132), plus the
-    force constants as FC2nd_/FC3rd_[/FC4th_]<prefix><T>K_F.h5 beside it.  Species become types
-    (typ_indx 1-based, one mass per type).  Reals are written in shortest round-trip form, so the
-    reader gets the same doubles."""
-    from . import h5io
-    os.makedirs(path, exist_ok=True)
+def _cell_nml(model, prefix, four_phonon=False):
+    """&FCInfo, &CrystalInfo, &AtmInfo lines (cell%init_data, src/Kappa/Src/unit_mod.f90:64-214).
+    Species become types (typ_indx 1-based, one mass per type); reals in shortest round-trip
+    form, so the reader gets the same doubles."""
     cr = model.crystal
     species, typ = np.unique(np.asarray(cr.species), return_inverse=True)
     mass_typ = [float(np.asarray(cr.mass)[typ == t][0]) for t in range(species.size)]
-    f = lambda xs: ", ".join(repr(float(x)) for x in xs)
-    lines = ["&FCInfo", f"  prefix = '{prefix}'", "  SecondFC = .true.", "  ThirdFC = .true.",
-             f"  FourthFC = {'.true.' if four_phonon else '.false.'}", "  Force_dd = .false.", "/",
-             "&CrystalInfo", f"  Nbasis = {cr.nb}", f"  Ntyp = {species.size}",
-             f"  Latvec = {f(np.asarray(cr.latvec).T.ravel())}", "  Sup_Cell = 5, 5, 5", "/",
-             "&AtmInfo", f"  atm_pos = {f(np.asarray(cr.basis_frac).ravel())}",
-             f"  typ_indx = {', '.join(str(int(t) + 1) for t in typ)}",
-             f"  typ_lbl = {', '.join(repr('X%d' % t) for t in range(species.size))}",
-             f"  mass = {f(mass_typ)}", "/",
-             "&AnhInfo", f"  SelectionRule3 = {', '.join(repr(c) for c in rules3)}",
-             f"  SelectionRule4 = {', '.join(repr(c) for c in rules4)}", f"  Sigma = {float(sigma)!r}",
-             f"  symprec = {float(symprec)!r}", f"  qmesh = {', '.join(str(int(x)) for x in qmesh)}",
-             f"  time_rev = {int(time_rev)}", f"  FourPhonon = {'.true.' if four_phonon else '.false.'}", "/", ""]
-    with open(os.path.join(path, "input.nml"), "w") as fh:
-        fh.write("\n".join(lines))
-    for ifc in (model.fc2, model.fc3, model.fc4 if four_phonon else None):
+    return ["&FCInfo", f"  prefix = '{prefix}'", "  SecondFC = .true.", "  ThirdFC = .true.",
+            f"  FourthFC = {'.true.' if four_phonon else '.false.'}", "  Force_dd = .false.", "/",
+            "&CrystalInfo", f"  Nbasis = {cr.nb}", f"  Ntyp = {species.size}",
+            f"  Latvec = {_reals(np.asarray(cr.latvec).T.ravel())}", "  Sup_Cell = 5, 5, 5", "/",
+            "&AtmInfo", f"  atm_pos = {_reals(np.asarray(cr.basis_frac).ravel())}",
+            f"  typ_indx = {', '.join(str(int(t) + 1) for t in typ)}",
+            f"  typ_lbl = {', '.join(repr('X%d' % t) for t in range(species.size))}",
+            f"  mass = {_reals(mass_typ)}", "/"]
+
+
+def _reals(xs):
+    return ", ".join(repr(float(x)) for x in xs)
+
+
+def _write_fc_files(path, model, prefix, T, orders):
+    from . import h5io
+    for ifc in (model.fc2, model.fc3, model.fc4 if 4 in orders else None):
         if ifc is not None:
             h5io.write_fc(os.path.join(path, h5io.fc_file_name(ifc.order, prefix, T)), ifc)
+
+
+def write_input_nml(path, model, qmesh, T: float, prefix: str = "synth", sigma: float = 5.1,
+                    time_rev: int = 0, symprec: float = 1e-5, rules3: str = "DDD", rules4: str = "DDDD",
+                    four_phonon: bool = False):
+    """A kappa.x input: `input.nml` with &FCInfo, &CrystalInfo, &AtmInfo and &AnhInfo (ReadNmlKappa,
+    src/Kappa/Src/Helper_m.f90:37-132), plus the force constants as
+    FC2nd_/FC3rd_[/FC4th_]<prefix><T>K_F.h5 beside it."""
+    os.makedirs(path, exist_ok=True)
+    lines = _cell_nml(model, prefix, four_phonon) + [
+        "&AnhInfo", f"  SelectionRule3 = {', '.join(repr(c) for c in rules3)}",
+        f"  SelectionRule4 = {', '.join(repr(c) for c in rules4)}", f"  Sigma = {float(sigma)!r}",
+        f"  symprec = {float(symprec)!r}", f"  qmesh = {', '.join(str(int(x)) for x in qmesh)}",
+        f"  time_rev = {int(time_rev)}", f"  FourPhonon = {'.true.' if four_phonon else '.false.'}", "/", ""]
+    with open(os.path.join(path, "input.nml"), "w") as fh:
+        fh.write("\n".join(lines))
+    _write_fc_files(path, model, prefix, T, (2, 3, 4) if four_phonon else (2, 3))
     return path
 
 
-def read_linewidth_file(fname, ndof):
-    """Rows of linewidth_IBZ_*.txt: (omega(ndof), lw(ndof)) per irreducible q0.
+def write_linewidth_nml(path, model, qmesh, T: float, q_high_sym, prefix: str = "synth", sigma: float = 0.1):
+    """A LineWidth.x input: `input.nml` with the cell namelists, &LWInfo (Sigma, qmesh, num_points)
+    and &HighSymPathLW (q_high_sym(:, n) = reduced coordinates and the point count of the segment
+    ending at node n; src/LineWidth/Src/main.f90:70-76), plus FC2nd_/FC3rd_<prefix><T>K_F.h5."""
+    os.makedirs(path, exist_ok=True)
+    nodes = np.asarray(q_high_sym, float)
+    lines = _cell_nml(model, prefix) + [
+        "&LWInfo", f"  Sigma = {float(sigma)!r}", f"  qmesh = {', '.join(str(int(x)) for x in qmesh)}",
+        f"  num_points = {nodes.shape[0]}", "/", "&HighSymPathLW"]
+    lines += [f"  q_high_sym(:, {n + 1}) = {_reals(row[:3])}, {int(row[3])}" for n, row in enumerate(nodes)]
+    lines += ["/", ""]
+    with open(os.path.join(path, "input.nml"), "w") as fh:
+        fh.write("\n".join(lines))
+    _write_fc_files(path, model, prefix, T, (2, 3))
+    return path
 
-    Fortran's G20.12 drops the exponent letter when the exponent has three digits
+
+def read_linewidth_path_file(fname, ndof):
+    """Rows of LineWidth.x's linewidth_*.txt: (distance, omega(ndof), lw(ndof)) per path point."""
+    data = _read_rows(fname)
+    return data[:, 0], data[:, 1:ndof + 1], data[:, ndof + 1:2 * ndof + 1]
+
+
+def read_linewidth_file(fname, ndof):
+    """Rows of linewidth_IBZ_*.txt: (omega(ndof), lw(ndof)) per irreducible q0."""
+    data = _read_rows(fname)
+    return data[:, :ndof], data[:, ndof:2 * ndof]
+
+
+def _read_rows(fname):
+    """Fortran's G20.12 drops the exponent letter when the exponent has three digits
     ('0.4920108584-171'), so such fields are repaired before parsing."""
     import re
     rows = []
@@ -121,5 +160,4 @@ def read_linewidth_file(fname, ndof):
             continue
         fields = [re.sub(r"(\d)([-+]\d{3})$", r"\1E\2", f) for f in line.split()]
         rows.append([float(f) for f in fields])
-    data = np.atleast_2d(np.array(rows))
-    return data[:, :ndof], data[:, ndof:2 * ndof]
+    return np.atleast_2d(np.array(rows))

@@ -77,3 +77,32 @@ def test_q2_search_order_and_boundary(rocksalt):
     i = int(np.flatnonzero(np.all(qm.q_int == 0, axis=1))[0])   # q1 = 0: q2 = G - q0
     # G = 0 gives x = -1/2 (inside), found before G = b1 (x = +1/2)
     assert np.allclose(x[i], [-0.5, 0.0, 0.0], atol=1e-12)
+
+
+def test_path_points_closed_form():
+    """q_points_highsymm on a cubic cell (G = 2 pi I): the points of a segment are k/count of the
+    way from its start (start excluded, end included), distances are cumulative arc length, a
+    segment's zero-centred flag follows the rule on its two ends, a zero-count segment adds no
+    point and keeps the distance, and the first node's count is not used."""
+    G = 2.0 * np.pi * np.eye(3)
+    nodes = [[0.0, 0.0, 0.0, 7],            # count of the first node: ignored
+             [0.0, 0.5, 0.5, 4],            # Gamma -> X: |start| <= 1/2 -> zero-centred
+             [0.6, 0.7, 0.8, 2],            # X -> generic: |X| <= 1/2 -> zero-centred
+             [0.9, 0.6, 0.7, 3],            # both ends outside [-1/2, 1/2], no negative: not
+             [0.9, 0.6, 0.7, 0],            # zero-count segment
+             [0.9, -0.6, 0.7, 1]]           # a negative coordinate -> zero-centred
+    q, d, zc = O.q_points_highsymm(nodes, G)
+    assert q.shape == (4 + 2 + 3 + 1, 3)
+    x = q / (2.0 * np.pi)
+    assert np.allclose(x[:4], [[0, k / 8, k / 8] for k in (1, 2, 3, 4)], atol=1e-15)
+    a, b = np.array([0.0, 0.5, 0.5]), np.array([0.6, 0.7, 0.8])
+    assert np.allclose(x[4:6], [a + (b - a) * k / 2 for k in (1, 2)], atol=1e-14)
+    c = np.array([0.9, 0.6, 0.7])
+    assert np.allclose(x[6:9], [b + (c - b) * k / 3 for k in (1, 2, 3)], atol=1e-14)
+    assert np.allclose(x[9], [0.9, -0.6, 0.7], atol=1e-14)
+    L = 2.0 * np.pi * np.array([np.linalg.norm([0, 0.5, 0.5]), np.linalg.norm(b - a), np.linalg.norm(c - b), 1.2])
+    cum = np.cumsum(L)
+    assert np.allclose(d[[3, 5, 8, 9]], cum, rtol=1e-14)
+    assert np.allclose(d[:4], L[0] * np.arange(1, 5) / 4, rtol=1e-14)
+    assert np.all(np.diff(d) > 0)
+    assert zc.tolist() == [True] * 6 + [False] * 3 + [True]

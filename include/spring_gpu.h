@@ -318,6 +318,12 @@ int spr_pair_scores(spr_ctx *ctx, spr_mlp *m, int n, const int *q0_index, double
 int spr_kernel_time(spr_ctx *ctx, int *launches, double *ms);
 int spr_kernel_time_reset(spr_ctx *ctx);
 
+/* The launch configuration the context's last work-item launch used: its kernel variant and tile
+ * shape as text (shape, capacity cap bytes, NUL-terminated and truncated to fit), the launch mode
+ * (0: one workgroup per item; 1, 2, 4: persistent grids of that many workgroups per CU) and the
+ * work units per item.  Diagnostics only; no counterpart in the reference. */
+int spr_last_launch(spr_ctx *ctx, char *shape, int cap, int *mode, int *units_per_item);
+
 #ifdef __cplusplus
 }
 #endif

@@ -152,6 +152,7 @@ int run_items(Ctx &c, ItemLaunch &L, int q0_first, int q0_count, int it0, double
     int rc;
     c.last_upi = L.upi;
     c.last_shape = L.shape;
+    c.last_name = L.name;
     if (L.dyn > 64 * 1024)
         SPR_HIP(hipFuncSetAttribute(L.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.dyn));
     // ---- launch mode: env override, tuned value, or default (persistent 2 x CUs) ----

@@ -1,0 +1,647 @@
+// Three-phonon Gaussian linewidths on the matrix cores for the 10-atom cell (Ndof = 30), FP32 and
+// split BF16: the same per-atom GEMM chain as k_lw3_mm.hip (FT, then U1 = T E2, U2 = U1 E1,
+// V += U2 e0 per basis atom mu; ScatterMatEl, ThirdOrder_m.f90:278-386, factorised), with the
+// work split among the waves by s2 instead of by s2 blocks of separate work units:
+//
+//   - a unit is a whole item (up to 64 pairs of one q0, all s2): the phase sum T of each atom is
+//     formed once per pair;
+//   - four waves, one per SIMD (V of a pair takes half of the register file); GEMM B (U1 = T E2,
+//     all s2) is shared (wave w computes the 6 tiles of N tile w) and written to LDS as the A
+//     operand of GEMM C;
+//   - wave w owns s2 in [8w, 8w + 8): GEMM C (rows (alpha, s2 local), 24) and GEMM D (rows
+//     (s1, s2 local), 240) run on its own rows only, with U2 in a private LDS slice and V of its s2
+//     (240 values per lane) in its accumulator registers across the atoms;
+//   - two barriers per atom: [GEMM B(mu) + staging of the next atom's FC, sites, phases] |
+//     [FT(mu + 1) + GEMM C(mu) + GEMM D(mu)];
+//   - the phase sum runs over a schedule built once per context: per atom the outputs of its
+//     non-empty (nu, eta) blocks sorted by cluster count (the lanes of a wave loop over equal
+//     counts), then the outputs the previous atom wrote and this one does not (cleared);
+//   - the eigenvector fragments of GEMMs B and C stay in registers for the whole pair and GEMM C
+//     loads its A fragments once per atom for all four N tiles.
+//
+// After the last atom each wave applies the Gaussian delta to its V (W = |V|^2 / (w0 w1 w2) ...;
+// CalculateLinewidth, ThirdOrder_m.f90:514-537) exactly as the per-atom kernel's FP32 delta
+// stage does: arguments in FP64, triplets beyond sqrt(100) sigma of every resonance dropped, terms
+// in FP32 summed per pair and then into a compensated FP32 sum per lane, and the per-s0 totals
+// reduced in FP64 in a fixed order.
+#include <algorithm>
+#include <vector>
+
+#include "k_lw3_mm.h"
+
+namespace spr {
+
+template <int MODE, int NB> struct MxShape {
+    static_assert(MODE != MM_F64, "FP32 / split BF16 only");
+    using Sh = MmShape<MODE, NB, 0>;   // K padding, pitches and fragments of the per-atom kernel
+    static constexpr int ND = 3 * NB, K2 = 2 * ND, KP = Sh::KP, PA = Sh::PA, PE = Sh::PE, S = Sh::S;
+    static constexpr int NW = 4;                       // waves, one per SIMD
+    static constexpr int NT = NW * 64;                 // threads
+    static constexpr int SBW = 8;                      // s2 per wave
+    static constexpr int MT_B = 6;                     // GEMM B: rows alpha * 32 + b
+    static constexpr int TROWS = MT_B * 16;            // T rows; b >= Ndof rows stay zero
+    static constexpr int U1ROWS = 3 * 32;              // U1 rows alpha * 32 + s2
+    static constexpr int MT_C = (3 * SBW + 15) / 16;   // GEMM C: rows alpha * 8 + s2 local
+    static constexpr int MT_D = (ND * SBW + 15) / 16;  // GEMM D: rows s1 * 8 + s2 local
+    static constexpr int U2ROWS = MT_D * 16;
+    static constexpr int NE = NB * NB * 27;            // FT outputs per atom
+    static_assert(ND <= NW * SBW && ND <= 32 && MT_C == 2 && KP == 64, "built for 27 <= Ndof <= 32");
+};
+
+// LDS byte offsets (kernel and host-side size check)
+struct MxLds {
+    size_t t, u1, u2, e1, e2, e0, p, fc, tup, sched, ph, mode, modef, m1, m2, total;
+};
+
+template <int MODE, int NB>
+__host__ __device__ inline MxLds mx_lds(int npmax, int nsched, int nsite, int L) {
+    using X = MxShape<MODE, NB>;
+    MxLds o;
+    size_t b = 0;
+    auto take = [&](size_t n) {
+        const size_t at = b;
+        b = (b + n + 15) & ~(size_t)15;
+        return at;
+    };
+    o.t = take((size_t)X::TROWS * X::PA * 4);
+    o.u1 = take((size_t)X::U1ROWS * X::PA * 4);
+    o.u2 = take((size_t)X::NW * X::U2ROWS * 3 * 8);
+    o.e1 = take((size_t)X::ND * X::PE * 8);
+    o.e2 = take((size_t)X::ND * X::PE * 8);
+    o.e0 = take((size_t)X::ND * X::ND * 8);
+    o.p = take((size_t)npmax * 8);
+    o.fc = take((size_t)npmax * 27 * 4);
+    o.tup = take((size_t)npmax * 2 * 4);
+    o.sched = take((size_t)nsched * 4);
+    o.ph = take((size_t)L * 8);
+    o.mode = take((size_t)8 * X::ND * 8);
+    o.modef = take((size_t)7 * X::ND * 4);
+    o.m1 = take((size_t)nsite * 4);
+    o.m2 = take((size_t)nsite * 4);
+    o.total = b;
+    return o;
+}
+
+// A fragments of 16 x 16 tiles whose rows are given per lane (K contiguous from `row`), times a B
+// fragment built by Frag::load_b.  NTILE independent chains share the B fragment.
+template <int MODE, int NB, int NTILE>
+__device__ __forceinline__ void mx_rows_b(const float *const *row, const typename Frag<MODE, NB, 0>::BF &b,
+                                          v4f *acc) {
+    using X = MxShape<MODE, NB>;
+    const int g = (threadIdx.x & 63) >> 4;
+    if constexpr (MODE == MM_F32) {
+#pragma unroll
+        for (int i = 0; i < X::S / 4; ++i) {
+            float4 x[NTILE];
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) x[t] = reinterpret_cast<const float4 *>(row[t] + g * X::S)[i];
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[t].x, b.h[4 * i], acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[t].y, b.h[4 * i + 1], acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[t].z, b.h[4 * i + 2], acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[t].w, b.h[4 * i + 3], acc[t], 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+        for (int sl = 0; sl < X::KP / 32; ++sl) {
+            bf16x8 ah[NTILE], al[NTILE];
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) {
+                const float4 *p = reinterpret_cast<const float4 *>(row[t] + 32 * sl + 8 * g);
+                const float4 x0 = p[0], x1 = p[1];
+                const float xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const __bf16 h = (__bf16)xs[e];
+                    ah[t][e] = h;
+                    al[t][e] = (__bf16)(xs[e] - (float)h);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], b.h[sl], acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], b.lo[sl], acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t], b.h[sl], acc[t], 0, 0, 0);
+        }
+    }
+}
+
+// A fragments of two 16-row tiles over the whole K (rows given per lane), loaded once and used
+// with several B fragments
+template <int MODE, int NB> struct MxA2 {
+    using X = MxShape<MODE, NB>;
+    static constexpr int NSL = X::KP / 32;
+    float f[MODE == MM_F32 ? 2 : 1][MODE == MM_F32 ? X::S : 1];
+    bf16x8 h[MODE == MM_F32 ? 1 : 2][NSL], lo[MODE == MM_F32 ? 1 : 2][NSL];
+    __device__ __forceinline__ void load(const float *const *row) {
+        const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            if constexpr (MODE == MM_F32) {
+#pragma unroll
+                for (int i = 0; i < X::S / 4; ++i) {
+                    const float4 x = reinterpret_cast<const float4 *>(row[t] + g * X::S)[i];
+                    f[t][4 * i] = x.x;
+                    f[t][4 * i + 1] = x.y;
+                    f[t][4 * i + 2] = x.z;
+                    f[t][4 * i + 3] = x.w;
+                }
+            } else {
+#pragma unroll
+                for (int sl = 0; sl < NSL; ++sl) {
+                    const float4 *p = reinterpret_cast<const float4 *>(row[t] + 32 * sl + 8 * g);
+                    const float4 x0 = p[0], x1 = p[1];
+                    const float xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const __bf16 hh = (__bf16)xs[e];
+                        h[t][sl][e] = hh;
+                        lo[t][sl][e] = (__bf16)(xs[e] - (float)hh);
+                    }
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ void mma(const typename Frag<MODE, NB, 0>::BF &b, v4f *acc) const {
+        if constexpr (MODE == MM_F32) {
+#pragma unroll
+            for (int k = 0; k < X::S; ++k)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(f[t][k], b.h[k], acc[t], 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int sl = 0; sl < NSL; ++sl) {
+#pragma unroll
+                for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h[t][sl], b.h[sl], acc[t], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h[t][sl], b.lo[sl], acc[t], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[t][sl], b.h[sl], acc[t], 0, 0, 0);
+            }
+        }
+    }
+};
+
+template <int MODE, int NB>
+__device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
+    using X = MxShape<MODE, NB>;
+    using F = Frag<MODE, NB, 0>;
+    using Z = cplx_t<float>;
+    constexpr int ND = X::ND, PA = X::PA, PE = X::PE;
+
+    extern __shared__ __align__(16) unsigned char smem[];
+    const MxLds o = mx_lds<MODE, NB>(A.npmax, A.ft_nmax, A.nsite, A.L);
+    float *sT = reinterpret_cast<float *>(smem + o.t);      // T rows alpha * 32 + b, K = (c, p)
+    float *sU1 = reinterpret_cast<float *>(smem + o.u1);    // U1 rows alpha * 32 + s2, K = (b, p)
+    Z *sE1 = reinterpret_cast<Z *>(smem + o.e1);            // E1[s1][b], pitch PE
+    Z *sE2 = reinterpret_cast<Z *>(smem + o.e2);            // E2[s2][c], pitch PE
+    Z *sE0 = reinterpret_cast<Z *>(smem + o.e0);            // e0[s0][a]
+    Z *sP = reinterpret_cast<Z *>(smem + o.p);              // cluster phases of one atom
+    float *sFC = reinterpret_cast<float *>(smem + o.fc);    // FC of one atom
+    int *sTup = reinterpret_cast<int *>(smem + o.tup);      // site pairs (N1, N2) of its clusters
+    Z *sPh = reinterpret_cast<Z *>(smem + o.ph);
+    double *sMode = reinterpret_cast<double *>(smem + o.mode);   // w0 iw0f | w1 n1 iw1 | w2 n2 iw2 | -
+    float *sModeF = reinterpret_cast<float *>(smem + o.modef);   // n1 iw1 n2 iw2 w0 w1 w2 (FP32)
+    int *sM1 = reinterpret_cast<int *>(smem + o.m1);
+    int *sM2 = reinterpret_cast<int *>(smem + o.m2);
+    unsigned *sSched = reinterpret_cast<unsigned *>(smem + o.sched);   // phase-sum schedule of one atom
+    const int *schOff = A.ft_sched;                                     // (Nb + 1) atom offsets
+    const unsigned *sch = reinterpret_cast<const unsigned *>(A.ft_sched + NB + 1);
+    double *sRed = reinterpret_cast<double *>(smem + o.t);   // end of unit only
+
+    constexpr int NT = X::NT;
+    const int tid = threadIdx.x, l = tid & 63, g = l >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    Z *sU2 = reinterpret_cast<Z *>(smem + o.u2) + (size_t)w * X::U2ROWS * 3;   // this wave's U2[(s1, s2l)][alpha]
+    const int qi = A.item_q0[unit];
+    const int i0 = A.q0_grid[qi];
+    const int cnt = A.item_cnt[unit];
+    const int poff = A.item_off[unit];
+    const float *fc = reinterpret_cast<const float *>(A.fc);
+    const Z *phase = reinterpret_cast<const Z *>(A.phase);
+    const Z *evec = reinterpret_cast<const Z *>(A.evec);
+    __syncthreads();   // the previous unit of a persistent workgroup is done with LDS
+
+    // ---- per unit: zero the operand buffers (padding stays zero), phase table, q0 modes ----
+    for (int i = tid; i < (int)((o.e1 - o.t) / 16); i += NT)
+        reinterpret_cast<uint4 *>(smem + o.t)[i] = make_uint4(0u, 0u, 0u, 0u);   // sT, sU1, sU2
+    for (int i = tid; i < ND * PE; i += NT) {
+        if (i % PE >= ND) {
+            sE1[i] = Z{0.f, 0.f};
+            sE2[i] = Z{0.f, 0.f};
+        }
+    }
+    for (int i = tid; i < A.L; i += NT) sPh[i] = phase[i];
+    for (int i = tid; i < ND * ND; i += NT) sE0[i] = evec[(size_t)i0 * ND * ND + i];
+    if (tid < ND) {
+        sMode[tid] = A.omega[(size_t)i0 * ND + tid];
+        sMode[ND + tid] = A.inv_omega[(size_t)i0 * ND + tid] * A.factor;
+        sModeF[4 * ND + tid] = (float)A.omega[(size_t)i0 * ND + tid];
+    }
+
+    // GEMM B: N tile w of (s2, q), all six M tiles of (alpha, b); this lane's A row in tile 0 and
+    // its U1 store base (row alpha * 32 + s2, column 2 b + q)
+    const int npB = w * 16 + (l & 15);
+    const int s2B = npB >> 1, qB = npB & 1;
+    const float *rowB = sT + (l & 15) * PA;
+    float *u1st = sU1 + s2B * PA + 8 * g + qB;
+    // GEMM C: rows (alpha, s2 local) = 16 t + l % 16 of this wave (t = 0, 1); alpha = 3 reads a
+    // zero row of T
+    const float *rowC[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int r = 16 * t + (l & 15), al = r >> 3, s2 = 8 * w + (r & 7);
+        rowC[t] = al < 3 ? sU1 + (al * 32 + s2) * PA : sT + 31 * PA;   // T row 31 (b = 31) is zero
+    }
+    // GEMM C stores: U2 float ((s1 * 8 + s2l) * 3 + alpha) * 2 + q
+    float *u2st = reinterpret_cast<float *>(sU2) + (((l & 15) >> 1) * 8 + 4 * (g & 1)) * 6 + (g >> 1) * 2 + (l & 1);
+    const int qD = l & 1;
+    KSum<float> accf[4];
+
+    // FC, cluster sites and schedule of atom mu -> sFC, sTup, sSched (direct global-to-LDS loads,
+    // 64 values per wave and chunk; wave w takes chunks w, w + 4, ...)
+    auto stage = [&](int mu) {
+        const int u0 = A.blk_start[mu * NB * NB], nu = A.blk_start[(mu + 1) * NB * NB] - u0;
+        for (int c = w; c * 64 < nu * 27; c += X::NW)
+            if (c * 64 + l < nu * 27) __builtin_amdgcn_global_load_lds(fc + u0 * 27 + c * 64 + l, sFC + c * 64, 4, 0, 0);
+        for (int c = w; c * 64 < nu * 2; c += X::NW)
+            if (c * 64 + l < nu * 2)
+                __builtin_amdgcn_global_load_lds(A.tup_site + 2 * u0 + c * 64 + l, sTup + c * 64, 4, 0, 0);
+        const int s0 = schOff[mu], ns = schOff[mu + 1] - s0;
+        for (int c = w; c * 64 < ns; c += X::NW)
+            if (c * 64 + l < ns)
+                __builtin_amdgcn_global_load_lds(sch + s0 + c * 64 + l, sSched + c * 64, 4, 0, 0);
+    };
+    // P(u) = e^{2 pi i (m1(N1) + m2(N2)) / L} of the clusters whose sites this wave staged (chunk c
+    // holds clusters 32 c .. 32 c + 31), once its own staging has landed: no barrier needed
+    auto phases = [&](int mu) {
+        const int nu = A.blk_start[(mu + 1) * NB * NB] - A.blk_start[mu * NB * NB];
+        for (int c = w; c * 64 < nu * 2; c += X::NW) {
+            const int u = c * 32 + l;
+            if (l < 32 && u < nu) {
+                int m = sM1[sTup[2 * u]] + sM2[sTup[2 * u + 1]];
+                m -= m >= A.L ? A.L : 0;
+                sP[u] = sPh[m];
+            }
+        }
+    };
+    // T[(alpha, 3 nu + beta)][2 (3 eta + gamma) + p] of atom mu from its schedule: entry = block
+    // (nu, eta) | component j << 7 | first cluster (within the atom) << 12 | cluster count << 23;
+    // lanes run along the components of a block (contiguous FC reads); count 0 clears the output
+    auto ft = [&](int mu) {
+        const int ns = schOff[mu + 1] - schOff[mu];
+        for (int e = tid; e < ns; e += NT) {
+            const unsigned code = sSched[e];
+            const int bl = code & 127, j = (code >> 7) & 31, uo = (code >> 12) & 2047, nc = code >> 23;
+            const int nu = bl / NB, eta = bl % NB;
+            const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
+            const float *fj = sFC + uo * 27 + j;
+            const Z *pj = sP + uo;
+            float re = 0.f, im = 0.f;
+#pragma unroll 4
+            for (int u = 0; u < nc; ++u) {
+                const float f = fj[u * 27];
+                const Z ph = pj[u];
+                re = fmaf(f, ph.x, re);
+                im = fmaf(f, ph.y, im);
+            }
+            float *dst = sT + (al * 32 + 3 * nu + be) * PA + 2 * (3 * eta + ga);
+            dst[0] = re;
+            dst[1] = im;
+        }
+    };
+
+    for (int pi = 0; pi < cnt; ++pi) {
+        const int2 pr = A.pairs[poff + pi];
+        const int q1 = pr.x, q2 = pr.y;
+        __syncthreads();   // the previous pair is done with sE1, sE2, sMode, sM1, sM2, sP, sFC, sT
+        // the previous pair's delta stage staged its tiles over T: restore T's zero padding
+        for (int i = tid; i < X::NW * 1280 / 4; i += NT)
+            reinterpret_cast<uint4 *>(smem + o.t)[i] = make_uint4(0u, 0u, 0u, 0u);
+        for (int i = tid; i < ND * ND; i += NT) {
+            const int s = i / ND, c = i % ND;
+            sE1[s * PE + c] = evec[(size_t)q1 * ND * ND + i];
+            sE2[s * PE + c] = evec[(size_t)q2 * ND * ND + i];
+        }
+        if (tid < ND) {
+            sMode[2 * ND + tid] = A.omega[(size_t)q1 * ND + tid];
+            sMode[3 * ND + tid] = A.nbe[(size_t)q1 * ND + tid];
+            sMode[4 * ND + tid] = A.inv_omega[(size_t)q1 * ND + tid];
+            sMode[5 * ND + tid] = A.omega[(size_t)q2 * ND + tid];
+            sMode[6 * ND + tid] = A.nbe[(size_t)q2 * ND + tid];
+            sMode[7 * ND + tid] = A.inv_omega[(size_t)q2 * ND + tid];
+            sModeF[tid] = (float)A.nbe[(size_t)q1 * ND + tid];
+            sModeF[ND + tid] = (float)A.inv_omega[(size_t)q1 * ND + tid];
+            sModeF[2 * ND + tid] = (float)A.nbe[(size_t)q2 * ND + tid];
+            sModeF[3 * ND + tid] = (float)A.inv_omega[(size_t)q2 * ND + tid];
+            sModeF[5 * ND + tid] = (float)A.omega[(size_t)q1 * ND + tid];
+            sModeF[6 * ND + tid] = (float)A.omega[(size_t)q2 * ND + tid];
+        }
+        {
+            const int k10 = q1 % A.m0, k11 = (q1 / A.m0) % A.m1, k12 = q1 / (A.m0 * A.m1);
+            const int k20 = q2 % A.m0, k21 = (q2 / A.m0) % A.m1, k22 = q2 / (A.m0 * A.m1);
+            for (int s = tid; s < A.nsite; s += NT) {
+                const int4 c = A.site[s];
+                sM1[s] = (k10 * c.x + k11 * c.y + k12 * c.z) % A.L;
+                sM2[s] = (k20 * c.x + k21 * c.y + k22 * c.z) % A.L;
+            }
+        }
+        stage(0);
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's staging has landed
+        __syncthreads();                 // sM1, sM2 are complete
+        phases(0);
+        __syncthreads();
+        ft(0);
+        // eigenvector fragments of GEMMs B and C, the same for every atom of the pair
+        const typename F::BF bE2 = F::load_b(sE2, w * 16);
+        typename F::BF bE1[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) bE1[n] = F::load_b(sE1, n * 16);
+        v4f V[X::MT_D][4];
+#pragma unroll
+        for (int i = 0; i < X::MT_D; ++i)
+#pragma unroll
+            for (int n = 0; n < 4; ++n) V[i][n] = v4f{0.f, 0.f, 0.f, 0.f};
+        __syncthreads();   // T of atom 0 is complete
+
+#pragma unroll 1
+        for (int mu = 0; mu < NB; ++mu) {
+            // ---- GEMM B (shared): U1[(alpha, b)][(s2, q)] = T x E2' -> sU1[(alpha, s2)][(b, q)];
+            //      the next atom's FC, sites and schedule are staged meanwhile ----
+            if (mu + 1 < NB) stage(mu + 1);
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {   // M tiles 3 hh .. 3 hh + 2
+                const float *rb[3];
+#pragma unroll
+                for (int t = 0; t < 3; ++t) rb[t] = rowB + (3 * hh + t) * 16 * PA;
+                v4f d[3] = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
+                mx_rows_b<MODE, NB, 3>(rb, bE2, d);
+                // output rows (3 hh + t) 16 + 4 g + r: alpha = (3 hh + t) / 2, b = 16 ((3 hh + t) & 1) + 4 g + r
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    const int mt = 3 * hh + t, al = mt >> 1, bh = 16 * (mt & 1);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (s2B < ND && bh + 4 * g + r < ND) u1st[al * 32 * PA + 2 * (bh + r)] = d[t][r];
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);   // this wave's staging of the next atom has landed
+            if (mu + 1 < NB) phases(mu + 1);
+            __syncthreads();
+            // ---- FT of the next atom (shared) ----
+            if (mu + 1 < NB) ft(mu + 1);
+            // ---- GEMM C (this wave's s2): U2[(alpha, s2l)][(s1, q)] = U1' x E1' -> sU2[(s1, s2l)][alpha] ----
+            {
+                MxA2<MODE, NB> a;
+                a.load(rowC);
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    v4f d[2] = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
+                    a.mma(bE1[n], d);
+                    // output rows 16 t + 4 g + r: alpha = 2 t + g / 2, s2l = 4 (g & 1) + r; column
+                    // (s1, q) with s1 = 8 n + (l % 16) / 2
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        if (2 * t + (g >> 1) < 3 && 8 * n + ((l & 15) >> 1) < ND)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) u2st[n * 8 * 8 * 6 + t * 4 + r * 6] = d[t][r];
+                }
+            }
+            mm_wave_sync();   // this wave's U2 stores precede its GEMM D loads
+            // ---- GEMM D (this wave's s2): V[(s1, s2l)][(s0, q)] += U2[.][(alpha, p)] E0'[(alpha, p)][(s0, q)] ----
+            {
+                float b0[4], b1[4];
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    const int np = n * 16 + (l & 15);
+                    const int s0 = np >> 1, q = np & 1;
+                    b0[n] = 0.f;
+                    b1[n] = 0.f;
+                    if (g < 3 && s0 < ND) {
+                        const Z e = sE0[s0 * ND + 3 * mu + g];
+                        b0[n] = q ? e.y : e.x;
+                        b1[n] = q ? e.x : -e.y;
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < X::MT_D; ++i) {
+                    const int m = i * 16 + (l & 15);
+                    Z a = Z{0.f, 0.f};
+                    if (g < 3) a = sU2[m * 3 + g];
+#pragma unroll
+                    for (int n = 0; n < 4; ++n) V[i][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b0[n], V[i][n], 0, 0, 0);
+#pragma unroll
+                    for (int n = 0; n < 4; ++n) V[i][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b1[n], V[i][n], 0, 0, 0);
+                }
+            }
+            mm_wave_sync();   // GEMM D's loads of U2 precede the next atom's GEMM C stores
+            __syncthreads();   // T of atom mu + 1 is complete; sU1 is free
+        }
+
+        // ---- Gaussian-delta weights of this wave's V (sT is free: every wave is past GEMM B of
+        //      the last atom).  Tile i, rows 4 g + r, column (s0, q): s1 = 2 i + g / 2,
+        //      s2 = 8 w + 4 (g & 1) + r; lanes l and l^1 hold Re and Im of the same element and the
+        //      lane with q = qD owns rows r = 2 qD + k ("its elements").  Per tile each wave
+        //      (1) stages the tile in LDS, (2) tests its elements in FP32 with a margin over the
+        //      FP32 rounding of the frequencies, (3) packs the candidates into a list (ballot
+        //      order), (4) evaluates the list with every lane -- arguments in FP64, a triplet with
+        //      all three Gaussians below e^-50 of their peak adds nothing and is dropped, the rest in
+        //      FP32 -- writing each term over the staged value of its element, and (5) each owner
+        //      adds its elements' terms in element order, so the sums do not depend on the packing ----
+        const double c0 = 1.0 / (A.sigma * 2.5066282746310002);   // 1/(sigma sqrt(2 pi))
+        const double hinv = 0.5 / (A.sigma * A.sigma);
+        const float hneg = (float)(-hinv);
+        const double dcut = A.sigma * 10.0;                 // sqrt(2 x 50) sigma
+        const float dcutf = (float)dcut + 1e-3f;            // FP32 pre-test: a superset of the FP64 test
+        float *sV = reinterpret_cast<float *>(smem + o.t) + w * 1280;   // 4 x 64 x 4 staged values
+        unsigned short *sList = reinterpret_cast<unsigned short *>(sV + 1024);   // up to 512 codes
+        float pairf[4] = {0.f, 0.f, 0.f, 0.f};
+        float w0f[4], w2f[2];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            const int s0 = 8 * n + ((l & 15) >> 1);
+            w0f[n] = s0 < ND ? sModeF[4 * ND + s0] : 1e30f;   // 1e30: never near
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int s2 = 8 * w + 4 * (g & 1) + 2 * qD + k;
+            w2f[k] = s2 < ND ? sModeF[6 * ND + s2] : 1e30f;
+        }
+#pragma unroll
+        for (int i = 0; i < X::MT_D; ++i) {
+            const int s1 = 2 * i + (g >> 1);
+            const float w1f = sModeF[5 * ND + s1];
+            // (1) stage
+#pragma unroll
+            for (int n = 0; n < 4; ++n) *reinterpret_cast<v4f *>(sV + (n * 64 + l) * 4) = V[i][n];
+            mm_wave_sync();
+            // (2) + (3) test and pack
+            int cnt = 0;
+            unsigned act = 0;
+#pragma unroll
+            for (int n = 0; n < 4; ++n)
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const float ap = w1f + w2f[k], am = w1f - w2f[k];
+                    const float dm = fminf(fabsf(w0f[n] - ap), fminf(fabsf(w0f[n] + am), fabsf(w0f[n] - am)));
+                    const bool near = dm < dcutf;
+                    const unsigned long long msk = __ballot(near);
+                    if (near) {
+                        const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32),
+                                                                              __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
+                        sList[pos] = (unsigned short)((n << 7) | (l << 1) | k);
+                        act |= 1u << (2 * n + k);
+                    }
+                    cnt += __popcll(msk);
+                }
+            mm_wave_sync();
+            // (4) evaluate: the term replaces the staged value of its element (read by this lane only)
+            for (int idx = l; idx < cnt; idx += 64) {
+                const int code = sList[idx];
+                const int n = code >> 7, lo = (code >> 1) & 63, k = code & 1;
+                const int r = 2 * (lo & 1) + k;
+                const float x = sV[(n * 64 + lo) * 4 + r], y = sV[(n * 64 + (lo ^ 1)) * 4 + r];
+                const int s0 = 8 * n + ((lo & 15) >> 1);
+                const int s1e = 2 * i + (lo >> 5);
+                const int s2 = 8 * w + 4 * ((lo >> 4) & 1) + r;
+                const double w0 = sMode[s0], w1 = sMode[2 * ND + s1e], w2 = sMode[5 * ND + s2];
+                const double ap = w1 + w2, am = w1 - w2;
+                const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
+                float term = 0.f;
+                if (fabs(d1) < dcut || fabs(d2) < dcut || fabs(d3) < dcut) {
+                    const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
+                    const float n1 = sModeF[s1e], iw1 = sModeF[ND + s1e];
+                    const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
+                    const float W = (x * x + y * y) * (iw1 * iw2);
+                    const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
+                    const float sm = A.sel ? (float)A.sel[ND + s1e * ND + s2] : 1.f;
+                    term = W * sm * ((n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3));
+                }
+                sV[(n * 64 + lo) * 4 + r] = term;
+            }
+            mm_wave_sync();
+            // (5) the owner adds its elements' terms in element order
+#pragma unroll
+            for (int n = 0; n < 4; ++n)
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+                    if (act & (1u << (2 * n + k))) pairf[n] += sV[(n * 64 + l) * 4 + 2 * qD + k];
+            mm_wave_sync();   // this wave has read the tile before the next one overwrites it
+        }
+        const double wpair = q1 == q2 ? 1.0 : 2.0;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            const int s0 = (n * 16 + (l & 15)) >> 1;
+            if (s0 < ND) accf[n].add(pairf[n] * (float)(wpair * c0 * sMode[ND + s0]));
+        }
+    }
+    // ---- reduce per s0 in a fixed order: lane l of every wave holds s0 = (16 n + l % 16) / 2 ----
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < 4; ++n) sRed[n * NT + tid] = accf[n].value();
+    __syncthreads();
+    if (tid < ND) {
+        const int s = tid;
+        const int n = (2 * s) >> 4, sj = (2 * s) & 15;
+        double tot = 0.0;
+        for (int ww = 0; ww < X::NW; ++ww)
+            for (int gg = 0; gg < 4; ++gg)
+                for (int q = 0; q < 2; ++q) tot += sRed[n * NT + ww * 64 + gg * 16 + sj + q];
+        A.ic.partial[(size_t)unit * ND + s] = A.sel ? tot * A.sel[s] : tot;
+    }
+}
+
+template <int MODE, int NB>
+__global__ __launch_bounds__(256) void k_lw3_mx(MmArgs A) {
+    // one unit per workgroup, or (persistent) units from the atomic counter until none is left
+    __shared__ int s_unit;
+    for (int it = 0;; ++it) {
+        int unit;
+        if (!A.ic.counter) {
+            if (it > 0) break;
+            unit = blockIdx.x;
+        } else {
+            __syncthreads();
+            if (threadIdx.x == 0) s_unit = atomicAdd(A.ic.counter, 1);
+            __syncthreads();
+            unit = s_unit;
+            if (unit >= A.ic.nitems) break;
+        }
+        mx_unit<MODE, NB>(A, unit);
+    }
+}
+
+// The phase-sum schedule of every atom (see the kernel's header); false when the FC3 layout does
+// not fit its entry format (2048 clusters per atom, 255 per block).
+static bool mx_schedule(Ctx &c, int nb) {
+    DevIFC &f = c.fc3;
+    if (f.ft_tried) return f.ft_ok;
+    f.ft_tried = true;
+    const std::vector<int> &bs = f.h_blk_start;
+    const int nb2 = nb * nb;
+    if ((int)bs.size() < nb * nb2 + 1 || nb2 > 128) return false;
+    std::vector<int> off(nb + 1), ent;
+    int nmax = 0;
+    for (int mu = 0; mu < nb; ++mu) {
+        off[mu] = (int)ent.size();
+        const int pm = (mu + nb - 1) % nb;   // the atom before (for atom 0: the last atom of the previous pair)
+        const int u0 = bs[mu * nb2];
+        std::vector<std::pair<int, unsigned>> v;
+        for (int bl = 0; bl < nb2; ++bl) {
+            const int b = mu * nb2 + bl, cnt = bs[b + 1] - bs[b], uo = bs[b] - u0;
+            const bool prev = bs[pm * nb2 + bl + 1] > bs[pm * nb2 + bl];
+            if (cnt > 255 || uo > 2047) return false;
+            if (cnt == 0 && !prev) continue;
+            for (int j = 0; j < 27; ++j)
+                v.push_back({cnt, (unsigned)bl | (unsigned)j << 7 | (unsigned)(cnt ? uo : 0) << 12 | (unsigned)cnt << 23});
+        }
+        std::sort(v.begin(), v.end(), [](const std::pair<int, unsigned> &x, const std::pair<int, unsigned> &y) {
+            return x.first != y.first ? x.first > y.first : x.second < y.second;
+        });
+        for (auto &e : v) ent.push_back((int)e.second);
+        nmax = std::max(nmax, (int)v.size());
+    }
+    off[nb] = (int)ent.size();
+    std::vector<int> h(off);
+    h.insert(h.end(), ent.begin(), ent.end());
+    if (hipMalloc((void **)&f.ft_sched, h.size() * sizeof(int)) != hipSuccess) {
+        f.ft_sched = nullptr;
+        return false;
+    }
+    if (hipMemcpy(f.ft_sched, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return false;
+    f.ft_nmax = std::max(nmax, 1);
+    f.ft_ok = true;
+    return true;
+}
+
+template <int MODE, int NB>
+static bool mx_launch_of_t(Ctx &c, MmArgs &A, int nitems, ItemLaunch &L) {
+    if (!mx_schedule(c, NB)) return false;
+    A.ft_sched = c.fc3.ft_sched;
+    A.ft_nmax = c.fc3.ft_nmax;
+    const size_t dyn = mx_lds<MODE, NB>(A.npmax, A.ft_nmax, A.nsite, A.L).total;
+    if (dyn > 160 * 1024 - 64) return false;
+    const void *fn = (const void *)k_lw3_mx<MODE, NB>;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess) return false;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, MxShape<MODE, NB>::NT, dyn) != hipSuccess || per_cu < 1)
+        return false;
+    L = ItemLaunch{fn, &A, sizeof(A), MxShape<MODE, NB>::NT, dyn, nitems, per_cu,
+                   std::string(MODE == MM_F32 ? "mx_f32" : "mx_bf16")};
+    L.upi = 1;
+    L.shape = "s2wave8";
+    return true;
+}
+
+bool mx_launch_of(Ctx &c, int mode, int nb, MmArgs &A, int nitems, ItemLaunch &L) {
+    if (nb != 10 || c.override_items) return false;
+    return mode == MM_F32 ? mx_launch_of_t<MM_F32, 10>(c, A, nitems, L)
+           : mode == MM_BF16 ? mx_launch_of_t<MM_BF16, 10>(c, A, nitems, L)
+                             : false;
+}
+
+}  // namespace spr

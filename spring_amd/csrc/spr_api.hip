@@ -163,6 +163,7 @@ static void free_ifc(DevIFC &d) {
     (void)hipFree(d.tup_site_p);
     (void)hipFree(d.fc64_p);
     (void)hipFree(d.fc32_p);
+    (void)hipFree(d.ft_sched);
     d = DevIFC();
 }
 
@@ -462,6 +463,20 @@ extern "C" int spr_kernel_time(spr_ctx *h, int *launches, double *ms) {
     if (!h) return SPR_E_ARG;
     if (launches) *launches = h->c.launches;
     if (ms) *ms = h->c.kernel_ms;
+    return SPR_OK;
+}
+
+extern "C" int spr_last_launch(spr_ctx *h, char *shape, int cap, int *mode, int *units_per_item) {
+    if (!h) return SPR_E_ARG;
+    const Ctx &c = h->c;
+    if (shape && cap > 0) {
+        const std::string t = c.last_name + (c.last_shape.empty() ? "" : ":" + c.last_shape);
+        const size_t n = std::min(t.size(), (size_t)cap - 1);
+        std::memcpy(shape, t.data(), n);
+        shape[n] = 0;
+    }
+    if (mode) *mode = c.last_mode;
+    if (units_per_item) *units_per_item = c.last_upi;
     return SPR_OK;
 }
 

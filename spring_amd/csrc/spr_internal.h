@@ -76,6 +76,11 @@ struct DevIFC {
     int *tup_site_p = nullptr;
     double *fc64_p = nullptr;
     float *fc32_p = nullptr;
+    // phase-sum schedule of the s2-per-wave kernel (k_lw3_mx.hip), built on first use: per basis
+    // atom the outputs of its non-empty blocks sorted by cluster count, then the outputs to clear
+    int *ft_sched = nullptr;
+    int ft_nmax = 0;
+    bool ft_tried = false, ft_ok = false;
 };
 
 constexpr int kClusterBatch = 4;
@@ -134,6 +139,7 @@ struct Ctx {
     int last_mode = -1;            // launch mode of the last item launch (0: one WG per item)
     int last_upi = 1;              // work units per item of the last item launch
     std::string last_shape;        // tile shape of the last item launch
+    std::string last_name;         // kernel variant of the last item launch
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
     double *ipartial = nullptr;    // (n_items, nd) item partials (stable: graphs keep the pointer)
     double *wout = nullptr;        // W parity output (also W of one q0 for the tetrahedron path)

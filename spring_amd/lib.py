@@ -18,7 +18,7 @@ PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
 SYMBOLS = [
     "spr_ctx_create", "spr_ctx_destroy", "spr_last_error", "spr_v3_w", "spr_lw3",
     "spr_lw3_all", "spr_lw4", "spr_lw4_all", "spr_kappa_rta", "spr_kernel_time",
-    "spr_kernel_time_reset", "spr_lw3_pairs", "spr_v3_w_pairs", "spr_mlp_create", "spr_mlp_destroy",
+    "spr_kernel_time_reset", "spr_last_launch", "spr_lw3_pairs", "spr_v3_w_pairs", "spr_mlp_create", "spr_mlp_destroy",
     "spr_mlp_num_params", "spr_mlp_get_params", "spr_mlp_set_params", "spr_mlp_grad",
     "spr_mlp_train", "spr_mlp_predict", "spr_lw3_tetra", "spr_lw3_tetra_all",
     "spr_lw4_tetra", "spr_lw4_tetra_all", "spr_lw3_kappa_all", "spr_mlp_score",
@@ -339,6 +339,13 @@ class GpuContext:
         ms = ct.c_double()
         self._check(self.lib.spr_kernel_time(self.h, ct.byref(n), ct.byref(ms)), "spr_kernel_time")
         return n.value, ms.value
+
+    def last_launch(self):
+        """(variant:shape, launch mode, units per item) of the last work-item launch."""
+        buf = ct.create_string_buffer(128)
+        mode, upi = ct.c_int(), ct.c_int()
+        self._check(self.lib.spr_last_launch(self.h, buf, 128, ct.byref(mode), ct.byref(upi)), "spr_last_launch")
+        return buf.value.decode(), mode.value, upi.value
 
     def kernel_time_reset(self):
         self._check(self.lib.spr_kernel_time_reset(self.h), "spr_kernel_time_reset")

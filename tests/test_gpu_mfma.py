@@ -138,3 +138,43 @@ def test_launch_modes_and_tile_shapes_at_a_tuned_size(rocksalt10, monkeypatch, t
     tune = (tmp_path / "tune.txt").read_text()
     assert "mm_f64|shape" in tune and "mm_f32" in tune
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_s2_per_wave_kernel_10atom(ten, rocksalt10, monkeypatch, prec):
+    """The s2-per-wave FP32 / BF16 kernel (k_lw3_mx, tile-shape candidate 1 of the 10-atom launch,
+    one unit per item, the phase sum once per pair) against the oracle at 2^3; against the per-atom
+    kernel (candidate 0) at 3^3 with time reversal (q1 == q2 pairs, items of every size) with and
+    without a branch selection rule: the two differ only by FP32 rounding."""
+    from spring_amd import mesh
+    from spring_amd.lib import GpuContext, get_phonon
+    qm, ph, ctx, ref = ten
+    monkeypatch.setenv("SPR_TILE_SHAPE", "1")
+    lw = ctx.lw3_all(0.1, prec)
+    name, _, upi = ctx.last_launch()
+    assert name.startswith("mx_") and upi == 1, name
+    err = _err(lw, ref)
+    print(f"{prec} s2-per-wave max/mean rel", err.max(), err.mean(), flush=True)
+    assert err.max() < 1e-3
+    if prec == "bf16":
+        assert err.mean() < 1e-4
+    m = rocksalt10
+    q3 = mesh.make_mesh(m.crystal, (3, 3, 3), time_rev=1)
+    p3 = get_phonon(m.crystal, m.fc2, q3.q_cart, q3.q_int, 300.0)
+    c3 = GpuContext(m, q3, p3)
+    tol = 2e-5 if prec == "fp32" else 2e-3
+    for rules in ("DDD", "ADD"):
+        c3.set_branch_rules(3, rules)
+        monkeypatch.setenv("SPR_TILE_SHAPE", "1")
+        a = c3.lw3_all(0.1, prec)
+        assert c3.last_launch()[0].startswith("mx_")
+        monkeypatch.setenv("SPR_TILE_SHAPE", "0")
+        b = c3.lw3_all(0.1, prec)
+        assert c3.last_launch()[0].startswith("mm_")
+        assert np.abs(b).max() > 0
+        if rules == "ADD":
+            assert np.all(a[:, 3:] == 0.0)
+        e = _err(a, b)
+        print(f"{prec} {rules}: s2-per-wave vs per-atom max rel {e.max():.2e}", flush=True)
+        assert e.max() < tol, rules
+    c3.close()

@@ -5,19 +5,21 @@
 //
 //   - a unit is a whole item (up to 64 pairs of one q0, all s2): the phase sum T of each atom is
 //     formed once per pair;
-//   - four waves, one per SIMD (V of a pair takes half of the register file); GEMM B (U1 = T E2,
-//     all s2) is shared (wave w computes the 6 tiles of N tile w) and written to LDS as the A
-//     operand of GEMM C;
-//   - wave w owns s2 in [8w, 8w + 8): GEMM C (rows (alpha, s2 local), 24) and GEMM D (rows
-//     (s1, s2 local), 240) run on its own rows only, with U2 in a private LDS slice and V of its s2
-//     (240 values per lane) in its accumulator registers across the atoms;
+//   - eight waves, two per SIMD; GEMM B (U1 = T E2, all s2) is shared (wave w computes three of
+//     the 6 x 4 output tiles, N tile w % 4) and written to LDS as the A operand of GEMM C;
+//   - wave w owns s2 in [4w, 4w + 4): GEMM C (rows (alpha, s2 local), 12) and GEMM D (rows
+//     (s1, s2 local), 120) run on its own rows only, with U2 in a private LDS slice and V of its s2
+//     (128 values per lane) in its accumulator registers across the atoms;
 //   - two barriers per atom: [GEMM B(mu) + staging of the next atom's FC, sites, phases] |
-//     [FT(mu + 1) + GEMM C(mu) + GEMM D(mu)];
+//     [FT(mu + 1) + GEMM C(mu) + GEMM D(mu)]; in the second segment the two waves of a SIMD take
+//     their parts in opposite orders (waves 0-3: phase sum first, waves 4-7: GEMMs first), so that
+//     the phase sum of one runs beside the matrix products of the other;
 //   - the phase sum runs over a schedule built once per context: per atom the outputs of its
 //     non-empty (nu, eta) blocks sorted by cluster count (the lanes of a wave loop over equal
-//     counts), then the outputs the previous atom wrote and this one does not (cleared);
-//   - the eigenvector fragments of GEMMs B and C stay in registers for the whole pair and GEMM C
-//     loads its A fragments once per atom for all four N tiles.
+//     counts), then one entry per block the previous atom wrote and this one does not (cleared);
+//   - in BF16 mode the producers of T and U1 (the phase sum, GEMM B) store the split operand
+//     (hi and lo bf16 planes), so the GEMMs load their A fragments without conversions; GEMM C
+//     loads its A fragment once per atom for all four N tiles.
 //
 // After the last atom each wave applies the Gaussian delta to its V (W = |V|^2 / (w0 w1 w2) ...;
 // CalculateLinewidth, ThirdOrder_m.f90:514-537) exactly as the per-atom kernel's FP32 delta
@@ -35,17 +37,25 @@ template <int MODE, int NB> struct MxShape {
     static_assert(MODE != MM_F64, "FP32 / split BF16 only");
     using Sh = MmShape<MODE, NB, 0>;   // K padding, pitches and fragments of the per-atom kernel
     static constexpr int ND = 3 * NB, K2 = 2 * ND, KP = Sh::KP, PA = Sh::PA, PE = Sh::PE, S = Sh::S;
-    static constexpr int NW = 4;                       // waves, one per SIMD
+    static constexpr int NW = 8;                       // waves, two per SIMD
     static constexpr int NT = NW * 64;                 // threads
-    static constexpr int SBW = 8;                      // s2 per wave
+    static constexpr int SBW = 4;                      // s2 per wave
     static constexpr int MT_B = 6;                     // GEMM B: rows alpha * 32 + b
     static constexpr int TROWS = MT_B * 16;            // T rows; b >= Ndof rows stay zero
     static constexpr int U1ROWS = 3 * 32;              // U1 rows alpha * 32 + s2
-    static constexpr int MT_C = (3 * SBW + 15) / 16;   // GEMM C: rows alpha * 8 + s2 local
-    static constexpr int MT_D = (ND * SBW + 15) / 16;  // GEMM D: rows s1 * 8 + s2 local
+    static constexpr int MT_C = (3 * SBW + 15) / 16;   // GEMM C: rows alpha * 4 + s2 local
+    static constexpr int MT_D = (ND * SBW + 15) / 16;  // GEMM D: rows s1 * 4 + s2 local
     static constexpr int U2ROWS = MT_D * 16;
     static constexpr int NE = NB * NB * 27;            // FT outputs per atom
-    static_assert(ND <= NW * SBW && ND <= 32 && MT_C == 2 && KP == 64, "built for 27 <= Ndof <= 32");
+    // GEMM operand buffers T and U1 (96 rows each): FP32 rows of pitch PA, or in BF16 mode the
+    // split operand x = hi + lo already formed by the producer, as a hi plane and a lo plane of
+    // bf16 rows of pitch PB
+    static constexpr bool SPLIT = MODE == MM_BF16;
+    static constexpr int PB = KP + 8;
+    static constexpr int ROWB = SPLIT ? PB * 2 : PA * 4;     // bytes per row
+    static constexpr int PLANE = 96 * ROWB;                  // bytes per plane
+    static constexpr int OPB = SPLIT ? 2 * PLANE : PLANE;    // bytes per buffer
+    static_assert(ND <= NW * SBW && ND <= 32 && MT_C == 1 && KP == 64, "built for 27 <= Ndof <= 32");
 };
 
 // LDS byte offsets (kernel and host-side size check)
@@ -63,8 +73,8 @@ __host__ __device__ inline MxLds mx_lds(int npmax, int nsched, int nsite, int L)
         b = (b + n + 15) & ~(size_t)15;
         return at;
     };
-    o.t = take((size_t)X::TROWS * X::PA * 4);
-    o.u1 = take((size_t)X::U1ROWS * X::PA * 4);
+    o.t = take((size_t)X::OPB);
+    o.u1 = take((size_t)X::OPB);
     o.u2 = take((size_t)X::NW * X::U2ROWS * 3 * 8);
     o.e1 = take((size_t)X::ND * X::PE * 8);
     o.e2 = take((size_t)X::ND * X::PE * 8);
@@ -82,69 +92,48 @@ __host__ __device__ inline MxLds mx_lds(int npmax, int nsched, int nsite, int L)
     return o;
 }
 
-// A fragments of 16 x 16 tiles whose rows are given per lane (K contiguous from `row`), times a B
-// fragment built by Frag::load_b.  NTILE independent chains share the B fragment.
-template <int MODE, int NB, int NTILE>
-__device__ __forceinline__ void mx_rows_b(const float *const *row, const typename Frag<MODE, NB, 0>::BF &b,
-                                          v4f *acc) {
+// Operand buffer element stores: one value, or two adjacent (k, k + 1); `at` is the byte
+// address of the element in the FP32 buffer or in the hi plane
+template <int MODE, int NB>
+__device__ __forceinline__ void mx_st1(unsigned char *at, float v) {
     using X = MxShape<MODE, NB>;
-    const int g = (threadIdx.x & 63) >> 4;
-    if constexpr (MODE == MM_F32) {
-#pragma unroll
-        for (int i = 0; i < X::S / 4; ++i) {
-            float4 x[NTILE];
-#pragma unroll
-            for (int t = 0; t < NTILE; ++t) x[t] = reinterpret_cast<const float4 *>(row[t] + g * X::S)[i];
-#pragma unroll
-            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[t].x, b.h[4 * i], acc[t], 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[t].y, b.h[4 * i + 1], acc[t], 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[t].z, b.h[4 * i + 2], acc[t], 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[t].w, b.h[4 * i + 3], acc[t], 0, 0, 0);
-        }
+    if constexpr (X::SPLIT) {
+        const __bf16 h = (__bf16)v;
+        *reinterpret_cast<__bf16 *>(at) = h;
+        *reinterpret_cast<__bf16 *>(at + X::PLANE) = (__bf16)(v - (float)h);
     } else {
-#pragma unroll
-        for (int sl = 0; sl < X::KP / 32; ++sl) {
-            bf16x8 ah[NTILE], al[NTILE];
-#pragma unroll
-            for (int t = 0; t < NTILE; ++t) {
-                const float4 *p = reinterpret_cast<const float4 *>(row[t] + 32 * sl + 8 * g);
-                const float4 x0 = p[0], x1 = p[1];
-                const float xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const __bf16 h = (__bf16)xs[e];
-                    ah[t][e] = h;
-                    al[t][e] = (__bf16)(xs[e] - (float)h);
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], b.h[sl], acc[t], 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], b.lo[sl], acc[t], 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t], b.h[sl], acc[t], 0, 0, 0);
-        }
+        *reinterpret_cast<float *>(at) = v;
+    }
+}
+template <int MODE, int NB>
+__device__ __forceinline__ void mx_st2(unsigned char *at, float a, float b) {
+    using X = MxShape<MODE, NB>;
+    if constexpr (X::SPLIT) {
+        typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+        const __bf16 ha = (__bf16)a, hb = (__bf16)b;
+        *reinterpret_cast<bf16x2 *>(at) = bf16x2{ha, hb};
+        *reinterpret_cast<bf16x2 *>(at + X::PLANE) = bf16x2{(__bf16)(a - (float)ha), (__bf16)(b - (float)hb)};
+    } else {
+        *reinterpret_cast<float2 *>(at) = make_float2(a, b);
     }
 }
 
-// A fragments of two 16-row tiles over the whole K (rows given per lane), loaded once and used
-// with several B fragments
-template <int MODE, int NB> struct MxA2 {
+// A fragments of NTILE 16-row tiles over the whole K (this lane's row of each given by its byte
+// address), loaded once and used with several B fragments: FP32 values, or the hi and lo bf16
+// slabs of the split operand
+template <int MODE, int NB, int NTILE> struct MxA {
     using X = MxShape<MODE, NB>;
     static constexpr int NSL = X::KP / 32;
-    float f[MODE == MM_F32 ? 2 : 1][MODE == MM_F32 ? X::S : 1];
-    bf16x8 h[MODE == MM_F32 ? 1 : 2][NSL], lo[MODE == MM_F32 ? 1 : 2][NSL];
-    __device__ __forceinline__ void load(const float *const *row) {
+    float f[MODE == MM_F32 ? NTILE : 1][MODE == MM_F32 ? X::S : 1];
+    bf16x8 h[X::SPLIT ? NTILE : 1][NSL], lo[X::SPLIT ? NTILE : 1][NSL];
+    __device__ __forceinline__ void load(const unsigned char *const *row) {
         const int g = (threadIdx.x & 63) >> 4;
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            if constexpr (MODE == MM_F32) {
+        for (int t = 0; t < NTILE; ++t) {
+            if constexpr (!X::SPLIT) {
 #pragma unroll
                 for (int i = 0; i < X::S / 4; ++i) {
-                    const float4 x = reinterpret_cast<const float4 *>(row[t] + g * X::S)[i];
+                    const float4 x = reinterpret_cast<const float4 *>(row[t] + 4 * g * X::S)[i];
                     f[t][4 * i] = x.x;
                     f[t][4 * i + 1] = x.y;
                     f[t][4 * i + 2] = x.z;
@@ -153,34 +142,27 @@ template <int MODE, int NB> struct MxA2 {
             } else {
 #pragma unroll
                 for (int sl = 0; sl < NSL; ++sl) {
-                    const float4 *p = reinterpret_cast<const float4 *>(row[t] + 32 * sl + 8 * g);
-                    const float4 x0 = p[0], x1 = p[1];
-                    const float xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const __bf16 hh = (__bf16)xs[e];
-                        h[t][sl][e] = hh;
-                        lo[t][sl][e] = (__bf16)(xs[e] - (float)hh);
-                    }
+                    h[t][sl] = *reinterpret_cast<const bf16x8 *>(row[t] + 2 * (32 * sl + 8 * g));
+                    lo[t][sl] = *reinterpret_cast<const bf16x8 *>(row[t] + X::PLANE + 2 * (32 * sl + 8 * g));
                 }
             }
         }
     }
     __device__ __forceinline__ void mma(const typename Frag<MODE, NB, 0>::BF &b, v4f *acc) const {
-        if constexpr (MODE == MM_F32) {
+        if constexpr (!X::SPLIT) {
 #pragma unroll
             for (int k = 0; k < X::S; ++k)
 #pragma unroll
-                for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(f[t][k], b.h[k], acc[t], 0, 0, 0);
+                for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(f[t][k], b.h[k], acc[t], 0, 0, 0);
         } else {
 #pragma unroll
             for (int sl = 0; sl < NSL; ++sl) {
 #pragma unroll
-                for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h[t][sl], b.h[sl], acc[t], 0, 0, 0);
+                for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h[t][sl], b.h[sl], acc[t], 0, 0, 0);
 #pragma unroll
-                for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h[t][sl], b.lo[sl], acc[t], 0, 0, 0);
+                for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h[t][sl], b.lo[sl], acc[t], 0, 0, 0);
 #pragma unroll
-                for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[t][sl], b.h[sl], acc[t], 0, 0, 0);
+                for (int t = 0; t < NTILE; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[t][sl], b.h[sl], acc[t], 0, 0, 0);
             }
         }
     }
@@ -195,8 +177,8 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
 
     extern __shared__ __align__(16) unsigned char smem[];
     const MxLds o = mx_lds<MODE, NB>(A.npmax, A.ft_nmax, A.nsite, A.L);
-    float *sT = reinterpret_cast<float *>(smem + o.t);      // T rows alpha * 32 + b, K = (c, p)
-    float *sU1 = reinterpret_cast<float *>(smem + o.u1);    // U1 rows alpha * 32 + s2, K = (b, p)
+    unsigned char *sT = smem + o.t;      // T rows alpha * 32 + b, K = (c, p)
+    unsigned char *sU1 = smem + o.u1;    // U1 rows alpha * 32 + s2, K = (b, p)
     Z *sE1 = reinterpret_cast<Z *>(smem + o.e1);            // E1[s1][b], pitch PE
     Z *sE2 = reinterpret_cast<Z *>(smem + o.e2);            // E2[s2][c], pitch PE
     Z *sE0 = reinterpret_cast<Z *>(smem + o.e0);            // e0[s0][a]
@@ -243,22 +225,22 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
         sModeF[4 * ND + tid] = (float)A.omega[(size_t)i0 * ND + tid];
     }
 
-    // GEMM B: N tile w of (s2, q), all six M tiles of (alpha, b); this lane's A row in tile 0 and
-    // its U1 store base (row alpha * 32 + s2, column 2 b + q)
-    const int npB = w * 16 + (l & 15);
+    // GEMM B: N tile w % 4 of (s2, q), M tiles w / 4 + 2 t (t = 0, 1, 2) of (alpha, b); this
+    // lane's A row in tile 0 and its U1 store base (row alpha * 32 + s2, column 2 b + q)
+    const int nB = w & 3, mB0 = w >> 2;
+    const int npB = nB * 16 + (l & 15);
     const int s2B = npB >> 1, qB = npB & 1;
-    const float *rowB = sT + (l & 15) * PA;
-    float *u1st = sU1 + s2B * PA + 8 * g + qB;
-    // GEMM C: rows (alpha, s2 local) = 16 t + l % 16 of this wave (t = 0, 1); alpha = 3 reads a
-    // zero row of T
-    const float *rowC[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const int r = 16 * t + (l & 15), al = r >> 3, s2 = 8 * w + (r & 7);
-        rowC[t] = al < 3 ? sU1 + (al * 32 + s2) * PA : sT + 31 * PA;   // T row 31 (b = 31) is zero
+    constexpr int RB = X::ROWB, EB = X::SPLIT ? 2 : 4;   // bytes per row, per element
+    const unsigned char *rowB = sT + (l & 15) * RB;
+    unsigned char *u1st = sU1 + s2B * RB + EB * (8 * g + qB);
+    // GEMM C: row (alpha, s2 local) = l % 16 of this wave; alpha = 3 reads a zero row of T
+    const unsigned char *rowC[1];
+    {
+        const int r = l & 15, al = r >> 2, s2 = 4 * w + (r & 3);
+        rowC[0] = al < 3 ? sU1 + (al * 32 + s2) * RB : sT + 31 * RB;   // T row 31 (b = 31) is zero
     }
-    // GEMM C stores: U2 float ((s1 * 8 + s2l) * 3 + alpha) * 2 + q
-    float *u2st = reinterpret_cast<float *>(sU2) + (((l & 15) >> 1) * 8 + 4 * (g & 1)) * 6 + (g >> 1) * 2 + (l & 1);
+    // GEMM C stores: U2 float ((s1 * 4 + s2l) * 3 + alpha) * 2 + q
+    float *u2st = reinterpret_cast<float *>(sU2) + ((l & 15) >> 1) * 24 + 2 * g + (l & 1);
     const int qD = l & 1;
     KSum<float> accf[4];
 
@@ -291,13 +273,22 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
     };
     // T[(alpha, 3 nu + beta)][2 (3 eta + gamma) + p] of atom mu from its schedule: entry = block
     // (nu, eta) | component j << 7 | first cluster (within the atom) << 12 | cluster count << 23;
-    // lanes run along the components of a block (contiguous FC reads); count 0 clears the output
+    // lanes run along the components of a block (contiguous FC reads); component 31 clears all 27
+    // outputs of the block
     auto ft = [&](int mu) {
         const int ns = schOff[mu + 1] - schOff[mu];
         for (int e = tid; e < ns; e += NT) {
             const unsigned code = sSched[e];
             const int bl = code & 127, j = (code >> 7) & 31, uo = (code >> 12) & 2047, nc = code >> 23;
             const int nu = bl / NB, eta = bl % NB;
+            if (j == 31) {
+#pragma unroll
+                for (int r = 0; r < 9; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        mx_st2<MODE, NB>(sT + ((r / 3) * 32 + 3 * nu + r % 3) * RB + EB * 2 * (3 * eta + c), 0.f, 0.f);
+                continue;
+            }
             const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
             const float *fj = sFC + uo * 27 + j;
             const Z *pj = sP + uo;
@@ -309,9 +300,7 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
                 re = fmaf(f, ph.x, re);
                 im = fmaf(f, ph.y, im);
             }
-            float *dst = sT + (al * 32 + 3 * nu + be) * PA + 2 * (3 * eta + ga);
-            dst[0] = re;
-            dst[1] = im;
+            mx_st2<MODE, NB>(sT + (al * 32 + 3 * nu + be) * RB + EB * 2 * (3 * eta + ga), re, im);
         }
     };
 
@@ -319,7 +308,7 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
         const int2 pr = A.pairs[poff + pi];
         const int q1 = pr.x, q2 = pr.y;
         __syncthreads();   // the previous pair is done with sE1, sE2, sMode, sM1, sM2, sP, sFC, sT
-        // the previous pair's delta stage staged its tiles over T: restore T's zero padding
+        // the previous pair's delta stage staged its tiles over T and U1: restore their zero padding
         for (int i = tid; i < X::NW * 1280 / 4; i += NT)
             reinterpret_cast<uint4 *>(smem + o.t)[i] = make_uint4(0u, 0u, 0u, 0u);
         for (int i = tid; i < ND * ND; i += NT) {
@@ -356,11 +345,6 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
         phases(0);
         __syncthreads();
         ft(0);
-        // eigenvector fragments of GEMMs B and C, the same for every atom of the pair
-        const typename F::BF bE2 = F::load_b(sE2, w * 16);
-        typename F::BF bE1[4];
-#pragma unroll
-        for (int n = 0; n < 4; ++n) bE1[n] = F::load_b(sE1, n * 16);
         v4f V[X::MT_D][4];
 #pragma unroll
         for (int i = 0; i < X::MT_D; ++i)
@@ -373,47 +357,48 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
             // ---- GEMM B (shared): U1[(alpha, b)][(s2, q)] = T x E2' -> sU1[(alpha, s2)][(b, q)];
             //      the next atom's FC, sites and schedule are staged meanwhile ----
             if (mu + 1 < NB) stage(mu + 1);
+            {
+                const unsigned char *rb[3];
 #pragma unroll
-            for (int hh = 0; hh < 2; ++hh) {   // M tiles 3 hh .. 3 hh + 2
-                const float *rb[3];
-#pragma unroll
-                for (int t = 0; t < 3; ++t) rb[t] = rowB + (3 * hh + t) * 16 * PA;
+                for (int t = 0; t < 3; ++t) rb[t] = rowB + (mB0 + 2 * t) * 16 * RB;
                 v4f d[3] = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
-                mx_rows_b<MODE, NB, 3>(rb, bE2, d);
-                // output rows (3 hh + t) 16 + 4 g + r: alpha = (3 hh + t) / 2, b = 16 ((3 hh + t) & 1) + 4 g + r
+                {
+                    const typename F::BF bE2 = F::load_b(sE2, nB * 16);
+                    MxA<MODE, NB, 3> a;
+                    a.load(rb);
+                    a.mma(bE2, d);
+                }
+                // output rows mt 16 + 4 g + r (mt = mB0 + 2 t): alpha = mt / 2, b = 16 (mt & 1) + 4 g + r
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
-                    const int mt = 3 * hh + t, al = mt >> 1, bh = 16 * (mt & 1);
+                    const int mt = mB0 + 2 * t, al = mt >> 1, bh = 16 * (mt & 1);
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        if (s2B < ND && bh + 4 * g + r < ND) u1st[al * 32 * PA + 2 * (bh + r)] = d[t][r];
+                        if (s2B < ND && bh + 4 * g + r < ND)
+                            mx_st1<MODE, NB>(u1st + al * 32 * RB + EB * 2 * (bh + r), d[t][r]);
                 }
             }
             __builtin_amdgcn_s_waitcnt(0);   // this wave's staging of the next atom has landed
             if (mu + 1 < NB) phases(mu + 1);
             __syncthreads();
-            // ---- FT of the next atom (shared) ----
-            if (mu + 1 < NB) ft(mu + 1);
-            // ---- GEMM C (this wave's s2): U2[(alpha, s2l)][(s1, q)] = U1' x E1' -> sU2[(s1, s2l)][alpha] ----
-            {
-                MxA2<MODE, NB> a;
-                a.load(rowC);
+            // ---- GEMM C (this wave's s2): U2[(alpha, s2l)][(s1, q)] = U1' x E1' -> sU2[(s1, s2l)][alpha];
+            //      GEMM D (this wave's s2): V[(s1, s2l)][(s0, q)] += U2[.][(alpha, p)] E0'[(alpha, p)][(s0, q)] ----
+            auto gemm_cd = [&]() {
+                {
+                    MxA<MODE, NB, 1> a;
+                    a.load(rowC);
 #pragma unroll
-                for (int n = 0; n < 4; ++n) {
-                    v4f d[2] = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
-                    a.mma(bE1[n], d);
-                    // output rows 16 t + 4 g + r: alpha = 2 t + g / 2, s2l = 4 (g & 1) + r; column
-                    // (s1, q) with s1 = 8 n + (l % 16) / 2
+                    for (int n = 0; n < 4; ++n) {
+                        const typename F::BF bE1 = F::load_b(sE1, n * 16);
+                        v4f d[1] = {v4f{0.f, 0.f, 0.f, 0.f}};
+                        a.mma(bE1, d);
+                        // output rows 4 g + r: alpha = g, s2l = r; column (s1, q), s1 = 8 n + (l % 16) / 2
+                        if (g < 3 && 8 * n + ((l & 15) >> 1) < ND)
 #pragma unroll
-                    for (int t = 0; t < 2; ++t)
-                        if (2 * t + (g >> 1) < 3 && 8 * n + ((l & 15) >> 1) < ND)
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) u2st[n * 8 * 8 * 6 + t * 4 + r * 6] = d[t][r];
+                            for (int r = 0; r < 4; ++r) u2st[n * 8 * 24 + r * 6] = d[0][r];
+                    }
                 }
-            }
-            mm_wave_sync();   // this wave's U2 stores precede its GEMM D loads
-            // ---- GEMM D (this wave's s2): V[(s1, s2l)][(s0, q)] += U2[.][(alpha, p)] E0'[(alpha, p)][(s0, q)] ----
-            {
+                mm_wave_sync();   // this wave's U2 stores precede its GEMM D loads
                 float b0[4], b1[4];
 #pragma unroll
                 for (int n = 0; n < 4; ++n) {
@@ -437,14 +422,24 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
 #pragma unroll
                     for (int n = 0; n < 4; ++n) V[i][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b1[n], V[i][n], 0, 0, 0);
                 }
+                mm_wave_sync();   // GEMM D's loads of U2 precede the next atom's GEMM C stores
+            };
+            // the two waves of a SIMD (w and w + 4) take the phase sum of the next atom and their
+            // GEMMs in opposite orders (one loop, so that each part is emitted once)
+#pragma unroll 1
+            for (int ph = 0; ph < 2; ++ph) {
+                if ((ph == 0) == (w < 4)) {
+                    if (mu + 1 < NB) ft(mu + 1);
+                } else {
+                    gemm_cd();
+                }
             }
-            mm_wave_sync();   // GEMM D's loads of U2 precede the next atom's GEMM C stores
             __syncthreads();   // T of atom mu + 1 is complete; sU1 is free
         }
 
         // ---- Gaussian-delta weights of this wave's V (sT is free: every wave is past GEMM B of
-        //      the last atom).  Tile i, rows 4 g + r, column (s0, q): s1 = 2 i + g / 2,
-        //      s2 = 8 w + 4 (g & 1) + r; lanes l and l^1 hold Re and Im of the same element and the
+        //      the last atom).  Tile i, rows 4 g + r, column (s0, q): s1 = 4 i + g, s2 = 4 w + r;
+        //      lanes l and l^1 hold Re and Im of the same element and the
         //      lane with q = qD owns rows r = 2 qD + k ("its elements").  Per tile each wave
         //      (1) stages the tile in LDS, (2) tests its elements in FP32 with a margin over the
         //      FP32 rounding of the frequencies, (3) packs the candidates into a list (ballot
@@ -457,7 +452,7 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
         const float hneg = (float)(-hinv);
         const double dcut = A.sigma * 10.0;                 // sqrt(2 x 50) sigma
         const float dcutf = (float)dcut + 1e-3f;            // FP32 pre-test: a superset of the FP64 test
-        float *sV = reinterpret_cast<float *>(smem + o.t) + w * 1280;   // 4 x 64 x 4 staged values
+        float *sV = reinterpret_cast<float *>(smem + o.t) + w * 1280;   // 4 x 64 x 4 staged values (40 KB of T, U1)
         unsigned short *sList = reinterpret_cast<unsigned short *>(sV + 1024);   // up to 512 codes
         float pairf[4] = {0.f, 0.f, 0.f, 0.f};
         float w0f[4], w2f[2];
@@ -468,13 +463,13 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
         }
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            const int s2 = 8 * w + 4 * (g & 1) + 2 * qD + k;
+            const int s2 = 4 * w + 2 * qD + k;
             w2f[k] = s2 < ND ? sModeF[6 * ND + s2] : 1e30f;
         }
 #pragma unroll
         for (int i = 0; i < X::MT_D; ++i) {
-            const int s1 = 2 * i + (g >> 1);
-            const float w1f = sModeF[5 * ND + s1];
+            const int s1 = 4 * i + g;
+            const float w1f = s1 < ND ? sModeF[5 * ND + s1] : 1e30f;   // 1e30: never near
             // (1) stage
 #pragma unroll
             for (int n = 0; n < 4; ++n) *reinterpret_cast<v4f *>(sV + (n * 64 + l) * 4) = V[i][n];
@@ -506,8 +501,8 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
                 const int r = 2 * (lo & 1) + k;
                 const float x = sV[(n * 64 + lo) * 4 + r], y = sV[(n * 64 + (lo ^ 1)) * 4 + r];
                 const int s0 = 8 * n + ((lo & 15) >> 1);
-                const int s1e = 2 * i + (lo >> 5);
-                const int s2 = 8 * w + 4 * ((lo >> 4) & 1) + r;
+                const int s1e = 4 * i + (lo >> 4);
+                const int s2 = 4 * w + r;
                 const double w0 = sMode[s0], w1 = sMode[2 * ND + s1e], w2 = sMode[5 * ND + s2];
                 const double ap = w1 + w2, am = w1 - w2;
                 const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
@@ -556,7 +551,7 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
 }
 
 template <int MODE, int NB>
-__global__ __launch_bounds__(256) void k_lw3_mx(MmArgs A) {
+__global__ __launch_bounds__(512) void k_lw3_mx(MmArgs A) {
     // one unit per workgroup, or (persistent) units from the atomic counter until none is left
     __shared__ int s_unit;
     for (int it = 0;; ++it) {
@@ -595,9 +590,12 @@ static bool mx_schedule(Ctx &c, int nb) {
             const int b = mu * nb2 + bl, cnt = bs[b + 1] - bs[b], uo = bs[b] - u0;
             const bool prev = bs[pm * nb2 + bl + 1] > bs[pm * nb2 + bl];
             if (cnt > 255 || uo > 2047) return false;
-            if (cnt == 0 && !prev) continue;
+            if (cnt == 0) {
+                if (prev) v.push_back({0, (unsigned)bl | 31u << 7});   // clear the whole block
+                continue;
+            }
             for (int j = 0; j < 27; ++j)
-                v.push_back({cnt, (unsigned)bl | (unsigned)j << 7 | (unsigned)(cnt ? uo : 0) << 12 | (unsigned)cnt << 23});
+                v.push_back({cnt, (unsigned)bl | (unsigned)j << 7 | (unsigned)uo << 12 | (unsigned)cnt << 23});
         }
         std::sort(v.begin(), v.end(), [](const std::pair<int, unsigned> &x, const std::pair<int, unsigned> &y) {
             return x.first != y.first ? x.first > y.first : x.second < y.second;
@@ -633,7 +631,7 @@ static bool mx_launch_of_t(Ctx &c, MmArgs &A, int nitems, ItemLaunch &L) {
     L = ItemLaunch{fn, &A, sizeof(A), MxShape<MODE, NB>::NT, dyn, nitems, per_cu,
                    std::string(MODE == MM_F32 ? "mx_f32" : "mx_bf16")};
     L.upi = 1;
-    L.shape = "s2wave8";
+    L.shape = "s2wave4";
     return true;
 }
 

@@ -33,6 +33,8 @@
 
 namespace spr {
 
+typedef float v16f __attribute__((ext_vector_type(16)));
+
 template <int MODE, int NB> struct MxShape {
     static_assert(MODE != MM_F64, "FP32 / split BF16 only");
     using Sh = MmShape<MODE, NB, 0>;   // K padding, pitches and fragments of the per-atom kernel
@@ -44,8 +46,8 @@ template <int MODE, int NB> struct MxShape {
     static constexpr int TROWS = MT_B * 16;            // T rows; b >= Ndof rows stay zero
     static constexpr int U1ROWS = 3 * 32;              // U1 rows alpha * 32 + s2
     static constexpr int MT_C = (3 * SBW + 15) / 16;   // GEMM C: rows alpha * 4 + s2 local
-    static constexpr int MT_D = (ND * SBW + 15) / 16;  // GEMM D: rows s1 * 4 + s2 local
-    static constexpr int U2ROWS = MT_D * 16;
+    static constexpr int MT_D = (ND * SBW + 31) / 32;  // GEMM D (32 x 32 tiles): rows s1 * 4 + s2 local
+    static constexpr int U2ROWS = MT_D * 32;
     static constexpr int NE = NB * NB * 27;            // FT outputs per atom
     // GEMM operand buffers T and U1 (96 rows each): FP32 rows of pitch PA, or in BF16 mode the
     // split operand x = hi + lo already formed by the producer, as a hi plane and a lo plane of
@@ -60,7 +62,7 @@ template <int MODE, int NB> struct MxShape {
 
 // LDS byte offsets (kernel and host-side size check)
 struct MxLds {
-    size_t t, u1, u2, e1, e2, e0, p, fc, tup, sched, ph, mode, modef, m1, m2, total;
+    size_t t, u1, u2, e1, e2, e0, p, fc, tup, sched, ph, mode, modef, m1, m2, atoms, total;
 };
 
 template <int MODE, int NB>
@@ -80,14 +82,15 @@ __host__ __device__ inline MxLds mx_lds(int npmax, int nsched, int nsite, int L)
     o.e2 = take((size_t)X::ND * X::PE * 8);
     o.e0 = take((size_t)X::ND * X::ND * 8);
     o.p = take((size_t)npmax * 8);
-    o.fc = take((size_t)npmax * 27 * 4);
-    o.tup = take((size_t)npmax * 2 * 4);
-    o.sched = take((size_t)nsched * 4);
+    o.fc = take((size_t)(npmax * 27 + 8) * 4);   // + the aligned superset's extra elements
+    o.tup = take((size_t)(npmax * 2 + 8) * 4);
+    o.sched = take((size_t)(nsched + 8) * 4);
     o.ph = take((size_t)L * 8);
     o.mode = take((size_t)8 * X::ND * 8);
     o.modef = take((size_t)7 * X::ND * 4);
     o.m1 = take((size_t)nsite * 4);
     o.m2 = take((size_t)nsite * 4);
+    o.atoms = take((size_t)2 * (NB + 1) * 4);
     o.total = b;
     return o;
 }
@@ -191,7 +194,11 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
     int *sM1 = reinterpret_cast<int *>(smem + o.m1);
     int *sM2 = reinterpret_cast<int *>(smem + o.m2);
     unsigned *sSched = reinterpret_cast<unsigned *>(smem + o.sched);   // phase-sum schedule of one atom
-    const int *schOff = A.ft_sched;                                     // (Nb + 1) atom offsets
+    // per atom: first cluster and first schedule entry (Nb + 1 each), read from LDS so that the
+    // staging issues no global load of its own (a wait for that load would also wait for the
+    // direct-to-LDS loads issued before it)
+    int *aU = reinterpret_cast<int *>(smem + o.atoms);
+    int *schOff = aU + NB + 1;
     const unsigned *sch = reinterpret_cast<const unsigned *>(A.ft_sched + NB + 1);
     double *sRed = reinterpret_cast<double *>(smem + o.t);   // end of unit only
 
@@ -218,6 +225,10 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
         }
     }
     for (int i = tid; i < A.L; i += NT) sPh[i] = phase[i];
+    for (int i = tid; i <= NB; i += NT) {
+        aU[i] = A.blk_start[i * NB * NB];
+        schOff[i] = A.ft_sched[i];
+    }
     for (int i = tid; i < ND * ND; i += NT) sE0[i] = evec[(size_t)i0 * ND * ND + i];
     if (tid < ND) {
         sMode[tid] = A.omega[(size_t)i0 * ND + tid];
@@ -242,32 +253,41 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
     // GEMM C stores: U2 float ((s1 * 4 + s2l) * 3 + alpha) * 2 + q
     float *u2st = reinterpret_cast<float *>(sU2) + ((l & 15) >> 1) * 24 + 2 * g + (l & 1);
     const int qD = l & 1;
-    KSum<float> accf[4];
+    KSum<float> accf[2];
 
-    // FC, cluster sites and schedule of atom mu -> sFC, sTup, sSched (direct global-to-LDS loads,
-    // 64 values per wave and chunk; wave w takes chunks w, w + 4, ...)
-    auto stage = [&](int mu) {
-        const int u0 = A.blk_start[mu * NB * NB], nu = A.blk_start[(mu + 1) * NB * NB] - u0;
-        for (int c = w; c * 64 < nu * 27; c += X::NW)
-            if (c * 64 + l < nu * 27) __builtin_amdgcn_global_load_lds(fc + u0 * 27 + c * 64 + l, sFC + c * 64, 4, 0, 0);
-        for (int c = w; c * 64 < nu * 2; c += X::NW)
-            if (c * 64 + l < nu * 2)
-                __builtin_amdgcn_global_load_lds(A.tup_site + 2 * u0 + c * 64 + l, sTup + c * 64, 4, 0, 0);
-        const int s0 = schOff[mu], ns = schOff[mu + 1] - s0;
-        for (int c = w; c * 64 < ns; c += X::NW)
-            if (c * 64 + l < ns)
-                __builtin_amdgcn_global_load_lds(sch + s0 + c * 64 + l, sSched + c * 64, 4, 0, 0);
+    // Elements [first, first + n) of a 4-byte array (readable for 3 elements past its end) into
+    // dst by 16-byte direct global-to-LDS loads of the aligned superset starting at first & ~3
+    // (1 KB per wave and chunk; wave w takes chunks w, w + 8, ...); element first lands at
+    // dst[first & 3]
+    auto dma16 = [&](const void *src, int first, int n, void *dst) {
+        const int a0 = first & ~3, m = first + n - a0;
+        const unsigned char *gs = reinterpret_cast<const unsigned char *>(src) + (size_t)a0 * 4;
+        unsigned char *ld = reinterpret_cast<unsigned char *>(dst);
+        for (int c = w; c * 256 < m; c += X::NW)
+            if (c * 256 + 4 * l < m) __builtin_amdgcn_global_load_lds(gs + c * 1024 + 16 * l, ld + c * 1024, 16, 0, 0);
     };
-    // P(u) = e^{2 pi i (m1(N1) + m2(N2)) / L} of the clusters whose sites this wave staged (chunk c
-    // holds clusters 32 c .. 32 c + 31), once its own staging has landed: no barrier needed
+    // FC, cluster sites and schedule of atom mu -> sFC, sTup, sSched
+    auto stage = [&](int mu) {
+        const int u0 = aU[mu], nu = aU[mu + 1] - u0;
+        dma16(fc, u0 * 27, nu * 27, sFC);
+        dma16(A.tup_site, 2 * u0, 2 * nu, sTup);
+        dma16(sch, schOff[mu], schOff[mu + 1] - schOff[mu], sSched);
+    };
+    // P(u) = e^{2 pi i (m1(N1) + m2(N2)) / L} of the clusters whose sites this wave staged (its chunk
+    // c holds sites 256 c .. 256 c + 255 of the superset, i.e. clusters (256 c - d) / 2 + 0 .. 127,
+    // d = (2 u0) & 3), once its own staging has landed: no barrier needed
     auto phases = [&](int mu) {
-        const int nu = A.blk_start[(mu + 1) * NB * NB] - A.blk_start[mu * NB * NB];
-        for (int c = w; c * 64 < nu * 2; c += X::NW) {
-            const int u = c * 32 + l;
-            if (l < 32 && u < nu) {
-                int m = sM1[sTup[2 * u]] + sM2[sTup[2 * u + 1]];
-                m -= m >= A.L ? A.L : 0;
-                sP[u] = sPh[m];
+        const int u0 = aU[mu], nu = aU[mu + 1] - u0, d = (2 * u0) & 3;
+        const int *tp = sTup + d;
+        for (int c = w; c * 256 < 2 * nu + d; c += X::NW) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int u = ((256 * c - d) >> 1) + l + 64 * k;
+                if (u >= 0 && u < nu) {
+                    int m = sM1[tp[2 * u]] + sM2[tp[2 * u + 1]];
+                    m -= m >= A.L ? A.L : 0;
+                    sP[u] = sPh[m];
+                }
             }
         }
     };
@@ -277,8 +297,10 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
     // outputs of the block
     auto ft = [&](int mu) {
         const int ns = schOff[mu + 1] - schOff[mu];
+        const unsigned *ss = sSched + (schOff[mu] & 3);
+        const float *sf = sFC + ((aU[mu] * 27) & 3);
         for (int e = tid; e < ns; e += NT) {
-            const unsigned code = sSched[e];
+            const unsigned code = ss[e];
             const int bl = code & 127, j = (code >> 7) & 31, uo = (code >> 12) & 2047, nc = code >> 23;
             const int nu = bl / NB, eta = bl % NB;
             if (j == 31) {
@@ -290,7 +312,7 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
                 continue;
             }
             const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
-            const float *fj = sFC + uo * 27 + j;
+            const float *fj = sf + uo * 27 + j;
             const Z *pj = sP + uo;
             float re = 0.f, im = 0.f;
 #pragma unroll 4
@@ -345,27 +367,31 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
         phases(0);
         __syncthreads();
         ft(0);
-        v4f V[X::MT_D][4];
+        v16f V[X::MT_D][2];
 #pragma unroll
         for (int i = 0; i < X::MT_D; ++i)
 #pragma unroll
-            for (int n = 0; n < 4; ++n) V[i][n] = v4f{0.f, 0.f, 0.f, 0.f};
+            for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) V[i][nt][e] = 0.f;
         __syncthreads();   // T of atom 0 is complete
 
 #pragma unroll 1
         for (int mu = 0; mu < NB; ++mu) {
             // ---- GEMM B (shared): U1[(alpha, b)][(s2, q)] = T x E2' -> sU1[(alpha, s2)][(b, q)];
             //      the next atom's FC, sites and schedule are staged meanwhile ----
-            if (mu + 1 < NB) stage(mu + 1);
             {
                 const unsigned char *rb[3];
 #pragma unroll
                 for (int t = 0; t < 3; ++t) rb[t] = rowB + (mB0 + 2 * t) * 16 * RB;
                 v4f d[3] = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
                 {
+                    // the operands are read before the next atom's staging is issued: an LDS read
+                    // after a direct-to-LDS load waits for that load
                     const typename F::BF bE2 = F::load_b(sE2, nB * 16);
                     MxA<MODE, NB, 3> a;
                     a.load(rb);
+                    if (mu + 1 < NB) stage(mu + 1);
                     a.mma(bE2, d);
                 }
                 // output rows mt 16 + 4 g + r (mt = mB0 + 2 t): alpha = mt / 2, b = 16 (mt & 1) + 4 g + r
@@ -399,28 +425,33 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
                     }
                 }
                 mm_wave_sync();   // this wave's U2 stores precede its GEMM D loads
-                float b0[4], b1[4];
+                // v_mfma_f32_32x32x2f32, K = 2 per instruction: one instruction per alpha with
+                // K = (alpha, p), p = l / 32; lane l: A[m = l % 32][p], B[p][n = l % 32]; B' =
+                // [[Re e0, Im e0], [-Im e0, Re e0]] on (p, q) with column (s0, q), s0 = 16 nt + (l % 32) / 2
+                const int hp = l >> 5;
+                float bd[3][2];
 #pragma unroll
-                for (int n = 0; n < 4; ++n) {
-                    const int np = n * 16 + (l & 15);
-                    const int s0 = np >> 1, q = np & 1;
-                    b0[n] = 0.f;
-                    b1[n] = 0.f;
-                    if (g < 3 && s0 < ND) {
-                        const Z e = sE0[s0 * ND + 3 * mu + g];
-                        b0[n] = q ? e.y : e.x;
-                        b1[n] = q ? e.x : -e.y;
+                for (int al = 0; al < 3; ++al)
+#pragma unroll
+                    for (int nt = 0; nt < 2; ++nt) {
+                        const int s0 = 16 * nt + ((l & 31) >> 1), q = l & 1;
+                        bd[al][nt] = 0.f;
+                        if (s0 < ND) {
+                            const Z e = sE0[s0 * ND + 3 * mu + al];
+                            bd[al][nt] = hp == 0 ? (q ? e.y : e.x) : (q ? e.x : -e.y);
+                        }
                     }
-                }
 #pragma unroll
                 for (int i = 0; i < X::MT_D; ++i) {
-                    const int m = i * 16 + (l & 15);
-                    Z a = Z{0.f, 0.f};
-                    if (g < 3) a = sU2[m * 3 + g];
+                    const int m = i * 32 + (l & 31);
 #pragma unroll
-                    for (int n = 0; n < 4; ++n) V[i][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b0[n], V[i][n], 0, 0, 0);
+                    for (int al = 0; al < 3; ++al) {
+                        const Z a = sU2[m * 3 + al];
+                        const float av = hp ? a.y : a.x;
 #pragma unroll
-                    for (int n = 0; n < 4; ++n) V[i][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b1[n], V[i][n], 0, 0, 0);
+                        for (int nt = 0; nt < 2; ++nt)
+                            V[i][nt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bd[al][nt], V[i][nt], 0, 0, 0);
+                    }
                 }
                 mm_wave_sync();   // GEMM D's loads of U2 precede the next atom's GEMM C stores
             };
@@ -437,16 +468,17 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
             __syncthreads();   // T of atom mu + 1 is complete; sU1 is free
         }
 
-        // ---- Gaussian-delta weights of this wave's V (sT is free: every wave is past GEMM B of
-        //      the last atom).  Tile i, rows 4 g + r, column (s0, q): s1 = 4 i + g, s2 = 4 w + r;
-        //      lanes l and l^1 hold Re and Im of the same element and the
-        //      lane with q = qD owns rows r = 2 qD + k ("its elements").  Per tile each wave
-        //      (1) stages the tile in LDS, (2) tests its elements in FP32 with a margin over the
-        //      FP32 rounding of the frequencies, (3) packs the candidates into a list (ballot
-        //      order), (4) evaluates the list with every lane -- arguments in FP64, a triplet with
-        //      all three Gaussians below e^-50 of their peak adds nothing and is dropped, the rest in
-        //      FP32 -- writing each term over the staged value of its element, and (5) each owner
-        //      adds its elements' terms in element order, so the sums do not depend on the packing ----
+        // ---- Gaussian-delta weights of this wave's V (T and U1 are free: every wave is past its
+        //      last GEMM C).  Tile (i, nt), element 4 a + b of lane l: row m = 32 i + 8 a + 4 h + b
+        //      (h = l / 32): s1 = 8 i + 2 a + h, s2 = 4 w + b; column (s0, q), s0 = 16 nt + (l % 32) / 2,
+        //      q = l & 1.  Lanes l and l^1 hold Re and Im of the same element and the lane with
+        //      q = qD owns b = 2 qD + k ("its elements").  Per tile each wave (1) stages the tile in
+        //      LDS, (2) tests its elements in FP32 with a margin over the FP32 rounding of the
+        //      frequencies, (3) packs the candidates into a list (ballot order), (4) evaluates the
+        //      list with every lane -- arguments in FP64, a triplet with all three Gaussians below
+        //      e^-50 of their peak adds nothing and is dropped, the rest in FP32 -- writing each term
+        //      over the staged value of its element, and (5) each owner adds its elements' terms in
+        //      element order, so the sums do not depend on the packing ----
         const double c0 = 1.0 / (A.sigma * 2.5066282746310002);   // 1/(sigma sqrt(2 pi))
         const double hinv = 0.5 / (A.sigma * A.sigma);
         const float hneg = (float)(-hinv);
@@ -454,98 +486,101 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
         const float dcutf = (float)dcut + 1e-3f;            // FP32 pre-test: a superset of the FP64 test
         float *sV = reinterpret_cast<float *>(smem + o.t) + w * 1280;   // 4 x 64 x 4 staged values (40 KB of T, U1)
         unsigned short *sList = reinterpret_cast<unsigned short *>(sV + 1024);   // up to 512 codes
-        float pairf[4] = {0.f, 0.f, 0.f, 0.f};
-        float w0f[4], w2f[2];
-#pragma unroll
-        for (int n = 0; n < 4; ++n) {
-            const int s0 = 8 * n + ((l & 15) >> 1);
-            w0f[n] = s0 < ND ? sModeF[4 * ND + s0] : 1e30f;   // 1e30: never near
-        }
+        const int hh = l >> 5;
+        float pairf[2] = {0.f, 0.f};
+        float w2f[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int s2 = 4 * w + 2 * qD + k;
-            w2f[k] = s2 < ND ? sModeF[6 * ND + s2] : 1e30f;
+            w2f[k] = s2 < ND ? sModeF[6 * ND + s2] : 1e30f;   // 1e30: never near
         }
 #pragma unroll
-        for (int i = 0; i < X::MT_D; ++i) {
-            const int s1 = 4 * i + g;
-            const float w1f = s1 < ND ? sModeF[5 * ND + s1] : 1e30f;   // 1e30: never near
-            // (1) stage
+        for (int i = 0; i < X::MT_D; ++i)
 #pragma unroll
-            for (int n = 0; n < 4; ++n) *reinterpret_cast<v4f *>(sV + (n * 64 + l) * 4) = V[i][n];
-            mm_wave_sync();
-            // (2) + (3) test and pack
-            int cnt = 0;
-            unsigned act = 0;
+            for (int nt = 0; nt < 2; ++nt) {
+                const int s0 = 16 * nt + ((l & 31) >> 1);
+                const float w0f = s0 < ND ? sModeF[4 * ND + s0] : 1e30f;
+                // (1) stage
 #pragma unroll
-            for (int n = 0; n < 4; ++n)
+                for (int a = 0; a < 4; ++a)
+                    *reinterpret_cast<v4f *>(sV + (a * 64 + l) * 4) =
+                        v4f{V[i][nt][4 * a], V[i][nt][4 * a + 1], V[i][nt][4 * a + 2], V[i][nt][4 * a + 3]};
+                mm_wave_sync();
+                // (2) + (3) test and pack
+                int cnt = 0;
+                unsigned act = 0;
 #pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const float ap = w1f + w2f[k], am = w1f - w2f[k];
-                    const float dm = fminf(fabsf(w0f[n] - ap), fminf(fabsf(w0f[n] + am), fabsf(w0f[n] - am)));
-                    const bool near = dm < dcutf;
-                    const unsigned long long msk = __ballot(near);
-                    if (near) {
-                        const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32),
-                                                                              __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
-                        sList[pos] = (unsigned short)((n << 7) | (l << 1) | k);
-                        act |= 1u << (2 * n + k);
+                for (int a = 0; a < 4; ++a) {
+                    const int s1 = 8 * i + 2 * a + hh;
+                    const float w1f = s1 < ND ? sModeF[5 * ND + s1] : 1e30f;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const float ap = w1f + w2f[k], am = w1f - w2f[k];
+                        const float dm = fminf(fabsf(w0f - ap), fminf(fabsf(w0f + am), fabsf(w0f - am)));
+                        const bool near = dm < dcutf;
+                        const unsigned long long msk = __ballot(near);
+                        if (near) {
+                            const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32),
+                                                                                  __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
+                            sList[pos] = (unsigned short)((a << 7) | (l << 1) | k);
+                            act |= 1u << (2 * a + k);
+                        }
+                        cnt += __popcll(msk);
                     }
-                    cnt += __popcll(msk);
                 }
-            mm_wave_sync();
-            // (4) evaluate: the term replaces the staged value of its element (read by this lane only)
-            for (int idx = l; idx < cnt; idx += 64) {
-                const int code = sList[idx];
-                const int n = code >> 7, lo = (code >> 1) & 63, k = code & 1;
-                const int r = 2 * (lo & 1) + k;
-                const float x = sV[(n * 64 + lo) * 4 + r], y = sV[(n * 64 + (lo ^ 1)) * 4 + r];
-                const int s0 = 8 * n + ((lo & 15) >> 1);
-                const int s1e = 4 * i + (lo >> 4);
-                const int s2 = 4 * w + r;
-                const double w0 = sMode[s0], w1 = sMode[2 * ND + s1e], w2 = sMode[5 * ND + s2];
-                const double ap = w1 + w2, am = w1 - w2;
-                const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
-                float term = 0.f;
-                if (fabs(d1) < dcut || fabs(d2) < dcut || fabs(d3) < dcut) {
-                    const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
-                    const float n1 = sModeF[s1e], iw1 = sModeF[ND + s1e];
-                    const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
-                    const float W = (x * x + y * y) * (iw1 * iw2);
-                    const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
-                    const float sm = A.sel ? (float)A.sel[ND + s1e * ND + s2] : 1.f;
-                    term = W * sm * ((n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3));
+                mm_wave_sync();
+                // (4) evaluate: the term replaces the staged value of its element (read by this lane only)
+                for (int idx = l; idx < cnt; idx += 64) {
+                    const int code = sList[idx];
+                    const int a = code >> 7, lo = (code >> 1) & 63, k = code & 1;
+                    const int b = 2 * (lo & 1) + k;
+                    const float x = sV[(a * 64 + lo) * 4 + b], y = sV[(a * 64 + (lo ^ 1)) * 4 + b];
+                    const int s0e = 16 * nt + ((lo & 31) >> 1);
+                    const int s1e = 8 * i + 2 * a + (lo >> 5);
+                    const int s2 = 4 * w + b;
+                    const double w0 = sMode[s0e], w1 = sMode[2 * ND + s1e], w2 = sMode[5 * ND + s2];
+                    const double ap = w1 + w2, am = w1 - w2;
+                    const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
+                    float term = 0.f;
+                    if (fabs(d1) < dcut || fabs(d2) < dcut || fabs(d3) < dcut) {
+                        const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
+                        const float n1 = sModeF[s1e], iw1 = sModeF[ND + s1e];
+                        const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
+                        const float W = (x * x + y * y) * (iw1 * iw2);
+                        const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
+                        const float sm = A.sel ? (float)A.sel[ND + s1e * ND + s2] : 1.f;
+                        term = W * sm * ((n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3));
+                    }
+                    sV[(a * 64 + lo) * 4 + b] = term;
                 }
-                sV[(n * 64 + lo) * 4 + r] = term;
+                mm_wave_sync();
+                // (5) the owner adds its elements' terms in element order
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+                        if (act & (1u << (2 * a + k))) pairf[nt] += sV[(a * 64 + l) * 4 + 2 * qD + k];
+                mm_wave_sync();   // this wave has read the tile before the next one overwrites it
             }
-            mm_wave_sync();
-            // (5) the owner adds its elements' terms in element order
-#pragma unroll
-            for (int n = 0; n < 4; ++n)
-#pragma unroll
-                for (int k = 0; k < 2; ++k)
-                    if (act & (1u << (2 * n + k))) pairf[n] += sV[(n * 64 + l) * 4 + 2 * qD + k];
-            mm_wave_sync();   // this wave has read the tile before the next one overwrites it
-        }
         const double wpair = q1 == q2 ? 1.0 : 2.0;
 #pragma unroll
-        for (int n = 0; n < 4; ++n) {
-            const int s0 = (n * 16 + (l & 15)) >> 1;
-            if (s0 < ND) accf[n].add(pairf[n] * (float)(wpair * c0 * sMode[ND + s0]));
+        for (int nt = 0; nt < 2; ++nt) {
+            const int s0 = 16 * nt + ((l & 31) >> 1);
+            if (s0 < ND) accf[nt].add(pairf[nt] * (float)(wpair * c0 * sMode[ND + s0]));
         }
     }
-    // ---- reduce per s0 in a fixed order: lane l of every wave holds s0 = (16 n + l % 16) / 2 ----
+    // ---- reduce per s0 in a fixed order: lane l of every wave holds s0 = 16 nt + (l % 32) / 2 ----
     __syncthreads();
 #pragma unroll
-    for (int n = 0; n < 4; ++n) sRed[n * NT + tid] = accf[n].value();
+    for (int nt = 0; nt < 2; ++nt) sRed[nt * NT + tid] = accf[nt].value();
     __syncthreads();
     if (tid < ND) {
         const int s = tid;
-        const int n = (2 * s) >> 4, sj = (2 * s) & 15;
+        const int nt = s >> 4, sj = s & 15;
         double tot = 0.0;
         for (int ww = 0; ww < X::NW; ++ww)
-            for (int gg = 0; gg < 4; ++gg)
-                for (int q = 0; q < 2; ++q) tot += sRed[n * NT + ww * 64 + gg * 16 + sj + q];
+            for (int h2 = 0; h2 < 2; ++h2)
+                for (int q = 0; q < 2; ++q) tot += sRed[nt * NT + ww * 64 + h2 * 32 + 2 * sj + q];
         A.ic.partial[(size_t)unit * ND + s] = A.sel ? tot * A.sel[s] : tot;
     }
 }
@@ -606,6 +641,7 @@ static bool mx_schedule(Ctx &c, int nb) {
     off[nb] = (int)ent.size();
     std::vector<int> h(off);
     h.insert(h.end(), ent.begin(), ent.end());
+    h.resize(h.size() + 4, 0);   // readable past the end by the 16-byte staging
     if (hipMalloc((void **)&f.ft_sched, h.size() * sizeof(int)) != hipSuccess) {
         f.ft_sched = nullptr;
         return false;

@@ -128,9 +128,16 @@ static int build_ifc(Ctx &c, const spr_ifc *f, const double *osm, DevIFC &d) {
     std::vector<float> v32(all_val.begin(), all_val.end());
     int rc;
     if ((rc = upload(&d.blk_start, start.data(), start.size()))) return rc;
+    // tup_site and fc32 carry 4 zero elements past their end: kernels that stage a range with
+    // 16-byte direct-to-LDS loads read the aligned superset of it
+    const size_t ntup = all_tup.size(), n32 = v32.size();
+    all_tup.resize(ntup + 4, 0);
+    v32.resize(n32 + 4, 0.f);
     if ((rc = upload(&d.tup_site, all_tup.data(), all_tup.size()))) return rc;
     if ((rc = upload(&d.fc64, all_val.data(), all_val.size()))) return rc;
     if ((rc = upload(&d.fc32, v32.data(), v32.size()))) return rc;
+    all_tup.resize(ntup);
+    v32.resize(n32);
     if ((rc = upload(&d.site, site.data(), site.size()))) return rc;
     // padded copy: block b of leading atom mu gets zero-valued tuples on site site_off[mu]
     std::vector<int> pstart(nblk + 1, 0), ptup;

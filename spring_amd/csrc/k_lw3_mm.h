@@ -42,6 +42,8 @@ struct MmArgs {
     // largest entry count of one atom
     const int *ft_sched;
     int ft_nmax;
+    // FP64 per-atom kernel: phase-sum task lists (NULL: every block of the atom by one loop)
+    const int *ft2;
 };
 
 constexpr int pow2ceil(int x) { return x <= 1 ? 1 : x <= 2 ? 2 : x <= 4 ? 4 : 8; }
@@ -113,9 +115,10 @@ template <int MODE, int NB, int VAR> struct MmLds {
     static constexpr size_t U2 = (size_t)Sh::U2ROWS * Sh::U2S1 * sizeof(Z);
     static constexpr size_t AC = cmax((size_t)Sh::MC * Sh::PA * sizeof(R), (size_t)512 * sizeof(double));
     static constexpr size_t E = (size_t)Sh::ND * Sh::PE * sizeof(Z);
-    // FP32 / BF16 delta stage: staged V and the element lists (FP64 works in registers)
-    static constexpr size_t DV = Sh::PALL ? (size_t)8 * kDvTiles * 256 * sizeof(float) : 0;
-    static constexpr size_t DL = Sh::PALL ? (size_t)8 * kDvTiles * 128 * sizeof(unsigned short) : 0;
+    // delta stage: staged V (in the accumulator type) and the element lists
+    using DT = std::conditional_t<MODE == MM_F64, double, float>;
+    static constexpr size_t DV = (size_t)8 * kDvTiles * 256 * sizeof(DT);
+    static constexpr size_t DL = (size_t)8 * kDvTiles * 128 * sizeof(unsigned short);
     __host__ __device__ static constexpr size_t tu() { return TU; }
     __host__ __device__ static constexpr size_t u2() { return U2; }
     __host__ __device__ static constexpr size_t ac() { return AC; }
@@ -134,6 +137,7 @@ template <int MODE, int NB, int VAR> struct MmLds {
              4 * Sh::ND * sizeof(float);
         if (Sh::PALL) b += (size_t)npmax * 27 * sizeof(R);      // FC of one atom (staged)
         b += (size_t)Sh::ND * Sh::ND * sizeof(Z) + 16;           // e0 of the unit's q0
+        if (!Sh::PALL) b += (size_t)NB * 2 * sizeof(int);        // FP64 phase-sum task-list heads
         b += 2 * (size_t)nsite * sizeof(int) + (size_t)(NB * NB * NB + 1) * sizeof(int);
         return (b + 15) & ~(size_t)15;
     }

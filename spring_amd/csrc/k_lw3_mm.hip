@@ -42,6 +42,13 @@
 
 namespace spr {
 
+// FP64 phase-sum task lists (built by ft2_schedule): per atom mu a head (offset, rounds) and the
+// tasks as [round][lane] over the 512 lanes of the workgroup.  Task = block (nu, eta) | component
+// j << 7 | first cluster within the atom << 12 | cluster count << 23; j = 31 clears the block;
+// kFt2Empty is no task.
+constexpr int kFt2Rounds = 4;
+constexpr unsigned kFt2Empty = 0xFFFFFFFFu;
+
 template <int MODE, int NB, int VAR, bool WOUT>
 __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     static_assert(!WOUT || MODE == MM_F64, "W output is FP64");
@@ -82,6 +89,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     int *sBlk = sM2 + A.nsite;                        // blk_start of every (mu, nu, eta) block
     Z *sE0 = reinterpret_cast<Z *>(((uintptr_t)(sBlk + NB * NB * NB + 1) + 15) & ~(uintptr_t)15);   // e0[s0][a]
     R *sFC = reinterpret_cast<R *>(sE0 + ND * ND);    // FC of atom mu (FP32 / BF16 modes)
+    int *sFt2 = reinterpret_cast<int *>(sE0 + ND * ND);   // FP64: task-list heads of every atom
 
     const int tid = threadIdx.x, l = tid & 63, g = l >> 4;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -108,6 +116,8 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     for (int i = tid; i < A.L; i += 512) sPh[i] = phase[i];
     for (int i = tid; i <= NB * NB * NB; i += 512) sBlk[i] = A.blk_start[i];
     for (int i = tid; i < ND * ND; i += 512) sE0[i] = evec[(size_t)i0 * ND * ND + i];
+    if (!Sh::PALL && A.ft2)
+        for (int i = tid; i < 2 * NB; i += 512) sFt2[i] = A.ft2[i];
     if (tid < ND) {
         sMode[tid] = A.omega[(size_t)i0 * ND + tid];
         sMode[ND + tid] = A.inv_omega[(size_t)i0 * ND + tid] * A.factor;
@@ -265,15 +275,67 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                 }
             }
         };
+        // FP64 phase sum of atom mu from its task list (one task per lane and round; within a
+        // round the lanes of a wave hold tasks of equal or next-to-equal cluster count, and the
+        // lane totals are even).  Each lane issues the FC loads of four clusters before their FMAs
+        // (clusters past the count add exact zeros, so each output is summed in cluster order).
+        auto ft_list = [&](int mu) {
+            const int off = sFt2[2 * mu], nr = sFt2[2 * mu + 1];
+            const unsigned *cs = reinterpret_cast<const unsigned *>(A.ft2) + off + tid;
+            const R *fcm = fc + (size_t)sBlk[mu * NB * NB] * 27;
+            unsigned code[kFt2Rounds];
+#pragma unroll
+            for (int r = 0; r < kFt2Rounds; ++r) code[r] = r < nr ? cs[(size_t)r * 512] : kFt2Empty;
+#pragma unroll
+            for (int r = 0; r < kFt2Rounds; ++r) {
+                const unsigned cd = code[r];
+                if (cd == kFt2Empty) continue;
+                const int bl = cd & 127, j = (cd >> 7) & 31, uo = (cd >> 12) & 2047, nc = (cd >> 23) & 255;
+                const int nu = bl / NB, eta = bl % NB;
+                if (j == 31) {   // a block the previous atom wrote and this one does not
+#pragma unroll
+                    for (int rr = 0; rr < 9; ++rr)
+#pragma unroll
+                        for (int cc = 0; cc < 3; ++cc) {
+                            R *dst = sT + ((rr / 3) * ND + 3 * nu + rr % 3) * PA + 2 * (3 * eta + cc);
+                            dst[0] = 0;
+                            dst[1] = 0;
+                        }
+                    continue;
+                }
+                const int al = j / 9, be = (j / 3) % 3, ga = j % 3;
+                const R *fj = fcm + (size_t)uo * 27 + j;
+                const Z *pj = sP + uo;
+                R re = 0, im = 0;
+                for (int u = 0; u < nc; u += 4) {
+                    R f[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) f[k] = u + k < nc ? fj[(size_t)(u + k) * 27] : (R)0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const Z ph = u + k < nc ? pj[u + k] : Z{0, 0};
+                        re = fma(f[k], ph.x, re);
+                        im = fma(f[k], ph.y, im);
+                    }
+                }
+                R *dst = sT + (al * ND + 3 * nu + be) * PA + 2 * (3 * eta + ga);
+                dst[0] = re;
+                dst[1] = im;
+            }
+        };
+        const bool tasks = !Sh::PALL && A.ft2 != nullptr;   // FP64 task lists
         if constexpr (!Sh::PALL) {
             phases(0);
             __syncthreads();
         }
-        ft(0);
+        if (tasks) ft_list(0);
+        else ft(0);
         __syncthreads();
 
-        // Per atom: [stage next FC; GEMM B] | [GEMM C (+ next phases, FP64)] | [GEMM D + FT of the
-        // next atom] -- three barriers per atom; the phase sum runs beside the GEMM D MFMAs
+        // Per atom, three barrier-separated segments: [GEMM B; stage the next atom's FC (FP32 /
+        // BF16) or its cluster phases (FP64 task lists)] | [GEMM C] | [GEMM D; the next atom's phase
+        // sum, before GEMM D on waves 0-3 and after it on waves 4-7, so that the phase sum of one
+        // wave of a SIMD runs beside the matrix products of the other].
 #pragma unroll 1
         for (int mu = 0; mu < NB; ++mu) {
             if constexpr (Sh::PALL)
@@ -306,6 +368,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                     }
                 }
             }
+            if (tasks && mu + 1 < NB) phases(mu + 1);   // sP of atom mu was read by the last phase sum
             __syncthreads();
             // ---- GEMM C: U2[(alpha, s2 local)][(s1, q)] = sAC x E1' -> sU2[(s1, s2 local)][alpha] ----
             if (cOK) {
@@ -336,12 +399,16 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             }
             if constexpr (Sh::PALL) {
                 __builtin_amdgcn_s_waitcnt(0);   // the next atom's FC has landed
-            } else {
+            } else if (!tasks) {
                 if (mu + 1 < NB) phases(mu + 1);   // sP of atom mu was read by the last FT
             }
             __syncthreads();
             // ---- GEMM D: V[(s1, s2 local)][(s0, q)] += U2[.][(alpha, p)] E0'[(alpha, p)][(s0, q)]
             //      K' = 8: step i, lane group g -> (alpha = g, p = i); alpha = 3 is padding ----
+            if (w < 4 && mu + 1 < NB) {
+                if (tasks) ft_list(mu + 1);
+                else ft(mu + 1);
+            }
             if (cOK) {
                 DR b0 = 0, b1 = 0;
                 if (g < 3 && s0 < ND) {
@@ -349,13 +416,21 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                     b0 = (DR)(qD ? e.y : e.x);
                     b1 = (DR)(qD ? e.x : -e.y);
                 }
+                // every tile index in range for each wave when the M tiles divide evenly; the U2 row of
+                // tile j at a fixed stride from tile 0 when the s2 block divides 16; lane group 3
+                // reads a neighbouring element (in range) and zeroes it, branch-free
+                constexpr bool kDFull = Sh::MT_D % MSTEPC == 0 && Sh::VT * MSTEPC == Sh::MT_D;
+                const int m0 = mtC0 * 16 + (l & 15);
+                const Z *dA = sU2 + (m0 / SB) * Sh::U2S1 + (m0 % SB) * 3 + g;
 #pragma unroll
                 for (int j = 0; j < Sh::VT; ++j) {
                     const int mt = mtC0 + MSTEPC * j;
-                    if (mt < Sh::MT_D) {
+                    if (kDFull || mt < Sh::MT_D) {
                         const int m = mt * 16 + (l & 15);
-                        Z a = Z{0, 0};
-                        if (g < 3) a = sU2[(m / SB) * Sh::U2S1 + (m % SB) * 3 + g];
+                        Z a;
+                        if constexpr (16 % SB == 0) a = dA[j * (MSTEPC * 16 / SB) * Sh::U2S1];
+                        else a = sU2[(m / SB) * Sh::U2S1 + (m % SB) * 3 + g];
+                        if (g >= 3) a = Z{0, 0};
                         if constexpr (MODE == MM_F64) {
                             vacc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b0, vacc[j], 0, 0, 0);
                             vacc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b1, vacc[j], 0, 0, 0);
@@ -366,7 +441,10 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                     }
                 }
             }
-            if (mu + 1 < NB) ft(mu + 1);   // T of atom mu was read by GEMM B
+            if (w >= 4 && mu + 1 < NB) {   // T of atom mu was read by GEMM B
+                if (tasks) ft_list(mu + 1);
+                else ft(mu + 1);
+            }
             __syncthreads();
         }
 
@@ -433,64 +511,23 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             constexpr size_t USED = (size_t)S0C * PER_S0 * sizeof(double);
             for (int i = tid; i < (int)((USED + 15) / 16); i += 512)
                 reinterpret_cast<uint4 *>(smem)[i] = make_uint4(0u, 0u, 0u, 0u);
-        } else if constexpr (MODE == MM_F64) {
-            // FP64: every element in registers.  Lanes l and l^1 hold Re and Im of the same
-            // (row, s0); lane q = 0 takes rows r = 0, 1 of every tile and lane q = 1 rows 2, 3, each
-            // receiving the other part from its partner.  A tile row whose three Gaussians
-            // underflow to exact zero on every lane of the wave adds nothing and skips the
-            // exponentials.
-            if (s0ok) {
-                const double w0 = sMode[s0], iw0 = sMode[ND + s0];
-                // s2 of this lane's two rows depends on the lane only when the s2 block divides 16
-                constexpr bool kS2Lane = 16 % SB == 0;
-                int s2k[2];
-                double w2k[2];
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    s2k[k] = h * SB + tile_row<true>(g, 2 * qD + k) % SB;
-                    w2k[k] = s2k[k] < ND ? sMode[5 * ND + s2k[k]] : 0.0;
-                }
-#pragma unroll
-                for (int j = 0; j < Sh::VT; ++j) {
-                    const int mt = mtC0 + MSTEPC * j;
-                    if (mt >= Sh::MT_D) continue;
-#pragma unroll
-                    for (int k = 0; k < 2; ++k) {
-                        const double x = qD ? vacc[j][2 + k] : vacc[j][k];          // row 2 qD + k, this lane
-                        const double y = __shfl_xor(qD ? vacc[j][k] : vacc[j][2 + k], 1, 64);   // same row, partner
-                        const int m = mt * 16 + tile_row<true>(g, 2 * qD + k);
-                        const int s1 = m / SB, s2 = kS2Lane ? s2k[k] : h * SB + m % SB;
-                        const bool ok = s1 < ND && s2 < ND;
-                        const double w1 = ok ? sMode[2 * ND + s1] : 0.0;
-                        const double w2 = kS2Lane ? w2k[k] : (ok ? sMode[5 * ND + s2] : 0.0);
-                        const double ap = w1 + w2, am = w1 - w2;
-                        const double d1 = w0 - ap, d2 = w0 + am, d3 = w0 - am;
-                        const double tmin = hinv * fmin(d1 * d1, fmin(d2 * d2, d3 * d3));
-                        // exp(-t) is zero beyond t = 745.2
-                        if (!__ballot(ok && tmin < 746.0)) continue;
-                        if (!ok) continue;
-                        const double n1 = sMode[3 * ND + s1], iw1 = sMode[4 * ND + s1];
-                        const double n2 = sMode[6 * ND + s2], iw2 = sMode[7 * ND + s2];
-                        const double W = (x * x + y * y) * iw0 * iw1 * iw2;
-                        const double G = (n1 + n2 + 1.0) * (c0 * exp(-hinv * d1 * d1)) +
-                                         (n1 - n2) * (c0 * exp(-hinv * d2 * d2) - c0 * exp(-hinv * d3 * d3));
-                        accd += A.sel ? wpair * W * G * A.sel[ND + s1 * ND + s2] : wpair * W * G;
-                    }
-                }
-            }
         } else {
-            constexpr double TCUT = 50.0;
+            // FP64 evaluates the list in FP64 and sums each pair's terms in FP64 (element order);
+            // FP32 and BF16 evaluate in FP32 and add each pair's sum into a compensated FP32 sum
+            using DT = typename L::DT;
+            constexpr bool F64 = MODE == MM_F64;
+            constexpr double TCUT = F64 ? 100.0 : 50.0;
             const double dcut = A.sigma * sqrt(2.0 * TCUT);
-            float *sV = reinterpret_cast<float *>(smem) + (size_t)w * (kDvTiles * 256);
+            DT *sV = reinterpret_cast<DT *>(smem) + (size_t)w * (kDvTiles * 256);
             unsigned short *sList = reinterpret_cast<unsigned short *>(smem + L::DV) + (size_t)w * (kDvTiles * 128);
             const double w0 = s0ok ? sMode[s0] : 0.0;
-            float pairf = 0.f;
+            DT pairv = 0;
 #pragma unroll
             for (int j0 = 0; j0 < Sh::VT; j0 += kDvTiles) {
                 // (1) stage this wave's tiles
 #pragma unroll
                 for (int t = 0; t < kDvTiles; ++t)
-                    if (j0 + t < Sh::VT) *reinterpret_cast<v4f *>(sV + (t * 64 + l) * 4) = vacc[j0 + t];
+                    if (j0 + t < Sh::VT) *reinterpret_cast<DACC *>(sV + (t * 64 + l) * 4) = vacc[j0 + t];
                 mm_wave_sync();
                 // (2) + (3): test this lane's elements, pack the resonant ones
                 int cnt = 0;
@@ -500,7 +537,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
 #pragma unroll
                     for (int k = 0; k < 2; ++k) {
                         const int mt = mtC0 + MSTEPC * (j0 + t);
-                        const int m = mt * 16 + tile_row<false>(g, 2 * qD + k);
+                        const int m = mt * 16 + tile_row<F64>(g, 2 * qD + k);
                         const int s1 = m / SB, s2 = h * SB + m % SB;
                         bool near = false;
                         if (s0ok && j0 + t < Sh::VT && mt < Sh::MT_D && s1 < ND && s2 < ND &&
@@ -526,20 +563,29 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                     const int code = sList[idx];
                     const int t = code >> 7, lo = (code >> 1) & 63, k = code & 1;
                     const int r = 2 * (lo & 1) + k;
-                    const float x = sV[(t * 64 + lo) * 4 + r], y = sV[(t * 64 + (lo ^ 1)) * 4 + r];
+                    const DT x = sV[(t * 64 + lo) * 4 + r], y = sV[(t * 64 + (lo ^ 1)) * 4 + r];
                     const int mt = mtC0 + MSTEPC * (j0 + t);
-                    const int m = mt * 16 + tile_row<false>(lo >> 4, r);
+                    const int m = mt * 16 + tile_row<F64>(lo >> 4, r);
                     const int s1 = m / SB, s2 = h * SB + m % SB;
                     const double we = sMode[(ntC * 16 + (lo & 15)) >> 1];
                     const double w1 = sMode[2 * ND + s1], w2 = sMode[5 * ND + s2];
                     const double ap = w1 + w2, am = w1 - w2;
-                    const float f1 = (float)(we - ap), f2 = (float)(we + am), f3 = (float)(we - am);
-                    const float n1 = sModeF[s1], iw1 = sModeF[ND + s1];
-                    const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
-                    const float W = (x * x + y * y) * (iw1 * iw2);
-                    const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
-                    const float sm = A.sel ? (float)A.sel[ND + s1 * ND + s2] : 1.f;
-                    sV[(t * 64 + lo) * 4 + r] = W * sm * ((n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3));
+                    if constexpr (F64) {
+                        const double d1 = we - ap, d2 = we + am, d3 = we - am;
+                        const double W = (x * x + y * y) * (sMode[4 * ND + s1] * sMode[7 * ND + s2]);
+                        const double n1 = sMode[3 * ND + s1], n2 = sMode[6 * ND + s2];
+                        const double G = (n1 + n2 + 1.0) * exp(-hinv * d1 * d1) +
+                                         (n1 - n2) * (exp(-hinv * d2 * d2) - exp(-hinv * d3 * d3));
+                        sV[(t * 64 + lo) * 4 + r] = A.sel ? W * G * A.sel[ND + s1 * ND + s2] : W * G;
+                    } else {
+                        const float f1 = (float)(we - ap), f2 = (float)(we + am), f3 = (float)(we - am);
+                        const float n1 = sModeF[s1], iw1 = sModeF[ND + s1];
+                        const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
+                        const float W = (x * x + y * y) * (iw1 * iw2);
+                        const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
+                        const float sm = A.sel ? (float)A.sel[ND + s1 * ND + s2] : 1.f;
+                        sV[(t * 64 + lo) * 4 + r] = W * sm * ((n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3));
+                    }
                 }
                 mm_wave_sync();
                 // (5): the owner adds its elements' terms in element order
@@ -547,10 +593,13 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                 for (int t = 0; t < kDvTiles; ++t)
 #pragma unroll
                     for (int k = 0; k < 2; ++k)
-                        if (act & (1u << (2 * t + k))) pairf += sV[(t * 64 + l) * 4 + 2 * qD + k];
+                        if (act & (1u << (2 * t + k))) pairv += sV[(t * 64 + l) * 4 + 2 * qD + k];
                 mm_wave_sync();   // this wave has read the slice before the next chunk overwrites it
             }
-            if (s0ok) accf.add(pairf * (float)(wpair * c0 * sMode[ND + s0]));
+            if (s0ok) {
+                if constexpr (F64) accd += pairv * (wpair * c0 * sMode[ND + s0]);
+                else accf.add(pairv * (float)(wpair * c0 * sMode[ND + s0]));
+            }
             // the staging overwrote zero padding of the GEMM operand buffers (set once per unit):
             // clear the prefix again once every wave is done with it
             __syncthreads();
@@ -590,6 +639,55 @@ __global__ __launch_bounds__(512) void k_lw3_mm(MmArgs A) {
     }
 }
 
+// The FP64 phase-sum task lists of every atom (format above the kernel), built once per context:
+// the outputs of the atom's non-empty blocks and a clear task per block the previous atom wrote
+// and this one does not, sorted by cluster count and laid out in rounds over the 512 lanes in
+// alternating directions, so that the lanes of a wave run equal counts and the lane totals (and
+// the shares of the two waves of a SIMD, which run theirs at different times) even out.  false
+// when the layout does not fit the task format or kFt2Rounds rounds.
+static bool ft2_schedule(Ctx &c, int nb) {
+    DevIFC &f = c.fc3;
+    if (f.ft2_tried) return f.ft2_ok;
+    f.ft2_tried = true;
+    const std::vector<int> &bs = f.h_blk_start;
+    const int nb2 = nb * nb;
+    if ((int)bs.size() < nb * nb2 + 1 || nb2 > 127) return false;
+    typedef std::pair<int, unsigned> Task;   // (cluster count, code)
+    std::vector<int> h(2 * nb);
+    for (int mu = 0; mu < nb; ++mu) {
+        const int pm = (mu + nb - 1) % nb, u0 = bs[mu * nb2];
+        std::vector<Task> v;
+        for (int bl = 0; bl < nb2; ++bl) {
+            const int b = mu * nb2 + bl, cnt = bs[b + 1] - bs[b], uo = bs[b] - u0;
+            if (cnt > 255 || uo + cnt > 2048) return false;
+            if (cnt == 0) {
+                if (bs[pm * nb2 + bl + 1] > bs[pm * nb2 + bl]) v.push_back({0, (unsigned)bl | 31u << 7});
+                continue;
+            }
+            for (int j = 0; j < 27; ++j)
+                v.push_back({cnt, (unsigned)bl | (unsigned)j << 7 | (unsigned)uo << 12 | (unsigned)cnt << 23});
+        }
+        std::stable_sort(v.begin(), v.end(), [](const Task &x, const Task &y) { return x.first > y.first; });
+        const int nr = ((int)v.size() + 511) / 512;
+        if (nr > kFt2Rounds) return false;
+        const size_t off = h.size();
+        h.resize(off + (size_t)nr * 512, (int)kFt2Empty);
+        for (size_t i = 0; i < v.size(); ++i) {
+            const int r = (int)(i / 512), k = (int)(i % 512);
+            h[off + (size_t)r * 512 + (r % 2 ? 511 - k : k)] = (int)v[i].second;
+        }
+        h[2 * mu] = (int)off;
+        h[2 * mu + 1] = nr;
+    }
+    if (hipMalloc((void **)&f.ft2, h.size() * sizeof(int)) != hipSuccess) {
+        f.ft2 = nullptr;
+        return false;
+    }
+    if (hipMemcpy(f.ft2, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return false;
+    f.ft2_ok = true;
+    return true;
+}
+
 // ItemLaunch of tile shape VAR, or false when it cannot run here (LDS, registers)
 template <int MODE, int NB, int VAR, bool WOUT>
 static bool mm_launch_of(MmArgs &A, int nitems, ItemLaunch &L) {
@@ -612,6 +710,13 @@ template <int MODE, int NB, bool WOUT = false>
 static int launch_mm_t(Ctx &c, MmArgs A, int nitems, int q0_first, int q0_count, int it0, double *d_lw) {
     ItemLaunch Ls[3];
     int n = 0;
+    // FP64: the phase sum over the task lists (SPR_FT_SCHEDULE=0 keeps the loop over every block
+    // of the atom)
+    A.ft2 = nullptr;
+    if constexpr (MODE == MM_F64) {
+        const char *fe = getenv("SPR_FT_SCHEDULE");
+        if (!(fe && std::string(fe) == "0") && ft2_schedule(c, NB)) A.ft2 = c.fc3.ft2;
+    }
     if (mm_launch_of<MODE, NB, 0, WOUT>(A, nitems, Ls[n])) ++n;
     // the larger s2 block is built for FP64 only: in FP32 / BF16 its accumulators spill
     if constexpr (MODE == MM_F64 && MmShape<MODE, NB, 1>::NTS != MmShape<MODE, NB, 0>::NTS)

@@ -171,6 +171,7 @@ static void free_ifc(DevIFC &d) {
     (void)hipFree(d.fc64_p);
     (void)hipFree(d.fc32_p);
     (void)hipFree(d.ft_sched);
+    (void)hipFree(d.ft2);
     d = DevIFC();
 }
 

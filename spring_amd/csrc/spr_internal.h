@@ -81,6 +81,9 @@ struct DevIFC {
     int *ft_sched = nullptr;
     int ft_nmax = 0;
     bool ft_tried = false, ft_ok = false;
+    // phase-sum task lists of the FP64 per-atom kernel (k_lw3_mm.hip), built on first use
+    int *ft2 = nullptr;
+    bool ft2_tried = false, ft2_ok = false;
 };
 
 constexpr int kClusterBatch = 4;

@@ -178,3 +178,42 @@ def test_s2_per_wave_kernel_10atom(ten, rocksalt10, monkeypatch, prec):
         print(f"{prec} {rules}: s2-per-wave vs per-atom max rel {e.max():.2e}", flush=True)
         assert e.max() < tol, rules
     c3.close()
+
+
+def test_fp64_phase_sum_task_lists_match_single_loop(rocksalt10, monkeypatch):
+    """FP64 per-atom kernel, 10-atom cell, 3^3 mesh with time reversal (q1 == q2 pairs): the phase
+    sum over the task lists (outputs sorted by cluster count, loads issued four clusters at a time)
+    against the loop over every block of the atom (SPR_FT_SCHEDULE=0), for both tile shapes, with
+    and without a branch selection rule, and in the W-output mode (tetrahedron path): both sum each
+    output in cluster order with the same FMAs, so the linewidths are bit-identical."""
+    from spring_amd import mesh
+    from spring_amd.lib import GpuContext, get_phonon
+    m = rocksalt10
+    q3 = mesh.make_mesh(m.crystal, (3, 3, 3), time_rev=1)
+    p3 = get_phonon(m.crystal, m.fc2, q3.q_cart, q3.q_int, 300.0)
+    c3 = GpuContext(m, q3, p3)
+    for rules in ("DDD", "ADD"):
+        c3.set_branch_rules(3, rules)
+        for sh in ("0", "1"):
+            monkeypatch.setenv("SPR_TILE_SHAPE", sh)
+            monkeypatch.delenv("SPR_FT_SCHEDULE", raising=False)
+            a = c3.lw3_all(0.1, "fp64")
+            assert c3.last_launch()[0].startswith("mm_f64")
+            monkeypatch.setenv("SPR_FT_SCHEDULE", "0")
+            b = c3.lw3_all(0.1, "fp64")
+            assert np.abs(b).max() > 0
+            if rules == "ADD":
+                assert np.all(a[:, 3:] == 0.0)
+            e = rel_err(a, b, 1e-9)
+            print(f"{rules} shape {sh}: task lists vs single loop max rel {e:.2e}", flush=True)
+            assert np.array_equal(a, b), (rules, sh)
+    monkeypatch.delenv("SPR_TILE_SHAPE")
+    c3.set_branch_rules(3, "DDD")
+    # the W-output mode (tetrahedron path of a 10-atom cell)
+    monkeypatch.delenv("SPR_FT_SCHEDULE", raising=False)
+    ta = c3.lw3_tetra_all()
+    monkeypatch.setenv("SPR_FT_SCHEDULE", "0")
+    tb = c3.lw3_tetra_all()
+    for x, y in zip(ta, tb):
+        assert np.array_equal(x, y)
+    c3.close()

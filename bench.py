@@ -153,7 +153,7 @@ def main():
         reps, dt = 0, 0.0
         t = time.perf_counter()
         while dt < args.cpu_seconds and reps < 64:
-            lw_cpu = O.lw3_gauss_all(inp, sample, args.sigma, threads=threads)
+            lw_cpu = _cpu_lw3_pairs(O, inp, qm, sample, args.sigma, threads)
             reps += 1
             dt = time.perf_counter() - t
         gs = np.abs(lw_cpu) > 1e-6 * np.abs(lw_cpu).max()
@@ -161,7 +161,8 @@ def main():
         cpu = {"value": reps * len(sample) * qm.nq * nd ** 3 / dt, "unit": "triplets/s",
                "cores": threads, "kind": "port",
                "sample": f"{reps} passes over {len(sample)} seeded irreducible q0 of the same "
-                         f"workload, all {qm.nq} q1 each (no pair-permutation saving), FP64 C "
+                         f"workload, every q1 of each through its pair-permutation reduced list (each "
+                         f"pair evaluated once for both orderings, as on the GPU), FP64 C "
                          f"restatement, {threads} threads, {dt:.1f} s wall; GPU FP64 linewidths "
                          f"agree to max rel err {cpu_err:.1e}"}
 
@@ -233,6 +234,27 @@ def _host_cpu():
     except OSError:
         pass
     return threads, cores, model_name, flags
+
+
+def _cpu_lw3_pairs(O, inp, qm, grids, sigma, threads):
+    """FP64 CPU linewidths of the given q0 (grid indices) over their pair-permutation reduced
+    lists (each pair evaluated once for both orderings, as the device does), the pairs of all q0
+    cut into one chunk per thread and the chunks summed per q0 in order."""
+    from concurrent.futures import ThreadPoolExecutor
+    from spring_amd import mesh
+    work = []
+    for j, g in enumerate(grids):
+        q1s, q2s, _ = mesh.pair_list(qm, int(g))
+        pr = np.stack([q1s, q2s], axis=1)
+        for c in np.array_split(pr, max(1, threads // len(grids))):
+            if len(c):
+                work.append((j, int(g), c))
+    with ThreadPoolExecutor(threads) as ex:
+        parts = list(ex.map(lambda w: O.lw3_gauss_pairs(inp, w[1], sigma, w[2]), work))
+    lw = np.zeros((len(grids), inp.nd))
+    for (j, _, _), p in zip(work, parts):
+        lw[j] += p
+    return lw
 
 
 def _profile_value(name, key):
@@ -344,33 +366,30 @@ def cell10(args):
 
     cpu = None
     if not args.no_cpu:
-        from concurrent.futures import ThreadPoolExecutor
         from oracle import oracle as O
         inp = O.Inputs(m, qm, ph)
         threads, cores, cpu_model, cflags = _host_cpu()
         rng = np.random.default_rng(0)
         qpos = rng.choice(qm.nq0, size=qm.nq0, replace=False)
-        # calibrate one thread on a short q1 range, then size the sample to ~args.cpu_seconds
+        # calibrate one thread on a few pairs, then size the sample (whole q0, each through its
+        # reduced pair list of about Nq/2 pairs) to ~args.cpu_seconds
+        g0 = int(qm.irr_idx[qpos[0]])
+        q1s, q2s, _ = mesh.pair_list(qm, g0)
         t = time.perf_counter()
-        O.lw3_gauss(inp, int(qm.irr_idx[qpos[0]]), args.sigma, 0, 8)
+        O.lw3_gauss_pairs(inp, g0, args.sigma, np.stack([q1s[:8], q2s[:8]], axis=1))
         per_pair = (time.perf_counter() - t) / 8
         pairs = max(threads, int(args.cpu_seconds * threads / per_pair))
-        nq0_s = max(1, min(len(qpos), pairs // qm.nq))
-        chunk = (qm.nq + threads - 1) // threads if nq0_s * qm.nq <= 2 * pairs else qm.nq
-        work = [(int(qm.irr_idx[qpos[j]]), lo, min(qm.nq, lo + chunk)) for j in range(nq0_s)
-                for lo in range(0, qm.nq, chunk)]
+        nq0_s = max(1, min(len(qpos), pairs // max(1, len(q1s))))
         t = time.perf_counter()
-        with ThreadPoolExecutor(threads) as ex:
-            parts = list(ex.map(lambda w: O.lw3_gauss(inp, w[0], args.sigma, w[1], w[2]), work))
+        lw_cpu = _cpu_lw3_pairs(O, inp, qm, qm.irr_idx[qpos[:nq0_s]], args.sigma, threads)
         dt = time.perf_counter() - t
-        lw_cpu = np.array([np.sum(parts[j * len(work) // nq0_s:(j + 1) * len(work) // nq0_s], axis=0)
-                           for j in range(nq0_s)])
         gs = np.abs(lw_cpu) > 1e-6 * np.abs(lw_cpu).max()
         cpu_err = float(np.max(np.abs(r64["lw"][qpos[:nq0_s]] - lw_cpu)[gs] / np.abs(lw_cpu)[gs]))
         cpu = {"value": nq0_s * qm.nq * nd ** 3 / dt, "unit": "triplets/s", "cores": threads,
                "kind": "port", "host_cores_visible": cores, "cpu_model": cpu_model, "compiler": cflags,
-               "sample": f"{nq0_s} seeded irreducible q0 of the same workload, all {qm.nq} q1 each (no "
-                         f"pair-permutation saving), split into q1 ranges over {threads} threads, FP64 C "
+               "sample": f"{nq0_s} seeded irreducible q0 of the same workload, every q1 of each "
+                         f"through its pair-permutation reduced list (each pair evaluated once for both "
+                         f"orderings, as on the GPU), pairs split over {threads} threads, FP64 C "
                          f"restatement, {dt:.1f} s wall; GPU FP64 linewidths of those q0 agree to max rel "
                          f"err {cpu_err:.1e}"}
     traffic = _profile_value("k_lw3_mm_cell10.md", f"traffic_bytes_per_launch_fp64_{n}_{steps}")

@@ -18,6 +18,8 @@
  *   orc_scatter_prob       constants recomputed from their formulas; transpose identity
  *   orc_lw3_gauss          Gaussian against scipy.stats.norm.pdf on hand-built W;
  *                          pair-swap symmetry; brute force over all q1 on tiny meshes
+ *   orc_lw3_gauss_pairs    == orc_lw3_gauss over all q1, with the reduced pair list, split lists
+ *                          and branch rules (the transposed W stands for the swapped pair)
  *   orc_scatter_mat_el4 / orc_v4_factorised   as the 3-phonon ones
  *   orc_lw4_gauss          process factors and weights (tests/test_lw4_process_terms.py): on
  *                          flat integer bands, exactly on or off shell, the ordered sum equals
@@ -277,6 +279,53 @@ void orc_lw3_gauss(int nb, int natm, const int *atm_num, const int *atm_indx,
         orc_lw3_from_w(nd, W, omega + (size_t)i0 * nd, omega + (size_t)q1 * nd, omega + (size_t)q2 * nd,
                        nbe + (size_t)q1 * nd, nbe + (size_t)q2 * nd, sigma, pol, lwq);
         for (int s0 = 0; s0 < nd; ++s0) lw[s0] += lwq[s0];
+    }
+    free(phi);
+    free(W);
+    free(Wp);
+}
+
+/* --------------------------------------------------------------------------------------
+ * The same linewidth over the pair-permutation reduced list, as ScatterProbAllq1 evaluates it
+ * (ThirdOrder_m.f90:98-164): each listed pair (q1, q2), q2 = -(q0+q1) mod G, is evaluated once;
+ * its W is the scattering matrix of q1 and its transpose W_per that of q2 (:134,160), so the
+ * pair adds the Gaussian sum of both orderings (only one when q1 == q2).  With the list of every
+ * q1 <= q2 this equals orc_lw3_gauss over all q1.  pairs[2k], pairs[2k+1] = (q1, q2).
+ * ------------------------------------------------------------------------------------*/
+void orc_lw3_gauss_pairs(int nb, int natm, const int *atm_num, const int *atm_indx,
+                         const double *fc3, const double *latvec, const double *osm,
+                         int nq, const int *q_int, const double *q_cart,
+                         const cplx *evec, const double *omega, const double *inv_omega,
+                         const double *nbe, int i0, double sigma, const int *pairs, int npairs,
+                         const int *pol, double *lw)
+{
+    const int nd = 3 * nb;
+    const size_t n3 = (size_t)nd * nd * nd;
+    cplx *phi = malloc(n3 * sizeof(cplx));
+    double *W = malloc(n3 * sizeof(double));
+    double *Wp = malloc(n3 * sizeof(double));
+    const double factor = SCTR3 / (double)nq;
+    for (int s = 0; s < nd; ++s) lw[s] = 0.0;
+    const int *k0 = q_int + 3 * i0;
+    const int q0g = k0[0] == 0 && k0[1] == 0 && k0[2] == 0;
+    for (int k = 0; k < npairs; ++k) {
+        const int q1 = pairs[2 * k], q2 = pairs[2 * k + 1];
+        const int *k1 = q_int + 3 * q1, *k2 = q_int + 3 * q2;
+        int chk[3] = {q0g, k1[0] == 0 && k1[1] == 0 && k1[2] == 0, k2[0] == 0 && k2[1] == 0 && k2[2] == 0};
+        orc_v3_factorised(nb, natm, atm_num, atm_indx, fc3, latvec, osm, q_cart + 3 * q1,
+                          q_cart + 3 * q2, chk, evec + (size_t)i0 * nd * nd,
+                          evec + (size_t)q1 * nd * nd, evec + (size_t)q2 * nd * nd, phi);
+        orc_scatter_prob(nd, chk, inv_omega + (size_t)i0 * nd, inv_omega + (size_t)q1 * nd,
+                         inv_omega + (size_t)q2 * nd, phi, factor, W, Wp);
+        double lwq[64];
+        orc_lw3_from_w(nd, W, omega + (size_t)i0 * nd, omega + (size_t)q1 * nd, omega + (size_t)q2 * nd,
+                       nbe + (size_t)q1 * nd, nbe + (size_t)q2 * nd, sigma, pol, lwq);
+        for (int s0 = 0; s0 < nd; ++s0) lw[s0] += lwq[s0];
+        if (q1 != q2) {
+            orc_lw3_from_w(nd, Wp, omega + (size_t)i0 * nd, omega + (size_t)q2 * nd, omega + (size_t)q1 * nd,
+                           nbe + (size_t)q2 * nd, nbe + (size_t)q1 * nd, sigma, pol, lwq);
+            for (int s0 = 0; s0 < nd; ++s0) lw[s0] += lwq[s0];
+        }
     }
     free(phi);
     free(W);

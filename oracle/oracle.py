@@ -136,6 +136,20 @@ def lw3_gauss(inp: Inputs, i0_grid: int, sigma: float, q1_lo=0, q1_hi=None, want
     return (lw, W) if want_w else lw
 
 
+def lw3_gauss_pairs(inp: Inputs, i0_grid: int, sigma: float, pairs, pol=None):
+    """Linewidth of one q0 over a list of (q1, q2) pairs, each evaluated once and counted for both
+    orderings (ScatterProbAllq1's pair-permutation reduction); pairs: (n, 2) grid indices."""
+    pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+    lw = np.zeros(inp.nd)
+    pa, pp = _pol(pol)
+    lib().orc_lw3_gauss_pairs(ct.c_int(inp.nb), ct.c_int(inp.natm), inp.p_atm_num, inp.p_atm_indx,
+                              inp.p_fc, inp.p_latvec, inp.p_osm, ct.c_int(inp.nq), inp.p_q_int,
+                              inp.p_q_cart, inp.p_evec, inp.p_omega, inp.p_inv_omega, inp.p_nbe,
+                              ct.c_int(i0_grid), ct.c_double(sigma), pairs.ctypes.data_as(ct.c_void_p),
+                              ct.c_int(len(pairs)), pp, lw.ctypes.data_as(ct.c_void_p))
+    return lw
+
+
 def lw4_gauss(inp: Inputs, i0_grid: int, sigma: float, q1_lo=0, q1_hi=None, pol=None):
     q1_hi = inp.nq if q1_hi is None else q1_hi
     lw = np.zeros(inp.nd)

@@ -174,6 +174,39 @@ def test_lw3_pair_swap_symmetry(tiny):
         assert np.allclose(red, full, rtol=1e-12, atol=1e-300)
 
 
+def test_lw3_gauss_pairs_equals_full_sum(tiny, rocksalt):
+    """orc_lw3_gauss_pairs (each reduced pair evaluated once, the transposed W standing for the
+    swapped pair, as ScatterProbAllq1 stores it) equals orc_lw3_gauss over every q1: at Gamma and
+    off it, on an odd mesh (3^3) and an even one (4^3, q1 == q2 pairs), with a branch rule, and
+    with the list split in two, to 1e-10 (measured 1.6e-12: the full sum evaluates V3 of the
+    swapped ordering on its own, and the (n1 - n2) terms cancel).  Dropping the swapped ordering
+    moves the sum by order one."""
+    m, qm, ph, inp = tiny
+    q4, p4 = problem(rocksalt, (4, 4, 4))
+    i4 = O.Inputs(rocksalt, q4, p4)
+    nonzero = 0
+    for q, inpx in ((qm, inp), (q4, i4)):
+        for i0 in (0, 1, q.nq0 - 1):
+            g = int(q.irr_idx[i0])
+            q1s, q2s, _ = mesh.pair_list(q, g)
+            pairs = np.stack([q1s, q2s], axis=1)
+            for pol in (None, (0, 5, 0, 2, 0, 5)):
+                full = O.lw3_gauss(inpx, g, 0.1, pol=pol)
+                red = O.lw3_gauss_pairs(inpx, g, 0.1, pairs, pol=pol)
+                half = O.lw3_gauss_pairs(inpx, g, 0.1, pairs[::2], pol=pol) + \
+                    O.lw3_gauss_pairs(inpx, g, 0.1, pairs[1::2], pol=pol)
+                scale = np.abs(full).max()
+                nonzero += scale > 0
+                assert np.allclose(red, full, rtol=1e-10, atol=1e-14 * scale)
+                assert np.allclose(half, full, rtol=1e-10, atol=1e-14 * scale)
+    assert nonzero >= 9
+    # the swapped ordering matters: counting each pair once (no transpose) gives a different sum
+    g = int(q4.irr_idx[1])
+    q1s, q2s, w = mesh.pair_list(q4, g)
+    once = sum(O.lw3_gauss(i4, g, 0.1, int(a), int(a) + 1) for a in q1s)
+    assert not np.allclose(once, O.lw3_gauss(i4, g, 0.1), rtol=1e-3)
+
+
 def test_lw3_fc_scaling(tiny):
     m, qm, ph, inp = tiny
     lw = O.lw3_gauss(inp, int(qm.irr_idx[2]), 0.1)

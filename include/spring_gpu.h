@@ -135,8 +135,9 @@ int spr_lw3_slice(spr_ctx *ctx, int q0_off, int q0_cnt, double sigma, int prec, 
 
 /* Tetrahedron-method three-phonon linewidth of irreducible q0 `i0` (CalculateLwTetrahedron,
  * ThirdOrder_m.f90:169-275, the reference's default delta): lw(Ndof) rad/ps and P3_q0(Ndof);
- * the reference's P3 of the q0 is sum(P3_q0).  W of every q1 comes from the FP64 W path of
- * spr_v3_w; FP64 throughout.  Needs mesh->tetra. */
+ * the reference's P3 of the q0 is sum(P3_q0).  W of every q1 is built on the device (the W-output
+ * mode of the small-cell kernel for 1- and 2-atom cells, of the FP64 matrix-core kernel for 3 to
+ * 10 atoms, the generic W path otherwise); FP64 throughout.  Needs mesh->tetra. */
 int spr_lw3_tetra(spr_ctx *ctx, int i0, double *lw, double *p3_q0);
 /* every irreducible q0: lw_all(Ndof, nq0), p3_all(Ndof, nq0) */
 int spr_lw3_tetra_all(spr_ctx *ctx, double *lw_all, double *p3_all);

@@ -490,11 +490,7 @@ def learned10(args):
         q0s, q1s, ws = np.concatenate(q0s), np.concatenate(q1s), np.concatenate(ws)
         t_draw = time.perf_counter() - t
         t = time.perf_counter()
-        est = np.zeros((qm.nq0, nd))
-        chunk = 400_000
-        for lo in range(0, q0s.size, chunk):
-            tt = ctx.lw3_pairs(q0s[lo:lo + chunk], q1s[lo:lo + chunk], args.sigma, args.pair_prec)
-            np.add.at(est, q0s[lo:lo + chunk], tt * ws[lo:lo + chunk, None])
+        est = ctx.lw3_sampled(q0s, q1s, ws, args.sigma, args.pair_prec)
         k_est, _ = ctx.kappa(est, args.T, symm=True)
         t_pairs = time.perf_counter() - t
         err = np.abs(est - lw_full)[sig] / np.abs(lw_full)[sig]

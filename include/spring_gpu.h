@@ -261,6 +261,16 @@ int spr_lw3_kappa_all(spr_ctx *ctx, double sigma, int prec, double T, int symm, 
 int spr_lw3_pairs(spr_ctx *ctx, int n, const int *q0_index, const int *q1_grid, double sigma,
                   int prec, double *out);
 
+/* Weighted sums of those terms per irreducible q0, the importance-sampled estimate of the
+ * linewidth: est(s0, qi) = sum over the n samples k with q0_index(k) == qi of
+ * weight(k) * out(s0, k) of spr_lw3_pairs (est(Ndof, nq0), zero for q0 with no sample).  The
+ * reference evaluates only the full sum over q1 (CalculateLinewidth, ThirdOrder_m.f90:442-543,
+ * the q1 loop of ScatterProbAllq1 :98-164); this entry replaces that loop by the sampled sum the
+ * learned proposal drives.  The samples are grouped by q0 into 64-pair work items that run on
+ * the same kernels as the full sum, each pair weighted inside the kernel.  Weights must be finite. */
+int spr_lw3_sampled(spr_ctx *ctx, int n, const int *q0_index, const int *q1_grid, const double *weight,
+                    double sigma, int prec, double *est);
+
 /* spr_lw3_pairs with no device allocation: the same terms for the same arguments (same kernels,
  * same summation order), with every device buffer it needs beyond the context's resident tables
  * held in a fixed library scratch array of each GPU (about 5 MB, kept until the process ends).

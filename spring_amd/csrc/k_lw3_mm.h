@@ -44,6 +44,8 @@ struct MmArgs {
     int ft_nmax;
     // FP64 per-atom kernel: phase-sum task lists (NULL: every block of the atom by one loop)
     const int *ft2;
+    // per-pair weights of caller-supplied items (NULL: 1 for q1 == q2, else 2)
+    const double *pair_w;
 };
 
 constexpr int pow2ceil(int x) { return x <= 1 ? 1 : x <= 2 ? 2 : x <= 4 ? 4 : 8; }

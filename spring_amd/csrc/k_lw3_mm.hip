@@ -165,7 +165,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
     for (int pi = 0; pi < cnt; ++pi) {
         const int2 pr = A.pairs[poff + pi];
         const int q1 = pr.x, q2 = pr.y;
-        const double wpair = q1 == q2 ? 1.0 : 2.0;
+        const double wpair = A.pair_w ? A.pair_w[poff + pi] : q1 == q2 ? 1.0 : 2.0;
         __syncthreads();   // the previous pair is done with sE1, sE2, sMode, sM1, sM2
         for (int i = tid; i < ND * ND; i += 512) {
             const int s = i / ND, c = i % ND;
@@ -730,6 +730,11 @@ static int launch_mm_t(Ctx &c, MmArgs A, int nitems, int q0_first, int q0_count,
         return SPR_E_UNSUPPORTED;
     }
     for (int k = 0; k < n; ++k) Ls[k].direct = c.override_items != nullptr;
+    // caller-supplied items run directly on one candidate: for items of one pair each the first
+    // (the per-atom kernel, four units per pair in FP64); for whole 64-pair items the last built
+    // (the s2-per-wave kernel in FP32 / BF16, the larger s2 block in FP64), the shapes the tuner
+    // picks for the context's own items at Ndof = 30
+    if (c.override_items && c.override_items->dense) return run_items_shapes(c, Ls + (n - 1), 1, q0_first, q0_count, it0, d_lw);
     return run_items_shapes(c, Ls, n, q0_first, q0_count, it0, d_lw);
 }
 
@@ -750,6 +755,7 @@ int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, do
     A.item_off = ov ? ov->item_off : c.d_item_off + it0;
     A.item_cnt = ov ? ov->item_cnt : c.d_item_cnt + it0;
     A.pairs = ov ? ov->pairs : c.d_pairs;
+    A.pair_w = ov ? ov->pair_w : nullptr;
     A.q0_grid = c.irr_grid;
     A.L = c.L;
     A.m0 = c.mesh[0];
@@ -813,6 +819,7 @@ int launch_lw3_mm_w(Ctx &c, int q_first, int q_count, double *d_w) {
     A.item_off = c.d_item_off + it0;
     A.item_cnt = c.d_item_cnt + it0;
     A.pairs = c.d_pairs;
+    A.pair_w = nullptr;
     A.q0_grid = c.irr_grid;
     A.L = c.L;
     A.m0 = c.mesh[0];

@@ -562,7 +562,7 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
                         if (act & (1u << (2 * a + k))) pairf[nt] += sV[(a * 64 + l) * 4 + 2 * qD + k];
                 mm_wave_sync();   // this wave has read the tile before the next one overwrites it
             }
-        const double wpair = q1 == q2 ? 1.0 : 2.0;
+        const double wpair = A.pair_w ? A.pair_w[poff + pi] : q1 == q2 ? 1.0 : 2.0;
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt) {
             const int s0 = 16 * nt + ((l & 31) >> 1);
@@ -672,7 +672,7 @@ static bool mx_launch_of_t(Ctx &c, MmArgs &A, int nitems, ItemLaunch &L) {
 }
 
 bool mx_launch_of(Ctx &c, int mode, int nb, MmArgs &A, int nitems, ItemLaunch &L) {
-    if (nb != 10 || c.override_items) return false;
+    if (nb != 10 || (c.override_items && !c.override_items->dense)) return false;
     return mode == MM_F32 ? mx_launch_of_t<MM_F32, 10>(c, A, nitems, L)
            : mode == MM_BF16 ? mx_launch_of_t<MM_BF16, 10>(c, A, nitems, L)
                              : false;

@@ -51,6 +51,8 @@ struct SmallArgs {
     int w_q0_first, w_nq;
     // branch selection (NULL: every branch): (nd) s0 mask, then the (nd, nd) pair mask [s1][s2]
     const double *sel;
+    // per-pair weights of caller-supplied items (NULL: 1 for q1 == q2, else 2)
+    const double *pair_w;
 };
 
 template <typename R, int NB, bool WOUT>
@@ -83,7 +85,7 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
     const int cnt = A.item_cnt[item];
     const int2 pr = A.pairs[A.item_off[item] + (lane < cnt ? lane : 0)];
     const int q1 = pr.x, q2 = pr.y;
-    const double wpair = lane < cnt ? (q1 == q2 ? 1.0 : 2.0) : 0.0;
+    const double wpair = lane >= cnt ? 0.0 : A.pair_w ? A.pair_w[A.item_off[item] + lane] : q1 == q2 ? 1.0 : 2.0;
     const Z *evec = reinterpret_cast<const Z *>(A.evec);
     const Z *phase = reinterpret_cast<const Z *>(A.phase);
 
@@ -409,6 +411,7 @@ int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec,
     A.w_out = nullptr;
     A.w_q0_first = A.w_nq = 0;
     A.sel = c.sel3 ? c.d_sel3 : nullptr;
+    A.pair_w = ov ? ov->pair_w : nullptr;
     const bool f64 = prec == SPR_PREC_FP64;
     if (c.nb == 2)
         return f64 ? launch_small_t<double, 2>(c, A, nitems, q0_first, q0_count, it0, d_lw)
@@ -456,6 +459,7 @@ int launch_lw3_small_w(Ctx &c, int q_first, int q_count, double *d_w) {
     A.ic.partial = c.ipartial;
     A.w_out = d_w;
     A.sel = nullptr;   // W output: the tetrahedron sums apply the branch ranges themselves
+    A.pair_w = nullptr;
     A.w_q0_first = q_first;
     A.w_nq = c.nq;
     // the per-item sums of this launch are zero; they are reduced into a scratch row per q0

@@ -616,6 +616,24 @@ extern "C" int spr_lw3_kappa_all(spr_ctx *h, double sigma, int prec, double T, i
 
 namespace spr {
 int launch_lw3_pairs(Ctx &c, int n, const int *q0_index, const int *q1_grid, double sigma, int prec, double *h_out);
+int launch_lw3_sampled(Ctx &c, int n, const int *q0_index, const int *q1_grid, const double *weight, double sigma,
+                       int prec, double *h_est);
+}
+
+extern "C" int spr_lw3_sampled(spr_ctx *h, int n, const int *q0_index, const int *q1_grid, const double *weight,
+                               double sigma, int prec, double *est) {
+    if (!h || n < 0 || !est || (n > 0 && (!q0_index || !q1_grid || !weight))) {
+        set_error("spr_lw3_sampled: bad argument");
+        return SPR_E_ARG;
+    }
+    int rc;
+    if ((rc = check_prec(prec))) return rc;
+    if (!(sigma > 0.0)) {
+        set_error("sigma must be positive");
+        return SPR_E_ARG;
+    }
+    SPR_HIP(hipSetDevice(h->c.device));
+    return launch_lw3_sampled(h->c, n, q0_index, q1_grid, weight, sigma, prec, est);
 }
 
 extern "C" int spr_lw3_pairs(spr_ctx *h, int n, const int *q0_index, const int *q1_grid, double sigma, int prec,

@@ -52,6 +52,11 @@ struct ItemSet {
     const int2 *pairs;
     int nitems;
     double *partial;
+    // per-pair weight in place of the kernels' 1 (q1 == q2) or 2 (NULL: those)
+    const double *pair_w = nullptr;
+    // items hold up to 64 pairs of one q0 as the context's own lists do (the kernels that take
+    // whole items are preferred), rather than one pair each
+    bool dense = false;
 };
 
 // Anharmonic IFC re-laid out for the phase sum.  The sum over (N1, N2[, N3]) is grouped by

@@ -182,10 +182,10 @@ def is_rms_error(terms, p, K):
 
 
 def sample_estimate(ctx, qi, p, K, sigma, prec, rng):
-    """Draw K q1 from p, evaluate their pair terms on the GPU, return the estimate (Ndof)."""
+    """Draw K q1 from p and return the importance estimate (1/K) sum t(q1)/p(q1) (Ndof), the
+    weighted sum formed on the GPU (spr_lw3_sampled)."""
     q1 = rng.choice(p.size, size=K, replace=True, p=p)
-    t = ctx.lw3_pairs(np.full(K, qi), q1, sigma, prec)
-    return (t / p[q1][:, None]).mean(axis=0)
+    return ctx.lw3_sampled(np.full(K, qi), q1, 1.0 / (K * p[q1]), sigma, prec)[qi]
 
 
 PAIR_NF = 12

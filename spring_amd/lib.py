@@ -24,7 +24,7 @@ SYMBOLS = [
     "spr_lw4_tetra", "spr_lw4_tetra_all", "spr_lw3_kappa_all", "spr_mlp_score",
     "spr_bte_irr", "spr_bte_tau_inv", "spr_bte_delta_f", "spr_bte_solve", "spr_get_phonon",
     "spr_lw3_slice", "spr_lw3_pairs_noalloc", "spr_set_branch_rules", "spr_kappa_rta_spectral",
-    "spr_lw3_offmesh", "spr_mesh_build", "spr_pair_scores",
+    "spr_lw3_offmesh", "spr_mesh_build", "spr_pair_scores", "spr_lw3_sampled",
 ]
 
 
@@ -240,6 +240,19 @@ class GpuContext:
         self._check(self.lib.spr_lw3_pairs(self.h, ct.c_int(q0.size), _ptr(q0), _ptr(q1), ct.c_double(sigma),
                                            ct.c_int(PREC[prec]), _ptr(out)), "spr_lw3_pairs")
         return out
+
+    def lw3_sampled(self, q0_index, q1_grid, weight, sigma: float, prec: str = "fp32") -> np.ndarray:
+        """Weighted sums of the q1 terms per irreducible q0, (nq0, Ndof): row qi is the sum of
+        weight[k] * lw3_pairs term k over the samples k with q0_index[k] == qi (spr_lw3_sampled)."""
+        q0 = np.ascontiguousarray(q0_index, np.int32)
+        q1 = np.ascontiguousarray(q1_grid, np.int32)
+        w = np.ascontiguousarray(weight, np.float64)
+        if not (q0.size == q1.size == w.size):
+            raise ValueError("lw3_sampled: q0_index, q1_grid and weight differ in length")
+        est = np.zeros((self.nq0, self.nd))
+        self._check(self.lib.spr_lw3_sampled(self.h, ct.c_int(q0.size), _ptr(q0), _ptr(q1), _ptr(w),
+                                             ct.c_double(sigma), ct.c_int(PREC[prec]), _ptr(est)), "spr_lw3_sampled")
+        return est
 
     def lw3_pairs_noalloc(self, q0_index, q1_grid, sigma: float, prec: str = "fp32") -> np.ndarray:
         """lw3_pairs through the entry point that allocates no device memory (at most

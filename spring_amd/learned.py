@@ -154,15 +154,20 @@ def mixture(S, lam=0.1):
 
 def head_tail_draw(S, K, head_frac, rng, lam=0.1):
     """K pair evaluations for one q0 split into a deterministic head and a sampled tail: the
-    H = round(head_frac K) q1 of highest score are evaluated exactly (weight 1), the other
+    H = round(head_frac K) q1 of highest score (any H of them when scores tie) are evaluated
+    exactly (weight 1), the other
     K - H are drawn with replacement from the mixture proposal restricted to the remaining q1
     (weight 1 / ((K - H) p)).  sum(weight * term) is an unbiased estimate of the full sum; the
     tail only has to cover what the head leaves, so a score that ranks the large terms first
     cuts the variance.  Returns (q1 indices, weights)."""
     nq = S.size
     H = min(nq, int(round(head_frac * K)))
-    order = np.argsort(-S, kind="stable")
-    head, rest = order[:H], order[H:]
+    if 0 < H < nq:   # the H highest scores (any H of them when scores tie), in linear time
+        part = np.argpartition(-S, H - 1)
+        head, rest = part[:H], np.sort(part[H:])
+    else:
+        order = np.arange(nq)
+        head, rest = order[:H], order[H:]
     M = K - H
     if M <= 0 or rest.size == 0:
         return head, np.ones(head.size)

@@ -160,3 +160,4 @@ def test_head_tail_draw_is_unbiased():
     assert abs(est.mean() - t.sum()) < 4 * est.std() / np.sqrt(est.size)
     q, w = L.head_tail_draw(S, 400, 1.0, rng)
     assert np.isclose((t[q] * w).sum(), t.sum(), rtol=1e-12)
+

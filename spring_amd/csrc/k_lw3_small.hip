@@ -20,6 +20,7 @@
 //      q0 in a fixed order (bitwise reproducible).
 #include <algorithm>
 #include <cmath>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -320,25 +321,54 @@ static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first,
 int ensure_pairs(Ctx &c) {
     if (c.pairs_built) return SPR_OK;
     const int nq = c.nq, m0 = c.mesh[0], m1 = c.mesh[1], m2 = c.mesh[2];
-    std::vector<int2> pairs;
-    std::vector<int> iq0, ioff, icnt, first(c.nq0 + 1, 0);
-    pairs.reserve((size_t)c.nq0 * (nq / 2 + 8));
-    for (int qi = 0; qi < c.nq0; ++qi) {
+    // q2 = -(q0 + q1) mod G per grid coordinate from three small tables per q0 (no divisions);
+    // the q0 are split over host threads, first counting the pairs of each q0, then filling
+    // each q0's range of the list, so the list is the same as a single pass in q0 order
+    auto for_pairs = [&](int qi, auto &&emit) {
         const int i0 = c.h_irr_grid[qi];
         const int k0 = i0 % m0, k1 = (i0 / m0) % m1, k2 = i0 / (m0 * m1);
-        const size_t start = pairs.size();
-        for (int q1 = 0; q1 < nq; ++q1) {
-            const int a0 = ((-(k0 + q1 % m0)) % m0 + m0) % m0;
-            const int a1 = ((-(k1 + (q1 / m0) % m1)) % m1 + m1) % m1;
-            const int a2 = ((-(k2 + q1 / (m0 * m1))) % m2 + m2) % m2;
-            const int q2 = a0 + m0 * (a1 + m1 * a2);
-            if (q1 <= q2) pairs.push_back(make_int2(q1, q2));
-        }
+        std::vector<int> n0(m0), n1(m1), n2(m2);
+        for (int x = 0; x < m0; ++x) n0[x] = (2 * m0 - k0 - x) % m0;
+        for (int y = 0; y < m1; ++y) n1[y] = m0 * ((2 * m1 - k1 - y) % m1);
+        for (int z = 0; z < m2; ++z) n2[z] = m0 * m1 * ((2 * m2 - k2 - z) % m2);
+        int q1 = 0;
+        for (int z = 0; z < m2; ++z)
+            for (int y = 0; y < m1; ++y) {
+                const int b = n1[y] + n2[z];
+                for (int x = 0; x < m0; ++x, ++q1) {
+                    const int q2 = n0[x] + b;
+                    if (q1 <= q2) emit(q1, q2);
+                }
+            }
+    };
+    const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    auto parallel_q0 = [&](auto &&body) {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nth; ++t)
+            th.emplace_back([&, t]() {
+                for (int qi = t; qi < c.nq0; qi += nth) body(qi);
+            });
+        for (auto &x : th) x.join();
+    };
+    std::vector<size_t> pstart(c.nq0 + 1, 0);
+    parallel_q0([&](int qi) {
+        size_t n = 0;
+        for_pairs(qi, [&](int, int) { ++n; });
+        pstart[qi + 1] = n;
+    });
+    for (int qi = 0; qi < c.nq0; ++qi) pstart[qi + 1] += pstart[qi];
+    std::vector<int2> pairs(pstart[c.nq0]);
+    parallel_q0([&](int qi) {
+        size_t at = pstart[qi];
+        for_pairs(qi, [&](int q1, int q2) { pairs[at++] = make_int2(q1, q2); });
+    });
+    std::vector<int> iq0, ioff, icnt, first(c.nq0 + 1, 0);
+    for (int qi = 0; qi < c.nq0; ++qi) {
         first[qi] = (int)iq0.size();
-        for (size_t o = start; o < pairs.size(); o += 64) {
+        for (size_t o = pstart[qi]; o < pstart[qi + 1]; o += 64) {
             iq0.push_back(qi);
             ioff.push_back((int)o);
-            icnt.push_back((int)std::min<size_t>(64, pairs.size() - o));
+            icnt.push_back((int)std::min<size_t>(64, pstart[qi + 1] - o));
         }
     }
     first[c.nq0] = (int)iq0.size();

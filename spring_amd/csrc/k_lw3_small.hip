@@ -56,7 +56,9 @@ struct SmallArgs {
     const double *pair_w;
 };
 
-template <typename R, int NB, bool WOUT>
+// SEL: a branch selection rule is set (A.sel); a compile-time flag, because a runtime test in the
+// delta loop costs this register-bound kernel about 40 % (measured in FP64)
+template <typename R, int NB, bool WOUT, bool SEL>
 __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, const R *__restrict__ fc,
                                            const int *__restrict__ blk_start, const int *__restrict__ tup_site) {
     using Z = cplx_t<R>;
@@ -227,7 +229,7 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
                 continue;
             }
             double g = (n1 + n2 + 1.0) * (c0 * exp(-t1)) + (n1 - n2) * (c0 * exp(-t2) - c0 * exp(-t3));
-            if (A.sel) g *= A.sel[ND + s1 * ND + s2];
+            if constexpr (SEL) g *= A.sel[ND + s1 * ND + s2];
             if (sizeof(R) == 8)
                 dacc[k] += wpair * W * g;
             else
@@ -249,17 +251,17 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
         const int h = threadIdx.x / 3, k = threadIdx.x % 3;
         double sum = 0.0;
         for (int sx = 0; sx < ND; ++sx) sum += s_red[(sx * NB + h) * 3 + k];
-        A.ic.partial[(size_t)item * ND + threadIdx.x] = A.sel ? sum * A.sel[threadIdx.x] : sum;
+        A.ic.partial[(size_t)item * ND + threadIdx.x] = SEL ? sum * A.sel[threadIdx.x] : sum;
     }
 }
 
 // One workgroup per item, or (counter set) a persistent grid pulling items until none are left.
-template <typename R, int NB, bool WOUT>
+template <typename R, int NB, bool WOUT, bool SEL>
 __global__ __launch_bounds__(NB * NB * 3 * 64) void k_lw3_small(SmallArgs A, const R *__restrict__ fc,
                                                                const int *__restrict__ blk_start,
                                                                const int *__restrict__ tup_site) {
     if (!A.ic.counter) {
-        small_item<R, NB, WOUT>(A, blockIdx.x, fc, blk_start, tup_site);
+        small_item<R, NB, WOUT, SEL>(A, blockIdx.x, fc, blk_start, tup_site);
         return;
     }
     __shared__ int s_item;
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(NB * NB * 3 * 64) void k_lw3_small(SmallArgs A, con
         __syncthreads();
         const int item = s_item;
         if (item >= A.ic.nitems) break;
-        small_item<R, NB, WOUT>(A, item, fc, blk_start, tup_site);
+        small_item<R, NB, WOUT, SEL>(A, item, fc, blk_start, tup_site);
     }
 }
 
@@ -300,7 +302,9 @@ static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first,
         set_error("small-cell kernel: LDS need exceeds 160 KB (too many neighbour sites)");
         return SPR_E_UNSUPPORTED;
     }
-    const void *fn = wout ? (const void *)k_lw3_small<R, NB, true> : (const void *)k_lw3_small<R, NB, false>;
+    const void *fn = wout     ? (const void *)k_lw3_small<R, NB, true, false>
+                     : A.sel ? (const void *)k_lw3_small<R, NB, false, true>
+                             : (const void *)k_lw3_small<R, NB, false, false>;
     if (dyn > 64 * 1024) SPR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
     int per_cu = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NW * 64, dyn) != hipSuccess || per_cu < 1)
@@ -308,7 +312,7 @@ static int launch_small_t(Ctx &c, const SmallArgs &A0, int nitems, int q0_first,
     const R *fcp = reinterpret_cast<const R *>(A.fc);
     const int *bsp = A.blk_start, *tsp = A.tup_site;
     ItemLaunch L{fn, &A, sizeof(A), NW * 64, dyn, nitems, per_cu,
-                 std::string("small") + (sizeof(R) == 8 ? "64" : "32") + (wout ? "w" : "")};
+                 std::string("small") + (sizeof(R) == 8 ? "64" : "32") + (wout ? "w" : A.sel ? "s" : "")};
     L.extra[0] = (void *)&fcp;
     L.extra[1] = (void *)&bsp;
     L.extra[2] = (void *)&tsp;

@@ -51,7 +51,9 @@ struct Lw4Args {
     const double *sel;   // branch selection (NULL: all): (4, nd) masks of s0, s1, s2, s3
 };
 
-template <typename R, int ND, int BLK, bool WST>
+// SEL: a branch selection rule is set (A.sel), a compile-time flag: the runtime test costs the
+// FP64 kernel about a fifth of its time (measured)
+template <typename R, int ND, int BLK, bool WST, bool SEL = false>
 __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
     using Z = cplx_t<R>;
     constexpr int NB = ND / 3;
@@ -226,7 +228,7 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
                 const double W = (double)cabs2(src[e]) * (s_md[0][2][s0] * s_md[1][2][s1] * s_md[2][2][s2] *
                                                           s_md[3][2][s3]) * A.factor;
                 double m1 = 1.0, m2a = 1.0, m2b = 1.0, m2c = 1.0, m3a = 1.0, m3b = 1.0, m3c = 1.0;
-                if (A.sel) {
+                if constexpr (SEL) {
                     const double *S1 = A.sel + ND, *S2 = A.sel + 2 * ND, *S3 = A.sel + 3 * ND;
                     const double p123 = S1[s1] * S2[s2] * S3[s3], p132 = S1[s1] * S2[s3] * S3[s2];
                     const double p213 = S1[s2] * S2[s1] * S3[s3], p231 = S1[s2] * S2[s3] * S3[s1];
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
                 t += fin(m3b * W * (1.0 + n1) * (1.0 + n3) * n2 * r0 * gs(w0 + w1 + w3 - w2));
                 t += fin(m3a * W * (1.0 + n2) * (1.0 + n3) * n1 * r0 * gs(w0 + w2 + w3 - w1));
                 t *= istab;
-                if (A.sel) t *= A.sel[s0];
+                if constexpr (SEL) t *= A.sel[s0];
 #pragma unroll
                 for (int s = 0; s < ND; ++s)
                     if (s == s0) acc[s] += t;
@@ -330,7 +332,10 @@ static int launch_lw4_t(Ctx &c, int q0_first, int q0_count, double sigma, double
     A.wst = nullptr;
     A.q1_lo = A.q2_chunk = A.nq2ch = 0;
     A.classes = 0;
-    hipLaunchKernelGGL((k_lw4<R, ND, BLK, false>), dim3(A.nchunks, q0_count), dim3(BLK), dyn, c.stream, A);
+    if (A.sel)
+        hipLaunchKernelGGL((k_lw4<R, ND, BLK, false, true>), dim3(A.nchunks, q0_count), dim3(BLK), dyn, c.stream, A);
+    else
+        hipLaunchKernelGGL((k_lw4<R, ND, BLK, false, false>), dim3(A.nchunks, q0_count), dim3(BLK), dyn, c.stream, A);
     SPR_HIP(hipGetLastError());
     SPR_HIP(hipEventRecord(c.ev1, c.stream));
     if ((rc = launch_reduce_chunks(c, c.partial, q0_count, A.nchunks, d_lw))) return rc;

@@ -171,7 +171,9 @@ template <int MODE, int NB, int NTILE> struct MxA {
     }
 };
 
-template <int MODE, int NB>
+// SEL: a branch selection rule is set (A.sel), a compile-time flag so that the common case
+// carries no test of it
+template <int MODE, int NB, bool SEL>
 __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
     using X = MxShape<MODE, NB>;
     using F = Frag<MODE, NB, 0>;
@@ -548,7 +550,7 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
                         const float n2 = sModeF[2 * ND + s2], iw2 = sModeF[3 * ND + s2];
                         const float W = (x * x + y * y) * (iw1 * iw2);
                         const float e1 = __expf(hneg * f1 * f1), e2 = __expf(hneg * f2 * f2), e3 = __expf(hneg * f3 * f3);
-                        const float sm = A.sel ? (float)A.sel[ND + s1e * ND + s2] : 1.f;
+                        const float sm = SEL ? (float)A.sel[ND + s1e * ND + s2] : 1.f;
                         term = W * sm * ((n1 + n2 + 1.f) * e1 + (n1 - n2) * (e2 - e3));
                     }
                     sV[(a * 64 + lo) * 4 + b] = term;
@@ -581,11 +583,11 @@ __device__ __forceinline__ void mx_unit(const MmArgs &A, const int unit) {
         for (int ww = 0; ww < X::NW; ++ww)
             for (int h2 = 0; h2 < 2; ++h2)
                 for (int q = 0; q < 2; ++q) tot += sRed[nt * NT + ww * 64 + h2 * 32 + 2 * sj + q];
-        A.ic.partial[(size_t)unit * ND + s] = A.sel ? tot * A.sel[s] : tot;
+        A.ic.partial[(size_t)unit * ND + s] = SEL ? tot * A.sel[s] : tot;
     }
 }
 
-template <int MODE, int NB>
+template <int MODE, int NB, bool SEL>
 __global__ __launch_bounds__(512) void k_lw3_mx(MmArgs A) {
     // one unit per workgroup, or (persistent) units from the atomic counter until none is left
     __shared__ int s_unit;
@@ -601,7 +603,7 @@ __global__ __launch_bounds__(512) void k_lw3_mx(MmArgs A) {
             unit = s_unit;
             if (unit >= A.ic.nitems) break;
         }
-        mx_unit<MODE, NB>(A, unit);
+        mx_unit<MODE, NB, SEL>(A, unit);
     }
 }
 
@@ -659,7 +661,7 @@ static bool mx_launch_of_t(Ctx &c, MmArgs &A, int nitems, ItemLaunch &L) {
     A.ft_nmax = c.fc3.ft_nmax;
     const size_t dyn = mx_lds<MODE, NB>(A.npmax, A.ft_nmax, A.nsite, A.L).total;
     if (dyn > 160 * 1024 - 64) return false;
-    const void *fn = (const void *)k_lw3_mx<MODE, NB>;
+    const void *fn = A.sel ? (const void *)k_lw3_mx<MODE, NB, true> : (const void *)k_lw3_mx<MODE, NB, false>;
     if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess) return false;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, MxShape<MODE, NB>::NT, dyn) != hipSuccess || per_cu < 1)

@@ -240,6 +240,37 @@ struct Frag {
         d[1] = a1;
     }
 
+    // FP64: one tile, the same K order as each chain of mma2_e (a wave with an odd tile count
+    // issues no product for a repeated tile)
+    __device__ static typename Sh::ACC mma1_e(const R *sA, int m0, const Z *sE, int n_base) {
+        static_assert(MODE == MM_F64, "FP64 only");
+        const int l = threadIdx.x & 63, g = l >> 4;
+        const int np = n_base + (l & 15);
+        const int s = np >> 1, q = np & 1;
+        const bool ok = np < Sh::K2;
+        const double2 *p0 = reinterpret_cast<const double2 *>(sA + (m0 + (l & 15)) * Sh::PA + g * Sh::S);
+        typename Sh::ACC a0 = {0, 0, 0, 0};
+        constexpr int HQ = Sh::S / 4;
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            double bv[2 * HQ];
+#pragma unroll
+            for (int i = 0; i < HQ; ++i) {
+                const int c = g * (Sh::S / 2) + hh * HQ + i;
+                const Z e = ok ? sE[s * Sh::PE + c] : Z{0, 0};
+                bv[2 * i] = q ? e.y : e.x;
+                bv[2 * i + 1] = q ? e.x : -e.y;
+            }
+#pragma unroll
+            for (int i = 0; i < HQ; ++i) {
+                const double2 x = p0[hh * HQ + i];
+                a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, bv[2 * i], a0, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, bv[2 * i + 1], a0, 0, 0, 0);
+            }
+        }
+        return a0;
+    }
+
     // two tiles (rows m0.., m1..) with the same B fragment; the two chains interleave
     __device__ static void mma2(const R *sA, int m0, int m1, const BF &b, typename Sh::ACC *d) {
         const int l = threadIdx.x & 63, g = l >> 4;

@@ -353,7 +353,10 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                 for (int mt = mtB0; mt < Sh::MT_B; mt += 2 * MSTEPB) {
                     const int mt2 = mt + MSTEPB < Sh::MT_B ? mt + MSTEPB : mt;
                     ACC d[2];
-                    if constexpr (MODE == MM_F64) F::mma2_e(sT, mt * 16, mt2 * 16, sE2, ntB * 16, d);
+                    if (mt2 == mt) {   // one tile left (wave-uniform)
+                        if constexpr (MODE == MM_F64) d[0] = F::mma1_e(sT, mt * 16, sE2, ntB * 16);
+                        else d[0] = F::mma(sT, mt * 16, bf, ACC{0, 0, 0, 0});
+                    } else if constexpr (MODE == MM_F64) F::mma2_e(sT, mt * 16, mt2 * 16, sE2, ntB * 16, d);
                     else F::mma2(sT, mt * 16, mt2 * 16, bf, d);
 #pragma unroll
                     for (int t = 0; t < 2; ++t) {
@@ -381,7 +384,10 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                 for (int mt = mtC0; mt < Sh::MT_C; mt += 2 * MSTEPC) {
                     const int mt2 = mt + MSTEPC < Sh::MT_C ? mt + MSTEPC : mt;
                     ACC d[2];
-                    if constexpr (MODE == MM_F64) F::mma2_e(sAC, mt * 16, mt2 * 16, sE1, ntC * 16, d);
+                    if (mt2 == mt) {
+                        if constexpr (MODE == MM_F64) d[0] = F::mma1_e(sAC, mt * 16, sE1, ntC * 16);
+                        else d[0] = F::mma(sAC, mt * 16, bf, ACC{0, 0, 0, 0});
+                    } else if constexpr (MODE == MM_F64) F::mma2_e(sAC, mt * 16, mt2 * 16, sE1, ntC * 16, d);
                     else F::mma2(sAC, mt * 16, mt2 * 16, bf, d);
 #pragma unroll
                     for (int t = 0; t < 2; ++t) {

@@ -37,7 +37,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--workload", default="cell10",
                    choices=["rocksalt16", "cell10", "learned10", "fourph10", "fourph_tetra", "bte16",
-                            "phonons10", "tetra16", "tetra10"],
+                            "bte10", "phonons10", "tetra16", "tetra10"],
                    help="cell10 (default, the headline line): 10-atom cell, 20^3 mesh with time "
                         "reversal, FP64 vs FP32 vs BF16; rocksalt16: 2-atom cell, 16^3 mesh; learned10: the "
                         "learned importance-sampling proposal on the cell10 workload; fourph10: "
@@ -46,7 +46,9 @@ def parse():
                         "linewidths with the tetrahedron method (the reference's default delta) "
                         "on the 2-atom --mesh mesh; tetra10: the same on the cell10 workload (10-atom "
                         "cell, --mesh10 mesh with time reversal); bte16: iterative "
-                        "BTE of the 2-atom cell on a 16^3 mesh (--mesh); phonons10: harmonic phonons "
+                        "BTE of the 2-atom cell on a 16^3 mesh (--mesh); bte10: the same for the 10-atom "
+                        "cell on a --mesh-bte10 mesh (W on the matrix cores, timed against the "
+                        "generic-kernel store); phonons10: harmonic phonons "
                         "(D(q), eigensolve, group velocities) of the 10-atom cell on its 20^3 mesh")
     p.add_argument("--samples", type=int, default=2_000_000, help="learned10: training samples")
     p.add_argument("--budget", type=float, default=0.05, help="learned10: sampled fraction of q1")
@@ -57,6 +59,7 @@ def parse():
                         "(the '+head' estimators)")
     p.add_argument("--mesh10", type=int, default=20)
     p.add_argument("--mesh4", type=int, default=8, help="fourph_tetra: mesh size")
+    p.add_argument("--mesh-bte10", type=int, default=8, help="bte10: mesh size")
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--mesh", type=int, default=16)
@@ -97,6 +100,8 @@ def main():
         return tetra16(args, cell10=True)
     if args.workload == "bte16":
         return bte16(args)
+    if args.workload == "bte10":
+        return bte16(args, cell10=True)
     if args.workload == "phonons10":
         return phonons10(args)
     from spring_amd import model, mesh, phonons, lib
@@ -713,17 +718,30 @@ def tetra16(args, cell10=False):
     print(json.dumps(out), flush=True)
 
 
-def bte16(args):
-    """Iterative BTE (kappa_iter.x) of the 2-atom model: W of every time-reversal representative
-    stored once on the device, then 1/tau and the self-consistent F updates; the CPU leg times
-    the oracle's F update on a sample of representatives."""
+def bte16(args, cell10=False):
+    """Iterative BTE (kappa_iter.x) of the 2-atom model (or, with cell10, the 10-atom cell): W of
+    every time-reversal representative stored once on the device, then 1/tau and the
+    self-consistent F updates; the CPU leg times the oracle's F update on a sample of
+    representatives.  For the 10-atom cell the W store runs on the matrix cores; the line also
+    times the generic-kernel store of the same W (SPR_KERNEL=generic) in a second context."""
     from spring_amd import model, mesh, phonons, lib
-    m = model.rocksalt2()
-    n = args.mesh
+    m = model.rocksalt10() if cell10 else model.rocksalt2()
+    n = args.mesh_bte10 if cell10 else args.mesh
     qm = mesh.make_mesh(m.crystal, (n, n, n))
     ph = phonons.solve(m.crystal, m.fc2, qm.q_cart, qm.q_int, args.T)
-    ctx = lib.GpuContext(m, qm, ph)
     nd = m.crystal.ndof
+    t_generic = None
+    if cell10:
+        os.environ["SPR_KERNEL"] = "generic"
+        try:
+            cg = lib.GpuContext(m, qm, ph)
+            t = time.perf_counter()
+            cg.bte_irr()
+            t_generic = time.perf_counter() - t
+            cg.close()
+        finally:
+            del os.environ["SPR_KERNEL"]
+    ctx = lib.GpuContext(m, qm, ph)
     t = time.perf_counter()
     irr, deg = ctx.bte_irr()                       # first call: W store
     t_setup = time.perf_counter() - t
@@ -746,7 +764,7 @@ def bte16(args):
         inp = O.Inputs(m, qm, ph)
         oirr, map_irr, map_sign, odeg = O.tr_irreducible(qm)
         threads = max(1, min(16, os.cpu_count() or 1))
-        per = 8
+        per = 1 if cell10 else 8
         rng = np.random.default_rng(1)
         sample = rng.choice(len(oirr), size=min(len(oirr), threads * per), replace=False)
         with ThreadPoolExecutor(threads) as ex:      # W of the sample (not timed)
@@ -778,10 +796,12 @@ def bte16(args):
     res = {
         "metric": "iterative BTE kappa time-to-solution from stored W (collision-matrix build + iterations)",
         "value": t_tau + t_solve, "unit": "s", "higher_is_better": False, "dtype": "f64",
-        "config": {"workload": f"iterative BTE, 2-atom rocksalt, {n}^3 mesh, {len(irr)} time-reversal "
-                               f"representatives x {6 * qm.nq} tetrahedra x {nd}^3 branches",
+        "config": {"workload": f"iterative BTE, {'10-atom cell' if cell10 else '2-atom rocksalt'}, {n}^3 "
+                               f"mesh, {len(irr)} time-reversal representatives x {6 * qm.nq} tetrahedra x "
+                               f"{nd}^3 branches",
                    "T_K": args.T, "eps_k_W_mK": 1e-5},
         "W_store_s": t_setup, "W_store_GB": len(irr) * qm.nq * nd ** 3 * 8 / 1e9,
+        "W_store_generic_s": t_generic,
         "build_s": t_tau, "evaluations_in_build": items, "solve_s": t_solve, "iterations": out["iterations"],
         "F_update_ms": per_it_ms,
         "roofline_F_update": {"bound": "hbm", "achieved": a_bytes / (per_it_ms * 1e-3) / 1e9, "peak": 8000.0,

@@ -431,9 +431,54 @@ int bte_setup(Ctx &c) {
     if ((rc = up(&b.d_W, (const double *)nullptr, (size_t)b.nirr * per_q))) return rc;
     if ((rc = up(&b.d_F, (const double *)nullptr, (size_t)b.nirr * c.nd * 3))) return rc;
     if ((rc = up(&b.d_out, (const double *)nullptr, (size_t)b.nirr * c.nd * 3))) return rc;
-    for (int k = 0; k < b.nirr; ++k)
-        if ((rc = launch_lw3_generic(c, b.d_irr + k, 1, 1.0, SPR_PREC_FP64, nullptr, b.d_W + (size_t)k * per_q)))
-            return rc;
+    // W of every q1 of each representative, each ordering evaluated on its own (as the oracle and
+    // Src_iterative evaluate them): on the matrix cores for cells of three or more atoms (items of
+    // 64 consecutive q1 with their partners q2 = -(q0 + q1)), otherwise, or with
+    // SPR_KERNEL=generic, by the generic kernel one representative at a time
+    const char *force = getenv("SPR_KERNEL");
+    const bool generic = force && std::string(force) == "generic";
+    if (c.nb >= 3 && !generic && lw3_mm_w_built(c.nb)) {
+        std::vector<int2> pairs((size_t)b.nirr * nq);
+        std::vector<int> iq0, ioff, icnt;
+        for (int k = 0; k < b.nirr; ++k) {
+            const int g0 = b.h_irr[k], a0 = g0 % m0, a1 = (g0 / m0) % m1, a2 = g0 / (m0 * m1);
+            for (int q = 0; q < nq; ++q) {
+                const int x = q % m0, y = (q / m0) % m1, z = q / (m0 * m1);
+                const int p = (2 * m0 - a0 - x) % m0 + m0 * ((2 * m1 - a1 - y) % m1 + m1 * ((2 * m2 - a2 - z) % m2));
+                pairs[(size_t)k * nq + q] = make_int2(q, p);
+            }
+            for (int o = 0; o < nq; o += 64) {
+                iq0.push_back(k);
+                ioff.push_back(k * nq + o);
+                icnt.push_back(std::min(64, nq - o));
+            }
+        }
+        const int nit = (int)iq0.size();
+        int2 *d_pairs = nullptr;
+        int *d_iq0 = nullptr, *d_ioff = nullptr, *d_icnt = nullptr;
+        double *d_part = nullptr;
+        auto done = [&](int r) {
+            c.override_items = nullptr;
+            for (void *q : {(void *)d_pairs, (void *)d_iq0, (void *)d_ioff, (void *)d_icnt, (void *)d_part})
+                if (q) (void)hipFree(q);
+            return r;
+        };
+        if ((rc = up(&d_pairs, pairs.data(), pairs.size())) || (rc = up(&d_iq0, iq0.data(), iq0.size())) ||
+            (rc = up(&d_ioff, ioff.data(), ioff.size())) || (rc = up(&d_icnt, icnt.data(), icnt.size())) ||
+            (rc = up(&d_part, (const double *)nullptr, (size_t)nit * kMaxUnitsPerItem * c.nd)))
+            return done(rc);
+        ItemSet set{d_iq0, d_ioff, d_icnt, d_pairs, nit, d_part};
+        set.dense = true;
+        c.override_items = &set;
+        rc = launch_lw3_mm_w_items(c, b.d_irr, b.nirr, b.d_W);
+        if (rc == SPR_OK && hipStreamSynchronize(c.stream) != hipSuccess) rc = SPR_E_HIP;
+        if (rc) return done(rc);
+        done(SPR_OK);
+    } else {
+        for (int k = 0; k < b.nirr; ++k)
+            if ((rc = launch_lw3_generic(c, b.d_irr + k, 1, 1.0, SPR_PREC_FP64, nullptr, b.d_W + (size_t)k * per_q)))
+                return rc;
+    }
     SPR_HIP(hipStreamSynchronize(c.stream));
     b.ready = true;
     return SPR_OK;

@@ -38,6 +38,9 @@ struct MmArgs {
     // q1, both orderings of each reduced pair written
     double *w_out;
     int w_q0_first, w_nq;
+    // W output: only the first ordering W(q1; s0, s1, s2) of each pair (items that list every q1
+    // of their q0, so that each ordering is evaluated on its own)
+    int w_one;
     // s2-per-wave kernel: phase-sum schedule ((Nb + 1) atom offsets, then the entries) and the
     // largest entry count of one atom
     const int *ft_sched;

@@ -505,7 +505,7 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                     if (h * SB + sl < ND)
                         wq[(((size_t)q1 * ND + c0 + s0l) * ND + s1) * ND + h * SB + sl] = sW[i];
                 }
-                if (q1 != q2)
+                if (q1 != q2 && !A.w_one)
                     for (int i = tid; i < n0; i += 512) {   // W(q2; s0, s2, s1): runs of Ndof in s1
                         const int s0l = i / PER_S0, r = i % PER_S0, sl = r / ND, s1 = r % ND;
                         if (h * SB + sl < ND)
@@ -757,7 +757,7 @@ int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, do
     const bool f64 = prec == SPR_PREC_FP64;
     MmArgs A;
     A.w_out = nullptr;
-    A.w_q0_first = A.w_nq = 0;
+    A.w_q0_first = A.w_nq = A.w_one = 0;
     A.ft_sched = nullptr;
     A.ft_nmax = 0;
     A.item_q0 = ov ? ov->item_q0 : c.d_item_q0 + it0;
@@ -811,6 +811,48 @@ int launch_lw3_mm(Ctx &c, int q0_first, int q0_count, double sigma, int prec, do
 
 bool lw3_mm_w_built(int nb) { return nb == 2 || nb == 3 || nb == 4 || nb == 5 || nb == 6 || nb == 8 || nb == 10; }
 
+// The W-output launch for the items in A (item lists, q0 grid table): FP64, no delta stage.
+static int launch_mm_w_args(Ctx &c, MmArgs &A, int nitems, int q_first, int q_count, double *d_w) {
+    int rc;
+    A.L = c.L;
+    A.m0 = c.mesh[0];
+    A.m1 = c.mesh[1];
+    A.blk_start = c.fc3.blk_start;
+    A.tup_site = c.fc3.tup_site;
+    A.fc = (const void *)c.fc3.fc64;
+    A.site = c.fc3.site;
+    A.nsite = c.fc3.nsite;
+    A.phase = (const void *)c.phase64;
+    A.evec = (const void *)c.evec;
+    A.omega = c.omega;
+    A.inv_omega = c.inv_omega;
+    A.nbe = c.nbe;
+    A.sel = nullptr;   // W output: the tetrahedron and BTE sums apply the branch ranges themselves
+    A.sigma = 1.0;     // unused without the delta stage
+    const double pi = 4.0 * std::atan(1.0);
+    A.factor = 1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) * 1000000.0 * pi *
+               0.0625 / (double)c.nq;   // Sctr3THz2 / Nq (constants.f90:59-60)
+    int npmax = 0;
+    const int nb2 = c.nb * c.nb;
+    for (int mu = 0; mu < c.nb; ++mu)
+        npmax = std::max(npmax, c.fc3.h_blk_start[(mu + 1) * nb2] - c.fc3.h_blk_start[mu * nb2]);
+    A.npmax = std::max(npmax, 1);
+    A.nterm = std::max(c.fc3.h_blk_start[c.nb * nb2], 1);
+    A.ft_sched = nullptr;
+    A.ft_nmax = 0;
+    A.w_out = d_w;
+    A.w_nq = c.nq;
+    // the per-unit sums of this launch are zero; they are reduced into a scratch row per q0
+    if ((rc = ensure((void **)&c.wsink, &c.wsink_cap, (size_t)q_count * c.nd * sizeof(double)))) return rc;
+    switch (c.nb) {
+#define SPR_NB(n) \
+    case n: return launch_mm_t<MM_F64, n, true>(c, A, nitems, q_first, q_count, 0, c.wsink);
+        SPR_NB(2) SPR_NB(3) SPR_NB(4) SPR_NB(5) SPR_NB(6) SPR_NB(8) SPR_NB(10)
+#undef SPR_NB
+        default: return SPR_E_UNSUPPORTED;
+    }
+}
+
 // W of every grid q1 for irreducible q0 [q_first, q_first + q_count) into
 // d_w[q0 - q_first][q1][s0][s1][s2] (FP64), from the pair-reduced items of the context: the
 // W-output mode of the matrix-core kernel, both orderings written, no underflow skip.
@@ -830,45 +872,32 @@ int launch_lw3_mm_w(Ctx &c, int q_first, int q_count, double *d_w) {
     A.pairs = c.d_pairs;
     A.pair_w = nullptr;
     A.q0_grid = c.irr_grid;
-    A.L = c.L;
-    A.m0 = c.mesh[0];
-    A.m1 = c.mesh[1];
-    A.blk_start = c.fc3.blk_start;
-    A.tup_site = c.fc3.tup_site;
-    A.fc = (const void *)c.fc3.fc64;
-    A.site = c.fc3.site;
-    A.nsite = c.fc3.nsite;
-    A.phase = (const void *)c.phase64;
-    A.evec = (const void *)c.evec;
-    A.omega = c.omega;
-    A.inv_omega = c.inv_omega;
-    A.nbe = c.nbe;
-    A.sel = nullptr;   // W output: the tetrahedron sums apply the branch ranges themselves
-    A.sigma = 1.0;     // unused without the delta stage
-    const double pi = 4.0 * std::atan(1.0);
-    A.factor = 1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) * 1000000.0 * pi *
-               0.0625 / (double)c.nq;   // Sctr3THz2 / Nq (constants.f90:59-60)
-    int npmax = 0;
-    const int nb2 = c.nb * c.nb;
-    for (int mu = 0; mu < c.nb; ++mu)
-        npmax = std::max(npmax, c.fc3.h_blk_start[(mu + 1) * nb2] - c.fc3.h_blk_start[mu * nb2]);
-    A.npmax = std::max(npmax, 1);
-    A.nterm = std::max(c.fc3.h_blk_start[c.nb * nb2], 1);
     A.ic.partial = c.ipartial;
-    A.ft_sched = nullptr;
-    A.ft_nmax = 0;
-    A.w_out = d_w;
     A.w_q0_first = q_first;
-    A.w_nq = c.nq;
-    // the per-unit sums of this launch are zero; they are reduced into a scratch row per q0
-    if ((rc = ensure((void **)&c.wsink, &c.wsink_cap, (size_t)q_count * c.nd * sizeof(double)))) return rc;
-    switch (c.nb) {
-#define SPR_NB(n) \
-    case n: return launch_mm_t<MM_F64, n, true>(c, A, nitems, q_first, q_count, it0, c.wsink);
-        SPR_NB(2) SPR_NB(3) SPR_NB(4) SPR_NB(5) SPR_NB(6) SPR_NB(8) SPR_NB(10)
-#undef SPR_NB
-        default: return SPR_E_UNSUPPORTED;
+    A.w_one = 0;
+    return launch_mm_w_args(c, A, nitems, q_first, q_count, d_w);
+}
+
+// W of every grid q1 for the q0 of caller-supplied items (c.override_items, whole items of up
+// to 64 pairs, every q1 of a q0 listed with its partner): d_w[k][q1][s0][s1][s2] for the q0 of
+// grid index q0_grid[k], k < nq0 (item_q0 holds k); each ordering evaluated on its own.
+int launch_lw3_mm_w_items(Ctx &c, const int *d_q0_grid, int nq0, double *d_w) {
+    const ItemSet *ov = c.override_items;
+    if (!ov || !ov->dense || !lw3_mm_w_built(c.nb)) {
+        set_error("matrix-core W path for listed items: Nb in 2-6, 8, 10 and whole items");
+        return SPR_E_UNSUPPORTED;
     }
+    MmArgs A;
+    A.item_q0 = ov->item_q0;
+    A.item_off = ov->item_off;
+    A.item_cnt = ov->item_cnt;
+    A.pairs = ov->pairs;
+    A.pair_w = nullptr;
+    A.q0_grid = d_q0_grid;
+    A.ic.partial = ov->partial;
+    A.w_q0_first = 0;
+    A.w_one = 1;
+    return launch_mm_w_args(c, A, ov->nitems, 0, nq0, d_w);
 }
 
 }  // namespace spr

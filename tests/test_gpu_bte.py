@@ -87,3 +87,42 @@ def test_bte_fallback_path(rocksalt, monkeypatch):
     if out["iterations"] == out_m["iterations"]:
         assert rel_err(out["kappa"], out_m["kappa"], 1e-6) < 1e-9
     ctx.close()
+
+
+def test_bte_matrix_core_w_store_4atom(monkeypatch):
+    """4-atom cell (Ndof 12), 3^3 mesh: the W store of cells with three or more atoms comes from
+    the matrix-core kernel's W mode over every q1 of each representative (each ordering evaluated
+    on its own).  1/tau and dF against the oracle as for the 2-atom cell, the same against the
+    generic-kernel store (SPR_KERNEL=generic) to rounding, and the whole loop against the
+    oracle's."""
+    from spring_amd import model
+    m = model.rocksalt_chain(2)
+    ctx, qm, ph, (irr, map_irr, map_sign, deg, Wp, Wm) = _setup(m, (3, 3, 3))
+    g_irr, g_deg = ctx.bte_irr()
+    assert np.array_equal(g_irr, irr) and np.array_equal(g_deg, deg)
+    itau = ctx.bte_tau_inv()
+    ref = O.bte_sums(qm, ph, irr, map_irr, map_sign, Wp, Wm)
+    e = rel_err(itau, ref, 1e-9)
+    print(f"4-atom 1/tau max rel err {e:.2e}", flush=True)
+    assert e < 1e-10
+    F = np.random.default_rng(5).normal(size=(len(irr), ph.omega.shape[1], 3))
+    dF = ctx.bte_delta_f(F)
+    refF = O.bte_sums(qm, ph, irr, map_irr, map_sign, Wp, Wm, F)
+    en = np.max(np.abs(dF - refF)) / np.max(np.abs(refF))
+    e = rel_err(dF, refF, 1e-4)
+    print(f"4-atom dF normwise rel err {en:.2e}, elementwise (above 1e-4 of max) {e:.2e}", flush=True)
+    assert en < 1e-12 and e < 1e-10
+    T, vol = 300.0, m.crystal.vol
+    out = ctx.bte_solve(T, eps_k=1e-6, max_iter=100)
+    k, k0, it, Fs = O.bte_solve(qm, ph, vol, T, irr, map_irr, map_sign, deg, Wp, Wm, 1e-6, 100)
+    _check_solve(out, k, k0, it, Fs)
+    from spring_amd.lib import GpuContext
+    monkeypatch.setenv("SPR_KERNEL", "generic")
+    cg = GpuContext(m, qm, ph)
+    itau_g = cg.bte_tau_inv()
+    dF_g = cg.bte_delta_f(F)
+    monkeypatch.delenv("SPR_KERNEL")
+    assert rel_err(itau, itau_g, 1e-9) < 1e-12
+    assert np.max(np.abs(dF - dF_g)) / np.max(np.abs(dF_g)) < 1e-12
+    cg.close()
+    ctx.close()

@@ -162,8 +162,9 @@ int spr_lw3_offmesh(spr_ctx *ctx, const double *q0_cart, const double *evec0, co
  * (make_tetrahedron, Src_iterative/Irr_q_point_m.f90:206-330): representative = the lower grid
  * index of q and -q, F(-q) = -F(q).  The first call stores W of every representative q and
  * every grid q1 in device memory (QconsrvMomentum, Quasi_momentum_consrv.f90:118-451;
- * nirr x Nq x Ndof^3 doubles; each ordering of (q1, q2) evaluated on its own, on the matrix
- * cores for cells of 3 to 10 atoms) and needs the context's tetrahedra.  F arrays are
+ * nirr x Nq x Ndof^3 doubles; each ordering of (q1, q2) evaluated on its own, by the W modes
+ * of the small-cell (1-2 atoms) and matrix-core (3-10 atoms) kernels) and needs the context's
+ * tetrahedra.  F arrays are
  * F[qi][s][3] (F_vecAll(3, s, qi)), 1/tau arrays [qi][s] (Inv_tauqs(s, qi)). */
 
 /* nirr, and (if not NULL) the grid index of each irreducible point and the number of grid

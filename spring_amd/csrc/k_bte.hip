@@ -245,69 +245,70 @@ __global__ __launch_bounds__(kBteBlock) void k_bte_build(BteArgs A, int R, doubl
     for (int i = threadIdx.x; i < R; i += kBteBlock) s_row[i] = 0.0;
     __syncthreads();
     double tau = 0.0;
-    for (int i = threadIdx.x; i < A.ntet * nd2; i += kBteBlock) {
-        // tetrahedron fastest across lanes: a wave holds neighbouring tetrahedra of one branch
-        // pair, whose energy ranges are alike, so whole waves skip together below
-        const int p = i / A.ntet, t = i - p * A.ntet;
-        const int s1 = p / nd, s2 = p % nd;
-        double Mp[4], Mm[4], Wp[4], Wm[4];
-        int c1[4], cp[4], cm[4];
-        double g1[4], gp[4], gm[4];
-        int q1v[4];
-        int4 k1v[4];
+    // each thread owns tetrahedra t = tid + kBteBlock k and walks every branch pair of each, so
+    // the vertex indices are formed once per tetrahedron; at every step the lanes of a wave hold
+    // neighbouring tetrahedra of one branch pair, whose energy ranges are alike, so whole waves
+    // skip together below
+    for (int t = threadIdx.x; t < A.ntet; t += kBteBlock) {
+        int q1v[4], qpv[4], qmv[4], nq1v[4], qq2v[4], mq2v[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int q1 = A.tetra[4 * t + v];
             const int4 k1 = A.coord[q1];
             q1v[v] = q1;
-            k1v[v] = k1;
-            const int qp = grid_index(kq.x + k1.x, kq.y + k1.y, kq.z + k1.z, A);
-            const int qm = grid_index(kq.x - k1.x, kq.y - k1.y, kq.z - k1.z, A);
-            const double w1 = A.omega[(size_t)q1 * nd + s1];
-            Mp[v] = A.omega[(size_t)qp * nd + s2] - w1;
-            Mm[v] = w1 + A.omega[(size_t)qm * nd + s2];
+            qpv[v] = grid_index(kq.x + k1.x, kq.y + k1.y, kq.z + k1.z, A);
+            qmv[v] = grid_index(kq.x - k1.x, kq.y - k1.y, kq.z - k1.z, A);
+            qq2v[v] = grid_index(-kq.x - k1.x, -kq.y - k1.y, -kq.z - k1.z, A);
+            nq1v[v] = grid_index(-k1.x, -k1.y, -k1.z, A);
+            mq2v[v] = grid_index(k1.x - kq.x, k1.y - kq.y, k1.z - kq.z, A);
         }
-        // both processes need Mq strictly inside their vertex-energy range (tetra_gI's cases);
-        // outside it they add nothing, so W and the occupations are not read
-        const double lo_p = fmin(fmin(Mp[0], Mp[1]), fmin(Mp[2], Mp[3]));
-        const double hi_p = fmax(fmax(Mp[0], Mp[1]), fmax(Mp[2], Mp[3]));
-        const double lo_m = fmin(fmin(Mm[0], Mm[1]), fmin(Mm[2], Mm[3]));
-        const double hi_m = fmax(fmax(Mm[0], Mm[1]), fmax(Mm[2], Mm[3]));
-        if (!((mq > lo_p && mq < hi_p) || (mq > lo_m && mq < hi_m))) continue;
+        for (int s1 = 0; s1 < nd; ++s1) {
+            double w1[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int q1 = q1v[v];
-            const int4 k1 = k1v[v];
-            const int qp = grid_index(kq.x + k1.x, kq.y + k1.y, kq.z + k1.z, A);
-            const int qq2 = grid_index(-kq.x - k1.x, -kq.y - k1.y, -kq.z - k1.z, A);
-            const int nq1 = grid_index(-k1.x, -k1.y, -k1.z, A);
-            const int qm = grid_index(kq.x - k1.x, kq.y - k1.y, kq.z - k1.z, A);
-            const int mq2 = grid_index(k1.x - kq.x, k1.y - kq.y, k1.z - kq.z, A);
-            Wp[v] = Wq[((size_t)q1 * nd + s) * nd2 + p] * A.scale *
-                    (A.nbe[(size_t)q1 * nd + s1] - A.nbe[(size_t)qq2 * nd + s2]);
-            Wm[v] = Wq[((size_t)nq1 * nd + s) * nd2 + p] * A.scale *
-                    (1.0 + A.nbe[(size_t)nq1 * nd + s1] + A.nbe[(size_t)mq2 * nd + s2]);
-            const int2 m1 = A.map[q1], mp = A.map[qp], mm = A.map[qm];
-            c1[v] = m1.x * nd + s1;
-            g1[v] = m1.y;
-            cp[v] = mp.x * nd + s2;
-            gp[v] = mp.y;
-            cm[v] = mm.x * nd + s2;
-            gm[v] = mm.y;
-        }
-        double kp[4], km[4];
-        tetra_coef(mq, Mp, Wp, 1.0, kp);
-        tetra_coef(mq, Mm, Wm, 0.5, km);
+            for (int v = 0; v < 4; ++v) w1[v] = A.omega[(size_t)q1v[v] * nd + s1];
+            for (int s2 = 0; s2 < nd; ++s2) {
+                const int p = s1 * nd + s2;
+                double Mp[4], Mm[4], Wp[4], Wm[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            tau += kp[v] + km[v];
-            if (kp[v] != 0.0) {
-                atomicAdd(&s_row[cp[v]], gp[v] * kp[v]);
-                atomicAdd(&s_row[c1[v]], -g1[v] * kp[v]);
-            }
-            if (km[v] != 0.0) {
-                atomicAdd(&s_row[cm[v]], gm[v] * km[v]);
-                atomicAdd(&s_row[c1[v]], g1[v] * km[v]);
+                for (int v = 0; v < 4; ++v) {
+                    Mp[v] = A.omega[(size_t)qpv[v] * nd + s2] - w1[v];
+                    Mm[v] = w1[v] + A.omega[(size_t)qmv[v] * nd + s2];
+                }
+                // both processes need Mq strictly inside their vertex-energy range (tetra_gI's
+                // cases); outside it they add nothing, so W and the occupations are not read
+                const double lo_p = fmin(fmin(Mp[0], Mp[1]), fmin(Mp[2], Mp[3]));
+                const double hi_p = fmax(fmax(Mp[0], Mp[1]), fmax(Mp[2], Mp[3]));
+                const double lo_m = fmin(fmin(Mm[0], Mm[1]), fmin(Mm[2], Mm[3]));
+                const double hi_m = fmax(fmax(Mm[0], Mm[1]), fmax(Mm[2], Mm[3]));
+                if (!((mq > lo_p && mq < hi_p) || (mq > lo_m && mq < hi_m))) continue;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    Wp[v] = Wq[((size_t)q1v[v] * nd + s) * nd2 + p] * A.scale *
+                            (A.nbe[(size_t)q1v[v] * nd + s1] - A.nbe[(size_t)qq2v[v] * nd + s2]);
+                    Wm[v] = Wq[((size_t)nq1v[v] * nd + s) * nd2 + p] * A.scale *
+                            (1.0 + A.nbe[(size_t)nq1v[v] * nd + s1] + A.nbe[(size_t)mq2v[v] * nd + s2]);
+                }
+                double kp[4], km[4];
+                tetra_coef(mq, Mp, Wp, 1.0, kp);
+                tetra_coef(mq, Mm, Wm, 0.5, km);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    tau += kp[v] + km[v];
+                    if (kp[v] != 0.0 || km[v] != 0.0) {
+                        const int2 m1 = A.map[q1v[v]];
+                        const int c1 = m1.x * nd + s1;
+                        if (kp[v] != 0.0) {
+                            const int2 mp = A.map[qpv[v]];
+                            atomicAdd(&s_row[mp.x * nd + s2], mp.y * kp[v]);
+                            atomicAdd(&s_row[c1], -m1.y * kp[v]);
+                        }
+                        if (km[v] != 0.0) {
+                            const int2 mm = A.map[qmv[v]];
+                            atomicAdd(&s_row[mm.x * nd + s2], mm.y * km[v]);
+                            atomicAdd(&s_row[c1], m1.y * km[v]);
+                        }
+                    }
+                }
             }
         }
     }
@@ -432,12 +433,13 @@ int bte_setup(Ctx &c) {
     if ((rc = up(&b.d_F, (const double *)nullptr, (size_t)b.nirr * c.nd * 3))) return rc;
     if ((rc = up(&b.d_out, (const double *)nullptr, (size_t)b.nirr * c.nd * 3))) return rc;
     // W of every q1 of each representative, each ordering evaluated on its own (as the oracle and
-    // Src_iterative evaluate them): on the matrix cores for cells of three or more atoms (items of
-    // 64 consecutive q1 with their partners q2 = -(q0 + q1)), otherwise, or with
-    // SPR_KERNEL=generic, by the generic kernel one representative at a time
+    // Src_iterative evaluate them), from items of 64 consecutive q1 with their partners
+    // q2 = -(q0 + q1): the W mode of the small-cell kernel (1 and 2 atoms) or of the matrix-core
+    // kernel (3 to 10 atoms); otherwise, or with SPR_KERNEL=generic, the generic kernel one
+    // representative at a time
     const char *force = getenv("SPR_KERNEL");
     const bool generic = force && std::string(force) == "generic";
-    if (c.nb >= 3 && !generic && lw3_mm_w_built(c.nb)) {
+    if (!generic && (c.nb <= 2 || lw3_mm_w_built(c.nb))) {
         std::vector<int2> pairs((size_t)b.nirr * nq);
         std::vector<int> iq0, ioff, icnt;
         for (int k = 0; k < b.nirr; ++k) {
@@ -470,7 +472,8 @@ int bte_setup(Ctx &c) {
         ItemSet set{d_iq0, d_ioff, d_icnt, d_pairs, nit, d_part};
         set.dense = true;
         c.override_items = &set;
-        rc = launch_lw3_mm_w_items(c, b.d_irr, b.nirr, b.d_W);
+        rc = c.nb <= 2 ? launch_lw3_small_w_items(c, b.d_irr, b.nirr, b.d_W)
+                       : launch_lw3_mm_w_items(c, b.d_irr, b.nirr, b.d_W);
         if (rc == SPR_OK && hipStreamSynchronize(c.stream) != hipSuccess) rc = SPR_E_HIP;
         if (rc) return done(rc);
         done(SPR_OK);

@@ -50,6 +50,9 @@ struct SmallArgs {
     // both orientations of each pair written
     double *w_out;
     int w_q0_first, w_nq;
+    // W output: only the first orientation W(q1; s0, s1, s2) of each pair (items that list every
+    // q1 of their q0, so that each orientation is evaluated on its own)
+    int w_one;
     // branch selection (NULL: every branch): (nd) s0 mask, then the (nd, nd) pair mask [s1][s2]
     const double *sel;
     // per-pair weights of caller-supplied items (NULL: 1 for q1 == q2, else 2)
@@ -224,7 +227,7 @@ __device__ __forceinline__ void small_item(const SmallArgs &A, const int item, c
                     const int s0 = s0b + k;
                     double *wq = A.w_out + (size_t)(qi - A.w_q0_first) * A.w_nq * ND * ND * ND;
                     wq[(((size_t)q1 * ND + s0) * ND + s1) * ND + s2] = W;
-                    if (q1 != q2) wq[(((size_t)q2 * ND + s0) * ND + s2) * ND + s1] = W;
+                    if (q1 != q2 && !A.w_one) wq[(((size_t)q2 * ND + s0) * ND + s2) * ND + s1] = W;
                 }
                 continue;
             }
@@ -443,7 +446,7 @@ int launch_lw3_small(Ctx &c, int q0_first, int q0_count, double sigma, int prec,
                1000000.0 * pi * 0.0625 / (double)c.nq;   // Sctr3THz2 / Nq, constants.f90:59-60
     A.ic.partial = ov ? ov->partial : c.ipartial;
     A.w_out = nullptr;
-    A.w_q0_first = A.w_nq = 0;
+    A.w_q0_first = A.w_nq = A.w_one = 0;
     A.sel = c.sel3 ? c.d_sel3 : nullptr;
     A.pair_w = ov ? ov->pair_w : nullptr;
     const bool f64 = prec == SPR_PREC_FP64;
@@ -496,10 +499,53 @@ int launch_lw3_small_w(Ctx &c, int q_first, int q_count, double *d_w) {
     A.pair_w = nullptr;
     A.w_q0_first = q_first;
     A.w_nq = c.nq;
+    A.w_one = 0;
     // the per-item sums of this launch are zero; they are reduced into a scratch row per q0
     if ((rc = ensure((void **)&c.wsink, &c.wsink_cap, (size_t)q_count * c.nd * sizeof(double)))) return rc;
     if (c.nb == 2) return launch_small_t<double, 2>(c, A, nitems, q_first, q_count, it0, c.wsink, true);
     return launch_small_t<double, 1>(c, A, nitems, q_first, q_count, it0, c.wsink, true);
+}
+
+// FP64 W of every grid q1 for the q0 of caller-supplied items (c.override_items, every q1 of a
+// q0 listed with its partner), Nb <= 2: d_w[k][q1][s0][s1][s2] for the q0 of grid index
+// q0_grid[k], k < nq0 (item_q0 holds k); each orientation evaluated on its own.
+int launch_lw3_small_w_items(Ctx &c, const int *d_q0_grid, int nq0, double *d_w) {
+    const ItemSet *ov = c.override_items;
+    if (c.nb > 2 || !ov) {
+        set_error("small-cell W path for listed items: Nb <= 2 and an item list");
+        return SPR_E_UNSUPPORTED;
+    }
+    int rc;
+    SmallArgs A;
+    A.item_q0 = ov->item_q0;
+    A.item_off = ov->item_off;
+    A.item_cnt = ov->item_cnt;
+    A.pairs = ov->pairs;
+    A.q0_grid = d_q0_grid;
+    A.L = c.L;
+    A.m0 = c.mesh[0];
+    A.m1 = c.mesh[1];
+    A.blk_start = c.fc3.blk_start_p;
+    A.tup_site = c.fc3.tup_site_p;
+    A.site = c.fc3.site;
+    A.nsite = c.fc3.nsite;
+    A.omega = c.omega;
+    A.inv_omega = c.inv_omega;
+    A.nbe = c.nbe;
+    A.sigma = 1.0;   // unused without the delta stage
+    const double pi = 4.0 * std::atan(1.0);
+    A.factor = 1.054571817 * (1.60217733 * 1.60217733) * (6.02214076 * 6.02214076 * 6.02214076) *
+               1000000.0 * pi * 0.0625 / (double)c.nq;   // Sctr3THz2 / Nq, constants.f90:59-60
+    A.ic.partial = ov->partial;
+    A.w_out = d_w;
+    A.sel = nullptr;
+    A.pair_w = nullptr;
+    A.w_q0_first = 0;
+    A.w_nq = c.nq;
+    A.w_one = 1;
+    if ((rc = ensure((void **)&c.wsink, &c.wsink_cap, (size_t)nq0 * c.nd * sizeof(double)))) return rc;
+    if (c.nb == 2) return launch_small_t<double, 2>(c, A, ov->nitems, 0, nq0, 0, c.wsink, true);
+    return launch_small_t<double, 1>(c, A, ov->nitems, 0, nq0, 0, c.wsink, true);
 }
 
 }  // namespace spr

@@ -220,6 +220,7 @@ int launch_lw3_tetra(Ctx &c, int q_first, int q_count, double *d_lw, double *d_p
 int launch_lw3_small_w(Ctx &c, int q_first, int q_count, double *d_w);
 int launch_lw3_mm_w(Ctx &c, int q_first, int q_count, double *d_w);
 int launch_lw3_mm_w_items(Ctx &c, const int *d_q0_grid, int nq0, double *d_w);
+int launch_lw3_small_w_items(Ctx &c, const int *d_q0_grid, int nq0, double *d_w);
 bool lw3_mm_w_built(int nb);
 // device tetrahedra (6 nq, 4) of the context (uploaded on first use)
 int ensure_tetra(Ctx &c);

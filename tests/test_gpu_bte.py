@@ -40,6 +40,18 @@ def test_bte_tau_inv_and_delta_f(rocksalt, dims):
     e = rel_err(dF, refF, 1e-4)
     print(f"{dims} dF normwise rel err {en:.2e}, elementwise (above 1e-4 of max) {e:.2e}", flush=True)
     assert en < 1e-12 and e < 1e-10
+    # the store of W (the small-cell kernel's W mode over every q1) against the generic kernel's
+    from spring_amd.lib import GpuContext
+    import os
+    os.environ["SPR_KERNEL"] = "generic"
+    try:
+        cg = GpuContext(rocksalt, qm, ph)
+        assert rel_err(itau, cg.bte_tau_inv(), 1e-9) < 1e-12
+        dF_g = cg.bte_delta_f(F)
+        assert np.max(np.abs(dF - dF_g)) / np.max(np.abs(dF_g)) < 1e-12
+        cg.close()
+    finally:
+        del os.environ["SPR_KERNEL"]
 
 
 def test_bte_solve(rocksalt):

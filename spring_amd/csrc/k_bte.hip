@@ -143,64 +143,71 @@ __global__ __launch_bounds__(kBteBlock) void k_bte(BteArgs A) {
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    const int nt = t_hi - t_lo;
-    for (int i = threadIdx.x; i < nt * nd2; i += kBteBlock) {
-        // tetrahedron fastest across lanes (neighbouring tetrahedra have alike energy ranges)
-        const int p = i / nt, t = t_lo + (i - p * nt);
-        const int s1 = p / nd, s2 = p % nd;
-        double Mp[4], Mm[4], Xp[4][K], Xm[4][K];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int4 k1 = A.coord[A.tetra[4 * t + v]];
-            const int q1 = A.tetra[4 * t + v];
-            const int qp = grid_index(kq.x + k1.x, kq.y + k1.y, kq.z + k1.z, A);
-            const int qm = grid_index(kq.x - k1.x, kq.y - k1.y, kq.z - k1.z, A);
-            const double w1 = A.omega[(size_t)q1 * nd + s1];
-            Mp[v] = A.omega[(size_t)qp * nd + s2] - w1;
-            Mm[v] = w1 + A.omega[(size_t)qm * nd + s2];
-        }
-        // both processes need mq strictly inside their vertex-energy range (tetra_gI's cases);
-        // outside it they add nothing, so W, n and F are not read
-        {
-            const double lo_p = fmin(fmin(Mp[0], Mp[1]), fmin(Mp[2], Mp[3]));
-            const double hi_p = fmax(fmax(Mp[0], Mp[1]), fmax(Mp[2], Mp[3]));
-            const double lo_m = fmin(fmin(Mm[0], Mm[1]), fmin(Mm[2], Mm[3]));
-            const double hi_m = fmax(fmax(Mm[0], Mm[1]), fmax(Mm[2], Mm[3]));
-            if (!((mq > lo_p && mq < hi_p) || (mq > lo_m && mq < hi_m))) continue;
-        }
+    // each thread owns tetrahedra t_lo + tid + kBteBlock k of the chunk and walks every branch
+    // pair of each, the vertex indices formed once per tetrahedron; at every step the lanes of a
+    // wave hold neighbouring tetrahedra (alike energy ranges) of one branch pair
+    for (int t = t_lo + (int)threadIdx.x; t < t_hi; t += kBteBlock) {
+        int q1v[4], qpv[4], qmv[4], nq1v[4], qq2v[4], mq2v[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int q1 = A.tetra[4 * t + v];
             const int4 k1 = A.coord[q1];
-            const int qp = grid_index(kq.x + k1.x, kq.y + k1.y, kq.z + k1.z, A);        // q + q1
-            const int qq2 = grid_index(-kq.x - k1.x, -kq.y - k1.y, -kq.z - k1.z, A);   // -q - q1
-            const int nq1 = grid_index(-k1.x, -k1.y, -k1.z, A);                        // -q1
-            const int qm = grid_index(kq.x - k1.x, kq.y - k1.y, kq.z - k1.z, A);        // q - q1
-            const int mq2 = grid_index(k1.x - kq.x, k1.y - kq.y, k1.z - kq.z, A);       // q1 - q
-            // + process, pair (q1, -q-q1)
-            const double wp = Wq[((size_t)q1 * nd + s) * nd2 + p] * A.scale *
-                              (A.nbe[(size_t)q1 * nd + s1] - A.nbe[(size_t)qq2 * nd + s2]);
-            // - process, pair (-q1, q1 - q)
-            const double wm = Wq[((size_t)nq1 * nd + s) * nd2 + p] * A.scale *
-                              (1.0 + A.nbe[(size_t)nq1 * nd + s1] + A.nbe[(size_t)mq2 * nd + s2]);
-            if (MODE == 0) {
-                Xp[v][0] = wp;
-                Xm[v][0] = wm;
-            } else {
-                const int2 m1 = A.map[q1], mp = A.map[qp], mm = A.map[qm];
-                const double *F1 = A.F + ((size_t)m1.x * nd + s1) * 3;
-                const double *Fp = A.F + ((size_t)mp.x * nd + s2) * 3;
-                const double *Fm = A.F + ((size_t)mm.x * nd + s2) * 3;
+            q1v[v] = q1;
+            qpv[v] = grid_index(kq.x + k1.x, kq.y + k1.y, kq.z + k1.z, A);        // q + q1
+            qmv[v] = grid_index(kq.x - k1.x, kq.y - k1.y, kq.z - k1.z, A);        // q - q1
+            qq2v[v] = grid_index(-kq.x - k1.x, -kq.y - k1.y, -kq.z - k1.z, A);   // -q - q1
+            nq1v[v] = grid_index(-k1.x, -k1.y, -k1.z, A);                        // -q1
+            mq2v[v] = grid_index(k1.x - kq.x, k1.y - kq.y, k1.z - kq.z, A);       // q1 - q
+        }
+        for (int s1 = 0; s1 < nd; ++s1) {
+            double w1[4];
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const double f1 = m1.y * F1[k];
-                    Xp[v][k] = wp * (mp.y * Fp[k] - f1);
-                    Xm[v][k] = wm * (mm.y * Fm[k] + f1);
+            for (int v = 0; v < 4; ++v) w1[v] = A.omega[(size_t)q1v[v] * nd + s1];
+            for (int s2 = 0; s2 < nd; ++s2) {
+                const int p = s1 * nd + s2;
+                double Mp[4], Mm[4], Xp[4][K], Xm[4][K];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    Mp[v] = A.omega[(size_t)qpv[v] * nd + s2] - w1[v];
+                    Mm[v] = w1[v] + A.omega[(size_t)qmv[v] * nd + s2];
                 }
+                // both processes need mq strictly inside their vertex-energy range (tetra_gI's
+                // cases); outside it they add nothing, so W, n and F are not read
+                {
+                    const double lo_p = fmin(fmin(Mp[0], Mp[1]), fmin(Mp[2], Mp[3]));
+                    const double hi_p = fmax(fmax(Mp[0], Mp[1]), fmax(Mp[2], Mp[3]));
+                    const double lo_m = fmin(fmin(Mm[0], Mm[1]), fmin(Mm[2], Mm[3]));
+                    const double hi_m = fmax(fmax(Mm[0], Mm[1]), fmax(Mm[2], Mm[3]));
+                    if (!((mq > lo_p && mq < hi_p) || (mq > lo_m && mq < hi_m))) continue;
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    // + process, pair (q1, -q-q1)
+                    const double wp = Wq[((size_t)q1v[v] * nd + s) * nd2 + p] * A.scale *
+                                      (A.nbe[(size_t)q1v[v] * nd + s1] - A.nbe[(size_t)qq2v[v] * nd + s2]);
+                    // - process, pair (-q1, q1 - q)
+                    const double wm = Wq[((size_t)nq1v[v] * nd + s) * nd2 + p] * A.scale *
+                                      (1.0 + A.nbe[(size_t)nq1v[v] * nd + s1] + A.nbe[(size_t)mq2v[v] * nd + s2]);
+                    if (MODE == 0) {
+                        Xp[v][0] = wp;
+                        Xm[v][0] = wm;
+                    } else {
+                        const int2 m1 = A.map[q1v[v]], mp = A.map[qpv[v]], mm = A.map[qmv[v]];
+                        const double *F1 = A.F + ((size_t)m1.x * nd + s1) * 3;
+                        const double *Fp = A.F + ((size_t)mp.x * nd + s2) * 3;
+                        const double *Fm = A.F + ((size_t)mm.x * nd + s2) * 3;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) {
+                            const double f1 = m1.y * F1[k];
+                            Xp[v][k] = wp * (mp.y * Fp[k] - f1);
+                            Xm[v][k] = wm * (mm.y * Fm[k] + f1);
+                        }
+                    }
+                }
+                tetra_acc<K>(mq, Mp, Xp, 1.0, acc);
+                tetra_acc<K>(mq, Mm, Xm, 0.5, acc);
             }
         }
-        tetra_acc<K>(mq, Mp, Xp, 1.0, acc);
-        tetra_acc<K>(mq, Mm, Xm, 0.5, acc);
     }
     __shared__ double s_red[K][kBteBlock];
 #pragma unroll
@@ -509,7 +516,9 @@ int bte_sums(Ctx &c, int mode) {
     // of the linewidth path's W
     A.scale = 4.0;
     const long rows = (long)b.nirr * c.nd;
-    A.nch = (int)std::max<long>(1, std::min<long>(A.ntet, (4096 + rows - 1) / rows));
+    // chunks of at least one tetrahedron per thread (each thread owns whole tetrahedra), enough
+    // of them for about 4096 workgroups
+    A.nch = (int)std::max<long>(1, std::min<long>(A.ntet / kBteBlock, (4096 + rows - 1) / rows));
     A.chunk = (A.ntet + A.nch - 1) / A.nch;
     A.nch = (A.ntet + A.chunk - 1) / A.chunk;
     const int K = mode == 0 ? 1 : 3;

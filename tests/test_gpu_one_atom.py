@@ -67,3 +67,31 @@ def test_one_atom_lw4(fcc, dims):
         tref = O.lw4_tetra(inp, qm, int(qm.irr_idx[qi]), threads=8, chunk=4)
         assert rel_err(tl[qi], tref, 1e-9) < 1e-10, qi
     ctx.close()
+
+
+def test_one_atom_bte(fcc):
+    """Iterative BTE of the one-atom cell on a 4^3 mesh: the W store from the small-cell kernel's
+    Ndof = 3 W mode over every q1 of each time-reversal representative; 1/tau and dF of a random
+    F against the oracle's tetrahedron sums (dF held normwise, and elementwise above 1e-4 of its
+    largest component, where a random F leaves no cancellation to rounding)."""
+    from spring_amd.lib import GpuContext
+    from oracle import oracle as O
+    qm, ph = problem(fcc, (4, 4, 4))
+    ctx = GpuContext(fcc, qm, ph)
+    inp = O.Inputs(fcc, qm, ph)
+    irr, map_irr, map_sign, deg = O.tr_irreducible(qm)
+    W = np.array([O.lw3_gauss(inp, int(g), 1.0, want_w=True)[1] for g in irr])
+    Wp, Wm = O.bte_w_pm(qm, ph, irr, W)
+    g_irr, g_deg = ctx.bte_irr()
+    assert np.array_equal(g_irr, irr) and np.array_equal(g_deg, deg)
+    itau = ctx.bte_tau_inv()
+    ref = O.bte_sums(qm, ph, irr, map_irr, map_sign, Wp, Wm)
+    assert np.abs(ref).max() > 0
+    assert rel_err(itau, ref, 1e-9) < 1e-10
+    F = np.random.default_rng(3).normal(size=(len(irr), ph.omega.shape[1], 3))
+    dF = ctx.bte_delta_f(F)
+    refF = O.bte_sums(qm, ph, irr, map_irr, map_sign, Wp, Wm, F)
+    en = np.max(np.abs(dF - refF)) / np.max(np.abs(refF))
+    print(f"one-atom BTE: 1/tau {rel_err(itau, ref, 1e-9):.2e}, dF normwise {en:.2e}", flush=True)
+    assert en < 1e-12 and rel_err(dF, refF, 1e-4) < 1e-10
+    ctx.close()

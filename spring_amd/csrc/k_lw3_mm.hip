@@ -529,6 +529,22 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
             DT *sV = reinterpret_cast<DT *>(smem) + (size_t)w * (kDvTiles * 256);
             unsigned short *sList = reinterpret_cast<unsigned short *>(smem + L::DV) + (size_t)w * (kDvTiles * 128);
             const double w0 = s0ok ? sMode[s0] : 0.0;
+            // s2 blocks of 16: an element's s1 is its tile's M index (the same for the whole wave)
+            // and its s2 depends only on the lane and k, so the test needs w0 - w2 and w0 + w2 of
+            // the lane's two s2 once per pair and w1 once per tile
+            constexpr bool kRowS2 = SB == 16;
+            double dA[2] = {0.0, 0.0}, dB[2] = {0.0, 0.0};
+            bool okA[2] = {false, false};
+            if constexpr (kRowS2) {
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int s2 = h * SB + tile_row<F64>(g, 2 * qD + k);
+                    okA[k] = s0ok && s2 < ND;
+                    const double w2 = okA[k] ? sMode[5 * ND + s2] : 0.0;
+                    dA[k] = w0 - w2;
+                    dB[k] = w0 + w2;
+                }
+            }
             DT pairv = 0;
 #pragma unroll
             for (int j0 = 0; j0 < Sh::VT; j0 += kDvTiles) {
@@ -548,8 +564,13 @@ __device__ __forceinline__ void mm_unit(const MmArgs &A, const int unit) {
                         const int m = mt * 16 + tile_row<F64>(g, 2 * qD + k);
                         const int s1 = m / SB, s2 = h * SB + m % SB;
                         bool near = false;
-                        if (s0ok && j0 + t < Sh::VT && mt < Sh::MT_D && s1 < ND && s2 < ND &&
-                            (!SEL || A.sel[ND + s1 * ND + s2] != 0.0)) {
+                        if constexpr (kRowS2) {
+                            if (okA[k] && j0 + t < Sh::VT && mt < Sh::MT_D && (!SEL || A.sel[ND + s1 * ND + s2] != 0.0)) {
+                                const double w1 = sMode[2 * ND + mt];
+                                near = fabs(dA[k] - w1) < dcut || fabs(dA[k] + w1) < dcut || fabs(dB[k] - w1) < dcut;
+                            }
+                        } else if (s0ok && j0 + t < Sh::VT && mt < Sh::MT_D && s1 < ND && s2 < ND &&
+                                   (!SEL || A.sel[ND + s1 * ND + s2] != 0.0)) {
                             const double w1 = sMode[2 * ND + s1], w2 = sMode[5 * ND + s2];
                             const double ap = w1 + w2, am = w1 - w2;
                             near = fabs(w0 - ap) < dcut || fabs(w0 + am) < dcut || fabs(w0 - am) < dcut;

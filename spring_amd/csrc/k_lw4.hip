@@ -147,9 +147,11 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
                 Z t = cmk<R>(0, 0);
                 const int u1 = A.blk_start[blk + 1];
                 for (int u = A.blk_start[blk]; u < u1; ++u) {
+                    // each site phase index is in [0, L), so the sum is in [0, 3L)
                     int m = s_m[A.tup_site[3 * u]] + s_m[A.nsite + A.tup_site[3 * u + 1]] +
                             s_m[2 * A.nsite + A.tup_site[3 * u + 2]];
-                    m %= A.L;
+                    m -= m >= A.L ? A.L : 0;
+                    m -= m >= A.L ? A.L : 0;
                     t = crfma(fc[(size_t)u * 81 + comp], s_ph[m], t);
                 }
                 s_a[e] = t;

@@ -49,11 +49,14 @@ struct Lw4Args {
     int classes;         // store mode with every row resident: one triple per permutation class,
                          // W4 written to each distinct ordering's place
     const double *sel;   // branch selection (NULL: all): (4, nd) masks of s0, s1, s2, s3
+    int nterm;           // FC4 clusters (tup_site rows)
 };
 
 // SEL: a branch selection rule is set (A.sel), a compile-time flag: the runtime test costs the
 // FP64 kernel about a fifth of its time (measured)
-template <typename R, int ND, int BLK, bool WST, bool SEL = false>
+// STG: the cluster site table and the block starts are staged in LDS (when they fit), so the
+// phase sum's index chain starts in LDS.
+template <typename R, int ND, int BLK, bool WST, bool SEL = false, bool STG = false>
 __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
     using Z = cplx_t<R>;
     constexpr int NB = ND / 3;
@@ -63,8 +66,13 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
     __shared__ Z s_e[4][ND2];
     __shared__ double s_md[4][3][ND];   // omega, nbe, 1/omega of q0..q3
     extern __shared__ __align__(16) int s_dyn[];
+    constexpr int NBLK = NB * NB * NB * NB;
+    const int nm = (3 * A.nsite + 3) & ~3;
+    const int nstg = STG ? ((3 * A.nterm + NBLK + 1 + 3) & ~3) : 0;
     int *s_m = s_dyn;                       // [3][nsite]
-    Z *s_ph = reinterpret_cast<Z *>(s_dyn + ((3 * A.nsite + 3) & ~3));
+    int *s_tup = s_dyn + nm;                // STG: [nterm][3], then the NBLK + 1 block starts
+    int *s_bst = s_tup + 3 * A.nterm;
+    Z *s_ph = reinterpret_cast<Z *>(s_dyn + nm + nstg);
 
     const int tid = threadIdx.x;
     const int ch = blockIdx.x, qb = blockIdx.y;
@@ -75,6 +83,12 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
     const Z *phase = reinterpret_cast<const Z *>(A.phase);
     const R *fc = reinterpret_cast<const R *>(A.fc);
     for (int i = tid; i < A.L; i += BLK) s_ph[i] = phase[i];
+    if constexpr (STG) {
+        for (int i = tid; i < 3 * A.nterm; i += BLK) s_tup[i] = A.tup_site[i];
+        for (int i = tid; i <= NBLK; i += BLK) s_bst[i] = A.blk_start[i];
+    }
+    const int *tup = STG ? s_tup : A.tup_site;
+    const int *bst = STG ? s_bst : A.blk_start;
     for (int i = tid; i < ND2; i += BLK) s_e[0][i] = evec[(size_t)i0 * ND2 + i];
     if (tid < ND) {
         s_md[0][0][tid] = A.omega[(size_t)i0 * ND + tid];
@@ -145,11 +159,10 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
                 const int blk = ((a / 3 * NB + b / 3) * NB + c / 3) * NB + d / 3;
                 const int comp = (a % 3) * 27 + (b % 3) * 9 + (c % 3) * 3 + (d % 3);
                 Z t = cmk<R>(0, 0);
-                const int u1 = A.blk_start[blk + 1];
-                for (int u = A.blk_start[blk]; u < u1; ++u) {
+                const int u1 = bst[blk + 1];
+                for (int u = bst[blk]; u < u1; ++u) {
                     // each site phase index is in [0, L), so the sum is in [0, 3L)
-                    int m = s_m[A.tup_site[3 * u]] + s_m[A.nsite + A.tup_site[3 * u + 1]] +
-                            s_m[2 * A.nsite + A.tup_site[3 * u + 2]];
+                    int m = s_m[tup[3 * u]] + s_m[A.nsite + tup[3 * u + 1]] + s_m[2 * A.nsite + tup[3 * u + 2]];
                     m -= m >= A.L ? A.L : 0;
                     m -= m >= A.L ? A.L : 0;
                     t = crfma(fc[(size_t)u * 81 + comp], s_ph[m], t);
@@ -288,6 +301,15 @@ __global__ void k_reduce_chunks(const double *partial, int nq0, int nchunks, int
 }
 
 // Sctr4THz2 (constants.f90:62-63) / Nq^2
+// ints of the staged cluster site table and block starts (0: they stay in global memory, when
+// more than 16 KB)
+template <int ND>
+static size_t lw4_stage_ints(int nterm) {
+    constexpr int NB = ND / 3;
+    const size_t n = ((size_t)3 * nterm + NB * NB * NB * NB + 1 + 3) & ~(size_t)3;
+    return n * sizeof(int) <= 16384 ? n : 0;
+}
+
 double lw4_factor(const Ctx &c) {
     const double pi = 4.0 * std::atan(1.0);
     return 0.125 * pi * std::pow(1.054571817 * 1.602176565, 2) * std::pow(6.022140857, 4) * 1000000.0 /
@@ -332,15 +354,21 @@ static int launch_lw4_t(Ctx &c, int q0_first, int q0_count, double sigma, double
         return rc;
     A.partial = c.partial;
     A.sel = c.sel4 ? c.d_sel4 : nullptr;
-    const size_t dyn = (size_t)((3 * A.nsite + 3) & ~3) * sizeof(int) + (size_t)c.L * sizeof(cplx_t<R>);
+    A.nterm = c.fc4.nterm;
+    const size_t nstg = lw4_stage_ints<ND>(A.nterm);
+    const size_t dyn = ((size_t)((3 * A.nsite + 3) & ~3) + nstg) * sizeof(int) + (size_t)c.L * sizeof(cplx_t<R>);
     SPR_HIP(hipEventRecord(c.ev0, c.stream));
     A.wst = nullptr;
     A.q1_lo = A.q2_chunk = A.nq2ch = 0;
     A.classes = 0;
-    if (A.sel)
-        hipLaunchKernelGGL((k_lw4<R, ND, BLK, false, true>), dim3(A.nchunks, q0_count), dim3(BLK), dyn, c.stream, A);
-    else
-        hipLaunchKernelGGL((k_lw4<R, ND, BLK, false, false>), dim3(A.nchunks, q0_count), dim3(BLK), dyn, c.stream, A);
+    const dim3 grid(A.nchunks, q0_count);
+    if (A.sel) {
+        if (nstg) hipLaunchKernelGGL((k_lw4<R, ND, BLK, false, true, true>), grid, dim3(BLK), dyn, c.stream, A);
+        else hipLaunchKernelGGL((k_lw4<R, ND, BLK, false, true, false>), grid, dim3(BLK), dyn, c.stream, A);
+    } else {
+        if (nstg) hipLaunchKernelGGL((k_lw4<R, ND, BLK, false, false, true>), grid, dim3(BLK), dyn, c.stream, A);
+        else hipLaunchKernelGGL((k_lw4<R, ND, BLK, false, false, false>), grid, dim3(BLK), dyn, c.stream, A);
+    }
     SPR_HIP(hipGetLastError());
     SPR_HIP(hipEventRecord(c.ev1, c.stream));
     if ((rc = launch_reduce_chunks(c, c.partial, q0_count, A.nchunks, d_lw))) return rc;
@@ -381,8 +409,13 @@ static int launch_w4_rows_t(Ctx &c, int qi, int q1_lo, int nq1, double *d_w, boo
     A.nq2ch = (int)std::min<long>(c.nq, std::max<long>(1, (target + nq1 - 1) / nq1));
     A.q2_chunk = (c.nq + A.nq2ch - 1) / A.nq2ch;
     A.nq2ch = (c.nq + A.q2_chunk - 1) / A.q2_chunk;
-    const size_t dyn = (size_t)((3 * A.nsite + 3) & ~3) * sizeof(int) + (size_t)c.L * sizeof(cplx_t<double>);
-    hipLaunchKernelGGL((k_lw4<double, ND, BLK, true>), dim3(nq1 * A.nq2ch, 1), dim3(BLK), dyn, c.stream, A);
+    A.nterm = c.fc4.nterm;
+    const size_t nstg = lw4_stage_ints<ND>(A.nterm);
+    const size_t dyn = ((size_t)((3 * A.nsite + 3) & ~3) + nstg) * sizeof(int) + (size_t)c.L * sizeof(cplx_t<double>);
+    if (nstg)
+        hipLaunchKernelGGL((k_lw4<double, ND, BLK, true, false, true>), dim3(nq1 * A.nq2ch, 1), dim3(BLK), dyn, c.stream, A);
+    else
+        hipLaunchKernelGGL((k_lw4<double, ND, BLK, true, false, false>), dim3(nq1 * A.nq2ch, 1), dim3(BLK), dyn, c.stream, A);
     SPR_HIP(hipGetLastError());
     return SPR_OK;
 }

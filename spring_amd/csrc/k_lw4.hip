@@ -176,12 +176,19 @@ __global__ __launch_bounds__(BLK) void k_lw4(Lw4Args A) {
             for (int pass = 0; pass < 4; ++pass) {
                 __syncthreads();
                 const Z *E = s_e[3 - pass];   // e3, e2, e1, e0
-                for (int e = tid; e < ND4; e += BLK) {
-                    const int s = e / ND3, xyz = e % ND3;
-                    Z z = cmk<R>(0, 0);
+                // a thread per row xyz: the row is read once and gives all ND outputs (each summed
+                // over c in order)
+                for (int xyz = tid; xyz < ND3; xyz += BLK) {
+                    Z a[ND];
 #pragma unroll
-                    for (int c = 0; c < ND; ++c) z = cfma(src[xyz * ND + c], E[s * ND + c], z);
-                    dst[e] = z;
+                    for (int c = 0; c < ND; ++c) a[c] = src[xyz * ND + c];
+#pragma unroll
+                    for (int s = 0; s < ND; ++s) {
+                        Z z = cmk<R>(0, 0);
+#pragma unroll
+                        for (int c = 0; c < ND; ++c) z = cfma(a[c], E[s * ND + c], z);
+                        dst[s * ND3 + xyz] = z;
+                    }
                 }
                 Z *t = src;
                 src = dst;
